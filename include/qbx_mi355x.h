@@ -1,0 +1,239 @@
+/*
+ * libqbx_mi355x: MI355X (gfx950) QBX-FMM layer-potential evaluation.
+ *
+ * Plain C ABI: raw device pointers, sizes and opaque handles; no torch or
+ * C++ types cross it.  Every call is ordered on the hipStream_t it is given
+ * (passed as void*), returns an int status (0 = ok), and on failure leaves a
+ * message in qbx_last_error().  The library never frees caller memory; the
+ * tree/traversal/geometry handles own their device arrays until destroyed.
+ *
+ * Layout conventions (the reference's, pytential/qbx/interactions.py:64-68):
+ *   coordinates      SoA double, axis d of point i at x[d*n + i]
+ *   indices          int32
+ *   complex values   interleaved (re, im) doubles
+ *   expansions       [nboxes][ncoeff] / [ncenters][ncoeff_qbx] complex, row-major
+ *                    Laplace 2D: coefficient k at k (k = 0..p)
+ *                    Helmholtz 2D: coefficient n at n + pmax (|n| <= p)
+ *
+ * Each entry point names the reference interface it replaces.
+ */
+#ifndef QBX_MI355X_H
+#define QBX_MI355X_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    QBX_OK = 0,
+    QBX_ERR_INVALID = 1,
+    QBX_ERR_HIP = 2,
+    QBX_ERR_TARGET_ASSOC_FAILED = 3,
+    QBX_ERR_UNSUPPORTED = 4
+};
+
+enum { QBX_LAPLACE2D = 0, QBX_HELMHOLTZ2D = 1 };
+enum { QBX_SOURCE_S = 0, QBX_SOURCE_D = 1 };
+
+typedef struct qbx_ctx qbx_ctx;
+typedef struct qbx_tree qbx_tree;
+typedef struct qbx_trav qbx_trav;
+typedef struct qbx_geo qbx_geo;
+
+/* ---- context / errors --------------------------------------------------- */
+int qbx_ctx_create(qbx_ctx **ctx, int device);
+int qbx_ctx_destroy(qbx_ctx *ctx);
+const char *qbx_last_error(void);
+int qbx_version(void);
+
+/* ---- tree build -----------------------------------------------------------
+ * Replaces boxtree TreeBuilder.__call__ as invoked by
+ * QBXFMMGeometryData.tree() (pytential/qbx/geometry.py:562-618) and by
+ * build_tree_with_qbx_metadata (pytential/qbx/utils.py:278-448).
+ * sources [dim*nsources], targets [dim*ntargets], target_radii [ntargets] or
+ * NULL, weights [n] or NULL (= 1).  extent_norm_linf: 0 = "l2", 1 = "linf". */
+int qbx_tree_build(qbx_ctx *ctx, void *stream, int dim,
+                   int nsources, const double *sources,
+                   int ntargets, const double *targets, const double *target_radii,
+                   const int *source_weights, const int *target_weights,
+                   int max_leaf_refine_weight, double stick_out_factor,
+                   int extent_norm_linf, qbx_tree **out);
+int qbx_tree_destroy(qbx_tree *tree);
+
+/* sizes: {dim, nsources, ntargets, nboxes, nlevels}; root: {min_x, min_y, min_z, extent} */
+int qbx_tree_sizes(const qbx_tree *tree, int *sizes5, double *root4);
+/* Copy one boxtree Tree field (by name, e.g. "box_source_starts",
+ * "box_centers", "user_source_ids", "sorted_target_ids", "level_start_box_nrs",
+ * "box_child_ids" [2^dim][nboxes], "box_coords" [dim][nboxes], "sources",
+ * "targets" (tree order)) into caller device memory `dst` of `nbytes`. */
+int qbx_tree_export(const qbx_tree *tree, const char *field, void *dst, long nbytes,
+                    void *stream);
+long qbx_tree_field_bytes(const qbx_tree *tree, const char *field);
+
+/* ---- traversal ------------------------------------------------------------
+ * Replaces boxtree FMMTraversalBuilder + merge_close_lists as invoked by
+ * QBXFMMGeometryData.traversal() (pytential/qbx/geometry.py:622-641). */
+int qbx_traverse(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
+                 int well_sep_is_n_away, int targets_have_extent,
+                 int from_sep_smaller_min_nsources_cumul, qbx_trav **out);
+int qbx_trav_destroy(qbx_trav *trav);
+/* counts: {nsource_boxes, nsource_parent_boxes, ntarget_boxes, nttp_boxes,
+ *          n_list1, n_list2, n_list3, n_list4} */
+int qbx_trav_sizes(const qbx_trav *trav, int *counts8);
+/* fields: "source_boxes", "source_parent_boxes", "target_boxes",
+ * "target_or_target_parent_boxes", "level_start_*" (same names as boxtree),
+ * "neighbor_source_boxes_starts/lists", "from_sep_siblings_starts/lists",
+ * "from_sep_smaller_starts/lists", "from_sep_bigger_starts/lists" */
+int qbx_trav_export(const qbx_trav *trav, const char *field, void *dst, long nbytes,
+                    void *stream);
+long qbx_trav_field_bytes(const qbx_trav *trav, const char *field);
+
+/* ---- target association ----------------------------------------------------
+ * Replaces associate_targets_to_qbx_centers (pytential/qbx/target_assoc.py:
+ * 853-952: mark_targets :534-634, find_centers :637-748).  Inputs on device:
+ * stage-1 source nodes + tunnel radii, interleaved centers + radii*(1+tol),
+ * targets + side flags.  Outputs (device): target_to_center (-1 = none),
+ * target_status (0 unmarked, 1 marked without center, 2 found).
+ * *nfailed (host) receives the number of status-1 targets; if it is nonzero
+ * the call returns QBX_ERR_TARGET_ASSOC_FAILED (outputs still written). */
+int qbx_target_assoc(qbx_ctx *ctx, void *stream, int dim,
+                     int nsrc1, const double *src1, const double *tunnel_radii,
+                     int ncenters, const double *centers, const double *radii_with_tol,
+                     int ntargets, const double *targets, const int *target_flags,
+                     int *target_to_center, int *target_status, int *nfailed);
+/* Replaces mark_elements_for_refinement (target_assoc.py:751-825):
+ * refine_flags[nelements] (device, int) set to 1 for elements owning a source
+ * within tunnel distance of a failed (status 1) target. */
+int qbx_refine_flags_for_failed(qbx_ctx *ctx, void *stream, int dim,
+                                int nsrc1, const double *src1, const double *tunnel_radii,
+                                int nelements, const int *element_to_source_starts,
+                                int ntargets, const double *targets,
+                                const int *target_status, int *refine_flags);
+
+/* ---- QBX geometry lookups ---------------------------------------------------
+ * Replaces QBXFMMGeometryData.qbx_center_to_target_box (:643-700),
+ * global_qbx_centers (:752-793), center_to_tree_targets (:839-872) and
+ * non_qbx_box_target_lists (:874-899).  user_target_to_center [ntargets of
+ * the tree] in user order: centers first (-1), then user targets. */
+int qbx_geo_build(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
+                  const qbx_trav *trav, int ncenters, const double *centers,
+                  const double *radii, const int *user_target_to_center,
+                  qbx_geo **out);
+int qbx_geo_destroy(qbx_geo *geo);
+/* counts: {ncenters, nglobal_qbx_centers, ncenter_targets, nfiltered_targets} */
+int qbx_geo_sizes(const qbx_geo *geo, int *counts4);
+/* fields: "qbx_center_to_target_box", "global_qbx_centers",
+ * "center_to_tree_targets_starts", "center_to_tree_targets_lists",
+ * "nqbtl_box_target_starts", "nqbtl_box_target_counts_nonchild",
+ * "nqbtl_unfiltered_from_filtered_target_indices" */
+int qbx_geo_export(const qbx_geo *geo, const char *field, void *dst, long nbytes,
+                   void *stream);
+long qbx_geo_field_bytes(const qbx_geo *geo, const char *field);
+
+/* ---- QBX-FMM wrangler stages ------------------------------------------------
+ * kernel description for all stages */
+typedef struct {
+    int kernel;             /* QBX_LAPLACE2D | QBX_HELMHOLTZ2D */
+    double k;               /* Helmholtz wave number (real) */
+    int source_kind;        /* QBX_SOURCE_S | QBX_SOURCE_D */
+    int qbx_order;          /* p_q */
+    int nlevels;
+    const int *level_orders;/* host array [nlevels]: fmm_level_to_order */
+} qbx_kernel_desc;
+
+/* number of complex coefficients per box / per center (storage strides) */
+int qbx_ncoeff_fmm(const qbx_kernel_desc *kd);
+int qbx_ncoeff_qbx(const qbx_kernel_desc *kd);
+
+/* Source data in tree order.  Replaces QBXExpansionWrangler.reorder_sources
+ * (pytential/qbx/fmm.py:198-199) together with get_flat_strengths_from_densities
+ * (pytential/qbx/__init__.py:1075-1096): strengths_tree[i] =
+ * weights[u] * density[u], u = user_source_ids[i]; normals gathered likewise
+ * (source.py:392-433).  density complex [nsources]; weights double [nsources];
+ * normals [dim*nsources] or NULL. Outputs complex / SoA doubles. */
+int qbx_reorder_sources(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
+                        const double *density, const double *weights,
+                        const double *normals, double *strengths_tree,
+                        double *normals_tree);
+
+/* Per-panel upsampling of stage-1 nodal values to quad-stage-2 nodes (the
+ * meshmode connection behind InterpolationPreprocessor,
+ * pytential/symbolic/mappers.py:743-826): out[e*nf + i] = sum_j
+ * mat[i*n1 + j] * in[e*n1 + j], complex in/out, mat double [nf*n1]. */
+int qbx_upsample(qbx_ctx *ctx, void *stream, int nelements, int n1, int nf,
+                 const double *mat, const double *in, double *out);
+
+/* The stages below mirror the wrangler methods called by drive_fmm
+ * (pytential/qbx/fmm.py:436-565).  `accumulate` = 0 overwrites the output.
+ *   mpole/local: [nboxes][qbx_ncoeff_fmm] complex
+ *   qbx_exp:     [ncenters][qbx_ncoeff_qbx] complex
+ *   pot:         [ntargets of the tree] complex, tree target order */
+int qbx_form_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
+                        const qbx_trav *trav, const qbx_kernel_desc *kd,
+                        const double *strengths_tree, const double *normals_tree,
+                        double *mpole);                         /* P2M, sumpy */
+int qbx_coarsen_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
+                           const qbx_trav *trav, const qbx_kernel_desc *kd,
+                           double *mpole);                      /* M2M */
+int qbx_eval_direct(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
+                    const qbx_trav *trav, const qbx_geo *geo,
+                    const qbx_kernel_desc *kd, const double *strengths_tree,
+                    const double *normals_tree, double *pot, int accumulate); /* P2P list 1 */
+int qbx_multipole_to_local(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
+                           const qbx_trav *trav, const qbx_kernel_desc *kd,
+                           const double *mpole, double *local); /* M2L list 2 */
+int qbx_eval_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
+                        const qbx_trav *trav, const qbx_geo *geo,
+                        const qbx_kernel_desc *kd, const double *mpole,
+                        double *pot, int accumulate);           /* M2P list 3 */
+int qbx_form_locals(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
+                    const qbx_trav *trav, const qbx_kernel_desc *kd,
+                    const double *strengths_tree, const double *normals_tree,
+                    double *local, int accumulate);             /* P2L list 4 */
+int qbx_refine_locals(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
+                      const qbx_trav *trav, const qbx_kernel_desc *kd,
+                      double *local);                           /* L2L */
+int qbx_eval_locals(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
+                    const qbx_trav *trav, const qbx_geo *geo,
+                    const qbx_kernel_desc *kd, const double *local,
+                    double *pot, int accumulate);               /* L2P */
+/* P2QBXL: form_global_qbx_locals (fmm.py:231-265, interactions.py:40-167) */
+int qbx_form_global_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
+                               const qbx_trav *trav, const qbx_geo *geo,
+                               const qbx_kernel_desc *kd,
+                               const double *strengths_tree, const double *normals_tree,
+                               double *qbx_exp, int accumulate);
+/* M2QBXL: translate_box_multipoles_to_qbx_local (fmm.py:267-312,
+ * interactions.py:174-294), all source levels in one call */
+int qbx_translate_box_multipoles_to_qbx_local(qbx_ctx *ctx, void *stream,
+                               const qbx_tree *tree, const qbx_trav *trav,
+                               const qbx_geo *geo, const qbx_kernel_desc *kd,
+                               const double *mpole, double *qbx_exp, int accumulate);
+/* L2QBXL: translate_box_local_to_qbx_local (fmm.py:314-356,
+ * interactions.py:301-412) */
+int qbx_translate_box_local_to_qbx_local(qbx_ctx *ctx, void *stream,
+                               const qbx_tree *tree, const qbx_trav *trav,
+                               const qbx_geo *geo, const qbx_kernel_desc *kd,
+                               const double *local, double *qbx_exp, int accumulate);
+/* QBXL2P: eval_qbx_expansions (fmm.py:358-389, interactions.py:419-532);
+ * overwrites pot at every QBX target */
+int qbx_eval_qbx_expansions(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
+                            const qbx_geo *geo, const qbx_kernel_desc *kd,
+                            const double *qbx_exp, double *pot);
+/* Reorder (fmm.py:569-588): user-order output of the user targets
+ * (tree targets >= ncenters): out[u - ncenters] = pot[tree index of u]. */
+int qbx_finalize_potentials(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
+                            const qbx_geo *geo, const double *pot, double *out_user);
+
+/* Whole drive_fmm (fmm.py:414-594) in one call: stages in reference order on
+ * `stream`, workspaces owned by ctx.  strengths_tree/normals_tree from
+ * qbx_reorder_sources; out_user [ntargets - ncenters] complex. */
+int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
+                  const qbx_trav *trav, const qbx_geo *geo,
+                  const qbx_kernel_desc *kd, const double *strengths_tree,
+                  const double *normals_tree, double *out_user);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,200 @@
+// Context, error state, device primitives (scan / sort / compaction) and the
+// small elementwise kernels at the boundary (source reorder, upsampling).
+#include <hipcub/hipcub.hpp>
+
+#include <cstdarg>
+#include <cstring>
+
+#include "qbx_internal.h"
+
+namespace qbx {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int scratch_reserve(qbx_ctx *ctx, size_t bytes, void **p)
+{
+    if (bytes > ctx->scratch_bytes) {
+        if (ctx->scratch) {
+            QBX_HIP(hipDeviceSynchronize());
+            QBX_HIP(hipFree(ctx->scratch));
+            ctx->scratch = nullptr;
+        }
+        size_t nb = bytes + bytes / 4 + 4096;
+        QBX_HIP(hipMalloc(&ctx->scratch, nb));
+        ctx->scratch_bytes = nb;
+    }
+    *p = ctx->scratch;
+    return QBX_OK;
+}
+
+int exclusive_scan(qbx_ctx *ctx, hipStream_t st, const int *in, int *out, int n, int *total)
+{
+    if (n == 0) {
+        QBX_HIP(hipMemsetAsync(out, 0, sizeof(int), st));
+        if (total) *total = 0;
+        return QBX_OK;
+    }
+    size_t tb = 0;
+    QBX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, n, st));
+    void *tmp;
+    QBX_TRY(scratch_reserve(ctx, tb, &tmp));
+    QBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb, in, out, n, st));
+    // out[n] = out[n-1] + in[n-1]
+    int last_out = 0, last_in = 0;
+    QBX_HIP(hipMemcpyAsync(&last_out, out + n - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    QBX_HIP(hipMemcpyAsync(&last_in, in + n - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    QBX_HIP(hipStreamSynchronize(st));
+    int tot = last_out + last_in;
+    QBX_HIP(hipMemcpyAsync(out + n, &tot, sizeof(int), hipMemcpyHostToDevice, st));
+    QBX_HIP(hipStreamSynchronize(st));
+    if (total) *total = tot;
+    return QBX_OK;
+}
+
+int sort_pairs_u64(qbx_ctx *ctx, hipStream_t st, const unsigned long long *kin,
+                   unsigned long long *kout, const int *vin, int *vout, int n, int end_bit)
+{
+    if (n == 0) return QBX_OK;
+    size_t tb = 0;
+    QBX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kin, kout, vin, vout, n, 0,
+                                                end_bit, st));
+    void *tmp;
+    QBX_TRY(scratch_reserve(ctx, tb, &tmp));
+    QBX_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, kin, kout, vin, vout, n, 0, end_bit,
+                                                st));
+    return QBX_OK;
+}
+
+int sort_pairs_i32(qbx_ctx *ctx, hipStream_t st, const int *kin, int *kout, const int *vin,
+                   int *vout, int n, int end_bit)
+{
+    if (n == 0) return QBX_OK;
+    size_t tb = 0;
+    QBX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kin, kout, vin, vout, n, 0,
+                                                end_bit, st));
+    void *tmp;
+    QBX_TRY(scratch_reserve(ctx, tb, &tmp));
+    QBX_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, kin, kout, vin, vout, n, 0, end_bit,
+                                                st));
+    return QBX_OK;
+}
+
+__global__ void k_scatter_flagged(const int *flags, const int *pos, int n, int *out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flags[i]) out[pos[i]] = i;
+}
+
+int compact_indices(qbx_ctx *ctx, hipStream_t st, const int *flags, int n, dbuf<int> &out,
+                    int *count)
+{
+    dbuf<int> pos;
+    QBX_TRY(pos.alloc(n + 1));
+    int tot = 0;
+    QBX_TRY(exclusive_scan(ctx, st, flags, pos.p, n, &tot));
+    QBX_TRY(out.alloc(tot > 0 ? tot : 1));
+    out.n = tot;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_scatter_flagged, dim3(cdiv(n, 256)), dim3(256), 0, st, flags,
+                           pos.p, n, out.p);
+        QBX_HIP(hipGetLastError());
+    }
+    QBX_HIP(hipStreamSynchronize(st));
+    *count = tot;
+    return QBX_OK;
+}
+
+__global__ void k_reorder_sources(int ns, int dim, const int *user_source_ids,
+                                  const double *density, const double *weights,
+                                  const double *normals, double *str, double *nrm)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    int u = user_source_ids[i];
+    double w = weights ? weights[u] : 1.0;
+    str[2 * i] = w * density[2 * u];
+    str[2 * i + 1] = w * density[2 * u + 1];
+    if (normals && nrm)
+        for (int d = 0; d < dim; ++d) nrm[(size_t)d * ns + i] = normals[(size_t)d * ns + u];
+}
+
+__global__ void k_upsample(int nel, int n1, int nf, const double *mat, const double *in,
+                           double *out)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nel * nf) return;
+    int e = idx / nf, i = idx % nf;
+    double re = 0, im = 0;
+    for (int j = 0; j < n1; ++j) {
+        double m = mat[i * n1 + j];
+        re += m * in[2 * (e * n1 + j)];
+        im += m * in[2 * (e * n1 + j) + 1];
+    }
+    out[2 * idx] = re;
+    out[2 * idx + 1] = im;
+}
+
+}  // namespace qbx
+
+using namespace qbx;
+
+extern "C" {
+
+const char *qbx_last_error(void) { return g_err; }
+
+int qbx_version(void) { return 1; }
+
+int qbx_ctx_create(qbx_ctx **ctx, int device)
+{
+    QBX_CHECK(ctx, "qbx_ctx_create: null output");
+    QBX_HIP(hipSetDevice(device));
+    qbx_ctx *c = new qbx_ctx();
+    c->device = device;
+    *ctx = c;
+    return QBX_OK;
+}
+
+int qbx_ctx_destroy(qbx_ctx *ctx)
+{
+    if (!ctx) return QBX_OK;
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    delete ctx;
+    return QBX_OK;
+}
+
+int qbx_reorder_sources(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
+                        const double *density, const double *weights,
+                        const double *normals, double *strengths_tree, double *normals_tree)
+{
+    QBX_CHECK(ctx && tree && density && strengths_tree, "qbx_reorder_sources: null argument");
+    int ns = tree->nsources;
+    if (ns == 0) return QBX_OK;
+    hipLaunchKernelGGL(k_reorder_sources, dim3(cdiv(ns, 256)), dim3(256), 0, S(stream), ns,
+                       tree->dim, tree->user_source_ids.p, density, weights, normals,
+                       strengths_tree, normals_tree);
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
+int qbx_upsample(qbx_ctx *ctx, void *stream, int nelements, int n1, int nf, const double *mat,
+                 const double *in, double *out)
+{
+    QBX_CHECK(ctx && mat && in && out, "qbx_upsample: null argument");
+    QBX_CHECK(nelements >= 0 && n1 > 0 && nf > 0, "qbx_upsample: bad sizes");
+    long n = (long)nelements * nf;
+    if (n == 0) return QBX_OK;
+    hipLaunchKernelGGL(k_upsample, dim3(cdiv(n, 256)), dim3(256), 0, S(stream), nelements, n1,
+                       nf, mat, in, out);
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,135 @@
+// Internal definitions shared by the libqbx_mi355x translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/qbx_mi355x.h"
+
+namespace qbx {
+
+void set_error(const char *fmt, ...);
+
+#define QBX_HIP(call)                                                            \
+    do {                                                                         \
+        hipError_t e_ = (call);                                                  \
+        if (e_ != hipSuccess) {                                                  \
+            ::qbx::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,          \
+                             hipGetErrorString(e_));                             \
+            return QBX_ERR_HIP;                                                  \
+        }                                                                        \
+    } while (0)
+
+#define QBX_CHECK(cond, ...)                                                     \
+    do {                                                                         \
+        if (!(cond)) {                                                           \
+            ::qbx::set_error(__VA_ARGS__);                                       \
+            return QBX_ERR_INVALID;                                              \
+        }                                                                        \
+    } while (0)
+
+#define QBX_TRY(call)                                                            \
+    do {                                                                         \
+        int r_ = (call);                                                         \
+        if (r_ != QBX_OK) return r_;                                             \
+    } while (0)
+
+inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+// Owned device buffer (freed with the handle that holds it).
+template <class T>
+struct dbuf {
+    T *p = nullptr;
+    long n = 0;
+    dbuf() = default;
+    dbuf(const dbuf &) = delete;
+    dbuf &operator=(const dbuf &) = delete;
+    ~dbuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    int alloc(long count) {
+        release();
+        n = count;
+        if (count <= 0) return QBX_OK;
+        QBX_HIP(hipMalloc(&p, sizeof(T) * count));
+        return QBX_OK;
+    }
+    long bytes() const { return (long)sizeof(T) * n; }
+};
+
+constexpr int MAX_LEVELS = 64;
+
+}  // namespace qbx
+
+struct qbx_ctx {
+    int device = 0;
+    // stream-ordered scratch (grown on demand, never shrunk)
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    // FMM workspaces for qbx_drive_fmm
+    qbx::dbuf<double> ws_mpole, ws_local, ws_qbx, ws_pot;
+};
+
+struct qbx_tree {
+    int dim = 0, nsources = 0, ntargets = 0, nboxes = 0, nlevels = 0, maxl = 0;
+    double root_min[3] = {0, 0, 0};
+    double root_extent = 1.0;
+    std::vector<int> level_start_box_nrs;   // host copy [nlevels + 1]
+    qbx::dbuf<int> box_levels, box_parent_ids, box_child_ids, box_coords;
+    qbx::dbuf<double> box_centers;
+    qbx::dbuf<int> box_source_starts, box_source_counts_nonchild, box_source_counts_cumul;
+    qbx::dbuf<int> box_target_starts, box_target_counts_nonchild, box_target_counts_cumul;
+    qbx::dbuf<int> d_level_start_box_nrs;
+    qbx::dbuf<int> user_source_ids, sorted_target_ids;
+    qbx::dbuf<double> sources, targets;      // tree order, SoA
+    qbx::dbuf<unsigned long long> box_morton;  // unpadded Morton code at the box's level
+};
+
+struct qbx_trav {
+    int nlevels = 0;
+    int nsource_boxes = 0, nsource_parent_boxes = 0, ntarget_boxes = 0, nttp_boxes = 0;
+    int n1 = 0, n2 = 0, n3 = 0, n4 = 0;
+    std::vector<int> h_level_start_source_box_nrs, h_level_start_source_parent_box_nrs,
+        h_level_start_target_box_nrs, h_level_start_ttp_box_nrs;
+    qbx::dbuf<int> source_boxes, source_parent_boxes, target_boxes, ttp_boxes;
+    qbx::dbuf<int> level_start_source_box_nrs, level_start_source_parent_box_nrs,
+        level_start_target_box_nrs, level_start_ttp_box_nrs;
+    qbx::dbuf<int> l1_starts, l1_lists, l2_starts, l2_lists, l3_starts, l3_lists,
+        l4_starts, l4_lists;
+    qbx::dbuf<int> box_to_ttp;        // box -> ttp index or -1
+    qbx::dbuf<int> box_to_target_box; // box -> target box index or -1
+};
+
+struct qbx_geo {
+    int ncenters = 0, nglobal = 0, nctt = 0, nfiltered = 0;
+    qbx::dbuf<double> centers, radii;          // copies, SoA [dim][ncenters]
+    qbx::dbuf<int> tree_ttc;                   // [ntargets] tree order
+    qbx::dbuf<int> center_to_target_box;       // [ncenters]
+    qbx::dbuf<int> global_qbx_centers;         // [nglobal]
+    qbx::dbuf<int> ctt_starts, ctt_lists;      // [ncenters + 1], [nctt]
+    qbx::dbuf<int> nqbtl_box_target_starts, nqbtl_box_target_counts_nonchild;
+    qbx::dbuf<int> nqbtl_unfiltered;           // [nfiltered] tree target indices
+    qbx::dbuf<int> user_to_tree_target;        // [ntargets]
+};
+
+namespace qbx {
+int scratch_reserve(qbx_ctx *ctx, size_t bytes, void **p);
+// exclusive scan of int flags/counts (n items) into out (n+1 items), returns total via host copy
+int exclusive_scan(qbx_ctx *ctx, hipStream_t st, const int *in, int *out, int n, int *total);
+// stable sort pairs by 64-bit or 32-bit key
+int sort_pairs_u64(qbx_ctx *ctx, hipStream_t st, const unsigned long long *kin,
+                   unsigned long long *kout, const int *vin, int *vout, int n, int end_bit);
+int sort_pairs_i32(qbx_ctx *ctx, hipStream_t st, const int *kin, int *kout, const int *vin,
+                   int *vout, int n, int end_bit);
+int compact_indices(qbx_ctx *ctx, hipStream_t st, const int *flags, int n, dbuf<int> &out,
+                    int *count);
+}  // namespace qbx

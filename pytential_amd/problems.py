@@ -1,0 +1,72 @@
+"""Synthetic QBX workloads (geometry, density, targets) built from a seed.
+
+Seeds follow ``numpy.random.default_rng(1000 + config_index)``.  Every
+builder returns a :class:`QBXProblem`; nothing here touches the GPU.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .discretization import (CurveDiscretization, discretize_curve, grid_targets,
+                             starfish)
+
+
+@dataclass
+class QBXProblem:
+    name: str
+    kernel: str                 # "laplace" | "helmholtz"
+    k: float
+    source_kind: str            # "S" | "D"
+    qbx_order: int
+    fmm_order: int
+    discr: CurveDiscretization = field(repr=False)
+    density: np.ndarray = field(repr=False)            # stage-1 nodal values
+    targets: np.ndarray = field(repr=False)            # (2, nt) user targets
+    target_sides: np.ndarray = field(repr=False)       # (nt,) int32 side flags
+    seed: int = 0
+
+    @property
+    def dtype(self):
+        return np.complex128 if self.kernel == "helmholtz" else np.float64
+
+    @property
+    def ntargets(self) -> int:
+        return self.targets.shape[1]
+
+
+def _targets_with_surface(discr, grid_n, surface_side, lo=-2.0, hi=2.0):
+    grid = grid_targets(grid_n, lo, hi)
+    tg = np.concatenate([discr.nodes, grid], axis=1)
+    sides = np.concatenate([np.full(discr.ndofs, surface_side, np.int32),
+                            np.zeros(grid.shape[1], np.int32)])
+    return tg, sides
+
+
+def config1(nelements=64, order=8, fine_order=16, grid_n=128, fmm_order=10,
+            qbx_order=4, surface_side=+1):
+    """2D Laplace S on the 5-arm starfish r = 1 + 0.25 cos 5t."""
+    rng = np.random.default_rng(1000 + 0)
+    discr = discretize_curve(starfish(5, 0.25), nelements, order, fine_order)
+    t = discr.t_nodes
+    sigma = np.sin(3 * t) + 0.1 * rng.standard_normal(discr.ndofs)
+    tg, sides = _targets_with_surface(discr, grid_n, surface_side)
+    return QBXProblem("laplace2d-S-starfish", "laplace", 0.0, "S", qbx_order,
+                      fmm_order, discr, sigma, tg, sides, seed=1000)
+
+
+def config2(nelements=4096, order=8, fine_order=16, grid_n=512, fmm_order=15,
+            qbx_order=5, k=20.0, surface_side=+1):
+    """2D Helmholtz k=20 D on a seeded starfish (arms 5..9, amplitude .2-.3)."""
+    rng = np.random.default_rng(1000 + 1)
+    arms = int(rng.integers(5, 10))
+    amp = float(rng.uniform(0.2, 0.3))
+    discr = discretize_curve(starfish(arms, amp), nelements, order, fine_order)
+    sigma = (rng.standard_normal(discr.ndofs)
+             + 1j * rng.standard_normal(discr.ndofs))
+    tg, sides = _targets_with_surface(discr, grid_n, surface_side)
+    p = QBXProblem(f"helmholtz2d-D-starfish{arms}", "helmholtz", k, "D", qbx_order,
+                   fmm_order, discr, sigma, tg, sides, seed=1001)
+    p.arms, p.amplitude = arms, amp
+    return p
