@@ -1,0 +1,328 @@
+#!/usr/bin/env python3
+"""QBX-FMM layer-potential throughput on one MI355X.
+
+Workload (the metric's configuration, 2D Helmholtz): k = 20 double layer on a
+seeded starfish, 4096 panels x 8 Gauss-Legendre nodes (split where target
+association needs it), 16-node oversampled sources, qbx_order 5, complex128
+density; targets = a 512 x 512 grid on [-2, 2]^2 plus every on-surface node.
+
+A step is one evaluation with the geometry (tree, traversal, association)
+cached, as the reference memoizes it: density upsampling, strengths, every
+FMM and QBX stage, and the potential at all user targets in user order.
+
+value   = user target points / step time, FMM orders chosen per level for a
+          1e-10 relative error against the FP64 direct-QBX oracle
+roofline: dominant kernel's algorithmic flops per launch / its average
+          duration (HIP events on the launch stream) against FP64 peak
+cpu_baseline: the oracle (C + OpenMP restatement of the same algorithm)
+          timed on this host on one full evaluation; its direct-QBX leg
+          supplies the accuracy reference on a sample of targets
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("QBX-FMM points/s at 1e-10 rel err vs FP64 oracle, "
+          "GPU batched vs plain vs CPU")
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic flop model (per unit); special functions J0/J1/Y0/Y1 count as
+# SPECIAL flops each (a minimal rational/asymptotic evaluation), complex
+# multiply-add 8, complex multiply 6
+SPECIAL = 40
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_problem(grid_n, nelements):
+    from pytential_amd import problems
+    from pytential_amd.geometry import assoc_status_and_flags
+    cfg = problems.config2(nelements=nelements, grid_n=grid_n)
+    return problems.refine_until_associated(cfg, assoc_status_and_flags)
+
+
+def interaction_counts(geo):
+    """Per-stage interaction counts from the traversal (host copies)."""
+    t, tr, g = geo.tree(), geo.traversal(), geo._geo
+
+    def h(x):
+        return x.cpu().numpy().astype(np.int64)
+
+    scount = h(t.box_source_counts_nonchild)
+    l1s, l1l = h(tr.neighbor_source_boxes_starts), h(tr.neighbor_source_boxes_lists)
+    l3s, l3l = h(tr.from_sep_smaller_starts), h(tr.from_sep_smaller_lists)
+    tboxes = h(tr.target_boxes)
+    nq = geo.non_qbx_box_target_lists()
+    nfilt_box = h(nq["box_target_counts_nonchild"])[tboxes]
+    src_in_l1 = np.add.reduceat(np.append(scount[l1l], 0), l1s[:-1]) * (np.diff(l1s) > 0)
+    n_l3 = np.diff(l3s)
+    gc = h(geo.global_qbx_centers())
+    c2tb = h(geo.qbx_center_to_target_box())[gc]
+    levels = h(t.box_levels)
+    return {
+        "p2p_pairs": int(np.sum(nfilt_box * src_in_l1)),
+        "p2qbxl_pairs": int(np.sum(src_in_l1[c2tb])),
+        "m2qbxl_pairs": int(np.sum(n_l3[c2tb])),
+        "m2qbxl_levels": levels[l3l] if len(l3l) else np.zeros(0, np.int64),
+        "m2qbxl_per_center_box": np.repeat(np.arange(len(tboxes)), n_l3),
+        "l2qbxl": int(len(gc)),
+        "qbxl2p": int(g.nctt),
+        "m2l_pairs": int(tr.from_sep_siblings_lists.numel()),
+        "m2p_pairs": int(np.sum(nfilt_box * n_l3)),
+        "nglobal": int(g.nglobal), "nfiltered": int(g.nfiltered),
+        "c2tb": c2tb, "l3s": l3s, "l3l": l3l, "levels": levels,
+    }
+
+
+def stage_flops(counts, orders, pq):
+    """Algorithmic flops per stage launch (Helmholtz 2D, double layer)."""
+    nq = 2 * pq + 1
+    # P2P D: r, proj (10), H1 = J1 + i Y1 (2 special), scale and accumulate (12)
+    p2p = counts["p2p_pairs"] * (22 + 2 * SPECIAL)
+    # P2QBXL D: rho/theta (6 + atan2 as special), H0,H1 (4 special), recurrence to
+    # pq+1 (6 each), phases (6 each), 2*nq complex MACs for the dipole
+    p2qbxl = counts["p2qbxl_pairs"] * (6 + 5 * SPECIAL + 12 * (pq + 1) + 2 * 8 * nq)
+    # M2QBXL: per (center, box): H0,H1 + atan2, recurrence to p+pq (8 each),
+    # 2*nq complex MACs per order
+    lv = counts["m2qbxl_levels"]
+    c2tb, l3s = counts["c2tb"], counts["l3s"]
+    per_box = np.array([orders[l] for l in lv], dtype=np.int64) if len(lv) else np.zeros(0)
+    m2q = 0
+    if len(per_box):
+        csum = np.concatenate([[0], np.cumsum(5 * SPECIAL + (per_box + pq + 1) *
+                                              (8 + 2 * nq * 8))])
+        m2q = int(np.sum(csum[l3s[c2tb + 1]] - csum[l3s[c2tb]]))
+    return {"p2p": p2p, "p2qbxl": p2qbxl, "m2qbxl": m2q}
+
+
+def run_gpu(args, cfg, orders_fn, label):
+    import torch
+
+    from pytential_amd import fmm as F
+    from pytential_amd.qbx import QBXLayerPotentialSource
+    src = QBXLayerPotentialSource(cfg.discr, fine_order=cfg.discr.fine_order,
+                                  qbx_order=cfg.qbx_order, fmm_level_to_order=orders_fn)
+    geo = src.qbx_fmm_geometry_data(((cfg.targets, cfg.target_sides),))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    geo.build_all()
+    torch.cuda.synchronize()
+    cold = time.perf_counter() - t0
+    w = src.make_wrangler(geo, cfg.kernel, cfg.k, cfg.source_kind)
+    orders = list(map(int, w._lo))
+    log(f"[{label}] geometry {cold:.3f}s nboxes={geo.tree().nboxes} "
+        f"nlevels={geo.tree().nlevels} orders={orders}")
+    dens = torch.as_tensor(cfg.density, dtype=torch.complex128, device="cuda")
+
+    stages = ["strengths", "p2m", "m2m", "p2p", "m2l", "m2p", "p2l", "l2l", "l2p",
+              "p2qbxl", "m2qbxl", "l2qbxl", "qbxl2p", "finalize"]
+
+    def step(events=None):
+        def mark(i):
+            if events is not None:
+                events[i].record()
+        mark(0)
+        qd = src.quad_density(dens)
+        st, nrm = w.reorder_sources(qd, src.dev["quad_weights"], src.dev["quad_normals"])
+        mark(1)
+        m = w.form_multipoles(st, nrm)
+        mark(2)
+        w.coarsen_multipoles(m)
+        mark(3)
+        pot = w.eval_direct(st, nrm)
+        mark(4)
+        loc = w.multipole_to_local(m)
+        mark(5)
+        w.eval_multipoles(m, pot)
+        mark(6)
+        w.form_locals(st, nrm, loc)
+        mark(7)
+        w.refine_locals(loc)
+        mark(8)
+        w.eval_locals(loc, pot)
+        mark(9)
+        q = w.form_global_qbx_locals(st, nrm)
+        mark(10)
+        w.translate_box_multipoles_to_qbx_local(m, q)
+        mark(11)
+        w.translate_box_local_to_qbx_local(loc, q)
+        mark(12)
+        w.eval_qbx_expansions(q, pot)
+        mark(13)
+        out = w.finalize_potentials(pot)
+        mark(14)
+        return out
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(len(stages) + 1)]
+           for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        out = step(evs[k])
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    per_stage = {s: float(np.mean([e[i].elapsed_time(e[i + 1]) for e in evs]))
+                 for i, s in enumerate(stages)}
+    # fused single-entry driver over the same work, for reference
+    qd = src.quad_density(dens)
+    st, nrm = w.reorder_sources(qd, src.dev["quad_weights"], src.dev["quad_normals"])
+    F.drive_fmm_fused(w, st, nrm)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        qd = src.quad_density(dens)
+        st, nrm = w.reorder_sources(qd, src.dev["quad_weights"], src.dev["quad_normals"])
+        F.drive_fmm_fused(w, st, nrm)
+    torch.cuda.synchronize()
+    dt_fused = (time.perf_counter() - t1) / args.steps
+    counts = interaction_counts(geo)
+    flops = stage_flops(counts, orders, cfg.qbx_order)
+    return {"label": label, "orders": orders, "cold_s": cold, "step_s": dt,
+            "step_fused_s": dt_fused, "per_stage_ms": per_stage, "counts": counts,
+            "flops": flops, "result": out.cpu().numpy(), "geo": geo, "src": src}
+
+
+def cpu_baseline(args, cfg, orders, sample_idx):
+    """The oracle on the host cores: one full evaluation (timed) plus the
+    direct-QBX accuracy reference on a sample of targets."""
+    import oracle
+    oracle.build(with_ref=False)
+    nthreads = args.cpu_threads or os.cpu_count()
+    g = oracle.GeometryData(cfg.discr, cfg.targets, cfg.target_sides)
+    st = cfg.discr.strengths(cfg.density)
+    t0 = time.perf_counter()
+    pot, _ = g.fmm(cfg.kernel, cfg.k, cfg.source_kind, cfg.qbx_order, orders, st,
+                   nthreads=nthreads)
+    t_fmm = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ref = g.direct(cfg.kernel, cfg.k, cfg.source_kind, cfg.qbx_order, st, sample_idx,
+                   nthreads=nthreads)
+    t_direct = time.perf_counter() - t0
+    return {"value": cfg.ntargets / t_fmm, "unit": "points/s", "cores": nthreads,
+            "kind": "port",
+            "sample": (f"one full evaluation of the same workload ({cfg.ntargets} targets, "
+                       f"same tree/lists/orders), {t_fmm:.2f}s; direct-QBX reference on "
+                       f"{len(sample_idx)} sampled targets, {t_direct:.2f}s"),
+            "fmm_result_sample": pot[sample_idx], "direct_sample": ref, "t_fmm": t_fmm}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--grid", type=int, default=512)
+    ap.add_argument("--nelements", type=int, default=4096)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--sample", type=int, default=1000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the QBX-FMM "
+              "path here is single-GPU, see DESIGN.md for how it would shard.",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    torch.cuda.set_device(0)
+    from pytential_amd.qbx import fmm_level_to_order_for_tolerance
+
+    t0 = time.perf_counter()
+    cfg = build_problem(args.grid, args.nelements)
+    log(f"problem: {cfg.name} panels={cfg.discr.nelements} ndofs={cfg.discr.ndofs} "
+        f"nsources={cfg.discr.nsources} ntargets={cfg.ntargets} "
+        f"({time.perf_counter() - t0:.2f}s)")
+
+    def tol_orders(kernel, kernel_args, tree, level):
+        return fmm_level_to_order_for_tolerance(cfg.kernel, cfg.k, args.tol,
+                                                tree.root_extent, tree.nlevels)[level]
+
+    def config_orders(kernel, kernel_args, tree, level):
+        return cfg.fmm_order
+
+    main_run = run_gpu(args, cfg, tol_orders, "tol")
+    cfg_run = run_gpu(args, cfg, config_orders, f"p{cfg.fmm_order}")
+
+    rng = np.random.default_rng(11)
+    ttc, _ = main_run["geo"]._assoc
+    ttc = ttc.cpu().numpy()
+    q_idx, n_idx = np.nonzero(ttc >= 0)[0], np.nonzero(ttc < 0)[0]
+    half = args.sample // 2
+    sample_idx = np.sort(np.concatenate([rng.choice(q_idx, min(half, len(q_idx)), False),
+                                         rng.choice(n_idx, min(half, len(n_idx)), False)]))
+    cpu = None
+    err = {"tol": None, "config_order": None}
+    if not args.no_cpu:
+        cpu = cpu_baseline(args, cfg, main_run["orders"], sample_idx)
+        ref = cpu["direct_sample"]
+        scale = np.max(np.abs(ref))
+        for key, run in (("tol", main_run), ("config_order", cfg_run)):
+            err[key] = float(np.max(np.abs(run["result"][sample_idx] - ref)) / scale)
+        err["gpu_vs_cpu_fmm"] = float(np.max(np.abs(main_run["result"][sample_idx]
+                                                    - cpu["fmm_result_sample"])) / scale)
+        log(f"errors: {err}")
+
+    ps = main_run["per_stage_ms"]
+    flops = main_run["flops"]
+    dominant = max(ps, key=ps.get)
+    rl = None
+    if dominant in flops:
+        achieved = flops[dominant] / (ps[dominant] * 1e-3) / 1e12
+        rl = {"bound": "fp64", "kernel": dominant, "achieved": round(achieved, 3),
+              "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+              "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+              "flops_per_launch": int(flops[dominant]),
+              "avg_launch_ms": round(ps[dominant], 4)}
+    line = {
+        "metric": METRIC,
+        "value": round(cfg.ntargets / main_run["step_s"], 1),
+        "unit": "points/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(main_run["step_s"] * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c128", "data": "synthetic",
+        "config": {
+            "workload": (f"2D Helmholtz k={cfg.k:g} double layer, starfish arms="
+                         f"{cfg.arms} amp={cfg.amplitude:.3f}, {cfg.discr.nelements} panels x "
+                         f"{cfg.discr.order} nodes, fine {cfg.discr.fine_order}, qbx_order "
+                         f"{cfg.qbx_order}, {args.grid}^2 grid + on-surface targets"),
+            "ntargets": cfg.ntargets, "nsources": cfg.discr.nsources,
+            "ncenters": 2 * cfg.discr.ndofs, "fmm_orders_per_level": main_run["orders"],
+            "tol": args.tol, "seed": cfg.seed},
+        "rel_err_vs_direct_qbx": err["tol"],
+        "at_config_order": {
+            "fmm_order": cfg.fmm_order,
+            "value": round(cfg.ntargets / cfg_run["step_s"], 1),
+            "ms_per_step": round(cfg_run["step_s"] * 1e3, 4),
+            "rel_err_vs_direct_qbx": err["config_order"]},
+        "fused_driver_ms_per_step": round(main_run["step_fused_s"] * 1e3, 4),
+        "cold_geometry_s": round(main_run["cold_s"], 4),
+        "per_stage_ms": {k: round(v, 4) for k, v in ps.items()},
+        "roofline": rl,
+        "cpu_baseline": None if cpu is None else {
+            k: (round(v, 1) if isinstance(v, float) else v) for k, v in cpu.items()
+            if k in ("value", "unit", "cores", "kind", "sample")},
+    }
+    if cpu is not None:
+        line["gpu_vs_cpu_fmm_rel_diff"] = err["gpu_vs_cpu_fmm"]
+        line["speedup_vs_cpu"] = round(line["value"] / cpu["value"], 2)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

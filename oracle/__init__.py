@@ -418,3 +418,15 @@ class GeometryData:
             ttc = ttc[target_idx]
         return direct_qbx(kp, self.discr.quad_nodes, self.discr.quad_normals,
                           strengths, self.centers, self.radii, tg, ttc)
+
+
+def assoc_status_and_flags(discr, targets, sides,
+                           target_association_tolerance=1.0e3 * 2.220446049250313e-16):
+    """(status, refine_flags) for ``problems.refine_until_associated``."""
+    ttc, status = target_assoc(discr.nodes, discr.tunnel_radii(), discr.centers(),
+                               discr.center_radii() * (1 + target_association_tolerance),
+                               targets, sides)
+    flags = refine_flags_for_failed(discr.nodes, discr.tunnel_radii(),
+                                    discr.element_to_source_starts(), targets, status) \
+        if np.any(status == 1) else np.zeros(discr.nelements, np.int32)
+    return status, flags

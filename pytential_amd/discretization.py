@@ -92,6 +92,8 @@ class CurveDiscretization:
     quad_normals: np.ndarray = field(repr=False)
     quad_weights: np.ndarray = field(repr=False)
     upsample: np.ndarray = field(repr=False)       # (fine_order, order) per panel
+    t_edges: np.ndarray = field(default=None, repr=False)  # (nelements+1,)
+    curve: object = field(default=None, repr=False)
 
     @property
     def ndofs(self) -> int:
@@ -131,32 +133,84 @@ class CurveDiscretization:
     def strengths(self, sigma: np.ndarray) -> np.ndarray:
         return self.quad_weights * self.upsample_density(sigma)
 
+    def refined(self, refine_flags) -> tuple["CurveDiscretization", "PanelConnection"]:
+        """Split every flagged panel in two (parameter midpoint).  Returns the
+        new discretization and the connection that carries stage-1 nodal data
+        across (per-panel Lagrange interpolation, the role of meshmode's
+        refinement connection)."""
+        flags = np.asarray(refine_flags).astype(bool)
+        te = self.t_edges
+        new_edges = [te[0]]
+        parent, which = [], []
+        for j in range(self.nelements):
+            if flags[j]:
+                new_edges += [0.5 * (te[j] + te[j + 1]), te[j + 1]]
+                parent += [j, j]
+                which += [1, 2]
+            else:
+                new_edges.append(te[j + 1])
+                parent.append(j)
+                which.append(0)
+        new = discretize_curve_edges(self.curve, np.array(new_edges), self.order,
+                                     self.fine_order)
+        xr, _ = gauss_legendre(self.order)
+        mats = [np.eye(self.order), lagrange_matrix(xr, (xr - 1) * 0.5),
+                lagrange_matrix(xr, (xr + 1) * 0.5)]
+        return new, PanelConnection(np.array(parent), np.array(which), mats, self.order)
 
-def discretize_curve(curve, nelements: int, order: int, fine_order: int,
-                     t0: float = 0.0) -> CurveDiscretization:
-    xr, _ = gauss_legendre(order)
+
+@dataclass
+class PanelConnection:
+    """Stage-1 data on parent panels -> refined panels (block interpolation)."""
+    parent: np.ndarray
+    which: np.ndarray      # 0 copy, 1 left half, 2 right half
+    mats: list
+    order: int
+
+    def __call__(self, x):
+        x = np.asarray(x)
+        blocks = x.reshape(-1, self.order)[self.parent]
+        out = np.empty((len(self.parent), self.order), dtype=x.dtype)
+        for k in range(3):
+            sel = self.which == k
+            out[sel] = blocks[sel] @ self.mats[k].T
+        return out.reshape(-1)
+
+
+def discretize_curve_edges(curve, t_edges, order: int, fine_order: int) -> CurveDiscretization:
+    t_edges = np.asarray(t_edges, dtype=np.float64)
+    nelements = len(t_edges) - 1
+    xr, wr = gauss_legendre(order)
     xf, wf = gauss_legendre(fine_order)
-    dt = 2 * np.pi / nelements
-    tl = t0 + dt * np.arange(nelements)
+    tl = t_edges[:-1]
+    dt = np.diff(t_edges)
 
     def panel_eval(xref):
-        t = (tl[:, None] + (xref[None, :] + 1) * 0.5 * dt).reshape(-1)
+        t = (tl[:, None] + (xref[None, :] + 1) * 0.5 * dt[:, None]).reshape(-1)
         x, y, dx, dy = curve(t)
         speed = np.hypot(dx, dy)
         nrm = np.stack([dy / speed, -dx / speed])
         return t, np.stack([x, y]), nrm, speed
 
+    dt1 = np.repeat(dt, order)
+    dt2 = np.repeat(dt, fine_order)
     t1, x1, n1, sp1 = panel_eval(xr)
-    _, wr = gauss_legendre(order)
-    w1 = (np.tile(wr, nelements) * sp1 * 0.5 * dt)
-    radii = 0.5 * sp1 * dt
+    w1 = np.tile(wr, nelements) * sp1 * 0.5 * dt1
+    radii = 0.5 * sp1 * dt1
     _, x2, n2, sp2 = panel_eval(xf)
-    w2 = np.tile(wf, nelements) * sp2 * 0.5 * dt
+    w2 = np.tile(wf, nelements) * sp2 * 0.5 * dt2
     return CurveDiscretization(
         nelements=nelements, order=order, fine_order=fine_order,
         t_nodes=t1, nodes=x1, normals=n1, weights=w1, expansion_radii=radii,
         quad_nodes=x2, quad_normals=n2, quad_weights=w2,
-        upsample=lagrange_matrix(xr, xf))
+        upsample=lagrange_matrix(xr, xf), t_edges=t_edges, curve=curve)
+
+
+def discretize_curve(curve, nelements: int, order: int, fine_order: int,
+                     t0: float = 0.0) -> CurveDiscretization:
+    """Uniform panels in the curve parameter t over [t0, t0 + 2 pi]."""
+    edges = t0 + 2 * np.pi * np.arange(nelements + 1) / nelements
+    return discretize_curve_edges(curve, edges, order, fine_order)
 
 
 def grid_targets(n: int, lo: float = -2.0, hi: float = 2.0) -> np.ndarray:

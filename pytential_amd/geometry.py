@@ -316,3 +316,19 @@ class _GeoHandle(_Exported):
         c = (ct.c_int * 4)()
         call("qbx_geo_sizes", h, c)
         self.ncenters, self.nglobal, self.nctt, self.nfiltered = c[:]
+
+
+def assoc_status_and_flags(discr, targets, sides,
+                           target_association_tolerance=TARGET_ASSOC_TOLERANCE):
+    """GPU (status, refine_flags) of a discretization against targets, for
+    ``problems.refine_until_associated``."""
+    dev = {"nodes": as_dev(discr.nodes), "tunnel": as_dev(discr.tunnel_radii()),
+           "centers": as_dev(discr.centers()), "center_radii": as_dev(discr.center_radii()),
+           "element_to_source_starts": as_dev(discr.element_to_source_starts(), torch.int32)}
+    try:
+        _, status = associate_targets_to_qbx_centers(dev, targets, sides,
+                                                     target_association_tolerance)
+        return status.cpu().numpy(), np.zeros(discr.nelements, np.int32)
+    except QBXTargetAssociationFailedError as e:
+        st = e.failed_target_flags.cpu().numpy().astype(np.int32)
+        return st, e.refine_flags.cpu().numpy()

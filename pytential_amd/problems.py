@@ -70,3 +70,32 @@ def config2(nelements=4096, order=8, fine_order=16, grid_n=512, fmm_order=15,
                    fmm_order, discr, sigma, tg, sides, seed=1001)
     p.arms, p.amplitude = arms, amp
     return p
+
+
+def refine_until_associated(problem: QBXProblem, assoc_fn, max_iter: int = 8) -> QBXProblem:
+    """Split the elements that ``QBXTargetAssociationFailedError.refine_flags``
+    names (pytential/qbx/target_assoc.py:751-825, 915-950) until every target
+    finds a QBX center.  ``assoc_fn(discr, targets, sides) -> (status,
+    refine_flags)``.  On-surface targets are the stage-1 nodes, so they move
+    with the refinement; the density is carried across by the per-panel
+    interpolation connection."""
+    p = problem
+    discr, dens = p.discr, p.density
+    nsurf = discr.ndofs
+    surf_side = int(p.target_sides[0]) if nsurf else 0
+    vol = p.targets[:, nsurf:]
+    vol_sides = p.target_sides[nsurf:]
+    for _ in range(max_iter):
+        tg = np.concatenate([discr.nodes, vol], axis=1)
+        sides = np.concatenate([np.full(discr.ndofs, surf_side, np.int32), vol_sides])
+        status, flags = assoc_fn(discr, tg, sides)
+        if not np.any(status == 1):
+            out = QBXProblem(p.name, p.kernel, p.k, p.source_kind, p.qbx_order,
+                             p.fmm_order, discr, dens, tg, sides, seed=p.seed)
+            out.__dict__.update({k: v for k, v in p.__dict__.items()
+                                 if k not in out.__dict__})
+            out.nrefined = discr.nelements - p.discr.nelements
+            return out
+        discr, conn = discr.refined(flags)
+        dens = conn(dens)
+    raise RuntimeError("target association still fails after refinement")

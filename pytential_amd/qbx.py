@@ -26,7 +26,7 @@ def fmm_level_to_order_for_tolerance(kernel, k, tol, root_extent, nlevels,
     convergence with ratio 0.3 for 2-away separation plus target stick-out;
     Helmholtz adds the excess-bandwidth term k*h*sqrt(2)*1.5 of the
     translated region."""
-    base = int(math.ceil(math.log(tol / 0.015) / math.log(0.3)))
+    base = int(math.ceil(math.log(tol / 0.15) / math.log(0.3)))
     orders = []
     for lev in range(nlevels):
         h = root_extent * 2.0 ** (-lev)

@@ -45,6 +45,8 @@ def case(request):
     cfg = getattr(problems, spec["builder"])(**spec["kw"])
     if "source_kind" in spec:
         cfg.source_kind = spec["source_kind"]
+    import oracle
+    cfg = problems.refine_until_associated(cfg, oracle.assoc_status_and_flags)
     print(f"\n[case] {request.param}: ndofs={cfg.discr.ndofs} ntargets={cfg.ntargets}",
           flush=True)
     src = QBXLayerPotentialSource(cfg.discr, fine_order=cfg.discr.fine_order,
@@ -183,3 +185,29 @@ def test_end_to_end_vs_direct_qbx(case):
     print(f"[{name}] FMM vs direct QBX rel err = {err:.3e}")
     # orders here are the config's; errors are stated, bounds are loose
     assert err < (1e-6 if cfg.kernel == "laplace" else 5e-2)
+
+
+def test_association_failure_parity():
+    """Failed association: same statuses and refine flags as the oracle, and
+    the reference's exception type and message."""
+    import oracle
+
+    from pytential_amd import problems
+    from pytential_amd.geometry import (QBXTargetAssociationFailedError,
+                                        assoc_status_and_flags,
+                                        associate_targets_to_qbx_centers, as_dev)
+    cfg = problems.config1(nelements=64, grid_n=96)
+    st_ref, fl_ref = oracle.assoc_status_and_flags(cfg.discr, cfg.targets, cfg.target_sides)
+    assert np.any(st_ref == 1)
+    st, fl = assoc_status_and_flags(cfg.discr, cfg.targets, cfg.target_sides)
+    assert np.array_equal(st.astype(bool), st_ref == 1)
+    assert np.array_equal(fl, fl_ref)
+    d = cfg.discr
+    dev = {"nodes": as_dev(d.nodes), "tunnel": as_dev(d.tunnel_radii()),
+           "centers": as_dev(d.centers()), "center_radii": as_dev(d.center_radii()),
+           "element_to_source_starts": as_dev(d.element_to_source_starts(), torch.int32)}
+    with pytest.raises(QBXTargetAssociationFailedError, match="not assigned a QBX center"):
+        associate_targets_to_qbx_centers(dev, cfg.targets, cfg.target_sides)
+    ref = problems.refine_until_associated(cfg, assoc_status_and_flags)
+    ref2 = problems.refine_until_associated(cfg, oracle.assoc_status_and_flags)
+    assert np.array_equal(ref.discr.t_edges, ref2.discr.t_edges)

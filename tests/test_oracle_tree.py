@@ -1,0 +1,281 @@
+"""Pin the oracle's tree build, traversal and target association (CPU).
+
+* tree: structural invariants, stick-out placement, refine-weight splitting
+* traversal: every list re-derived by brute force over all box pairs from
+  the written definitions, plus the constant-one completeness check of the
+  reference's test_cost_model.py:462-643 (every target box reaches every
+  source exactly once through lists 1-4)
+* association: brute force == accelerated, the property checks of
+  test_global_qbx.py:464-506 and the forced failure of :511-566
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from pytential_amd import problems
+
+
+def _small_geometry(grid_n=24, nel=24):
+    cfg = problems.config1(nelements=nel, grid_n=grid_n)
+    cfg = problems.refine_until_associated(cfg, O.assoc_status_and_flags)
+    return cfg, O.GeometryData(cfg.discr, cfg.targets, cfg.target_sides,
+                               max_leaf_refine_weight=16)
+
+
+@pytest.fixture(scope="module")
+def geo(oracle_lib):
+    return _small_geometry()
+
+
+def _children(t, b):
+    return [c for c in t.box_child_ids[:, b] if c >= 0]
+
+
+def test_tree_structure(geo):
+    cfg, g = geo
+    t = g.tree
+    nb = t.nboxes
+    assert t.box_parent_ids[0] == -1 and t.box_levels[0] == 0
+    for b in range(1, nb):
+        p = t.box_parent_ids[b]
+        assert t.box_levels[b] == t.box_levels[p] + 1
+        assert b in _children(t, p)
+        cidx = list(t.box_child_ids[:, p]).index(b)
+        for d in range(2):
+            assert t.box_coords[d, b] == 2 * t.box_coords[d, p] + ((cidx >> d) & 1)
+    # boxes numbered level by level
+    assert np.all(np.diff(t.box_levels) >= 0)
+    for kind in ["source", "target"]:
+        starts = getattr(t, f"box_{kind}_starts")
+        nonchild = getattr(t, f"box_{kind}_counts_nonchild")
+        cumul = getattr(t, f"box_{kind}_counts_cumul")
+        n = t.nsources if kind == "source" else t.ntargets
+        cover = np.zeros(n, int)
+        for b in range(nb):
+            cover[starts[b]:starts[b] + nonchild[b]] += 1
+            kids = _children(t, b)
+            assert cumul[b] == nonchild[b] + sum(cumul[c] for c in kids)
+            # own particles first, then children in child order (pre-order)
+            pos = starts[b] + nonchild[b]
+            for c in kids:
+                assert starts[c] == pos
+                pos += cumul[c]
+        assert np.all(cover == 1)
+    # sources only in leaves, all leaves at or below weight limit unless unsplittable
+    for b in range(nb):
+        if _children(t, b):
+            assert t.box_source_counts_nonchild[b] == 0
+
+
+def test_tree_particle_positions(geo):
+    cfg, g = geo
+    t = g.tree
+    h = t.root_extent * 2.0 ** -t.box_levels
+    lo = t.bounding_box_min[:, None] + t.box_coords * h
+    for b in range(t.nboxes):
+        s0, n = t.box_source_starts[b], t.box_source_counts_nonchild[b]
+        p = t.sources[:, s0:s0 + n]
+        assert np.all(p >= lo[:, b:b + 1] - 1e-14) and np.all(p <= lo[:, b:b + 1] + h[b] + 1e-14)
+    # centers stick out by at most the l2 spec: |x-c| + r <= 1.5 * sqrt(2)/2 * h
+    nc = g.ncenters
+    radii = np.concatenate([g.radii, np.zeros(t.ntargets - nc)])[t.sorted_target_ids]
+    for b in range(t.nboxes):
+        s0, n = t.box_target_starts[b], t.box_target_counts_nonchild[b]
+        if n == 0 or b == 0:
+            continue
+        x = t.targets[:, s0:s0 + n]
+        r = radii[s0:s0 + n]
+        dist = np.hypot(*(x - t.box_centers[:, b:b + 1]))
+        assert np.all(dist + r <= 1.5 * np.sqrt(2) / 2 * h[b] * (1 + 1e-12))
+
+
+def _gap(t, a, b, grow=0):
+    la, lb = t.box_levels[a], t.box_levels[b]
+    m = max(la, lb)
+    g = 0
+    for d in range(2):
+        sa, sb = 2 ** (m - la), 2 ** (m - lb)
+        alo, ahi = t.box_coords[d, a] * sa, (t.box_coords[d, a] + 1) * sa
+        blo, bhi = (t.box_coords[d, b] - grow) * sb, (t.box_coords[d, b] + 1 + grow) * sb
+        g = max(g, alo - bhi, blo - ahi, 0)
+    return g
+
+
+def _ancestors(t, b):
+    out = []
+    while b >= 0:
+        out.append(b)
+        b = t.box_parent_ids[b]
+    return out
+
+
+def _desc(t, b):
+    out = []
+    stack = list(_children(t, b))
+    while stack:
+        x = stack.pop()
+        out.append(x)
+        stack.extend(_children(t, x))
+    return out
+
+
+def _csr(starts, lists, i):
+    return sorted(lists[starts[i]:starts[i + 1]])
+
+
+def test_lists_match_brute_force_definitions(geo):
+    """Re-derive lists 1-4 from their definitions by brute force (n = 2,
+    targets with extent, min_nsources_cumul = 15) and compare exactly."""
+    cfg, g = geo
+    t, tr = g.tree, g.trav
+    n, e, mins = 2, 1, 15
+    leaf = np.array([not _children(t, b) for b in range(t.nboxes)])
+    srcleaf = leaf & (t.box_source_counts_nonchild > 0)
+    for it, B in enumerate(tr.target_boxes):
+        lb = t.box_levels[B]
+        l1, l3 = set(), set()
+        # (a) source leaves not smaller than B, not separated
+        for S in np.nonzero(srcleaf)[0]:
+            if t.box_levels[S] <= lb and _gap(t, S, B) < n:
+                l1.add(S)
+        # (b) below same-level colleagues
+        cols = [C for C in range(t.nboxes)
+                if t.box_levels[C] == lb and _gap(t, C, B) <= n - 1]
+        for C in cols:
+            for D in _desc(t, C):
+                if t.box_source_counts_cumul[D] == 0:
+                    continue
+                anc = _ancestors(t, D)
+                path = anc[:anc.index(C)]      # D up to (excluding) C
+                seps = [x for x in path if _gap(t, x, B, e) >= n]
+                if seps:
+                    top = seps[-1]             # first separated box from above
+                    if top == D:
+                        if t.box_source_counts_cumul[D] < mins:
+                            l1.update(x for x in [D] + _desc(t, D) if srcleaf[x])
+                        else:
+                            l3.add(D)
+                elif srcleaf[D]:
+                    l1.add(D)
+        assert _csr(tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists, it) \
+            == sorted(l1), B
+        assert _csr(tr.from_sep_smaller_starts, tr.from_sep_smaller_lists, it) == sorted(l3)
+    for it, P in enumerate(tr.target_or_target_parent_boxes):
+        lp = t.box_levels[P]
+        l2, l4 = set(), set()
+        if lp > 0:
+            Q = t.box_parent_ids[P]
+            for A in range(t.nboxes):
+                if (t.box_levels[A] == lp and t.box_source_counts_cumul[A] > 0
+                        and _gap(t, t.box_parent_ids[A], Q) <= n - 1 and _gap(t, A, P) > n - 1):
+                    l2.add(A)
+            for S in np.nonzero(srcleaf)[0]:
+                if t.box_levels[S] < lp and _gap(t, S, P) >= n and _gap(t, S, Q) < n:
+                    l4.add(S)
+        assert _csr(tr.from_sep_siblings_starts, tr.from_sep_siblings_lists, it) == sorted(l2)
+        assert _csr(tr.from_sep_bigger_starts, tr.from_sep_bigger_lists, it) == sorted(l4)
+
+
+@pytest.mark.parametrize("nel,grid_n,w", [(24, 24, 16), (64, 64, 64), (40, 50, 8)])
+def test_interaction_lists_complete(oracle_lib, nel, grid_n, w):
+    """Constant-one completeness: through lists 1-4 every target box sees every
+    source exactly once (test_cost_model.py:735-738)."""
+    cfg = problems.config1(nelements=nel, grid_n=grid_n)
+    cfg = problems.refine_until_associated(cfg, O.assoc_status_and_flags)
+    g = O.GeometryData(cfg.discr, cfg.targets, cfg.target_sides, max_leaf_refine_weight=w)
+    t, tr = g.tree, g.trav
+    ttp_index = {b: i for i, b in enumerate(tr.target_or_target_parent_boxes)}
+
+    def add_box(cnt, b, sub):
+        s0 = t.box_source_starts[b]
+        cnt[s0:s0 + (t.box_source_counts_cumul[b] if sub
+                     else t.box_source_counts_nonchild[b])] += 1
+
+    for it, B in enumerate(tr.target_boxes):
+        cnt = np.zeros(t.nsources, int)
+        for S in _csr(tr.neighbor_source_boxes_starts, tr.neighbor_source_boxes_lists, it):
+            add_box(cnt, S, False)
+        for D in _csr(tr.from_sep_smaller_starts, tr.from_sep_smaller_lists, it):
+            add_box(cnt, D, True)
+        for P in _ancestors(t, B):
+            i = ttp_index[P]
+            for A in _csr(tr.from_sep_siblings_starts, tr.from_sep_siblings_lists, i):
+                add_box(cnt, A, True)
+            for S in _csr(tr.from_sep_bigger_starts, tr.from_sep_bigger_lists, i):
+                add_box(cnt, S, False)
+        assert np.all(cnt == 1), (B, np.unique(cnt))
+
+
+def test_association_brute_equals_accelerated(geo):
+    cfg, g = geo
+    d = cfg.discr
+    args = (d.nodes, d.tunnel_radii(), d.centers(), d.center_radii() * (1 + 1e-13),
+            cfg.targets, cfg.target_sides)
+    a = O.target_assoc(*args, accelerated=True)
+    b = O.target_assoc(*args, accelerated=False)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_association_properties(geo):
+    """Side and distance invariants (test_global_qbx.py:464-506)."""
+    cfg, g = geo
+    d = cfg.discr
+    ttc, status = g.target_to_center, g.target_status
+    c, r = d.centers(), d.center_radii()
+    tg, sides = cfg.targets, cfg.target_sides
+    has = ttc >= 0
+    assert np.all(status[has] == 2) and np.all(status[~has] == 0)
+    dist = np.hypot(*(tg[:, has] - c[:, ttc[has]]))
+    assert np.all(dist <= r[ttc[has]] * (1 + 1e-12))
+    cside = 2 * (ttc[has] % 2) - 1
+    assert np.all(cside * sides[has] >= 0)
+    # surface targets all associated, on their requested side
+    nsurf = d.ndofs
+    assert np.all(ttc[:nsurf] >= 0)
+    # marked <=> some stage-1 node within its tunnel radius
+    dd = np.hypot(tg[0][:, None] - d.nodes[0][None], tg[1][:, None] - d.nodes[1][None])
+    marked = np.any(dd <= d.tunnel_radii()[None, :], axis=1)
+    assert np.array_equal(marked, status > 0)
+    # chosen center minimises the relative distance on its side
+    for i in np.nonzero(has)[0][::7]:
+        side = 2 * (ttc[i] % 2) - 1
+        cand = np.arange(len(r))[(2 * (np.arange(len(r)) % 2) - 1) == side]
+        rel = np.hypot(*(tg[:, i:i + 1] - c[:, cand])) / r[cand]
+        assert rel.min() >= np.hypot(*(tg[:, i] - c[:, ttc[i]])) / r[ttc[i]] - 1e-12
+
+
+def test_association_failure_and_refine_flags(oracle_lib):
+    """Targets just inside a circle that demand exterior centers cannot be
+    associated (test_global_qbx.py:511-566); the refine flags name the
+    elements whose sources are within tunnel distance."""
+    from pytential_amd.discretization import discretize_curve, ellipse
+    d = discretize_curve(ellipse(1, 1), 32, 8, 16)
+    th = np.linspace(0, 2 * np.pi, 50, endpoint=False)
+    tg = 0.999 * np.stack([np.cos(th), np.sin(th)])
+    sides = np.full(50, +1, np.int32)
+    ttc, status = O.target_assoc(d.nodes, d.tunnel_radii(), d.centers(),
+                                 d.center_radii() * (1 + 1e-13), tg, sides)
+    assert np.all(status == 1) and np.all(ttc == -1)
+    flags = O.refine_flags_for_failed(d.nodes, d.tunnel_radii(),
+                                      d.element_to_source_starts(), tg, status)
+    dd = np.hypot(tg[0][:, None] - d.nodes[0][None], tg[1][:, None] - d.nodes[1][None])
+    close_nodes = np.nonzero(np.any(dd <= d.tunnel_radii()[None, :], axis=0))[0]
+    expect = np.zeros(d.nelements, np.int32)
+    expect[close_nodes // d.order] = 1
+    assert np.array_equal(flags, expect)
+
+
+def test_refinement_resolves_failed_targets(oracle_lib):
+    cfg = problems.config1(nelements=64, grid_n=96)
+    st, _ = O.assoc_status_and_flags(cfg.discr, cfg.targets, cfg.target_sides)
+    assert np.any(st == 1)
+    ref = problems.refine_until_associated(cfg, O.assoc_status_and_flags)
+    st2, _ = O.assoc_status_and_flags(ref.discr, ref.targets, ref.target_sides)
+    assert not np.any(st2 == 1)
+    assert ref.discr.nelements > cfg.discr.nelements
+    # the density is carried over by interpolation: sin(3t) part exact to 1e-10
+    smooth = np.sin(3 * ref.discr.t_nodes)
+    cfg_smooth = problems.config1(nelements=64, grid_n=96)
+    cfg_smooth.density = np.sin(3 * cfg_smooth.discr.t_nodes)
+    r2 = problems.refine_until_associated(cfg_smooth, O.assoc_status_and_flags)
+    assert np.max(np.abs(r2.density - smooth)) < 1e-6
