@@ -11,9 +11,10 @@
  *   coordinates      SoA double, axis d of point i at x[d*n + i]
  *   indices          int32
  *   complex values   interleaved (re, im) doubles
- *   expansions       [nboxes][ncoeff] / [ncenters][ncoeff_qbx] complex, row-major
- *                    Laplace 2D: coefficient k at k (k = 0..p)
- *                    Helmholtz 2D: coefficient n at n + pmax (|n| <= p)
+ *   expansions       box expansions level-compact (qbx_expansion_size);
+ *                    QBX expansions [ncenters][ncoeff_qbx] complex, row-major;
+ *                    Laplace 2D coefficient k at k (k = 0..p),
+ *                    Helmholtz 2D coefficient n at n + p (|n| <= p)
  *
  * Each entry point names the reference interface it replaces.
  */
@@ -32,7 +33,7 @@ enum {
     QBX_ERR_UNSUPPORTED = 4
 };
 
-enum { QBX_LAPLACE2D = 0, QBX_HELMHOLTZ2D = 1 };
+enum { QBX_LAPLACE2D = 0, QBX_HELMHOLTZ2D = 1, QBX_LAPLACE3D = 2, QBX_HELMHOLTZ3D = 3 };
 enum { QBX_SOURCE_S = 0, QBX_SOURCE_D = 1 };
 
 typedef struct qbx_ctx qbx_ctx;
@@ -141,8 +142,15 @@ typedef struct {
     const int *level_orders;/* host array [nlevels]: fmm_level_to_order */
 } qbx_kernel_desc;
 
-/* number of complex coefficients per box / per center (storage strides) */
-int qbx_ncoeff_fmm(const qbx_kernel_desc *kd);
+/* Box expansions (multipoles and locals) are level-compact: the boxes of
+ * level l are stored contiguously with ncoeff(p_l) complex coefficients each
+ * (Laplace p_l + 1, Helmholtz 2 p_l + 1 with coefficient n at n + p_l),
+ * level l starting at complex offset level_offsets[l].  Total size in complex
+ * entries, or -1 on error: */
+long qbx_expansion_size(const qbx_kernel_desc *kd, const qbx_tree *tree);
+int qbx_expansion_level_offsets(const qbx_kernel_desc *kd, const qbx_tree *tree,
+                                long *level_offsets /* [nlevels + 1] */);
+/* complex coefficients per QBX center (Laplace p_q + 1, Helmholtz 2 p_q + 1) */
 int qbx_ncoeff_qbx(const qbx_kernel_desc *kd);
 
 /* Source data in tree order.  Replaces QBXExpansionWrangler.reorder_sources
@@ -165,7 +173,7 @@ int qbx_upsample(qbx_ctx *ctx, void *stream, int nelements, int n1, int nf,
 
 /* The stages below mirror the wrangler methods called by drive_fmm
  * (pytential/qbx/fmm.py:436-565).  `accumulate` = 0 overwrites the output.
- *   mpole/local: [nboxes][qbx_ncoeff_fmm] complex
+ *   mpole/local: level-compact, qbx_expansion_size complex entries
  *   qbx_exp:     [ncenters][qbx_ncoeff_qbx] complex
  *   pot:         [ntargets of the tree] complex, tree target order */
 int qbx_form_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
@@ -232,6 +240,28 @@ int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
                   const qbx_trav *trav, const qbx_geo *geo,
                   const qbx_kernel_desc *kd, const double *strengths_tree,
                   const double *normals_tree, double *out_user);
+
+/* ---- target-specific QBX (3D) ---------------------------------------------
+ * Replaces eval_target_specific_qbx_locals (pytential/qbx/target_specific/
+ * impl.pyx:566-785, called from pytential/qbx/fmmlib.py:587-639): for each
+ * (center c, target t) pair of the association and each source s of list 1 of
+ * c's target box, the order-p_q expansion about c evaluated directly at t:
+ *   QBX_LAPLACE3D   S  q sum_n rho_t^n / rho_s^(n+1) P_n(cos th)
+ *   QBX_HELMHOLTZ3D S  i k q sum_n (2n+1) j_n(k rho_t) h_n(k rho_s) P_n(cos th)
+ * (fmmlib scaling: 4 pi times the Green's function), op QBX_TSQBX_SP its
+ * target gradient, QBX_TSQBX_D its dipole-direction source derivative with
+ * strength q.  Tree must be 3D; k = k_re + i k_im; p_q in [0, 12].
+ * strengths_tree complex [nsources] (weights * density, tree order);
+ * dipvec_tree [3*nsources] (op D only); pot complex [ntargets] (ops S, D) or
+ * grad complex [3][ntargets] (op Sp), tree target order.  Entries at QBX
+ * targets are overwritten; other entries are left untouched. */
+enum { QBX_TSQBX_S = 0, QBX_TSQBX_D = 1, QBX_TSQBX_SP = 2 };
+int qbx_eval_target_specific_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
+                                        const qbx_trav *trav, const qbx_geo *geo,
+                                        int kernel, int op, double k_re, double k_im,
+                                        int qbx_order, const double *strengths_tree,
+                                        const double *dipvec_tree, double *pot,
+                                        double *grad);
 
 #ifdef __cplusplus
 }

@@ -110,12 +110,11 @@ def _declare(L):
                           P, ct.c_double, P, ct.c_double, P, P]
     L.orc_p2p.argtypes = [ct.POINTER(KParams), ct.c_int, P, P, P, ct.c_int, P, P]
     L.orc_bessel_jy.argtypes = [ct.c_int, ct.c_double, P, P]
-    if hasattr(L, "orc_tsqbx"):
-        L.orc_tsqbx.argtypes = [ct.c_int, ct.c_int, ct.c_double, ct.c_double,
-                                ct.c_int, P, P, P, P, P, P, ct.c_int, P, P,
-                                ct.c_int, P, P, P, P, P]
-        L.orc_sph_jn.argtypes = [ct.c_int, ct.c_double, ct.c_double, ct.c_double, P, P]
-        L.orc_sph_hn.argtypes = [ct.c_int, ct.c_double, ct.c_double, ct.c_double, P, P]
+    L.orc_tsqbx.argtypes = [ct.c_int, ct.c_int, ct.c_double, ct.c_double, ct.c_int,
+                            ct.c_int, P, ct.c_int, P, ct.c_int, P,
+                            ct.c_int, P, P, P, P, P, P, P, P, P, P, P, P]
+    L.orc_sph_jn.argtypes = [ct.c_int, ct.c_double, ct.c_double, ct.c_double, P, P]
+    L.orc_sph_hn.argtypes = [ct.c_int, ct.c_double, ct.c_double, ct.c_double, P, P]
 
 
 def _ptr(a):
@@ -430,3 +429,43 @@ def assoc_status_and_flags(discr, targets, sides,
                                     discr.element_to_source_starts(), targets, status) \
         if np.any(status == 1) else np.zeros(discr.nelements, np.int32)
     return status, flags
+
+
+def sph_jn(nmax, z):
+    """Spherical Bessel j_n(z), n = 0..nmax, and derivatives (complex z)."""
+    j = np.zeros(nmax + 1, np.complex128)
+    jd = np.zeros(nmax + 1, np.complex128)
+    lib().orc_sph_jn(nmax, complex(z).real, complex(z).imag, 1.0, _ptr(j), _ptr(jd))
+    return j, jd
+
+
+def sph_hn(nmax, z):
+    """Spherical Hankel h_n^(1)(z), n = 0..nmax, and derivatives (complex z)."""
+    h = np.zeros(nmax + 1, np.complex128)
+    hd = np.zeros(nmax + 1, np.complex128)
+    lib().orc_sph_hn(nmax, complex(z).real, complex(z).imag, 1.0, _ptr(h), _ptr(hd))
+    return h, hd
+
+
+TSQBX_OPS = {"S": 0, "D": 1, "Sp": 2}
+
+
+def tsqbx(kernel, op, k, order, sources, targets, centers, qbx_centers, c2tb,
+          ctt_starts, ctt_lists, l1_starts, l1_lists, box_src_starts, box_src_counts,
+          charge, dipvec=None):
+    """Target-specific QBX (impl.pyx:566-785) on tree-ordered arrays.
+    Returns pot [nt] (S, D) or grad [3, nt] (Sp), complex."""
+    src, tgt, ctr = _f64(sources), _f64(targets), _f64(centers)
+    ints = [_i32(a) for a in (qbx_centers, c2tb, ctt_starts, ctt_lists, l1_starts,
+                              l1_lists, box_src_starts, box_src_counts)]
+    q = _c128(charge)
+    dv = _f64(dipvec) if dipvec is not None else None
+    nt = tgt.shape[1]
+    pot = np.zeros(nt, np.complex128)
+    grad = np.zeros((3, nt), np.complex128)
+    rc = lib().orc_tsqbx(0 if kernel == "laplace" else 1, TSQBX_OPS[op], complex(k).real,
+                         complex(k).imag, order, src.shape[1], _ptr(src), nt, _ptr(tgt),
+                         ctr.shape[1], _ptr(ctr), len(ints[0]), *[_ptr(a) for a in ints],
+                         _ptr(q), _ptr(dv), _ptr(pot), _ptr(grad))
+    assert rc == 0
+    return grad if op == "Sp" else pot

@@ -14,7 +14,8 @@ LIB_PATH = os.path.join(HERE, "libqbx_mi355x.so")
 
 QBX_OK = 0
 QBX_ERR_TARGET_ASSOC_FAILED = 3
-LAPLACE2D, HELMHOLTZ2D = 0, 1
+LAPLACE2D, HELMHOLTZ2D, LAPLACE3D, HELMHOLTZ3D = 0, 1, 2, 3
+TSQBX_S, TSQBX_D, TSQBX_SP = 0, 1, 2
 SOURCE_S, SOURCE_D = 0, 1
 
 P = ct.c_void_p
@@ -58,7 +59,8 @@ _SIGS = {
     "qbx_geo_sizes": [P, IP],
     "qbx_geo_export": [P, ct.c_char_p, P, L, P],
     "qbx_geo_field_bytes": [P, ct.c_char_p],
-    "qbx_ncoeff_fmm": [ct.POINTER(KernelDesc)],
+    "qbx_expansion_size": [ct.POINTER(KernelDesc), P],
+    "qbx_expansion_level_offsets": [ct.POINTER(KernelDesc), P, ct.POINTER(L)],
     "qbx_ncoeff_qbx": [ct.POINTER(KernelDesc)],
     "qbx_reorder_sources": [P, P, P, P, P, P, P, P],
     "qbx_upsample": [P, P, I, I, I, P, P, P],
@@ -77,6 +79,7 @@ _SIGS = {
     "qbx_eval_qbx_expansions": [P, P, P, P, ct.POINTER(KernelDesc), P, P],
     "qbx_finalize_potentials": [P, P, P, P, P, P],
     "qbx_drive_fmm": [P, P, P, P, P, ct.POINTER(KernelDesc), P, P, P],
+    "qbx_eval_target_specific_qbx_locals": [P, P, P, P, P, I, I, D, D, I, P, P, P, P],
 }
 
 
@@ -98,7 +101,8 @@ def lib():
         for name, args in _SIGS.items():
             fn = getattr(lb, name)
             fn.argtypes = args
-            fn.restype = L if name.endswith("_field_bytes") else I
+            fn.restype = L if (name.endswith("_field_bytes")
+                               or name == "qbx_expansion_size") else I
         _lib = lb
     return _lib
 

@@ -77,6 +77,9 @@ struct qbx_ctx {
     size_t scratch_bytes = 0;
     // FMM workspaces for qbx_drive_fmm
     qbx::dbuf<double> ws_mpole, ws_local, ws_qbx, ws_pot;
+    // Helmholtz translation tables and the (tree, k, orders) they were built for
+    qbx::dbuf<double> ws_tab_m2l, ws_tab_mm;
+    std::vector<double> tab_key;
 };
 
 struct qbx_tree {
@@ -96,6 +99,7 @@ struct qbx_tree {
 
 struct qbx_trav {
     int nlevels = 0;
+    int nsep = 2;  // well_sep_is_n_away
     int nsource_boxes = 0, nsource_parent_boxes = 0, ntarget_boxes = 0, nttp_boxes = 0;
     int n1 = 0, n2 = 0, n3 = 0, n4 = 0;
     std::vector<int> h_level_start_source_box_nrs, h_level_start_source_parent_box_nrs,

@@ -51,10 +51,12 @@ __device__ __forceinline__ void stc(double *p, cplx v)
 }
 
 // H_0(x), H_1(x) for x > 0
+template <bool WANT0, bool WANT1>
+__device__ __forceinline__ void hankel01_fast(double x, cplx &h0, cplx &h1);
+
 __device__ __forceinline__ void hankel01(double x, cplx &h0, cplx &h1)
 {
-    h0 = {j0(x), y0(x)};
-    h1 = {j1(x), y1(x)};
+    hankel01_fast<true, true>(x, h0, h1);
 }
 
 __device__ __forceinline__ int miller_start(int nmax, double x)
@@ -89,6 +91,92 @@ __device__ __forceinline__ double miller_j(double x, int nmax, Consume consume, 
     }
     consume(0, f);
     return 1.0 / (f + 2.0 * sum);
+}
+
+// ---------------------------------------------------------------------------
+// H_0 and H_1 of real argument, evaluated together.  For 0 < x <= 4 the power
+// series of Abramowitz & Stegun 9.1.10 / 9.1.11 in t = x^2/4:
+//   J0 = sum A_k t^k,  J1 = (x/2) sum B_k t^k,
+//   Y0 = (2/pi) ln(x/2) J0 - (1/pi) sum C_k t^k,
+//   Y1 = -2/(pi x) + (2/pi) ln(x/2) J1 - (x/(2 pi)) sum D_k t^k,
+//   A_k = (-1)^k/(k!)^2,  B_k = (-1)^k/(k!(k+1)!),
+//   C_k = (-1)^k 2 psi(k+1)/(k!)^2,  D_k = (-1)^k (psi(k+1)+psi(k+2))/(k!(k+1)!)
+// (coefficients generated at 40 digits).  Relative error of H0 and H1 against
+// scipy: 10 terms < 1.1e-15 for x <= 1.5, 14 terms < 9e-16 for x <= 3, 18
+// terms < 1.8e-15 for x <= 4; beyond, the device library.
+// ---------------------------------------------------------------------------
+static __constant__ double c_ser_A[18] = {
+    1.0, -1.0, 0.25,
+    -0.027777777777777776, 0.001736111111111111, -6.944444444444444e-05,
+    1.9290123456790124e-06, -3.936759889140842e-08, 6.151187326782565e-10,
+    -7.594058428126624e-12, 7.594058428126623e-14, -6.276081345559193e-16,
+    4.358389823304995e-18, -2.5789288895295828e-20, 1.3157800456783586e-22,
+    -5.8479113141260385e-25, 2.2843403570804838e-27, -7.904291893012054e-30,
+};
+static __constant__ double c_ser_B[18] = {
+    1.0, -0.5, 0.08333333333333333,
+    -0.006944444444444444, 0.00034722222222222224, -1.1574074074074073e-05,
+    2.755731922398589e-07, -4.920949861426052e-09, 6.834652585313961e-11,
+    -7.594058428126623e-13, 6.903689480115112e-15, -5.230067787965994e-17,
+    3.352607556388458e-19, -1.842092063949702e-21, 8.771866971189057e-24,
+    -3.654944571328774e-26, 1.3437296218120491e-28, -4.391273273895586e-31,
+};
+static __constant__ double c_ser_C[18] = {
+    -1.1544313298030657, -0.8455686701969343, 0.46139216754923357,
+    -0.0697843149128778, 0.0052295752376104185, -0.00023696078728219452,
+    7.2252482063984075e-06, -1.5870193042608417e-07, 2.6334973460771295e-09,
+    -3.419988145411616e-11, 3.571869313974148e-13, -3.0660686806416945e-15,
+    2.2018541919451488e-17, -1.3425481319958094e-19, 7.03770394527961e-22,
+    -3.205840570979285e-24, 1.2808357275022893e-26, -4.5249492039395507e-29,
+};
+static __constant__ double c_ser_D[18] = {
+    -0.15443132980306573, -0.6727843350984671, 0.18157516696085563,
+    -0.019182189839330562, 0.001115359491966528, -4.142247689271143e-05,
+    1.071545914091181e-06, -2.045286003593878e-08, 3.002048746589188e-10,
+    -3.495928729692882e-12, 3.309914735250272e-14, -2.5986411321011286e-16,
+    1.7195232826992565e-18, -9.721207518823618e-21, 4.750281743327667e-23,
+    -2.0264937604328578e-25, 7.613370727767116e-28, -2.5382566314880593e-30,
+};
+
+template <int K, bool WANT0, bool WANT1>
+__device__ __forceinline__ void hankel01_series(double x, cplx &h0, cplx &h1)
+{
+    const double t = 0.25 * x * x;
+    double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+#pragma unroll
+    for (int k = K - 1; k >= 0; --k) {
+        if (WANT0) {
+            a = a * t + c_ser_A[k];
+            c = c * t + c_ser_C[k];
+        }
+        if (WANT1) {
+            b = b * t + c_ser_B[k];
+            d = d * t + c_ser_D[k];
+        }
+    }
+    const double lg = log(0.5 * x);
+    constexpr double TWO_OVER_PI = 0.63661977236758134308;
+    constexpr double ONE_OVER_PI = 0.31830988618379067154;
+    if (WANT0) h0 = {a, TWO_OVER_PI * lg * a - ONE_OVER_PI * c};
+    if (WANT1) {
+        double j1v = 0.5 * x * b;
+        h1 = {j1v, -TWO_OVER_PI / x + TWO_OVER_PI * lg * j1v - 0.5 * x * ONE_OVER_PI * d};
+    }
+}
+
+template <bool WANT0, bool WANT1>
+__device__ __forceinline__ void hankel01_fast(double x, cplx &h0, cplx &h1)
+{
+    if (x <= 1.5) {
+        hankel01_series<10, WANT0, WANT1>(x, h0, h1);
+    } else if (x <= 3.0) {
+        hankel01_series<14, WANT0, WANT1>(x, h0, h1);
+    } else if (x <= 4.0) {
+        hankel01_series<18, WANT0, WANT1>(x, h0, h1);
+    } else {
+        if (WANT0) h0 = {j0(x), y0(x)};
+        if (WANT1) h1 = {j1(x), y1(x)};
+    }
 }
 
 }  // namespace qbx

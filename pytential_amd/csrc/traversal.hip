@@ -305,6 +305,7 @@ static int traverse(qbx_ctx *ctx, hipStream_t st, const qbx_tree *T, int n, int 
     t.src_cumul = T->box_source_counts_cumul.p;
     int nb = T->nboxes;
     R->nlevels = T->nlevels;
+    R->nsep = n;
 
     dbuf<int> f_src, f_srcpar, f_tgt, f_ttp;
     QBX_TRY(f_src.alloc(nb));

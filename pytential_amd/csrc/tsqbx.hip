@@ -1,0 +1,357 @@
+// Target-specific QBX (TSQBX) for 3D Laplace and Helmholtz: the GPU
+// counterpart of eval_target_specific_qbx_locals
+// (pytential/qbx/target_specific/impl.pyx:566-785, called from
+// pytential/qbx/fmmlib.py:587-639).
+//
+// For every QBX center c, every target t associated with c and every source s
+// in list 1 of c's target box:
+//   Laplace   S   q   sum_{n<=p} rho_t^n / rho_s^{n+1} P_n(cos th)
+//             Sp  grad_t of S,   D  dipstr nu . grad_s of S
+//   Helmholtz S   i k q sum_{n<=p} (2n+1) j_n(k rho_t) h_n(k rho_s) P_n(cos th)
+//             Sp  grad_t,        D  dipstr nu . grad_s
+// (fmmlib convention: 4 pi times the free-space Green's function), with
+// rho_t = |t-c|, rho_s = |s-c| and th the angle between t-c and s-c.
+//
+// Mapping: TSQ_LANES lanes per (center, target) pair split the sources; the
+// Legendre and spherical-Hankel sequences run as recurrences per source, the
+// target's j_n(k rho_t) (complex k) comes from Miller's backward recurrence
+// into registers; lane groups reduce with shuffles.  Output overwrites
+// pot / grad at the QBX targets, like the reference.
+#include "qbx_internal.h"
+
+namespace qbx {
+
+struct zc {
+    double x, y;
+};
+__device__ __forceinline__ zc operator+(zc a, zc b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ zc operator-(zc a, zc b) { return {a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ zc operator*(zc a, zc b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+__device__ __forceinline__ zc operator*(double s, zc a) { return {s * a.x, s * a.y}; }
+__device__ __forceinline__ zc zinv(zc a)
+{
+    const double d = 1.0 / (a.x * a.x + a.y * a.y);
+    return {a.x * d, -a.y * d};
+}
+__device__ __forceinline__ double zabs(zc a) { return sqrt(a.x * a.x + a.y * a.y); }
+__device__ __forceinline__ zc zsin(zc z)
+{
+    return {sin(z.x) * cosh(z.y), cos(z.x) * sinh(z.y)};
+}
+__device__ __forceinline__ zc zcos(zc z)
+{
+    return {cos(z.x) * cosh(z.y), -sin(z.x) * sinh(z.y)};
+}
+__device__ __forceinline__ zc zexp_i(zc z)  // e^{i z}
+{
+    const double e = exp(-z.y);
+    return {e * cos(z.x), e * sin(z.x)};
+}
+
+constexpr int TSQ_LANES = 16;
+
+// j_n(z), j_n'(z) for n = 0..P (complex z) into registers
+template <int P>
+__device__ void sph_j(zc z, zc *j, zc *jd)
+{
+    const double az = zabs(z);
+    if (az < 1e-300) {
+#pragma unroll
+        for (int n = 0; n <= P; ++n) {
+            j[n] = {n == 0 ? 1.0 : 0.0, 0.0};
+            jd[n] = {n == 1 ? 1.0 / 3.0 : 0.0, 0.0};
+        }
+        return;
+    }
+    const double m = fmax((double)(P + 1), az);
+    const int M = (int)m + 30 + (int)sqrt(40.0 * m);
+    const zc iz = zinv(z);
+    zc fp1 = {0, 0}, f = {1e-30, 0};
+    zc jj[P + 2];
+#pragma unroll
+    for (int n = 0; n <= P + 1; ++n) jj[n] = {0, 0};
+    for (int n = M; n >= 1; --n) {
+#pragma unroll
+        for (int q = 0; q <= P + 1; ++q)
+            if (q == n) jj[q] = f;
+        const zc fm1 = ((2.0 * n + 1.0) * iz) * f - fp1;
+        fp1 = f;
+        f = fm1;
+        if (fabs(f.x) + fabs(f.y) > 1e250) {
+            f = 1e-250 * f;
+            fp1 = 1e-250 * fp1;
+#pragma unroll
+            for (int q = 0; q <= P + 1; ++q) jj[q] = 1e-250 * jj[q];
+        }
+    }
+    jj[0] = f;
+    const zc j0 = zsin(z) * iz;
+    const zc j1 = zsin(z) * iz * iz - zcos(z) * iz;
+    const zc sc = zabs(j0) >= zabs(j1) ? j0 * zinv(jj[0]) : j1 * zinv(jj[1]);
+#pragma unroll
+    for (int n = 0; n <= P + 1; ++n) jj[n] = jj[n] * sc;
+#pragma unroll
+    for (int n = 0; n <= P; ++n) {
+        j[n] = jj[n];
+        jd[n] = n == 0 ? zc{-jj[1].x, -jj[1].y} : jj[n - 1] - ((n + 1.0) * iz) * jj[n];
+    }
+}
+
+template <int KERNEL, int OP, int P>
+__global__ void __launch_bounds__(256) k_tsqbx(double kre, double kim, int nctt,
+                                               const int *ctt_lists, const int *tree_ttc,
+                                               const int *c2tb, const double *ctr, int nc,
+                                               const double *tgt, int nt, const double *src,
+                                               int ns, const int *l1s, const int *l1l,
+                                               const int *sstart, const int *scount,
+                                               const double *q, const double *dip,
+                                               double *pot, double *grad)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int pr = gid / TSQ_LANES, lane = gid % TSQ_LANES;
+    const bool active = pr < nctt;
+    const zc k = {kre, kim};
+    zc res = {0, 0};
+    zc g[3] = {{0, 0}, {0, 0}, {0, 0}};
+    int ti = 0;
+    if (active) {
+        ti = ctt_lists[pr];
+        const int c = tree_ttc[ti];
+        const int tb = c2tb[c];
+        double tc[3];
+        for (int m = 0; m < 3; ++m) tc[m] = tgt[(size_t)m * nt + ti] - ctr[(size_t)m * nc + c];
+        const double rt = sqrt(tc[0] * tc[0] + tc[1] * tc[1] + tc[2] * tc[2]);
+        zc jv[P + 1], jdv[P + 1];
+        if (KERNEL == 1) sph_j<P>(rt * k, jv, jdv);
+        for (int e = l1s[tb]; e < l1s[tb + 1]; ++e) {
+            const int sb = l1l[e];
+            const int s1 = sstart[sb] + scount[sb];
+            for (int s = sstart[sb] + lane; s < s1; s += TSQ_LANES) {
+                double sc[3];
+                for (int m = 0; m < 3; ++m) sc[m] = src[(size_t)m * ns + s] - ctr[(size_t)m * nc + c];
+                const double rs = sqrt(sc[0] * sc[0] + sc[1] * sc[1] + sc[2] * sc[2]);
+                const double ct = (tc[0] * sc[0] + tc[1] * sc[1] + tc[2] * sc[2]) / (rt * rs);
+                const zc qs = {q[2 * s], q[2 * s + 1]};
+                // Legendre recurrence state
+                double Pm2 = 1.0, Pm1 = ct, dPm2 = 0.0, dPm1 = 1.0;
+                if (KERNEL == 0) {
+                    if (OP == 0) {
+                        double r = 1.0 / rs, acc = 0.0;
+#pragma unroll
+                        for (int n = 0; n <= P; ++n) {
+                            double Pn;
+                            if (n == 0) Pn = 1.0;
+                            else if (n == 1) Pn = ct;
+                            else {
+                                Pn = ((2.0 * n - 1.0) * ct * Pm1 - (n - 1.0) * Pm2) / n;
+                                Pm2 = Pm1;
+                                Pm1 = Pn;
+                            }
+                            acc += r * Pn;
+                            r *= rt / rs;
+                        }
+                        res = res + acc * qs;
+                    } else {
+                        double gv[3] = {0, 0, 0};
+                        double Rn = 1.0 / (rs * rs);
+#pragma unroll
+                        for (int n = 0; n <= P; ++n) {
+                            double Pn, dPn;
+                            if (n == 0) {
+                                Pn = 1.0;
+                                dPn = 0.0;
+                            } else if (n == 1) {
+                                Pn = ct;
+                                dPn = 1.0;
+                            } else {
+                                Pn = ((2.0 * n - 1.0) * ct * Pm1 - (n - 1.0) * Pm2) / n;
+                                dPn = ((2.0 * n - 1.0) * (Pm1 + ct * dPm1) - (n - 1.0) * dPm2) / n;
+                                Pm2 = Pm1;
+                                Pm1 = Pn;
+                                dPm2 = dPm1;
+                                dPm1 = dPn;
+                            }
+                            if (OP == 2) {
+                                if (n >= 1) {
+                                    for (int m = 0; m < 3; ++m)
+                                        gv[m] += Rn * (n * (tc[m] / rt) * Pn +
+                                                       (sc[m] / rs - ct * tc[m] / rt) * dPn);
+                                    Rn *= rt / rs;
+                                }
+                            } else {
+                                for (int m = 0; m < 3; ++m)
+                                    gv[m] += Rn * (-(n + 1.0) * (sc[m] / rs) * Pn +
+                                                   (tc[m] / rt - ct * sc[m] / rs) * dPn);
+                                Rn *= rt / rs;
+                            }
+                        }
+                        if (OP == 2) {
+                            for (int m = 0; m < 3; ++m) g[m] = g[m] + gv[m] * qs;
+                        } else {
+                            const double dv = dip[s] * gv[0] + dip[(size_t)ns + s] * gv[1] +
+                                              dip[2 * (size_t)ns + s] * gv[2];
+                            res = res + dv * qs;
+                        }
+                    }
+                } else {
+                    // spherical Hankel h_n(k rho_s) by forward recurrence
+                    const zc z = rs * k;
+                    const zc iz = zinv(z);
+                    const zc ei = zexp_i(z);
+                    const zc h0 = ei * zinv(zc{-z.y, z.x});  // e^{iz} / (iz)
+                    const zc h1 = h0 * (iz - zc{0, 1});
+                    zc hm2 = h0, hm1 = h1;
+                    zc acc = {0, 0};
+                    zc av[3] = {{0, 0}, {0, 0}, {0, 0}};
+#pragma unroll
+                    for (int n = 0; n <= P; ++n) {
+                        double Pn, dPn;
+                        if (n == 0) {
+                            Pn = 1.0;
+                            dPn = 0.0;
+                        } else if (n == 1) {
+                            Pn = ct;
+                            dPn = 1.0;
+                        } else {
+                            Pn = ((2.0 * n - 1.0) * ct * Pm1 - (n - 1.0) * Pm2) / n;
+                            dPn = ((2.0 * n - 1.0) * (Pm1 + ct * dPm1) - (n - 1.0) * dPm2) / n;
+                            Pm2 = Pm1;
+                            Pm1 = Pn;
+                            dPm2 = dPm1;
+                            dPm1 = dPn;
+                        }
+                        zc hn, hprev;
+                        if (n == 0) {
+                            hn = h0;
+                            hprev = {0, 0};
+                        } else if (n == 1) {
+                            hn = h1;
+                            hprev = h0;
+                        } else {
+                            hn = ((2.0 * n - 1.0) * iz) * hm1 - hm2;
+                            hprev = hm1;
+                            hm2 = hm1;
+                            hm1 = hn;
+                        }
+                        const double w = 2.0 * n + 1.0;
+                        if (OP == 0) {
+                            acc = acc + (w * Pn) * (jv[n] * hn);
+                        } else if (OP == 2) {
+                            const zc a = (w * Pn) * (k * jdv[n]);
+                            const zc b = (w * dPn) * jv[n];
+                            for (int m = 0; m < 3; ++m)
+                                av[m] = av[m] + (1.0 / rt) * (hn * (tc[m] * a +
+                                                       (sc[m] / rs - ct * tc[m] / rt) * b));
+                        } else {
+                            // h_n' = h_{n-1} - (n+1)/z h_n,  h_0' = -h_1
+                            const zc hd = n == 0 ? zc{-h1.x, -h1.y} : hprev - ((n + 1.0) * iz) * hn;
+                            const zc a = (w * Pn) * (k * hd);
+                            const zc b = (w * dPn) * hn;
+                            for (int m = 0; m < 3; ++m)
+                                av[m] = av[m] + (1.0 / rs) * (jv[n] * (sc[m] * a +
+                                                       (tc[m] / rt - ct * sc[m] / rs) * b));
+                        }
+                    }
+                    const zc ikq = zc{-k.y, k.x} * qs;  // i k q
+                    if (OP == 0) {
+                        res = res + ikq * acc;
+                    } else if (OP == 2) {
+                        for (int m = 0; m < 3; ++m) g[m] = g[m] + ikq * av[m];
+                    } else {
+                        const zc dv = dip[s] * av[0] + dip[(size_t)ns + s] * av[1] +
+                                      dip[2 * (size_t)ns + s] * av[2];
+                        res = res + ikq * dv;
+                    }
+                }
+            }
+        }
+    }
+    for (int m = TSQ_LANES / 2; m >= 1; m >>= 1) {
+        res.x += __shfl_xor(res.x, m, TSQ_LANES);
+        res.y += __shfl_xor(res.y, m, TSQ_LANES);
+        for (int d = 0; d < 3; ++d) {
+            g[d].x += __shfl_xor(g[d].x, m, TSQ_LANES);
+            g[d].y += __shfl_xor(g[d].y, m, TSQ_LANES);
+        }
+    }
+    if (active && lane == 0) {
+        if (OP == 2) {
+            for (int d = 0; d < 3; ++d) {
+                grad[2 * ((size_t)d * nt + ti)] = g[d].x;
+                grad[2 * ((size_t)d * nt + ti) + 1] = g[d].y;
+            }
+        } else {
+            pot[2 * (size_t)ti] = res.x;
+            pot[2 * (size_t)ti + 1] = res.y;
+        }
+    }
+}
+
+template <int KERNEL, int OP>
+static int launch_p(int order, dim3 grid, hipStream_t st, double kre, double kim, int nctt,
+                    const int *a0, const int *a1, const int *a2, const double *ctr, int nc,
+                    const double *tgt, int nt, const double *src, int ns, const int *l1s,
+                    const int *l1l, const int *ss, const int *sc, const double *q,
+                    const double *dip, double *pot, double *grad)
+{
+#define TSQ_CASE(PP)                                                                        \
+    case PP:                                                                                \
+        hipLaunchKernelGGL((k_tsqbx<KERNEL, OP, PP>), grid, dim3(256), 0, st, kre, kim,     \
+                           nctt, a0, a1, a2, ctr, nc, tgt, nt, src, ns, l1s, l1l, ss, sc, q, \
+                           dip, pot, grad);                                                 \
+        break;
+    switch (order) {
+        TSQ_CASE(0) TSQ_CASE(1) TSQ_CASE(2) TSQ_CASE(3) TSQ_CASE(4) TSQ_CASE(5) TSQ_CASE(6)
+        TSQ_CASE(7) TSQ_CASE(8) TSQ_CASE(9) TSQ_CASE(10) TSQ_CASE(11) TSQ_CASE(12)
+    default:
+        set_error("TSQBX order must be in [0, 12]");
+        return QBX_ERR_INVALID;
+    }
+#undef TSQ_CASE
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
+}  // namespace qbx
+
+using namespace qbx;
+
+extern "C" {
+
+int qbx_eval_target_specific_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T,
+                                        const qbx_trav *R, const qbx_geo *G, int kernel,
+                                        int op, double k_re, double k_im, int qbx_order,
+                                        const double *strengths_tree, const double *dipvec_tree,
+                                        double *pot, double *grad)
+{
+    QBX_CHECK(ctx && T && R && G && strengths_tree, "TSQBX: null argument");
+    QBX_CHECK(T->dim == 3, "TSQBX is defined for 3D trees (got dim %d)", T->dim);
+    QBX_CHECK(kernel == QBX_LAPLACE3D || kernel == QBX_HELMHOLTZ3D, "TSQBX: bad kernel %d",
+              kernel);
+    QBX_CHECK(op >= QBX_TSQBX_S && op <= QBX_TSQBX_SP, "TSQBX: bad op %d", op);
+    QBX_CHECK(op != QBX_TSQBX_SP || grad, "TSQBX gradient needs a grad array");
+    QBX_CHECK(op == QBX_TSQBX_SP || pot, "TSQBX potential needs a pot array");
+    QBX_CHECK(op != QBX_TSQBX_D || dipvec_tree, "TSQBX double layer needs dipole vectors");
+    QBX_CHECK(qbx_order >= 0 && qbx_order <= 12, "TSQBX order must be in [0, 12]");
+    if (G->nctt == 0) return QBX_OK;
+    hipStream_t st = S(stream);
+    const dim3 grid(cdiv((long)G->nctt * TSQ_LANES, 256));
+    const int hk = kernel == QBX_HELMHOLTZ3D;
+#define TSQ_ARGS                                                                          \
+    qbx_order, grid, st, k_re, k_im, G->nctt, G->ctt_lists.p, G->tree_ttc.p,             \
+        G->center_to_target_box.p, G->centers.p, G->ncenters, T->targets.p, T->ntargets, \
+        T->sources.p, T->nsources, R->l1_starts.p, R->l1_lists.p, T->box_source_starts.p, \
+        T->box_source_counts_nonchild.p, strengths_tree, dipvec_tree, pot, grad
+    if (!hk && op == QBX_TSQBX_S) return launch_p<0, 0>(TSQ_ARGS);
+    if (!hk && op == QBX_TSQBX_D) return launch_p<0, 1>(TSQ_ARGS);
+    if (!hk && op == QBX_TSQBX_SP) return launch_p<0, 2>(TSQ_ARGS);
+    if (hk && op == QBX_TSQBX_S) return launch_p<1, 0>(TSQ_ARGS);
+    if (hk && op == QBX_TSQBX_D) return launch_p<1, 1>(TSQ_ARGS);
+    return launch_p<1, 2>(TSQ_ARGS);
+#undef TSQ_ARGS
+}
+
+}  // extern "C"

@@ -94,6 +94,15 @@ class CurveDiscretization:
     upsample: np.ndarray = field(repr=False)       # (fine_order, order) per panel
     t_edges: np.ndarray = field(default=None, repr=False)  # (nelements+1,)
     curve: object = field(default=None, repr=False)
+    ambient_dim: int = 2
+
+    @property
+    def nodes_per_element(self) -> int:
+        return self.order
+
+    @property
+    def fine_nodes_per_element(self) -> int:
+        return self.fine_order
 
     @property
     def ndofs(self) -> int:

@@ -19,29 +19,68 @@ from .geometry import _p, _stream
 class QBXExpansionWrangler:
     """Stages of the QBX FMM on one tree (pytential/qbx/fmm.py:118-396)."""
 
-    def __init__(self, geo_data, kernel, k, source_kind, qbx_order, level_orders):
+    def __init__(self, geo_data, kernel, k, source_kind, qbx_order, level_orders,
+                 output="pot"):
         self.geo_data = geo_data
         self.tree = geo_data.tree()
         self.traversal = geo_data.traversal()
         self.geo = geo_data._geo
         self.ctx = geo_data.ctx
         self.kernel = kernel
+        self.dim = self.tree.dimensions
+        self.k = complex(k)
+        if output not in ("pot", "grad"):
+            raise ValueError(f"unknown output {output!r}")
+        if output == "grad" and source_kind == "D":
+            # impl.pyx:637-638
+            raise ValueError("Does not support computing gradient of dipole sources")
+        self.output = output
         lo = np.ascontiguousarray(level_orders, dtype=np.int32)
         if len(lo) < self.tree.nlevels:
             raise ValueError(f"need {self.tree.nlevels} level orders, got {len(lo)}")
         self._lo = lo
-        self.kd = KernelDesc(_lib.LAPLACE2D if kernel == "laplace" else _lib.HELMHOLTZ2D,
-                             float(k), _lib.SOURCE_S if source_kind == "S" else _lib.SOURCE_D,
+        code = {(2, "laplace"): _lib.LAPLACE2D, (2, "helmholtz"): _lib.HELMHOLTZ2D,
+                (3, "laplace"): _lib.LAPLACE3D, (3, "helmholtz"): _lib.HELMHOLTZ3D}
+        self.kernel_code = code[(self.dim, kernel)]
+        self.kd = KernelDesc(self.kernel_code, self.k.real,
+                             _lib.SOURCE_S if source_kind == "S" else _lib.SOURCE_D,
                              qbx_order, len(lo), lo.ctypes.data_as(ct.POINTER(ct.c_int)))
         self.source_kind = source_kind
         self.qbx_order = qbx_order
-        self.ncoeff_fmm = _lib.lib().qbx_ncoeff_fmm(ct.byref(self.kd))
-        self.ncoeff_qbx = _lib.lib().qbx_ncoeff_qbx(ct.byref(self.kd))
+        self.expansion_size = None
+        if self.dim == 2:
+            self.expansion_size = _lib.lib().qbx_expansion_size(ct.byref(self.kd),
+                                                                self.tree.h)
+            if self.expansion_size < 0:
+                raise _lib.QBXError(1, _lib.lib().qbx_last_error().decode())
+            offs = (ct.c_long * (self.tree.nlevels + 1))()
+            call("qbx_expansion_level_offsets", ct.byref(self.kd), self.tree.h, offs)
+            self.level_offsets = np.array(offs[:], dtype=np.int64)
+            self.ncoeff_qbx = _lib.lib().qbx_ncoeff_qbx(ct.byref(self.kd))
 
     # ---- data vector utilities
     def multipole_expansion_zeros(self):
-        return torch.zeros((self.tree.nboxes, self.ncoeff_fmm), dtype=torch.complex128,
+        """Level-compact box expansions (see qbx_expansion_size)."""
+        return torch.zeros(max(self.expansion_size, 1), dtype=torch.complex128,
                            device="cuda")
+
+    def expansions_to_dense(self, e):
+        """[nboxes][ncoeff(pmax)] view of level-compact expansions (Helmholtz
+        coefficient n at column n + pmax), for inspection and tests."""
+        e = e.detach().cpu().numpy()
+        lo = self._lo[:self.tree.nlevels]
+        pmax = int(max(lo))
+        helm = self.kernel != "laplace"
+        ncf = 2 * pmax + 1 if helm else pmax + 1
+        ls = self.tree.level_start_box_nrs.cpu().numpy()
+        out = np.zeros((self.tree.nboxes, ncf), np.complex128)
+        for lev in range(self.tree.nlevels):
+            p = int(lo[lev])
+            nc = 2 * p + 1 if helm else p + 1
+            blk = e[self.level_offsets[lev]:self.level_offsets[lev + 1]].reshape(-1, nc)
+            c0 = pmax - p if helm else 0
+            out[ls[lev]:ls[lev + 1], c0:c0 + nc] = blk
+        return out
 
     local_expansion_zeros = multipole_expansion_zeros
 
@@ -146,9 +185,29 @@ class QBXExpansionWrangler:
                 _p(qexp), _p(pot))
         return pot
 
-    def eval_target_specific_qbx_locals(self, strengths):
-        # TSQBX is 3D-only in the reference (qbx/fmmlib.py:141-148); zero here
-        return self.full_output_zeros()
+    def eval_target_specific_qbx_locals(self, strengths, normals=None):
+        """TSQBX (qbx/fmmlib.py:587-639, impl.pyx:566-785): potential (or its
+        gradient, ``output="grad"``) at every QBX target from the list-1
+        sources of its center's box, by the target-specific expansion.  3D
+        only; zeros otherwise, like the reference (qbx/fmmlib.py:141-148).
+        Returns [ntargets] complex, or [3, ntargets] for the gradient."""
+        if self.dim != 3:
+            return self.full_output_zeros()
+        nt = self.tree.ntargets
+        grad = pot = None
+        if self.output == "grad":
+            op = _lib.TSQBX_SP
+            grad = torch.zeros((3, nt), dtype=torch.complex128, device="cuda")
+        else:
+            op = _lib.TSQBX_D if self.source_kind == "D" else _lib.TSQBX_S
+            pot = self.full_output_zeros()
+        if op == _lib.TSQBX_D and normals is None:
+            raise ValueError("Missing dipvec")
+        kernel = _lib.LAPLACE3D if self.k == 0 else _lib.HELMHOLTZ3D
+        self._a("qbx_eval_target_specific_qbx_locals", self.tree.h, self.traversal.h,
+                self.geo.h, kernel, op, self.k.real, self.k.imag, self.qbx_order,
+                _p(strengths), _p(normals), _p(pot), _p(grad))
+        return grad if op == _lib.TSQBX_SP else pot
 
     def finalize_potentials(self, pot_tree):
         nuser = self.tree.ntargets - self.geo_data.ncenters

@@ -196,7 +196,7 @@ def associate_targets_to_qbx_centers(discr_dev, targets, flags,
 
 class QBXFMMGeometryData:
     """Mirror of pytential.qbx.geometry.QBXFMMGeometryData for one source
-    curve and a list of ``(targets, qbx_side)`` pairs; everything on the GPU."""
+    curve or surface and a list of ``(targets, qbx_side)`` pairs; everything on the GPU."""
 
     def __init__(self, lpot_source, target_discrs_and_qbx_sides):
         self.lpot_source = lpot_source
@@ -226,8 +226,8 @@ class QBXFMMGeometryData:
             s = torch.as_tensor(side, dtype=torch.int32, device="cuda")
             sides.append(s.expand(t.shape[1]) if s.dim() == 0 else s)
         starts.append(n)
-        user = torch.cat(pts, dim=1) if pts else torch.zeros((2, 0), dtype=torch.float64,
-                                                               device="cuda")
+        user = torch.cat(pts, dim=1) if pts else torch.zeros(
+            (self.lpot_source.ambient_dim, 0), dtype=torch.float64, device="cuda")
         side = torch.cat(sides) if sides else torch.zeros(0, dtype=torch.int32, device="cuda")
         return {"user_targets": user.contiguous(), "sides": side.contiguous(),
                 "target_discr_starts": starts}

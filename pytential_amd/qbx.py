@@ -40,7 +40,8 @@ def fmm_level_to_order_for_tolerance(kernel, k, tol, root_extent, nlevels,
 
 
 class QBXLayerPotentialSource:
-    """QBX layer potential source for a 2D panel discretization."""
+    """QBX layer potential source for a panel discretization of a curve (2D)
+    or a triangulated surface (3D)."""
 
     def __init__(self, density_discr: CurveDiscretization, fine_order=None, qbx_order=None,
                  fmm_order=None, fmm_level_to_order=None,
@@ -70,15 +71,17 @@ class QBXLayerPotentialSource:
         self.target_association_tolerance = (
             1.0e3 * float(np.finfo(np.float64).eps)
             if target_association_tolerance is None else target_association_tolerance)
-        self._max_leaf_refine_weight = 64 if _max_leaf_refine_weight is None \
-            else _max_leaf_refine_weight
+        self.ambient_dim = getattr(density_discr, "ambient_dim", 2)
+        # qbx/__init__.py:290-304: 64 in 2D, 512 in 3D
+        self._max_leaf_refine_weight = (
+            (64 if self.ambient_dim == 2 else 512) if _max_leaf_refine_weight is None
+            else _max_leaf_refine_weight)
         self._expansion_stick_out_factor = _expansion_stick_out_factor
         self._well_sep_is_n_away = _well_sep_is_n_away
         self._from_sep_smaller_min_nsources_cumul = (
             15 if _from_sep_smaller_min_nsources_cumul is None
             else _from_sep_smaller_min_nsources_cumul)
         self._box_extent_norm = "l2" if _box_extent_norm is None else _box_extent_norm
-        self.ambient_dim = 2
         self.ctx = Context.get(device)
         d = density_discr
         self.dev = {
@@ -112,7 +115,8 @@ class QBXLayerPotentialSource:
         dens = as_dev(density, torch.complex128) if torch.is_complex(
             torch.as_tensor(density)) else as_dev(density).to(torch.complex128)
         out = torch.empty(d.nsources, dtype=torch.complex128, device="cuda")
-        call("qbx_upsample", self.ctx.h, _stream(), d.nelements, d.order, d.fine_order,
+        call("qbx_upsample", self.ctx.h, _stream(), d.nelements, d.nodes_per_element,
+             d.fine_nodes_per_element,
              _p(self.dev["upsample"]), _p(dens.contiguous()), _p(out))
         return out
 

@@ -43,14 +43,15 @@ def test_library_reports_errors_without_gpu_work():
     from pytential_amd import _lib
     lib = _lib.lib()
     kd = _lib.KernelDesc(7, 0.0, 0, 4, 1, None)
-    assert lib.qbx_ncoeff_fmm(ct.byref(kd)) == -1
+    assert lib.qbx_ncoeff_qbx(ct.byref(kd)) == -1
     assert b"unsupported kernel" in lib.qbx_last_error()
     lo = (ct.c_int * 2)(10, 12)
     kd = _lib.KernelDesc(_lib.HELMHOLTZ2D, 20.0, _lib.SOURCE_D, 5, 2, lo)
-    assert lib.qbx_ncoeff_fmm(ct.byref(kd)) == 25
     assert lib.qbx_ncoeff_qbx(ct.byref(kd)) == 11
     kd = _lib.KernelDesc(_lib.LAPLACE2D, 0.0, _lib.SOURCE_S, 4, 2, lo)
-    assert lib.qbx_ncoeff_fmm(ct.byref(kd)) == 13
+    assert lib.qbx_ncoeff_qbx(ct.byref(kd)) == 5
+    # a null tree is rejected without touching the device
+    assert lib.qbx_expansion_size(ct.byref(kd), None) == -1
 
 
 def test_no_cpu_fallback_when_library_missing(monkeypatch):
