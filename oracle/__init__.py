@@ -110,6 +110,7 @@ def _declare(L):
                           P, ct.c_double, P, ct.c_double, P, P]
     L.orc_p2p.argtypes = [ct.POINTER(KParams), ct.c_int, P, P, P, ct.c_int, P, P]
     L.orc_bessel_jy.argtypes = [ct.c_int, ct.c_double, P, P]
+    L.orc_sh3d.argtypes = [ct.c_int, P, P, P]
     L.orc_tsqbx.argtypes = [ct.c_int, ct.c_int, ct.c_double, ct.c_double, ct.c_int,
                             ct.c_int, P, ct.c_int, P, ct.c_int, P,
                             ct.c_int, P, P, P, P, P, P, P, P, P, P, P, P]
@@ -255,10 +256,13 @@ def refine_flags_for_failed(src1, tunnel, el_starts, targets, status):
     return out
 
 
+KERNEL_CODES = {("laplace", 2): 0, ("helmholtz", 2): 1, ("laplace", 3): 2}
+
+
 def kparams(kernel="laplace", k=0.0, source_kind="S", qbx_order=4,
-            level_orders=None, nthreads=0):
+            level_orders=None, nthreads=0, dim=2):
     lo = _i32(level_orders if level_orders is not None else [0])
-    kp = KParams(0 if kernel == "laplace" else 1, float(k),
+    kp = KParams(KERNEL_CODES[(kernel, dim)], float(k),
                  0 if source_kind == "S" else 1, qbx_order,
                  lo.ctypes.data_as(IP), nthreads)
     kp._lo = lo
@@ -294,7 +298,9 @@ def direct_qbx(kp, sources, normals, strengths, centers, radii, targets, ttc):
     return out
 
 
-def ncoeff(kernel, p):
+def ncoeff(kernel, p, dim=2):
+    if dim == 3:
+        return (p + 1) ** 2
     return p + 1 if kernel == "laplace" else 2 * p + 1
 
 
@@ -303,7 +309,7 @@ def p2e(kp, kernel, local, order, center, rscale, sources, normals, strengths):
     src, nrm = _f64(sources), _f64(normals)
     st = _c128(strengths)
     c = _f64(center)
-    out = np.zeros(ncoeff(kernel, order), np.complex128)
+    out = np.zeros(ncoeff(kernel, order, len(c)), np.complex128)
     L.orc_p2e(ct.byref(kp), int(local), order, _ptr(c), rscale, src.shape[1],
               _ptr(src), _ptr(nrm), _ptr(st), _ptr(out))
     return out
@@ -326,7 +332,7 @@ E2E_KIND = {"m2m": 0, "m2l": 1, "l2l": 2}
 def e2e(kp, kernel, kind, p1, p2, c1, r1, c2, r2, coeffs):
     L = lib()
     cf = _c128(coeffs)
-    out = np.zeros(ncoeff(kernel, p2), np.complex128)
+    out = np.zeros(ncoeff(kernel, p2, len(np.ravel(c1))), np.complex128)
     L.orc_e2e(ct.byref(kp), E2E_KIND[kind], p1, p2, _ptr(_f64(c1)), r1,
               _ptr(_f64(c2)), r2, _ptr(cf), _ptr(out))
     return out
@@ -383,14 +389,15 @@ class GeometryData:
 
     def fmm(self, kernel, k, source_kind, qbx_order, level_orders, strengths,
             nthreads=0, return_stages=False):
-        kp = kparams(kernel, k, source_kind, qbx_order, level_orders, nthreads)
+        dim = self.discr.nodes.shape[0]
+        kp = kparams(kernel, k, source_kind, qbx_order, level_orders, nthreads, dim)
         if not return_stages:
             out, times = qbx_fmm(self.tree, self.trav, kp, self.discr.quad_nodes,
                                  self.discr.quad_normals, strengths, self.centers,
                                  self.radii, self.all_targets, self.user_target_to_center)
             return out[self.ncenters:], times
         pmax = max(level_orders[:self.tree.nlevels])
-        ncf, ncq = ncoeff(kernel, pmax), ncoeff(kernel, qbx_order)
+        ncf, ncq = ncoeff(kernel, pmax, dim), ncoeff(kernel, qbx_order, dim)
         mp = np.zeros((self.tree.nboxes, ncf), np.complex128)
         lc = np.zeros((self.tree.nboxes, ncf), np.complex128)
         qx = np.zeros((self.ncenters, ncq), np.complex128)
@@ -409,7 +416,8 @@ class GeometryData:
 
     def direct(self, kernel, k, source_kind, qbx_order, strengths, target_idx=None,
                nthreads=0):
-        kp = kparams(kernel, k, source_kind, qbx_order, [0], nthreads)
+        kp = kparams(kernel, k, source_kind, qbx_order, [0], nthreads,
+                     self.discr.nodes.shape[0])
         tg = self.user_targets
         ttc = self.target_to_center
         if target_idx is not None:
@@ -429,6 +437,16 @@ def assoc_status_and_flags(discr, targets, sides,
                                     discr.element_to_source_starts(), targets, status) \
         if np.any(status == 1) else np.zeros(discr.nelements, np.int32)
     return status, flags
+
+
+def solid_harmonics(p, x):
+    """Regular Y_n^m and irregular T_n^m at x (3,), coefficient (n, m) at
+    n*n + n + m (fmm3d_oracle.c)."""
+    x = _f64(x)
+    Y = np.zeros((p + 1) ** 2, np.complex128)
+    T = np.zeros((p + 1) ** 2, np.complex128)
+    lib().orc_sh3d(p, _ptr(x), _ptr(Y), _ptr(T))
+    return Y, T
 
 
 def sph_jn(nmax, z):

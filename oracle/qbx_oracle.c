@@ -1037,6 +1037,7 @@ static cplx p2p_one(const orc_kparams *kp, cplx s, cplx q, cplx nu, cplx t)
 int orc_p2e(const orc_kparams *kp, int local, int order, const double *c, double rscale,
             int ns, const double *src, const double *nrm, const double *str, double *coeffs)
 {
+    if (kp->kernel == 2) return orc_p2e3(kp, local, order, c, ns, src, nrm, str, coeffs);
     int nc = ncoeff_of(kp->kernel, order);
     cplx *cf = (cplx *)coeffs;
     for (int i = 0; i < nc; ++i) cf[i] = 0;
@@ -1052,6 +1053,7 @@ int orc_p2e(const orc_kparams *kp, int local, int order, const double *c, double
 int orc_e2p(const orc_kparams *kp, int local, int order, const double *c, double rscale,
             const double *coeffs, int nt, const double *tgt, double *pot)
 {
+    if (kp->kernel == 2) return orc_e2p3(local, order, c, coeffs, nt, tgt, pot);
     cplx cc = c[0] + JI * c[1];
     for (int t = 0; t < nt; ++t) {
         cplx v = e2p_one(kp, local, order, cc, rscale, (const cplx *)coeffs, cpt(tgt, nt, t));
@@ -1064,6 +1066,7 @@ int orc_e2e(const orc_kparams *kp, int kind, int p1, int p2,
             const double *c1, double R1, const double *c2, double R2,
             const double *src, double *tgt)
 {
+    if (kp->kernel == 2) return orc_e2e3(kind, p1, p2, c1, c2, src, tgt);
     int nc = ncoeff_of(kp->kernel, p2);
     cplx *b = (cplx *)tgt;
     for (int i = 0; i < nc; ++i) b[i] = 0;
@@ -1075,6 +1078,7 @@ int orc_e2e(const orc_kparams *kp, int kind, int p1, int p2,
 int orc_p2p(const orc_kparams *kp, int ns, const double *src, const double *nrm,
             const double *str, int nt, const double *tgt, double *pot)
 {
+    if (kp->kernel == 2) return orc_p2p3(kp, ns, src, nrm, str, nt, tgt, pot);
 #pragma omp parallel for schedule(static)
     for (int t = 0; t < nt; ++t) {
         cplx acc = 0, tt = cpt(tgt, nt, t);
@@ -1128,7 +1132,11 @@ int orc_qbx_fmm_ex(const orc_tree *t, const orc_trav *tr, const orc_kparams *kp,
                    const double *targets, const int *uttc, double *result, double *stage_times,
                    double *mpole_out, double *local_out, double *qexp_out)
 {
-    if (t->dim != 2) return -1;
+    if (t->dim == 3)
+        return orc_qbx_fmm3d_ex(t, tr, kp, sources, normals, strengths, ncenters, centers,
+                                radii, targets, uttc, result, stage_times, mpole_out,
+                                local_out, qexp_out);
+    if (t->dim != 2 || kp->kernel > 1) return -1;
     if (kp->nthreads > 0) omp_set_num_threads(kp->nthreads);
     binom_init();
     const int ns = t->nsources, nT = t->ntargets, nb = t->nboxes, nl = t->nlevels;
@@ -1385,7 +1393,12 @@ int orc_direct_qbx(int dim, const orc_kparams *kp,
                    int ncenters, const double *centers, const double *radii,
                    int nt, const double *tgt, const int *ttc, double *result)
 {
-    if (dim != 2) return -1;
+    if (dim == 3) {
+        if (kp->nthreads > 0) omp_set_num_threads(kp->nthreads);
+        return orc_direct_qbx3d(kp, ns, src, nrm, str, ncenters, centers, nt, tgt, ttc,
+                                result);
+    }
+    if (dim != 2 || kp->kernel > 1) return -1;
     if (kp->nthreads > 0) omp_set_num_threads(kp->nthreads);
     const int pq = kp->qbx_order;
     const int ncq = ncoeff_of(kp->kernel, pq);
