@@ -1181,6 +1181,7 @@ extern "C" {
 
 long qbx_expansion_size(const qbx_kernel_desc *d, const qbx_tree *T)
 {
+    if (d && d->kernel == QBX_LAPLACE3D) return f3_expansion_size(d, T);
     KD kd;
     if (!T || make_kd(d, T, &kd) != QBX_OK) return -1;
     return kd.lcoef[kd.nlevels];
@@ -1189,6 +1190,7 @@ long qbx_expansion_size(const qbx_kernel_desc *d, const qbx_tree *T)
 int qbx_expansion_level_offsets(const qbx_kernel_desc *d, const qbx_tree *T, long *offsets)
 {
     QBX_CHECK(T && offsets, "qbx_expansion_level_offsets: null argument");
+    if (d && d->kernel == QBX_LAPLACE3D) return f3_expansion_level_offsets(d, T, offsets);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd));
     for (int l = 0; l <= kd.nlevels; ++l) offsets[l] = kd.lcoef[l];
@@ -1197,6 +1199,7 @@ int qbx_expansion_level_offsets(const qbx_kernel_desc *d, const qbx_tree *T, lon
 
 int qbx_ncoeff_qbx(const qbx_kernel_desc *d)
 {
+    if (d && d->kernel == QBX_LAPLACE3D) return f3_ncoeff_qbx(d);
     KD kd;
     if (make_kd(d, nullptr, &kd) != QBX_OK) return -1;
     return kd.ncq;
@@ -1207,6 +1210,7 @@ int qbx_form_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx
                         double *mpole)
 {
     QBX_CHECK(ctx && T && R && str && mpole, "qbx_form_multipoles: null argument");
+    if (d && d->kernel == QBX_LAPLACE3D) return f3_form_multipoles(ctx, stream, T, R, d, str, nrm, mpole);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd));
     QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
@@ -1225,6 +1229,7 @@ int qbx_coarsen_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const 
                            const qbx_kernel_desc *d, double *mpole)
 {
     QBX_CHECK(ctx && T && R && mpole, "qbx_coarsen_multipoles: null argument");
+    if (d && d->kernel == QBX_LAPLACE3D) return f3_coarsen_multipoles(ctx, stream, T, R, d, mpole);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd, R->nsep));
     const double *B;
@@ -1249,6 +1254,8 @@ static int eval_targets(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx
                         int accumulate)
 {
     QBX_CHECK(ctx && T && R && G && pot, "stage: null argument");
+    if (d && d->kernel == QBX_LAPLACE3D)
+        return f3_eval_targets(ctx, stream, T, R, G, d, mode, str, nrm, expn, pot, accumulate);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd));
     hipStream_t st = S(stream);
@@ -1302,6 +1309,7 @@ int qbx_multipole_to_local(qbx_ctx *ctx, void *stream, const qbx_tree *T, const 
                            const qbx_kernel_desc *d, const double *mpole, double *local)
 {
     QBX_CHECK(ctx && T && R && mpole && local, "qbx_multipole_to_local: null argument");
+    if (d && d->kernel == QBX_LAPLACE3D) return f3_multipole_to_local(ctx, stream, T, R, d, mpole, local);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd, R->nsep));
     const double *B;
@@ -1323,6 +1331,7 @@ int qbx_form_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_tra
                     double *local, int accumulate)
 {
     QBX_CHECK(ctx && T && R && str && local, "qbx_form_locals: null argument");
+    if (d && d->kernel == QBX_LAPLACE3D) return f3_form_locals(ctx, stream, T, R, d, str, nrm, local, accumulate);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd));
     QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
@@ -1342,6 +1351,7 @@ int qbx_refine_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_t
                       const qbx_kernel_desc *d, double *local)
 {
     QBX_CHECK(ctx && T && R && local, "qbx_refine_locals: null argument");
+    if (d && d->kernel == QBX_LAPLACE3D) return f3_refine_locals(ctx, stream, T, R, d, local);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd, R->nsep));
     const double *B;
@@ -1364,6 +1374,8 @@ int qbx_form_global_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, co
                                const double *nrm, double *qexp, int accumulate)
 {
     QBX_CHECK(ctx && T && R && G && str && qexp, "qbx_form_global_qbx_locals: null argument");
+    if (d && d->kernel == QBX_LAPLACE3D)
+        return f3_form_global_qbx_locals(ctx, stream, T, R, G, d, str, nrm, qexp, accumulate);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd));
     QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
@@ -1386,6 +1398,9 @@ int qbx_translate_box_multipoles_to_qbx_local(qbx_ctx *ctx, void *stream, const 
                                               double *qexp, int accumulate)
 {
     QBX_CHECK(ctx && T && R && G && mpole && qexp, "M2QBXL: null argument");
+    if (d && d->kernel == QBX_LAPLACE3D)
+        return f3_translate_box_multipoles_to_qbx_local(ctx, stream, T, R, G, d, mpole, qexp,
+                                                        accumulate);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd));
     const double *B;
@@ -1409,6 +1424,9 @@ int qbx_translate_box_local_to_qbx_local(qbx_ctx *ctx, void *stream, const qbx_t
                                          double *qexp, int accumulate)
 {
     QBX_CHECK(ctx && T && R && G && local && qexp, "L2QBXL: null argument");
+    if (d && d->kernel == QBX_LAPLACE3D)
+        return f3_translate_box_local_to_qbx_local(ctx, stream, T, R, G, d, local, qexp,
+                                                   accumulate);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd));
     const double *B;
@@ -1429,6 +1447,7 @@ int qbx_eval_qbx_expansions(qbx_ctx *ctx, void *stream, const qbx_tree *T, const
                             const qbx_kernel_desc *d, const double *qexp, double *pot)
 {
     QBX_CHECK(ctx && T && G && qexp && pot, "qbx_eval_qbx_expansions: null argument");
+    if (d && d->kernel == QBX_LAPLACE3D) return f3_eval_qbx_expansions(ctx, stream, T, G, d, qexp, pot);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd));
     hipStream_t st = S(stream);
@@ -1457,10 +1476,11 @@ int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav 
                   const double *nrm, double *out_user)
 {
     QBX_CHECK(ctx && T && R && G && str && out_user, "qbx_drive_fmm: null argument");
-    KD kd;
-    QBX_TRY(make_kd(d, T, &kd));
-    const long nm = 2L * std::max(kd.lcoef[kd.nlevels], 1L);
-    const long nq = 2L * std::max(G->ncenters, 1) * kd.ncq;
+    const long nexp = qbx_expansion_size(d, T);
+    const int ncq = qbx_ncoeff_qbx(d);
+    if (nexp < 0 || ncq < 0) return QBX_ERR_INVALID;
+    const long nm = 2L * std::max(nexp, 1L);
+    const long nq = 2L * std::max(G->ncenters, 1) * ncq;
     const long np = 2L * std::max(T->ntargets, 1);
     if (ctx->ws_mpole.n < nm) QBX_TRY(ctx->ws_mpole.alloc(nm));
     if (ctx->ws_local.n < nm) QBX_TRY(ctx->ws_local.alloc(nm));

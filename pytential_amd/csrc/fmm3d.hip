@@ -1,0 +1,1205 @@
+// QBX-FMM wrangler stages for 3D Laplace (pytential/qbx/fmm.py drive_fmm and
+// pytential/qbx/interactions.py), FP64 / complex128.
+//
+// Expansions in solid harmonics (coefficient (n, m) at n*n + n + m):
+//   Y_n^m(x) = r^n P_n^m(cos th) e^{i m phi} / (n+m)!        regular
+//   T_n^m(x) = (n-m)! P_n^m(cos th) e^{i m phi} / r^{n+1}    irregular
+//   Y_n^{-m} = (-1)^m conj(Y_n^m), T_n^{-m} = (-1)^m conj(T_n^m)
+//   1/|x-y| = sum conj(Y_n^m(y)) T_n^m(x) for |y| < |x|
+//   multipole  M_n^m = sum q conj(Y_n^m(s-c)),  phi(x) = sum M T(x-c)
+//   local      L_n^m = sum q T_n^m(s-c),        phi(x) = sum L conj(Y(x-c))
+//   M2M  M'_n^m = sum_{k,l} conj(Y_k^l(c-c')) M_{n-k}^{m-l}
+//   M2L  L_k^l  = (-1)^k sum_{n,m} M_n^m T_{n+k}^{m+l}(c'-c)
+//   L2L  L'_a^b = sum_{j,i} L_{a+j}^{b+i} conj(Y_j^i(c'-c))
+// Double layer: q nu . grad_s through
+//   d_z Y_n^m = Y_{n-1}^m, (d_x +- i d_y) Y_n^m = +-Y_{n-1}^{m+-1},
+//   d_z T_n^m = -T_{n+1}^m, (d_x +- i d_y) T_n^m = +-T_{n+1}^{m+-1}.
+// Potentials carry G = 1/(4 pi r).
+//
+// Recurrences (no trigonometry, w = x + i y, r^2 = |x|^2):
+//   Y_m^m = -(w/2m) Y_{m-1}^{m-1}, (n-m)(n+m) Y_n^m = (2n-1) z Y_{n-1}^m - r^2 Y_{n-2}^m
+//   T_m^m = -(2m-1)(w/r^2) T_{m-1}^{m-1},
+//   T_n^m = ((2n-1) z T_{n-1}^m - (n+m-1)(n-m-1) T_{n-2}^m) / r^2
+//
+// Storage (HBM): box expansions level-compact, (p_l+1)^2 complex per box of
+// level l; QBX expansions [ncenters][(p_q+1)^2].
+//
+// Thread mappings:
+//  P2M, P2L        one thread per (box, coefficient)
+//  M2M, L2L        one thread per (box, coefficient), child offsets from a
+//                  per-tree table of Y at the 8 unit child offsets
+//  M2L             one 64-lane block per target box; each list-2 multipole is
+//                  staged (rescaled by h^-n) in LDS and contracted against
+//                  the irregular harmonics of its integer offset class, read
+//                  from a table T(delta), |delta_i| <= 2 nsep + 1
+//  P2P, M2P, L2P   one block per target box, lane groups per target
+//  P2QBXL          16 lanes per QBX center over its list-1 sources, the
+//                  order-p_q expansion in registers, shuffle reduction
+//  M2QBXL          16 lanes per QBX center: per list-3 box the irregular
+//                  harmonics of the offset are built in LDS, then each lane
+//                  contracts its output coefficients
+//  L2QBXL          one thread per QBX center, expansion in registers
+//  QBXL2P          one thread per (center, target) pair
+#include "qbx_internal.h"
+#include "qbx_math.h"
+
+#include <complex>
+#include <cstring>
+#include <vector>
+
+namespace qbx {
+namespace f3 {
+
+constexpr int MAXP = 20;       // box expansion orders
+constexpr int MAXPQ = 8;       // QBX expansion orders (templated)
+constexpr int FML = 21;        // levels of a 3D tree (19-bit coordinates)
+constexpr double INV4PI = 0.079577471545947667884;
+
+__host__ __device__ constexpr int IDX(int n, int m) { return n * n + n + m; }
+
+struct KD {
+    int dip, pq, nlevels, ncq, nsep, W, pmax, tabn;   // tabn = 2 pmax + 1
+    int order[FML];
+    int ncl[FML];
+    int lbox[FML + 1];
+    long lcoef[FML + 1];
+    double h[FML];
+    const double *tab_m2l;   // [(2W+1)^3][tabn^2] complex: T(delta), unit box size
+    const double *tab_ch;    // [8][(pmax+1)^2] complex: Y(bit - 1/2), unit child size
+};
+
+struct TD {
+    int nboxes;
+    const int *level, *parent, *child, *coords;
+    const double *centers;  // [3][nboxes]
+    const int *sstart, *scount, *scumul;
+};
+
+struct Flat {
+    int nl;
+    long tstart[FML + 1];
+    int istart[FML];
+    int nper[FML];
+};
+
+__device__ __forceinline__ bool flat_locate(const Flat &f, long gid, int &lv, int &i, int &j)
+{
+    if (gid >= f.tstart[f.nl]) return false;
+    int l = 0;
+    while (gid >= f.tstart[l + 1]) ++l;
+    const long r = gid - f.tstart[l];
+    lv = l;
+    i = f.istart[l] + (int)(r / f.nper[l]);
+    j = (int)(r % f.nper[l]);
+    return true;
+}
+
+struct v3 {
+    double x, y, z;
+};
+__device__ __forceinline__ v3 ld3(const double *p, long n, long i)
+{
+    return {p[i], p[n + i], p[2 * n + i]};
+}
+__device__ __forceinline__ v3 sub(v3 a, v3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ v3 boxc(const TD &t, int b) { return ld3(t.centers, t.nboxes, b); }
+
+__device__ __forceinline__ long ecoef_off(const KD &kd, int b, int lv)
+{
+    return 2 * (kd.lcoef[lv] + (long)(b - kd.lbox[lv]) * kd.ncl[lv]);
+}
+
+__device__ __forceinline__ void nm_of(int i, int &n, int &m)
+{
+    n = (int)sqrt((double)i);
+    while (n * n > i) --n;
+    while ((n + 1) * (n + 1) <= i) ++n;
+    m = i - n * n - n;
+}
+
+__device__ __forceinline__ cplx neg_m(cplx v, int am)  // (-1)^m conj(v)
+{
+    return (am & 1) ? cplx{-v.x, v.y} : cplx{v.x, -v.y};
+}
+
+// single Y_n^m(d), O(n)
+__device__ cplx ups_nm(int n, int m, v3 d)
+{
+    const int am = m < 0 ? -m : m;
+    if (am > n || n < 0) return {0, 0};
+    const double r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+    const cplx hw = {-0.5 * d.x, -0.5 * d.y};
+    cplx b = {1, 0}, a = {0, 0};
+    for (int i = 1; i <= am; ++i) b = (1.0 / i) * (b * hw);
+    for (int k = am + 1; k <= n; ++k) {
+        const cplx c = (1.0 / ((double)(k - am) * (k + am))) * (((2.0 * k - 1.0) * d.z) * b - r2 * a);
+        a = b;
+        b = c;
+    }
+    return m < 0 ? neg_m(b, am) : b;
+}
+
+// single T_n^m(d), O(n)
+__device__ cplx the_nm(int n, int m, v3 d)
+{
+    const int am = m < 0 ? -m : m;
+    if (am > n || n < 0) return {0, 0};
+    const double r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+    const double ir2 = 1.0 / r2;
+    const cplx w = {d.x * ir2, d.y * ir2};
+    cplx b = {sqrt(ir2), 0}, a = {0, 0};
+    for (int i = 1; i <= am; ++i) b = (-(2.0 * i - 1.0)) * (b * w);
+    for (int k = am + 1; k <= n; ++k) {
+        const cplx c = ir2 * (((2.0 * k - 1.0) * d.z) * b -
+                              ((double)(k + am - 1) * (k - am - 1)) * a);
+        a = b;
+        b = c;
+    }
+    return m < 0 ? neg_m(b, am) : b;
+}
+
+// full tables for n <= P in registers
+template <int P>
+__device__ __forceinline__ void ups_table(v3 d, cplx *Y)
+{
+    const double r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+    const cplx hw = {-0.5 * d.x, -0.5 * d.y};
+    cplx ymm = {1, 0};
+#pragma unroll
+    for (int m = 0; m <= P; ++m) {
+        if (m > 0) ymm = (1.0 / m) * (ymm * hw);
+        Y[IDX(m, m)] = ymm;
+        cplx a = {0, 0}, b = ymm;
+#pragma unroll
+        for (int n = m + 1; n <= P; ++n) {
+            const cplx c = (1.0 / ((double)(n - m) * (n + m))) *
+                           (((2.0 * n - 1.0) * d.z) * b - r2 * a);
+            Y[IDX(n, m)] = c;
+            a = b;
+            b = c;
+        }
+    }
+#pragma unroll
+    for (int n = 1; n <= P; ++n)
+#pragma unroll
+        for (int m = 1; m <= n; ++m) Y[IDX(n, -m)] = neg_m(Y[IDX(n, m)], m);
+}
+
+template <int P>
+__device__ __forceinline__ void the_table(v3 d, cplx *T)
+{
+    const double r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+    const double ir2 = 1.0 / r2;
+    const cplx w = {d.x * ir2, d.y * ir2};
+    cplx tmm = {sqrt(ir2), 0};
+#pragma unroll
+    for (int m = 0; m <= P; ++m) {
+        if (m > 0) tmm = (-(2.0 * m - 1.0)) * (tmm * w);
+        T[IDX(m, m)] = tmm;
+        cplx a = {0, 0}, b = tmm;
+#pragma unroll
+        for (int n = m + 1; n <= P; ++n) {
+            const cplx c = ir2 * (((2.0 * n - 1.0) * d.z) * b -
+                                  ((double)(n + m - 1) * (n - m - 1)) * a);
+            T[IDX(n, m)] = c;
+            a = b;
+            b = c;
+        }
+    }
+#pragma unroll
+    for (int n = 1; n <= P; ++n)
+#pragma unroll
+        for (int m = 1; m <= n; ++m) T[IDX(n, -m)] = neg_m(T[IDX(n, m)], m);
+}
+
+// sum_{n<=p,m} coef[n,m] T_n^m(d): streamed column by column, O(1) state
+__device__ cplx sum_irregular(const double *coef, int p, v3 d)
+{
+    const double r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+    const double ir2 = 1.0 / r2;
+    const cplx w = {d.x * ir2, d.y * ir2};
+    cplx tmm = {sqrt(ir2), 0};
+    cplx acc = {0, 0};
+    for (int m = 0; m <= p; ++m) {
+        if (m > 0) tmm = (-(2.0 * m - 1.0)) * (tmm * w);
+        cplx a = {0, 0}, b = tmm;
+        for (int n = m; n <= p; ++n) {
+            if (n > m) {
+                const cplx c = ir2 * (((2.0 * n - 1.0) * d.z) * b -
+                                      ((double)(n + m - 1) * (n - m - 1)) * a);
+                a = b;
+                b = c;
+            }
+            acc += ldc(coef + 2 * IDX(n, m)) * b;
+            if (m > 0) acc += ldc(coef + 2 * IDX(n, -m)) * neg_m(b, m);
+        }
+    }
+    return acc;
+}
+
+// sum_{n<=p,m} coef[n,m] conj(Y_n^m(d)); conj(Y_n^{-m}) = (-1)^m Y_n^m
+__device__ cplx sum_regular_conj(const double *coef, int p, v3 d)
+{
+    const double r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+    const cplx hw = {-0.5 * d.x, -0.5 * d.y};
+    cplx ymm = {1, 0};
+    cplx acc = {0, 0};
+    for (int m = 0; m <= p; ++m) {
+        if (m > 0) ymm = (1.0 / m) * (ymm * hw);
+        cplx a = {0, 0}, b = ymm;
+        for (int n = m; n <= p; ++n) {
+            if (n > m) {
+                const cplx c = (1.0 / ((double)(n - m) * (n + m))) *
+                               (((2.0 * n - 1.0) * d.z) * b - r2 * a);
+                a = b;
+                b = c;
+            }
+            acc += ldc(coef + 2 * IDX(n, m)) * conj(b);
+            if (m > 0) {
+                const cplx v = ldc(coef + 2 * IDX(n, -m)) * b;
+                acc += (m & 1) ? -v : v;
+            }
+        }
+    }
+    return acc;
+}
+
+__device__ __forceinline__ cplx p2p_pair(const KD &kd, v3 s, cplx q, v3 nu, v3 t)
+{
+    const v3 d = sub(t, s);
+    const double r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+    if (r2 == 0.0) return {0, 0};
+    const double ir = 1.0 / sqrt(r2);
+    if (!kd.dip) return (INV4PI * ir) * q;
+    return (INV4PI * ir * ir * ir * (nu.x * d.x + nu.y * d.y + nu.z * d.z)) * q;
+}
+
+// coefficient (n, m) of one source's multipole (local = 0) or local (1)
+__device__ __forceinline__ cplx p2e_coeff(const KD &kd, int local, int n, int m, v3 d, cplx q,
+                                          v3 nu)
+{
+    if (!kd.dip) return local ? q * the_nm(n, m, d) : q * conj(ups_nm(n, m, d));
+    const cplx nm = {nu.x, -nu.y}, np = {nu.x, nu.y};
+    cplx g;
+    if (local)
+        g = (-nu.z) * the_nm(n + 1, m, d) +
+            0.5 * (nm * the_nm(n + 1, m + 1, d) - np * the_nm(n + 1, m - 1, d));
+    else
+        g = nu.z * ups_nm(n - 1, m, d) +
+            0.5 * (nm * ups_nm(n - 1, m + 1, d) - np * ups_nm(n - 1, m - 1, d));
+    return local ? q * g : q * conj(g);
+}
+
+// P2M over source boxes: thread per (box, coefficient)
+__global__ void k3_p2m(KD kd, TD t, Flat fl, const int *boxes, const double *src, int ns,
+                       const double *str, const double *nrm, double *mpole)
+{
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int lv, i, j;
+    if (!flat_locate(fl, gid, lv, i, j)) return;
+    const int b = boxes[i];
+    int n, m;
+    nm_of(j, n, m);
+    const v3 c = boxc(t, b);
+    const int s0 = t.sstart[b], s1 = s0 + t.scount[b];
+    cplx acc = {0, 0};
+    for (int s = s0; s < s1; ++s) {
+        const v3 nu = kd.dip ? ld3(nrm, ns, s) : v3{0, 0, 0};
+        acc += p2e_coeff(kd, 0, n, m, sub(ld3(src, ns, s), c), ldc(str + 2 * s), nu);
+    }
+    stc(mpole + ecoef_off(kd, b, lv) + 2 * j, acc);
+}
+
+// P2L over list 4: thread per (ttp box, coefficient), accumulating
+__global__ void k3_p2l(KD kd, TD t, Flat fl, const int *ttp, const int *starts, const int *lists,
+                       const double *src, int ns, const double *str, const double *nrm,
+                       double *local)
+{
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int lv, i, j;
+    if (!flat_locate(fl, gid, lv, i, j)) return;
+    const int e0 = starts[i], e1 = starts[i + 1];
+    if (e0 == e1) return;
+    const int b = ttp[i];
+    int n, m;
+    nm_of(j, n, m);
+    const v3 c = boxc(t, b);
+    cplx acc = {0, 0};
+    for (int e = e0; e < e1; ++e) {
+        const int S = lists[e];
+        const int s0 = t.sstart[S], s1 = s0 + t.scount[S];
+        for (int s = s0; s < s1; ++s) {
+            const v3 nu = kd.dip ? ld3(nrm, ns, s) : v3{0, 0, 0};
+            acc += p2e_coeff(kd, 1, n, m, sub(ld3(src, ns, s), c), ldc(str + 2 * s), nu);
+        }
+    }
+    double *o = local + ecoef_off(kd, b, lv) + 2 * j;
+    stc(o, ldc(o) + acc);
+}
+
+// child offset index bits: bit d set = upper half in dimension d
+__device__ __forceinline__ const double *child_tab(const KD &kd, int ci)
+{
+    return kd.tab_ch + 2L * ci * (kd.pmax + 1) * (kd.pmax + 1);
+}
+
+// M2M into parents of level lvp: thread per (parent, coefficient)
+__global__ void k3_m2m(KD kd, TD t, int lvp, int i0, int i1, const int *spb, double *mpole)
+{
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ncp = kd.ncl[lvp];
+    const long nthr = (long)(i1 - i0) * ncp;
+    if (gid >= nthr) return;
+    const int P = spb[i0 + (int)(gid / ncp)];
+    const int j = (int)(gid % ncp);
+    int n, m;
+    nm_of(j, n, m);
+    const int lvc = lvp + 1;
+    const int pc = kd.order[lvc];
+    const double hc = kd.h[lvc];
+    cplx acc = {0, 0};
+    for (int ci = 0; ci < 8; ++ci) {
+        const int C = t.child[(size_t)ci * t.nboxes + P];
+        if (C < 0 || t.scumul[C] == 0) continue;
+        const double *a = mpole + ecoef_off(kd, C, lvc);
+        const double *U = child_tab(kd, ci);
+        // conj(Y_k^l(c_child - c_parent)) = conj(U_k^l) hc^k
+        double hk = 1.0;
+        for (int k = 0; k <= n; ++k) {
+            const int nn = n - k;
+            if (nn <= pc) {
+                for (int l = -k; l <= k; ++l) {
+                    const int mm = m - l;
+                    if (mm < -nn || mm > nn) continue;
+                    acc += (hk * conj(ldc(U + 2 * IDX(k, l)))) * ldc(a + 2 * IDX(nn, mm));
+                }
+            }
+            hk *= hc;
+        }
+    }
+    stc(mpole + ecoef_off(kd, P, lvp) + 2 * j, acc);
+}
+
+// L2L into ttp boxes of level lv: thread per (box, coefficient), accumulating
+__global__ void k3_l2l(KD kd, TD t, int lv, int i0, int i1, const int *ttp, double *local)
+{
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nc = kd.ncl[lv];
+    const long nthr = (long)(i1 - i0) * nc;
+    if (gid >= nthr) return;
+    const int Bx = ttp[i0 + (int)(gid / nc)];
+    const int j = (int)(gid % nc);
+    int a, b;
+    nm_of(j, a, b);
+    const int Pp = t.parent[Bx];
+    const int lvp = lv - 1;
+    const int pp = kd.order[lvp];
+    const double hc = kd.h[lv];
+    // child index of Bx within its parent from coordinates
+    int ci = 0;
+    ci |= (t.coords[Bx] & 1);
+    ci |= (t.coords[(size_t)t.nboxes + Bx] & 1) << 1;
+    ci |= (t.coords[2 * (size_t)t.nboxes + Bx] & 1) << 2;
+    const double *U = child_tab(kd, ci);
+    const double *L = local + ecoef_off(kd, Pp, lvp);
+    cplx acc = {0, 0};
+    double hj = 1.0;
+    for (int jj = 0; a + jj <= pp; ++jj) {
+        for (int i = -jj; i <= jj; ++i) {
+            const int mm = b + i;
+            if (mm < -(a + jj) || mm > a + jj) continue;
+            acc += (hj * conj(ldc(U + 2 * IDX(jj, i)))) * ldc(L + 2 * IDX(a + jj, mm));
+        }
+        hj *= hc;
+    }
+    double *o = local + ecoef_off(kd, Bx, lv) + 2 * j;
+    stc(o, ldc(o) + acc);
+}
+
+constexpr int M2L_THREADS = 64;
+
+// M2L: one 64-lane block per ttp box with list-2 entries
+__global__ void __launch_bounds__(M2L_THREADS) k3_m2l(KD kd, TD t, const int *ttp,
+                                                      const int *starts, const int *lists,
+                                                      const double *mpole, double *local)
+{
+    __shared__ double Ms[2 * (MAXP + 1) * (MAXP + 1)];
+    const int i = blockIdx.x;
+    const int e0 = starts[i], e1 = starts[i + 1];
+    if (e0 == e1) return;
+    const int Bx = ttp[i];
+    const int lv = t.level[Bx];
+    const int p = kd.order[lv];
+    const int nc = kd.ncl[lv];
+    const double h = kd.h[lv];
+    const double ih = 1.0 / h;
+    const int W = kd.W, S1 = 2 * W + 1;
+    const int tn = kd.tabn;
+    const int bx = t.coords[Bx], by = t.coords[(size_t)t.nboxes + Bx],
+              bz = t.coords[2 * (size_t)t.nboxes + Bx];
+    constexpr int MAXO = ((MAXP + 1) * (MAXP + 1) + M2L_THREADS - 1) / M2L_THREADS;
+    cplx acc[MAXO];
+#pragma unroll
+    for (int r = 0; r < MAXO; ++r) acc[r] = {0, 0};
+    for (int e = e0; e < e1; ++e) {
+        const int A = lists[e];
+        const double *a = mpole + ecoef_off(kd, A, lv);
+        __syncthreads();
+        for (int q = threadIdx.x; q < nc; q += M2L_THREADS) {
+            int n, m;
+            nm_of(q, n, m);
+            double s = 1.0;
+            for (int k = 0; k < n; ++k) s *= ih;
+            stc(Ms + 2 * q, s * ldc(a + 2 * q));
+        }
+        __syncthreads();
+        const int dx = bx - t.coords[A] + W;
+        const int dy = by - t.coords[(size_t)t.nboxes + A] + W;
+        const int dz = bz - t.coords[2 * (size_t)t.nboxes + A] + W;
+        // list 2 offsets satisfy |delta_i| <= 2 nsep + 1 by construction
+        if ((unsigned)dx >= (unsigned)S1 || (unsigned)dy >= (unsigned)S1 ||
+            (unsigned)dz >= (unsigned)S1)
+            continue;
+        const double *T = kd.tab_m2l + 2L * ((dx * S1 + dy) * S1 + dz) * tn * tn;
+#pragma unroll
+        for (int r = 0; r < MAXO; ++r) {
+            const int q = threadIdx.x + r * M2L_THREADS;
+            if (q < nc) {
+                int k, l;
+                nm_of(q, k, l);
+                cplx s = {0, 0};
+                for (int n = 0; n <= p; ++n) {
+                    const int N = n + k;
+                    for (int m = -n; m <= n; ++m) {
+                        const int M = m + l;
+                        if (M < -N || M > N) continue;
+                        s += ldc(Ms + 2 * IDX(n, m)) * ldc(T + 2 * IDX(N, M));
+                    }
+                }
+                acc[r] += s;
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < MAXO; ++r) {
+        const int q = threadIdx.x + r * M2L_THREADS;
+        if (q < nc) {
+            int k, l;
+            nm_of(q, k, l);
+            double s = (k & 1) ? -ih : ih;  // (-1)^k h^{-k-1}
+            for (int kk = 0; kk < k; ++kk) s *= ih;
+            double *o = local + ecoef_off(kd, Bx, lv) + 2 * q;
+            stc(o, ldc(o) + s * acc[r]);
+        }
+    }
+}
+
+// P2P (MODE 0, list 1), M2P (1, list 3), L2P (2) for non-QBX targets
+template <int MODE>
+__global__ void __launch_bounds__(64) k3_eval_targets(KD kd, TD t, int ntb, const int *tboxes,
+                                                      const int *starts, const int *lists,
+                                                      const int *fstart, const int *fcount,
+                                                      const int *unfilt, const double *tgt,
+                                                      int nt, const double *src, int ns,
+                                                      const double *str, const double *nrm,
+                                                      const double *expn, double *pot)
+{
+    const int it = blockIdx.x;
+    if (it >= ntb) return;
+    const int B = tboxes[it];
+    const int f0 = fstart[B], nf = fcount[B];
+    if (nf == 0) return;
+    const int lane = threadIdx.x;
+    if (MODE == 2) {
+        const int lv = t.level[B];
+        const v3 c = boxc(t, B);
+        for (int f = f0 + lane; f < f0 + nf; f += 64) {
+            const int ti = unfilt[f];
+            const cplx v = INV4PI * sum_regular_conj(expn + ecoef_off(kd, B, lv), kd.order[lv],
+                                                     sub(ld3(tgt, nt, ti), c));
+            double *o = pot + 2 * (size_t)ti;
+            stc(o, ldc(o) + v);
+        }
+        return;
+    }
+    int G = 64;
+    while (G > 1 && G * nf > 64) G >>= 1;
+    const int tpr = 64 / G;
+    const int tsub = lane / G, sub_ = lane % G;
+    const int e0 = starts[it], e1 = starts[it + 1];
+    for (int base = 0; base < nf; base += tpr) {
+        const bool active = base + tsub < nf;
+        int ti = 0;
+        cplx acc = {0, 0};
+        if (active) {
+            ti = unfilt[f0 + base + tsub];
+            const v3 tp = ld3(tgt, nt, ti);
+            if (MODE == 0) {
+                for (int e = e0; e < e1; ++e) {
+                    const int Sb = lists[e];
+                    const int s0 = t.sstart[Sb], s1 = s0 + t.scount[Sb];
+                    for (int s = s0 + sub_; s < s1; s += G) {
+                        const v3 nu = kd.dip ? ld3(nrm, ns, s) : v3{0, 0, 0};
+                        acc += p2p_pair(kd, ld3(src, ns, s), ldc(str + 2 * s), nu, tp);
+                    }
+                }
+            } else {
+                for (int e = e0 + sub_; e < e1; e += G) {
+                    const int D = lists[e];
+                    const int lv = t.level[D];
+                    acc += INV4PI * sum_irregular(expn + ecoef_off(kd, D, lv), kd.order[lv],
+                                                  sub(tp, boxc(t, D)));
+                }
+            }
+        }
+        for (int m = G / 2; m >= 1; m >>= 1) {
+            acc.x += __shfl_xor(acc.x, m);
+            acc.y += __shfl_xor(acc.y, m);
+        }
+        if (active && sub_ == 0) {
+            double *o = pot + 2 * (size_t)ti;
+            stc(o, ldc(o) + acc);
+        }
+    }
+}
+
+// ---- QBX stages
+template <int PQ>
+struct QAcc {
+    cplx v[(PQ + 1) * (PQ + 1)];
+};
+
+template <int PQ>
+__device__ void qbx_store(double *qexp, int c, const QAcc<PQ> &A, int accumulate)
+{
+    constexpr int NQ = (PQ + 1) * (PQ + 1);
+    double *o = qexp + 2 * (size_t)c * NQ;
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+        cplx v = A.v[i];
+        if (accumulate) v += ldc(o + 2 * i);
+        stc(o + 2 * i, v);
+    }
+}
+
+constexpr int P2Q_LANES = 16;
+
+// P2QBXL: 16 lanes per global center over its list-1 sources
+template <int PQ>
+__global__ void __launch_bounds__(256) k3_p2qbxl(KD kd, TD t, int ng, const int *gcent,
+                                                 const int *c2tb, const double *ctr, int nc,
+                                                 const int *l1s, const int *l1l,
+                                                 const double *src, int ns, const double *str,
+                                                 const double *nrm, double *qexp, int accumulate)
+{
+    constexpr int NQ = (PQ + 1) * (PQ + 1);
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = gid / P2Q_LANES, lane = gid % P2Q_LANES;
+    const bool active = g < ng;
+    QAcc<PQ> A;
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) A.v[i] = {0, 0};
+    int c = 0;
+    if (active) {
+        c = gcent[g];
+        const int it = c2tb[c];
+        const v3 cc = ld3(ctr, nc, c);
+        const int e0 = l1s[it], e1 = l1s[it + 1];
+        for (int e = e0; e < e1; ++e) {
+            const int Sb = l1l[e];
+            const int s0 = t.sstart[Sb], s1 = s0 + t.scount[Sb];
+            for (int s = s0 + lane; s < s1; s += P2Q_LANES) {
+                const v3 d = sub(ld3(src, ns, s), cc);
+                const cplx q = ldc(str + 2 * s);
+                if (!kd.dip) {
+                    cplx T[NQ];
+                    the_table<PQ>(d, T);
+#pragma unroll
+                    for (int i = 0; i < NQ; ++i) A.v[i] += q * T[i];
+                } else {
+                    constexpr int P1 = PQ + 1;
+                    cplx T[(P1 + 1) * (P1 + 1)];
+                    the_table<P1>(d, T);
+                    const v3 nu = ld3(nrm, ns, s);
+                    const cplx qnm = q * cplx{nu.x, -nu.y}, qnp = q * cplx{nu.x, nu.y};
+                    const double qz = -nu.z;
+#pragma unroll
+                    for (int n = 0; n <= PQ; ++n)
+#pragma unroll
+                        for (int m = -n; m <= n; ++m) {
+                            cplx g2 = (qz * T[IDX(n + 1, m)]) * q;
+                            g2 += 0.5 * (qnm * T[IDX(n + 1, m + 1)] - qnp * T[IDX(n + 1, m - 1)]);
+                            A.v[IDX(n, m)] += g2;
+                        }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+#pragma unroll
+        for (int m = P2Q_LANES / 2; m >= 1; m >>= 1) {
+            A.v[i].x += __shfl_xor(A.v[i].x, m, P2Q_LANES);
+            A.v[i].y += __shfl_xor(A.v[i].y, m, P2Q_LANES);
+        }
+    }
+    if (active && lane == 0) qbx_store<PQ>(qexp, c, A, accumulate);
+}
+
+constexpr int M2Q_LANES = 16;
+constexpr int M2Q_CENTERS = 256 / M2Q_LANES;
+
+// A lane group lives inside one wavefront: order its LDS writes before the
+// group's reads (and reads before the next overwrite) without a block barrier,
+// which the groups' different trip counts would make unsafe.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// M2QBXL: 16 lanes per global center; per list-3 box the irregular harmonics
+// T_N^M(c - b), N <= p_b + PQ, are built in LDS (one column m >= 0 per lane),
+// then lane j contracts output coefficients j, j + 16, ...
+template <int PQ>
+__global__ void __launch_bounds__(256) k3_m2qbxl(KD kd, TD t, int ng, const int *gcent,
+                                                 const int *c2tb, const double *ctr, int nc,
+                                                 const int *l3s, const int *l3l,
+                                                 const double *mpole, double *qexp)
+{
+    constexpr int NQ = (PQ + 1) * (PQ + 1);
+    constexpr int NO = (NQ + M2Q_LANES - 1) / M2Q_LANES;
+    extern __shared__ double lds[];
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = gid / M2Q_LANES, lane = gid % M2Q_LANES;
+    const int slot = threadIdx.x / M2Q_LANES;
+    const int pn = kd.pmax + PQ;
+    const int ntab = (pn + 1) * (pn + 1);
+    double *T = lds + 2L * slot * ntab;
+    const bool active = g < ng;
+    const int c = active ? gcent[g] : 0;
+    const int it = active ? c2tb[c] : 0;
+    const v3 cc = active ? ld3(ctr, nc, c) : v3{0, 0, 0};
+    cplx acc[NO];
+#pragma unroll
+    for (int r = 0; r < NO; ++r) acc[r] = {0, 0};
+    const int e0 = active ? l3s[it] : 0, e1 = active ? l3s[it + 1] : 0;
+    for (int e = e0; e < e1; ++e) {
+        const int D = l3l[e];
+        const int lv = t.level[D];
+        const int p1 = kd.order[lv];
+        const int N1 = p1 + PQ;
+        const v3 d = sub(cc, boxc(t, D));
+        const double r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+        const double ir2 = 1.0 / r2;
+        const cplx w = {d.x * ir2, d.y * ir2};
+        // column m = lane, lane + 16, ...: T_m^m by its product formula, then up in n
+        for (int m = lane; m <= N1; m += M2Q_LANES) {
+            cplx tmm = {sqrt(ir2), 0};
+            for (int i = 1; i <= m; ++i) tmm = (-(2.0 * i - 1.0)) * (tmm * w);
+            cplx a = {0, 0}, b = tmm;
+            for (int n = m; n <= N1; ++n) {
+                if (n > m) {
+                    const cplx cn = ir2 * (((2.0 * n - 1.0) * d.z) * b -
+                                           ((double)(n + m - 1) * (n - m - 1)) * a);
+                    a = b;
+                    b = cn;
+                }
+                stc(T + 2 * IDX(n, m), b);
+                if (m > 0) stc(T + 2 * IDX(n, -m), neg_m(b, m));
+            }
+        }
+        wave_sync();
+        const double *a = mpole + ecoef_off(kd, D, lv);
+#pragma unroll
+        for (int r = 0; r < NO; ++r) {
+            const int q = lane + r * M2Q_LANES;
+            if (q < NQ) {
+                int k, l;
+                nm_of(q, k, l);
+                cplx s = {0, 0};
+                for (int n = 0; n <= p1; ++n) {
+                    const int N = n + k;
+                    for (int m = -n; m <= n; ++m) {
+                        const int M = m + l;
+                        if (M < -N || M > N) continue;
+                        s += ldc(a + 2 * IDX(n, m)) * ldc(T + 2 * IDX(N, M));
+                    }
+                }
+                acc[r] += (k & 1) ? -s : s;
+            }
+        }
+        wave_sync();
+    }
+    if (active) {
+        double *o = qexp + 2 * (size_t)c * NQ;
+#pragma unroll
+        for (int r = 0; r < NO; ++r) {
+            const int q = lane + r * M2Q_LANES;
+            if (q < NQ) stc(o + 2 * q, ldc(o + 2 * q) + acc[r]);
+        }
+    }
+}
+
+// L2QBXL: one thread per global center
+template <int PQ>
+__global__ void k3_l2qbxl(KD kd, TD t, int ng, const int *gcent, const int *c2tb,
+                          const int *tboxes, const double *ctr, int nc, const double *local,
+                          double *qexp)
+{
+    constexpr int NQ = (PQ + 1) * (PQ + 1);
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    const int c = gcent[g];
+    const int Bx = tboxes[c2tb[c]];
+    const int lv = t.level[Bx];
+    const int p1 = kd.order[lv];
+    const v3 d = sub(ld3(ctr, nc, c), boxc(t, Bx));
+    const double *L = local + ecoef_off(kd, Bx, lv);
+    QAcc<PQ> A;
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) A.v[i] = {0, 0};
+    const double r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+    const cplx hw = {-0.5 * d.x, -0.5 * d.y};
+    cplx ymm = {1, 0};
+    for (int i = 0; i <= p1; ++i) {        // column i >= 0 of Y_j^i, j = i..p1
+        if (i > 0) ymm = (1.0 / i) * (ymm * hw);
+        cplx ya = {0, 0}, yb = ymm;
+        for (int j = i; j <= p1; ++j) {
+            if (j > i) {
+                const cplx cn = (1.0 / ((double)(j - i) * (j + i))) *
+                                (((2.0 * j - 1.0) * d.z) * yb - r2 * ya);
+                ya = yb;
+                yb = cn;
+            }
+            const cplx yc = conj(yb);                  // conj(Y_j^i)
+            const cplx yn = (i & 1) ? -yb : yb;        // conj(Y_j^{-i}) = (-1)^i Y_j^i
+#pragma unroll
+            for (int a = 0; a <= PQ; ++a) {
+                if (a + j > p1) continue;
+#pragma unroll
+                for (int b = -a; b <= a; ++b) {
+                    const int n = a + j;
+                    const int m1 = b + i;
+                    if (m1 >= -n && m1 <= n) A.v[IDX(a, b)] += ldc(L + 2 * IDX(n, m1)) * yc;
+                    const int m2 = b - i;
+                    if (i > 0 && m2 >= -n && m2 <= n)
+                        A.v[IDX(a, b)] += ldc(L + 2 * IDX(n, m2)) * yn;
+                }
+            }
+        }
+    }
+    qbx_store<PQ>(qexp, c, A, 1);
+}
+
+// QBXL2P: one thread per (center, target) entry
+__global__ void k3_qbxl2p(KD kd, int nctt, const int *ctt_lists, const int *tree_ttc,
+                          const double *tgt, int nt, const double *ctr, int nc,
+                          const double *qexp, double *pot)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nctt) return;
+    const int ti = ctt_lists[j];
+    const int c = tree_ttc[ti];
+    const v3 d = sub(ld3(tgt, nt, ti), ld3(ctr, nc, c));
+    stc(pot + 2 * (size_t)ti,
+        INV4PI * sum_regular_conj(qexp + 2 * (size_t)c * kd.ncq, kd.pq, d));
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+using zc = std::complex<double>;
+
+static void host_irregular(int p, const double *x, zc *T)
+{
+    const double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+    const double ir2 = 1.0 / r2;
+    const zc w(x[0] * ir2, x[1] * ir2);
+    zc tmm(std::sqrt(ir2), 0.0);
+    for (int m = 0; m <= p; ++m) {
+        if (m > 0) tmm = -(2.0 * m - 1.0) * w * tmm;
+        zc a = 0.0, b = tmm;
+        T[IDX(m, m)] = tmm;
+        for (int n = m + 1; n <= p; ++n) {
+            zc c = ir2 * ((2.0 * n - 1.0) * x[2] * b - (double)(n + m - 1) * (n - m - 1) * a);
+            T[IDX(n, m)] = c;
+            a = b;
+            b = c;
+        }
+    }
+    for (int n = 1; n <= p; ++n)
+        for (int m = 1; m <= n; ++m) T[IDX(n, -m)] = ((m & 1) ? -1.0 : 1.0) * std::conj(T[IDX(n, m)]);
+}
+
+static void host_regular(int p, const double *x, zc *Y)
+{
+    const double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+    const zc hw(-0.5 * x[0], -0.5 * x[1]);
+    zc ymm = 1.0;
+    for (int m = 0; m <= p; ++m) {
+        if (m > 0) ymm = (1.0 / m) * hw * ymm;
+        zc a = 0.0, b = ymm;
+        Y[IDX(m, m)] = ymm;
+        for (int n = m + 1; n <= p; ++n) {
+            zc c = ((2.0 * n - 1.0) * x[2] * b - r2 * a) / ((double)(n - m) * (n + m));
+            Y[IDX(n, m)] = c;
+            a = b;
+            b = c;
+        }
+    }
+    for (int n = 1; n <= p; ++n)
+        for (int m = 1; m <= n; ++m) Y[IDX(n, -m)] = ((m & 1) ? -1.0 : 1.0) * std::conj(Y[IDX(n, m)]);
+}
+
+static int make_kd(const qbx_kernel_desc *d, const qbx_tree *T, KD *kd, int nsep = 2)
+{
+    QBX_CHECK(d, "null kernel description");
+    QBX_CHECK(d->kernel == QBX_LAPLACE3D, "3D stages support QBX_LAPLACE3D only (got %d)",
+              d->kernel);
+    QBX_CHECK(d->source_kind == QBX_SOURCE_S || d->source_kind == QBX_SOURCE_D,
+              "unsupported source kind %d", d->source_kind);
+    QBX_CHECK(d->qbx_order >= 0 && d->qbx_order <= MAXPQ, "3D qbx_order must be in [0, %d]",
+              MAXPQ);
+    QBX_CHECK(d->level_orders && d->nlevels >= 1, "level_orders missing");
+    if (T) {
+        QBX_CHECK(T->dim == 3, "3D kernels need a 3D tree");
+        QBX_CHECK(T->nlevels <= FML, "tree has too many levels (%d)", T->nlevels);
+        QBX_CHECK(d->nlevels >= T->nlevels, "level_orders has %d entries, tree has %d levels",
+                  d->nlevels, T->nlevels);
+    }
+    memset(kd, 0, sizeof(*kd));
+    kd->dip = d->source_kind == QBX_SOURCE_D;
+    kd->pq = d->qbx_order;
+    kd->nsep = nsep;
+    kd->W = 2 * nsep + 1;
+    const int nl = T ? T->nlevels : std::min(d->nlevels, FML);
+    kd->nlevels = nl;
+    long coef = 0;
+    int pmax = 0;
+    for (int l = 0; l < nl; ++l) {
+        const int p = d->level_orders[l];
+        QBX_CHECK(p >= 1 && p <= MAXP, "3D fmm order %d at level %d outside [1, %d]", p, l, MAXP);
+        kd->order[l] = p;
+        pmax = std::max(pmax, p);
+        kd->ncl[l] = (p + 1) * (p + 1);
+        kd->h[l] = T ? ldexp(T->root_extent, -l) : 1.0;
+        kd->lbox[l] = T ? T->level_start_box_nrs[l] : 0;
+        kd->lcoef[l] = coef;
+        if (T)
+            coef += (long)(T->level_start_box_nrs[l + 1] - T->level_start_box_nrs[l]) * kd->ncl[l];
+    }
+    kd->lbox[nl] = T ? T->nboxes : 0;
+    kd->lcoef[nl] = coef;
+    kd->pmax = pmax;
+    kd->tabn = 2 * pmax + 1;
+    kd->ncq = (kd->pq + 1) * (kd->pq + 1);
+    return QBX_OK;
+}
+
+// unit-offset tables, cached in the context per (nsep, pmax)
+static int ensure_tables(qbx_ctx *ctx, KD *kd)
+{
+    std::vector<double> key = {3.0, (double)kd->nsep, (double)kd->pmax};
+    if (key != ctx->tab3_key) {
+        const int W = kd->W, S1 = 2 * W + 1, tn = kd->tabn;
+        const long ncls = (long)S1 * S1 * S1;
+        std::vector<zc> tab((size_t)ncls * tn * tn, zc(0.0, 0.0));
+        for (int ix = -W; ix <= W; ++ix)
+            for (int iy = -W; iy <= W; ++iy)
+                for (int iz = -W; iz <= W; ++iz) {
+                    if (!ix && !iy && !iz) continue;
+                    const double x[3] = {(double)ix, (double)iy, (double)iz};
+                    const long cl = ((long)(ix + W) * S1 + (iy + W)) * S1 + (iz + W);
+                    host_irregular(2 * kd->pmax, x, tab.data() + cl * tn * tn);
+                }
+        const int nc1 = (kd->pmax + 1) * (kd->pmax + 1);
+        std::vector<zc> ch((size_t)8 * nc1);
+        for (int ci = 0; ci < 8; ++ci) {
+            const double x[3] = {(ci & 1) ? 0.5 : -0.5, (ci & 2) ? 0.5 : -0.5,
+                                 (ci & 4) ? 0.5 : -0.5};
+            host_regular(kd->pmax, x, ch.data() + (size_t)ci * nc1);
+        }
+        QBX_TRY(ctx->ws_tab3_m2l.alloc(2L * (long)tab.size()));
+        QBX_TRY(ctx->ws_tab3_ch.alloc(2L * (long)ch.size()));
+        QBX_HIP(hipMemcpy(ctx->ws_tab3_m2l.p, tab.data(), sizeof(zc) * tab.size(),
+                          hipMemcpyHostToDevice));
+        QBX_HIP(hipMemcpy(ctx->ws_tab3_ch.p, ch.data(), sizeof(zc) * ch.size(),
+                          hipMemcpyHostToDevice));
+        ctx->tab3_key = key;
+    }
+    kd->tab_m2l = ctx->ws_tab3_m2l.p;
+    kd->tab_ch = ctx->ws_tab3_ch.p;
+    return QBX_OK;
+}
+
+static TD make_td(const qbx_tree *T)
+{
+    TD t;
+    t.nboxes = T->nboxes;
+    t.level = T->box_levels.p;
+    t.parent = T->box_parent_ids.p;
+    t.child = T->box_child_ids.p;
+    t.coords = T->box_coords.p;
+    t.centers = T->box_centers.p;
+    t.sstart = T->box_source_starts.p;
+    t.scount = T->box_source_counts_nonchild.p;
+    t.scumul = T->box_source_counts_cumul.p;
+    return t;
+}
+
+static Flat make_flat(const KD &kd, const std::vector<int> &level_starts)
+{
+    Flat f;
+    memset(&f, 0, sizeof(f));
+    f.nl = kd.nlevels;
+    long t = 0;
+    for (int l = 0; l < kd.nlevels; ++l) {
+        f.tstart[l] = t;
+        f.istart[l] = level_starts[l];
+        f.nper[l] = kd.ncl[l];
+        t += (long)(level_starts[l + 1] - level_starts[l]) * f.nper[l];
+    }
+    f.tstart[kd.nlevels] = t;
+    return f;
+}
+
+#define DISPATCH3_PQ(pq, KERNEL, ...)                                                   \
+    switch (pq) {                                                                        \
+    case 0: hipLaunchKernelGGL(KERNEL<0>, __VA_ARGS__); break;                           \
+    case 1: hipLaunchKernelGGL(KERNEL<1>, __VA_ARGS__); break;                           \
+    case 2: hipLaunchKernelGGL(KERNEL<2>, __VA_ARGS__); break;                           \
+    case 3: hipLaunchKernelGGL(KERNEL<3>, __VA_ARGS__); break;                           \
+    case 4: hipLaunchKernelGGL(KERNEL<4>, __VA_ARGS__); break;                           \
+    case 5: hipLaunchKernelGGL(KERNEL<5>, __VA_ARGS__); break;                           \
+    case 6: hipLaunchKernelGGL(KERNEL<6>, __VA_ARGS__); break;                           \
+    case 7: hipLaunchKernelGGL(KERNEL<7>, __VA_ARGS__); break;                           \
+    case 8: hipLaunchKernelGGL(KERNEL<8>, __VA_ARGS__); break;                           \
+    default: set_error("qbx_order out of range"); return QBX_ERR_INVALID;               \
+    }
+
+static inline long exp_bytes(const KD &kd) { return 16L * kd.lcoef[kd.nlevels]; }
+
+}  // namespace f3
+
+using namespace f3;
+
+long f3_expansion_size(const qbx_kernel_desc *d, const qbx_tree *T)
+{
+    KD kd;
+    if (!T || make_kd(d, T, &kd) != QBX_OK) return -1;
+    return kd.lcoef[kd.nlevels];
+}
+
+int f3_expansion_level_offsets(const qbx_kernel_desc *d, const qbx_tree *T, long *offsets)
+{
+    KD kd;
+    QBX_TRY(make_kd(d, T, &kd));
+    for (int l = 0; l <= kd.nlevels; ++l) offsets[l] = kd.lcoef[l];
+    return QBX_OK;
+}
+
+int f3_ncoeff_qbx(const qbx_kernel_desc *d)
+{
+    KD kd;
+    if (make_kd(d, nullptr, &kd) != QBX_OK) return -1;
+    return kd.ncq;
+}
+
+int f3_form_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                       const qbx_kernel_desc *d, const double *str, const double *nrm,
+                       double *mpole)
+{
+    KD kd;
+    QBX_TRY(make_kd(d, T, &kd));
+    QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
+    hipStream_t st = S(stream);
+    QBX_HIP(hipMemsetAsync(mpole, 0, exp_bytes(kd), st));
+    const Flat fl = make_flat(kd, R->h_level_start_source_box_nrs);
+    const long nthr = fl.tstart[fl.nl];
+    if (nthr > 0)
+        hipLaunchKernelGGL(k3_p2m, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, make_td(T), fl,
+                           R->source_boxes.p, T->sources.p, T->nsources, str, nrm, mpole);
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
+int f3_coarsen_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                          const qbx_kernel_desc *d, double *mpole)
+{
+    KD kd;
+    QBX_TRY(make_kd(d, T, &kd, R->nsep));
+    QBX_TRY(ensure_tables(ctx, &kd));
+    hipStream_t st = S(stream);
+    for (int l = T->nlevels - 2; l >= 0; --l) {
+        const int i0 = R->h_level_start_source_parent_box_nrs[l];
+        const int i1 = R->h_level_start_source_parent_box_nrs[l + 1];
+        const long nthr = (long)(i1 - i0) * kd.ncl[l];
+        if (nthr > 0)
+            hipLaunchKernelGGL(k3_m2m, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, make_td(T),
+                               l, i0, i1, R->source_parent_boxes.p, mpole);
+    }
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
+int f3_eval_targets(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                    const qbx_geo *G, const qbx_kernel_desc *d, int mode, const double *str,
+                    const double *nrm, const double *expn, double *pot, int accumulate)
+{
+    KD kd;
+    QBX_TRY(make_kd(d, T, &kd));
+    hipStream_t st = S(stream);
+    if (!accumulate)
+        QBX_HIP(hipMemsetAsync(pot, 0, sizeof(double) * 2 * (size_t)T->ntargets, st));
+    const int ntb = R->ntarget_boxes;
+    if (ntb == 0 || G->nfiltered == 0) return QBX_OK;
+    const TD td = make_td(T);
+    const int *starts = mode == 1 ? R->l3_starts.p : R->l1_starts.p;
+    const int *lists = mode == 1 ? R->l3_lists.p : R->l1_lists.p;
+#define EVAL3_LAUNCH(M)                                                                    \
+    hipLaunchKernelGGL(k3_eval_targets<M>, dim3(ntb), dim3(64), 0, st, kd, td, ntb,        \
+                       R->target_boxes.p, starts, lists, G->nqbtl_box_target_starts.p,    \
+                       G->nqbtl_box_target_counts_nonchild.p, G->nqbtl_unfiltered.p,      \
+                       T->targets.p, T->ntargets, T->sources.p, T->nsources, str, nrm,    \
+                       expn, pot)
+    if (mode == 0) EVAL3_LAUNCH(0);
+    else if (mode == 1) EVAL3_LAUNCH(1);
+    else EVAL3_LAUNCH(2);
+#undef EVAL3_LAUNCH
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
+int f3_multipole_to_local(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                          const qbx_kernel_desc *d, const double *mpole, double *local)
+{
+    KD kd;
+    QBX_TRY(make_kd(d, T, &kd, R->nsep));
+    QBX_TRY(ensure_tables(ctx, &kd));
+    hipStream_t st = S(stream);
+    QBX_HIP(hipMemsetAsync(local, 0, exp_bytes(kd), st));
+    if (R->nttp_boxes > 0 && R->n2 > 0)
+        hipLaunchKernelGGL(k3_m2l, dim3(R->nttp_boxes), dim3(M2L_THREADS), 0, st, kd,
+                           make_td(T), R->ttp_boxes.p, R->l2_starts.p, R->l2_lists.p, mpole,
+                           local);
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
+int f3_form_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                   const qbx_kernel_desc *d, const double *str, const double *nrm, double *local,
+                   int accumulate)
+{
+    KD kd;
+    QBX_TRY(make_kd(d, T, &kd));
+    QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
+    hipStream_t st = S(stream);
+    if (!accumulate) QBX_HIP(hipMemsetAsync(local, 0, exp_bytes(kd), st));
+    const Flat fl = make_flat(kd, R->h_level_start_ttp_box_nrs);
+    const long nthr = fl.tstart[fl.nl];
+    if (nthr > 0 && R->n4 > 0)
+        hipLaunchKernelGGL(k3_p2l, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, make_td(T), fl,
+                           R->ttp_boxes.p, R->l4_starts.p, R->l4_lists.p, T->sources.p,
+                           T->nsources, str, nrm, local);
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
+int f3_refine_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                     const qbx_kernel_desc *d, double *local)
+{
+    KD kd;
+    QBX_TRY(make_kd(d, T, &kd, R->nsep));
+    QBX_TRY(ensure_tables(ctx, &kd));
+    hipStream_t st = S(stream);
+    for (int l = 1; l < T->nlevels; ++l) {
+        const int i0 = R->h_level_start_ttp_box_nrs[l], i1 = R->h_level_start_ttp_box_nrs[l + 1];
+        const long nthr = (long)(i1 - i0) * kd.ncl[l];
+        if (nthr > 0)
+            hipLaunchKernelGGL(k3_l2l, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, make_td(T), l,
+                               i0, i1, R->ttp_boxes.p, local);
+    }
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
+int f3_form_global_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T,
+                              const qbx_trav *R, const qbx_geo *G, const qbx_kernel_desc *d,
+                              const double *str, const double *nrm, double *qexp, int accumulate)
+{
+    KD kd;
+    QBX_TRY(make_kd(d, T, &kd));
+    QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
+    hipStream_t st = S(stream);
+    if (!accumulate)
+        QBX_HIP(hipMemsetAsync(qexp, 0, sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq, st));
+    const int ng = G->nglobal;
+    if (ng == 0) return QBX_OK;
+    DISPATCH3_PQ(kd.pq, k3_p2qbxl, dim3(cdiv((long)ng * P2Q_LANES, 256)), dim3(256), 0, st, kd,
+                 make_td(T), ng, G->global_qbx_centers.p, G->center_to_target_box.p,
+                 G->centers.p, G->ncenters, R->l1_starts.p, R->l1_lists.p, T->sources.p,
+                 T->nsources, str, nrm, qexp, accumulate);
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
+int f3_translate_box_multipoles_to_qbx_local(qbx_ctx *ctx, void *stream, const qbx_tree *T,
+                                             const qbx_trav *R, const qbx_geo *G,
+                                             const qbx_kernel_desc *d, const double *mpole,
+                                             double *qexp, int accumulate)
+{
+    KD kd;
+    QBX_TRY(make_kd(d, T, &kd));
+    hipStream_t st = S(stream);
+    if (!accumulate)
+        QBX_HIP(hipMemsetAsync(qexp, 0, sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq, st));
+    const int ng = G->nglobal;
+    if (ng == 0 || R->n3 == 0) return QBX_OK;
+    const int pn = kd.pmax + kd.pq;
+    const size_t lds = sizeof(double) * 2 * (size_t)M2Q_CENTERS * (pn + 1) * (pn + 1);
+    QBX_CHECK(lds <= 64 * 1024, "M2QBXL: orders too high for LDS staging (%d + %d)", kd.pmax,
+              kd.pq);
+    // every block runs whole: the grid covers ng rounded up to whole centers
+    DISPATCH3_PQ(kd.pq, k3_m2qbxl, dim3(cdiv((long)ng * M2Q_LANES, 256)), dim3(256), lds, st, kd,
+                 make_td(T), ng, G->global_qbx_centers.p, G->center_to_target_box.p,
+                 G->centers.p, G->ncenters, R->l3_starts.p, R->l3_lists.p, mpole, qexp);
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
+int f3_translate_box_local_to_qbx_local(qbx_ctx *ctx, void *stream, const qbx_tree *T,
+                                        const qbx_trav *R, const qbx_geo *G,
+                                        const qbx_kernel_desc *d, const double *local,
+                                        double *qexp, int accumulate)
+{
+    KD kd;
+    QBX_TRY(make_kd(d, T, &kd));
+    hipStream_t st = S(stream);
+    if (!accumulate)
+        QBX_HIP(hipMemsetAsync(qexp, 0, sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq, st));
+    const int ng = G->nglobal;
+    if (ng == 0) return QBX_OK;
+    DISPATCH3_PQ(kd.pq, k3_l2qbxl, dim3(cdiv(ng, 64)), dim3(64), 0, st, kd, make_td(T), ng,
+                 G->global_qbx_centers.p, G->center_to_target_box.p, R->target_boxes.p,
+                 G->centers.p, G->ncenters, local, qexp);
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
+int f3_eval_qbx_expansions(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_geo *G,
+                           const qbx_kernel_desc *d, const double *qexp, double *pot)
+{
+    KD kd;
+    QBX_TRY(make_kd(d, T, &kd));
+    hipStream_t st = S(stream);
+    if (G->nctt > 0)
+        hipLaunchKernelGGL(k3_qbxl2p, dim3(cdiv(G->nctt, 64)), dim3(64), 0, st, kd, G->nctt,
+                           G->ctt_lists.p, G->tree_ttc.p, T->targets.p, T->ntargets,
+                           G->centers.p, G->ncenters, qexp, pot);
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
+}  // namespace qbx

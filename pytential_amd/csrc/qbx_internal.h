@@ -80,6 +80,9 @@ struct qbx_ctx {
     // Helmholtz translation tables and the (tree, k, orders) they were built for
     qbx::dbuf<double> ws_tab_m2l, ws_tab_mm;
     std::vector<double> tab_key;
+    // 3D Laplace unit-offset harmonic tables and the (nsep, pmax) they hold
+    qbx::dbuf<double> ws_tab3_m2l, ws_tab3_ch;
+    std::vector<double> tab3_key;
 };
 
 struct qbx_tree {
@@ -136,4 +139,37 @@ int sort_pairs_i32(qbx_ctx *ctx, hipStream_t st, const int *kin, int *kout, cons
                    int *vout, int n, int end_bit);
 int compact_indices(qbx_ctx *ctx, hipStream_t st, const int *flags, int n, dbuf<int> &out,
                     int *count);
+
+// 3D Laplace stages (fmm3d.hip), same contracts as the C-ABI entry points
+long f3_expansion_size(const qbx_kernel_desc *d, const qbx_tree *T);
+int f3_expansion_level_offsets(const qbx_kernel_desc *d, const qbx_tree *T, long *offsets);
+int f3_ncoeff_qbx(const qbx_kernel_desc *d);
+int f3_form_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                       const qbx_kernel_desc *d, const double *str, const double *nrm,
+                       double *mpole);
+int f3_coarsen_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                          const qbx_kernel_desc *d, double *mpole);
+int f3_eval_targets(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                    const qbx_geo *G, const qbx_kernel_desc *d, int mode, const double *str,
+                    const double *nrm, const double *expn, double *pot, int accumulate);
+int f3_multipole_to_local(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                          const qbx_kernel_desc *d, const double *mpole, double *local);
+int f3_form_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                   const qbx_kernel_desc *d, const double *str, const double *nrm, double *local,
+                   int accumulate);
+int f3_refine_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                     const qbx_kernel_desc *d, double *local);
+int f3_form_global_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T,
+                              const qbx_trav *R, const qbx_geo *G, const qbx_kernel_desc *d,
+                              const double *str, const double *nrm, double *qexp, int accumulate);
+int f3_translate_box_multipoles_to_qbx_local(qbx_ctx *ctx, void *stream, const qbx_tree *T,
+                                             const qbx_trav *R, const qbx_geo *G,
+                                             const qbx_kernel_desc *d, const double *mpole,
+                                             double *qexp, int accumulate);
+int f3_translate_box_local_to_qbx_local(qbx_ctx *ctx, void *stream, const qbx_tree *T,
+                                        const qbx_trav *R, const qbx_geo *G,
+                                        const qbx_kernel_desc *d, const double *local,
+                                        double *qexp, int accumulate);
+int f3_eval_qbx_expansions(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_geo *G,
+                           const qbx_kernel_desc *d, const double *qexp, double *pot);
 }  // namespace qbx

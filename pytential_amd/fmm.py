@@ -20,7 +20,7 @@ class QBXExpansionWrangler:
     """Stages of the QBX FMM on one tree (pytential/qbx/fmm.py:118-396)."""
 
     def __init__(self, geo_data, kernel, k, source_kind, qbx_order, level_orders,
-                 output="pot"):
+                 output="pot", use_target_specific_qbx=False):
         self.geo_data = geo_data
         self.tree = geo_data.tree()
         self.traversal = geo_data.traversal()
@@ -47,16 +47,17 @@ class QBXExpansionWrangler:
                              qbx_order, len(lo), lo.ctypes.data_as(ct.POINTER(ct.c_int)))
         self.source_kind = source_kind
         self.qbx_order = qbx_order
-        self.expansion_size = None
-        if self.dim == 2:
-            self.expansion_size = _lib.lib().qbx_expansion_size(ct.byref(self.kd),
-                                                                self.tree.h)
-            if self.expansion_size < 0:
-                raise _lib.QBXError(1, _lib.lib().qbx_last_error().decode())
-            offs = (ct.c_long * (self.tree.nlevels + 1))()
-            call("qbx_expansion_level_offsets", ct.byref(self.kd), self.tree.h, offs)
-            self.level_offsets = np.array(offs[:], dtype=np.int64)
-            self.ncoeff_qbx = _lib.lib().qbx_ncoeff_qbx(ct.byref(self.kd))
+        # TSQBX replaces P2QBXL's list-1 part (qbx/fmmlib.py:73-108, 295-298)
+        if use_target_specific_qbx and (self.dim != 3 or output != "pot"):
+            raise ValueError("target-specific QBX needs a 3D geometry and potential output")
+        self.using_tsqbx = bool(use_target_specific_qbx)
+        self.expansion_size = _lib.lib().qbx_expansion_size(ct.byref(self.kd), self.tree.h)
+        if self.expansion_size < 0:
+            raise _lib.QBXError(1, _lib.lib().qbx_last_error().decode())
+        offs = (ct.c_long * (self.tree.nlevels + 1))()
+        call("qbx_expansion_level_offsets", ct.byref(self.kd), self.tree.h, offs)
+        self.level_offsets = np.array(offs[:], dtype=np.int64)
+        self.ncoeff_qbx = _lib.lib().qbx_ncoeff_qbx(ct.byref(self.kd))
 
     # ---- data vector utilities
     def multipole_expansion_zeros(self):
@@ -65,12 +66,21 @@ class QBXExpansionWrangler:
                            device="cuda")
 
     def expansions_to_dense(self, e):
-        """[nboxes][ncoeff(pmax)] view of level-compact expansions (Helmholtz
-        coefficient n at column n + pmax), for inspection and tests."""
+        """[nboxes][ncoeff(pmax)] view of level-compact expansions (2D
+        Helmholtz coefficient n at column n + pmax; 3D (n, m) at n*n + n + m),
+        for inspection and tests."""
         e = e.detach().cpu().numpy()
         lo = self._lo[:self.tree.nlevels]
         pmax = int(max(lo))
         helm = self.kernel != "laplace"
+        if self.dim == 3:
+            ls = self.tree.level_start_box_nrs.cpu().numpy()
+            out = np.zeros((self.tree.nboxes, (pmax + 1) ** 2), np.complex128)
+            for lev in range(self.tree.nlevels):
+                nc = (int(lo[lev]) + 1) ** 2
+                blk = e[self.level_offsets[lev]:self.level_offsets[lev + 1]].reshape(-1, nc)
+                out[ls[lev]:ls[lev + 1], :nc] = blk
+            return out
         ncf = 2 * pmax + 1 if helm else pmax + 1
         ls = self.tree.level_start_box_nrs.cpu().numpy()
         out = np.zeros((self.tree.nboxes, ncf), np.complex128)
@@ -161,6 +171,8 @@ class QBXExpansionWrangler:
     # ---- QBX stages
     def form_global_qbx_locals(self, strengths, normals):
         q = self.qbx_local_expansion_zeros()
+        if self.using_tsqbx:
+            return q
         self._a("qbx_form_global_qbx_locals", self.tree.h, self.traversal.h, self.geo.h,
                 ct.byref(self.kd), _p(strengths), _p(normals), _p(q), 0)
         return q
@@ -188,8 +200,10 @@ class QBXExpansionWrangler:
     def eval_target_specific_qbx_locals(self, strengths, normals=None):
         """TSQBX (qbx/fmmlib.py:587-639, impl.pyx:566-785): potential (or its
         gradient, ``output="grad"``) at every QBX target from the list-1
-        sources of its center's box, by the target-specific expansion.  3D
-        only; zeros otherwise, like the reference (qbx/fmmlib.py:141-148).
+        sources of its center's box, by the target-specific expansion, in the
+        wrangler's units (G = 1/(4 pi r); the C entry point returns the
+        unscaled 4 pi G form like impl.pyx).  3D only; zeros otherwise, like
+        the reference (qbx/fmmlib.py:141-148, 590-591).
         Returns [ntargets] complex, or [3, ntargets] for the gradient."""
         if self.dim != 3:
             return self.full_output_zeros()
@@ -207,7 +221,8 @@ class QBXExpansionWrangler:
         self._a("qbx_eval_target_specific_qbx_locals", self.tree.h, self.traversal.h,
                 self.geo.h, kernel, op, self.k.real, self.k.imag, self.qbx_order,
                 _p(strengths), _p(normals), _p(pot), _p(grad))
-        return grad if op == _lib.TSQBX_SP else pot
+        out = grad if op == _lib.TSQBX_SP else pot
+        return out.mul_(1.0 / (4 * np.pi))
 
     def finalize_potentials(self, pot_tree):
         nuser = self.tree.ntargets - self.geo_data.ncenters
@@ -232,12 +247,17 @@ def drive_fmm(wrangler: QBXExpansionWrangler, strengths, normals):
     w.translate_box_multipoles_to_qbx_local(mpole, qexp)
     w.translate_box_local_to_qbx_local(local, qexp)
     w.eval_qbx_expansions(qexp, pot)
+    if w.using_tsqbx:
+        # QBX targets: QBXL2P overwrote them; add the list-1 part (fmm.py:559-565)
+        pot.add_(w.eval_target_specific_qbx_locals(strengths, normals))
     return w.finalize_potentials(pot)
 
 
 def drive_fmm_fused(wrangler: QBXExpansionWrangler, strengths, normals, out=None):
     """Same stage sequence from the single ``qbx_drive_fmm`` entry point."""
     w = wrangler
+    if w.using_tsqbx:
+        raise ValueError("the fused driver has no target-specific QBX stage; use drive_fmm")
     nuser = w.tree.ntargets - w.geo_data.ncenters
     if out is None:
         out = torch.empty(nuser, dtype=torch.complex128, device="cuda")

@@ -48,7 +48,8 @@ class QBXLayerPotentialSource:
                  target_association_tolerance=None, _max_leaf_refine_weight=None,
                  _expansion_stick_out_factor=0.5, _well_sep_is_n_away=2,
                  _from_sep_smaller_min_nsources_cumul=None, _box_extent_norm=None,
-                 _expansions_in_tree_have_extent=True, device=None):
+                 _expansions_in_tree_have_extent=True, _use_target_specific_qbx=False,
+                 device=None):
         if fine_order is None:
             raise ValueError("'fine_order' must be provided.")
         if qbx_order is None:
@@ -82,6 +83,7 @@ class QBXLayerPotentialSource:
             15 if _from_sep_smaller_min_nsources_cumul is None
             else _from_sep_smaller_min_nsources_cumul)
         self._box_extent_norm = "l2" if _box_extent_norm is None else _box_extent_norm
+        self._use_target_specific_qbx = bool(_use_target_specific_qbx)
         self.ctx = Context.get(device)
         d = density_discr
         self.dev = {
@@ -122,8 +124,9 @@ class QBXLayerPotentialSource:
 
     def make_wrangler(self, geo_data, kernel, k, source_kind):
         tree = geo_data.tree()
-        return _fmm.QBXExpansionWrangler(geo_data, kernel, k, source_kind, self.qbx_order,
-                                         self.level_orders(kernel, k, tree))
+        return _fmm.QBXExpansionWrangler(
+            geo_data, kernel, k, source_kind, self.qbx_order, self.level_orders(kernel, k, tree),
+            use_target_specific_qbx=self._use_target_specific_qbx)
 
     def exec_compute_potential(self, kernel, density, targets_and_sides, k=0.0,
                                source_kind="S", fused=True):
@@ -147,7 +150,7 @@ class QBXLayerPotentialSource:
         qd = self.quad_density(density)
         normals = self.dev["quad_normals"] if source_kind == "D" else None
         strengths, nrm = w.reorder_sources(qd, self.dev["quad_weights"], normals)
-        if fused:
+        if fused and not w.using_tsqbx:
             pot = _fmm.drive_fmm_fused(w, strengths, nrm)
         else:
             pot = _fmm.drive_fmm(w, strengths, nrm)

@@ -1,0 +1,132 @@
+"""GPU parity for the 3D Laplace QBX-FMM: libqbx_mi355x through its C ABI
+against the CPU oracle on the same triangulated sphere.
+
+* every stage (P2M+M2M, M2L+P2L+L2L, P2QBXL+M2QBXL+L2QBXL, drive_fmm and the
+  fused driver) to 1e-12 relative to the array's largest entry;
+* the FMM against direct QBX (error stated);
+* the target-specific pipeline equals the expansion pipeline, the identity the
+  reference asserts at 1e-13 (test/test_target_specific_qbx.py:204-214);
+* the sphere's single-layer eigenvalue S Y_3^2 = Y_3^2 / 7 on the surface.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+REL_TOL = 1e-12
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    scale = max(np.max(np.abs(b)), 1e-300)
+    return float(np.max(np.abs(a - b)) / scale) if a.size else 0.0
+
+
+@pytest.fixture(scope="module", params=["S", "D"])
+def case3(request):
+    import oracle
+
+    from pytential_amd.qbx import QBXLayerPotentialSource
+    from pytential_amd.surface import icosphere
+    kind = request.param
+    discr = icosphere(2, order=4, fine_order=5)
+    rng = np.random.default_rng(1003)
+    v = rng.normal(size=(3, 400))
+    v /= np.linalg.norm(v, axis=0)
+    tgt = v * np.where(rng.uniform(size=400) < 0.5, rng.uniform(0.3, 0.85, 400),
+                       rng.uniform(1.15, 2.0, 400))
+    targets = (("self", +1), (tgt, 0))
+    user_targets = np.concatenate([discr.nodes, tgt], axis=1)
+    sides = np.concatenate([np.ones(discr.ndofs, np.int32), np.zeros(400, np.int32)])
+    print(f"\n[case3] {kind}: icosphere(2) ndofs={discr.ndofs} nsources={discr.nsources}",
+          flush=True)
+    src = QBXLayerPotentialSource(discr, fine_order=discr.fine_order, qbx_order=3,
+                                  fmm_order=8, _max_leaf_refine_weight=64)
+    geo = src.qbx_fmm_geometry_data(targets)
+    ref = oracle.GeometryData(discr, user_targets, sides, max_leaf_refine_weight=64)
+    sigma = rng.normal(size=discr.ndofs)
+    return kind, discr, src, geo, ref, sigma, targets
+
+
+def test_fmm3d_stages_match_oracle(case3):
+    from pytential_amd import fmm as F
+    kind, discr, src, geo, ref, sigma, _ = case3
+    tree = geo.tree()
+    print(f"[case3] nboxes={tree.nboxes} nlevels={tree.nlevels}", flush=True)
+    assert tree.nlevels >= 3
+    w = src.make_wrangler(geo, "laplace", 0.0, kind)
+    lo = list(w._lo)
+    pot_ref, st = ref.fmm("laplace", 0.0, kind, 3, lo, discr.strengths(sigma),
+                          return_stages=True)
+    qd = src.quad_density(sigma)
+    normals = src.dev["quad_normals"] if kind == "D" else None
+    strengths, nrm = w.reorder_sources(qd, src.dev["quad_weights"], normals)
+    mpole = w.coarsen_multipoles(w.form_multipoles(strengths, nrm))
+    torch.cuda.synchronize()
+    print("[case3] P2M+M2M done", flush=True)
+    assert _rel(w.expansions_to_dense(mpole), st["mpole"]) < REL_TOL, "P2M+M2M"
+    local = w.multipole_to_local(mpole)
+    torch.cuda.synchronize()
+    print("[case3] M2L done", flush=True)
+    w.form_locals(strengths, nrm, local)
+    w.refine_locals(local)
+    assert _rel(w.expansions_to_dense(local), st["local"]) < REL_TOL, "M2L+P2L+L2L"
+    qexp = w.form_global_qbx_locals(strengths, nrm)
+    torch.cuda.synchronize()
+    print("[case3] P2QBXL done", flush=True)
+    w.translate_box_multipoles_to_qbx_local(mpole, qexp)
+    torch.cuda.synchronize()
+    print("[case3] M2QBXL done", flush=True)
+    w.translate_box_local_to_qbx_local(local, qexp)
+    assert _rel(_np(qexp), st["qbx"]) < REL_TOL, "P2QBXL+M2QBXL+L2QBXL"
+    pot = F.drive_fmm(w, strengths, nrm)
+    assert _rel(_np(pot), pot_ref) < REL_TOL, "drive_fmm"
+    fused = F.drive_fmm_fused(w, strengths, nrm)
+    assert _rel(_np(fused), pot_ref) < REL_TOL, "drive_fmm_fused"
+
+
+def test_fmm3d_vs_direct_qbx(case3):
+    kind, discr, src, geo, ref, sigma, targets = case3
+    pot_s, pot_t = src.exec_compute_potential("laplace", sigma, list(targets), source_kind=kind)
+    got = np.concatenate([_np(pot_s), _np(pot_t)])
+    d = ref.direct("laplace", 0.0, kind, 3, discr.strengths(sigma)).real
+    err = _rel(got, d)
+    print(f"[case3 {kind}] FMM (p=8) vs direct QBX rel err = {err:.3e}", flush=True)
+    assert err < 1e-4
+
+
+def test_tsqbx_pipeline_equals_expansion_pipeline(case3):
+    from pytential_amd.qbx import QBXLayerPotentialSource
+    kind, discr, src, geo, ref, sigma, targets = case3
+    src_ts = QBXLayerPotentialSource(discr, fine_order=discr.fine_order, qbx_order=3,
+                                     fmm_order=8, _max_leaf_refine_weight=64,
+                                     _use_target_specific_qbx=True)
+    a = np.concatenate([_np(x) for x in src.exec_compute_potential(
+        "laplace", sigma, list(targets), source_kind=kind, fused=False)])
+    b = np.concatenate([_np(x) for x in src_ts.exec_compute_potential(
+        "laplace", sigma, list(targets), source_kind=kind)])
+    err = _rel(b, a)
+    print(f"[case3 {kind}] TSQBX vs P2QBXL pipeline rel diff = {err:.3e}", flush=True)
+    assert err < 1e-12
+
+
+def test_sphere_eigenvalue_gpu():
+    from scipy.special import sph_harm_y
+
+    from pytential_amd.qbx import QBXLayerPotentialSource
+    from pytential_amd.surface import icosphere
+    discr = icosphere(2, order=4, fine_order=6)
+    th = np.arccos(discr.nodes[2] / np.linalg.norm(discr.nodes, axis=0))
+    ph = np.arctan2(discr.nodes[1], discr.nodes[0])
+    sigma = sph_harm_y(3, 2, th, ph).real
+    src = QBXLayerPotentialSource(discr, fine_order=6, qbx_order=3, fmm_order=12)
+    pot, = src.exec_compute_potential("laplace", sigma, [("self", +1)])
+    err = _rel(_np(pot), sigma / 7)
+    print(f"[sphere] S Y_3^2 vs Y_3^2/7 rel err = {err:.3e}", flush=True)
+    assert err < 2e-3

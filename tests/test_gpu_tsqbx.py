@@ -102,7 +102,7 @@ def test_tsqbx_matches_oracle(sphere, op, k, qbx_order):
         _np(trav.neighbor_source_boxes_starts), _np(trav.neighbor_source_boxes_lists),
         _np(tree.box_source_starts), _np(tree.box_source_counts_nonchild),
         _np(strengths), _np(nrm) if nrm is not None else None)
-    got = _np(out)
+    got = _np(out) * (4 * np.pi)   # the wrangler returns G = 1/(4 pi r) units
     assert np.isfinite(got).all()
     assert np.abs(exp).max() > 0
     err = _rel(got, exp)
