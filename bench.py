@@ -232,6 +232,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--sample", type=int, default=1000)
+    ap.add_argument("--profile", action="store_true",
+                    help="one configuration only (tolerance orders), no CPU leg: "
+                         "the run profiled under rocprofv3")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy a 512 MiB device buffer 5 times (HBM counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the QBX-FMM "
@@ -240,6 +245,19 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        n = 512 << 20
+        a = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / 5
+        print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n,
+                          "GBps_read_plus_write": round(2 * n / dt / 1e9, 1)}), flush=True)
+        return
     from pytential_amd.qbx import fmm_level_to_order_for_tolerance
 
     t0 = time.perf_counter()
@@ -256,6 +274,10 @@ def main():
         return cfg.fmm_order
 
     main_run = run_gpu(args, cfg, tol_orders, "tol")
+    if args.profile:
+        log(f"profile run: {main_run['step_s'] * 1e3:.3f} ms/step, per stage "
+            f"{ {k: round(v, 4) for k, v in main_run['per_stage_ms'].items()} }")
+        return
     cfg_run = run_gpu(args, cfg, config_orders, f"p{cfg.fmm_order}")
 
     rng = np.random.default_rng(11)
@@ -283,7 +305,10 @@ def main():
     rl = None
     if dominant in flops:
         achieved = flops[dominant] / (ps[dominant] * 1e-3) / 1e12
-        rl = {"bound": "fp64", "kernel": dominant, "achieved": round(achieved, 3),
+        # compute-bound FP64 work; MI355X's FP64 vector and FP64 matrix peaks
+        # are the same 78.6 TFLOP/s, the figure this fraction is taken against
+        rl = {"bound": "mfma", "compute": "fp64", "kernel": dominant,
+              "achieved": round(achieved, 3),
               "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
               "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
               "flops_per_launch": int(flops[dominant]),

@@ -14,7 +14,13 @@
  *   expansions       box expansions level-compact (qbx_expansion_size);
  *                    QBX expansions [ncenters][ncoeff_qbx] complex, row-major;
  *                    Laplace 2D coefficient k at k (k = 0..p),
- *                    Helmholtz 2D coefficient n at n + p (|n| <= p)
+ *                    Helmholtz 2D coefficient n at n + p (|n| <= p),
+ *                    Laplace 3D coefficient (n, m) at n*n + n + m
+ *   right-hand sides several densities share one evaluation (nrhs in the
+ *                    kernel description); every per-source, per-coefficient
+ *                    and per-target array then carries nrhs values
+ *                    innermost: strengths [nsources][nrhs], expansions
+ *                    [...][ncoeff][nrhs], potentials [ntargets][nrhs]
  *
  * Each entry point names the reference interface it replaces.
  */
@@ -140,6 +146,7 @@ typedef struct {
     int qbx_order;          /* p_q */
     int nlevels;
     const int *level_orders;/* host array [nlevels]: fmm_level_to_order */
+    int nrhs;               /* right-hand sides per evaluation (0 is taken as 1) */
 } qbx_kernel_desc;
 
 /* Box expansions (multipoles and locals) are level-compact: the boxes of
@@ -155,20 +162,21 @@ int qbx_ncoeff_qbx(const qbx_kernel_desc *kd);
 
 /* Source data in tree order.  Replaces QBXExpansionWrangler.reorder_sources
  * (pytential/qbx/fmm.py:198-199) together with get_flat_strengths_from_densities
- * (pytential/qbx/__init__.py:1075-1096): strengths_tree[i] =
- * weights[u] * density[u], u = user_source_ids[i]; normals gathered likewise
- * (source.py:392-433).  density complex [nsources]; weights double [nsources];
- * normals [dim*nsources] or NULL. Outputs complex / SoA doubles. */
-int qbx_reorder_sources(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
+ * (pytential/qbx/__init__.py:1075-1096): strengths_tree[i][r] =
+ * weights[u] * density[u][r], u = user_source_ids[i]; normals gathered likewise
+ * (source.py:392-433).  density complex [nsources][nrhs]; weights double
+ * [nsources]; normals [dim*nsources] or NULL. Outputs complex / SoA doubles. */
+int qbx_reorder_sources(qbx_ctx *ctx, void *stream, const qbx_tree *tree, int nrhs,
                         const double *density, const double *weights,
                         const double *normals, double *strengths_tree,
                         double *normals_tree);
 
 /* Per-panel upsampling of stage-1 nodal values to quad-stage-2 nodes (the
  * meshmode connection behind InterpolationPreprocessor,
- * pytential/symbolic/mappers.py:743-826): out[e*nf + i] = sum_j
- * mat[i*n1 + j] * in[e*n1 + j], complex in/out, mat double [nf*n1]. */
-int qbx_upsample(qbx_ctx *ctx, void *stream, int nelements, int n1, int nf,
+ * pytential/symbolic/mappers.py:743-826): out[e*nf + i][r] = sum_j
+ * mat[i*n1 + j] * in[e*n1 + j][r], complex in/out with nrhs values innermost,
+ * mat double [nf*n1]. */
+int qbx_upsample(qbx_ctx *ctx, void *stream, int nelements, int n1, int nf, int nrhs,
                  const double *mat, const double *in, double *out);
 
 /* The stages below mirror the wrangler methods called by drive_fmm
@@ -229,13 +237,14 @@ int qbx_eval_qbx_expansions(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
                             const qbx_geo *geo, const qbx_kernel_desc *kd,
                             const double *qbx_exp, double *pot);
 /* Reorder (fmm.py:569-588): user-order output of the user targets
- * (tree targets >= ncenters): out[u - ncenters] = pot[tree index of u]. */
+ * (tree targets >= ncenters): out[u - ncenters][r] = pot[tree index of u][r]. */
 int qbx_finalize_potentials(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
-                            const qbx_geo *geo, const double *pot, double *out_user);
+                            const qbx_geo *geo, int nrhs, const double *pot,
+                            double *out_user);
 
 /* Whole drive_fmm (fmm.py:414-594) in one call: stages in reference order on
  * `stream`, workspaces owned by ctx.  strengths_tree/normals_tree from
- * qbx_reorder_sources; out_user [ntargets - ncenters] complex. */
+ * qbx_reorder_sources; out_user [ntargets - ncenters][nrhs] complex. */
 int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
                   const qbx_trav *trav, const qbx_geo *geo,
                   const qbx_kernel_desc *kd, const double *strengths_tree,

@@ -1022,7 +1022,7 @@ static cplx p2p_one(const orc_kparams *kp, cplx s, cplx q, cplx nu, cplx t)
     if (r == 0.0) return 0;
     if (kp->kernel == 0) {
         if (kp->source_kind == 0) return -q * log(r) / (2 * PI);
-        return creal(nu * q / w) / (2 * PI);
+        return creal(nu / w) * q / (2 * PI);
     } else {
         double kr = kp->k * r;
         if (kp->source_kind == 0) return 0.25 * JI * q * (j0(kr) + JI * y0(kr));

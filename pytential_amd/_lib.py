@@ -27,7 +27,7 @@ IP = ct.POINTER(ct.c_int)
 
 class KernelDesc(ct.Structure):
     _fields_ = [("kernel", I), ("k", D), ("source_kind", I), ("qbx_order", I),
-                ("nlevels", I), ("level_orders", IP)]
+                ("nlevels", I), ("level_orders", IP), ("nrhs", I)]
 
 
 class QBXError(RuntimeError):
@@ -62,8 +62,8 @@ _SIGS = {
     "qbx_expansion_size": [ct.POINTER(KernelDesc), P],
     "qbx_expansion_level_offsets": [ct.POINTER(KernelDesc), P, ct.POINTER(L)],
     "qbx_ncoeff_qbx": [ct.POINTER(KernelDesc)],
-    "qbx_reorder_sources": [P, P, P, P, P, P, P, P],
-    "qbx_upsample": [P, P, I, I, I, P, P, P],
+    "qbx_reorder_sources": [P, P, P, I, P, P, P, P, P],
+    "qbx_upsample": [P, P, I, I, I, I, P, P, P],
     "qbx_form_multipoles": [P, P, P, P, ct.POINTER(KernelDesc), P, P, P],
     "qbx_coarsen_multipoles": [P, P, P, P, ct.POINTER(KernelDesc), P],
     "qbx_eval_direct": [P, P, P, P, P, ct.POINTER(KernelDesc), P, P, P, I],
@@ -77,7 +77,7 @@ _SIGS = {
                                                   P, P, I],
     "qbx_translate_box_local_to_qbx_local": [P, P, P, P, P, ct.POINTER(KernelDesc), P, P, I],
     "qbx_eval_qbx_expansions": [P, P, P, P, ct.POINTER(KernelDesc), P, P],
-    "qbx_finalize_potentials": [P, P, P, P, P, P],
+    "qbx_finalize_potentials": [P, P, P, P, I, P, P],
     "qbx_drive_fmm": [P, P, P, P, P, ct.POINTER(KernelDesc), P, P, P],
     "qbx_eval_target_specific_qbx_locals": [P, P, P, P, P, I, I, D, D, I, P, P, P, P],
 }

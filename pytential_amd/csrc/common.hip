@@ -112,31 +112,37 @@ int compact_indices(qbx_ctx *ctx, hipStream_t st, const int *flags, int n, dbuf<
     return QBX_OK;
 }
 
-__global__ void k_reorder_sources(int ns, int dim, const int *user_source_ids,
+// one thread per (tree source, right-hand side); density [nsources][nrhs]
+__global__ void k_reorder_sources(int ns, int nr, int dim, const int *user_source_ids,
                                   const double *density, const double *weights,
                                   const double *normals, double *str, double *nrm)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ns) return;
-    int u = user_source_ids[i];
-    double w = weights ? weights[u] : 1.0;
-    str[2 * i] = w * density[2 * u];
-    str[2 * i + 1] = w * density[2 * u + 1];
-    if (normals && nrm)
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)ns * nr) return;
+    const int i = (int)(idx / nr), r = (int)(idx % nr);
+    const int u = user_source_ids[i];
+    const double w = weights ? weights[u] : 1.0;
+    str[2 * idx] = w * density[2 * ((size_t)u * nr + r)];
+    str[2 * idx + 1] = w * density[2 * ((size_t)u * nr + r) + 1];
+    if (r == 0 && normals && nrm)
         for (int d = 0; d < dim; ++d) nrm[(size_t)d * ns + i] = normals[(size_t)d * ns + u];
 }
 
-__global__ void k_upsample(int nel, int n1, int nf, const double *mat, const double *in,
+// one thread per (element, fine node, right-hand side); in [nel*n1][nrhs]
+__global__ void k_upsample(int nel, int n1, int nf, int nr, const double *mat, const double *in,
                            double *out)
 {
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= nel * nf) return;
-    int e = idx / nf, i = idx % nf;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)nel * nf * nr) return;
+    const long en = idx / nr;
+    const int r = (int)(idx % nr);
+    const int e = (int)(en / nf), i = (int)(en % nf);
     double re = 0, im = 0;
     for (int j = 0; j < n1; ++j) {
-        double m = mat[i * n1 + j];
-        re += m * in[2 * (e * n1 + j)];
-        im += m * in[2 * (e * n1 + j) + 1];
+        const double m = mat[i * n1 + j];
+        const size_t k = ((size_t)e * n1 + j) * nr + r;
+        re += m * in[2 * k];
+        im += m * in[2 * k + 1];
     }
     out[2 * idx] = re;
     out[2 * idx + 1] = im;
@@ -170,29 +176,30 @@ int qbx_ctx_destroy(qbx_ctx *ctx)
     return QBX_OK;
 }
 
-int qbx_reorder_sources(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
+int qbx_reorder_sources(qbx_ctx *ctx, void *stream, const qbx_tree *tree, int nrhs,
                         const double *density, const double *weights,
                         const double *normals, double *strengths_tree, double *normals_tree)
 {
     QBX_CHECK(ctx && tree && density && strengths_tree, "qbx_reorder_sources: null argument");
+    QBX_CHECK(nrhs >= 1, "qbx_reorder_sources: nrhs must be >= 1");
     int ns = tree->nsources;
     if (ns == 0) return QBX_OK;
-    hipLaunchKernelGGL(k_reorder_sources, dim3(cdiv(ns, 256)), dim3(256), 0, S(stream), ns,
-                       tree->dim, tree->user_source_ids.p, density, weights, normals,
-                       strengths_tree, normals_tree);
+    hipLaunchKernelGGL(k_reorder_sources, dim3(cdiv((long)ns * nrhs, 256)), dim3(256), 0,
+                       S(stream), ns, nrhs, tree->dim, tree->user_source_ids.p, density, weights,
+                       normals, strengths_tree, normals_tree);
     QBX_HIP(hipGetLastError());
     return QBX_OK;
 }
 
-int qbx_upsample(qbx_ctx *ctx, void *stream, int nelements, int n1, int nf, const double *mat,
-                 const double *in, double *out)
+int qbx_upsample(qbx_ctx *ctx, void *stream, int nelements, int n1, int nf, int nrhs,
+                 const double *mat, const double *in, double *out)
 {
     QBX_CHECK(ctx && mat && in && out, "qbx_upsample: null argument");
-    QBX_CHECK(nelements >= 0 && n1 > 0 && nf > 0, "qbx_upsample: bad sizes");
-    long n = (long)nelements * nf;
+    QBX_CHECK(nelements >= 0 && n1 > 0 && nf > 0 && nrhs >= 1, "qbx_upsample: bad sizes");
+    long n = (long)nelements * nf * nrhs;
     if (n == 0) return QBX_OK;
     hipLaunchKernelGGL(k_upsample, dim3(cdiv(n, 256)), dim3(256), 0, S(stream), nelements, n1,
-                       nf, mat, in, out);
+                       nf, nrhs, mat, in, out);
     QBX_HIP(hipGetLastError());
     return QBX_OK;
 }

@@ -4,15 +4,23 @@
 // Expansions:
 //  Laplace   G = -log|x-y|/(2 pi); complex potential Phi, phi = -Re(Phi)/(2 pi)
 //            multipole a0 log(z-c) + sum_k a_k (R/(z-c))^k, local sum_l b_l ((z-c)/R)^l
+//            (strengths must be real: the real part of Phi is taken)
 //  Helmholtz G = (i/4) H0(k|x-y|); multipole sum_n M_n H_n(k rho) e^{in theta},
 //            local sum_n L_n J_n(k rho) e^{in theta}; translations by Graf's
 //            addition theorem (M2M/L2L with J, M2L/M2QBXL with H).
 // The double layer is the source-normal derivative of G.
 //
+// Right-hand sides: nr densities share one evaluation.  Every per-source,
+// per-coefficient and per-target array carries its nr values innermost, and
+// every kernel computes the geometry of an interaction (distances, Bessel and
+// Hankel values, translation weights) once and applies it to a chunk of up to
+// NRC right-hand sides held in registers; the host launches one chunk
+// [r0, r0 + rc) at a time.
+//
 // Storage (HBM): box expansions are level-compact.  Level l holds its boxes
-// contiguously with p_l + 1 (Laplace) or 2 p_l + 1 (Helmholtz) complex
-// coefficients each; coefficient n of a Helmholtz box sits at n + p_l.  QBX
-// expansions are [ncenters][ncoeff(p_q)].
+// contiguously with p_l + 1 (Laplace) or 2 p_l + 1 (Helmholtz) coefficients
+// each (times nr); coefficient n of a Helmholtz box sits at n + p_l.  QBX
+// expansions are [ncenters][ncoeff(p_q)][nr].
 //
 // Thread mappings:
 //  P2M, P2L        one thread per (box, |n|)
@@ -20,10 +28,11 @@
 //                  translation vectors come from per-level tables (4 child
 //                  offsets for M2M/L2L, the offset classes of list 2 for
 //                  M2L), so each output is one dot product
-//  P2P, M2P, L2P   one block per target box, one thread per target
+//  P2P, M2P, L2P   one block per target box, lane groups per target
 //  P2QBXL          16 lanes per QBX center over its list-1 sources, shuffle
 //                  reduction, order-p_q expansion in registers
-//  M2QBXL, L2QBXL  one thread per QBX center, expansion in registers
+//  M2QBXL          4 lanes per QBX center over its list-3 boxes
+//  L2QBXL          one thread per QBX center
 //  QBXL2P          one thread per (center, target) pair
 #include "qbx_internal.h"
 #include "qbx_math.h"
@@ -35,11 +44,15 @@ namespace qbx {
 constexpr int MAXP = 120;
 constexpr int BINOM_N = 2 * MAXP + 4;
 constexpr int FML = 30;        // levels of a 2D tree (coordinates have 28 bits)
+constexpr int NRC_S = 8;       // RHS chunk of the scalar-accumulator kernels
+constexpr int NRC_Q = 4;       // RHS chunk of the QBX-expansion kernels
 
 struct KD {
     int kernel, dip, pq, nlevels, ncq;
     int nsep;                  // well_sep_is_n_away of the traversal
     int m2l_w;                 // list-2 offsets satisfy |dI| <= m2l_w = 2 nsep + 1
+    int nr;                    // right-hand sides (stride of the innermost axis)
+    int r0, rc;                // the chunk of right-hand sides of this launch
     double k;
     int order[FML];
     int ncl[FML];
@@ -89,11 +102,24 @@ __device__ __forceinline__ cplx ldpt(const double *x, int n, int i)
     return {x[i], x[(size_t)n + i]};
 }
 
-// pointer to coefficient 0 of box b (level lv)
+// coefficient 0 (Helmholtz: n = 0, centred), right-hand side 0 of box b;
+// coefficient n, right-hand side r at [2 (n nr + r)]
 __device__ __forceinline__ long ecoef_off(const KD &kd, int b, int lv)
 {
     const long off = kd.lcoef[lv] + (long)(b - kd.lbox[lv]) * kd.ncl[lv];
-    return 2 * (off + (kd.kernel == QBX_HELMHOLTZ2D ? kd.order[lv] : 0));
+    return 2 * (off + (kd.kernel == QBX_HELMHOLTZ2D ? kd.order[lv] : 0)) * kd.nr;
+}
+
+// strength of source s, right-hand side r
+__device__ __forceinline__ cplx ldq(const KD &kd, const double *str, int s, int r)
+{
+    return ldc(str + 2 * ((size_t)s * kd.nr + r));
+}
+
+// coefficient n, right-hand side r relative to an ecoef_off pointer
+__device__ __forceinline__ cplx ldcn(const KD &kd, const double *a, int n, int r)
+{
+    return ldc(a + 2 * ((long)n * kd.nr + r));
 }
 
 constexpr double INV2PI = 0.15915494309189533577;
@@ -110,153 +136,153 @@ __device__ __forceinline__ cplx cpow_int(cplx z, int m)
 }
 
 // -------------------------------------------------------------------------
-// pairwise kernel
+// pairwise kernel without the strength: potential = value * q
 // -------------------------------------------------------------------------
-__device__ __forceinline__ cplx p2p_pair(const KD &kd, cplx s, cplx q, cplx nu, cplx t)
+__device__ __forceinline__ cplx p2p_value(const KD &kd, cplx s, cplx nu, cplx t)
 {
     const cplx w = t - s;
     const double r2 = abs2(w);
     if (r2 == 0.0) return {0.0, 0.0};
     if (kd.kernel == QBX_LAPLACE2D) {
-        if (!kd.dip) return (-0.5 * log(r2) * INV2PI) * q;
-        const cplx v = (nu * q) * cinv(w);
-        return {v.x * INV2PI, 0.0};
+        if (!kd.dip) return {-0.5 * log(r2) * INV2PI, 0.0};
+        return {(nu * cinv(w)).x * INV2PI, 0.0};
     }
     const double r = sqrt(r2);
     const double kr = kd.k * r;
     cplx h0, h1;
     if (!kd.dip) {
         hankel01_fast<true, false>(kr, h0, h1);
-        const cplx v = q * h0;
-        return {-0.25 * v.y, 0.25 * v.x};
+        return {-0.25 * h0.y, 0.25 * h0.x};
     }
     hankel01_fast<false, true>(kr, h0, h1);
     const double proj = (w.x * nu.x + w.y * nu.y) / r;
-    const cplx v = (kd.k * proj) * (q * h1);
+    const cplx v = (kd.k * proj) * h1;
     return {-0.25 * v.y, 0.25 * v.x};
 }
 
 // -------------------------------------------------------------------------
-// P2M / P2L
+// P2M / P2L: the basis value(s) one source contributes to coefficient(s)
 // -------------------------------------------------------------------------
 // Helmholtz: coefficients n = m and n = -m (m >= 0) of the expansion about c
-// of sources [s0,s1).  g_j = Z_j(k rho) e^{-i j theta} of v = s - c, Z = H for
+// of the source at v + c.  g_j = Z_j(k rho) e^{-i j theta} of v, Z = H for
 // locals and J for multipoles, g_{-j} = (-1)^j Z_j e^{+i j theta}.
-__device__ void helm_p2e_pm(const KD &kd, int local, int m, cplx c, const double *src, int ns,
-                            int s0, int s1, const double *str, const double *nrm, cplx &cp,
-                            cplx &cm)
+__device__ void helm_p2e_basis(const KD &kd, int local, int m, cplx v, cplx nu, cplx &bp,
+                               cplx &bm)
 {
-    cp = {0, 0};
-    cm = {0, 0};
     const int top = m + 1;
     const double sm = (m & 1) ? -1.0 : 1.0;  // (-1)^m
-    for (int s = s0; s < s1; ++s) {
-        const cplx v = ldpt(src, ns, s) - c;
-        const double rho = sqrt(abs2(v));
-        const double x = kd.k * rho;
-        const cplx q = ldc(str + 2 * s);
-        const cplx e1 = rho > 0 ? cplx{v.x / rho, -v.y / rho} : cplx{1, 0};  // e^{-i theta}
-        // Z_{|m-1|}, Z_m, Z_{m+1}
-        cplx zlo, z0, zp1;
-        if (local) {
-            cplx a, b;
-            hankel01_fast<true, true>(x, a, b);
-            const cplx h1v = b;
-            const double tox = 2.0 / x;
-            cplx hs0 = {0, 0}, hs1 = {0, 0}, hs2 = {0, 0};
-            for (int j = 0; j <= top; ++j) {
-                if (j == m - 1) hs0 = a;
-                if (j == m) hs1 = a;
-                if (j == m + 1) hs2 = a;
-                const cplx nb = ((j + 1) * tox) * b - a;
-                a = b;
-                b = nb;
-            }
-            zlo = m >= 1 ? hs0 : h1v;
-            z0 = hs1;
-            zp1 = hs2;
-        } else {
-            double jlo = 0, j0v = 0, jp1 = 0;
-            if (x < 1e-300) {
-                j0v = m == 0 ? 1.0 : 0.0;
-                jlo = m == 1 ? 1.0 : 0.0;
-            } else {
-                const double norm = miller_j(
-                    x, top,
-                    [&](int j, double f) {
-                        if (j == m - 1 || (m == 0 && j == 1)) jlo = f;
-                        if (j == m) j0v = f;
-                        if (j == m + 1) jp1 = f;
-                    },
-                    [&](double sc) {
-                        jlo *= sc;
-                        j0v *= sc;
-                        jp1 *= sc;
-                    });
-                jlo *= norm;
-                j0v *= norm;
-                jp1 *= norm;
-            }
-            zlo = {jlo, 0};
-            z0 = {j0v, 0};
-            zp1 = {jp1, 0};
+    const double rho = sqrt(abs2(v));
+    const double x = kd.k * rho;
+    const cplx e1 = rho > 0 ? cplx{v.x / rho, -v.y / rho} : cplx{1, 0};  // e^{-i theta}
+    cplx zlo, z0, zp1;
+    if (local) {
+        cplx a, b;
+        hankel01_fast<true, true>(x, a, b);
+        const cplx h1v = b;
+        const double tox = 2.0 / x;
+        cplx hs0 = {0, 0}, hs1 = {0, 0}, hs2 = {0, 0};
+        for (int j = 0; j <= top; ++j) {
+            if (j == m - 1) hs0 = a;
+            if (j == m) hs1 = a;
+            if (j == m + 1) hs2 = a;
+            const cplx nb = ((j + 1) * tox) * b - a;
+            a = b;
+            b = nb;
         }
-        const cplx em = cpow_int(e1, m);  // e^{-i m theta}
-        const cplx emp1 = em * e1;        // e^{-i (m+1) theta}
-        const cplx g0 = z0 * em;
-        if (!kd.dip) {
-            cp += q * g0;
-            if (m > 0) cm += (sm * q) * (z0 * conj(em));
+        zlo = m >= 1 ? hs0 : h1v;
+        z0 = hs1;
+        zp1 = hs2;
+    } else {
+        double jlo = 0, j0v = 0, jp1 = 0;
+        if (x < 1e-300) {
+            j0v = m == 0 ? 1.0 : 0.0;
+            jlo = m == 1 ? 1.0 : 0.0;
         } else {
-            const cplx nu = ldpt(nrm, ns, s);
-            const cplx hq = (0.5 * kd.k) * q;
-            // g_{m-1}: for m = 0 it is g_{-1} = -Z_1 e^{+i theta}
-            const cplx gm1 = m >= 1 ? zlo * (em * conj(e1)) : -(zlo * conj(e1));
-            const cplx gp1 = zp1 * emp1;
-            cp += hq * (conj(nu) * gm1 - nu * gp1);
-            if (m > 0) {
-                // n = -m: g_{-m-1} = (-1)^{m+1} Z_{m+1} e^{+i(m+1)th},
-                //         g_{-m+1} = (-1)^{m-1} Z_{m-1} e^{+i(m-1)th}
-                const cplx gA = (-sm) * (zp1 * conj(emp1));
-                const cplx gB = (-sm) * (zlo * conj(em * conj(e1)));
-                cm += hq * (conj(nu) * gA - nu * gB);
-            }
+            const double norm = miller_j(
+                x, top,
+                [&](int j, double f) {
+                    if (j == m - 1 || (m == 0 && j == 1)) jlo = f;
+                    if (j == m) j0v = f;
+                    if (j == m + 1) jp1 = f;
+                },
+                [&](double sc) {
+                    jlo *= sc;
+                    j0v *= sc;
+                    jp1 *= sc;
+                });
+            jlo *= norm;
+            j0v *= norm;
+            jp1 *= norm;
         }
+        zlo = {jlo, 0};
+        z0 = {j0v, 0};
+        zp1 = {jp1, 0};
+    }
+    const cplx em = cpow_int(e1, m);  // e^{-i m theta}
+    const cplx emp1 = em * e1;        // e^{-i (m+1) theta}
+    if (!kd.dip) {
+        bp = z0 * em;
+        bm = sm * (z0 * conj(em));
+    } else {
+        const double hk = 0.5 * kd.k;
+        // g_{m-1}: for m = 0 it is g_{-1} = -Z_1 e^{+i theta}
+        const cplx gm1 = m >= 1 ? zlo * (em * conj(e1)) : -(zlo * conj(e1));
+        const cplx gp1 = zp1 * emp1;
+        bp = hk * (conj(nu) * gm1 - nu * gp1);
+        // n = -m: g_{-m-1} = (-1)^{m+1} Z_{m+1} e^{+i(m+1)th},
+        //         g_{-m+1} = (-1)^{m-1} Z_{m-1} e^{+i(m-1)th}
+        const cplx gA = (-sm) * (zp1 * conj(emp1));
+        const cplx gB = (-sm) * (zlo * conj(em * conj(e1)));
+        bm = hk * (conj(nu) * gA - nu * gB);
     }
 }
 
-__device__ cplx lap_p2e_coeff(const KD &kd, int local, int l, cplx c, double R,
-                              const double *src, int ns, int s0, int s1, const double *str,
-                              const double *nrm)
+// Laplace: coefficient l of one source at w + c (R the expansion's rscale)
+__device__ __forceinline__ cplx lap_p2e_basis(const KD &kd, int local, int l, cplx w, double R,
+                                              cplx nu)
 {
-    cplx acc = {0, 0};
+    if (local) {
+        const cplx pw = cpow_int(R * cinv(w), l);
+        if (!kd.dip) return l == 0 ? clogz(-w) : (-1.0 / l) * pw;
+        return (nu * cinv(w)) * pw;
+    }
+    const cplx t = (1.0 / R) * w;
+    if (!kd.dip) return l == 0 ? cplx{1, 0} : (-1.0 / l) * cpow_int(t, l);
+    if (l == 0) return {0, 0};
+    return -((1.0 / R) * nu) * cpow_int(t, l - 1);
+}
+
+// contributions of sources [s0, s1) to coefficient j (and -j for Helmholtz)
+template <int NRC>
+__device__ __forceinline__ void p2e_sources(const KD &kd, int local, int j, cplx c, double R,
+                                            const double *src, int ns, int s0, int s1,
+                                            const double *str, const double *nrm, cplx *ap,
+                                            cplx *am)
+{
     for (int s = s0; s < s1; ++s) {
         const cplx w = ldpt(src, ns, s) - c;
-        const cplx q = ldc(str + 2 * s);
-        if (local) {
-            const cplx pw = cpow_int(R * cinv(w), l);
-            if (!kd.dip) {
-                if (l == 0) acc += q * clogz(-w);
-                else acc += (-1.0 / l) * (q * pw);
-            } else {
-                const cplx nu = ldpt(nrm, ns, s);
-                acc += ((nu * q) * cinv(w)) * pw;
-            }
+        const cplx nu = kd.dip ? ldpt(nrm, ns, s) : cplx{0, 0};
+        if (kd.kernel == QBX_LAPLACE2D) {
+            const cplx b = lap_p2e_basis(kd, local, j, w, R, nu);
+#pragma unroll
+            for (int q = 0; q < NRC; ++q)
+                if (q < kd.rc) ap[q] += ldq(kd, str, s, kd.r0 + q) * b;
         } else {
-            const cplx t = (1.0 / R) * w;
-            if (!kd.dip) {
-                if (l == 0) acc += q;
-                else acc += (-1.0 / l) * (q * cpow_int(t, l));
-            } else if (l >= 1) {
-                const cplx nu = ldpt(nrm, ns, s);
-                acc += -((1.0 / R) * (nu * q)) * cpow_int(t, l - 1);
-            }
+            cplx bp, bm;
+            helm_p2e_basis(kd, local, j, w, nu, bp, bm);
+#pragma unroll
+            for (int q = 0; q < NRC; ++q)
+                if (q < kd.rc) {
+                    const cplx qs = ldq(kd, str, s, kd.r0 + q);
+                    ap[q] += qs * bp;
+                    am[q] += qs * bm;
+                }
         }
     }
-    return acc;
 }
 
 // P2M over source leaf boxes (flat over levels, p+1 threads per box)
+template <int NRC>
 __global__ void k_p2m(KD kd, TD t, Flat fl, const int *boxes, const double *src, int ns,
                       const double *str, const double *nrm, double *mpole)
 {
@@ -264,19 +290,23 @@ __global__ void k_p2m(KD kd, TD t, Flat fl, const int *boxes, const double *src,
     if (!flat_locate(fl, blockIdx.x * (long)blockDim.x + threadIdx.x, lv, i, j)) return;
     const int b = boxes[i];
     const int s0 = t.sstart[b], s1 = s0 + t.scount[b];
-    const cplx c = boxc(t, b);
+    cplx ap[NRC], am[NRC];
+#pragma unroll
+    for (int q = 0; q < NRC; ++q) ap[q] = am[q] = {0, 0};
+    p2e_sources<NRC>(kd, 0, j, boxc(t, b), kd.rscale[lv], src, ns, s0, s1, str, nrm, ap, am);
     double *a = mpole + ecoef_off(kd, b, lv);
-    if (kd.kernel == QBX_LAPLACE2D) {
-        stc(a + 2 * j, lap_p2e_coeff(kd, 0, j, c, kd.rscale[lv], src, ns, s0, s1, str, nrm));
-    } else {
-        cplx cp, cm;
-        helm_p2e_pm(kd, 0, j, c, src, ns, s0, s1, str, nrm, cp, cm);
-        stc(a + 2 * j, cp);
-        if (j > 0) stc(a - 2 * j, cm);
-    }
+    const bool helm = kd.kernel == QBX_HELMHOLTZ2D;
+#pragma unroll
+    for (int q = 0; q < NRC; ++q)
+        if (q < kd.rc) {
+            const int r = kd.r0 + q;
+            stc(a + 2 * ((long)j * kd.nr + r), ap[q]);
+            if (helm && j > 0) stc(a + 2 * ((long)-j * kd.nr + r), am[q]);
+        }
 }
 
 // P2L from list 4 (flat over ttp levels, p+1 threads per box), accumulates
+template <int NRC>
 __global__ void k_p2l(KD kd, TD t, Flat fl, const int *ttp, const int *starts, const int *lists,
                       const double *src, int ns, const double *str, const double *nrm,
                       double *local)
@@ -286,22 +316,27 @@ __global__ void k_p2l(KD kd, TD t, Flat fl, const int *ttp, const int *starts, c
     if (starts[i] == starts[i + 1]) return;
     const int P = ttp[i];
     const cplx c = boxc(t, P);
-    cplx acc = {0, 0}, accm = {0, 0};
+    cplx ap[NRC], am[NRC];
+#pragma unroll
+    for (int q = 0; q < NRC; ++q) ap[q] = am[q] = {0, 0};
     for (int e = starts[i]; e < starts[i + 1]; ++e) {
         const int Sb = lists[e];
         const int s0 = t.sstart[Sb], s1 = s0 + t.scount[Sb];
-        if (kd.kernel == QBX_LAPLACE2D) {
-            acc += lap_p2e_coeff(kd, 1, j, c, kd.rscale[lv], src, ns, s0, s1, str, nrm);
-        } else {
-            cplx cp, cm;
-            helm_p2e_pm(kd, 1, j, c, src, ns, s0, s1, str, nrm, cp, cm);
-            acc += cp;
-            accm += cm;
-        }
+        p2e_sources<NRC>(kd, 1, j, c, kd.rscale[lv], src, ns, s0, s1, str, nrm, ap, am);
     }
     double *a = local + ecoef_off(kd, P, lv);
-    stc(a + 2 * j, ldc(a + 2 * j) + acc);
-    if (kd.kernel != QBX_LAPLACE2D && j > 0) stc(a - 2 * j, ldc(a - 2 * j) + accm);
+    const bool helm = kd.kernel == QBX_HELMHOLTZ2D;
+#pragma unroll
+    for (int q = 0; q < NRC; ++q)
+        if (q < kd.rc) {
+            const int r = kd.r0 + q;
+            double *o = a + 2 * ((long)j * kd.nr + r);
+            stc(o, ldc(o) + ap[q]);
+            if (helm && j > 0) {
+                double *o2 = a + 2 * ((long)-j * kd.nr + r);
+                stc(o2, ldc(o2) + am[q]);
+            }
+        }
 }
 
 // -------------------------------------------------------------------------
@@ -377,7 +412,7 @@ __global__ void k_tab_mm(KD kd, int nl, double *tab)
 }
 
 // -------------------------------------------------------------------------
-// Laplace translations, one output coefficient
+// Laplace translations, one output coefficient, a chunk of right-hand sides
 // -------------------------------------------------------------------------
 __device__ __forceinline__ double g_binom(const double *B, int n, int k)
 {
@@ -386,64 +421,134 @@ __device__ __forceinline__ double g_binom(const double *B, int n, int k)
 
 // M2M (c1 -> c2, z0 = c1 - c2), output l:
 // -a0 u^l / l + sum_{k=1}^{min(l,p1)} a_k r^k u^{l-k} C(l-1,k-1), u = z0/R2, r = R1/R2
-__device__ cplx lap_m2m(const double *B, const double *a, int p1, int l, cplx z0, double R1,
-                        double R2)
+template <int NRC>
+__device__ void lap_m2m(const KD &kd, const double *B, const double *a, int p1, int l, cplx z0,
+                        double R1, double R2, cplx *acc)
 {
-    if (l == 0) return ldc(a);
+    if (l == 0) {
+#pragma unroll
+        for (int q = 0; q < NRC; ++q)
+            if (q < kd.rc) acc[q] += ldcn(kd, a, 0, kd.r0 + q);
+        return;
+    }
     const cplx u = (1.0 / R2) * z0;
     const double r = R1 / R2;
-    const cplx acc = (-1.0 / l) * (ldc(a) * cpow_int(u, l));
+    const cplx ul = cpow_int(u, l);
     const int kmax = l < p1 ? l : p1;
-    cplx hsum = {0, 0};
+    cplx hsum[NRC];
+#pragma unroll
+    for (int q = 0; q < NRC; ++q) hsum[q] = {0, 0};
     double rk = 1.0;
     for (int k = 1; k <= kmax; ++k) {
         rk *= r;
-        hsum = hsum * u + (rk * g_binom(B, l - 1, k - 1)) * ldc(a + 2 * k);
+        const double wk = rk * g_binom(B, l - 1, k - 1);
+#pragma unroll
+        for (int q = 0; q < NRC; ++q)
+            if (q < kd.rc) hsum[q] = hsum[q] * u + wk * ldcn(kd, a, k, kd.r0 + q);
     }
-    return acc + hsum * cpow_int(u, l - kmax);
+    const cplx ut = cpow_int(u, l - kmax);
+#pragma unroll
+    for (int q = 0; q < NRC; ++q)
+        if (q < kd.rc)
+            acc[q] += (-1.0 / l) * (ldcn(kd, a, 0, kd.r0 + q) * ul) + hsum[q] * ut;
 }
 
 // M2L (c1 multipole -> c2 local, z0 = c1 - c2), output l
-__device__ cplx lap_m2l(const double *B, const double *a, int p1, int l, cplx z0, double R1,
-                        double R2)
+template <int NRC>
+__device__ void lap_m2l(const KD &kd, const double *B, const double *a, int p1, int l, cplx z0,
+                        double R1, double R2, cplx *acc)
 {
     const cplx iz = cinv(z0);
     const cplx ms = -(R1 * iz), t = R2 * iz;
+    cplx s[NRC];
     if (l == 0) {
-        cplx acc = ldc(a) * clogz(-z0);
+        const cplx lg = clogz(-z0);
+#pragma unroll
+        for (int q = 0; q < NRC; ++q)
+            s[q] = q < kd.rc ? ldcn(kd, a, 0, kd.r0 + q) * lg : cplx{0, 0};
         cplx pw = {1, 0};
         for (int k = 1; k <= p1; ++k) {
             pw = pw * ms;
-            acc += ldc(a + 2 * k) * pw;
+#pragma unroll
+            for (int q = 0; q < NRC; ++q)
+                if (q < kd.rc) s[q] += ldcn(kd, a, k, kd.r0 + q) * pw;
         }
-        return acc;
+#pragma unroll
+        for (int q = 0; q < NRC; ++q)
+            if (q < kd.rc) acc[q] += s[q];
+        return;
     }
-    cplx acc = (-1.0 / l) * ldc(a);
+#pragma unroll
+    for (int q = 0; q < NRC; ++q)
+        s[q] = q < kd.rc ? (-1.0 / l) * ldcn(kd, a, 0, kd.r0 + q) : cplx{0, 0};
     cplx pw = {1, 0};
     for (int k = 1; k <= p1; ++k) {
         pw = pw * ms;
-        acc += g_binom(B, l + k - 1, k - 1) * (ldc(a + 2 * k) * pw);
+        const cplx wk = g_binom(B, l + k - 1, k - 1) * pw;
+#pragma unroll
+        for (int q = 0; q < NRC; ++q)
+            if (q < kd.rc) s[q] += ldcn(kd, a, k, kd.r0 + q) * wk;
     }
-    return acc * cpow_int(t, l);
+    const cplx tl = cpow_int(t, l);
+#pragma unroll
+    for (int q = 0; q < NRC; ++q)
+        if (q < kd.rc) acc[q] += s[q] * tl;
 }
 
 // L2L (c1 -> c2, z0 = c2 - c1), output j
-__device__ cplx lap_l2l(const double *B, const double *a, int p1, int j, cplx z0, double R1,
-                        double R2)
+template <int NRC>
+__device__ void lap_l2l(const KD &kd, const double *B, const double *a, int p1, int j, cplx z0,
+                        double R1, double R2, cplx *acc)
 {
-    if (j > p1) return {0, 0};
+    if (j > p1) return;
     const cplx u = (1.0 / R1) * z0;
-    cplx acc = {0, 0};
-    for (int l = p1; l >= j; --l) acc = acc * u + g_binom(B, l, j) * ldc(a + 2 * l);
+    cplx s[NRC];
+#pragma unroll
+    for (int q = 0; q < NRC; ++q) s[q] = {0, 0};
+    for (int l = p1; l >= j; --l) {
+        const double bl = g_binom(B, l, j);
+#pragma unroll
+        for (int q = 0; q < NRC; ++q)
+            if (q < kd.rc) s[q] = s[q] * u + bl * ldcn(kd, a, l, kd.r0 + q);
+    }
     double rp = 1.0;
     const double r21 = R2 / R1;
     for (int i = 0; i < j; ++i) rp *= r21;
-    return rp * acc;
+#pragma unroll
+    for (int q = 0; q < NRC; ++q)
+        if (q < kd.rc) acc[q] += rp * s[q];
+}
+
+// Helmholtz table translation: acc[q] += sum_n a[n] row[n - m] ((-1)^{n-m} if flip)
+template <int NRC>
+__device__ __forceinline__ void helm_row_dot(const KD &kd, const double *a, int p1, int m,
+                                             const double *row, bool flip, cplx *acc)
+{
+    for (int n = -p1; n <= p1; ++n) {
+        cplx w = ldc(row + 2 * (n - m));
+        if (flip && ((n - m) & 1)) w = -w;
+#pragma unroll
+        for (int q = 0; q < NRC; ++q)
+            if (q < kd.rc) acc[q] += ldcn(kd, a, n, kd.r0 + q) * w;
+    }
+}
+
+// o: coefficient (or target) pointer for right-hand side 0
+template <int NRC>
+__device__ __forceinline__ void store_add(const KD &kd, double *o, const cplx *acc)
+{
+#pragma unroll
+    for (int q = 0; q < NRC; ++q)
+        if (q < kd.rc) {
+            double *p = o + 2 * (kd.r0 + q);
+            stc(p, ldc(p) + acc[q]);
+        }
 }
 
 // -------------------------------------------------------------------------
 // M2M (parents at one level), M2L (list 2), L2L (one level)
 // -------------------------------------------------------------------------
+template <int NRC>
 __global__ void k_m2m(KD kd, TD t, const double *B, int lvp, int i0, int i1, const int *spb,
                       double *mpole)
 {
@@ -457,25 +562,25 @@ __global__ void k_m2m(KD kd, TD t, const double *B, int lvp, int i0, int i1, con
     const bool helm = kd.kernel == QBX_HELMHOLTZ2D;
     const int m = helm ? j - p2 : j;
     const cplx cp = boxc(t, P);
-    cplx acc = {0, 0};
+    cplx acc[NRC];
+#pragma unroll
+    for (int q = 0; q < NRC; ++q) acc[q] = {0, 0};
     for (int c = 0; c < 4; ++c) {
         const int C = t.child[(size_t)c * t.nboxes + P];
         if (C < 0 || t.scumul[C] == 0) continue;
         const double *a = mpole + ecoef_off(kd, C, lvc);
         if (!helm) {
-            acc += lap_m2m(B, a, p1, m, boxc(t, C) - cp, kd.rscale[lvc], kd.rscale[lvp]);
+            lap_m2m<NRC>(kd, B, a, p1, m, boxc(t, C) - cp, kd.rscale[lvc], kd.rscale[lvp], acc);
         } else {
             const int L = p1 + p2;
             const double *row = kd.tab_mm + 2 * (kd.mm_off[lvc] + (long)c * (2 * L + 1) + L);
-            cplx s = {0, 0};
-            for (int n = -p1; n <= p1; ++n) s += ldc(a + 2 * n) * ldc(row + 2 * (n - m));
-            acc += s;
+            helm_row_dot<NRC>(kd, a, p1, m, row, false, acc);
         }
     }
-    double *o = mpole + ecoef_off(kd, P, lvp) + 2 * m;
-    stc(o, ldc(o) + acc);
+    store_add<NRC>(kd, mpole + ecoef_off(kd, P, lvp) + 2L * m * kd.nr, acc);
 }
 
+template <int NRC>
 __global__ void k_m2l(KD kd, TD t, const double *B, Flat fl, const int *ttp, const int *starts,
                       const int *lists, const double *mpole, double *local)
 {
@@ -489,25 +594,25 @@ __global__ void k_m2l(KD kd, TD t, const double *B, Flat fl, const int *ttp, con
     const cplx cp = boxc(t, P);
     const int Ix = t.coords[P], Iy = t.coords[(size_t)t.nboxes + P];
     const int L = 2 * p;
-    cplx acc = {0, 0};
+    cplx acc[NRC];
+#pragma unroll
+    for (int q = 0; q < NRC; ++q) acc[q] = {0, 0};
     for (int e = starts[i]; e < starts[i + 1]; ++e) {
         const int A = lists[e];
         const double *a = mpole + ecoef_off(kd, A, lv);
         if (!helm) {
-            acc += lap_m2l(B, a, p, m, boxc(t, A) - cp, kd.rscale[lv], kd.rscale[lv]);
+            lap_m2l<NRC>(kd, B, a, p, m, boxc(t, A) - cp, kd.rscale[lv], kd.rscale[lv], acc);
         } else {
             const int dx = Ix - t.coords[A], dy = Iy - t.coords[(size_t)t.nboxes + A];
             const int cls = (dx + kd.m2l_w) * (2 * kd.m2l_w + 1) + (dy + kd.m2l_w);
             const double *row = kd.tab_m2l + 2 * (kd.m2l_off[lv] + (long)cls * (2 * L + 1) + L);
-            cplx s = {0, 0};
-            for (int n = -p; n <= p; ++n) s += ldc(a + 2 * n) * ldc(row + 2 * (n - m));
-            acc += s;
+            helm_row_dot<NRC>(kd, a, p, m, row, false, acc);
         }
     }
-    double *o = local + ecoef_off(kd, P, lv) + 2 * m;
-    stc(o, ldc(o) + acc);
+    store_add<NRC>(kd, local + ecoef_off(kd, P, lv) + 2L * m * kd.nr, acc);
 }
 
+template <int NRC>
 __global__ void k_l2l(KD kd, TD t, const double *B, int lv, int i0, int i1, const int *ttp,
                       double *local)
 {
@@ -522,72 +627,109 @@ __global__ void k_l2l(KD kd, TD t, const double *B, int lv, int i0, int i1, cons
     const bool helm = kd.kernel == QBX_HELMHOLTZ2D;
     const int m = helm ? j - p2 : j;
     const double *a = local + ecoef_off(kd, Q, lv - 1);
-    cplx v;
+    cplx acc[NRC];
+#pragma unroll
+    for (int q = 0; q < NRC; ++q) acc[q] = {0, 0};
     if (!helm) {
-        v = lap_l2l(B, a, p1, m, boxc(t, P) - boxc(t, Q), kd.rscale[lv - 1], kd.rscale[lv]);
+        lap_l2l<NRC>(kd, B, a, p1, m, boxc(t, P) - boxc(t, Q), kd.rscale[lv - 1], kd.rscale[lv],
+                     acc);
     } else {
         // uhat_j(c_P - c_Q) = (-1)^j uhat_j(c_Q - c_P): the M2M row with sign
         const int c = (t.coords[P] & 1) | ((t.coords[(size_t)t.nboxes + P] & 1) << 1);
         const int L = p1 + p2;
         const double *row = kd.tab_mm + 2 * (kd.mm_off[lv] + (long)c * (2 * L + 1) + L);
-        v = {0, 0};
-        for (int n = -p1; n <= p1; ++n) {
-            const cplx term = ldc(a + 2 * n) * ldc(row + 2 * (n - m));
-            v += ((n - m) & 1) ? -term : term;
-        }
+        helm_row_dot<NRC>(kd, a, p1, m, row, true, acc);
     }
-    double *o = local + ecoef_off(kd, P, lv) + 2 * m;
-    stc(o, ldc(o) + v);
+    store_add<NRC>(kd, local + ecoef_off(kd, P, lv) + 2L * m * kd.nr, acc);
 }
 
 // -------------------------------------------------------------------------
-// evaluation at points; a: coefficient 0 (Helmholtz n at a[2n])
+// evaluation at points of a chunk of expansions; a: ecoef_off-style pointer
+// (coefficient n, right-hand side r at a[2 (n nr + r)]); out[q] receives the
+// potential of right-hand side r0 + q
 // -------------------------------------------------------------------------
-__device__ cplx eval_local(const KD &kd, const double *a, int p, cplx c, double R, cplx tp)
+template <int NRC>
+__device__ void eval_local_n(const KD &kd, const double *a, int p, cplx c, double R, cplx tp,
+                             cplx *out)
 {
+    const int r0 = kd.r0, rc = kd.rc;
     if (kd.kernel == QBX_LAPLACE2D) {
         const cplx x = (1.0 / R) * (tp - c);
-        cplx phi = {0, 0};
-        for (int l = p; l >= 0; --l) phi = phi * x + ldc(a + 2 * l);
-        return {-phi.x * INV2PI, 0.0};
+        cplx phi[NRC];
+#pragma unroll
+        for (int q = 0; q < NRC; ++q) phi[q] = {0, 0};
+        for (int l = p; l >= 0; --l)
+#pragma unroll
+            for (int q = 0; q < NRC; ++q)
+                if (q < rc) phi[q] = phi[q] * x + ldcn(kd, a, l, r0 + q);
+#pragma unroll
+        for (int q = 0; q < NRC; ++q) out[q] = {-phi[q].x * INV2PI, 0.0};
+        return;
     }
     const cplx d = tp - c;
     const double rho = sqrt(abs2(d));
     const double x = kd.k * rho;
-    cplx phi;
+    cplx acc[NRC];
+#pragma unroll
+    for (int q = 0; q < NRC; ++q) acc[q] = {0, 0};
+    double norm = 1.0;
     if (x < 1e-300) {
-        phi = ldc(a);
+#pragma unroll
+        for (int q = 0; q < NRC; ++q)
+            if (q < rc) acc[q] = ldcn(kd, a, 0, r0 + q);
     } else {
         const cplx e1 = {d.x / rho, d.y / rho};
         const cplx em1 = conj(e1);
         cplx e = cpow_int(e1, p);
-        cplx acc = {0, 0};
-        const double norm = miller_j(
+        norm = miller_j(
             x, p,
             [&](int n, double f) {
-                cplx v = ldc(a + 2 * n) * e;
-                if (n > 0) {
-                    const cplx w = ldc(a - 2 * n) * conj(e);
-                    v += (n & 1) ? -w : w;
-                }
-                acc += f * v;
+                const cplx fe = f * e;
+                cplx fc = f * conj(e);
+                if (n & 1) fc = -fc;
+#pragma unroll
+                for (int q = 0; q < NRC; ++q)
+                    if (q < rc) {
+                        acc[q] += ldcn(kd, a, n, r0 + q) * fe;
+                        if (n > 0) acc[q] += ldcn(kd, a, -n, r0 + q) * fc;
+                    }
                 e = e * em1;
             },
-            [&](double s) { acc = s * acc; });
-        phi = norm * acc;
+            [&](double s) {
+#pragma unroll
+                for (int q = 0; q < NRC; ++q) acc[q] = s * acc[q];
+            });
     }
-    return {-0.25 * phi.y, 0.25 * phi.x};
+#pragma unroll
+    for (int q = 0; q < NRC; ++q) {
+        const cplx phi = norm * acc[q];
+        out[q] = {-0.25 * phi.y, 0.25 * phi.x};
+    }
 }
 
-__device__ cplx eval_mpole(const KD &kd, const double *a, int p, cplx c, double R, cplx tp)
+template <int NRC>
+__device__ void eval_mpole_add(const KD &kd, const double *a, int p, cplx c, double R, cplx tp,
+                               cplx *out)
 {
+    const int r0 = kd.r0, rc = kd.rc;
     const cplx w = tp - c;
     if (kd.kernel == QBX_LAPLACE2D) {
         const cplx x = R * cinv(w);
-        cplx phi = {0, 0};
-        for (int k = p; k >= 1; --k) phi = (phi + ldc(a + 2 * k)) * x;
-        phi += ldc(a) * clogz(w);
-        return {-phi.x * INV2PI, 0.0};
+        const cplx lw = clogz(w);
+        cplx phi[NRC];
+#pragma unroll
+        for (int q = 0; q < NRC; ++q) phi[q] = {0, 0};
+        for (int k = p; k >= 1; --k)
+#pragma unroll
+            for (int q = 0; q < NRC; ++q)
+                if (q < rc) phi[q] = (phi[q] + ldcn(kd, a, k, r0 + q)) * x;
+#pragma unroll
+        for (int q = 0; q < NRC; ++q)
+            if (q < rc) {
+                const cplx v = phi[q] + ldcn(kd, a, 0, r0 + q) * lw;
+                out[q] += cplx{-v.x * INV2PI, 0.0};
+            }
+        return;
     }
     const double rho = sqrt(abs2(w));
     const double x = kd.k * rho;
@@ -596,7 +738,9 @@ __device__ cplx eval_mpole(const KD &kd, const double *a, int p, cplx c, double 
     cplx hm1, hm;
     hankel01_fast<true, true>(x, hm1, hm);
     const double tox = 2.0 / x;
-    cplx acc = ldc(a) * hm1;
+    cplx acc[NRC];
+#pragma unroll
+    for (int q = 0; q < NRC; ++q) acc[q] = q < rc ? ldcn(kd, a, 0, r0 + q) * hm1 : cplx{0, 0};
     for (int n = 1; n <= p; ++n) {
         cplx h;
         if (n == 1) {
@@ -607,12 +751,15 @@ __device__ cplx eval_mpole(const KD &kd, const double *a, int p, cplx c, double 
             hm = h;
         }
         e = e * e1;
-        cplx v = ldc(a + 2 * n) * e;
-        const cplx wv = ldc(a - 2 * n) * conj(e);
-        v += (n & 1) ? -wv : wv;
-        acc += h * v;
+        const cplx he = h * e;
+        cplx hc = h * conj(e);
+        if (n & 1) hc = -hc;
+#pragma unroll
+        for (int q = 0; q < NRC; ++q)
+            if (q < rc) acc[q] += ldcn(kd, a, n, r0 + q) * he + ldcn(kd, a, -n, r0 + q) * hc;
     }
-    return {-0.25 * acc.y, 0.25 * acc.x};
+#pragma unroll
+    for (int q = 0; q < NRC; ++q) out[q] += cplx{-0.25 * acc[q].y, 0.25 * acc[q].x};
 }
 
 // One 64-lane block per target box.  P2P and M2P give each of the box's
@@ -620,7 +767,7 @@ __device__ cplx eval_mpole(const KD &kd, const double *a, int p, cplx c, double 
 // that splits the list-1 sources / list-3 boxes and is reduced by shuffles,
 // so boxes with few targets still keep the wavefront busy.
 // MODE 0: P2P (list 1), 1: M2P (list 3), 2: L2P (one lane per target)
-template <int MODE>
+template <int MODE, int NRC>
 __global__ void __launch_bounds__(64) k_eval_targets(KD kd, TD t, int ntb, const int *tboxes,
                                                      const int *starts, const int *lists,
                                                      const int *fstart, const int *fcount,
@@ -639,10 +786,10 @@ __global__ void __launch_bounds__(64) k_eval_targets(KD kd, TD t, int ntb, const
         const int lv = t.level[B];
         for (int f = f0 + lane; f < f0 + nf; f += 64) {
             const int ti = unfilt[f];
-            const cplx v = eval_local(kd, expn + ecoef_off(kd, B, lv), kd.order[lv], boxc(t, B),
-                                      kd.rscale[lv], ldpt(tgt, nt, ti));
-            double *o = pot + 2 * (size_t)ti;
-            stc(o, ldc(o) + v);
+            cplx v[NRC];
+            eval_local_n<NRC>(kd, expn + ecoef_off(kd, B, lv), kd.order[lv], boxc(t, B),
+                              kd.rscale[lv], ldpt(tgt, nt, ti), v);
+            store_add<NRC>(kd, pot + 2 * (size_t)ti * kd.nr, v);
         }
         return;
     }
@@ -654,7 +801,9 @@ __global__ void __launch_bounds__(64) k_eval_targets(KD kd, TD t, int ntb, const
     for (int base = 0; base < nf; base += tpr) {
         const bool active = base + tsub < nf;
         int ti = 0;
-        cplx acc = {0, 0};
+        cplx acc[NRC];
+#pragma unroll
+        for (int q = 0; q < NRC; ++q) acc[q] = {0, 0};
         if (active) {
             ti = unfilt[f0 + base + tsub];
             const cplx tp = ldpt(tgt, nt, ti);
@@ -664,49 +813,66 @@ __global__ void __launch_bounds__(64) k_eval_targets(KD kd, TD t, int ntb, const
                     const int s0 = t.sstart[Sb], s1 = s0 + t.scount[Sb];
                     for (int s = s0 + sub; s < s1; s += G) {
                         const cplx nu = kd.dip ? ldpt(nrm, ns, s) : cplx{0, 0};
-                        acc += p2p_pair(kd, ldpt(src, ns, s), ldc(str + 2 * s), nu, tp);
+                        const cplx g = p2p_value(kd, ldpt(src, ns, s), nu, tp);
+#pragma unroll
+                        for (int q = 0; q < NRC; ++q)
+                            if (q < kd.rc) acc[q] += ldq(kd, str, s, kd.r0 + q) * g;
                     }
                 }
             } else {
                 for (int e = e0 + sub; e < e1; e += G) {
                     const int D = lists[e];
                     const int lv = t.level[D];
-                    acc += eval_mpole(kd, expn + ecoef_off(kd, D, lv), kd.order[lv], boxc(t, D),
-                                      kd.rscale[lv], tp);
+                    eval_mpole_add<NRC>(kd, expn + ecoef_off(kd, D, lv), kd.order[lv],
+                                        boxc(t, D), kd.rscale[lv], tp, acc);
                 }
             }
         }
-        for (int m = G / 2; m >= 1; m >>= 1) {
-            acc.x += __shfl_xor(acc.x, m);
-            acc.y += __shfl_xor(acc.y, m);
-        }
-        if (active && sub == 0) {
-            double *o = pot + 2 * (size_t)ti;
-            stc(o, ldc(o) + acc);
-        }
+#pragma unroll
+        for (int q = 0; q < NRC; ++q)
+            for (int m = G / 2; m >= 1; m >>= 1) {
+                acc[q].x += __shfl_xor(acc[q].x, m);
+                acc[q].y += __shfl_xor(acc[q].y, m);
+            }
+        if (active && sub == 0) store_add<NRC>(kd, pot + 2 * (size_t)ti * kd.nr, acc);
     }
 }
 
 // -------------------------------------------------------------------------
 // QBX stages
 // -------------------------------------------------------------------------
-template <int PQ>
+template <int PQ, int NRC>
 struct QAcc {
-    cplx v[2 * PQ + 1];
+    cplx v[NRC][2 * PQ + 1];
 };
 
-// A holds Laplace coefficients l at v[l], Helmholtz n at v[n + PQ]
-template <int PQ>
-__device__ void qbx_store(const KD &kd, double *qexp, int c, const QAcc<PQ> &A, int accumulate)
+template <int PQ, int NRC>
+__device__ __forceinline__ void qacc_zero(QAcc<PQ, NRC> &A)
+{
+#pragma unroll
+    for (int q = 0; q < NRC; ++q)
+#pragma unroll
+        for (int i = 0; i < 2 * PQ + 1; ++i) A.v[q][i] = {0, 0};
+}
+
+// A holds Laplace coefficients l at v[.][l], Helmholtz n at v[.][n + PQ]
+template <int PQ, int NRC>
+__device__ void qbx_store(const KD &kd, double *qexp, int c, const QAcc<PQ, NRC> &A,
+                          int accumulate)
 {
     const int nq = kd.kernel == QBX_LAPLACE2D ? PQ + 1 : 2 * PQ + 1;
-    double *o = qexp + 2 * (size_t)c * kd.ncq;
+    double *o = qexp + 2 * (size_t)c * kd.ncq * kd.nr;
 #pragma unroll
     for (int i = 0; i < 2 * PQ + 1; ++i) {
         if (i < nq) {
-            cplx v = A.v[i];
-            if (accumulate) v += ldc(o + 2 * i);
-            stc(o + 2 * i, v);
+#pragma unroll
+            for (int q = 0; q < NRC; ++q)
+                if (q < kd.rc) {
+                    double *p = o + 2 * ((long)i * kd.nr + kd.r0 + q);
+                    cplx v = A.v[q][i];
+                    if (accumulate) v += ldc(p);
+                    stc(p, v);
+                }
         }
     }
 }
@@ -722,7 +888,7 @@ __device__ __forceinline__ double shfl_xor_d(double v, int m)
 // center's target box; the order-PQ expansion stays in registers and is
 // reduced across the lane group with shuffles.  No atan2/sincos: the phase
 // e^{-i theta} is conj(w)/|w|.
-template <int PQ>
+template <int PQ, int NRC>
 __global__ void __launch_bounds__(256) k_p2qbxl(KD kd, TD t, int ng, const int *gcent,
                                                 const int *c2tb, const double *ctr,
                                                 const double *rad, int nc, const int *l1s,
@@ -733,9 +899,8 @@ __global__ void __launch_bounds__(256) k_p2qbxl(KD kd, TD t, int ng, const int *
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int g = gid / P2Q_LANES, lane = gid % P2Q_LANES;
     const bool active = g < ng;
-    QAcc<PQ> A;
-#pragma unroll
-    for (int i = 0; i < 2 * PQ + 1; ++i) A.v[i] = {0, 0};
+    QAcc<PQ, NRC> A;
+    qacc_zero<PQ, NRC>(A);
     int c = 0;
     if (active) {
         c = gcent[g];
@@ -748,25 +913,35 @@ __global__ void __launch_bounds__(256) k_p2qbxl(KD kd, TD t, int ng, const int *
             const int s0 = t.sstart[Sb], s1 = s0 + t.scount[Sb];
             for (int s = s0 + lane; s < s1; s += P2Q_LANES) {
                 const cplx w = ldpt(src, ns, s) - cc;
-                const cplx q = ldc(str + 2 * s);
+                cplx qs[NRC];
+#pragma unroll
+                for (int q = 0; q < NRC; ++q)
+                    qs[q] = q < kd.rc ? ldq(kd, str, s, kd.r0 + q) : cplx{0, 0};
                 if (kd.kernel == QBX_LAPLACE2D) {
                     const cplx iw = cinv(w);
                     const cplx tt = R * iw;
                     cplx pw = {1, 0};
                     if (!kd.dip) {
-                        A.v[0] += q * clogz(-w);
+                        const cplx lg = clogz(-w);
+#pragma unroll
+                        for (int q = 0; q < NRC; ++q) A.v[q][0] += qs[q] * lg;
 #pragma unroll
                         for (int l = 1; l <= PQ; ++l) {
                             pw = pw * tt;
-                            A.v[l] += (-1.0 / l) * (q * pw);
+                            const cplx b = (-1.0 / l) * pw;
+#pragma unroll
+                            for (int q = 0; q < NRC; ++q) A.v[q][l] += qs[q] * b;
                         }
                     } else {
-                        const cplx f = (ldpt(nrm, ns, s) * q) * iw;
-                        A.v[0] += f;
+                        const cplx f = ldpt(nrm, ns, s) * iw;
+#pragma unroll
+                        for (int q = 0; q < NRC; ++q) A.v[q][0] += qs[q] * f;
 #pragma unroll
                         for (int l = 1; l <= PQ; ++l) {
                             pw = pw * tt;
-                            A.v[l] += f * pw;
+                            const cplx b = f * pw;
+#pragma unroll
+                            for (int q = 0; q < NRC; ++q) A.v[q][l] += qs[q] * b;
                         }
                     }
                 } else {
@@ -780,46 +955,54 @@ __global__ void __launch_bounds__(256) k_p2qbxl(KD kd, TD t, int ng, const int *
 #pragma unroll
                     for (int m = 2; m <= PQ + 1; ++m)
                         H[m] = ((m - 1) * tox) * H[m - 1] - H[m - 2];
-                    // G[m + PQ + 1] = H_m e^{-i m theta}, m in [-(PQ+1), PQ+1]
-                    cplx G[2 * PQ + 3];
+                    // Gv[m + PQ + 1] = H_m e^{-i m theta}, m in [-(PQ+1), PQ+1]
+                    cplx Gv[2 * PQ + 3];
                     cplx e = {1, 0};
 #pragma unroll
                     for (int m = 0; m <= PQ + 1; ++m) {
-                        G[PQ + 1 + m] = H[m] * e;
+                        Gv[PQ + 1 + m] = H[m] * e;
                         const cplx gm = H[m] * conj(e);
-                        G[PQ + 1 - m] = (m & 1) ? -gm : gm;
+                        Gv[PQ + 1 - m] = (m & 1) ? -gm : gm;
                         e = e * em;
                     }
                     if (!kd.dip) {
 #pragma unroll
-                        for (int n = -PQ; n <= PQ; ++n) A.v[n + PQ] += q * G[n + PQ + 1];
+                        for (int n = -PQ; n <= PQ; ++n)
+#pragma unroll
+                            for (int q = 0; q < NRC; ++q)
+                                A.v[q][n + PQ] += qs[q] * Gv[n + PQ + 1];
                     } else {
                         const cplx nu = ldpt(nrm, ns, s);
-                        const cplx hq = (0.5 * kd.k) * q;
-                        const cplx a1 = hq * conj(nu), a2 = hq * nu;
+                        const double hk = 0.5 * kd.k;
+                        const cplx a1 = hk * conj(nu), a2 = hk * nu;
 #pragma unroll
-                        for (int n = -PQ; n <= PQ; ++n)
-                            A.v[n + PQ] += a1 * G[n + PQ] - a2 * G[n + PQ + 2];
+                        for (int n = -PQ; n <= PQ; ++n) {
+                            const cplx b = a1 * Gv[n + PQ] - a2 * Gv[n + PQ + 2];
+#pragma unroll
+                            for (int q = 0; q < NRC; ++q) A.v[q][n + PQ] += qs[q] * b;
+                        }
                     }
                 }
             }
         }
     }
 #pragma unroll
-    for (int i = 0; i < 2 * PQ + 1; ++i) {
+    for (int q = 0; q < NRC; ++q)
 #pragma unroll
-        for (int m = P2Q_LANES / 2; m >= 1; m >>= 1) {
-            A.v[i].x += shfl_xor_d(A.v[i].x, m);
-            A.v[i].y += shfl_xor_d(A.v[i].y, m);
+        for (int i = 0; i < 2 * PQ + 1; ++i) {
+#pragma unroll
+            for (int m = P2Q_LANES / 2; m >= 1; m >>= 1) {
+                A.v[q][i].x += shfl_xor_d(A.v[q][i].x, m);
+                A.v[q][i].y += shfl_xor_d(A.v[q][i].y, m);
+            }
         }
-    }
-    if (active && lane == 0) qbx_store<PQ>(kd, qexp, c, A, accumulate);
+    if (active && lane == 0) qbx_store<PQ, NRC>(kd, qexp, c, A, accumulate);
 }
 
 constexpr int M2Q_LANES = 4;
 
 // M2QBXL: M2Q_LANES lanes per global center split its list-3 boxes
-template <int PQ>
+template <int PQ, int NRC>
 __global__ void __launch_bounds__(256) k_m2qbxl(KD kd, TD t, const double *B, int ng,
                                                 const int *gcent, const int *c2tb,
                                                 const double *ctr, const double *rad, int nc,
@@ -834,9 +1017,8 @@ __global__ void __launch_bounds__(256) k_m2qbxl(KD kd, TD t, const double *B, in
     const int it = active ? c2tb[c] : 0;
     const cplx cc = active ? ldpt(ctr, nc, c) : cplx{0, 0};
     const double R2 = active ? rad[c] : 1.0;
-    QAcc<PQ> A;
-#pragma unroll
-    for (int i = 0; i < 2 * PQ + 1; ++i) A.v[i] = {0, 0};
+    QAcc<PQ, NRC> A;
+    qacc_zero<PQ, NRC>(A);
     const int ebeg = active ? l3s[it] + sub : 0, eend = active ? l3s[it + 1] : 0;
     for (int e = ebeg; e < eend; e += M2Q_LANES) {
         const int D = l3l[e];
@@ -848,23 +1030,32 @@ __global__ void __launch_bounds__(256) k_m2qbxl(KD kd, TD t, const double *B, in
             const cplx z0 = cd - cc;
             const cplx iz = cinv(z0);
             const cplx ms = -(kd.rscale[lv] * iz), tt = R2 * iz;
-            const cplx a0 = ldc(a);
-            cplx Sv[PQ + 1];
-            Sv[0] = a0 * clogz(-z0);
+            const cplx lg = clogz(-z0);
+            cplx Sv[NRC][PQ + 1];
 #pragma unroll
-            for (int l = 1; l <= PQ; ++l) Sv[l] = (-1.0 / l) * a0;
+            for (int q = 0; q < NRC; ++q) {
+                const cplx a0 = q < kd.rc ? ldcn(kd, a, 0, kd.r0 + q) : cplx{0, 0};
+                Sv[q][0] = a0 * lg;
+#pragma unroll
+                for (int l = 1; l <= PQ; ++l) Sv[q][l] = (-1.0 / l) * a0;
+            }
             cplx pw = {1, 0};
             for (int k = 1; k <= p1; ++k) {
                 pw = pw * ms;
-                const cplx ak = ldc(a + 2 * k) * pw;
-                Sv[0] += ak;
 #pragma unroll
-                for (int l = 1; l <= PQ; ++l) Sv[l] += g_binom(B, l + k - 1, k - 1) * ak;
+                for (int q = 0; q < NRC; ++q) {
+                    if (q >= kd.rc) continue;
+                    const cplx ak = ldcn(kd, a, k, kd.r0 + q) * pw;
+                    Sv[q][0] += ak;
+#pragma unroll
+                    for (int l = 1; l <= PQ; ++l) Sv[q][l] += g_binom(B, l + k - 1, k - 1) * ak;
+                }
             }
             cplx tl = {1, 0};
 #pragma unroll
             for (int l = 0; l <= PQ; ++l) {
-                A.v[l] += Sv[l] * tl;
+#pragma unroll
+                for (int q = 0; q < NRC; ++q) A.v[q][l] += Sv[q][l] * tl;
                 tl = tl * tt;
             }
         } else {
@@ -892,26 +1083,34 @@ __global__ void __launch_bounds__(256) k_m2qbxl(KD kd, TD t, const double *B, in
 #pragma unroll
                 for (int m = -PQ; m <= PQ; ++m) {
                     const int n1 = m + l;
-                    if (n1 >= -p1 && n1 <= p1) A.v[m + PQ] += ldc(a + 2 * n1) * he;
                     const int n2 = m - l;
-                    if (l > 0 && n2 >= -p1 && n2 <= p1) A.v[m + PQ] += ldc(a + 2 * n2) * hc;
+#pragma unroll
+                    for (int q = 0; q < NRC; ++q) {
+                        if (q >= kd.rc) continue;
+                        if (n1 >= -p1 && n1 <= p1)
+                            A.v[q][m + PQ] += ldcn(kd, a, n1, kd.r0 + q) * he;
+                        if (l > 0 && n2 >= -p1 && n2 <= p1)
+                            A.v[q][m + PQ] += ldcn(kd, a, n2, kd.r0 + q) * hc;
+                    }
                 }
                 ep = ep * e1;
             }
         }
     }
 #pragma unroll
-    for (int i = 0; i < 2 * PQ + 1; ++i) {
+    for (int q = 0; q < NRC; ++q)
 #pragma unroll
-        for (int m = M2Q_LANES / 2; m >= 1; m >>= 1) {
-            A.v[i].x += __shfl_xor(A.v[i].x, m, M2Q_LANES);
-            A.v[i].y += __shfl_xor(A.v[i].y, m, M2Q_LANES);
+        for (int i = 0; i < 2 * PQ + 1; ++i) {
+#pragma unroll
+            for (int m = M2Q_LANES / 2; m >= 1; m >>= 1) {
+                A.v[q][i].x += __shfl_xor(A.v[q][i].x, m, M2Q_LANES);
+                A.v[q][i].y += __shfl_xor(A.v[q][i].y, m, M2Q_LANES);
+            }
         }
-    }
-    if (active && sub == 0) qbx_store<PQ>(kd, qexp, c, A, accumulate);
+    if (active && sub == 0) qbx_store<PQ, NRC>(kd, qexp, c, A, accumulate);
 }
 
-template <int PQ>
+template <int PQ, int NRC>
 __global__ void k_l2qbxl(KD kd, TD t, const double *B, int ng, const int *gcent, const int *c2tb,
                          const int *tboxes, const double *ctr, const double *rad, int nc,
                          const double *local, double *qexp, int accumulate)
@@ -926,23 +1125,27 @@ __global__ void k_l2qbxl(KD kd, TD t, const double *B, int ng, const int *gcent,
     const double R2 = rad[c];
     const double *a = local + ecoef_off(kd, Bx, lv);
     const cplx cb = boxc(t, Bx);
-    QAcc<PQ> A;
-#pragma unroll
-    for (int i = 0; i < 2 * PQ + 1; ++i) A.v[i] = {0, 0};
+    QAcc<PQ, NRC> A;
+    qacc_zero<PQ, NRC>(A);
     if (kd.kernel == QBX_LAPLACE2D) {
         const double R1 = kd.rscale[lv];
         const cplx u = (1.0 / R1) * (cc - cb);
         for (int l = p1; l >= 0; --l) {
-            const cplx al = ldc(a + 2 * l);
 #pragma unroll
-            for (int j = 0; j <= PQ; ++j)
-                if (j <= l) A.v[j] = A.v[j] * u + g_binom(B, l, j) * al;
+            for (int q = 0; q < NRC; ++q) {
+                if (q >= kd.rc) continue;
+                const cplx al = ldcn(kd, a, l, kd.r0 + q);
+#pragma unroll
+                for (int j = 0; j <= PQ; ++j)
+                    if (j <= l) A.v[q][j] = A.v[q][j] * u + g_binom(B, l, j) * al;
+            }
         }
         double rp = 1.0;
         const double r21 = R2 / R1;
 #pragma unroll
         for (int j = 0; j <= PQ; ++j) {
-            A.v[j] = rp * A.v[j];
+#pragma unroll
+            for (int q = 0; q < NRC; ++q) A.v[q][j] = rp * A.v[q][j];
             rp *= r21;
         }
     } else {
@@ -953,7 +1156,9 @@ __global__ void k_l2qbxl(KD kd, TD t, const double *B, int ng, const int *gcent,
         if (x < 1e-300) {
 #pragma unroll
             for (int m = -PQ; m <= PQ; ++m)
-                if (abs(m) <= p1) A.v[m + PQ] = ldc(a + 2 * m);
+#pragma unroll
+                for (int q = 0; q < NRC; ++q)
+                    if (abs(m) <= p1 && q < kd.rc) A.v[q][m + PQ] = ldcn(kd, a, m, kd.r0 + q);
         } else {
             const cplx e1 = {d.x / rho, d.y / rho};
             const cplx em1 = conj(e1);
@@ -967,24 +1172,35 @@ __global__ void k_l2qbxl(KD kd, TD t, const double *B, int ng, const int *gcent,
 #pragma unroll
                     for (int m = -PQ; m <= PQ; ++m) {
                         const int n1 = m + l;
-                        if (n1 >= -p1 && n1 <= p1) A.v[m + PQ] += ldc(a + 2 * n1) * fe;
                         const int n2 = m - l;
-                        if (l > 0 && n2 >= -p1 && n2 <= p1) A.v[m + PQ] += ldc(a + 2 * n2) * fc;
+#pragma unroll
+                        for (int q = 0; q < NRC; ++q) {
+                            if (q >= kd.rc) continue;
+                            if (n1 >= -p1 && n1 <= p1)
+                                A.v[q][m + PQ] += ldcn(kd, a, n1, kd.r0 + q) * fe;
+                            if (l > 0 && n2 >= -p1 && n2 <= p1)
+                                A.v[q][m + PQ] += ldcn(kd, a, n2, kd.r0 + q) * fc;
+                        }
                     }
                     e = e * em1;
                 },
                 [&](double s) {
 #pragma unroll
-                    for (int m = 0; m < 2 * PQ + 1; ++m) A.v[m] = s * A.v[m];
+                    for (int q = 0; q < NRC; ++q)
+#pragma unroll
+                        for (int m = 0; m < 2 * PQ + 1; ++m) A.v[q][m] = s * A.v[q][m];
                 });
 #pragma unroll
-            for (int m = 0; m < 2 * PQ + 1; ++m) A.v[m] = norm * A.v[m];
+            for (int q = 0; q < NRC; ++q)
+#pragma unroll
+                for (int m = 0; m < 2 * PQ + 1; ++m) A.v[q][m] = norm * A.v[q][m];
         }
     }
-    qbx_store<PQ>(kd, qexp, c, A, accumulate);
+    qbx_store<PQ, NRC>(kd, qexp, c, A, accumulate);
 }
 
 // QBXL2P: one thread per (center, target) entry of center_to_tree_targets
+template <int NRC>
 __global__ void k_qbxl2p(KD kd, int nctt, const int *ctt_lists, const int *tree_ttc,
                          const double *tgt, int nt, const double *ctr, const double *rad, int nc,
                          const double *qexp, double *pot)
@@ -995,19 +1211,24 @@ __global__ void k_qbxl2p(KD kd, int nctt, const int *ctt_lists, const int *tree_
     const int c = tree_ttc[ti];
     const cplx tp = ldpt(tgt, nt, ti);
     const cplx cc = ldpt(ctr, nc, c);
-    const double *a =
-        qexp + 2 * ((size_t)c * kd.ncq + (kd.kernel == QBX_HELMHOLTZ2D ? kd.pq : 0));
-    stc(pot + 2 * (size_t)ti, eval_local(kd, a, kd.pq, cc, rad[c], tp));
+    const double *a = qexp + 2 * ((size_t)c * kd.ncq +
+                                  (kd.kernel == QBX_HELMHOLTZ2D ? kd.pq : 0)) * kd.nr;
+    cplx v[NRC];
+    eval_local_n<NRC>(kd, a, kd.pq, cc, rad[c], tp, v);
+#pragma unroll
+    for (int q = 0; q < NRC; ++q)
+        if (q < kd.rc) stc(pot + 2 * ((size_t)ti * kd.nr + kd.r0 + q), v[q]);
 }
 
-__global__ void k_finalize(int nt, int nc, const int *user_to_tree, const double *pot,
+__global__ void k_finalize(int nt, int nc, int nr, const int *user_to_tree, const double *pot,
                            double *out)
 {
-    const int u = nc + blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= nt) return;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)(nt - nc) * nr) return;
+    const int u = nc + (int)(idx / nr), r = (int)(idx % nr);
     const int ti = user_to_tree[u];
-    out[2 * (size_t)(u - nc)] = pot[2 * (size_t)ti];
-    out[2 * (size_t)(u - nc) + 1] = pot[2 * (size_t)ti + 1];
+    out[2 * idx] = pot[2 * ((size_t)ti * nr + r)];
+    out[2 * idx + 1] = pot[2 * ((size_t)ti * nr + r) + 1];
 }
 
 // -------------------------------------------------------------------------
@@ -1047,6 +1268,7 @@ static int make_kd(const qbx_kernel_desc *d, const qbx_tree *T, KD *kd, int nsep
     QBX_CHECK(d->qbx_order >= 0 && d->qbx_order <= 12, "qbx_order must be in [0, 12]");
     QBX_CHECK(d->level_orders && d->nlevels >= 1 && d->nlevels <= FML,
               "level_orders missing or nlevels outside [1, %d]", FML);
+    QBX_CHECK(d->nrhs >= 0, "nrhs must be >= 0 (0 means 1)");
     if (T) {
         QBX_CHECK(T->dim == 2, "2D kernels need a 2D tree");
         QBX_CHECK(T->nlevels <= FML, "tree has too many levels (%d)", T->nlevels);
@@ -1061,6 +1283,9 @@ static int make_kd(const qbx_kernel_desc *d, const qbx_tree *T, KD *kd, int nsep
     kd->k = d->k;
     kd->nsep = nsep;
     kd->m2l_w = 2 * nsep + 1;
+    kd->nr = d->nrhs > 0 ? d->nrhs : 1;
+    kd->r0 = 0;
+    kd->rc = 1;
     const int ncls = (2 * kd->m2l_w + 1) * (2 * kd->m2l_w + 1);
     const int nl = T ? T->nlevels : d->nlevels;
     kd->nlevels = nl;
@@ -1153,25 +1378,42 @@ static Flat make_flat(const KD &kd, const std::vector<int> &level_starts, bool h
     return f;
 }
 
-#define DISPATCH_PQ(pq, KERNEL, ...)                                                    \
+// Run LAUNCH(N) over the right-hand-side chunks: N = 1 for a single
+// right-hand side, CH otherwise; kd.r0 / kd.rc are set per chunk.
+#define FOR_RHS_CHUNKS(kd, CH, LAUNCH)                                                  \
+    do {                                                                                 \
+        if ((kd).nr == 1) {                                                              \
+            (kd).r0 = 0;                                                                 \
+            (kd).rc = 1;                                                                 \
+            LAUNCH(1);                                                                   \
+        } else {                                                                         \
+            for (int r0_ = 0; r0_ < (kd).nr; r0_ += (CH)) {                             \
+                (kd).r0 = r0_;                                                           \
+                (kd).rc = std::min((CH), (kd).nr - r0_);                                 \
+                LAUNCH(CH);                                                              \
+            }                                                                            \
+        }                                                                                \
+    } while (0)
+
+#define DISPATCH_PQ_NR(pq, NR, KERNEL, ...)                                             \
     switch (pq) {                                                                        \
-    case 0: hipLaunchKernelGGL(KERNEL<0>, __VA_ARGS__); break;                           \
-    case 1: hipLaunchKernelGGL(KERNEL<1>, __VA_ARGS__); break;                           \
-    case 2: hipLaunchKernelGGL(KERNEL<2>, __VA_ARGS__); break;                           \
-    case 3: hipLaunchKernelGGL(KERNEL<3>, __VA_ARGS__); break;                           \
-    case 4: hipLaunchKernelGGL(KERNEL<4>, __VA_ARGS__); break;                           \
-    case 5: hipLaunchKernelGGL(KERNEL<5>, __VA_ARGS__); break;                           \
-    case 6: hipLaunchKernelGGL(KERNEL<6>, __VA_ARGS__); break;                           \
-    case 7: hipLaunchKernelGGL(KERNEL<7>, __VA_ARGS__); break;                           \
-    case 8: hipLaunchKernelGGL(KERNEL<8>, __VA_ARGS__); break;                           \
-    case 9: hipLaunchKernelGGL(KERNEL<9>, __VA_ARGS__); break;                           \
-    case 10: hipLaunchKernelGGL(KERNEL<10>, __VA_ARGS__); break;                         \
-    case 11: hipLaunchKernelGGL(KERNEL<11>, __VA_ARGS__); break;                         \
-    case 12: hipLaunchKernelGGL(KERNEL<12>, __VA_ARGS__); break;                         \
+    case 0: hipLaunchKernelGGL((KERNEL<0, NR>), __VA_ARGS__); break;                     \
+    case 1: hipLaunchKernelGGL((KERNEL<1, NR>), __VA_ARGS__); break;                     \
+    case 2: hipLaunchKernelGGL((KERNEL<2, NR>), __VA_ARGS__); break;                     \
+    case 3: hipLaunchKernelGGL((KERNEL<3, NR>), __VA_ARGS__); break;                     \
+    case 4: hipLaunchKernelGGL((KERNEL<4, NR>), __VA_ARGS__); break;                     \
+    case 5: hipLaunchKernelGGL((KERNEL<5, NR>), __VA_ARGS__); break;                     \
+    case 6: hipLaunchKernelGGL((KERNEL<6, NR>), __VA_ARGS__); break;                     \
+    case 7: hipLaunchKernelGGL((KERNEL<7, NR>), __VA_ARGS__); break;                     \
+    case 8: hipLaunchKernelGGL((KERNEL<8, NR>), __VA_ARGS__); break;                     \
+    case 9: hipLaunchKernelGGL((KERNEL<9, NR>), __VA_ARGS__); break;                     \
+    case 10: hipLaunchKernelGGL((KERNEL<10, NR>), __VA_ARGS__); break;                   \
+    case 11: hipLaunchKernelGGL((KERNEL<11, NR>), __VA_ARGS__); break;                   \
+    case 12: hipLaunchKernelGGL((KERNEL<12, NR>), __VA_ARGS__); break;                   \
     default: set_error("qbx_order out of range"); return QBX_ERR_INVALID;               \
     }
 
-static inline long exp_bytes(const KD &kd) { return 16L * kd.lcoef[kd.nlevels]; }
+static inline long exp_bytes(const KD &kd) { return 16L * kd.lcoef[kd.nlevels] * kd.nr; }
 
 }  // namespace qbx
 
@@ -1184,7 +1426,7 @@ long qbx_expansion_size(const qbx_kernel_desc *d, const qbx_tree *T)
     if (d && d->kernel == QBX_LAPLACE3D) return f3_expansion_size(d, T);
     KD kd;
     if (!T || make_kd(d, T, &kd) != QBX_OK) return -1;
-    return kd.lcoef[kd.nlevels];
+    return kd.lcoef[kd.nlevels] * kd.nr;
 }
 
 int qbx_expansion_level_offsets(const qbx_kernel_desc *d, const qbx_tree *T, long *offsets)
@@ -1193,7 +1435,7 @@ int qbx_expansion_level_offsets(const qbx_kernel_desc *d, const qbx_tree *T, lon
     if (d && d->kernel == QBX_LAPLACE3D) return f3_expansion_level_offsets(d, T, offsets);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd));
-    for (int l = 0; l <= kd.nlevels; ++l) offsets[l] = kd.lcoef[l];
+    for (int l = 0; l <= kd.nlevels; ++l) offsets[l] = kd.lcoef[l] * kd.nr;
     return QBX_OK;
 }
 
@@ -1210,7 +1452,8 @@ int qbx_form_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx
                         double *mpole)
 {
     QBX_CHECK(ctx && T && R && str && mpole, "qbx_form_multipoles: null argument");
-    if (d && d->kernel == QBX_LAPLACE3D) return f3_form_multipoles(ctx, stream, T, R, d, str, nrm, mpole);
+    if (d && d->kernel == QBX_LAPLACE3D)
+        return f3_form_multipoles(ctx, stream, T, R, d, str, nrm, mpole);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd));
     QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
@@ -1218,9 +1461,14 @@ int qbx_form_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx
     QBX_HIP(hipMemsetAsync(mpole, 0, exp_bytes(kd), st));
     const Flat fl = make_flat(kd, R->h_level_start_source_box_nrs, true);
     const long nthr = fl.tstart[fl.nl];
-    if (nthr > 0)
-        hipLaunchKernelGGL(k_p2m, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, make_td(T), fl,
-                           R->source_boxes.p, T->sources.p, T->nsources, str, nrm, mpole);
+    if (nthr > 0) {
+        const TD td = make_td(T);
+#define L_P2M(N)                                                                           \
+    hipLaunchKernelGGL(k_p2m<N>, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, td, fl,      \
+                       R->source_boxes.p, T->sources.p, T->nsources, str, nrm, mpole)
+        FOR_RHS_CHUNKS(kd, NRC_S, L_P2M);
+#undef L_P2M
+    }
     QBX_HIP(hipGetLastError());
     return QBX_OK;
 }
@@ -1236,13 +1484,18 @@ int qbx_coarsen_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const 
     QBX_TRY(binom_table(&B));
     hipStream_t st = S(stream);
     QBX_TRY(ensure_tables(ctx, st, T, &kd));
+    const TD td = make_td(T);
     for (int l = T->nlevels - 2; l >= 0; --l) {
         const int i0 = R->h_level_start_source_parent_box_nrs[l];
         const int i1 = R->h_level_start_source_parent_box_nrs[l + 1];
         const long nthr = (long)(i1 - i0) * kd.ncl[l];
-        if (nthr > 0)
-            hipLaunchKernelGGL(k_m2m, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, make_td(T), B,
-                               l, i0, i1, R->source_parent_boxes.p, mpole);
+        if (nthr > 0) {
+#define L_M2M(N)                                                                           \
+    hipLaunchKernelGGL(k_m2m<N>, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, td, B, l, i0, \
+                       i1, R->source_parent_boxes.p, mpole)
+            FOR_RHS_CHUNKS(kd, NRC_S, L_M2M);
+#undef L_M2M
+        }
     }
     QBX_HIP(hipGetLastError());
     return QBX_OK;
@@ -1260,21 +1513,27 @@ static int eval_targets(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx
     QBX_TRY(make_kd(d, T, &kd));
     hipStream_t st = S(stream);
     if (!accumulate)
-        QBX_HIP(hipMemsetAsync(pot, 0, sizeof(double) * 2 * (size_t)T->ntargets, st));
+        QBX_HIP(hipMemsetAsync(pot, 0, sizeof(double) * 2 * (size_t)T->ntargets * kd.nr, st));
     const int ntb = R->ntarget_boxes;
     if (ntb == 0 || G->nfiltered == 0) return QBX_OK;
     const TD td = make_td(T);
     const int *starts = mode == 1 ? R->l3_starts.p : R->l1_starts.p;
     const int *lists = mode == 1 ? R->l3_lists.p : R->l1_lists.p;
-#define EVAL_LAUNCH(M)                                                                     \
-    hipLaunchKernelGGL(k_eval_targets<M>, dim3(ntb), dim3(64), 0, st, kd, td, ntb,         \
+#define EVAL_LAUNCH(M, N)                                                                  \
+    hipLaunchKernelGGL((k_eval_targets<M, N>), dim3(ntb), dim3(64), 0, st, kd, td, ntb,    \
                        R->target_boxes.p, starts, lists, G->nqbtl_box_target_starts.p,    \
                        G->nqbtl_box_target_counts_nonchild.p, G->nqbtl_unfiltered.p,      \
                        T->targets.p, T->ntargets, T->sources.p, T->nsources, str, nrm,    \
                        expn, pot)
-    if (mode == 0) EVAL_LAUNCH(0);
-    else if (mode == 1) EVAL_LAUNCH(1);
-    else EVAL_LAUNCH(2);
+#define EV0(N) EVAL_LAUNCH(0, N)
+#define EV1(N) EVAL_LAUNCH(1, N)
+#define EV2(N) EVAL_LAUNCH(2, N)
+    if (mode == 0) FOR_RHS_CHUNKS(kd, NRC_S, EV0);
+    else if (mode == 1) FOR_RHS_CHUNKS(kd, NRC_S, EV1);
+    else FOR_RHS_CHUNKS(kd, NRC_S, EV2);
+#undef EV0
+#undef EV1
+#undef EV2
 #undef EVAL_LAUNCH
     QBX_HIP(hipGetLastError());
     return QBX_OK;
@@ -1309,7 +1568,8 @@ int qbx_multipole_to_local(qbx_ctx *ctx, void *stream, const qbx_tree *T, const 
                            const qbx_kernel_desc *d, const double *mpole, double *local)
 {
     QBX_CHECK(ctx && T && R && mpole && local, "qbx_multipole_to_local: null argument");
-    if (d && d->kernel == QBX_LAPLACE3D) return f3_multipole_to_local(ctx, stream, T, R, d, mpole, local);
+    if (d && d->kernel == QBX_LAPLACE3D)
+        return f3_multipole_to_local(ctx, stream, T, R, d, mpole, local);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd, R->nsep));
     const double *B;
@@ -1319,9 +1579,14 @@ int qbx_multipole_to_local(qbx_ctx *ctx, void *stream, const qbx_tree *T, const 
     QBX_HIP(hipMemsetAsync(local, 0, exp_bytes(kd), st));
     const Flat fl = make_flat(kd, R->h_level_start_ttp_box_nrs, false);
     const long nthr = fl.tstart[fl.nl];
-    if (nthr > 0 && R->n2 > 0)
-        hipLaunchKernelGGL(k_m2l, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, make_td(T), B, fl,
-                           R->ttp_boxes.p, R->l2_starts.p, R->l2_lists.p, mpole, local);
+    if (nthr > 0 && R->n2 > 0) {
+        const TD td = make_td(T);
+#define L_M2L(N)                                                                           \
+    hipLaunchKernelGGL(k_m2l<N>, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, td, B, fl,   \
+                       R->ttp_boxes.p, R->l2_starts.p, R->l2_lists.p, mpole, local)
+        FOR_RHS_CHUNKS(kd, NRC_S, L_M2L);
+#undef L_M2L
+    }
     QBX_HIP(hipGetLastError());
     return QBX_OK;
 }
@@ -1331,7 +1596,8 @@ int qbx_form_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_tra
                     double *local, int accumulate)
 {
     QBX_CHECK(ctx && T && R && str && local, "qbx_form_locals: null argument");
-    if (d && d->kernel == QBX_LAPLACE3D) return f3_form_locals(ctx, stream, T, R, d, str, nrm, local, accumulate);
+    if (d && d->kernel == QBX_LAPLACE3D)
+        return f3_form_locals(ctx, stream, T, R, d, str, nrm, local, accumulate);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd));
     QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
@@ -1339,10 +1605,15 @@ int qbx_form_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_tra
     if (!accumulate) QBX_HIP(hipMemsetAsync(local, 0, exp_bytes(kd), st));
     const Flat fl = make_flat(kd, R->h_level_start_ttp_box_nrs, true);
     const long nthr = fl.tstart[fl.nl];
-    if (nthr > 0 && R->n4 > 0)
-        hipLaunchKernelGGL(k_p2l, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, make_td(T), fl,
-                           R->ttp_boxes.p, R->l4_starts.p, R->l4_lists.p, T->sources.p,
-                           T->nsources, str, nrm, local);
+    if (nthr > 0 && R->n4 > 0) {
+        const TD td = make_td(T);
+#define L_P2L(N)                                                                           \
+    hipLaunchKernelGGL(k_p2l<N>, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, td, fl,      \
+                       R->ttp_boxes.p, R->l4_starts.p, R->l4_lists.p, T->sources.p,        \
+                       T->nsources, str, nrm, local)
+        FOR_RHS_CHUNKS(kd, NRC_S, L_P2L);
+#undef L_P2L
+    }
     QBX_HIP(hipGetLastError());
     return QBX_OK;
 }
@@ -1358,12 +1629,17 @@ int qbx_refine_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_t
     QBX_TRY(binom_table(&B));
     hipStream_t st = S(stream);
     QBX_TRY(ensure_tables(ctx, st, T, &kd));
+    const TD td = make_td(T);
     for (int l = 1; l < T->nlevels; ++l) {
         const int i0 = R->h_level_start_ttp_box_nrs[l], i1 = R->h_level_start_ttp_box_nrs[l + 1];
         const long nthr = (long)(i1 - i0) * kd.ncl[l];
-        if (nthr > 0)
-            hipLaunchKernelGGL(k_l2l, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, make_td(T), B,
-                               l, i0, i1, R->ttp_boxes.p, local);
+        if (nthr > 0) {
+#define L_L2L(N)                                                                           \
+    hipLaunchKernelGGL(k_l2l<N>, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, td, B, l, i0, \
+                       i1, R->ttp_boxes.p, local)
+            FOR_RHS_CHUNKS(kd, NRC_S, L_L2L);
+#undef L_L2L
+        }
     }
     QBX_HIP(hipGetLastError());
     return QBX_OK;
@@ -1381,13 +1657,18 @@ int qbx_form_global_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, co
     QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
     hipStream_t st = S(stream);
     if (!accumulate)
-        QBX_HIP(hipMemsetAsync(qexp, 0, sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq, st));
+        QBX_HIP(hipMemsetAsync(qexp, 0,
+                               sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq * kd.nr, st));
     const int ng = G->nglobal;
     if (ng == 0) return QBX_OK;
-    DISPATCH_PQ(kd.pq, k_p2qbxl, dim3(cdiv((long)ng * P2Q_LANES, 256)), dim3(256), 0, st, kd,
-                make_td(T), ng, G->global_qbx_centers.p, G->center_to_target_box.p,
-                G->centers.p, G->radii.p, G->ncenters, R->l1_starts.p, R->l1_lists.p,
-                T->sources.p, T->nsources, str, nrm, qexp, accumulate);
+    const TD td = make_td(T);
+#define L_P2Q(N)                                                                           \
+    DISPATCH_PQ_NR(kd.pq, N, k_p2qbxl, dim3(cdiv((long)ng * P2Q_LANES, 256)), dim3(256), 0, \
+                   st, kd, td, ng, G->global_qbx_centers.p, G->center_to_target_box.p,     \
+                   G->centers.p, G->radii.p, G->ncenters, R->l1_starts.p, R->l1_lists.p,   \
+                   T->sources.p, T->nsources, str, nrm, qexp, accumulate)
+    FOR_RHS_CHUNKS(kd, NRC_Q, L_P2Q);
+#undef L_P2Q
     QBX_HIP(hipGetLastError());
     return QBX_OK;
 }
@@ -1407,13 +1688,18 @@ int qbx_translate_box_multipoles_to_qbx_local(qbx_ctx *ctx, void *stream, const 
     QBX_TRY(binom_table(&B));
     hipStream_t st = S(stream);
     if (!accumulate)
-        QBX_HIP(hipMemsetAsync(qexp, 0, sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq, st));
+        QBX_HIP(hipMemsetAsync(qexp, 0,
+                               sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq * kd.nr, st));
     const int ng = G->nglobal;
     if (ng == 0 || R->n3 == 0) return QBX_OK;
-    DISPATCH_PQ(kd.pq, k_m2qbxl, dim3(cdiv((long)ng * M2Q_LANES, 256)), dim3(256), 0, st, kd,
-                make_td(T), B, ng,
-                G->global_qbx_centers.p, G->center_to_target_box.p, G->centers.p, G->radii.p,
-                G->ncenters, R->l3_starts.p, R->l3_lists.p, mpole, qexp, 1);
+    const TD td = make_td(T);
+#define L_M2Q(N)                                                                           \
+    DISPATCH_PQ_NR(kd.pq, N, k_m2qbxl, dim3(cdiv((long)ng * M2Q_LANES, 256)), dim3(256), 0, \
+                   st, kd, td, B, ng, G->global_qbx_centers.p, G->center_to_target_box.p,  \
+                   G->centers.p, G->radii.p, G->ncenters, R->l3_starts.p, R->l3_lists.p,   \
+                   mpole, qexp, 1)
+    FOR_RHS_CHUNKS(kd, NRC_Q, L_M2Q);
+#undef L_M2Q
     QBX_HIP(hipGetLastError());
     return QBX_OK;
 }
@@ -1433,12 +1719,17 @@ int qbx_translate_box_local_to_qbx_local(qbx_ctx *ctx, void *stream, const qbx_t
     QBX_TRY(binom_table(&B));
     hipStream_t st = S(stream);
     if (!accumulate)
-        QBX_HIP(hipMemsetAsync(qexp, 0, sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq, st));
+        QBX_HIP(hipMemsetAsync(qexp, 0,
+                               sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq * kd.nr, st));
     const int ng = G->nglobal;
     if (ng == 0) return QBX_OK;
-    DISPATCH_PQ(kd.pq, k_l2qbxl, dim3(cdiv(ng, 64)), dim3(64), 0, st, kd, make_td(T), B, ng,
-                G->global_qbx_centers.p, G->center_to_target_box.p, R->target_boxes.p,
-                G->centers.p, G->radii.p, G->ncenters, local, qexp, 1);
+    const TD td = make_td(T);
+#define L_L2Q(N)                                                                           \
+    DISPATCH_PQ_NR(kd.pq, N, k_l2qbxl, dim3(cdiv(ng, 64)), dim3(64), 0, st, kd, td, B, ng,  \
+                   G->global_qbx_centers.p, G->center_to_target_box.p, R->target_boxes.p,  \
+                   G->centers.p, G->radii.p, G->ncenters, local, qexp, 1)
+    FOR_RHS_CHUNKS(kd, NRC_Q, L_L2Q);
+#undef L_L2Q
     QBX_HIP(hipGetLastError());
     return QBX_OK;
 }
@@ -1447,26 +1738,32 @@ int qbx_eval_qbx_expansions(qbx_ctx *ctx, void *stream, const qbx_tree *T, const
                             const qbx_kernel_desc *d, const double *qexp, double *pot)
 {
     QBX_CHECK(ctx && T && G && qexp && pot, "qbx_eval_qbx_expansions: null argument");
-    if (d && d->kernel == QBX_LAPLACE3D) return f3_eval_qbx_expansions(ctx, stream, T, G, d, qexp, pot);
+    if (d && d->kernel == QBX_LAPLACE3D)
+        return f3_eval_qbx_expansions(ctx, stream, T, G, d, qexp, pot);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd));
     hipStream_t st = S(stream);
-    if (G->nctt > 0)
-        hipLaunchKernelGGL(k_qbxl2p, dim3(cdiv(G->nctt, 64)), dim3(64), 0, st, kd, G->nctt,
-                           G->ctt_lists.p, G->tree_ttc.p, T->targets.p, T->ntargets,
-                           G->centers.p, G->radii.p, G->ncenters, qexp, pot);
+    if (G->nctt > 0) {
+#define L_Q2P(N)                                                                           \
+    hipLaunchKernelGGL(k_qbxl2p<N>, dim3(cdiv(G->nctt, 64)), dim3(64), 0, st, kd, G->nctt, \
+                       G->ctt_lists.p, G->tree_ttc.p, T->targets.p, T->ntargets,           \
+                       G->centers.p, G->radii.p, G->ncenters, qexp, pot)
+        FOR_RHS_CHUNKS(kd, NRC_S, L_Q2P);
+#undef L_Q2P
+    }
     QBX_HIP(hipGetLastError());
     return QBX_OK;
 }
 
 int qbx_finalize_potentials(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_geo *G,
-                            const double *pot, double *out)
+                            int nrhs, const double *pot, double *out)
 {
     QBX_CHECK(ctx && T && G && pot && out, "qbx_finalize_potentials: null argument");
+    QBX_CHECK(nrhs >= 1, "qbx_finalize_potentials: nrhs must be >= 1");
     const int nt = T->ntargets, nc = G->ncenters;
     if (nt > nc)
-        hipLaunchKernelGGL(k_finalize, dim3(cdiv(nt - nc, 256)), dim3(256), 0, S(stream), nt, nc,
-                           G->user_to_tree_target.p, pot, out);
+        hipLaunchKernelGGL(k_finalize, dim3(cdiv((long)(nt - nc) * nrhs, 256)), dim3(256), 0,
+                           S(stream), nt, nc, nrhs, G->user_to_tree_target.p, pot, out);
     QBX_HIP(hipGetLastError());
     return QBX_OK;
 }
@@ -1475,13 +1772,14 @@ int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav 
                   const qbx_geo *G, const qbx_kernel_desc *d, const double *str,
                   const double *nrm, double *out_user)
 {
-    QBX_CHECK(ctx && T && R && G && str && out_user, "qbx_drive_fmm: null argument");
+    QBX_CHECK(ctx && T && R && G && d && str && out_user, "qbx_drive_fmm: null argument");
     const long nexp = qbx_expansion_size(d, T);
     const int ncq = qbx_ncoeff_qbx(d);
     if (nexp < 0 || ncq < 0) return QBX_ERR_INVALID;
+    const int nr = d->nrhs > 0 ? d->nrhs : 1;
     const long nm = 2L * std::max(nexp, 1L);
-    const long nq = 2L * std::max(G->ncenters, 1) * ncq;
-    const long np = 2L * std::max(T->ntargets, 1);
+    const long nq = 2L * std::max(G->ncenters, 1) * ncq * nr;
+    const long np = 2L * std::max(T->ntargets, 1) * nr;
     if (ctx->ws_mpole.n < nm) QBX_TRY(ctx->ws_mpole.alloc(nm));
     if (ctx->ws_local.n < nm) QBX_TRY(ctx->ws_local.alloc(nm));
     if (ctx->ws_qbx.n < nq) QBX_TRY(ctx->ws_qbx.alloc(nq));
@@ -1499,7 +1797,7 @@ int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav 
     QBX_TRY(qbx_translate_box_multipoles_to_qbx_local(ctx, stream, T, R, G, d, M, Q, 1));
     QBX_TRY(qbx_translate_box_local_to_qbx_local(ctx, stream, T, R, G, d, L, Q, 1));
     QBX_TRY(qbx_eval_qbx_expansions(ctx, stream, T, G, d, Q, P));
-    QBX_TRY(qbx_finalize_potentials(ctx, stream, T, G, P, out_user));
+    QBX_TRY(qbx_finalize_potentials(ctx, stream, T, G, nr, P, out_user));
     return QBX_OK;
 }
 

@@ -21,13 +21,18 @@
 //   T_m^m = -(2m-1)(w/r^2) T_{m-1}^{m-1},
 //   T_n^m = ((2n-1) z T_{n-1}^m - (n+m-1)(n-m-1) T_{n-2}^m) / r^2
 //
+// Right-hand sides: as in the 2D stages, nr densities share one evaluation,
+// arrays carry them innermost, and each kernel applies an interaction's
+// geometry (harmonics, translation weights) to a chunk of right-hand sides
+// held in registers; the host launches one chunk [r0, r0 + rc) at a time.
+//
 // Storage (HBM): box expansions level-compact, (p_l+1)^2 complex per box of
-// level l; QBX expansions [ncenters][(p_q+1)^2].
+// level l (times nr); QBX expansions [ncenters][(p_q+1)^2][nr].
 //
 // Thread mappings:
 //  P2M, P2L        one thread per (box, coefficient)
 //  M2M, L2L        one thread per (box, coefficient), child offsets from a
-//                  per-tree table of Y at the 8 unit child offsets
+//                  table of Y at the 8 unit child offsets
 //  M2L             one 64-lane block per target box; each list-2 multipole is
 //                  staged (rescaled by h^-n) in LDS and contracted against
 //                  the irregular harmonics of its integer offset class, read
@@ -53,12 +58,15 @@ namespace f3 {
 constexpr int MAXP = 20;       // box expansion orders
 constexpr int MAXPQ = 8;       // QBX expansion orders (templated)
 constexpr int FML = 21;        // levels of a 3D tree (19-bit coordinates)
+constexpr int NRC_S = 8;       // RHS chunk of the scalar-accumulator kernels
+constexpr int NRC_Q = 2;       // RHS chunk of the QBX-expansion kernels
 constexpr double INV4PI = 0.079577471545947667884;
 
 __host__ __device__ constexpr int IDX(int n, int m) { return n * n + n + m; }
 
 struct KD {
     int dip, pq, nlevels, ncq, nsep, W, pmax, tabn;   // tabn = 2 pmax + 1
+    int nr, r0, rc;            // right-hand sides; the chunk of this launch
     int order[FML];
     int ncl[FML];
     int lbox[FML + 1];
@@ -104,9 +112,21 @@ __device__ __forceinline__ v3 ld3(const double *p, long n, long i)
 __device__ __forceinline__ v3 sub(v3 a, v3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
 __device__ __forceinline__ v3 boxc(const TD &t, int b) { return ld3(t.centers, t.nboxes, b); }
 
+// coefficient 0, right-hand side 0 of box b; coefficient i, right-hand side
+// r at [2 (i nr + r)]
 __device__ __forceinline__ long ecoef_off(const KD &kd, int b, int lv)
 {
-    return 2 * (kd.lcoef[lv] + (long)(b - kd.lbox[lv]) * kd.ncl[lv]);
+    return 2 * (kd.lcoef[lv] + (long)(b - kd.lbox[lv]) * kd.ncl[lv]) * kd.nr;
+}
+
+__device__ __forceinline__ cplx ldq(const KD &kd, const double *str, int s, int r)
+{
+    return ldc(str + 2 * ((size_t)s * kd.nr + r));
+}
+
+__device__ __forceinline__ cplx ldcn(const KD &kd, const double *a, int i, int r)
+{
+    return ldc(a + 2 * ((long)i * kd.nr + r));
 }
 
 __device__ __forceinline__ void nm_of(int i, int &n, int &m)
@@ -132,7 +152,8 @@ __device__ cplx ups_nm(int n, int m, v3 d)
     cplx b = {1, 0}, a = {0, 0};
     for (int i = 1; i <= am; ++i) b = (1.0 / i) * (b * hw);
     for (int k = am + 1; k <= n; ++k) {
-        const cplx c = (1.0 / ((double)(k - am) * (k + am))) * (((2.0 * k - 1.0) * d.z) * b - r2 * a);
+        const cplx c =
+            (1.0 / ((double)(k - am) * (k + am))) * (((2.0 * k - 1.0) * d.z) * b - r2 * a);
         a = b;
         b = c;
     }
@@ -158,33 +179,7 @@ __device__ cplx the_nm(int n, int m, v3 d)
     return m < 0 ? neg_m(b, am) : b;
 }
 
-// full tables for n <= P in registers
-template <int P>
-__device__ __forceinline__ void ups_table(v3 d, cplx *Y)
-{
-    const double r2 = d.x * d.x + d.y * d.y + d.z * d.z;
-    const cplx hw = {-0.5 * d.x, -0.5 * d.y};
-    cplx ymm = {1, 0};
-#pragma unroll
-    for (int m = 0; m <= P; ++m) {
-        if (m > 0) ymm = (1.0 / m) * (ymm * hw);
-        Y[IDX(m, m)] = ymm;
-        cplx a = {0, 0}, b = ymm;
-#pragma unroll
-        for (int n = m + 1; n <= P; ++n) {
-            const cplx c = (1.0 / ((double)(n - m) * (n + m))) *
-                           (((2.0 * n - 1.0) * d.z) * b - r2 * a);
-            Y[IDX(n, m)] = c;
-            a = b;
-            b = c;
-        }
-    }
-#pragma unroll
-    for (int n = 1; n <= P; ++n)
-#pragma unroll
-        for (int m = 1; m <= n; ++m) Y[IDX(n, -m)] = neg_m(Y[IDX(n, m)], m);
-}
-
+// full table T_n^m(d), n <= P, in registers
 template <int P>
 __device__ __forceinline__ void the_table(v3 d, cplx *T)
 {
@@ -212,14 +207,15 @@ __device__ __forceinline__ void the_table(v3 d, cplx *T)
         for (int m = 1; m <= n; ++m) T[IDX(n, -m)] = neg_m(T[IDX(n, m)], m);
 }
 
-// sum_{n<=p,m} coef[n,m] T_n^m(d): streamed column by column, O(1) state
-__device__ cplx sum_irregular(const double *coef, int p, v3 d)
+// out[q] += sum_{n<=p,m} coef[n,m][r0+q] T_n^m(d); coef strided by nr,
+// streamed column by column with O(1) harmonic state
+template <int NRC>
+__device__ void sum_irregular_n(const KD &kd, const double *coef, int p, v3 d, cplx *out)
 {
     const double r2 = d.x * d.x + d.y * d.y + d.z * d.z;
     const double ir2 = 1.0 / r2;
     const cplx w = {d.x * ir2, d.y * ir2};
     cplx tmm = {sqrt(ir2), 0};
-    cplx acc = {0, 0};
     for (int m = 0; m <= p; ++m) {
         if (m > 0) tmm = (-(2.0 * m - 1.0)) * (tmm * w);
         cplx a = {0, 0}, b = tmm;
@@ -230,20 +226,24 @@ __device__ cplx sum_irregular(const double *coef, int p, v3 d)
                 a = b;
                 b = c;
             }
-            acc += ldc(coef + 2 * IDX(n, m)) * b;
-            if (m > 0) acc += ldc(coef + 2 * IDX(n, -m)) * neg_m(b, m);
+            const cplx bn = neg_m(b, m);
+#pragma unroll
+            for (int q = 0; q < NRC; ++q)
+                if (q < kd.rc) {
+                    out[q] += ldcn(kd, coef, IDX(n, m), kd.r0 + q) * b;
+                    if (m > 0) out[q] += ldcn(kd, coef, IDX(n, -m), kd.r0 + q) * bn;
+                }
         }
     }
-    return acc;
 }
 
-// sum_{n<=p,m} coef[n,m] conj(Y_n^m(d)); conj(Y_n^{-m}) = (-1)^m Y_n^m
-__device__ cplx sum_regular_conj(const double *coef, int p, v3 d)
+// out[q] += sum_{n<=p,m} coef[n,m][r0+q] conj(Y_n^m(d)); conj(Y_n^{-m}) = (-1)^m Y_n^m
+template <int NRC>
+__device__ void sum_regular_conj_n(const KD &kd, const double *coef, int p, v3 d, cplx *out)
 {
     const double r2 = d.x * d.x + d.y * d.y + d.z * d.z;
     const cplx hw = {-0.5 * d.x, -0.5 * d.y};
     cplx ymm = {1, 0};
-    cplx acc = {0, 0};
     for (int m = 0; m <= p; ++m) {
         if (m > 0) ymm = (1.0 / m) * (ymm * hw);
         cplx a = {0, 0}, b = ymm;
@@ -254,31 +254,33 @@ __device__ cplx sum_regular_conj(const double *coef, int p, v3 d)
                 a = b;
                 b = c;
             }
-            acc += ldc(coef + 2 * IDX(n, m)) * conj(b);
-            if (m > 0) {
-                const cplx v = ldc(coef + 2 * IDX(n, -m)) * b;
-                acc += (m & 1) ? -v : v;
-            }
+            const cplx bc = conj(b);
+            const cplx bs = (m & 1) ? -b : b;
+#pragma unroll
+            for (int q = 0; q < NRC; ++q)
+                if (q < kd.rc) {
+                    out[q] += ldcn(kd, coef, IDX(n, m), kd.r0 + q) * bc;
+                    if (m > 0) out[q] += ldcn(kd, coef, IDX(n, -m), kd.r0 + q) * bs;
+                }
         }
     }
-    return acc;
 }
 
-__device__ __forceinline__ cplx p2p_pair(const KD &kd, v3 s, cplx q, v3 nu, v3 t)
+__device__ __forceinline__ double p2p_value(const KD &kd, v3 s, v3 nu, v3 t)
 {
     const v3 d = sub(t, s);
     const double r2 = d.x * d.x + d.y * d.y + d.z * d.z;
-    if (r2 == 0.0) return {0, 0};
+    if (r2 == 0.0) return 0.0;
     const double ir = 1.0 / sqrt(r2);
-    if (!kd.dip) return (INV4PI * ir) * q;
-    return (INV4PI * ir * ir * ir * (nu.x * d.x + nu.y * d.y + nu.z * d.z)) * q;
+    if (!kd.dip) return INV4PI * ir;
+    return INV4PI * ir * ir * ir * (nu.x * d.x + nu.y * d.y + nu.z * d.z);
 }
 
-// coefficient (n, m) of one source's multipole (local = 0) or local (1)
-__device__ __forceinline__ cplx p2e_coeff(const KD &kd, int local, int n, int m, v3 d, cplx q,
-                                          v3 nu)
+// coefficient (n, m) one unit-strength source contributes to a multipole
+// (local = 0) or local (1)
+__device__ __forceinline__ cplx p2e_basis(const KD &kd, int local, int n, int m, v3 d, v3 nu)
 {
-    if (!kd.dip) return local ? q * the_nm(n, m, d) : q * conj(ups_nm(n, m, d));
+    if (!kd.dip) return local ? the_nm(n, m, d) : conj(ups_nm(n, m, d));
     const cplx nm = {nu.x, -nu.y}, np = {nu.x, nu.y};
     cplx g;
     if (local)
@@ -287,10 +289,22 @@ __device__ __forceinline__ cplx p2e_coeff(const KD &kd, int local, int n, int m,
     else
         g = nu.z * ups_nm(n - 1, m, d) +
             0.5 * (nm * ups_nm(n - 1, m + 1, d) - np * ups_nm(n - 1, m - 1, d));
-    return local ? q * g : q * conj(g);
+    return local ? g : conj(g);
+}
+
+template <int NRC>
+__device__ __forceinline__ void store_add(const KD &kd, double *o, const cplx *acc)
+{
+#pragma unroll
+    for (int q = 0; q < NRC; ++q)
+        if (q < kd.rc) {
+            double *p = o + 2 * (kd.r0 + q);
+            stc(p, ldc(p) + acc[q]);
+        }
 }
 
 // P2M over source boxes: thread per (box, coefficient)
+template <int NRC>
 __global__ void k3_p2m(KD kd, TD t, Flat fl, const int *boxes, const double *src, int ns,
                        const double *str, const double *nrm, double *mpole)
 {
@@ -302,15 +316,24 @@ __global__ void k3_p2m(KD kd, TD t, Flat fl, const int *boxes, const double *src
     nm_of(j, n, m);
     const v3 c = boxc(t, b);
     const int s0 = t.sstart[b], s1 = s0 + t.scount[b];
-    cplx acc = {0, 0};
+    cplx acc[NRC];
+#pragma unroll
+    for (int q = 0; q < NRC; ++q) acc[q] = {0, 0};
     for (int s = s0; s < s1; ++s) {
         const v3 nu = kd.dip ? ld3(nrm, ns, s) : v3{0, 0, 0};
-        acc += p2e_coeff(kd, 0, n, m, sub(ld3(src, ns, s), c), ldc(str + 2 * s), nu);
+        const cplx bs = p2e_basis(kd, 0, n, m, sub(ld3(src, ns, s), c), nu);
+#pragma unroll
+        for (int q = 0; q < NRC; ++q)
+            if (q < kd.rc) acc[q] += ldq(kd, str, s, kd.r0 + q) * bs;
     }
-    stc(mpole + ecoef_off(kd, b, lv) + 2 * j, acc);
+    double *o = mpole + ecoef_off(kd, b, lv) + 2L * j * kd.nr;
+#pragma unroll
+    for (int q = 0; q < NRC; ++q)
+        if (q < kd.rc) stc(o + 2 * (kd.r0 + q), acc[q]);
 }
 
 // P2L over list 4: thread per (ttp box, coefficient), accumulating
+template <int NRC>
 __global__ void k3_p2l(KD kd, TD t, Flat fl, const int *ttp, const int *starts, const int *lists,
                        const double *src, int ns, const double *str, const double *nrm,
                        double *local)
@@ -324,17 +347,21 @@ __global__ void k3_p2l(KD kd, TD t, Flat fl, const int *ttp, const int *starts, 
     int n, m;
     nm_of(j, n, m);
     const v3 c = boxc(t, b);
-    cplx acc = {0, 0};
+    cplx acc[NRC];
+#pragma unroll
+    for (int q = 0; q < NRC; ++q) acc[q] = {0, 0};
     for (int e = e0; e < e1; ++e) {
         const int S = lists[e];
         const int s0 = t.sstart[S], s1 = s0 + t.scount[S];
         for (int s = s0; s < s1; ++s) {
             const v3 nu = kd.dip ? ld3(nrm, ns, s) : v3{0, 0, 0};
-            acc += p2e_coeff(kd, 1, n, m, sub(ld3(src, ns, s), c), ldc(str + 2 * s), nu);
+            const cplx bs = p2e_basis(kd, 1, n, m, sub(ld3(src, ns, s), c), nu);
+#pragma unroll
+            for (int q = 0; q < NRC; ++q)
+                if (q < kd.rc) acc[q] += ldq(kd, str, s, kd.r0 + q) * bs;
         }
     }
-    double *o = local + ecoef_off(kd, b, lv) + 2 * j;
-    stc(o, ldc(o) + acc);
+    store_add<NRC>(kd, local + ecoef_off(kd, b, lv) + 2L * j * kd.nr, acc);
 }
 
 // child offset index bits: bit d set = upper half in dimension d
@@ -344,6 +371,7 @@ __device__ __forceinline__ const double *child_tab(const KD &kd, int ci)
 }
 
 // M2M into parents of level lvp: thread per (parent, coefficient)
+template <int NRC>
 __global__ void k3_m2m(KD kd, TD t, int lvp, int i0, int i1, const int *spb, double *mpole)
 {
     const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -357,7 +385,9 @@ __global__ void k3_m2m(KD kd, TD t, int lvp, int i0, int i1, const int *spb, dou
     const int lvc = lvp + 1;
     const int pc = kd.order[lvc];
     const double hc = kd.h[lvc];
-    cplx acc = {0, 0};
+    cplx acc[NRC];
+#pragma unroll
+    for (int q = 0; q < NRC; ++q) acc[q] = {0, 0};
     for (int ci = 0; ci < 8; ++ci) {
         const int C = t.child[(size_t)ci * t.nboxes + P];
         if (C < 0 || t.scumul[C] == 0) continue;
@@ -371,16 +401,23 @@ __global__ void k3_m2m(KD kd, TD t, int lvp, int i0, int i1, const int *spb, dou
                 for (int l = -k; l <= k; ++l) {
                     const int mm = m - l;
                     if (mm < -nn || mm > nn) continue;
-                    acc += (hk * conj(ldc(U + 2 * IDX(k, l)))) * ldc(a + 2 * IDX(nn, mm));
+                    const cplx wgt = hk * conj(ldc(U + 2 * IDX(k, l)));
+#pragma unroll
+                    for (int q = 0; q < NRC; ++q)
+                        if (q < kd.rc) acc[q] += wgt * ldcn(kd, a, IDX(nn, mm), kd.r0 + q);
                 }
             }
             hk *= hc;
         }
     }
-    stc(mpole + ecoef_off(kd, P, lvp) + 2 * j, acc);
+    double *o = mpole + ecoef_off(kd, P, lvp) + 2L * j * kd.nr;
+#pragma unroll
+    for (int q = 0; q < NRC; ++q)
+        if (q < kd.rc) stc(o + 2 * (kd.r0 + q), acc[q]);
 }
 
 // L2L into ttp boxes of level lv: thread per (box, coefficient), accumulating
+template <int NRC>
 __global__ void k3_l2l(KD kd, TD t, int lv, int i0, int i1, const int *ttp, double *local)
 {
     const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -402,28 +439,35 @@ __global__ void k3_l2l(KD kd, TD t, int lv, int i0, int i1, const int *ttp, doub
     ci |= (t.coords[2 * (size_t)t.nboxes + Bx] & 1) << 2;
     const double *U = child_tab(kd, ci);
     const double *L = local + ecoef_off(kd, Pp, lvp);
-    cplx acc = {0, 0};
+    cplx acc[NRC];
+#pragma unroll
+    for (int q = 0; q < NRC; ++q) acc[q] = {0, 0};
     double hj = 1.0;
     for (int jj = 0; a + jj <= pp; ++jj) {
         for (int i = -jj; i <= jj; ++i) {
             const int mm = b + i;
             if (mm < -(a + jj) || mm > a + jj) continue;
-            acc += (hj * conj(ldc(U + 2 * IDX(jj, i)))) * ldc(L + 2 * IDX(a + jj, mm));
+            const cplx wgt = hj * conj(ldc(U + 2 * IDX(jj, i)));
+#pragma unroll
+            for (int q = 0; q < NRC; ++q)
+                if (q < kd.rc) acc[q] += wgt * ldcn(kd, L, IDX(a + jj, mm), kd.r0 + q);
         }
         hj *= hc;
     }
-    double *o = local + ecoef_off(kd, Bx, lv) + 2 * j;
-    stc(o, ldc(o) + acc);
+    store_add<NRC>(kd, local + ecoef_off(kd, Bx, lv) + 2L * j * kd.nr, acc);
 }
 
 constexpr int M2L_THREADS = 64;
 
-// M2L: one 64-lane block per ttp box with list-2 entries
+// M2L: one 64-lane block per ttp box with list-2 entries; NO output
+// coefficients per lane, NRC right-hand sides; LDS holds the rescaled source
+// multipole [ncl][NRC]
+template <int NO, int NRC>
 __global__ void __launch_bounds__(M2L_THREADS) k3_m2l(KD kd, TD t, const int *ttp,
                                                       const int *starts, const int *lists,
                                                       const double *mpole, double *local)
 {
-    __shared__ double Ms[2 * (MAXP + 1) * (MAXP + 1)];
+    extern __shared__ double Ms[];
     const int i = blockIdx.x;
     const int e0 = starts[i], e1 = starts[i + 1];
     if (e0 == e1) return;
@@ -437,22 +481,13 @@ __global__ void __launch_bounds__(M2L_THREADS) k3_m2l(KD kd, TD t, const int *tt
     const int tn = kd.tabn;
     const int bx = t.coords[Bx], by = t.coords[(size_t)t.nboxes + Bx],
               bz = t.coords[2 * (size_t)t.nboxes + Bx];
-    constexpr int MAXO = ((MAXP + 1) * (MAXP + 1) + M2L_THREADS - 1) / M2L_THREADS;
-    cplx acc[MAXO];
+    cplx acc[NO][NRC];
 #pragma unroll
-    for (int r = 0; r < MAXO; ++r) acc[r] = {0, 0};
+    for (int r = 0; r < NO; ++r)
+#pragma unroll
+        for (int q = 0; q < NRC; ++q) acc[r][q] = {0, 0};
     for (int e = e0; e < e1; ++e) {
         const int A = lists[e];
-        const double *a = mpole + ecoef_off(kd, A, lv);
-        __syncthreads();
-        for (int q = threadIdx.x; q < nc; q += M2L_THREADS) {
-            int n, m;
-            nm_of(q, n, m);
-            double s = 1.0;
-            for (int k = 0; k < n; ++k) s *= ih;
-            stc(Ms + 2 * q, s * ldc(a + 2 * q));
-        }
-        __syncthreads();
         const int dx = bx - t.coords[A] + W;
         const int dy = by - t.coords[(size_t)t.nboxes + A] + W;
         const int dz = bz - t.coords[2 * (size_t)t.nboxes + A] + W;
@@ -460,42 +495,58 @@ __global__ void __launch_bounds__(M2L_THREADS) k3_m2l(KD kd, TD t, const int *tt
         if ((unsigned)dx >= (unsigned)S1 || (unsigned)dy >= (unsigned)S1 ||
             (unsigned)dz >= (unsigned)S1)
             continue;
+        const double *a = mpole + ecoef_off(kd, A, lv);
+        __syncthreads();
+        for (int qq = threadIdx.x; qq < nc; qq += M2L_THREADS) {
+            int n, m;
+            nm_of(qq, n, m);
+            double s = 1.0;
+            for (int k = 0; k < n; ++k) s *= ih;
+#pragma unroll
+            for (int q = 0; q < NRC; ++q)
+                stc(Ms + 2 * (qq * NRC + q),
+                    q < kd.rc ? s * ldcn(kd, a, qq, kd.r0 + q) : cplx{0, 0});
+        }
+        __syncthreads();
         const double *T = kd.tab_m2l + 2L * ((dx * S1 + dy) * S1 + dz) * tn * tn;
 #pragma unroll
-        for (int r = 0; r < MAXO; ++r) {
-            const int q = threadIdx.x + r * M2L_THREADS;
-            if (q < nc) {
+        for (int r = 0; r < NO; ++r) {
+            const int qo = threadIdx.x + r * M2L_THREADS;
+            if (qo < nc) {
                 int k, l;
-                nm_of(q, k, l);
-                cplx s = {0, 0};
+                nm_of(qo, k, l);
                 for (int n = 0; n <= p; ++n) {
                     const int N = n + k;
                     for (int m = -n; m <= n; ++m) {
                         const int M = m + l;
                         if (M < -N || M > N) continue;
-                        s += ldc(Ms + 2 * IDX(n, m)) * ldc(T + 2 * IDX(N, M));
+                        const cplx tv = ldc(T + 2 * IDX(N, M));
+#pragma unroll
+                        for (int q = 0; q < NRC; ++q)
+                            acc[r][q] += ldc(Ms + 2 * (IDX(n, m) * NRC + q)) * tv;
                     }
                 }
-                acc[r] += s;
             }
         }
     }
 #pragma unroll
-    for (int r = 0; r < MAXO; ++r) {
-        const int q = threadIdx.x + r * M2L_THREADS;
-        if (q < nc) {
+    for (int r = 0; r < NO; ++r) {
+        const int qo = threadIdx.x + r * M2L_THREADS;
+        if (qo < nc) {
             int k, l;
-            nm_of(q, k, l);
+            nm_of(qo, k, l);
             double s = (k & 1) ? -ih : ih;  // (-1)^k h^{-k-1}
             for (int kk = 0; kk < k; ++kk) s *= ih;
-            double *o = local + ecoef_off(kd, Bx, lv) + 2 * q;
-            stc(o, ldc(o) + s * acc[r]);
+            cplx v[NRC];
+#pragma unroll
+            for (int q = 0; q < NRC; ++q) v[q] = s * acc[r][q];
+            store_add<NRC>(kd, local + ecoef_off(kd, Bx, lv) + 2L * qo * kd.nr, v);
         }
     }
 }
 
 // P2P (MODE 0, list 1), M2P (1, list 3), L2P (2) for non-QBX targets
-template <int MODE>
+template <int MODE, int NRC>
 __global__ void __launch_bounds__(64) k3_eval_targets(KD kd, TD t, int ntb, const int *tboxes,
                                                       const int *starts, const int *lists,
                                                       const int *fstart, const int *fcount,
@@ -515,10 +566,14 @@ __global__ void __launch_bounds__(64) k3_eval_targets(KD kd, TD t, int ntb, cons
         const v3 c = boxc(t, B);
         for (int f = f0 + lane; f < f0 + nf; f += 64) {
             const int ti = unfilt[f];
-            const cplx v = INV4PI * sum_regular_conj(expn + ecoef_off(kd, B, lv), kd.order[lv],
-                                                     sub(ld3(tgt, nt, ti), c));
-            double *o = pot + 2 * (size_t)ti;
-            stc(o, ldc(o) + v);
+            cplx v[NRC];
+#pragma unroll
+            for (int q = 0; q < NRC; ++q) v[q] = {0, 0};
+            sum_regular_conj_n<NRC>(kd, expn + ecoef_off(kd, B, lv), kd.order[lv],
+                                    sub(ld3(tgt, nt, ti), c), v);
+#pragma unroll
+            for (int q = 0; q < NRC; ++q) v[q] = INV4PI * v[q];
+            store_add<NRC>(kd, pot + 2 * (size_t)ti * kd.nr, v);
         }
         return;
     }
@@ -530,7 +585,9 @@ __global__ void __launch_bounds__(64) k3_eval_targets(KD kd, TD t, int ntb, cons
     for (int base = 0; base < nf; base += tpr) {
         const bool active = base + tsub < nf;
         int ti = 0;
-        cplx acc = {0, 0};
+        cplx acc[NRC];
+#pragma unroll
+        for (int q = 0; q < NRC; ++q) acc[q] = {0, 0};
         if (active) {
             ti = unfilt[f0 + base + tsub];
             const v3 tp = ld3(tgt, nt, ti);
@@ -540,52 +597,61 @@ __global__ void __launch_bounds__(64) k3_eval_targets(KD kd, TD t, int ntb, cons
                     const int s0 = t.sstart[Sb], s1 = s0 + t.scount[Sb];
                     for (int s = s0 + sub_; s < s1; s += G) {
                         const v3 nu = kd.dip ? ld3(nrm, ns, s) : v3{0, 0, 0};
-                        acc += p2p_pair(kd, ld3(src, ns, s), ldc(str + 2 * s), nu, tp);
+                        const double g = p2p_value(kd, ld3(src, ns, s), nu, tp);
+#pragma unroll
+                        for (int q = 0; q < NRC; ++q)
+                            if (q < kd.rc) acc[q] += g * ldq(kd, str, s, kd.r0 + q);
                     }
                 }
             } else {
                 for (int e = e0 + sub_; e < e1; e += G) {
                     const int D = lists[e];
                     const int lv = t.level[D];
-                    acc += INV4PI * sum_irregular(expn + ecoef_off(kd, D, lv), kd.order[lv],
-                                                  sub(tp, boxc(t, D)));
+                    sum_irregular_n<NRC>(kd, expn + ecoef_off(kd, D, lv), kd.order[lv],
+                                         sub(tp, boxc(t, D)), acc);
                 }
+#pragma unroll
+                for (int q = 0; q < NRC; ++q) acc[q] = INV4PI * acc[q];
             }
         }
-        for (int m = G / 2; m >= 1; m >>= 1) {
-            acc.x += __shfl_xor(acc.x, m);
-            acc.y += __shfl_xor(acc.y, m);
-        }
-        if (active && sub_ == 0) {
-            double *o = pot + 2 * (size_t)ti;
-            stc(o, ldc(o) + acc);
-        }
+#pragma unroll
+        for (int q = 0; q < NRC; ++q)
+            for (int m = G / 2; m >= 1; m >>= 1) {
+                acc[q].x += __shfl_xor(acc[q].x, m);
+                acc[q].y += __shfl_xor(acc[q].y, m);
+            }
+        if (active && sub_ == 0) store_add<NRC>(kd, pot + 2 * (size_t)ti * kd.nr, acc);
     }
 }
 
 // ---- QBX stages
-template <int PQ>
+template <int PQ, int NRC>
 struct QAcc {
-    cplx v[(PQ + 1) * (PQ + 1)];
+    cplx v[NRC][(PQ + 1) * (PQ + 1)];
 };
 
-template <int PQ>
-__device__ void qbx_store(double *qexp, int c, const QAcc<PQ> &A, int accumulate)
+template <int PQ, int NRC>
+__device__ void qbx_store(const KD &kd, double *qexp, int c, const QAcc<PQ, NRC> &A,
+                          int accumulate)
 {
     constexpr int NQ = (PQ + 1) * (PQ + 1);
-    double *o = qexp + 2 * (size_t)c * NQ;
+    double *o = qexp + 2 * (size_t)c * NQ * kd.nr;
 #pragma unroll
-    for (int i = 0; i < NQ; ++i) {
-        cplx v = A.v[i];
-        if (accumulate) v += ldc(o + 2 * i);
-        stc(o + 2 * i, v);
-    }
+    for (int i = 0; i < NQ; ++i)
+#pragma unroll
+        for (int q = 0; q < NRC; ++q)
+            if (q < kd.rc) {
+                double *p = o + 2 * ((long)i * kd.nr + kd.r0 + q);
+                cplx v = A.v[q][i];
+                if (accumulate) v += ldc(p);
+                stc(p, v);
+            }
 }
 
 constexpr int P2Q_LANES = 16;
 
 // P2QBXL: 16 lanes per global center over its list-1 sources
-template <int PQ>
+template <int PQ, int NRC>
 __global__ void __launch_bounds__(256) k3_p2qbxl(KD kd, TD t, int ng, const int *gcent,
                                                  const int *c2tb, const double *ctr, int nc,
                                                  const int *l1s, const int *l1l,
@@ -596,9 +662,11 @@ __global__ void __launch_bounds__(256) k3_p2qbxl(KD kd, TD t, int ng, const int 
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int g = gid / P2Q_LANES, lane = gid % P2Q_LANES;
     const bool active = g < ng;
-    QAcc<PQ> A;
+    QAcc<PQ, NRC> A;
 #pragma unroll
-    for (int i = 0; i < NQ; ++i) A.v[i] = {0, 0};
+    for (int q = 0; q < NRC; ++q)
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) A.v[q][i] = {0, 0};
     int c = 0;
     if (active) {
         c = gcent[g];
@@ -610,40 +678,48 @@ __global__ void __launch_bounds__(256) k3_p2qbxl(KD kd, TD t, int ng, const int 
             const int s0 = t.sstart[Sb], s1 = s0 + t.scount[Sb];
             for (int s = s0 + lane; s < s1; s += P2Q_LANES) {
                 const v3 d = sub(ld3(src, ns, s), cc);
-                const cplx q = ldc(str + 2 * s);
+                cplx qs[NRC];
+#pragma unroll
+                for (int q = 0; q < NRC; ++q)
+                    qs[q] = q < kd.rc ? ldq(kd, str, s, kd.r0 + q) : cplx{0, 0};
                 if (!kd.dip) {
                     cplx T[NQ];
                     the_table<PQ>(d, T);
 #pragma unroll
-                    for (int i = 0; i < NQ; ++i) A.v[i] += q * T[i];
+                    for (int i = 0; i < NQ; ++i)
+#pragma unroll
+                        for (int q = 0; q < NRC; ++q) A.v[q][i] += qs[q] * T[i];
                 } else {
                     constexpr int P1 = PQ + 1;
                     cplx T[(P1 + 1) * (P1 + 1)];
                     the_table<P1>(d, T);
                     const v3 nu = ld3(nrm, ns, s);
-                    const cplx qnm = q * cplx{nu.x, -nu.y}, qnp = q * cplx{nu.x, nu.y};
-                    const double qz = -nu.z;
+                    const cplx nmn = {nu.x, -nu.y}, npl = {nu.x, nu.y};
 #pragma unroll
                     for (int n = 0; n <= PQ; ++n)
 #pragma unroll
                         for (int m = -n; m <= n; ++m) {
-                            cplx g2 = (qz * T[IDX(n + 1, m)]) * q;
-                            g2 += 0.5 * (qnm * T[IDX(n + 1, m + 1)] - qnp * T[IDX(n + 1, m - 1)]);
-                            A.v[IDX(n, m)] += g2;
+                            const cplx g2 = (-nu.z) * T[IDX(n + 1, m)] +
+                                            0.5 * (nmn * T[IDX(n + 1, m + 1)] -
+                                                   npl * T[IDX(n + 1, m - 1)]);
+#pragma unroll
+                            for (int q = 0; q < NRC; ++q) A.v[q][IDX(n, m)] += qs[q] * g2;
                         }
                 }
             }
         }
     }
 #pragma unroll
-    for (int i = 0; i < NQ; ++i) {
+    for (int q = 0; q < NRC; ++q)
 #pragma unroll
-        for (int m = P2Q_LANES / 2; m >= 1; m >>= 1) {
-            A.v[i].x += __shfl_xor(A.v[i].x, m, P2Q_LANES);
-            A.v[i].y += __shfl_xor(A.v[i].y, m, P2Q_LANES);
+        for (int i = 0; i < NQ; ++i) {
+#pragma unroll
+            for (int m = P2Q_LANES / 2; m >= 1; m >>= 1) {
+                A.v[q][i].x += __shfl_xor(A.v[q][i].x, m, P2Q_LANES);
+                A.v[q][i].y += __shfl_xor(A.v[q][i].y, m, P2Q_LANES);
+            }
         }
-    }
-    if (active && lane == 0) qbx_store<PQ>(qexp, c, A, accumulate);
+    if (active && lane == 0) qbx_store<PQ, NRC>(kd, qexp, c, A, accumulate);
 }
 
 constexpr int M2Q_LANES = 16;
@@ -662,7 +738,7 @@ __device__ __forceinline__ void wave_sync()
 // M2QBXL: 16 lanes per global center; per list-3 box the irregular harmonics
 // T_N^M(c - b), N <= p_b + PQ, are built in LDS (one column m >= 0 per lane),
 // then lane j contracts output coefficients j, j + 16, ...
-template <int PQ>
+template <int PQ, int NRC>
 __global__ void __launch_bounds__(256) k3_m2qbxl(KD kd, TD t, int ng, const int *gcent,
                                                  const int *c2tb, const double *ctr, int nc,
                                                  const int *l3s, const int *l3l,
@@ -681,9 +757,11 @@ __global__ void __launch_bounds__(256) k3_m2qbxl(KD kd, TD t, int ng, const int 
     const int c = active ? gcent[g] : 0;
     const int it = active ? c2tb[c] : 0;
     const v3 cc = active ? ld3(ctr, nc, c) : v3{0, 0, 0};
-    cplx acc[NO];
+    cplx acc[NO][NRC];
 #pragma unroll
-    for (int r = 0; r < NO; ++r) acc[r] = {0, 0};
+    for (int r = 0; r < NO; ++r)
+#pragma unroll
+        for (int q = 0; q < NRC; ++q) acc[r][q] = {0, 0};
     const int e0 = active ? l3s[it] : 0, e1 = active ? l3s[it + 1] : 0;
     for (int e = e0; e < e1; ++e) {
         const int D = l3l[e];
@@ -714,36 +792,41 @@ __global__ void __launch_bounds__(256) k3_m2qbxl(KD kd, TD t, int ng, const int 
         const double *a = mpole + ecoef_off(kd, D, lv);
 #pragma unroll
         for (int r = 0; r < NO; ++r) {
-            const int q = lane + r * M2Q_LANES;
-            if (q < NQ) {
+            const int qo = lane + r * M2Q_LANES;
+            if (qo < NQ) {
                 int k, l;
-                nm_of(q, k, l);
-                cplx s = {0, 0};
+                nm_of(qo, k, l);
+                cplx s[NRC];
+#pragma unroll
+                for (int q = 0; q < NRC; ++q) s[q] = {0, 0};
                 for (int n = 0; n <= p1; ++n) {
                     const int N = n + k;
                     for (int m = -n; m <= n; ++m) {
                         const int M = m + l;
                         if (M < -N || M > N) continue;
-                        s += ldc(a + 2 * IDX(n, m)) * ldc(T + 2 * IDX(N, M));
+                        const cplx tv = ldc(T + 2 * IDX(N, M));
+#pragma unroll
+                        for (int q = 0; q < NRC; ++q)
+                            if (q < kd.rc) s[q] += ldcn(kd, a, IDX(n, m), kd.r0 + q) * tv;
                     }
                 }
-                acc[r] += (k & 1) ? -s : s;
+#pragma unroll
+                for (int q = 0; q < NRC; ++q) acc[r][q] += (k & 1) ? -s[q] : s[q];
             }
         }
         wave_sync();
     }
     if (active) {
-        double *o = qexp + 2 * (size_t)c * NQ;
 #pragma unroll
         for (int r = 0; r < NO; ++r) {
-            const int q = lane + r * M2Q_LANES;
-            if (q < NQ) stc(o + 2 * q, ldc(o + 2 * q) + acc[r]);
+            const int qo = lane + r * M2Q_LANES;
+            if (qo < NQ) store_add<NRC>(kd, qexp + 2 * ((size_t)c * NQ + qo) * kd.nr, acc[r]);
         }
     }
 }
 
 // L2QBXL: one thread per global center
-template <int PQ>
+template <int PQ, int NRC>
 __global__ void k3_l2qbxl(KD kd, TD t, int ng, const int *gcent, const int *c2tb,
                           const int *tboxes, const double *ctr, int nc, const double *local,
                           double *qexp)
@@ -757,9 +840,11 @@ __global__ void k3_l2qbxl(KD kd, TD t, int ng, const int *gcent, const int *c2tb
     const int p1 = kd.order[lv];
     const v3 d = sub(ld3(ctr, nc, c), boxc(t, Bx));
     const double *L = local + ecoef_off(kd, Bx, lv);
-    QAcc<PQ> A;
+    QAcc<PQ, NRC> A;
 #pragma unroll
-    for (int i = 0; i < NQ; ++i) A.v[i] = {0, 0};
+    for (int q = 0; q < NRC; ++q)
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) A.v[q][i] = {0, 0};
     const double r2 = d.x * d.x + d.y * d.y + d.z * d.z;
     const cplx hw = {-0.5 * d.x, -0.5 * d.y};
     cplx ymm = {1, 0};
@@ -782,18 +867,24 @@ __global__ void k3_l2qbxl(KD kd, TD t, int ng, const int *gcent, const int *c2tb
                 for (int b = -a; b <= a; ++b) {
                     const int n = a + j;
                     const int m1 = b + i;
-                    if (m1 >= -n && m1 <= n) A.v[IDX(a, b)] += ldc(L + 2 * IDX(n, m1)) * yc;
                     const int m2 = b - i;
-                    if (i > 0 && m2 >= -n && m2 <= n)
-                        A.v[IDX(a, b)] += ldc(L + 2 * IDX(n, m2)) * yn;
+#pragma unroll
+                    for (int q = 0; q < NRC; ++q) {
+                        if (q >= kd.rc) continue;
+                        if (m1 >= -n && m1 <= n)
+                            A.v[q][IDX(a, b)] += ldcn(kd, L, IDX(n, m1), kd.r0 + q) * yc;
+                        if (i > 0 && m2 >= -n && m2 <= n)
+                            A.v[q][IDX(a, b)] += ldcn(kd, L, IDX(n, m2), kd.r0 + q) * yn;
+                    }
                 }
             }
         }
     }
-    qbx_store<PQ>(qexp, c, A, 1);
+    qbx_store<PQ, NRC>(kd, qexp, c, A, 1);
 }
 
 // QBXL2P: one thread per (center, target) entry
+template <int NRC>
 __global__ void k3_qbxl2p(KD kd, int nctt, const int *ctt_lists, const int *tree_ttc,
                           const double *tgt, int nt, const double *ctr, int nc,
                           const double *qexp, double *pot)
@@ -803,8 +894,13 @@ __global__ void k3_qbxl2p(KD kd, int nctt, const int *ctt_lists, const int *tree
     const int ti = ctt_lists[j];
     const int c = tree_ttc[ti];
     const v3 d = sub(ld3(tgt, nt, ti), ld3(ctr, nc, c));
-    stc(pot + 2 * (size_t)ti,
-        INV4PI * sum_regular_conj(qexp + 2 * (size_t)c * kd.ncq, kd.pq, d));
+    cplx v[NRC];
+#pragma unroll
+    for (int q = 0; q < NRC; ++q) v[q] = {0, 0};
+    sum_regular_conj_n<NRC>(kd, qexp + 2 * (size_t)c * kd.ncq * kd.nr, kd.pq, d, v);
+#pragma unroll
+    for (int q = 0; q < NRC; ++q)
+        if (q < kd.rc) stc(pot + 2 * ((size_t)ti * kd.nr + kd.r0 + q), INV4PI * v[q]);
 }
 
 // ---------------------------------------------------------------------------
@@ -863,6 +959,7 @@ static int make_kd(const qbx_kernel_desc *d, const qbx_tree *T, KD *kd, int nsep
     QBX_CHECK(d->qbx_order >= 0 && d->qbx_order <= MAXPQ, "3D qbx_order must be in [0, %d]",
               MAXPQ);
     QBX_CHECK(d->level_orders && d->nlevels >= 1, "level_orders missing");
+    QBX_CHECK(d->nrhs >= 0, "nrhs must be >= 0 (0 means 1)");
     if (T) {
         QBX_CHECK(T->dim == 3, "3D kernels need a 3D tree");
         QBX_CHECK(T->nlevels <= FML, "tree has too many levels (%d)", T->nlevels);
@@ -874,6 +971,9 @@ static int make_kd(const qbx_kernel_desc *d, const qbx_tree *T, KD *kd, int nsep
     kd->pq = d->qbx_order;
     kd->nsep = nsep;
     kd->W = 2 * nsep + 1;
+    kd->nr = d->nrhs > 0 ? d->nrhs : 1;
+    kd->r0 = 0;
+    kd->rc = 1;
     const int nl = T ? T->nlevels : std::min(d->nlevels, FML);
     kd->nlevels = nl;
     long coef = 0;
@@ -965,21 +1065,50 @@ static Flat make_flat(const KD &kd, const std::vector<int> &level_starts)
     return f;
 }
 
-#define DISPATCH3_PQ(pq, KERNEL, ...)                                                   \
+// Run LAUNCH(N) over the right-hand-side chunks: N = 1 for a single
+// right-hand side, CH otherwise; kd.r0 / kd.rc are set per chunk.
+#define FOR_RHS_CHUNKS3(kd, CH, LAUNCH)                                                 \
+    do {                                                                                 \
+        if ((kd).nr == 1) {                                                              \
+            (kd).r0 = 0;                                                                 \
+            (kd).rc = 1;                                                                 \
+            LAUNCH(1);                                                                   \
+        } else {                                                                         \
+            for (int r0_ = 0; r0_ < (kd).nr; r0_ += (CH)) {                             \
+                (kd).r0 = r0_;                                                           \
+                (kd).rc = std::min((CH), (kd).nr - r0_);                                 \
+                LAUNCH(CH);                                                              \
+            }                                                                            \
+        }                                                                                \
+    } while (0)
+
+#define DISPATCH3_PQ_NR(pq, NR, KERNEL, ...)                                            \
     switch (pq) {                                                                        \
-    case 0: hipLaunchKernelGGL(KERNEL<0>, __VA_ARGS__); break;                           \
-    case 1: hipLaunchKernelGGL(KERNEL<1>, __VA_ARGS__); break;                           \
-    case 2: hipLaunchKernelGGL(KERNEL<2>, __VA_ARGS__); break;                           \
-    case 3: hipLaunchKernelGGL(KERNEL<3>, __VA_ARGS__); break;                           \
-    case 4: hipLaunchKernelGGL(KERNEL<4>, __VA_ARGS__); break;                           \
-    case 5: hipLaunchKernelGGL(KERNEL<5>, __VA_ARGS__); break;                           \
-    case 6: hipLaunchKernelGGL(KERNEL<6>, __VA_ARGS__); break;                           \
-    case 7: hipLaunchKernelGGL(KERNEL<7>, __VA_ARGS__); break;                           \
-    case 8: hipLaunchKernelGGL(KERNEL<8>, __VA_ARGS__); break;                           \
+    case 0: hipLaunchKernelGGL((KERNEL<0, NR>), __VA_ARGS__); break;                     \
+    case 1: hipLaunchKernelGGL((KERNEL<1, NR>), __VA_ARGS__); break;                     \
+    case 2: hipLaunchKernelGGL((KERNEL<2, NR>), __VA_ARGS__); break;                     \
+    case 3: hipLaunchKernelGGL((KERNEL<3, NR>), __VA_ARGS__); break;                     \
+    case 4: hipLaunchKernelGGL((KERNEL<4, NR>), __VA_ARGS__); break;                     \
+    case 5: hipLaunchKernelGGL((KERNEL<5, NR>), __VA_ARGS__); break;                     \
+    case 6: hipLaunchKernelGGL((KERNEL<6, NR>), __VA_ARGS__); break;                     \
+    case 7: hipLaunchKernelGGL((KERNEL<7, NR>), __VA_ARGS__); break;                     \
+    case 8: hipLaunchKernelGGL((KERNEL<8, NR>), __VA_ARGS__); break;                     \
     default: set_error("qbx_order out of range"); return QBX_ERR_INVALID;               \
     }
 
-static inline long exp_bytes(const KD &kd) { return 16L * kd.lcoef[kd.nlevels]; }
+#define DISPATCH3_NO_NR(no, NR, KERNEL, ...)                                            \
+    switch (no) {                                                                        \
+    case 1: hipLaunchKernelGGL((KERNEL<1, NR>), __VA_ARGS__); break;                     \
+    case 2: hipLaunchKernelGGL((KERNEL<2, NR>), __VA_ARGS__); break;                     \
+    case 3: hipLaunchKernelGGL((KERNEL<3, NR>), __VA_ARGS__); break;                     \
+    case 4: hipLaunchKernelGGL((KERNEL<4, NR>), __VA_ARGS__); break;                     \
+    case 5: hipLaunchKernelGGL((KERNEL<5, NR>), __VA_ARGS__); break;                     \
+    case 6: hipLaunchKernelGGL((KERNEL<6, NR>), __VA_ARGS__); break;                     \
+    case 7: hipLaunchKernelGGL((KERNEL<7, NR>), __VA_ARGS__); break;                     \
+    default: set_error("M2L: order out of range"); return QBX_ERR_INVALID;              \
+    }
+
+static inline long exp_bytes(const KD &kd) { return 16L * kd.lcoef[kd.nlevels] * kd.nr; }
 
 }  // namespace f3
 
@@ -989,14 +1118,14 @@ long f3_expansion_size(const qbx_kernel_desc *d, const qbx_tree *T)
 {
     KD kd;
     if (!T || make_kd(d, T, &kd) != QBX_OK) return -1;
-    return kd.lcoef[kd.nlevels];
+    return kd.lcoef[kd.nlevels] * kd.nr;
 }
 
 int f3_expansion_level_offsets(const qbx_kernel_desc *d, const qbx_tree *T, long *offsets)
 {
     KD kd;
     QBX_TRY(make_kd(d, T, &kd));
-    for (int l = 0; l <= kd.nlevels; ++l) offsets[l] = kd.lcoef[l];
+    for (int l = 0; l <= kd.nlevels; ++l) offsets[l] = kd.lcoef[l] * kd.nr;
     return QBX_OK;
 }
 
@@ -1018,9 +1147,14 @@ int f3_form_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_
     QBX_HIP(hipMemsetAsync(mpole, 0, exp_bytes(kd), st));
     const Flat fl = make_flat(kd, R->h_level_start_source_box_nrs);
     const long nthr = fl.tstart[fl.nl];
-    if (nthr > 0)
-        hipLaunchKernelGGL(k3_p2m, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, make_td(T), fl,
-                           R->source_boxes.p, T->sources.p, T->nsources, str, nrm, mpole);
+    if (nthr > 0) {
+        const TD td = make_td(T);
+#define L3_P2M(N)                                                                          \
+    hipLaunchKernelGGL(k3_p2m<N>, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, td, fl,     \
+                       R->source_boxes.p, T->sources.p, T->nsources, str, nrm, mpole)
+        FOR_RHS_CHUNKS3(kd, NRC_S, L3_P2M);
+#undef L3_P2M
+    }
     QBX_HIP(hipGetLastError());
     return QBX_OK;
 }
@@ -1032,13 +1166,18 @@ int f3_coarsen_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const q
     QBX_TRY(make_kd(d, T, &kd, R->nsep));
     QBX_TRY(ensure_tables(ctx, &kd));
     hipStream_t st = S(stream);
+    const TD td = make_td(T);
     for (int l = T->nlevels - 2; l >= 0; --l) {
         const int i0 = R->h_level_start_source_parent_box_nrs[l];
         const int i1 = R->h_level_start_source_parent_box_nrs[l + 1];
         const long nthr = (long)(i1 - i0) * kd.ncl[l];
-        if (nthr > 0)
-            hipLaunchKernelGGL(k3_m2m, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, make_td(T),
-                               l, i0, i1, R->source_parent_boxes.p, mpole);
+        if (nthr > 0) {
+#define L3_M2M(N)                                                                          \
+    hipLaunchKernelGGL(k3_m2m<N>, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, td, l, i0,  \
+                       i1, R->source_parent_boxes.p, mpole)
+            FOR_RHS_CHUNKS3(kd, NRC_S, L3_M2M);
+#undef L3_M2M
+        }
     }
     QBX_HIP(hipGetLastError());
     return QBX_OK;
@@ -1052,21 +1191,27 @@ int f3_eval_targets(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_tra
     QBX_TRY(make_kd(d, T, &kd));
     hipStream_t st = S(stream);
     if (!accumulate)
-        QBX_HIP(hipMemsetAsync(pot, 0, sizeof(double) * 2 * (size_t)T->ntargets, st));
+        QBX_HIP(hipMemsetAsync(pot, 0, sizeof(double) * 2 * (size_t)T->ntargets * kd.nr, st));
     const int ntb = R->ntarget_boxes;
     if (ntb == 0 || G->nfiltered == 0) return QBX_OK;
     const TD td = make_td(T);
     const int *starts = mode == 1 ? R->l3_starts.p : R->l1_starts.p;
     const int *lists = mode == 1 ? R->l3_lists.p : R->l1_lists.p;
-#define EVAL3_LAUNCH(M)                                                                    \
-    hipLaunchKernelGGL(k3_eval_targets<M>, dim3(ntb), dim3(64), 0, st, kd, td, ntb,        \
+#define EVAL3_LAUNCH(M, N)                                                                 \
+    hipLaunchKernelGGL((k3_eval_targets<M, N>), dim3(ntb), dim3(64), 0, st, kd, td, ntb,   \
                        R->target_boxes.p, starts, lists, G->nqbtl_box_target_starts.p,    \
                        G->nqbtl_box_target_counts_nonchild.p, G->nqbtl_unfiltered.p,      \
                        T->targets.p, T->ntargets, T->sources.p, T->nsources, str, nrm,    \
                        expn, pot)
-    if (mode == 0) EVAL3_LAUNCH(0);
-    else if (mode == 1) EVAL3_LAUNCH(1);
-    else EVAL3_LAUNCH(2);
+#define EV30(N) EVAL3_LAUNCH(0, N)
+#define EV31(N) EVAL3_LAUNCH(1, N)
+#define EV32(N) EVAL3_LAUNCH(2, N)
+    if (mode == 0) FOR_RHS_CHUNKS3(kd, NRC_S, EV30);
+    else if (mode == 1) FOR_RHS_CHUNKS3(kd, NRC_S, EV31);
+    else FOR_RHS_CHUNKS3(kd, NRC_S, EV32);
+#undef EV30
+#undef EV31
+#undef EV32
 #undef EVAL3_LAUNCH
     QBX_HIP(hipGetLastError());
     return QBX_OK;
@@ -1080,10 +1225,16 @@ int f3_multipole_to_local(qbx_ctx *ctx, void *stream, const qbx_tree *T, const q
     QBX_TRY(ensure_tables(ctx, &kd));
     hipStream_t st = S(stream);
     QBX_HIP(hipMemsetAsync(local, 0, exp_bytes(kd), st));
-    if (R->nttp_boxes > 0 && R->n2 > 0)
-        hipLaunchKernelGGL(k3_m2l, dim3(R->nttp_boxes), dim3(M2L_THREADS), 0, st, kd,
-                           make_td(T), R->ttp_boxes.p, R->l2_starts.p, R->l2_lists.p, mpole,
-                           local);
+    if (R->nttp_boxes > 0 && R->n2 > 0) {
+        const TD td = make_td(T);
+        const int no = ((kd.pmax + 1) * (kd.pmax + 1) + M2L_THREADS - 1) / M2L_THREADS;
+#define L3_M2L(N)                                                                          \
+    DISPATCH3_NO_NR(no, N, k3_m2l, dim3(R->nttp_boxes), dim3(M2L_THREADS),                 \
+                    sizeof(double) * 2 * (size_t)(kd.pmax + 1) * (kd.pmax + 1) * (N), st,  \
+                    kd, td, R->ttp_boxes.p, R->l2_starts.p, R->l2_lists.p, mpole, local)
+        FOR_RHS_CHUNKS3(kd, NRC_S, L3_M2L);
+#undef L3_M2L
+    }
     QBX_HIP(hipGetLastError());
     return QBX_OK;
 }
@@ -1099,10 +1250,15 @@ int f3_form_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav
     if (!accumulate) QBX_HIP(hipMemsetAsync(local, 0, exp_bytes(kd), st));
     const Flat fl = make_flat(kd, R->h_level_start_ttp_box_nrs);
     const long nthr = fl.tstart[fl.nl];
-    if (nthr > 0 && R->n4 > 0)
-        hipLaunchKernelGGL(k3_p2l, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, make_td(T), fl,
-                           R->ttp_boxes.p, R->l4_starts.p, R->l4_lists.p, T->sources.p,
-                           T->nsources, str, nrm, local);
+    if (nthr > 0 && R->n4 > 0) {
+        const TD td = make_td(T);
+#define L3_P2L(N)                                                                          \
+    hipLaunchKernelGGL(k3_p2l<N>, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, td, fl,     \
+                       R->ttp_boxes.p, R->l4_starts.p, R->l4_lists.p, T->sources.p,        \
+                       T->nsources, str, nrm, local)
+        FOR_RHS_CHUNKS3(kd, NRC_S, L3_P2L);
+#undef L3_P2L
+    }
     QBX_HIP(hipGetLastError());
     return QBX_OK;
 }
@@ -1114,12 +1270,17 @@ int f3_refine_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_tr
     QBX_TRY(make_kd(d, T, &kd, R->nsep));
     QBX_TRY(ensure_tables(ctx, &kd));
     hipStream_t st = S(stream);
+    const TD td = make_td(T);
     for (int l = 1; l < T->nlevels; ++l) {
         const int i0 = R->h_level_start_ttp_box_nrs[l], i1 = R->h_level_start_ttp_box_nrs[l + 1];
         const long nthr = (long)(i1 - i0) * kd.ncl[l];
-        if (nthr > 0)
-            hipLaunchKernelGGL(k3_l2l, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, make_td(T), l,
-                               i0, i1, R->ttp_boxes.p, local);
+        if (nthr > 0) {
+#define L3_L2L(N)                                                                          \
+    hipLaunchKernelGGL(k3_l2l<N>, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, td, l, i0,  \
+                       i1, R->ttp_boxes.p, local)
+            FOR_RHS_CHUNKS3(kd, NRC_S, L3_L2L);
+#undef L3_L2L
+        }
     }
     QBX_HIP(hipGetLastError());
     return QBX_OK;
@@ -1134,13 +1295,18 @@ int f3_form_global_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T,
     QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
     hipStream_t st = S(stream);
     if (!accumulate)
-        QBX_HIP(hipMemsetAsync(qexp, 0, sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq, st));
+        QBX_HIP(hipMemsetAsync(qexp, 0,
+                               sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq * kd.nr, st));
     const int ng = G->nglobal;
     if (ng == 0) return QBX_OK;
-    DISPATCH3_PQ(kd.pq, k3_p2qbxl, dim3(cdiv((long)ng * P2Q_LANES, 256)), dim3(256), 0, st, kd,
-                 make_td(T), ng, G->global_qbx_centers.p, G->center_to_target_box.p,
-                 G->centers.p, G->ncenters, R->l1_starts.p, R->l1_lists.p, T->sources.p,
-                 T->nsources, str, nrm, qexp, accumulate);
+    const TD td = make_td(T);
+#define L3_P2Q(N)                                                                          \
+    DISPATCH3_PQ_NR(kd.pq, N, k3_p2qbxl, dim3(cdiv((long)ng * P2Q_LANES, 256)), dim3(256), \
+                    0, st, kd, td, ng, G->global_qbx_centers.p, G->center_to_target_box.p, \
+                    G->centers.p, G->ncenters, R->l1_starts.p, R->l1_lists.p, T->sources.p, \
+                    T->nsources, str, nrm, qexp, accumulate)
+    FOR_RHS_CHUNKS3(kd, NRC_Q, L3_P2Q);
+#undef L3_P2Q
     QBX_HIP(hipGetLastError());
     return QBX_OK;
 }
@@ -1154,17 +1320,21 @@ int f3_translate_box_multipoles_to_qbx_local(qbx_ctx *ctx, void *stream, const q
     QBX_TRY(make_kd(d, T, &kd));
     hipStream_t st = S(stream);
     if (!accumulate)
-        QBX_HIP(hipMemsetAsync(qexp, 0, sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq, st));
+        QBX_HIP(hipMemsetAsync(qexp, 0,
+                               sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq * kd.nr, st));
     const int ng = G->nglobal;
     if (ng == 0 || R->n3 == 0) return QBX_OK;
     const int pn = kd.pmax + kd.pq;
     const size_t lds = sizeof(double) * 2 * (size_t)M2Q_CENTERS * (pn + 1) * (pn + 1);
     QBX_CHECK(lds <= 64 * 1024, "M2QBXL: orders too high for LDS staging (%d + %d)", kd.pmax,
               kd.pq);
-    // every block runs whole: the grid covers ng rounded up to whole centers
-    DISPATCH3_PQ(kd.pq, k3_m2qbxl, dim3(cdiv((long)ng * M2Q_LANES, 256)), dim3(256), lds, st, kd,
-                 make_td(T), ng, G->global_qbx_centers.p, G->center_to_target_box.p,
-                 G->centers.p, G->ncenters, R->l3_starts.p, R->l3_lists.p, mpole, qexp);
+    const TD td = make_td(T);
+#define L3_M2Q(N)                                                                          \
+    DISPATCH3_PQ_NR(kd.pq, N, k3_m2qbxl, dim3(cdiv((long)ng * M2Q_LANES, 256)), dim3(256), \
+                    lds, st, kd, td, ng, G->global_qbx_centers.p, G->center_to_target_box.p, \
+                    G->centers.p, G->ncenters, R->l3_starts.p, R->l3_lists.p, mpole, qexp)
+    FOR_RHS_CHUNKS3(kd, NRC_Q, L3_M2Q);
+#undef L3_M2Q
     QBX_HIP(hipGetLastError());
     return QBX_OK;
 }
@@ -1178,12 +1348,17 @@ int f3_translate_box_local_to_qbx_local(qbx_ctx *ctx, void *stream, const qbx_tr
     QBX_TRY(make_kd(d, T, &kd));
     hipStream_t st = S(stream);
     if (!accumulate)
-        QBX_HIP(hipMemsetAsync(qexp, 0, sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq, st));
+        QBX_HIP(hipMemsetAsync(qexp, 0,
+                               sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq * kd.nr, st));
     const int ng = G->nglobal;
     if (ng == 0) return QBX_OK;
-    DISPATCH3_PQ(kd.pq, k3_l2qbxl, dim3(cdiv(ng, 64)), dim3(64), 0, st, kd, make_td(T), ng,
-                 G->global_qbx_centers.p, G->center_to_target_box.p, R->target_boxes.p,
-                 G->centers.p, G->ncenters, local, qexp);
+    const TD td = make_td(T);
+#define L3_L2Q(N)                                                                          \
+    DISPATCH3_PQ_NR(kd.pq, N, k3_l2qbxl, dim3(cdiv(ng, 64)), dim3(64), 0, st, kd, td, ng,  \
+                    G->global_qbx_centers.p, G->center_to_target_box.p, R->target_boxes.p, \
+                    G->centers.p, G->ncenters, local, qexp)
+    FOR_RHS_CHUNKS3(kd, NRC_Q, L3_L2Q);
+#undef L3_L2Q
     QBX_HIP(hipGetLastError());
     return QBX_OK;
 }
@@ -1194,10 +1369,14 @@ int f3_eval_qbx_expansions(qbx_ctx *ctx, void *stream, const qbx_tree *T, const 
     KD kd;
     QBX_TRY(make_kd(d, T, &kd));
     hipStream_t st = S(stream);
-    if (G->nctt > 0)
-        hipLaunchKernelGGL(k3_qbxl2p, dim3(cdiv(G->nctt, 64)), dim3(64), 0, st, kd, G->nctt,
-                           G->ctt_lists.p, G->tree_ttc.p, T->targets.p, T->ntargets,
-                           G->centers.p, G->ncenters, qexp, pot);
+    if (G->nctt > 0) {
+#define L3_Q2P(N)                                                                          \
+    hipLaunchKernelGGL(k3_qbxl2p<N>, dim3(cdiv(G->nctt, 64)), dim3(64), 0, st, kd,         \
+                       G->nctt, G->ctt_lists.p, G->tree_ttc.p, T->targets.p, T->ntargets,  \
+                       G->centers.p, G->ncenters, qexp, pot)
+        FOR_RHS_CHUNKS3(kd, NRC_S, L3_Q2P);
+#undef L3_Q2P
+    }
     QBX_HIP(hipGetLastError());
     return QBX_OK;
 }

@@ -3,6 +3,11 @@
 Each wrangler method is one C-ABI call into libqbx_mi355x; ``drive_fmm``
 calls them in the reference's stage order (pytential/qbx/fmm.py:414-594).
 ``drive_fmm_fused`` runs the same sequence from a single library entry point.
+
+A wrangler built with ``nrhs > 1`` evaluates that many densities at once:
+strengths are ``[nsources, nrhs]``, potentials ``[ntargets, nrhs]`` and every
+expansion array carries the right-hand sides innermost.  With ``nrhs == 1``
+arrays keep the reference's one-density shapes.
 """
 from __future__ import annotations
 
@@ -20,7 +25,7 @@ class QBXExpansionWrangler:
     """Stages of the QBX FMM on one tree (pytential/qbx/fmm.py:118-396)."""
 
     def __init__(self, geo_data, kernel, k, source_kind, qbx_order, level_orders,
-                 output="pot", use_target_specific_qbx=False):
+                 output="pot", use_target_specific_qbx=False, nrhs=1):
         self.geo_data = geo_data
         self.tree = geo_data.tree()
         self.traversal = geo_data.traversal()
@@ -42,15 +47,25 @@ class QBXExpansionWrangler:
         code = {(2, "laplace"): _lib.LAPLACE2D, (2, "helmholtz"): _lib.HELMHOLTZ2D,
                 (3, "laplace"): _lib.LAPLACE3D, (3, "helmholtz"): _lib.HELMHOLTZ3D}
         self.kernel_code = code[(self.dim, kernel)]
+        if nrhs < 1:
+            raise ValueError("nrhs must be >= 1")
+        self.nrhs = int(nrhs)
         self.kd = KernelDesc(self.kernel_code, self.k.real,
                              _lib.SOURCE_S if source_kind == "S" else _lib.SOURCE_D,
-                             qbx_order, len(lo), lo.ctypes.data_as(ct.POINTER(ct.c_int)))
+                             qbx_order, len(lo), lo.ctypes.data_as(ct.POINTER(ct.c_int)),
+                             self.nrhs)
         self.source_kind = source_kind
         self.qbx_order = qbx_order
         # TSQBX replaces P2QBXL's list-1 part (qbx/fmmlib.py:73-108, 295-298)
         if use_target_specific_qbx and (self.dim != 3 or output != "pot"):
             raise ValueError("target-specific QBX needs a 3D geometry and potential output")
+        if use_target_specific_qbx and self.nrhs != 1:
+            raise NotImplementedError("target-specific QBX evaluates one density at a time")
         self.using_tsqbx = bool(use_target_specific_qbx)
+        self.expansion_size = None
+        if self.kernel_code == _lib.HELMHOLTZ3D:
+            # 3D Helmholtz: only the target-specific stage exists in the library
+            return
         self.expansion_size = _lib.lib().qbx_expansion_size(ct.byref(self.kd), self.tree.h)
         if self.expansion_size < 0:
             raise _lib.QBXError(1, _lib.lib().qbx_last_error().decode())
@@ -66,40 +81,43 @@ class QBXExpansionWrangler:
                            device="cuda")
 
     def expansions_to_dense(self, e):
-        """[nboxes][ncoeff(pmax)] view of level-compact expansions (2D
-        Helmholtz coefficient n at column n + pmax; 3D (n, m) at n*n + n + m),
-        for inspection and tests."""
+        """[nboxes][ncoeff(pmax)] (and [nrhs] when nrhs > 1) view of
+        level-compact expansions (2D Helmholtz coefficient n at column
+        n + pmax; 3D (n, m) at n*n + n + m), for inspection and tests."""
         e = e.detach().cpu().numpy()
         lo = self._lo[:self.tree.nlevels]
         pmax = int(max(lo))
         helm = self.kernel != "laplace"
-        if self.dim == 3:
-            ls = self.tree.level_start_box_nrs.cpu().numpy()
-            out = np.zeros((self.tree.nboxes, (pmax + 1) ** 2), np.complex128)
-            for lev in range(self.tree.nlevels):
-                nc = (int(lo[lev]) + 1) ** 2
-                blk = e[self.level_offsets[lev]:self.level_offsets[lev + 1]].reshape(-1, nc)
-                out[ls[lev]:ls[lev + 1], :nc] = blk
-            return out
-        ncf = 2 * pmax + 1 if helm else pmax + 1
+        nr = self.nrhs
         ls = self.tree.level_start_box_nrs.cpu().numpy()
-        out = np.zeros((self.tree.nboxes, ncf), np.complex128)
+        if self.dim == 3:
+            ncf = (pmax + 1) ** 2
+        else:
+            ncf = 2 * pmax + 1 if helm else pmax + 1
+        out = np.zeros((self.tree.nboxes, ncf, nr), np.complex128)
         for lev in range(self.tree.nlevels):
             p = int(lo[lev])
-            nc = 2 * p + 1 if helm else p + 1
-            blk = e[self.level_offsets[lev]:self.level_offsets[lev + 1]].reshape(-1, nc)
-            c0 = pmax - p if helm else 0
+            if self.dim == 3:
+                nc, c0 = (p + 1) ** 2, 0
+            else:
+                nc = 2 * p + 1 if helm else p + 1
+                c0 = pmax - p if helm else 0
+            blk = e[self.level_offsets[lev]:self.level_offsets[lev + 1]].reshape(-1, nc, nr)
             out[ls[lev]:ls[lev + 1], c0:c0 + nc] = blk
-        return out
+        return out[:, :, 0] if nr == 1 else out
 
     local_expansion_zeros = multipole_expansion_zeros
 
+    def _rhs_shape(self, *lead):
+        return lead if self.nrhs == 1 else lead + (self.nrhs,)
+
     def qbx_local_expansion_zeros(self):
-        return torch.zeros((self.geo_data.ncenters, self.ncoeff_qbx), dtype=torch.complex128,
-                           device="cuda")
+        return torch.zeros(self._rhs_shape(self.geo_data.ncenters, self.ncoeff_qbx),
+                           dtype=torch.complex128, device="cuda")
 
     def full_output_zeros(self):
-        return torch.zeros(self.tree.ntargets, dtype=torch.complex128, device="cuda")
+        return torch.zeros(self._rhs_shape(self.tree.ntargets), dtype=torch.complex128,
+                           device="cuda")
 
     output_zeros = full_output_zeros
 
@@ -108,14 +126,19 @@ class QBXExpansionWrangler:
 
     # ---- sources
     def reorder_sources(self, quad_density, weights, normals):
-        """strengths = weights * density in tree order (+ normals for D)."""
+        """strengths = weights * density in tree order (+ normals for D);
+        quad_density is [nsources] or [nsources, nrhs]."""
         dens = quad_density.to(torch.complex128).contiguous()
-        out = torch.empty(self.tree.nsources, dtype=torch.complex128, device="cuda")
+        if dens.numel() != self.tree.nsources * self.nrhs:
+            raise ValueError(f"density has {dens.numel()} values, expected "
+                             f"{self.tree.nsources} x {self.nrhs}")
+        out = torch.empty(self._rhs_shape(self.tree.nsources), dtype=torch.complex128,
+                          device="cuda")
         nout = None
         if normals is not None:
             nout = torch.empty_like(normals)
-        self._a("qbx_reorder_sources", self.tree.h, _p(dens), _p(weights), _p(normals),
-                _p(out), _p(nout))
+        self._a("qbx_reorder_sources", self.tree.h, self.nrhs, _p(dens), _p(weights),
+                _p(normals), _p(out), _p(nout))
         return out, nout
 
     # ---- point FMM stages
@@ -207,6 +230,8 @@ class QBXExpansionWrangler:
         Returns [ntargets] complex, or [3, ntargets] for the gradient."""
         if self.dim != 3:
             return self.full_output_zeros()
+        if self.nrhs != 1:
+            raise NotImplementedError("target-specific QBX evaluates one density at a time")
         nt = self.tree.ntargets
         grad = pot = None
         if self.output == "grad":
@@ -226,8 +251,9 @@ class QBXExpansionWrangler:
 
     def finalize_potentials(self, pot_tree):
         nuser = self.tree.ntargets - self.geo_data.ncenters
-        out = torch.empty(nuser, dtype=torch.complex128, device="cuda")
-        self._a("qbx_finalize_potentials", self.tree.h, self.geo.h, _p(pot_tree), _p(out))
+        out = torch.empty(self._rhs_shape(nuser), dtype=torch.complex128, device="cuda")
+        self._a("qbx_finalize_potentials", self.tree.h, self.geo.h, self.nrhs, _p(pot_tree),
+                _p(out))
         return out
 
 
@@ -260,7 +286,7 @@ def drive_fmm_fused(wrangler: QBXExpansionWrangler, strengths, normals, out=None
         raise ValueError("the fused driver has no target-specific QBX stage; use drive_fmm")
     nuser = w.tree.ntargets - w.geo_data.ncenters
     if out is None:
-        out = torch.empty(nuser, dtype=torch.complex128, device="cuda")
+        out = torch.empty(w._rhs_shape(nuser), dtype=torch.complex128, device="cuda")
     call("qbx_drive_fmm", w.ctx.h, _stream(), w.tree.h, w.traversal.h, w.geo.h,
          ct.byref(w.kd), _p(strengths), _p(normals), _p(out))
     return out

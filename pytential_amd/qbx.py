@@ -110,30 +110,37 @@ class QBXLayerPotentialSource:
                 for lev in range(tree.nlevels)]
 
     def quad_density(self, density):
-        """Stage-1 nodal density -> quad-stage-2 nodal values (GPU)."""
+        """Stage-1 nodal density ([ndofs] or [ndofs, nrhs]) -> quad-stage-2
+        nodal values (GPU)."""
         from ._lib import call
         from .geometry import _p, _stream
         d = self.density_discr
         dens = as_dev(density, torch.complex128) if torch.is_complex(
             torch.as_tensor(density)) else as_dev(density).to(torch.complex128)
-        out = torch.empty(d.nsources, dtype=torch.complex128, device="cuda")
+        if dens.shape[0] != d.ndofs:
+            raise ValueError(f"density has {dens.shape[0]} nodal values, expected {d.ndofs}")
+        nrhs = 1 if dens.dim() == 1 else dens.shape[1]
+        out = torch.empty((d.nsources,) + tuple(dens.shape[1:]), dtype=torch.complex128,
+                          device="cuda")
         call("qbx_upsample", self.ctx.h, _stream(), d.nelements, d.nodes_per_element,
-             d.fine_nodes_per_element,
+             d.fine_nodes_per_element, nrhs,
              _p(self.dev["upsample"]), _p(dens.contiguous()), _p(out))
         return out
 
-    def make_wrangler(self, geo_data, kernel, k, source_kind):
+    def make_wrangler(self, geo_data, kernel, k, source_kind, nrhs=1):
         tree = geo_data.tree()
         return _fmm.QBXExpansionWrangler(
             geo_data, kernel, k, source_kind, self.qbx_order, self.level_orders(kernel, k, tree),
-            use_target_specific_qbx=self._use_target_specific_qbx)
+            use_target_specific_qbx=self._use_target_specific_qbx, nrhs=nrhs)
 
     def exec_compute_potential(self, kernel, density, targets_and_sides, k=0.0,
                                source_kind="S", fused=True):
-        """Evaluate ``S`` or ``D`` of ``density`` (stage-1 nodal values) at each
-        ``(targets, qbx_forced_limit)`` pair; ``targets`` is ``"self"`` (the
-        stage-1 nodes) or a (2, n) array.  Returns one complex tensor per pair.
-        Raises RuntimeError("geometry has failed targets") like the reference."""
+        """Evaluate ``S`` or ``D`` of ``density`` (stage-1 nodal values,
+        ``[ndofs]`` or ``[ndofs, nrhs]`` for several densities in one
+        evaluation) at each ``(targets, qbx_forced_limit)`` pair; ``targets``
+        is ``"self"`` (the stage-1 nodes) or a (dim, n) array.  Returns one
+        tensor per pair ([n] or [n, nrhs]).  Target-association failure raises
+        QBXTargetAssociationFailedError like the reference."""
         if kernel not in ("laplace", "helmholtz"):
             raise ValueError(f"unknown kernel {kernel!r}")
         if source_kind not in ("S", "D"):
@@ -146,7 +153,23 @@ class QBXLayerPotentialSource:
                 raise ValueError("'avg' must be split into +1 and -1 before evaluation")
             sides.append((t, s))
         geo = self.qbx_fmm_geometry_data(tuple(sides))
-        w = self.make_wrangler(geo, kernel, k, source_kind)
+        dens = torch.as_tensor(density)
+        if kernel == "laplace" and self.ambient_dim == 2 and torch.is_complex(dens) and \
+                bool((dens.imag != 0).any()):
+            # the 2D Laplace expansions take real strengths: evaluate the real
+            # and imaginary parts as two right-hand sides (linearity)
+            two = torch.stack([dens.real, dens.imag], dim=-1)
+            outs = self.exec_compute_potential(kernel, two.reshape(dens.shape[0], -1),
+                                               targets_and_sides, k, source_kind, fused)
+            nr = dens.shape[1] if dens.dim() == 2 else 1
+            res = []
+            for o in outs:
+                o = o.reshape(o.shape[0], nr, 2)
+                c = torch.complex(o[..., 0], o[..., 1])
+                res.append(c if dens.dim() == 2 else c[:, 0])
+            return res
+        nrhs = 1 if dens.dim() == 1 else dens.shape[1]
+        w = self.make_wrangler(geo, kernel, k, source_kind, nrhs)
         qd = self.quad_density(density)
         normals = self.dev["quad_normals"] if source_kind == "D" else None
         strengths, nrm = w.reorder_sources(qd, self.dev["quad_weights"], normals)
