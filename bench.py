@@ -197,6 +197,37 @@ def run_gpu(args, cfg, orders_fn, label):
             "flops": flops, "result": out.cpu().numpy(), "geo": geo, "src": src}
 
 
+def run_batched(args, cfg, src, geo, nrhs):
+    """nrhs densities per evaluation on the same geometry and orders (the
+    first is the workload's own density): a step is one evaluation of all of
+    them through the single-call driver."""
+    import torch
+
+    from pytential_amd import fmm as F
+    rng = np.random.default_rng(77)
+    n = cfg.discr.ndofs
+    extra = rng.standard_normal((n, nrhs - 1)) + 1j * rng.standard_normal((n, nrhs - 1))
+    dens_np = np.concatenate([np.asarray(cfg.density, np.complex128)[:, None], extra], axis=1)
+    dens = torch.as_tensor(dens_np, dtype=torch.complex128, device="cuda")
+    w = src.make_wrangler(geo, cfg.kernel, cfg.k, cfg.source_kind, nrhs)
+
+    def step():
+        qd = src.quad_density(dens)
+        st, nrm = w.reorder_sources(qd, src.dev["quad_weights"], src.dev["quad_normals"])
+        return F.drive_fmm_fused(w, st, nrm)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    log(f"[batched nrhs={nrhs}] {dt * 1e3:.3f} ms/step")
+    return {"nrhs": nrhs, "step_s": dt, "result0": out[:, 0].cpu().numpy()}
+
+
 def cpu_baseline(args, cfg, orders, sample_idx):
     """The oracle on the host cores: one full evaluation (timed) plus the
     direct-QBX accuracy reference on a sample of targets."""
@@ -221,6 +252,134 @@ def cpu_baseline(args, cfg, orders, sample_idx):
             "fmm_result_sample": pot[sample_idx], "direct_sample": ref, "t_fmm": t_fmm}
 
 
+def sphere_densities(discr, nrhs, seed=1002):
+    """nrhs densities: real spherical harmonics Y_lm (l <= 6, in order) plus
+    N(0, 0.01) noise, seeded."""
+    from scipy.special import sph_harm_y
+    x = discr.nodes
+    r = np.linalg.norm(x, axis=0)
+    th, ph = np.arccos(np.clip(x[2] / r, -1, 1)), np.arctan2(x[1], x[0])
+    lm = [(l, m) for l in range(7) for m in range(-l, l + 1)]
+    rng = np.random.default_rng(seed)
+    cols = []
+    for i in range(nrhs):
+        l, m = lm[i % len(lm)]
+        y = sph_harm_y(l, abs(m), th, ph)
+        y = y.real if m >= 0 else y.imag
+        cols.append(y + 0.01 * rng.standard_normal(len(y)))
+    return np.stack(cols, axis=1)
+
+
+def main_sphere3d(args):
+    """3D Laplace single layer on the unit sphere, many densities in one
+    evaluation: icosahedron refined nsub times, order-4 triangles, qbx_order 3,
+    fmm_order 8, on-surface targets (side +1)."""
+    import torch
+
+    from pytential_amd import fmm as F
+    from pytential_amd.qbx import QBXLayerPotentialSource
+    from pytential_amd.surface import icosphere
+    t0 = time.perf_counter()
+    discr = icosphere(args.nsub, order=4, fine_order=args.fine3d)
+    dens_np = sphere_densities(discr, args.nrhs3d)
+    log(f"sphere: {discr.nelements} triangles, ndofs={discr.ndofs} "
+        f"nsources={discr.nsources} ({time.perf_counter() - t0:.1f}s)")
+    src = QBXLayerPotentialSource(discr, fine_order=args.fine3d, qbx_order=3, fmm_order=8)
+    geo = src.qbx_fmm_geometry_data((("self", +1),))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    geo.build_all()
+    torch.cuda.synchronize()
+    cold = time.perf_counter() - t0
+    tree = geo.tree()
+    log(f"geometry {cold:.3f}s nboxes={tree.nboxes} nlevels={tree.nlevels}")
+    nrhs = args.nrhs3d
+    dens = torch.as_tensor(dens_np, dtype=torch.float64, device="cuda")
+    w = src.make_wrangler(geo, "laplace", 0.0, "S", nrhs)
+
+    def step():
+        qd = src.quad_density(dens)
+        st, _ = w.reorder_sources(qd, src.dev["quad_weights"], None)
+        return F.drive_fmm_fused(w, st, None)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    res = out.real.cpu().numpy()
+    # plain: one density per evaluation
+    w1 = src.make_wrangler(geo, "laplace", 0.0, "S", 1)
+    d1 = dens[:, 0].contiguous()
+
+    def step1():
+        qd = src.quad_density(d1)
+        st, _ = w1.reorder_sources(qd, src.dev["quad_weights"], None)
+        return F.drive_fmm_fused(w1, st, None)
+
+    step1()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        out1 = step1()
+    torch.cuda.synchronize()
+    dt1 = (time.perf_counter() - t1) / args.steps
+    # accuracy: S Y_00 = Y_00 on the unit sphere, and direct QBX on a sample
+    err_eig = float(np.max(np.abs(res[:, 0] - dens_np[:, 0])) / np.max(np.abs(dens_np[:, 0])))
+    cpu = None
+    err_direct = None
+    if not args.no_cpu:
+        import oracle
+        oracle.build(with_ref=False)
+        nthreads = args.cpu_threads or os.cpu_count()
+        g = oracle.GeometryData(discr, discr.nodes, np.ones(discr.ndofs, np.int32),
+                                max_leaf_refine_weight=512)
+        rng = np.random.default_rng(12)
+        idx = np.sort(rng.choice(discr.ndofs, min(args.sample, discr.ndofs), replace=False))
+        st0 = discr.strengths(dens_np[:, 0])
+        ref = g.direct("laplace", 0.0, "S", 3, st0, idx, nthreads=nthreads).real
+        err_direct = float(np.max(np.abs(res[idx, 0] - ref)) / np.max(np.abs(ref)))
+        lo = list(map(int, w._lo))
+        t2 = time.perf_counter()
+        g.fmm("laplace", 0.0, "S", 3, lo, st0, nthreads=nthreads)
+        t_fmm = time.perf_counter() - t2
+        cpu = {"value": round(discr.ndofs / t_fmm, 1), "unit": "points*rhs/s",
+               "cores": nthreads, "kind": "port",
+               "sample": (f"one density (RHS 0) of the same workload through the oracle's "
+                          f"QBX-FMM, same tree/lists/orders, {t_fmm:.1f}s")}
+    line = {
+        "metric": METRIC,
+        "value": round(discr.ndofs * nrhs / dt, 1),
+        "unit": "points*rhs/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": (f"3D Laplace S, unit sphere: icosahedron refined {args.nsub}x "
+                                f"({discr.nelements} triangles), order-4 nodes "
+                                f"({discr.ndofs}), quad {args.fine3d}x{args.fine3d} "
+                                f"collapsed Gauss ({discr.nsources} sources), qbx_order 3, "
+                                f"fmm_order 8, {nrhs} densities (Y_lm l<=6 + noise), "
+                                f"on-surface targets"),
+                   "nrhs": nrhs, "ntargets": discr.ndofs, "nboxes": tree.nboxes,
+                   "nlevels": tree.nlevels},
+        "plain_one_density": {"value": round(discr.ndofs / dt1, 1), "unit": "points/s",
+                              "ms_per_step": round(dt1 * 1e3, 4),
+                              "rel_diff_vs_batched_rhs0": float(
+                                  np.max(np.abs(out1.real.cpu().numpy() - res[:, 0])) /
+                                  np.max(np.abs(res[:, 0])))},
+        "rel_err_vs_direct_qbx_rhs0": err_direct,
+        "rel_err_S_Y00_eigen": err_eig,
+        "cold_geometry_s": round(cold, 4),
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -232,6 +391,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--sample", type=int, default=1000)
+    ap.add_argument("--nrhs", type=int, default=8,
+                    help="densities per evaluation for the batched figure")
+    ap.add_argument("--workload", choices=["helmholtz2d", "sphere3d"], default="helmholtz2d")
+    ap.add_argument("--nsub", type=int, default=5, help="sphere3d: icosahedron refinements")
+    ap.add_argument("--fine3d", type=int, default=6, help="sphere3d: quad rule n x n")
+    ap.add_argument("--nrhs3d", type=int, default=32, help="sphere3d: densities")
     ap.add_argument("--profile", action="store_true",
                     help="one configuration only (tolerance orders), no CPU leg: "
                          "the run profiled under rocprofv3")
@@ -258,6 +423,9 @@ def main():
         print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n,
                           "GBps_read_plus_write": round(2 * n / dt / 1e9, 1)}), flush=True)
         return
+    if args.workload == "sphere3d":
+        main_sphere3d(args)
+        return
     from pytential_amd.qbx import fmm_level_to_order_for_tolerance
 
     t0 = time.perf_counter()
@@ -279,6 +447,7 @@ def main():
             f"{ {k: round(v, 4) for k, v in main_run['per_stage_ms'].items()} }")
         return
     cfg_run = run_gpu(args, cfg, config_orders, f"p{cfg.fmm_order}")
+    batched = run_batched(args, cfg, main_run["src"], main_run["geo"], args.nrhs)
 
     rng = np.random.default_rng(11)
     ttc, _ = main_run["geo"]._assoc
@@ -335,6 +504,14 @@ def main():
             "value": round(cfg.ntargets / cfg_run["step_s"], 1),
             "ms_per_step": round(cfg_run["step_s"] * 1e3, 4),
             "rel_err_vs_direct_qbx": err["config_order"]},
+        "batched": {
+            "nrhs": batched["nrhs"],
+            "value": round(cfg.ntargets * batched["nrhs"] / batched["step_s"], 1),
+            "unit": "points*rhs/s",
+            "ms_per_step": round(batched["step_s"] * 1e3, 4),
+            "rel_diff_rhs0_vs_plain": float(
+                np.max(np.abs(batched["result0"] - main_run["result"])) /
+                np.max(np.abs(main_run["result"])))},
         "fused_driver_ms_per_step": round(main_run["step_fused_s"] * 1e3, 4),
         "cold_geometry_s": round(main_run["cold_s"], 4),
         "per_stage_ms": {k: round(v, 4) for k, v in ps.items()},
