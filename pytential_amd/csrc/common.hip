@@ -7,6 +7,8 @@
 
 #include "qbx_internal.h"
 
+#include <algorithm>
+
 namespace qbx {
 
 static thread_local char g_err[1024] = "";
@@ -109,6 +111,24 @@ int compact_indices(qbx_ctx *ctx, hipStream_t st, const int *flags, int n, dbuf<
     }
     QBX_HIP(hipStreamSynchronize(st));
     *count = tot;
+    return QBX_OK;
+}
+
+// zero fill by a kernel (a memset node recorded into a captured graph is not
+// reliable on this platform, so the stages clear their outputs this way)
+__global__ void k_zero(double *p, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long k = i; k < n; k += stride) p[k] = 0.0;
+}
+
+int zero_fill(double *p, long ndoubles, hipStream_t st)
+{
+    if (ndoubles <= 0) return QBX_OK;
+    const long blocks = std::min<long>(cdiv(ndoubles, 256), 8192);
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, st, p, ndoubles);
+    QBX_HIP(hipGetLastError());
     return QBX_OK;
 }
 

@@ -1458,7 +1458,7 @@ int qbx_form_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx
     QBX_TRY(make_kd(d, T, &kd));
     QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
     hipStream_t st = S(stream);
-    QBX_HIP(hipMemsetAsync(mpole, 0, exp_bytes(kd), st));
+    QBX_TRY(zero_fill(mpole, (exp_bytes(kd)) / 8, st));
     const Flat fl = make_flat(kd, R->h_level_start_source_box_nrs, true);
     const long nthr = fl.tstart[fl.nl];
     if (nthr > 0) {
@@ -1513,7 +1513,7 @@ static int eval_targets(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx
     QBX_TRY(make_kd(d, T, &kd));
     hipStream_t st = S(stream);
     if (!accumulate)
-        QBX_HIP(hipMemsetAsync(pot, 0, sizeof(double) * 2 * (size_t)T->ntargets * kd.nr, st));
+        QBX_TRY(zero_fill(pot, (sizeof(double) * 2 * (size_t)T->ntargets * kd.nr) / 8, st));
     const int ntb = R->ntarget_boxes;
     if (ntb == 0 || G->nfiltered == 0) return QBX_OK;
     const TD td = make_td(T);
@@ -1576,7 +1576,7 @@ int qbx_multipole_to_local(qbx_ctx *ctx, void *stream, const qbx_tree *T, const 
     QBX_TRY(binom_table(&B));
     hipStream_t st = S(stream);
     QBX_TRY(ensure_tables(ctx, st, T, &kd));
-    QBX_HIP(hipMemsetAsync(local, 0, exp_bytes(kd), st));
+    QBX_TRY(zero_fill(local, (exp_bytes(kd)) / 8, st));
     const Flat fl = make_flat(kd, R->h_level_start_ttp_box_nrs, false);
     const long nthr = fl.tstart[fl.nl];
     if (nthr > 0 && R->n2 > 0) {
@@ -1602,7 +1602,7 @@ int qbx_form_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_tra
     QBX_TRY(make_kd(d, T, &kd));
     QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
     hipStream_t st = S(stream);
-    if (!accumulate) QBX_HIP(hipMemsetAsync(local, 0, exp_bytes(kd), st));
+    if (!accumulate) QBX_TRY(zero_fill(local, (exp_bytes(kd)) / 8, st));
     const Flat fl = make_flat(kd, R->h_level_start_ttp_box_nrs, true);
     const long nthr = fl.tstart[fl.nl];
     if (nthr > 0 && R->n4 > 0) {
@@ -1657,8 +1657,7 @@ int qbx_form_global_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, co
     QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
     hipStream_t st = S(stream);
     if (!accumulate)
-        QBX_HIP(hipMemsetAsync(qexp, 0,
-                               sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq * kd.nr, st));
+        QBX_TRY(zero_fill(qexp, (sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq * kd.nr) / 8, st));
     const int ng = G->nglobal;
     if (ng == 0) return QBX_OK;
     const TD td = make_td(T);
@@ -1688,8 +1687,7 @@ int qbx_translate_box_multipoles_to_qbx_local(qbx_ctx *ctx, void *stream, const 
     QBX_TRY(binom_table(&B));
     hipStream_t st = S(stream);
     if (!accumulate)
-        QBX_HIP(hipMemsetAsync(qexp, 0,
-                               sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq * kd.nr, st));
+        QBX_TRY(zero_fill(qexp, (sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq * kd.nr) / 8, st));
     const int ng = G->nglobal;
     if (ng == 0 || R->n3 == 0) return QBX_OK;
     const TD td = make_td(T);
@@ -1719,8 +1717,7 @@ int qbx_translate_box_local_to_qbx_local(qbx_ctx *ctx, void *stream, const qbx_t
     QBX_TRY(binom_table(&B));
     hipStream_t st = S(stream);
     if (!accumulate)
-        QBX_HIP(hipMemsetAsync(qexp, 0,
-                               sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq * kd.nr, st));
+        QBX_TRY(zero_fill(qexp, (sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq * kd.nr) / 8, st));
     const int ng = G->nglobal;
     if (ng == 0) return QBX_OK;
     const TD td = make_td(T);

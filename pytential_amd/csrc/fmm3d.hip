@@ -1144,7 +1144,7 @@ int f3_form_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_
     QBX_TRY(make_kd(d, T, &kd));
     QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
     hipStream_t st = S(stream);
-    QBX_HIP(hipMemsetAsync(mpole, 0, exp_bytes(kd), st));
+    QBX_TRY(zero_fill(mpole, (exp_bytes(kd)) / 8, st));
     const Flat fl = make_flat(kd, R->h_level_start_source_box_nrs);
     const long nthr = fl.tstart[fl.nl];
     if (nthr > 0) {
@@ -1191,7 +1191,7 @@ int f3_eval_targets(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_tra
     QBX_TRY(make_kd(d, T, &kd));
     hipStream_t st = S(stream);
     if (!accumulate)
-        QBX_HIP(hipMemsetAsync(pot, 0, sizeof(double) * 2 * (size_t)T->ntargets * kd.nr, st));
+        QBX_TRY(zero_fill(pot, (sizeof(double) * 2 * (size_t)T->ntargets * kd.nr) / 8, st));
     const int ntb = R->ntarget_boxes;
     if (ntb == 0 || G->nfiltered == 0) return QBX_OK;
     const TD td = make_td(T);
@@ -1224,7 +1224,7 @@ int f3_multipole_to_local(qbx_ctx *ctx, void *stream, const qbx_tree *T, const q
     QBX_TRY(make_kd(d, T, &kd, R->nsep));
     QBX_TRY(ensure_tables(ctx, &kd));
     hipStream_t st = S(stream);
-    QBX_HIP(hipMemsetAsync(local, 0, exp_bytes(kd), st));
+    QBX_TRY(zero_fill(local, (exp_bytes(kd)) / 8, st));
     if (R->nttp_boxes > 0 && R->n2 > 0) {
         const TD td = make_td(T);
         const int no = ((kd.pmax + 1) * (kd.pmax + 1) + M2L_THREADS - 1) / M2L_THREADS;
@@ -1247,7 +1247,7 @@ int f3_form_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav
     QBX_TRY(make_kd(d, T, &kd));
     QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
     hipStream_t st = S(stream);
-    if (!accumulate) QBX_HIP(hipMemsetAsync(local, 0, exp_bytes(kd), st));
+    if (!accumulate) QBX_TRY(zero_fill(local, (exp_bytes(kd)) / 8, st));
     const Flat fl = make_flat(kd, R->h_level_start_ttp_box_nrs);
     const long nthr = fl.tstart[fl.nl];
     if (nthr > 0 && R->n4 > 0) {
@@ -1295,8 +1295,7 @@ int f3_form_global_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T,
     QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
     hipStream_t st = S(stream);
     if (!accumulate)
-        QBX_HIP(hipMemsetAsync(qexp, 0,
-                               sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq * kd.nr, st));
+        QBX_TRY(zero_fill(qexp, (sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq * kd.nr) / 8, st));
     const int ng = G->nglobal;
     if (ng == 0) return QBX_OK;
     const TD td = make_td(T);
@@ -1320,8 +1319,7 @@ int f3_translate_box_multipoles_to_qbx_local(qbx_ctx *ctx, void *stream, const q
     QBX_TRY(make_kd(d, T, &kd));
     hipStream_t st = S(stream);
     if (!accumulate)
-        QBX_HIP(hipMemsetAsync(qexp, 0,
-                               sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq * kd.nr, st));
+        QBX_TRY(zero_fill(qexp, (sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq * kd.nr) / 8, st));
     const int ng = G->nglobal;
     if (ng == 0 || R->n3 == 0) return QBX_OK;
     const int pn = kd.pmax + kd.pq;
@@ -1348,8 +1346,7 @@ int f3_translate_box_local_to_qbx_local(qbx_ctx *ctx, void *stream, const qbx_tr
     QBX_TRY(make_kd(d, T, &kd));
     hipStream_t st = S(stream);
     if (!accumulate)
-        QBX_HIP(hipMemsetAsync(qexp, 0,
-                               sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq * kd.nr, st));
+        QBX_TRY(zero_fill(qexp, (sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq * kd.nr) / 8, st));
     const int ng = G->nglobal;
     if (ng == 0) return QBX_OK;
     const TD td = make_td(T);

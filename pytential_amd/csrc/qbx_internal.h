@@ -139,6 +139,8 @@ int sort_pairs_i32(qbx_ctx *ctx, hipStream_t st, const int *kin, int *kout, cons
                    int *vout, int n, int end_bit);
 int compact_indices(qbx_ctx *ctx, hipStream_t st, const int *flags, int n, dbuf<int> &out,
                     int *count);
+// stream-ordered zero fill of n doubles by a kernel (safe inside graph capture)
+int zero_fill(double *p, long ndoubles, hipStream_t st);
 
 // 3D Laplace stages (fmm3d.hip), same contracts as the C-ABI entry points
 long f3_expansion_size(const qbx_kernel_desc *d, const qbx_tree *T);
