@@ -136,28 +136,38 @@ __device__ __forceinline__ cplx cpow_int(cplx z, int m)
 }
 
 // -------------------------------------------------------------------------
-// pairwise kernel without the strength: potential = value * q
+// pairwise kernel without the strength (potential = value * q): Laplace
+// -log(r)/(2 pi) and its dipole Re(nu / w)/(2 pi); Helmholtz (i/4) H0(k r)
+// and its dipole (i/4) k (w . nu / r) H1(k r), w = t - s.  The Hankel series
+// length is chosen per wavefront; coincident points give zero.
 // -------------------------------------------------------------------------
-__device__ __forceinline__ cplx p2p_value(const KD &kd, cplx s, cplx nu, cplx t)
+__device__ __forceinline__ cplx p2p_value_wave(const KD &kd, cplx s, cplx nu, cplx t,
+                                            const double *tab)
 {
     const cplx w = t - s;
     const double r2 = abs2(w);
-    if (r2 == 0.0) return {0.0, 0.0};
+    const bool zero = r2 == 0.0;
+    const cplx ws = zero ? cplx{1.0, 0.0} : w;
+    const double r2s = zero ? 1.0 : r2;
+    cplx v;
     if (kd.kernel == QBX_LAPLACE2D) {
-        if (!kd.dip) return {-0.5 * log(r2) * INV2PI, 0.0};
-        return {(nu * cinv(w)).x * INV2PI, 0.0};
+        v = !kd.dip ? cplx{-0.5 * log(r2s) * INV2PI, 0.0}
+                    : cplx{(nu * cinv(ws)).x * INV2PI, 0.0};
+    } else {
+        const double r = sqrt(r2s);
+        const double kr = kd.k * r;
+        cplx h0, h1;
+        if (!kd.dip) {
+            hankel01_wave<true, false>(kr, tab, h0, h1);
+            v = {-0.25 * h0.y, 0.25 * h0.x};
+        } else {
+            hankel01_wave<false, true>(kr, tab, h0, h1);
+            const double proj = (ws.x * nu.x + ws.y * nu.y) / r;
+            const cplx hv = (kd.k * proj) * h1;
+            v = {-0.25 * hv.y, 0.25 * hv.x};
+        }
     }
-    const double r = sqrt(r2);
-    const double kr = kd.k * r;
-    cplx h0, h1;
-    if (!kd.dip) {
-        hankel01_fast<true, false>(kr, h0, h1);
-        return {-0.25 * h0.y, 0.25 * h0.x};
-    }
-    hankel01_fast<false, true>(kr, h0, h1);
-    const double proj = (w.x * nu.x + w.y * nu.y) / r;
-    const cplx v = (kd.k * proj) * h1;
-    return {-0.25 * v.y, 0.25 * v.x};
+    return zero ? cplx{0.0, 0.0} : v;
 }
 
 // -------------------------------------------------------------------------
@@ -762,19 +772,122 @@ __device__ void eval_mpole_add(const KD &kd, const double *a, int p, cplx c, dou
     for (int q = 0; q < NRC; ++q) out[q] += cplx{-0.25 * acc[q].y, 0.25 * acc[q].x};
 }
 
-// One 64-lane block per target box.  P2P and M2P give each of the box's
-// non-QBX targets a group of G lanes (G a power of two, G * ntargets <= 64)
-// that splits the list-1 sources / list-3 boxes and is reduced by shuffles,
-// so boxes with few targets still keep the wavefront busy.
-// MODE 0: P2P (list 1), 1: M2P (list 3), 2: L2P (one lane per target)
+// P2P over balanced work units (one 64-lane wavefront each, see unit_plan):
+// the unit's TG targets take G lanes apiece, the box's list-1 sources are
+// staged through LDS in chunks of CH (positions, normals, strengths, read
+// coalesced), each target's lanes split a chunk, and the G partial sums are
+// reduced by shuffles in a fixed order.
+template <int NRC>
+struct P2PTile {
+    static constexpr int CH = NRC == 1 ? 256 : 64;
+};
+
+template <int NRC>
+__global__ void __launch_bounds__(64) k_p2p(KD kd, TD t, const int *ubox, const int *ut0,
+                                            const int *box_g, const int *tboxes,
+                                            const int *starts, const int *lists,
+                                            const int *fstart, const int *fcount,
+                                            const int *unfilt, const double *tgt, int nt,
+                                            const double *src, int ns, const double *str,
+                                            const double *nrm, double *pot)
+{
+    constexpr int CH = P2PTile<NRC>::CH;
+    __shared__ double sx[CH], sy[CH], snx[CH], sny[CH];
+    __shared__ double sq[CH * 2 * NRC];
+    __shared__ double htab[4 * HANKEL_NSER];
+    const int u = blockIdx.x;
+    const int it = ubox[u];
+    const int B = tboxes[it];
+    const int G = box_g[it];
+    const int lane = threadIdx.x;
+    const int tsub = lane / G, sub = lane & (G - 1);
+    const int tl = ut0[u] + tsub;
+    const bool active = tl < fcount[B];
+    const bool dip = kd.dip;
+    const int rc = kd.rc, r0 = kd.r0, nr = kd.nr;
+    int ti = 0;
+    cplx tp = {0, 0};
+    if (active) {
+        ti = unfilt[fstart[B] + tl];
+        tp = ldpt(tgt, nt, ti);
+    }
+    hankel_table_fill(htab, lane, 64);
+    cplx acc[NRC];
+#pragma unroll
+    for (int q = 0; q < NRC; ++q) acc[q] = {0, 0};
+
+    auto flush = [&](int n) {
+        __syncthreads();
+        if (active) {
+            for (int j = sub; j < n; j += G) {
+                // an offset the compiler cannot see through keeps the series
+                // coefficients as LDS reads inside the loop instead of
+                // hoisting all of them into registers
+                int toff = 0;
+                asm volatile("" : "+s"(toff));
+                const cplx nu = dip ? cplx{snx[j], sny[j]} : cplx{0, 0};
+                const cplx g = p2p_value_wave(kd, cplx{sx[j], sy[j]}, nu, tp, htab + toff);
+#pragma unroll
+                for (int q = 0; q < NRC; ++q)
+                    if (q < rc) acc[q] += cplx{sq[2 * (j * NRC + q)], sq[2 * (j * NRC + q) + 1]} * g;
+            }
+        }
+        __syncthreads();
+    };
+
+    int fill = 0;
+    const int e1 = starts[it + 1];
+    for (int e = starts[it]; e < e1; ++e) {
+        const int Sb = lists[e];
+        int s = t.sstart[Sb];
+        const int s1 = s + t.scount[Sb];
+        while (s < s1) {
+            const int take = min(s1 - s, CH - fill);
+            for (int j = lane; j < take; j += 64) {
+                const int si = s + j, d = fill + j;
+                sx[d] = src[si];
+                sy[d] = src[(size_t)ns + si];
+                if (dip) {
+                    snx[d] = nrm[si];
+                    sny[d] = nrm[(size_t)ns + si];
+                }
+#pragma unroll
+                for (int q = 0; q < NRC; ++q)
+                    if (q < rc) {
+                        const double *p = str + 2 * ((size_t)si * nr + r0 + q);
+                        sq[2 * (d * NRC + q)] = p[0];
+                        sq[2 * (d * NRC + q) + 1] = p[1];
+                    }
+            }
+            fill += take;
+            s += take;
+            if (fill == CH) {
+                flush(CH);
+                fill = 0;
+            }
+        }
+    }
+    if (fill > 0) flush(fill);
+#pragma unroll
+    for (int q = 0; q < NRC; ++q)
+        for (int m = G / 2; m >= 1; m >>= 1) {
+            acc[q].x += __shfl_xor(acc[q].x, m);
+            acc[q].y += __shfl_xor(acc[q].y, m);
+        }
+    if (active && sub == 0) store_add<NRC>(kd, pot + 2 * (size_t)ti * nr, acc);
+}
+
+// One 64-lane block per target box.  M2P gives each of the box's non-QBX
+// targets a group of G lanes (G a power of two, G * ntargets <= 64) that
+// splits the list-3 boxes and is reduced by shuffles, so boxes with few
+// targets still keep the wavefront busy.
+// MODE 1: M2P (list 3), 2: L2P (one lane per target)
 template <int MODE, int NRC>
 __global__ void __launch_bounds__(64) k_eval_targets(KD kd, TD t, int ntb, const int *tboxes,
                                                      const int *starts, const int *lists,
                                                      const int *fstart, const int *fcount,
                                                      const int *unfilt, const double *tgt,
-                                                     int nt, const double *src, int ns,
-                                                     const double *str, const double *nrm,
-                                                     const double *expn, double *pot)
+                                                     int nt, const double *expn, double *pot)
 {
     const int it = blockIdx.x;
     if (it >= ntb) return;
@@ -807,25 +920,11 @@ __global__ void __launch_bounds__(64) k_eval_targets(KD kd, TD t, int ntb, const
         if (active) {
             ti = unfilt[f0 + base + tsub];
             const cplx tp = ldpt(tgt, nt, ti);
-            if (MODE == 0) {
-                for (int e = e0; e < e1; ++e) {
-                    const int Sb = lists[e];
-                    const int s0 = t.sstart[Sb], s1 = s0 + t.scount[Sb];
-                    for (int s = s0 + sub; s < s1; s += G) {
-                        const cplx nu = kd.dip ? ldpt(nrm, ns, s) : cplx{0, 0};
-                        const cplx g = p2p_value(kd, ldpt(src, ns, s), nu, tp);
-#pragma unroll
-                        for (int q = 0; q < NRC; ++q)
-                            if (q < kd.rc) acc[q] += ldq(kd, str, s, kd.r0 + q) * g;
-                    }
-                }
-            } else {
-                for (int e = e0 + sub; e < e1; e += G) {
-                    const int D = lists[e];
-                    const int lv = t.level[D];
-                    eval_mpole_add<NRC>(kd, expn + ecoef_off(kd, D, lv), kd.order[lv],
-                                        boxc(t, D), kd.rscale[lv], tp, acc);
-                }
+            for (int e = e0 + sub; e < e1; e += G) {
+                const int D = lists[e];
+                const int lv = t.level[D];
+                eval_mpole_add<NRC>(kd, expn + ecoef_off(kd, D, lv), kd.order[lv], boxc(t, D),
+                                    kd.rscale[lv], tp, acc);
             }
         }
 #pragma unroll
@@ -1519,19 +1618,30 @@ static int eval_targets(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx
     const TD td = make_td(T);
     const int *starts = mode == 1 ? R->l3_starts.p : R->l1_starts.p;
     const int *lists = mode == 1 ? R->l3_lists.p : R->l1_lists.p;
+    if (mode == 0) {
+        const unit_plan &U = G->p2p_plan;
+        QBX_CHECK(G->trav == R, "qbx_eval_direct: geometry was built for another traversal");
+        if (U.nunits == 0) return QBX_OK;
+#define P2P_LAUNCH(N)                                                                      \
+    hipLaunchKernelGGL((k_p2p<N>), dim3(U.nunits), dim3(64), 0, st, kd, td, U.unit_box.p,  \
+                       U.unit_t0.p, U.box_g.p, R->target_boxes.p, starts, lists,          \
+                       G->nqbtl_box_target_starts.p, G->nqbtl_box_target_counts_nonchild.p, \
+                       G->nqbtl_unfiltered.p, T->targets.p, T->ntargets, T->sources.p,    \
+                       T->nsources, str, nrm, pot)
+        FOR_RHS_CHUNKS(kd, NRC_S, P2P_LAUNCH);
+#undef P2P_LAUNCH
+        QBX_HIP(hipGetLastError());
+        return QBX_OK;
+    }
 #define EVAL_LAUNCH(M, N)                                                                  \
     hipLaunchKernelGGL((k_eval_targets<M, N>), dim3(ntb), dim3(64), 0, st, kd, td, ntb,    \
                        R->target_boxes.p, starts, lists, G->nqbtl_box_target_starts.p,    \
                        G->nqbtl_box_target_counts_nonchild.p, G->nqbtl_unfiltered.p,      \
-                       T->targets.p, T->ntargets, T->sources.p, T->nsources, str, nrm,    \
-                       expn, pot)
-#define EV0(N) EVAL_LAUNCH(0, N)
+                       T->targets.p, T->ntargets, expn, pot)
 #define EV1(N) EVAL_LAUNCH(1, N)
 #define EV2(N) EVAL_LAUNCH(2, N)
-    if (mode == 0) FOR_RHS_CHUNKS(kd, NRC_S, EV0);
-    else if (mode == 1) FOR_RHS_CHUNKS(kd, NRC_S, EV1);
+    if (mode == 1) FOR_RHS_CHUNKS(kd, NRC_S, EV1);
     else FOR_RHS_CHUNKS(kd, NRC_S, EV2);
-#undef EV0
 #undef EV1
 #undef EV2
 #undef EVAL_LAUNCH

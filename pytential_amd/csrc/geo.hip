@@ -67,6 +67,64 @@ __global__ void k_nqbtl_boxes(int nb, const int *tstarts, const int *tcounts, co
     fcounts[b] = pos[s + tcounts[b]] - pos[s];
 }
 
+// per target box: units and lanes per target from its non-QBX target count
+// and the sources of its list-1 boxes
+__global__ void k_unit_work(int ntb, const int *tboxes, const int *starts, const int *lists,
+                            const int *scount, const int *fcount, int *units, int *glanes)
+{
+    int it = blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= ntb) return;
+    const int nf = fcount[tboxes[it]];
+    long work = 0;
+    for (int e = starts[it]; e < starts[it + 1]; ++e) work += scount[lists[e]];
+    int g = 1;
+    while (g < 64 && 2 * g * nf <= 64) g <<= 1;                  // fill the wavefront
+    while (g < 64 && (long)g * UNIT_ITER < work) g <<= 1;        // bound a lane's walk
+    const int tg = 64 / g;
+    units[it] = (nf > 0 && work > 0) ? (nf + tg - 1) / tg : 0;
+    glanes[it] = g;
+}
+
+int build_unit_plan(hipStream_t st, const qbx_tree *T, const qbx_trav *R, const qbx_geo *G,
+                    unit_plan &plan)
+{
+    const int ntb = R->ntarget_boxes;
+    plan.nunits = 0;
+    QBX_TRY(plan.box_g.alloc(std::max(ntb, 1)));
+    if (ntb == 0) return QBX_OK;
+    dbuf<int> units;
+    QBX_TRY(units.alloc(ntb));
+    hipLaunchKernelGGL(k_unit_work, dim3(cdiv(ntb, 256)), dim3(256), 0, st, ntb,
+                       R->target_boxes.p, R->l1_starts.p, R->l1_lists.p,
+                       T->box_source_counts_nonchild.p, G->nqbtl_box_target_counts_nonchild.p,
+                       units.p, plan.box_g.p);
+    QBX_HIP(hipGetLastError());
+    std::vector<int> hu(ntb), hg(ntb);
+    QBX_HIP(hipMemcpyAsync(hu.data(), units.p, sizeof(int) * ntb, hipMemcpyDeviceToHost, st));
+    QBX_HIP(hipMemcpyAsync(hg.data(), plan.box_g.p, sizeof(int) * ntb, hipMemcpyDeviceToHost,
+                           st));
+    QBX_HIP(hipStreamSynchronize(st));
+    std::vector<int> ub, ut;
+    for (int it = 0; it < ntb; ++it) {
+        const int tg = 64 / hg[it];
+        for (int u = 0; u < hu[it]; ++u) {
+            ub.push_back(it);
+            ut.push_back(u * tg);
+        }
+    }
+    plan.nunits = (int)ub.size();
+    QBX_TRY(plan.unit_box.alloc(std::max(plan.nunits, 1)));
+    QBX_TRY(plan.unit_t0.alloc(std::max(plan.nunits, 1)));
+    if (plan.nunits > 0) {
+        QBX_HIP(hipMemcpyAsync(plan.unit_box.p, ub.data(), sizeof(int) * plan.nunits,
+                               hipMemcpyHostToDevice, st));
+        QBX_HIP(hipMemcpyAsync(plan.unit_t0.p, ut.data(), sizeof(int) * plan.nunits,
+                               hipMemcpyHostToDevice, st));
+    }
+    QBX_HIP(hipStreamSynchronize(st));
+    return QBX_OK;
+}
+
 static int build_geo(qbx_ctx *ctx, hipStream_t st, const qbx_tree *T, const qbx_trav *R,
                      int nc, const double *centers, const double *radii, const int *uttc,
                      qbx_geo *G)
@@ -148,6 +206,8 @@ static int build_geo(qbx_ctx *ctx, hipStream_t st, const qbx_tree *T, const qbx_
                            G->nqbtl_box_target_starts.p, G->nqbtl_box_target_counts_nonchild.p);
         QBX_HIP(hipGetLastError());
     }
+    G->trav = R;
+    QBX_TRY(build_unit_plan(st, T, R, G, G->p2p_plan));
     QBX_HIP(hipStreamSynchronize(st));
     return QBX_OK;
 }

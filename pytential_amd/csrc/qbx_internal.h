@@ -68,6 +68,22 @@ struct dbuf {
 
 constexpr int MAX_LEVELS = 64;
 
+// Balanced work units for the per-target-box evaluation kernels.  A unit is
+// one 64-lane wavefront over TG = 64 / G consecutive non-QBX targets of one
+// target box, G lanes per target splitting the box's interaction list; G is
+// chosen per box from its list work (list-1 sources for P2P) so that no lane
+// walks more than about UNIT_ITER items, and so that boxes with few targets
+// still fill the wavefront.  A target belongs to exactly one unit, so units
+// write their targets' sums without atomics.
+constexpr int UNIT_ITER = 64;
+
+struct unit_plan {
+    int nunits = 0;
+    dbuf<int> unit_box;  // [nunits] index into the traversal's target boxes
+    dbuf<int> unit_t0;   // [nunits] first non-QBX target of the unit (box-local)
+    dbuf<int> box_g;     // [ntarget_boxes] lanes per target (power of two <= 64)
+};
+
 }  // namespace qbx
 
 struct qbx_ctx {
@@ -126,6 +142,9 @@ struct qbx_geo {
     qbx::dbuf<int> nqbtl_box_target_starts, nqbtl_box_target_counts_nonchild;
     qbx::dbuf<int> nqbtl_unfiltered;           // [nfiltered] tree target indices
     qbx::dbuf<int> user_to_tree_target;        // [ntargets]
+    // P2P work units over the traversal the geometry was built with
+    const qbx_trav *trav = nullptr;
+    qbx::unit_plan p2p_plan;
 };
 
 namespace qbx {
@@ -141,6 +160,9 @@ int compact_indices(qbx_ctx *ctx, hipStream_t st, const int *flags, int n, dbuf<
                     int *count);
 // stream-ordered zero fill of n doubles by a kernel (safe inside graph capture)
 int zero_fill(double *p, long ndoubles, hipStream_t st);
+// work units of list 1 (P2P) for the geometry's non-QBX targets
+int build_unit_plan(hipStream_t st, const qbx_tree *T, const qbx_trav *R, const qbx_geo *G,
+                    unit_plan &plan);
 
 // 3D Laplace stages (fmm3d.hip), same contracts as the C-ABI entry points
 long f3_expansion_size(const qbx_kernel_desc *d, const qbx_tree *T);

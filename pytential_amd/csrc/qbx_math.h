@@ -179,4 +179,81 @@ __device__ __forceinline__ void hankel01_fast(double x, cplx &h0, cplx &h1)
     }
 }
 
+// H_0, H_1 for x > 4 from the device library, kept out of line so that its
+// coefficients are materialised inside the call rather than held in
+// registers across a caller's loop
+template <bool WANT0, bool WANT1>
+__device__ __noinline__ void hankel01_libm(double x, cplx &h0, cplx &h1)
+{
+    if (WANT0) h0 = {j0(x), y0(x)};
+    if (WANT1) h1 = {j1(x), y1(x)};
+}
+
+// The series of hankel01_series with its coefficients read from tab
+// (A, B, C, D consecutively, HANKEL_NSER each; typically a copy in LDS, so
+// that a loop around it does not pin the coefficients in registers)
+constexpr int HANKEL_NSER = 18;
+
+template <int K, bool WANT0, bool WANT1>
+__device__ __forceinline__ void hankel01_series_tab(double x, const double *tab, cplx &h0,
+                                                    cplx &h1)
+{
+    const double t = 0.25 * x * x;
+    double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+#pragma unroll
+    for (int k = K - 1; k >= 0; --k) {
+        if (WANT0) {
+            a = a * t + tab[k];
+            c = c * t + tab[2 * HANKEL_NSER + k];
+        }
+        if (WANT1) {
+            b = b * t + tab[HANKEL_NSER + k];
+            d = d * t + tab[3 * HANKEL_NSER + k];
+        }
+    }
+    const double lg = log(0.5 * x);
+    constexpr double TWO_OVER_PI = 0.63661977236758134308;
+    constexpr double ONE_OVER_PI = 0.31830988618379067154;
+    if (WANT0) h0 = {a, TWO_OVER_PI * lg * a - ONE_OVER_PI * c};
+    if (WANT1) {
+        double j1v = 0.5 * x * b;
+        h1 = {j1v, -TWO_OVER_PI / x + TWO_OVER_PI * lg * j1v - 0.5 * x * ONE_OVER_PI * d};
+    }
+}
+
+// fill tab (4 * HANKEL_NSER doubles) cooperatively by the lanes of a block
+__device__ __forceinline__ void hankel_table_fill(double *tab, int lane, int nlanes)
+{
+    for (int i = lane; i < 4 * HANKEL_NSER; i += nlanes) {
+        const int w = i / HANKEL_NSER, k = i % HANKEL_NSER;
+        tab[i] = w == 0 ? c_ser_A[k] : w == 1 ? c_ser_B[k] : w == 2 ? c_ser_C[k] : c_ser_D[k];
+    }
+}
+
+// The values of hankel01_fast with the series length chosen once per
+// wavefront: the lanes with x <= 4 share one series of the length the
+// largest of them needs (one pass instead of up to three divergent ones);
+// lanes beyond 4 take the device library in their own pass.
+template <bool WANT0, bool WANT1>
+__device__ __forceinline__ void hankel01_wave(double x, const double *tab, cplx &h0, cplx &h1)
+{
+    const bool big = x > 4.0;
+    if (__any(big)) {
+        if (big) hankel01_libm<WANT0, WANT1>(x, h0, h1);
+    }
+    if (__any(!big)) {
+        cplx a0 = {0, 0}, a1 = {0, 0};
+        if (__any(!big && x > 3.0))
+            hankel01_series_tab<18, WANT0, WANT1>(x, tab, a0, a1);
+        else if (__any(!big && x > 1.5))
+            hankel01_series_tab<14, WANT0, WANT1>(x, tab, a0, a1);
+        else
+            hankel01_series_tab<10, WANT0, WANT1>(x, tab, a0, a1);
+        if (!big) {
+            h0 = a0;
+            h1 = a1;
+        }
+    }
+}
+
 }  // namespace qbx
