@@ -46,6 +46,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_helmholtz2d_starfish.json")
+
+
+def measured_traffic(stage):
+    """HBM bytes per launch of a stage's kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (scaled by the copy
+    calibration recorded there), or None when the summary lacks the stage."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f)["stages"].get(stage)
+    except (OSError, ValueError, KeyError):
+        return None
+    if not rec:
+        return None
+    return int(rec["read_bytes"] + rec["write_bytes"])
+
+
 def build_problem(grid_n, nelements):
     from pytential_amd import problems
     from pytential_amd.geometry import assoc_status_and_flags
@@ -60,34 +77,57 @@ def interaction_counts(geo):
     def h(x):
         return x.cpu().numpy().astype(np.int64)
 
+    def per_list(starts, lists, weight):
+        """sum of weight[list box] over each CSR row"""
+        if len(lists) == 0:
+            return np.zeros(len(starts) - 1, np.int64)
+        c = np.concatenate([[0], np.cumsum(weight[lists])])
+        return c[starts[1:]] - c[starts[:-1]]
+
     scount = h(t.box_source_counts_nonchild)
+    scumul = h(t.box_source_counts_cumul)
+    levels = h(t.box_levels)
     l1s, l1l = h(tr.neighbor_source_boxes_starts), h(tr.neighbor_source_boxes_lists)
+    l2s, l2l = h(tr.from_sep_siblings_starts), h(tr.from_sep_siblings_lists)
     l3s, l3l = h(tr.from_sep_smaller_starts), h(tr.from_sep_smaller_lists)
+    l4s, l4l = h(tr.from_sep_bigger_starts), h(tr.from_sep_bigger_lists)
     tboxes = h(tr.target_boxes)
+    ttp = h(tr.target_or_target_parent_boxes)
+    sboxes = h(tr.source_boxes)
+    spar = h(tr.source_parent_boxes)
+    child = h(t.box_child_ids).reshape(-1, t.nboxes) if t.nboxes else np.zeros((4, 0), np.int64)
+    parent = h(t.box_parent_ids)
     nq = geo.non_qbx_box_target_lists()
     nfilt_box = h(nq["box_target_counts_nonchild"])[tboxes]
-    src_in_l1 = np.add.reduceat(np.append(scount[l1l], 0), l1s[:-1]) * (np.diff(l1s) > 0)
+    src_in_l1 = per_list(l1s, l1l, scount)
     n_l3 = np.diff(l3s)
     gc = h(geo.global_qbx_centers())
     c2tb = h(geo.qbx_center_to_target_box())[gc]
-    levels = h(t.box_levels)
     return {
         "p2p_pairs": int(np.sum(nfilt_box * src_in_l1)),
         "p2qbxl_pairs": int(np.sum(src_in_l1[c2tb])),
-        "m2qbxl_pairs": int(np.sum(n_l3[c2tb])),
         "m2qbxl_levels": levels[l3l] if len(l3l) else np.zeros(0, np.int64),
-        "m2qbxl_per_center_box": np.repeat(np.arange(len(tboxes)), n_l3),
         "l2qbxl": int(len(gc)),
         "qbxl2p": int(g.nctt),
-        "m2l_pairs": int(tr.from_sep_siblings_lists.numel()),
-        "m2p_pairs": int(np.sum(nfilt_box * n_l3)),
         "nglobal": int(g.nglobal), "nfiltered": int(g.nfiltered),
         "c2tb": c2tb, "l3s": l3s, "l3l": l3l, "levels": levels,
+        "scount": scount, "scumul": scumul, "tboxes": tboxes, "nfilt_box": nfilt_box,
+        "ttp": ttp, "l2s": l2s, "l2l": l2l, "l4s": l4s, "l4l": l4l, "sboxes": sboxes,
+        "spar": spar, "child": child, "parent": parent, "n_l3": n_l3,
     }
 
 
+def _miller_steps(p):
+    m = max(p, 1)
+    return m + 16 + int(np.sqrt(40.0 * m + 1.0))
+
+
 def stage_flops(counts, orders, pq):
-    """Algorithmic flops per stage launch (Helmholtz 2D, double layer)."""
+    """Algorithmic flops per stage launch (Helmholtz 2D, double layer).  A
+    Bessel/Hankel function value of order 0 or 1 counts SPECIAL flops, a
+    complex multiply-add 8, a complex multiply 6."""
+    o = np.asarray(orders, np.int64)
+    lv = counts["levels"]
     nq = 2 * pq + 1
     # P2P D: r, proj (10), H1 = J1 + i Y1 (2 special), scale and accumulate (12)
     p2p = counts["p2p_pairs"] * (22 + 2 * SPECIAL)
@@ -96,15 +136,53 @@ def stage_flops(counts, orders, pq):
     p2qbxl = counts["p2qbxl_pairs"] * (6 + 5 * SPECIAL + 12 * (pq + 1) + 2 * 8 * nq)
     # M2QBXL: per (center, box): H0,H1 + atan2, recurrence to p+pq (8 each),
     # 2*nq complex MACs per order
-    lv = counts["m2qbxl_levels"]
+    m2q_lv = counts["m2qbxl_levels"]
     c2tb, l3s = counts["c2tb"], counts["l3s"]
-    per_box = np.array([orders[l] for l in lv], dtype=np.int64) if len(lv) else np.zeros(0)
+    per_box = o[m2q_lv] if len(m2q_lv) else np.zeros(0, np.int64)
     m2q = 0
     if len(per_box):
         csum = np.concatenate([[0], np.cumsum(5 * SPECIAL + (per_box + pq + 1) *
                                               (8 + 2 * nq * 8))])
         m2q = int(np.sum(csum[l3s[c2tb + 1]] - csum[l3s[c2tb]]))
-    return {"p2p": p2p, "p2qbxl": p2qbxl, "m2qbxl": m2q}
+    # M2L (table rows): (2p+1)^2 complex MACs per (box, list-2 box)
+    ttp, l2s = counts["ttp"], counts["l2s"]
+    pt = o[lv[ttp]]
+    m2l = int(np.sum(np.diff(l2s) * (2 * pt + 1) ** 2 * 8))
+    # P2L D: per (box, list-4 source): H0,H1 (4 special), recurrence and phases
+    # to p+1 (4 + 12 each), b_n from two neighbours and one MAC (6 + 6 + 8 each)
+    scount = counts["scount"]
+    l4s, l4l = counts["l4s"], counts["l4l"]
+    c4 = np.concatenate([[0], np.cumsum(scount[l4l])]) if len(l4l) else np.zeros(1, np.int64)
+    nsrc4 = c4[l4s[1:]] - c4[l4s[:-1]] if len(l4l) else np.zeros(len(ttp), np.int64)
+    p2l = int(np.sum(nsrc4 * (4 * SPECIAL + 16 * (pt + 2) + 20 * (2 * pt + 1))))
+    # P2M D: per leaf source: Miller's recurrence (4 per step), normalisation and
+    # phases (16 each to p+1), b_n and one MAC (20 each)
+    sb = counts["sboxes"]
+    ps = o[lv[sb]]
+    steps = np.array([_miller_steps(int(p) + 1) for p in ps], np.int64)
+    p2m = int(np.sum(scount[sb] * (4 * steps + 16 * (ps + 2) + 20 * (2 * ps + 1))))
+    # M2M: (2 p_parent + 1)(2 p_child + 1) complex MACs per nonempty child
+    spar, child, scumul = counts["spar"], counts["child"], counts["scumul"]
+    m2m = 0
+    for P in spar:
+        pp = o[lv[P]]
+        for c in child[:, P]:
+            if c >= 0 and scumul[c] > 0:
+                m2m += (2 * pp + 1) * (2 * o[lv[c]] + 1) * 8
+    # L2L: (2p+1)(2 p_parent + 1) complex MACs per ttp box below the root
+    par = counts["parent"][ttp]
+    has = par >= 0
+    l2l = int(np.sum((2 * pt[has] + 1) * (2 * o[lv[par[has]]] + 1) * 8))
+    # M2P: per (target, list-3 box): H0,H1 (4 special), per order a recurrence
+    # step, a phase step and two complex MACs (4 + 6 + 16)
+    tb, l3l, nf = counts["tboxes"], counts["l3l"], counts["nfilt_box"]
+    w3 = np.zeros(len(l3s) - 1, np.int64)
+    if len(l3l):
+        c3 = np.concatenate([[0], np.cumsum(4 * SPECIAL + 26 * o[lv[l3l]])])
+        w3 = c3[l3s[1:]] - c3[l3s[:-1]]
+    m2p = int(np.sum(nf * w3))
+    return {"p2p": p2p, "p2qbxl": p2qbxl, "m2qbxl": m2q, "m2l": m2l, "p2l": p2l,
+            "p2m": p2m, "m2m": int(m2m), "l2l": l2l, "m2p": m2p}
 
 
 def run_gpu(args, cfg, orders_fn, label):
@@ -470,6 +548,8 @@ def main():
 
     ps = main_run["per_stage_ms"]
     flops = main_run["flops"]
+    # the dominant kernel: the slowest stage (every stage is one kernel launch,
+    # or one per level for the level-by-level translations)
     dominant = max(ps, key=ps.get)
     rl = None
     if dominant in flops:
@@ -479,9 +559,14 @@ def main():
         rl = {"bound": "mfma", "compute": "fp64", "kernel": dominant,
               "achieved": round(achieved, 3),
               "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-              "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+              "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+              "traffic": measured_traffic(dominant),
               "flops_per_launch": int(flops[dominant]),
               "avg_launch_ms": round(ps[dominant], 4)}
+    stage_rl = {k: {"ms": round(ps[k], 4), "gflop": round(flops[k] / 1e9, 3),
+                    "tflops": round(flops[k] / (ps[k] * 1e-3) / 1e12, 3),
+                    "frac_fp64": round(flops[k] / (ps[k] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4)}
+                for k in flops if ps.get(k, 0) > 0}
     line = {
         "metric": METRIC,
         "value": round(cfg.ntargets / main_run["step_s"], 1),
@@ -515,6 +600,7 @@ def main():
         "fused_driver_ms_per_step": round(main_run["step_fused_s"] * 1e3, 4),
         "cold_geometry_s": round(main_run["cold_s"], 4),
         "per_stage_ms": {k: round(v, 4) for k, v in ps.items()},
+        "stage_roofline": stage_rl,
         "roofline": rl,
         "cpu_baseline": None if cpu is None else {
             k: (round(v, 1) if isinstance(v, float) else v) for k, v in cpu.items()

@@ -350,6 +350,162 @@ __global__ void k_p2l(KD kd, TD t, Flat fl, const int *ttp, const int *starts, c
 }
 
 // -------------------------------------------------------------------------
+// Helmholtz P2M / P2L through LDS, one block per box.  Sources are taken S
+// at a time.  Phase A (one lane per source) runs the source's sequence once:
+// J_j by Miller's recurrence (multipoles) or H_j by forward recurrence
+// (locals), j = 0..p+1, and stores g+_j = Z_j e^{-ij theta} and
+// g-_j = Z_j e^{+ij theta} of v = source - center.  Phase B (one lane per
+// coefficient n in [-p, p]) sums q_s b_n(s) over the chunk, with
+// g_{-j} = (-1)^j g-_j and b_n = g_n (single layer) or
+// (k/2)(conj(nu) g_{n-1} - nu g_{n+1}) (double layer): O(p) work per source
+// instead of a recurrence per coefficient.
+// -------------------------------------------------------------------------
+struct P2EShape {
+    int S;  // sources per chunk
+    int R;  // row stride of g+- in complex entries (odd, >= p + 2)
+    __host__ __device__ int doubles(int nrc) const { return 4 * S * R + 2 * S * nrc + 2 * S + S; }
+};
+
+template <int NRC, bool LOCAL>
+__global__ void __launch_bounds__(256) k_p2e_h(KD kd, TD t, int lv, int i0, P2EShape sh, const int *boxes,
+                        const int *starts, const int *lists, const double *src, int ns,
+                        const double *str, const double *nrm, double *expn)
+{
+    extern __shared__ double lds[];
+    const int i = i0 + blockIdx.x;
+    const int Bx = boxes[i];
+    int e0 = 0, e1 = 1;
+    if (LOCAL) {
+        e0 = starts[i];
+        e1 = starts[i + 1];
+        if (e0 == e1) return;
+    }
+    const int p = kd.order[lv];
+    const int S = sh.S, R = sh.R;
+    double *Gp = lds, *Gm = Gp + 2 * S * R, *Qs = Gm + 2 * S * R, *Nu = Qs + 2 * S * NRC;
+    int *idx = (int *)(Nu + 2 * S);
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const bool dip = kd.dip;
+    const int rc = kd.rc, r0 = kd.r0, nr = kd.nr;
+    const cplx c = boxc(t, Bx);
+    const bool act = tid < 2 * p + 1;
+    const int n = tid - p;
+    cplx acc[NRC];
+#pragma unroll
+    for (int q = 0; q < NRC; ++q) acc[q] = {0, 0};
+
+    auto gval = [&](int sl, int j) -> cplx {
+        if (j >= 0) return {Gp[2 * (sl * R + j)], Gp[2 * (sl * R + j) + 1]};
+        const cplx g = {Gm[2 * (sl * R - j)], Gm[2 * (sl * R - j) + 1]};
+        return (j & 1) ? -g : g;
+    };
+
+    auto flush = [&](int cnt) {
+        __syncthreads();
+        if (tid < cnt) {
+            const int si = idx[tid];
+            double *gp = Gp + 2 * tid * R, *gm = Gm + 2 * tid * R;
+            const cplx v = ldpt(src, ns, si) - c;
+            const double rho = sqrt(abs2(v));
+            const double x = kd.k * rho;
+            const cplx e1 = rho > 0 ? cplx{v.x / rho, -v.y / rho} : cplx{1, 0};
+            if (LOCAL) {
+                cplx a, b;
+                hankel01_call(x, a, b);
+                const double tox = 2.0 / x;
+                for (int j = 0; j <= p + 1; ++j) {
+                    gp[2 * j] = a.x;
+                    gp[2 * j + 1] = a.y;
+                    const cplx nb = ((j + 1) * tox) * b - a;
+                    a = b;
+                    b = nb;
+                }
+            } else if (x < 1e-300) {
+                for (int j = 0; j <= p + 1; ++j) {
+                    gp[2 * j] = j == 0 ? 1.0 : 0.0;
+                    gp[2 * j + 1] = 0.0;
+                }
+            } else {
+                const double norm = miller_j(
+                    x, p + 1, [&](int j, double f) { gp[2 * j] = f; },
+                    [&](double sc) {
+                        for (int j = 0; j <= p + 1; ++j) gp[2 * j] *= sc;
+                    });
+                for (int j = 0; j <= p + 1; ++j) {
+                    gp[2 * j] *= norm;
+                    gp[2 * j + 1] = 0.0;
+                }
+            }
+            cplx e = {1, 0};
+            for (int j = 0; j <= p + 1; ++j) {
+                const cplx z = {gp[2 * j], gp[2 * j + 1]};
+                const cplx zp = z * e, zm = z * conj(e);
+                gp[2 * j] = zp.x;
+                gp[2 * j + 1] = zp.y;
+                gm[2 * j] = zm.x;
+                gm[2 * j + 1] = zm.y;
+                e = e * e1;
+            }
+#pragma unroll
+            for (int q = 0; q < NRC; ++q)
+                if (q < rc) {
+                    const double *pq = str + 2 * ((size_t)si * nr + r0 + q);
+                    Qs[2 * (tid * NRC + q)] = pq[0];
+                    Qs[2 * (tid * NRC + q) + 1] = pq[1];
+                }
+            if (dip) {
+                Nu[2 * tid] = nrm[si];
+                Nu[2 * tid + 1] = nrm[(size_t)ns + si];
+            }
+        }
+        __syncthreads();
+        if (act) {
+            const double hk = 0.5 * kd.k;
+            for (int sl = 0; sl < cnt; ++sl) {
+                cplx b;
+                if (!dip) {
+                    b = gval(sl, n);
+                } else {
+                    const cplx nu = {Nu[2 * sl], Nu[2 * sl + 1]};
+                    b = hk * (conj(nu) * gval(sl, n - 1) - nu * gval(sl, n + 1));
+                }
+#pragma unroll
+                for (int q = 0; q < NRC; ++q)
+                    if (q < rc)
+                        acc[q] += cplx{Qs[2 * (sl * NRC + q)], Qs[2 * (sl * NRC + q) + 1]} * b;
+            }
+        }
+    };
+
+    int fill = 0;
+    for (int e = e0; e < e1; ++e) {
+        const int Sb = LOCAL ? lists[e] : Bx;
+        int s = t.sstart[Sb];
+        const int s1 = s + t.scount[Sb];
+        while (s < s1) {
+            const int take = min(s1 - s, S - fill);
+            for (int j = tid; j < take; j += nth) idx[fill + j] = s + j;
+            fill += take;
+            s += take;
+            if (fill == S) {
+                flush(S);
+                fill = 0;
+            }
+        }
+    }
+    if (fill > 0) flush(fill);
+    if (act) {
+        double *o = expn + ecoef_off(kd, Bx, lv) + 2L * n * nr;
+#pragma unroll
+        for (int q = 0; q < NRC; ++q)
+            if (q < rc) {
+                double *pq = o + 2 * (r0 + q);
+                stc(pq, LOCAL ? ldc(pq) + acc[q] : acc[q]);
+            }
+    }
+}
+
+// -------------------------------------------------------------------------
 // translation tables (Helmholtz)
 // -------------------------------------------------------------------------
 // M2L row (level l, class (dx,dy)): u_j(d) = H_j(k|d|) e^{i j arg d} with
@@ -654,6 +810,120 @@ __global__ void k_l2l(KD kd, TD t, const double *B, int lv, int i0, int i1, cons
 }
 
 // -------------------------------------------------------------------------
+// Helmholtz M2M / L2L through LDS.  A block owns one output box and its
+// first 2 p_out + 1 threads one output coefficient m each.  Source boxes are
+// taken one at a time: the coefficient vector (2 p_in + 1 coefficients of
+// the chunk's right-hand sides) and the translation row (2 L + 1, centred)
+// are staged into LDS with coalesced loads, then
+//     out_m += sum_n a_n row[n - m]      (row sign-flipped at odd n - m: L2L)
+// reads only LDS (the a_n are broadcast, the rows contiguous across m).  The
+// per-level launches at coarse levels have few boxes and long rows, where
+// the plain one-thread-per-coefficient kernels wait on serial global loads.
+// (M2L keeps the plain kernel: its ~75 list-2 boxes per target box keep the
+// lanes busy from L1 and it measured faster than staging each box.)
+// -------------------------------------------------------------------------
+constexpr int HT_MAX_THREADS = 256;
+constexpr long HT_LDS_BUDGET = 32768;  // bytes of LDS per block for staged sources
+
+// doubles of LDS one staged source box takes
+__host__ __device__ __forceinline__ int ht_entry_doubles(int nrc, int p_in, int L)
+{
+    return 2 * nrc * (2 * p_in + 1) + 2 * (2 * L + 1);
+}
+
+// sa[n + p_in][q] <- coefficient n, right-hand side r0 + q; ag points at n = -p_in
+template <int NRC>
+__device__ __forceinline__ void ht_stage_coeffs(const KD &kd, double *sa, const double *ag,
+                                                int na, int tid, int nth)
+{
+    const int rc = kd.rc;
+    for (int i = tid; i < na * rc; i += nth) {
+        const int n = i / rc, q = i - n * rc;
+        const double *g = ag + 2 * ((long)n * kd.nr + kd.r0 + q);
+        sa[2 * (n * NRC + q)] = g[0];
+        sa[2 * (n * NRC + q) + 1] = g[1];
+    }
+}
+
+__device__ __forceinline__ void ht_stage_row(double *sr, const double *rg, int nrow, int tid,
+                                             int nth)
+{
+    for (int i = tid; i < 2 * nrow; i += nth) sr[i] = rg[i];
+}
+
+// acc[q] += sum_{n=-p1}^{p1} sa[n + p1][q] row[n - m]; row centred
+template <int NRC>
+__device__ __forceinline__ void ht_dot(int rc, const double *sa, int p1, int m,
+                                       const double *row, bool flip, cplx *acc)
+{
+    for (int n = -p1; n <= p1; ++n) {
+        const int d = n - m;
+        cplx w = {row[2 * d], row[2 * d + 1]};
+        if (flip && (d & 1)) w = -w;
+        const double *an = sa + 2 * NRC * (n + p1);
+#pragma unroll
+        for (int q = 0; q < NRC; ++q)
+            if (q < rc) acc[q] += cplx{an[2 * q], an[2 * q + 1]} * w;
+    }
+}
+
+// M2M: one block per source parent box at level lvp, children one at a time
+template <int NRC>
+__global__ void k_m2m_h(KD kd, TD t, int lvp, int i0, const int *spb, double *mpole)
+{
+    extern __shared__ double lds[];
+    const int P = spb[i0 + blockIdx.x];
+    const int lvc = lvp + 1, p1 = kd.order[lvc], p2 = kd.order[lvp], L = p1 + p2;
+    const int na = 2 * p1 + 1, nrow = 2 * L + 1;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    double *sa = lds, *sr = lds + 2 * NRC * na;
+    const bool act = tid < 2 * p2 + 1;
+    const int m = tid - p2;
+    cplx acc[NRC];
+#pragma unroll
+    for (int q = 0; q < NRC; ++q) acc[q] = {0, 0};
+    for (int c = 0; c < 4; ++c) {
+        const int C = t.child[(size_t)c * t.nboxes + P];
+        if (C < 0 || t.scumul[C] == 0) continue;
+        __syncthreads();
+        ht_stage_coeffs<NRC>(kd, sa, mpole + ecoef_off(kd, C, lvc) - 2L * p1 * kd.nr, na, tid,
+                             nth);
+        ht_stage_row(sr, kd.tab_mm + 2 * (kd.mm_off[lvc] + (long)c * nrow), nrow, tid, nth);
+        __syncthreads();
+        if (act) ht_dot<NRC>(kd.rc, sa, p1, m, sr + 2 * L, false, acc);
+    }
+    if (act) store_add<NRC>(kd, mpole + ecoef_off(kd, P, lvp) + 2L * m * kd.nr, acc);
+}
+
+// L2L: one block per target-or-target-parent box at level lv
+template <int NRC>
+__global__ void k_l2l_h(KD kd, TD t, int lv, int i0, const int *ttp, double *local)
+{
+    extern __shared__ double lds[];
+    const int P = ttp[i0 + blockIdx.x];
+    const int Q = t.parent[P];
+    if (Q < 0) return;
+    const int p2 = kd.order[lv], p1 = kd.order[lv - 1], L = p1 + p2;
+    const int na = 2 * p1 + 1, nrow = 2 * L + 1;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    double *sa = lds, *sr = lds + 2 * NRC * na;
+    // uhat_j(c_P - c_Q) = (-1)^j uhat_j(c_Q - c_P): the M2M row of P's child slot
+    const int c = (t.coords[P] & 1) | ((t.coords[(size_t)t.nboxes + P] & 1) << 1);
+    ht_stage_coeffs<NRC>(kd, sa, local + ecoef_off(kd, Q, lv - 1) - 2L * p1 * kd.nr, na, tid,
+                         nth);
+    ht_stage_row(sr, kd.tab_mm + 2 * (kd.mm_off[lv] + (long)c * nrow), nrow, tid, nth);
+    __syncthreads();
+    if (tid < 2 * p2 + 1) {
+        const int m = tid - p2;
+        cplx acc[NRC];
+#pragma unroll
+        for (int q = 0; q < NRC; ++q) acc[q] = {0, 0};
+        ht_dot<NRC>(kd.rc, sa, p1, m, sr + 2 * L, true, acc);
+        store_add<NRC>(kd, local + ecoef_off(kd, P, lv) + 2L * m * kd.nr, acc);
+    }
+}
+
+// -------------------------------------------------------------------------
 // evaluation at points of a chunk of expansions; a: ecoef_off-style pointer
 // (coefficient n, right-hand side r at a[2 (n nr + r)]); out[q] receives the
 // potential of right-hand side r0 + q
@@ -746,7 +1016,7 @@ __device__ void eval_mpole_add(const KD &kd, const double *a, int p, cplx c, dou
     const cplx e1 = {w.x / rho, w.y / rho};
     cplx e = {1, 0};
     cplx hm1, hm;
-    hankel01_fast<true, true>(x, hm1, hm);
+    hankel01_call(x, hm1, hm);
     const double tox = 2.0 / x;
     cplx acc[NRC];
 #pragma unroll
@@ -877,11 +1147,48 @@ __global__ void __launch_bounds__(64) k_p2p(KD kd, TD t, const int *ubox, const 
     if (active && sub == 0) store_add<NRC>(kd, pot + 2 * (size_t)ti * nr, acc);
 }
 
-// One 64-lane block per target box.  M2P gives each of the box's non-QBX
-// targets a group of G lanes (G a power of two, G * ntargets <= 64) that
-// splits the list-3 boxes and is reduced by shuffles, so boxes with few
-// targets still keep the wavefront busy.
-// MODE 1: M2P (list 3), 2: L2P (one lane per target)
+// M2P over balanced work units (see unit_plan): the unit's TG targets take
+// G lanes apiece that split the box's list-3 boxes; shuffle reduction.
+template <int NRC>
+__global__ void __launch_bounds__(64) k_m2p(KD kd, TD t, const int *ubox, const int *ut0,
+                                            const int *box_g, const int *tboxes,
+                                            const int *starts, const int *lists,
+                                            const int *fstart, const int *fcount,
+                                            const int *unfilt, const double *tgt, int nt,
+                                            const double *mpole, double *pot)
+{
+    const int u = blockIdx.x;
+    const int it = ubox[u];
+    const int B = tboxes[it];
+    const int G = box_g[it];
+    const int lane = threadIdx.x;
+    const int tsub = lane / G, sub = lane & (G - 1);
+    const int tl = ut0[u] + tsub;
+    const bool active = tl < fcount[B];
+    int ti = 0;
+    cplx acc[NRC];
+#pragma unroll
+    for (int q = 0; q < NRC; ++q) acc[q] = {0, 0};
+    if (active) {
+        ti = unfilt[fstart[B] + tl];
+        const cplx tp = ldpt(tgt, nt, ti);
+        for (int e = starts[it] + sub; e < starts[it + 1]; e += G) {
+            const int D = lists[e];
+            const int lv = t.level[D];
+            eval_mpole_add<NRC>(kd, mpole + ecoef_off(kd, D, lv), kd.order[lv], boxc(t, D),
+                                kd.rscale[lv], tp, acc);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NRC; ++q)
+        for (int m = G / 2; m >= 1; m >>= 1) {
+            acc[q].x += __shfl_xor(acc[q].x, m);
+            acc[q].y += __shfl_xor(acc[q].y, m);
+        }
+    if (active && sub == 0) store_add<NRC>(kd, pot + 2 * (size_t)ti * kd.nr, acc);
+}
+
+// L2P: one 64-lane block per target box, one lane per non-QBX target
 template <int MODE, int NRC>
 __global__ void __launch_bounds__(64) k_eval_targets(KD kd, TD t, int ntb, const int *tboxes,
                                                      const int *starts, const int *lists,
@@ -889,51 +1196,18 @@ __global__ void __launch_bounds__(64) k_eval_targets(KD kd, TD t, int ntb, const
                                                      const int *unfilt, const double *tgt,
                                                      int nt, const double *expn, double *pot)
 {
+    static_assert(MODE == 2, "k_eval_targets evaluates box locals only");
     const int it = blockIdx.x;
     if (it >= ntb) return;
     const int B = tboxes[it];
     const int f0 = fstart[B], nf = fcount[B];
-    if (nf == 0) return;
-    const int lane = threadIdx.x;
-    if (MODE == 2) {
-        const int lv = t.level[B];
-        for (int f = f0 + lane; f < f0 + nf; f += 64) {
-            const int ti = unfilt[f];
-            cplx v[NRC];
-            eval_local_n<NRC>(kd, expn + ecoef_off(kd, B, lv), kd.order[lv], boxc(t, B),
-                              kd.rscale[lv], ldpt(tgt, nt, ti), v);
-            store_add<NRC>(kd, pot + 2 * (size_t)ti * kd.nr, v);
-        }
-        return;
-    }
-    int G = 64;
-    while (G > 1 && G * nf > 64) G >>= 1;
-    const int tpr = 64 / G;
-    const int tsub = lane / G, sub = lane % G;
-    const int e0 = starts[it], e1 = starts[it + 1];
-    for (int base = 0; base < nf; base += tpr) {
-        const bool active = base + tsub < nf;
-        int ti = 0;
-        cplx acc[NRC];
-#pragma unroll
-        for (int q = 0; q < NRC; ++q) acc[q] = {0, 0};
-        if (active) {
-            ti = unfilt[f0 + base + tsub];
-            const cplx tp = ldpt(tgt, nt, ti);
-            for (int e = e0 + sub; e < e1; e += G) {
-                const int D = lists[e];
-                const int lv = t.level[D];
-                eval_mpole_add<NRC>(kd, expn + ecoef_off(kd, D, lv), kd.order[lv], boxc(t, D),
-                                    kd.rscale[lv], tp, acc);
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < NRC; ++q)
-            for (int m = G / 2; m >= 1; m >>= 1) {
-                acc[q].x += __shfl_xor(acc[q].x, m);
-                acc[q].y += __shfl_xor(acc[q].y, m);
-            }
-        if (active && sub == 0) store_add<NRC>(kd, pot + 2 * (size_t)ti * kd.nr, acc);
+    const int lv = t.level[B];
+    for (int f = f0 + threadIdx.x; f < f0 + nf; f += 64) {
+        const int ti = unfilt[f];
+        cplx v[NRC];
+        eval_local_n<NRC>(kd, expn + ecoef_off(kd, B, lv), kd.order[lv], boxc(t, B),
+                          kd.rscale[lv], ldpt(tgt, nt, ti), v);
+        store_add<NRC>(kd, pot + 2 * (size_t)ti * kd.nr, v);
     }
 }
 
@@ -976,150 +1250,218 @@ __device__ void qbx_store(const KD &kd, double *qexp, int c, const QAcc<PQ, NRC>
     }
 }
 
-constexpr int P2Q_LANES = 16;
-
-__device__ __forceinline__ double shfl_xor_d(double v, int m)
-{
-    return __shfl_xor(v, m, P2Q_LANES);
-}
-
-// P2QBXL: P2Q_LANES lanes per global center split the list-1 sources of the
-// center's target box; the order-PQ expansion stays in registers and is
-// reduced across the lane group with shuffles.  No atan2/sincos: the phase
-// e^{-i theta} is conj(w)/|w|.
+// P2QBXL contribution of one source (w = source - center, normal nu,
+// strengths qs) to the order-PQ expansion A about a center of radius R.  No
+// atan2/sincos: the phase e^{-i theta} is conj(w)/|w|.  tab: the Hankel
+// series coefficients (hankel01_wave).
 template <int PQ, int NRC>
-__global__ void __launch_bounds__(256) k_p2qbxl(KD kd, TD t, int ng, const int *gcent,
-                                                const int *c2tb, const double *ctr,
-                                                const double *rad, int nc, const int *l1s,
-                                                const int *l1l, const double *src, int ns,
-                                                const double *str, const double *nrm,
-                                                double *qexp, int accumulate)
+__device__ __forceinline__ void p2qbxl_pair(const KD &kd, cplx w, cplx nu, const cplx *qs,
+                                            double R, const double *tab, QAcc<PQ, NRC> &A)
 {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int g = gid / P2Q_LANES, lane = gid % P2Q_LANES;
-    const bool active = g < ng;
-    QAcc<PQ, NRC> A;
-    qacc_zero<PQ, NRC>(A);
-    int c = 0;
-    if (active) {
-        c = gcent[g];
-        const int it = c2tb[c];
-        const cplx cc = ldpt(ctr, nc, c);
-        const double R = rad[c];
-        const int e0 = l1s[it], e1 = l1s[it + 1];
-        for (int e = e0; e < e1; ++e) {
-            const int Sb = l1l[e];
-            const int s0 = t.sstart[Sb], s1 = s0 + t.scount[Sb];
-            for (int s = s0 + lane; s < s1; s += P2Q_LANES) {
-                const cplx w = ldpt(src, ns, s) - cc;
-                cplx qs[NRC];
+    if (kd.kernel == QBX_LAPLACE2D) {
+        const cplx iw = cinv(w);
+        const cplx tt = R * iw;
+        cplx pw = {1, 0};
+        if (!kd.dip) {
+            const cplx lg = clogz(-w);
 #pragma unroll
-                for (int q = 0; q < NRC; ++q)
-                    qs[q] = q < kd.rc ? ldq(kd, str, s, kd.r0 + q) : cplx{0, 0};
-                if (kd.kernel == QBX_LAPLACE2D) {
-                    const cplx iw = cinv(w);
-                    const cplx tt = R * iw;
-                    cplx pw = {1, 0};
-                    if (!kd.dip) {
-                        const cplx lg = clogz(-w);
+            for (int q = 0; q < NRC; ++q) A.v[q][0] += qs[q] * lg;
 #pragma unroll
-                        for (int q = 0; q < NRC; ++q) A.v[q][0] += qs[q] * lg;
+            for (int l = 1; l <= PQ; ++l) {
+                pw = pw * tt;
+                const cplx b = (-1.0 / l) * pw;
 #pragma unroll
-                        for (int l = 1; l <= PQ; ++l) {
-                            pw = pw * tt;
-                            const cplx b = (-1.0 / l) * pw;
+                for (int q = 0; q < NRC; ++q) A.v[q][l] += qs[q] * b;
+            }
+        } else {
+            const cplx f = nu * iw;
 #pragma unroll
-                            for (int q = 0; q < NRC; ++q) A.v[q][l] += qs[q] * b;
-                        }
-                    } else {
-                        const cplx f = ldpt(nrm, ns, s) * iw;
+            for (int q = 0; q < NRC; ++q) A.v[q][0] += qs[q] * f;
 #pragma unroll
-                        for (int q = 0; q < NRC; ++q) A.v[q][0] += qs[q] * f;
+            for (int l = 1; l <= PQ; ++l) {
+                pw = pw * tt;
+                const cplx b = f * pw;
 #pragma unroll
-                        for (int l = 1; l <= PQ; ++l) {
-                            pw = pw * tt;
-                            const cplx b = f * pw;
-#pragma unroll
-                            for (int q = 0; q < NRC; ++q) A.v[q][l] += qs[q] * b;
-                        }
-                    }
-                } else {
-                    const double r = sqrt(abs2(w));
-                    const double x = kd.k * r;
-                    const double ir = 1.0 / r;
-                    const cplx em = {w.x * ir, -w.y * ir};  // e^{-i theta}
-                    cplx H[PQ + 2];
-                    hankel01_fast<true, true>(x, H[0], H[1]);
-                    const double tox = 2.0 / x;
-#pragma unroll
-                    for (int m = 2; m <= PQ + 1; ++m)
-                        H[m] = ((m - 1) * tox) * H[m - 1] - H[m - 2];
-                    // Gv[m + PQ + 1] = H_m e^{-i m theta}, m in [-(PQ+1), PQ+1]
-                    cplx Gv[2 * PQ + 3];
-                    cplx e = {1, 0};
-#pragma unroll
-                    for (int m = 0; m <= PQ + 1; ++m) {
-                        Gv[PQ + 1 + m] = H[m] * e;
-                        const cplx gm = H[m] * conj(e);
-                        Gv[PQ + 1 - m] = (m & 1) ? -gm : gm;
-                        e = e * em;
-                    }
-                    if (!kd.dip) {
-#pragma unroll
-                        for (int n = -PQ; n <= PQ; ++n)
-#pragma unroll
-                            for (int q = 0; q < NRC; ++q)
-                                A.v[q][n + PQ] += qs[q] * Gv[n + PQ + 1];
-                    } else {
-                        const cplx nu = ldpt(nrm, ns, s);
-                        const double hk = 0.5 * kd.k;
-                        const cplx a1 = hk * conj(nu), a2 = hk * nu;
-#pragma unroll
-                        for (int n = -PQ; n <= PQ; ++n) {
-                            const cplx b = a1 * Gv[n + PQ] - a2 * Gv[n + PQ + 2];
-#pragma unroll
-                            for (int q = 0; q < NRC; ++q) A.v[q][n + PQ] += qs[q] * b;
-                        }
-                    }
-                }
+                for (int q = 0; q < NRC; ++q) A.v[q][l] += qs[q] * b;
             }
         }
+        return;
     }
+    const double r = sqrt(abs2(w));
+    const double x = kd.k * r;
+    const double ir = 1.0 / r;
+    const cplx em = {w.x * ir, -w.y * ir};  // e^{-i theta}
+    cplx H[PQ + 2];
+    hankel01_wave<true, true>(x, tab, H[0], H[1]);
+    const double tox = 2.0 / x;
+#pragma unroll
+    for (int m = 2; m <= PQ + 1; ++m) H[m] = ((m - 1) * tox) * H[m - 1] - H[m - 2];
+    // Gv[m + PQ + 1] = H_m e^{-i m theta}, m in [-(PQ+1), PQ+1]
+    cplx Gv[2 * PQ + 3];
+    cplx e = {1, 0};
+#pragma unroll
+    for (int m = 0; m <= PQ + 1; ++m) {
+        Gv[PQ + 1 + m] = H[m] * e;
+        const cplx gm = H[m] * conj(e);
+        Gv[PQ + 1 - m] = (m & 1) ? -gm : gm;
+        e = e * em;
+    }
+    if (!kd.dip) {
+#pragma unroll
+        for (int n = -PQ; n <= PQ; ++n)
+#pragma unroll
+            for (int q = 0; q < NRC; ++q) A.v[q][n + PQ] += qs[q] * Gv[n + PQ + 1];
+    } else {
+        const double hk = 0.5 * kd.k;
+        const cplx a1 = hk * conj(nu), a2 = hk * nu;
+#pragma unroll
+        for (int n = -PQ; n <= PQ; ++n) {
+            const cplx b = a1 * Gv[n + PQ] - a2 * Gv[n + PQ + 2];
+#pragma unroll
+            for (int q = 0; q < NRC; ++q) A.v[q][n + PQ] += qs[q] * b;
+        }
+    }
+}
+
+// reduce A over the G lanes of a group (consecutive lanes, G a power of two)
+template <int PQ, int NRC>
+__device__ __forceinline__ void qacc_group_reduce(QAcc<PQ, NRC> &A, int G)
+{
 #pragma unroll
     for (int q = 0; q < NRC; ++q)
 #pragma unroll
-        for (int i = 0; i < 2 * PQ + 1; ++i) {
-#pragma unroll
-            for (int m = P2Q_LANES / 2; m >= 1; m >>= 1) {
-                A.v[q][i].x += shfl_xor_d(A.v[q][i].x, m);
-                A.v[q][i].y += shfl_xor_d(A.v[q][i].y, m);
+        for (int i = 0; i < 2 * PQ + 1; ++i)
+            for (int m = G / 2; m >= 1; m >>= 1) {
+                A.v[q][i].x += __shfl_xor(A.v[q][i].x, m);
+                A.v[q][i].y += __shfl_xor(A.v[q][i].y, m);
             }
-        }
-    if (active && lane == 0) qbx_store<PQ, NRC>(kd, qexp, c, A, accumulate);
 }
 
-constexpr int M2Q_LANES = 4;
+// P2QBXL over balanced work units (see unit_plan): the unit's TG global
+// centers of one target box take G lanes apiece; the box's list-1 sources are
+// staged through LDS in chunks (positions, normals, strengths) and each
+// center's lanes split a chunk; the expansions are reduced by shuffles.
+template <int NRC>
+struct P2QTile {
+    static constexpr int CH = NRC == 1 ? 256 : 64;
+};
 
-// M2QBXL: M2Q_LANES lanes per global center split its list-3 boxes
 template <int PQ, int NRC>
-__global__ void __launch_bounds__(256) k_m2qbxl(KD kd, TD t, const double *B, int ng,
-                                                const int *gcent, const int *c2tb,
-                                                const double *ctr, const double *rad, int nc,
-                                                const int *l3s, const int *l3l,
-                                                const double *mpole, double *qexp,
-                                                int accumulate)
+__global__ void __launch_bounds__(64) k_p2qbxl(KD kd, TD t, const int *ubox, const int *ut0,
+                                               const int *box_g, const int *gcs,
+                                               const int *gcl, const double *ctr,
+                                               const double *rad, int nc, const int *l1s,
+                                               const int *l1l, const double *src, int ns,
+                                               const double *str, const double *nrm,
+                                               double *qexp, int accumulate)
 {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int g = gid / M2Q_LANES, sub = gid % M2Q_LANES;
-    const bool active = g < ng;
-    const int c = active ? gcent[g] : 0;
-    const int it = active ? c2tb[c] : 0;
+    constexpr int CH = P2QTile<NRC>::CH;
+    __shared__ double sx[CH], sy[CH], snx[CH], sny[CH];
+    __shared__ double sq[CH * 2 * NRC];
+    __shared__ double htab[4 * HANKEL_NSER];
+    const int u = blockIdx.x;
+    const int it = ubox[u];
+    const int G = box_g[it];
+    const int lane = threadIdx.x;
+    const int tsub = lane / G, sub = lane & (G - 1);
+    const int cl = ut0[u] + tsub;
+    const bool active = cl < gcs[it + 1] - gcs[it];
+    const bool dip = kd.dip;
+    const int rc = kd.rc, r0 = kd.r0, nr = kd.nr;
+    int c = 0;
+    cplx cc = {0, 0};
+    double R = 1.0;
+    if (active) {
+        c = gcl[gcs[it] + cl];
+        cc = ldpt(ctr, nc, c);
+        R = rad[c];
+    }
+    hankel_table_fill(htab, lane, 64);
+    QAcc<PQ, NRC> A;
+    qacc_zero<PQ, NRC>(A);
+
+    auto flush = [&](int n) {
+        __syncthreads();
+        if (active) {
+            for (int j = sub; j < n; j += G) {
+                // an offset the compiler cannot see through keeps the series
+                // coefficients as LDS reads inside the loop
+                int toff = 0;
+                asm volatile("" : "+s"(toff));
+                cplx qs[NRC];
+#pragma unroll
+                for (int q = 0; q < NRC; ++q)
+                    qs[q] = q < rc ? cplx{sq[2 * (j * NRC + q)], sq[2 * (j * NRC + q) + 1]}
+                                   : cplx{0, 0};
+                const cplx nu = dip ? cplx{snx[j], sny[j]} : cplx{0, 0};
+                p2qbxl_pair<PQ, NRC>(kd, cplx{sx[j], sy[j]} - cc, nu, qs, R, htab + toff, A);
+            }
+        }
+        __syncthreads();
+    };
+
+    int fill = 0;
+    const int e1 = l1s[it + 1];
+    for (int e = l1s[it]; e < e1; ++e) {
+        const int Sb = l1l[e];
+        int s = t.sstart[Sb];
+        const int s1 = s + t.scount[Sb];
+        while (s < s1) {
+            const int take = min(s1 - s, CH - fill);
+            for (int j = lane; j < take; j += 64) {
+                const int si = s + j, d = fill + j;
+                sx[d] = src[si];
+                sy[d] = src[(size_t)ns + si];
+                if (dip) {
+                    snx[d] = nrm[si];
+                    sny[d] = nrm[(size_t)ns + si];
+                }
+#pragma unroll
+                for (int q = 0; q < NRC; ++q)
+                    if (q < rc) {
+                        const double *pq = str + 2 * ((size_t)si * nr + r0 + q);
+                        sq[2 * (d * NRC + q)] = pq[0];
+                        sq[2 * (d * NRC + q) + 1] = pq[1];
+                    }
+            }
+            fill += take;
+            s += take;
+            if (fill == CH) {
+                flush(CH);
+                fill = 0;
+            }
+        }
+    }
+    if (fill > 0) flush(fill);
+    qacc_group_reduce<PQ, NRC>(A, G);
+    if (active && sub == 0) qbx_store<PQ, NRC>(kd, qexp, c, A, accumulate);
+}
+
+// M2QBXL over balanced work units (see unit_plan): the unit's TG global
+// centers of one target box take G lanes apiece that split the box's list-3
+// boxes; shuffle reduction, then the expansions are added.
+template <int PQ, int NRC>
+__global__ void __launch_bounds__(64) k_m2qbxl(KD kd, TD t, const double *B, const int *ubox,
+                                               const int *ut0, const int *box_g,
+                                               const int *gcs, const int *gcl,
+                                               const double *ctr, const double *rad, int nc,
+                                               const int *l3s, const int *l3l,
+                                               const double *mpole, double *qexp)
+{
+    const int u = blockIdx.x;
+    const int it = ubox[u];
+    const int G = box_g[it];
+    const int lane = threadIdx.x;
+    const int tsub = lane / G, sub = lane & (G - 1);
+    const int cl = ut0[u] + tsub;
+    const bool active = cl < gcs[it + 1] - gcs[it];
+    const int c = active ? gcl[gcs[it] + cl] : 0;
     const cplx cc = active ? ldpt(ctr, nc, c) : cplx{0, 0};
     const double R2 = active ? rad[c] : 1.0;
     QAcc<PQ, NRC> A;
     qacc_zero<PQ, NRC>(A);
     const int ebeg = active ? l3s[it] + sub : 0, eend = active ? l3s[it + 1] : 0;
-    for (int e = ebeg; e < eend; e += M2Q_LANES) {
+    for (int e = ebeg; e < eend; e += G) {
         const int D = l3l[e];
         const int lv = t.level[D];
         const int p1 = kd.order[lv];
@@ -1164,7 +1506,7 @@ __global__ void __launch_bounds__(256) k_m2qbxl(KD kd, TD t, const double *B, in
             const cplx e1 = {d.x / rho, d.y / rho};
             cplx ep = {1, 0};
             cplx hm1, hm;
-            hankel01_fast<true, true>(x, hm1, hm);
+            hankel01_call(x, hm1, hm);
             const double tox = 2.0 / x;
             const int lmax = p1 + PQ;
             cplx h = hm1;
@@ -1196,17 +1538,8 @@ __global__ void __launch_bounds__(256) k_m2qbxl(KD kd, TD t, const double *B, in
             }
         }
     }
-#pragma unroll
-    for (int q = 0; q < NRC; ++q)
-#pragma unroll
-        for (int i = 0; i < 2 * PQ + 1; ++i) {
-#pragma unroll
-            for (int m = M2Q_LANES / 2; m >= 1; m >>= 1) {
-                A.v[q][i].x += __shfl_xor(A.v[q][i].x, m, M2Q_LANES);
-                A.v[q][i].y += __shfl_xor(A.v[q][i].y, m, M2Q_LANES);
-            }
-        }
-    if (active && sub == 0) qbx_store<PQ, NRC>(kd, qexp, c, A, accumulate);
+    qacc_group_reduce<PQ, NRC>(A, G);
+    if (active && sub == 0) qbx_store<PQ, NRC>(kd, qexp, c, A, 1);
 }
 
 template <int PQ, int NRC>
@@ -1461,6 +1794,47 @@ static TD make_td(const qbx_tree *T)
 }
 
 // thread layout over a level-sorted box list: p+1 (half) or ncl threads per box
+// Launch shape of the LDS translation kernels (Helmholtz only): threads
+// cover the 2 p_out + 1 outputs and one staged source box fits in 64 KiB.
+// False means the plain kernels are used instead.
+struct HtLaunch {
+    int threads = 0, p_in = 0, L = 0;
+};
+
+template <int NRC>
+static size_t ht_bytes(const HtLaunch &h)
+{
+    return 8UL * ht_entry_doubles(NRC, h.p_in, h.L);
+}
+
+static bool ht_launch(const KD &kd, int p_out, int p_in, int L, HtLaunch *h)
+{
+    if (kd.kernel != QBX_HELMHOLTZ2D) return false;
+    const int nout = 2 * p_out + 1;
+    if (nout > HT_MAX_THREADS) return false;
+    if (8L * ht_entry_doubles(NRC_S, p_in, L) > 65536) return false;
+    h->threads = 64 * ((nout + 63) / 64);
+    h->p_in = p_in;
+    h->L = L;
+    return true;
+}
+
+// Shape of k_p2e_h at order p: S sources per chunk within the LDS budget,
+// threads covering max(2 p + 1, S).  False: use the plain kernels.
+static bool p2e_launch(const KD &kd, int p, P2EShape *sh, int *threads, size_t *bytes)
+{
+    if (kd.kernel != QBX_HELMHOLTZ2D || 2 * p + 1 > HT_MAX_THREADS) return false;
+    P2EShape h;
+    h.R = (p + 2) | 1;
+    const long per = 8L * (4 * h.R + 2 * NRC_S + 2 + 1);
+    h.S = (int)std::min(64L, HT_LDS_BUDGET / per);
+    if (h.S < 4) return false;
+    *sh = h;
+    *threads = 64 * ((std::max(2 * p + 1, h.S) + 63) / 64);
+    *bytes = 8UL * h.doubles(NRC_S);
+    return true;
+}
+
 static Flat make_flat(const KD &kd, const std::vector<int> &level_starts, bool half)
 {
     Flat f;
@@ -1558,6 +1932,35 @@ int qbx_form_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx
     QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
     hipStream_t st = S(stream);
     QBX_TRY(zero_fill(mpole, (exp_bytes(kd)) / 8, st));
+    if (kd.kernel == QBX_HELMHOLTZ2D) {
+        bool ok = true;
+        for (int l = 0; l < T->nlevels; ++l) {
+            P2EShape sh;
+            int th;
+            size_t by;
+            ok = ok && p2e_launch(kd, kd.order[l], &sh, &th, &by);
+        }
+        if (ok) {
+            const TD td = make_td(T);
+            for (int l = 0; l < T->nlevels; ++l) {
+                const int i0 = R->h_level_start_source_box_nrs[l];
+                const int i1 = R->h_level_start_source_box_nrs[l + 1];
+                if (i1 == i0) continue;
+                P2EShape sh;
+                int th;
+                size_t by;
+                p2e_launch(kd, kd.order[l], &sh, &th, &by);
+#define L_P2MH(N)                                                                          \
+    hipLaunchKernelGGL((k_p2e_h<N, false>), dim3(i1 - i0), dim3(th), by, st, kd, td, l, i0, sh, \
+                       R->source_boxes.p, nullptr, nullptr, T->sources.p, T->nsources, str,  \
+                       nrm, mpole)
+                FOR_RHS_CHUNKS(kd, NRC_S, L_P2MH);
+#undef L_P2MH
+            }
+            QBX_HIP(hipGetLastError());
+            return QBX_OK;
+        }
+    }
     const Flat fl = make_flat(kd, R->h_level_start_source_box_nrs, true);
     const long nthr = fl.tstart[fl.nl];
     if (nthr > 0) {
@@ -1588,13 +1991,21 @@ int qbx_coarsen_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const 
         const int i0 = R->h_level_start_source_parent_box_nrs[l];
         const int i1 = R->h_level_start_source_parent_box_nrs[l + 1];
         const long nthr = (long)(i1 - i0) * kd.ncl[l];
-        if (nthr > 0) {
+        if (nthr == 0) continue;
+        HtLaunch hl;
+        if (ht_launch(kd, kd.order[l], kd.order[l + 1], kd.order[l] + kd.order[l + 1], &hl)) {
+#define L_M2MH(N)                                                                          \
+    hipLaunchKernelGGL(k_m2m_h<N>, dim3(i1 - i0), dim3(hl.threads), ht_bytes<N>(hl), st, kd, \
+                       td, l, i0, R->source_parent_boxes.p, mpole)
+            FOR_RHS_CHUNKS(kd, NRC_S, L_M2MH);
+#undef L_M2MH
+            continue;
+        }
 #define L_M2M(N)                                                                           \
     hipLaunchKernelGGL(k_m2m<N>, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, td, B, l, i0, \
                        i1, R->source_parent_boxes.p, mpole)
-            FOR_RHS_CHUNKS(kd, NRC_S, L_M2M);
+        FOR_RHS_CHUNKS(kd, NRC_S, L_M2M);
 #undef L_M2M
-        }
     }
     QBX_HIP(hipGetLastError());
     return QBX_OK;
@@ -1618,6 +2029,20 @@ static int eval_targets(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx
     const TD td = make_td(T);
     const int *starts = mode == 1 ? R->l3_starts.p : R->l1_starts.p;
     const int *lists = mode == 1 ? R->l3_lists.p : R->l1_lists.p;
+    if (mode == 1) {
+        const unit_plan &U = G->m2p_plan;
+        QBX_CHECK(G->trav == R, "qbx_eval_multipoles: geometry was built for another traversal");
+        if (U.nunits == 0) return QBX_OK;
+#define M2P_LAUNCH(N)                                                                      \
+    hipLaunchKernelGGL((k_m2p<N>), dim3(U.nunits), dim3(64), 0, st, kd, td, U.unit_box.p,  \
+                       U.unit_t0.p, U.box_g.p, R->target_boxes.p, starts, lists,          \
+                       G->nqbtl_box_target_starts.p, G->nqbtl_box_target_counts_nonchild.p, \
+                       G->nqbtl_unfiltered.p, T->targets.p, T->ntargets, expn, pot)
+        FOR_RHS_CHUNKS(kd, NRC_S, M2P_LAUNCH);
+#undef M2P_LAUNCH
+        QBX_HIP(hipGetLastError());
+        return QBX_OK;
+    }
     if (mode == 0) {
         const unit_plan &U = G->p2p_plan;
         QBX_CHECK(G->trav == R, "qbx_eval_direct: geometry was built for another traversal");
@@ -1638,11 +2063,8 @@ static int eval_targets(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx
                        R->target_boxes.p, starts, lists, G->nqbtl_box_target_starts.p,    \
                        G->nqbtl_box_target_counts_nonchild.p, G->nqbtl_unfiltered.p,      \
                        T->targets.p, T->ntargets, expn, pot)
-#define EV1(N) EVAL_LAUNCH(1, N)
 #define EV2(N) EVAL_LAUNCH(2, N)
-    if (mode == 1) FOR_RHS_CHUNKS(kd, NRC_S, EV1);
-    else FOR_RHS_CHUNKS(kd, NRC_S, EV2);
-#undef EV1
+    FOR_RHS_CHUNKS(kd, NRC_S, EV2);
 #undef EV2
 #undef EVAL_LAUNCH
     QBX_HIP(hipGetLastError());
@@ -1687,10 +2109,11 @@ int qbx_multipole_to_local(qbx_ctx *ctx, void *stream, const qbx_tree *T, const 
     hipStream_t st = S(stream);
     QBX_TRY(ensure_tables(ctx, st, T, &kd));
     QBX_TRY(zero_fill(local, (exp_bytes(kd)) / 8, st));
+    if (R->n2 == 0) return QBX_OK;
+    const TD td = make_td(T);
     const Flat fl = make_flat(kd, R->h_level_start_ttp_box_nrs, false);
     const long nthr = fl.tstart[fl.nl];
-    if (nthr > 0 && R->n2 > 0) {
-        const TD td = make_td(T);
+    if (nthr > 0) {
 #define L_M2L(N)                                                                           \
     hipLaunchKernelGGL(k_m2l<N>, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, td, B, fl,   \
                        R->ttp_boxes.p, R->l2_starts.p, R->l2_lists.p, mpole, local)
@@ -1713,6 +2136,35 @@ int qbx_form_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_tra
     QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
     hipStream_t st = S(stream);
     if (!accumulate) QBX_TRY(zero_fill(local, (exp_bytes(kd)) / 8, st));
+    if (kd.kernel == QBX_HELMHOLTZ2D && R->n4 > 0) {
+        bool ok = true;
+        for (int l = 0; l < T->nlevels; ++l) {
+            P2EShape sh;
+            int th;
+            size_t by;
+            ok = ok && p2e_launch(kd, kd.order[l], &sh, &th, &by);
+        }
+        if (ok) {
+            const TD td = make_td(T);
+            for (int l = 0; l < T->nlevels; ++l) {
+                const int i0 = R->h_level_start_ttp_box_nrs[l];
+                const int i1 = R->h_level_start_ttp_box_nrs[l + 1];
+                if (i1 == i0) continue;
+                P2EShape sh;
+                int th;
+                size_t by;
+                p2e_launch(kd, kd.order[l], &sh, &th, &by);
+#define L_P2LH(N)                                                                          \
+    hipLaunchKernelGGL((k_p2e_h<N, true>), dim3(i1 - i0), dim3(th), by, st, kd, td, l, i0, sh, \
+                       R->ttp_boxes.p, R->l4_starts.p, R->l4_lists.p, T->sources.p,         \
+                       T->nsources, str, nrm, local)
+                FOR_RHS_CHUNKS(kd, NRC_S, L_P2LH);
+#undef L_P2LH
+            }
+            QBX_HIP(hipGetLastError());
+            return QBX_OK;
+        }
+    }
     const Flat fl = make_flat(kd, R->h_level_start_ttp_box_nrs, true);
     const long nthr = fl.tstart[fl.nl];
     if (nthr > 0 && R->n4 > 0) {
@@ -1743,13 +2195,21 @@ int qbx_refine_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_t
     for (int l = 1; l < T->nlevels; ++l) {
         const int i0 = R->h_level_start_ttp_box_nrs[l], i1 = R->h_level_start_ttp_box_nrs[l + 1];
         const long nthr = (long)(i1 - i0) * kd.ncl[l];
-        if (nthr > 0) {
+        if (nthr == 0) continue;
+        HtLaunch hl;
+        if (ht_launch(kd, kd.order[l], kd.order[l - 1], kd.order[l] + kd.order[l - 1], &hl)) {
+#define L_L2LH(N)                                                                          \
+    hipLaunchKernelGGL(k_l2l_h<N>, dim3(i1 - i0), dim3(hl.threads), ht_bytes<N>(hl), st, kd, \
+                       td, l, i0, R->ttp_boxes.p, local)
+            FOR_RHS_CHUNKS(kd, NRC_S, L_L2LH);
+#undef L_L2LH
+            continue;
+        }
 #define L_L2L(N)                                                                           \
     hipLaunchKernelGGL(k_l2l<N>, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, td, B, l, i0, \
                        i1, R->ttp_boxes.p, local)
-            FOR_RHS_CHUNKS(kd, NRC_S, L_L2L);
+        FOR_RHS_CHUNKS(kd, NRC_S, L_L2L);
 #undef L_L2L
-        }
     }
     QBX_HIP(hipGetLastError());
     return QBX_OK;
@@ -1768,14 +2228,15 @@ int qbx_form_global_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, co
     hipStream_t st = S(stream);
     if (!accumulate)
         QBX_TRY(zero_fill(qexp, (sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq * kd.nr) / 8, st));
-    const int ng = G->nglobal;
-    if (ng == 0) return QBX_OK;
+    const unit_plan &U = G->p2q_plan;
+    QBX_CHECK(G->trav == R, "P2QBXL: geometry was built for another traversal");
+    if (G->nglobal == 0 || U.nunits == 0) return QBX_OK;
     const TD td = make_td(T);
 #define L_P2Q(N)                                                                           \
-    DISPATCH_PQ_NR(kd.pq, N, k_p2qbxl, dim3(cdiv((long)ng * P2Q_LANES, 256)), dim3(256), 0, \
-                   st, kd, td, ng, G->global_qbx_centers.p, G->center_to_target_box.p,     \
-                   G->centers.p, G->radii.p, G->ncenters, R->l1_starts.p, R->l1_lists.p,   \
-                   T->sources.p, T->nsources, str, nrm, qexp, accumulate)
+    DISPATCH_PQ_NR(kd.pq, N, k_p2qbxl, dim3(U.nunits), dim3(64), 0, st, kd, td,            \
+                   U.unit_box.p, U.unit_t0.p, U.box_g.p, G->gc_by_tbox_starts.p,            \
+                   G->gc_by_tbox.p, G->centers.p, G->radii.p, G->ncenters, R->l1_starts.p,  \
+                   R->l1_lists.p, T->sources.p, T->nsources, str, nrm, qexp, accumulate)
     FOR_RHS_CHUNKS(kd, NRC_Q, L_P2Q);
 #undef L_P2Q
     QBX_HIP(hipGetLastError());
@@ -1798,14 +2259,15 @@ int qbx_translate_box_multipoles_to_qbx_local(qbx_ctx *ctx, void *stream, const 
     hipStream_t st = S(stream);
     if (!accumulate)
         QBX_TRY(zero_fill(qexp, (sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq * kd.nr) / 8, st));
-    const int ng = G->nglobal;
-    if (ng == 0 || R->n3 == 0) return QBX_OK;
+    const unit_plan &U = G->m2q_plan;
+    QBX_CHECK(G->trav == R, "M2QBXL: geometry was built for another traversal");
+    if (G->nglobal == 0 || R->n3 == 0 || U.nunits == 0) return QBX_OK;
     const TD td = make_td(T);
 #define L_M2Q(N)                                                                           \
-    DISPATCH_PQ_NR(kd.pq, N, k_m2qbxl, dim3(cdiv((long)ng * M2Q_LANES, 256)), dim3(256), 0, \
-                   st, kd, td, B, ng, G->global_qbx_centers.p, G->center_to_target_box.p,  \
-                   G->centers.p, G->radii.p, G->ncenters, R->l3_starts.p, R->l3_lists.p,   \
-                   mpole, qexp, 1)
+    DISPATCH_PQ_NR(kd.pq, N, k_m2qbxl, dim3(U.nunits), dim3(64), 0, st, kd, td, B,         \
+                   U.unit_box.p, U.unit_t0.p, U.box_g.p, G->gc_by_tbox_starts.p,            \
+                   G->gc_by_tbox.p, G->centers.p, G->radii.p, G->ncenters, R->l3_starts.p,  \
+                   R->l3_lists.p, mpole, qexp)
     FOR_RHS_CHUNKS(kd, NRC_Q, L_M2Q);
 #undef L_M2Q
     QBX_HIP(hipGetLastError());

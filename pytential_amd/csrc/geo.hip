@@ -67,26 +67,33 @@ __global__ void k_nqbtl_boxes(int nb, const int *tstarts, const int *tcounts, co
     fcounts[b] = pos[s + tcounts[b]] - pos[s];
 }
 
-// per target box: units and lanes per target from its non-QBX target count
-// and the sources of its list-1 boxes
+// per target box: units and lanes per target from its target count (the
+// box's non-QBX targets, fcount by box id, or the global QBX centers of the
+// box, cstarts by target box index) and its list work: the sources of its
+// list boxes (scount given) or the number of list entries (scount null)
 __global__ void k_unit_work(int ntb, const int *tboxes, const int *starts, const int *lists,
-                            const int *scount, const int *fcount, int *units, int *glanes)
+                            const int *scount, const int *fcount, const int *cstarts, int iter,
+                            int *units, int *glanes)
 {
     int it = blockIdx.x * blockDim.x + threadIdx.x;
     if (it >= ntb) return;
-    const int nf = fcount[tboxes[it]];
+    const int nf = cstarts ? cstarts[it + 1] - cstarts[it] : fcount[tboxes[it]];
     long work = 0;
-    for (int e = starts[it]; e < starts[it + 1]; ++e) work += scount[lists[e]];
+    if (scount)
+        for (int e = starts[it]; e < starts[it + 1]; ++e) work += scount[lists[e]];
+    else
+        work = starts[it + 1] - starts[it];
     int g = 1;
-    while (g < 64 && 2 * g * nf <= 64) g <<= 1;                  // fill the wavefront
-    while (g < 64 && (long)g * UNIT_ITER < work) g <<= 1;        // bound a lane's walk
+    while (g < 64 && 2 * g * nf <= 64) g <<= 1;             // fill the wavefront
+    while (g < 64 && (long)g * iter < work) g <<= 1;        // bound a lane's walk
     const int tg = 64 / g;
     units[it] = (nf > 0 && work > 0) ? (nf + tg - 1) / tg : 0;
     glanes[it] = g;
 }
 
 int build_unit_plan(hipStream_t st, const qbx_tree *T, const qbx_trav *R, const qbx_geo *G,
-                    unit_plan &plan)
+                    const int *starts, const int *lists, bool by_sources, bool centers,
+                    int iter, unit_plan &plan)
 {
     const int ntb = R->ntarget_boxes;
     plan.nunits = 0;
@@ -95,9 +102,10 @@ int build_unit_plan(hipStream_t st, const qbx_tree *T, const qbx_trav *R, const 
     dbuf<int> units;
     QBX_TRY(units.alloc(ntb));
     hipLaunchKernelGGL(k_unit_work, dim3(cdiv(ntb, 256)), dim3(256), 0, st, ntb,
-                       R->target_boxes.p, R->l1_starts.p, R->l1_lists.p,
-                       T->box_source_counts_nonchild.p, G->nqbtl_box_target_counts_nonchild.p,
-                       units.p, plan.box_g.p);
+                       R->target_boxes.p, starts, lists,
+                       by_sources ? T->box_source_counts_nonchild.p : nullptr,
+                       G->nqbtl_box_target_counts_nonchild.p,
+                       centers ? G->gc_by_tbox_starts.p : nullptr, iter, units.p, plan.box_g.p);
     QBX_HIP(hipGetLastError());
     std::vector<int> hu(ntb), hg(ntb);
     QBX_HIP(hipMemcpyAsync(hu.data(), units.p, sizeof(int) * ntb, hipMemcpyDeviceToHost, st));
@@ -121,6 +129,40 @@ int build_unit_plan(hipStream_t st, const qbx_tree *T, const qbx_trav *R, const 
         QBX_HIP(hipMemcpyAsync(plan.unit_t0.p, ut.data(), sizeof(int) * plan.nunits,
                                hipMemcpyHostToDevice, st));
     }
+    QBX_HIP(hipStreamSynchronize(st));
+    return QBX_OK;
+}
+
+__global__ void k_gc_keys(int ng, const int *gc, const int *c2tb, int *keys, int *counts)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ng) return;
+    const int it = c2tb[gc[i]];
+    keys[i] = it;
+    atomicAdd(&counts[it], 1);
+}
+
+// global QBX centers grouped by target box (ascending center id within a box)
+static int build_gc_by_tbox(qbx_ctx *ctx, hipStream_t st, const qbx_trav *R, qbx_geo *G)
+{
+    const int ntb = R->ntarget_boxes, ng = G->nglobal;
+    QBX_TRY(G->gc_by_tbox_starts.alloc(ntb + 1));
+    QBX_TRY(G->gc_by_tbox.alloc(std::max(ng, 1)));
+    dbuf<int> keys, keys2, counts;
+    QBX_TRY(keys.alloc(std::max(ng, 1)));
+    QBX_TRY(keys2.alloc(std::max(ng, 1)));
+    QBX_TRY(counts.alloc(std::max(ntb, 1)));
+    QBX_HIP(hipMemsetAsync(counts.p, 0, sizeof(int) * std::max(ntb, 1), st));
+    if (ng > 0) {
+        hipLaunchKernelGGL(k_gc_keys, dim3(cdiv(ng, 256)), dim3(256), 0, st, ng,
+                           G->global_qbx_centers.p, G->center_to_target_box.p, keys.p, counts.p);
+        QBX_HIP(hipGetLastError());
+        int bits = 1;
+        while ((1L << bits) < (long)ntb + 1) ++bits;
+        QBX_TRY(sort_pairs_i32(ctx, st, keys.p, keys2.p, G->global_qbx_centers.p,
+                               G->gc_by_tbox.p, ng, bits));
+    }
+    if (ntb > 0) QBX_TRY(exclusive_scan(ctx, st, counts.p, G->gc_by_tbox_starts.p, ntb, nullptr));
     QBX_HIP(hipStreamSynchronize(st));
     return QBX_OK;
 }
@@ -207,7 +249,15 @@ static int build_geo(qbx_ctx *ctx, hipStream_t st, const qbx_tree *T, const qbx_
         QBX_HIP(hipGetLastError());
     }
     G->trav = R;
-    QBX_TRY(build_unit_plan(st, T, R, G, G->p2p_plan));
+    QBX_TRY(build_gc_by_tbox(ctx, st, R, G));
+    QBX_TRY(build_unit_plan(st, T, R, G, R->l1_starts.p, R->l1_lists.p, true, false, UNIT_ITER,
+                            G->p2p_plan));
+    QBX_TRY(build_unit_plan(st, T, R, G, R->l3_starts.p, R->l3_lists.p, false, false,
+                            M2P_UNIT_ITER, G->m2p_plan));
+    QBX_TRY(build_unit_plan(st, T, R, G, R->l1_starts.p, R->l1_lists.p, true, true,
+                            P2Q_UNIT_ITER, G->p2q_plan));
+    QBX_TRY(build_unit_plan(st, T, R, G, R->l3_starts.p, R->l3_lists.p, false, true,
+                            M2Q_UNIT_ITER, G->m2q_plan));
     QBX_HIP(hipStreamSynchronize(st));
     return QBX_OK;
 }

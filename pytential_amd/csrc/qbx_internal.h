@@ -68,14 +68,18 @@ struct dbuf {
 
 constexpr int MAX_LEVELS = 64;
 
-// Balanced work units for the per-target-box evaluation kernels.  A unit is
-// one 64-lane wavefront over TG = 64 / G consecutive non-QBX targets of one
-// target box, G lanes per target splitting the box's interaction list; G is
-// chosen per box from its list work (list-1 sources for P2P) so that no lane
-// walks more than about UNIT_ITER items, and so that boxes with few targets
+// Balanced work units for the per-target-box kernels.  A unit is one 64-lane
+// wavefront over TG = 64 / G consecutive "targets" of one target box (its
+// non-QBX targets for P2P / M2P, its global QBX centers for P2QBXL / M2QBXL),
+// G lanes per target splitting the box's interaction list; G is chosen per
+// box from its list work (list-1 sources, or list-3 boxes) so that no lane
+// walks more than about *_UNIT_ITER items, and so that boxes with few targets
 // still fill the wavefront.  A target belongs to exactly one unit, so units
 // write their targets' sums without atomics.
-constexpr int UNIT_ITER = 64;
+constexpr int UNIT_ITER = 64;      // P2P: list-1 sources per lane
+constexpr int M2P_UNIT_ITER = 8;   // M2P: list-3 boxes per lane
+constexpr int P2Q_UNIT_ITER = 32;  // P2QBXL: list-1 sources per lane
+constexpr int M2Q_UNIT_ITER = 2;   // M2QBXL: list-3 boxes per lane
 
 struct unit_plan {
     int nunits = 0;
@@ -142,9 +146,12 @@ struct qbx_geo {
     qbx::dbuf<int> nqbtl_box_target_starts, nqbtl_box_target_counts_nonchild;
     qbx::dbuf<int> nqbtl_unfiltered;           // [nfiltered] tree target indices
     qbx::dbuf<int> user_to_tree_target;        // [ntargets]
-    // P2P work units over the traversal the geometry was built with
+    // global QBX centers grouped by target box: CSR over target box indices
+    qbx::dbuf<int> gc_by_tbox_starts, gc_by_tbox;
+    // work units over the traversal the geometry was built with: P2P (list 1)
+    // and M2P (list 3) over non-QBX targets, P2QBXL and M2QBXL over centers
     const qbx_trav *trav = nullptr;
-    qbx::unit_plan p2p_plan;
+    qbx::unit_plan p2p_plan, m2p_plan, p2q_plan, m2q_plan;
 };
 
 namespace qbx {
@@ -160,9 +167,12 @@ int compact_indices(qbx_ctx *ctx, hipStream_t st, const int *flags, int n, dbuf<
                     int *count);
 // stream-ordered zero fill of n doubles by a kernel (safe inside graph capture)
 int zero_fill(double *p, long ndoubles, hipStream_t st);
-// work units of list 1 (P2P) for the geometry's non-QBX targets
+// work units of a target-box list (starts/lists over the traversal's target
+// boxes) for the geometry's non-QBX targets, or its global QBX centers
+// (centers); work per entry is the entry box's source count (by_sources) or one
 int build_unit_plan(hipStream_t st, const qbx_tree *T, const qbx_trav *R, const qbx_geo *G,
-                    unit_plan &plan);
+                    const int *starts, const int *lists, bool by_sources, bool centers,
+                    int iter, unit_plan &plan);
 
 // 3D Laplace stages (fmm3d.hip), same contracts as the C-ABI entry points
 long f3_expansion_size(const qbx_kernel_desc *d, const qbx_tree *T);

@@ -189,6 +189,14 @@ __device__ __noinline__ void hankel01_libm(double x, cplx &h0, cplx &h1)
     if (WANT1) h1 = {j1(x), y1(x)};
 }
 
+// hankel01_fast out of line: for callers that evaluate it once per outer
+// iteration, so that neither the series coefficients nor the device library's
+// are held in registers across the caller's loops
+__device__ __noinline__ void hankel01_call(double x, cplx &h0, cplx &h1)
+{
+    hankel01_fast<true, true>(x, h0, h1);
+}
+
 // The series of hankel01_series with its coefficients read from tab
 // (A, B, C, D consecutively, HANKEL_NSER each; typically a copy in LDS, so
 // that a loop around it does not pin the coefficients in registers)
