@@ -685,11 +685,34 @@ __device__ void lap_l2l(const KD &kd, const double *B, const double *a, int p1, 
         if (q < kd.rc) acc[q] += rp * s[q];
 }
 
-// Helmholtz table translation: acc[q] += sum_n a[n] row[n - m] ((-1)^{n-m} if flip)
+// Helmholtz table translation: acc[q] += sum_n a[n] row[n - m] ((-1)^{n-m} if flip).
+// One right-hand side: four partial sums over n mod 4, so that four
+// coefficient/row load pairs are in flight and the multiply-add chains are
+// independent (the loop is latency-bound, not throughput-bound).
 template <int NRC>
 __device__ __forceinline__ void helm_row_dot(const KD &kd, const double *a, int p1, int m,
                                              const double *row, bool flip, cplx *acc)
 {
+    if (NRC == 1) {
+        const int r = kd.r0;
+        cplx s4[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+        int n = -p1;
+        for (; n + 3 <= p1; n += 4) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                cplx w = ldc(row + 2 * (n + u - m));
+                if (flip && ((n + u - m) & 1)) w = -w;
+                s4[u] += ldcn(kd, a, n + u, r) * w;
+            }
+        }
+        for (; n <= p1; ++n) {
+            cplx w = ldc(row + 2 * (n - m));
+            if (flip && ((n - m) & 1)) w = -w;
+            s4[0] += ldcn(kd, a, n, r) * w;
+        }
+        acc[0] += (s4[0] + s4[1]) + (s4[2] + s4[3]);
+        return;
+    }
     for (int n = -p1; n <= p1; ++n) {
         cplx w = ldc(row + 2 * (n - m));
         if (flip && ((n - m) & 1)) w = -w;
@@ -851,11 +874,33 @@ __device__ __forceinline__ void ht_stage_row(double *sr, const double *rg, int n
     for (int i = tid; i < 2 * nrow; i += nth) sr[i] = rg[i];
 }
 
-// acc[q] += sum_{n=-p1}^{p1} sa[n + p1][q] row[n - m]; row centred
+// acc[q] += sum_{n=-p1}^{p1} sa[n + p1][q] row[n - m]; row centred.  One
+// right-hand side: four independent partial sums (n mod 4) for latency.
 template <int NRC>
 __device__ __forceinline__ void ht_dot(int rc, const double *sa, int p1, int m,
                                        const double *row, bool flip, cplx *acc)
 {
+    if (NRC == 1) {
+        cplx s4[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+        int n = -p1;
+        for (; n + 3 <= p1; n += 4) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int d = n + u - m;
+                cplx w = {row[2 * d], row[2 * d + 1]};
+                if (flip && (d & 1)) w = -w;
+                s4[u] += cplx{sa[2 * (n + u + p1)], sa[2 * (n + u + p1) + 1]} * w;
+            }
+        }
+        for (; n <= p1; ++n) {
+            const int d = n - m;
+            cplx w = {row[2 * d], row[2 * d + 1]};
+            if (flip && (d & 1)) w = -w;
+            s4[0] += cplx{sa[2 * (n + p1)], sa[2 * (n + p1) + 1]} * w;
+        }
+        acc[0] += (s4[0] + s4[1]) + (s4[2] + s4[3]);
+        return;
+    }
     for (int n = -p1; n <= p1; ++n) {
         const int d = n - m;
         cplx w = {row[2 * d], row[2 * d + 1]};
@@ -867,30 +912,45 @@ __device__ __forceinline__ void ht_dot(int rc, const double *sa, int p1, int m,
     }
 }
 
-// M2M: one block per source parent box at level lvp, children one at a time
+// M2M: one block per source parent box at level lvp; its children are staged
+// NS at a time (all four when they fit), so one round of global loads serves
+// several children
 template <int NRC>
-__global__ void k_m2m_h(KD kd, TD t, int lvp, int i0, const int *spb, double *mpole)
+__global__ void k_m2m_h(KD kd, TD t, int lvp, int i0, int NS, const int *spb, double *mpole)
 {
     extern __shared__ double lds[];
     const int P = spb[i0 + blockIdx.x];
     const int lvc = lvp + 1, p1 = kd.order[lvc], p2 = kd.order[lvp], L = p1 + p2;
     const int na = 2 * p1 + 1, nrow = 2 * L + 1;
+    const int per = ht_entry_doubles(NRC, p1, L);
     const int tid = threadIdx.x, nth = blockDim.x;
-    double *sa = lds, *sr = lds + 2 * NRC * na;
     const bool act = tid < 2 * p2 + 1;
     const int m = tid - p2;
+    int kids[4], nk = 0;
+    for (int c = 0; c < 4; ++c) {
+        const int C = t.child[(size_t)c * t.nboxes + P];
+        if (C >= 0 && t.scumul[C] > 0) kids[nk++] = c;
+    }
     cplx acc[NRC];
 #pragma unroll
     for (int q = 0; q < NRC; ++q) acc[q] = {0, 0};
-    for (int c = 0; c < 4; ++c) {
-        const int C = t.child[(size_t)c * t.nboxes + P];
-        if (C < 0 || t.scumul[C] == 0) continue;
+    for (int k0 = 0; k0 < nk; k0 += NS) {
+        const int kn = min(NS, nk - k0);
         __syncthreads();
-        ht_stage_coeffs<NRC>(kd, sa, mpole + ecoef_off(kd, C, lvc) - 2L * p1 * kd.nr, na, tid,
-                             nth);
-        ht_stage_row(sr, kd.tab_mm + 2 * (kd.mm_off[lvc] + (long)c * nrow), nrow, tid, nth);
+        for (int k = 0; k < kn; ++k) {
+            const int c = kids[k0 + k];
+            const int C = t.child[(size_t)c * t.nboxes + P];
+            double *sa = lds + (long)k * per, *sr = sa + 2 * NRC * na;
+            ht_stage_coeffs<NRC>(kd, sa, mpole + ecoef_off(kd, C, lvc) - 2L * p1 * kd.nr, na,
+                                 tid, nth);
+            ht_stage_row(sr, kd.tab_mm + 2 * (kd.mm_off[lvc] + (long)c * nrow), nrow, tid, nth);
+        }
         __syncthreads();
-        if (act) ht_dot<NRC>(kd.rc, sa, p1, m, sr + 2 * L, false, acc);
+        if (act)
+            for (int k = 0; k < kn; ++k) {
+                const double *sa = lds + (long)k * per;
+                ht_dot<NRC>(kd.rc, sa, p1, m, sa + 2 * NRC * na + 2 * L, false, acc);
+            }
     }
     if (act) store_add<NRC>(kd, mpole + ecoef_off(kd, P, lvp) + 2L * m * kd.nr, acc);
 }
@@ -1064,7 +1124,7 @@ __global__ void __launch_bounds__(64) k_p2p(KD kd, TD t, const int *ubox, const 
     constexpr int CH = P2PTile<NRC>::CH;
     __shared__ double sx[CH], sy[CH], snx[CH], sny[CH];
     __shared__ double sq[CH * 2 * NRC];
-    __shared__ double htab[4 * HANKEL_NSER];
+    __shared__ __align__(16) double htab[4 * HANKEL_NSER];
     const int u = blockIdx.x;
     const int it = ubox[u];
     const int B = tboxes[it];
@@ -1358,7 +1418,7 @@ __global__ void __launch_bounds__(64) k_p2qbxl(KD kd, TD t, const int *ubox, con
     constexpr int CH = P2QTile<NRC>::CH;
     __shared__ double sx[CH], sy[CH], snx[CH], sny[CH];
     __shared__ double sq[CH * 2 * NRC];
-    __shared__ double htab[4 * HANKEL_NSER];
+    __shared__ __align__(16) double htab[4 * HANKEL_NSER];
     const int u = blockIdx.x;
     const int it = ubox[u];
     const int G = box_g[it];
@@ -1995,8 +2055,11 @@ int qbx_coarsen_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const 
         HtLaunch hl;
         if (ht_launch(kd, kd.order[l], kd.order[l + 1], kd.order[l] + kd.order[l + 1], &hl)) {
 #define L_M2MH(N)                                                                          \
-    hipLaunchKernelGGL(k_m2m_h<N>, dim3(i1 - i0), dim3(hl.threads), ht_bytes<N>(hl), st, kd, \
-                       td, l, i0, R->source_parent_boxes.p, mpole)
+    do {                                                                                   \
+        const int ns_ = (int)std::max(1L, std::min(4L, 65536L / (long)ht_bytes<N>(hl)));   \
+        hipLaunchKernelGGL(k_m2m_h<N>, dim3(i1 - i0), dim3(hl.threads), ns_ * ht_bytes<N>(hl), \
+                           st, kd, td, l, i0, ns_, R->source_parent_boxes.p, mpole);        \
+    } while (0)
             FOR_RHS_CHUNKS(kd, NRC_S, L_M2MH);
 #undef L_M2MH
             continue;

@@ -197,9 +197,10 @@ __device__ __noinline__ void hankel01_call(double x, cplx &h0, cplx &h1)
     hankel01_fast<true, true>(x, h0, h1);
 }
 
-// The series of hankel01_series with its coefficients read from tab
-// (A, B, C, D consecutively, HANKEL_NSER each; typically a copy in LDS, so
-// that a loop around it does not pin the coefficients in registers)
+// The series of hankel01_series with its coefficients read from tab, laid
+// out [k][A_k, C_k, B_k, D_k] so that each step reads its pair with one
+// 16-byte load (typically a copy in LDS, so that a loop around it does not
+// pin the coefficients in registers)
 constexpr int HANKEL_NSER = 18;
 
 template <int K, bool WANT0, bool WANT1>
@@ -211,12 +212,14 @@ __device__ __forceinline__ void hankel01_series_tab(double x, const double *tab,
 #pragma unroll
     for (int k = K - 1; k >= 0; --k) {
         if (WANT0) {
-            a = a * t + tab[k];
-            c = c * t + tab[2 * HANKEL_NSER + k];
+            const double2 ac = *reinterpret_cast<const double2 *>(tab + 4 * k);
+            a = a * t + ac.x;
+            c = c * t + ac.y;
         }
         if (WANT1) {
-            b = b * t + tab[HANKEL_NSER + k];
-            d = d * t + tab[3 * HANKEL_NSER + k];
+            const double2 bd = *reinterpret_cast<const double2 *>(tab + 4 * k + 2);
+            b = b * t + bd.x;
+            d = d * t + bd.y;
         }
     }
     const double lg = log(0.5 * x);
@@ -229,12 +232,13 @@ __device__ __forceinline__ void hankel01_series_tab(double x, const double *tab,
     }
 }
 
-// fill tab (4 * HANKEL_NSER doubles) cooperatively by the lanes of a block
+// fill tab (4 * HANKEL_NSER doubles, 16-byte aligned) cooperatively by the
+// lanes of a block
 __device__ __forceinline__ void hankel_table_fill(double *tab, int lane, int nlanes)
 {
     for (int i = lane; i < 4 * HANKEL_NSER; i += nlanes) {
-        const int w = i / HANKEL_NSER, k = i % HANKEL_NSER;
-        tab[i] = w == 0 ? c_ser_A[k] : w == 1 ? c_ser_B[k] : w == 2 ? c_ser_C[k] : c_ser_D[k];
+        const int k = i / 4, w = i % 4;
+        tab[i] = w == 0 ? c_ser_A[k] : w == 1 ? c_ser_C[k] : w == 2 ? c_ser_B[k] : c_ser_D[k];
     }
 }
 
