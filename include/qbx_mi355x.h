@@ -272,6 +272,31 @@ int qbx_eval_target_specific_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tr
                                         const double *dipvec_tree, double *pot,
                                         double *grad);
 
+/* ---- differentiable 3D Helmholtz evaluation at smooth targets --------------
+ * Targets far from the surface need no QBX: pytential evaluates them through
+ * the point-FMM part of drive_fmm (pytential/qbx/fmm.py:216-225, 436-565) with
+ * sumpy's HelmholtzKernel(3), G_k(t, s) = e^{ik|t-s|} / (4 pi |t-s|).  These
+ * entry points are the exact sums, with the k-derivative and the adjoint that
+ * a torch.autograd.Function backward needs (the reference computes gradients
+ * only by differencing or through its dense matrix builder,
+ * pytential/symbolic/matrix.py:376-522).
+ *   pot[t]     = sum_s G_k(t, s) q_s
+ *   dpot_dk[t] = sum_s i e^{ik|t-s|} / (4 pi) q_s        (NULL: skipped)
+ *   v[s]       = sum_t conj(G_k(t, s)) u_t               (adjoint)
+ * sources/targets SoA double [3][n]; q, pot, dpot_dk, u, v complex; real k.
+ * Coincident points contribute zero.  The forward uses the context's scratch
+ * for per-chunk partial sums (added in a fixed order). */
+int qbx_helmholtz3d_p2p(qbx_ctx *ctx, void *stream, int nsources, const double *sources,
+                        const double *strengths, int ntargets, const double *targets, double k,
+                        double *pot, double *dpot_dk);
+int qbx_helmholtz3d_p2p_adjoint(qbx_ctx *ctx, void *stream, int nsources, const double *sources,
+                                int ntargets, const double *targets, double k, const double *u,
+                                double *v);
+/* The transpose of qbx_upsample: out[e*n1 + j][r] = sum_i mat[i*n1 + j] *
+ * in[e*nf + i][r] (the adjoint of the stage-1 -> quad-stage-2 interpolation). */
+int qbx_upsample_adjoint(qbx_ctx *ctx, void *stream, int nelements, int n1, int nf, int nrhs,
+                         const double *mat, const double *in, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -487,3 +487,36 @@ def tsqbx(kernel, op, k, order, sources, targets, centers, qbx_centers, c2tb,
                          _ptr(q), _ptr(dv), _ptr(pot), _ptr(grad))
     assert rc == 0
     return grad if op == "Sp" else pot
+
+
+# ---- 3D Helmholtz point sums at smooth targets (the differentiable path)
+
+def helmholtz3d_dense(sources, targets, k):
+    """G_k[t, s] = e^{ik|t-s|} / (4 pi |t-s|) for SoA (3, n) point sets: the
+    matrix of sumpy's HelmholtzKernel(3) between quad-stage-2 nodes and
+    smooth targets (pytential/symbolic/matrix.py:376-522 builds its entries
+    from the same kernel).  Coincident points give zero."""
+    s = np.asarray(sources, np.float64)
+    t = np.asarray(targets, np.float64)
+    d = t[:, :, None] - s[:, None, :]
+    r = np.sqrt(np.sum(d * d, axis=0))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        g = np.where(r > 0, np.exp(1j * k * r) / (4 * np.pi * np.where(r > 0, r, 1.0)), 0.0)
+    return g
+
+
+def helmholtz3d_far_operator(discr, targets, k):
+    """A = G_k W U (nodal density -> potential at smooth targets), dense."""
+    G = helmholtz3d_dense(discr.quad_nodes, targets, k)
+    U = np.kron(np.eye(discr.nelements), discr.upsample)
+    return (G * discr.quad_weights[None, :]) @ U
+
+
+def sphere_single_layer_exterior(l, k, a, r):
+    """Exterior value of the Helmholtz single layer of Y_lm on the sphere of
+    radius a at distance r > a: S_k Y_lm = i k a^2 j_l(k a) h_l(k r) Y_lm,
+    from the addition theorem e^{ik|x-y|}/(4 pi |x-y|) = i k sum_l j_l(k a)
+    h_l(k r) sum_m Y_lm(x^) conj(Y_lm(y^)) (|y| = a < |x| = r)."""
+    from scipy.special import spherical_jn, spherical_yn
+    h = spherical_jn(l, k * r) + 1j * spherical_yn(l, k * r)
+    return 1j * k * a * a * spherical_jn(l, k * a) * h

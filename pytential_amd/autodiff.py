@@ -1,0 +1,104 @@
+"""Differentiable single-layer potential of a surface density at smooth
+targets, 3D Helmholtz: a ``torch.autograd.Function`` whose backward applies
+hand-written adjoint kernels.
+
+The forward is what pytential computes for ``bind(places, S_k(sigma))``
+evaluated at targets far from the surface (``qbx_forced_limit=None``, the
+targets need no QBX center, so drive_fmm's point-FMM part evaluates them:
+pytential/qbx/fmm.py:216-225, pytential/qbx/__init__.py:694-793):
+
+    phi = G_k W U sigma,
+    U  the stage-1 -> quad-stage-2 interpolation (qbx_upsample),
+    W  the quad-stage-2 weights (area element times rule weight,
+       pytential/qbx/__init__.py:1075-1096),
+    G_k[t, s] = e^{ik|t-s|} / (4 pi |t-s|)  (sumpy HelmholtzKernel(3)).
+
+Backward (torch's convention for complex tensors: for a real loss L the
+incoming gradient is g = dL/d(conj phi) times 2):
+
+    d/dsigma = U^T W G_k^H g                  (qbx_helmholtz3d_p2p_adjoint,
+                                               qbx_upsample_adjoint)
+    d/dk     = Re sum_t conj(g_t) dphi_t/dk,  dphi/dk = sum_s i e^{ik r}/(4 pi) q_s
+                                              (a by-product of the forward).
+
+The sums over sources are exact (no FMM truncation); the reference offers
+derivatives only by differencing or through its dense matrix builder
+(pytential/symbolic/matrix.py:376-522), whose transpose is what the adjoint
+applies.
+"""
+from __future__ import annotations
+
+import torch
+
+from ._lib import call
+from .geometry import _p, _stream, as_dev
+
+
+def _check_far(src, targets):
+    """Targets must be well separated from the surface: outside the sources'
+    bounding box grown by the largest element extent."""
+    q = src.dev["quad_nodes"]
+    lo, hi = q.min(dim=1).values, q.max(dim=1).values
+    h = float((hi - lo).max()) / max(src.density_discr.nelements, 1) ** 0.5
+    inside = ((targets >= (lo - h)[:, None]) & (targets <= (hi + h)[:, None])).all(dim=0)
+    if bool(inside.any()):
+        raise ValueError("targets near the surface need QBX: use exec_compute_potential")
+
+
+def helmholtz3d_point_sum(src, strengths, targets, k, want_dk=False):
+    """phi[t] = sum_s G_k(t, s) strengths[s] over the quad-stage-2 nodes
+    (and dphi/dk when want_dk)."""
+    nt = targets.shape[1]
+    pot = torch.empty(nt, dtype=torch.complex128, device="cuda")
+    dk = torch.empty_like(pot) if want_dk else None
+    call("qbx_helmholtz3d_p2p", src.ctx.h, _stream(), src.density_discr.nsources,
+         _p(src.dev["quad_nodes"]), _p(strengths), nt, _p(targets), float(k), _p(pot), _p(dk))
+    return pot, dk
+
+
+class HelmholtzSingleLayerFar(torch.autograd.Function):
+    """phi = S_k sigma at smooth targets; sigma complex [ndofs] (stage-1
+    nodal values), k a real 0-dim tensor, targets (3, nt) on the device."""
+
+    @staticmethod
+    def forward(ctx, sigma, k, src, targets):
+        q = src.quad_density(sigma) * src.dev["quad_weights"]
+        pot, dk = helmholtz3d_point_sum(src, q, targets, float(k),
+                                        want_dk=ctx.needs_input_grad[1])
+        ctx.src, ctx.targets, ctx.k = src, targets, float(k)
+        ctx.k_device = k.device
+        ctx.dk = dk
+        return pot
+
+    @staticmethod
+    def backward(ctx, g):
+        src, targets, k = ctx.src, ctx.targets, ctx.k
+        d = src.density_discr
+        g = g.to(torch.complex128).contiguous()
+        grad_sigma = grad_k = None
+        if ctx.needs_input_grad[0]:
+            v = torch.empty(d.nsources, dtype=torch.complex128, device="cuda")
+            call("qbx_helmholtz3d_p2p_adjoint", src.ctx.h, _stream(), d.nsources,
+                 _p(src.dev["quad_nodes"]), targets.shape[1], _p(targets), k, _p(g), _p(v))
+            v = (v * src.dev["quad_weights"]).contiguous()
+            grad_sigma = torch.empty(d.ndofs, dtype=torch.complex128, device="cuda")
+            call("qbx_upsample_adjoint", src.ctx.h, _stream(), d.nelements,
+                 d.nodes_per_element, d.fine_nodes_per_element, 1, _p(src.dev["upsample"]),
+                 _p(v), _p(grad_sigma))
+        if ctx.needs_input_grad[1]:
+            grad_k = torch.real(torch.sum(torch.conj(g) * ctx.dk)).to(
+                device=ctx.k_device, dtype=torch.float64)
+        return grad_sigma, grad_k, None, None
+
+
+def single_layer_far(src, sigma, k, targets):
+    """S_k sigma at smooth targets (3, nt), differentiable in sigma and k."""
+    if src.ambient_dim != 3:
+        raise ValueError("single_layer_far needs a 3D surface source")
+    tg = as_dev(targets)
+    _check_far(src, tg)
+    if not isinstance(k, torch.Tensor):
+        k = torch.tensor(float(k), dtype=torch.float64)
+    sig = sigma if isinstance(sigma, torch.Tensor) else torch.as_tensor(sigma)
+    sig = sig.to(device="cuda", dtype=torch.complex128)
+    return HelmholtzSingleLayerFar.apply(sig, k, src, tg)
