@@ -613,6 +613,117 @@ def main_ellipsoid3d(args):
     print(json.dumps(line), flush=True)
 
 
+def main_dirichlet64(args):
+    """Config 4: ncurves independent starfish curves (arms 3..9, radius
+    0.5..2, amplitude 0.25, element counts log-uniform in 256..4096, 8
+    Gauss-Legendre nodes per element), the interior Laplace Dirichlet problem
+    (0.5 u - D u with D at "avg") on each, boundary data from a point charge
+    outside each curve, all solved by one lockstep GMRES (gmres_many) to 1e-10
+    with every matvec a HIP-graph replay; the solutions' fields are checked
+    against the point charges inside each curve.  A step is the whole batch of
+    solves; points/s counts boundary nodes times matvecs."""
+    import torch
+
+    from pytential_amd.discretization import discretize_curve, starfish
+    from pytential_amd.linalg import MatVecGraph, gmres_many
+    from pytential_amd.pde import DirichletOperator
+    from pytential_amd.qbx import QBXLayerPotentialSource
+    rng = np.random.default_rng(1003)
+    t0 = time.perf_counter()
+    curves = []
+    for _ in range(args.ncurves):
+        arms = int(rng.integers(3, 10))
+        rad = float(rng.uniform(0.5, 2.0))
+        nel = int(round(np.exp(rng.uniform(np.log(256), np.log(4096)))))
+        phase = float(rng.uniform(0, 2 * np.pi))
+        ang = rng.uniform(0, 2 * np.pi)
+        x0 = 1.6 * rad * np.array([np.cos(ang), np.sin(ang)])
+        d = discretize_curve(starfish(arms, 0.25, rad, phase=phase), nel, 8, 16)
+        src = QBXLayerPotentialSource(d, fine_order=16, qbx_order=5, fmm_order=15)
+        op = DirichletOperator(src, loc_sign=-1)
+        f = -np.log(np.hypot(d.nodes[0] - x0[0], d.nodes[1] - x0[1])) / (2 * np.pi)
+        curves.append({"d": d, "src": src, "op": op, "x0": x0, "rad": rad, "nel": nel,
+                       "f": torch.as_tensor(f, device="cuda")})
+    ndofs = [c["d"].ndofs for c in curves]
+    log(f"[dirichlet64] {len(curves)} curves, {sum(c['nel'] for c in curves)} elements, "
+        f"{sum(ndofs)} nodes ({time.perf_counter() - t0:.1f}s)")
+    # eager pass over every geometry first (geometry, workspaces at their
+    # largest), then one captured matvec per geometry
+    t1 = time.perf_counter()
+    for c in curves:
+        c["op"].matvec(c["f"])
+    torch.cuda.synchronize()
+    for c in curves:
+        c["mv"] = MatVecGraph(c["op"].matvec, c["f"])
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t1
+    counts = [0] * len(curves)
+
+    def counted(i, mv):
+        def f(x):
+            counts[i] += 1
+            return mv(x)
+        return f
+
+    ops = [counted(i, c["mv"]) for i, c in enumerate(curves)]
+    rhss = [c["f"] for c in curves]
+    for _ in range(args.warmup):
+        gmres_many(ops, rhss, tol=1e-10, restart=30, maxiter=200)
+    torch.cuda.synchronize()
+    counts[:] = [0] * len(curves)
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        res = gmres_many(ops, rhss, tol=1e-10, restart=30, maxiter=200)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t2) / args.steps
+    nmv = sum(counts) / args.steps
+    node_mv = sum(n * c for n, c in zip(ndofs, counts)) / args.steps
+    # one captured matvec per geometry, timed alone (the per-matvec figure)
+    t3 = time.perf_counter()
+    for c in curves:
+        c["mv"](c["f"])
+    torch.cuda.synchronize()
+    t_mv_all = time.perf_counter() - t3
+    errs = []
+    for c, r in zip(curves, res):
+        th = rng.uniform(0, 2 * np.pi, 64)
+        rr = 0.5 * 0.75 * c["rad"] * np.sqrt(rng.uniform(0, 1, 64))
+        pts = np.stack([rr * np.cos(th), rr * np.sin(th)])
+        u = c["op"].representation(r.solution, pts).real.cpu().numpy()
+        ex = -np.log(np.hypot(pts[0] - c["x0"][0], pts[1] - c["x0"][1])) / (2 * np.pi)
+        errs.append(float(np.max(np.abs(u - ex)) / np.max(np.abs(ex))))
+    its = [r.iteration_count for r in res]
+    states = sorted(set(r.state for r in res))
+    log(f"[dirichlet64] {dt * 1e3:.1f} ms for all solves, {nmv:.0f} matvecs, iterations "
+        f"{min(its)}..{max(its)}, states {states}, interior error max {max(errs):.2e}")
+    line = {
+        "metric": METRIC,
+        "value": round(node_mv / dt, 1),
+        "unit": "points*matvecs/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": (f"{len(curves)} starfish curves (arms 3..9, radius 0.5..2, "
+                                f"amplitude 0.25, 256..4096 elements log-uniform, 8 nodes "
+                                f"each, fine 16, qbx_order 5, fmm_order 15): interior Laplace "
+                                f"Dirichlet, 0.5u - D u, GMRES(30) to 1e-10 in lockstep, "
+                                f"HIP-graph matvecs"),
+                   "ncurves": len(curves), "nodes": int(sum(ndofs)),
+                   "elements": int(sum(c["nel"] for c in curves)), "seed": 1003},
+        "iterations": {"min": int(min(its)), "max": int(max(its)),
+                       "mean": round(float(np.mean(its)), 2)},
+        "states": states,
+        "matvecs_per_step": nmv,
+        "all_geometries_one_matvec_ms": round(t_mv_all * 1e3, 3),
+        "interior_rel_err_max": max(errs),
+        "setup_s": round(t_setup, 3),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -626,8 +737,9 @@ def main():
     ap.add_argument("--sample", type=int, default=1000)
     ap.add_argument("--nrhs", type=int, default=8,
                     help="densities per evaluation for the batched figure")
-    ap.add_argument("--workload", choices=["helmholtz2d", "sphere3d", "ellipsoid3d"],
-                    default="helmholtz2d")
+    ap.add_argument("--workload", choices=["helmholtz2d", "sphere3d", "ellipsoid3d",
+                                           "dirichlet64"], default="helmholtz2d")
+    ap.add_argument("--ncurves", type=int, default=64, help="dirichlet64: curves")
     ap.add_argument("--nsub-ell", type=int, default=5,
                     help="ellipsoid3d: icosahedron refinements (5: 20480 triangles)")
     ap.add_argument("--nsub", type=int, default=5, help="sphere3d: icosahedron refinements")
@@ -661,6 +773,9 @@ def main():
         return
     if args.workload == "ellipsoid3d":
         main_ellipsoid3d(args)
+        return
+    if args.workload == "dirichlet64":
+        main_dirichlet64(args)
         return
     if args.workload == "sphere3d":
         main_sphere3d(args)

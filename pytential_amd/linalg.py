@@ -185,3 +185,153 @@ class MatVecGraph:
         return self._out.clone()
 
     __call__ = matvec
+
+
+def gmres_many(ops, rhss, restart=None, tol=None, maxiter=None, hard_failure=False,
+               no_progress_factor=None, stall_iterations=None,
+               require_monotonicity=True) -> list[GMRESResult]:
+    """Solve independent systems ``ops[i](x) = rhss[i]`` in lockstep: every
+    iteration advances each unfinished system by one Arnoldi step (its
+    operator application, classical Gram-Schmidt applied twice, all on the
+    device) and then transfers every system's new Hessenberg column to the
+    host in one synchronisation, so the host round trip is paid once per
+    iteration for the whole batch instead of once per system.  Restart,
+    tolerance, stall and monotonicity rules and the outcome states are those
+    of ``gmres`` (the reference's, gmres.py:352-413), applied per system;
+    ``hard_failure`` raises on the first failed system."""
+    m = len(ops)
+    if len(rhss) != m:
+        raise ValueError("need one right-hand side per operator")
+    tol = 1e-5 if tol is None else tol
+    stall_iterations = 10 if stall_iterations is None else stall_iterations
+    no_progress_factor = 1.25 if no_progress_factor is None else no_progress_factor
+    applies = [op.matvec if hasattr(op, "matvec") else op for op in ops]
+
+    class _S:
+        pass
+
+    sys_ = []
+    for i in range(m):
+        s = _S()
+        s.b = rhss[i]
+        s.n = s.b.numel()
+        s.restart = min(s.n, 20) if restart is None else restart
+        s.maxiter = 2 * s.n if maxiter is None else maxiter
+        s.x = torch.zeros_like(s.b)
+        s.res, s.it, s.state, s.done = [], 0, None, False
+        sys_.append(s)
+    nb = torch.stack([torch.linalg.vector_norm(s.b) for s in sys_]).cpu().numpy()
+    for s, v in zip(sys_, nb):
+        s.norm_b = float(v)
+        if s.norm_b == 0:
+            s.done, s.state = True, "success"
+            s.res = [0.0]
+
+    def start_cycle(active):
+        rs = [s.b - applies[i](s.x) if s.it > 0 else s.b.clone() for i, s in active]
+        betas = torch.stack([torch.linalg.vector_norm(r) for r in rs]).cpu().numpy()
+        out = []
+        for (i, s), r, beta in zip(active, rs, betas):
+            beta = float(beta)
+            if s.res:
+                s.res[-1] = beta / s.norm_b
+            else:
+                s.res.append(beta / s.norm_b)
+            if beta / s.norm_b < tol:
+                s.done, s.state = True, "success"
+                continue
+            s.V = torch.empty((s.restart + 1,) + tuple(s.b.shape), dtype=s.b.dtype,
+                              device=s.b.device)
+            s.V[0] = r / beta
+            s.H = np.zeros((s.restart + 1, s.restart), dtype=np.complex128)
+            s.cs = np.zeros(s.restart, dtype=np.complex128)
+            s.sn = np.zeros(s.restart, dtype=np.complex128)
+            s.g = np.zeros(s.restart + 1, dtype=np.complex128)
+            s.g[0] = beta
+            s.j = 0
+            out.append((i, s))
+        return out
+
+    def finish_cycle(s, i):
+        jd = s.j
+        y = np.zeros(jd, dtype=np.complex128)
+        for a in range(jd - 1, -1, -1):
+            y[a] = (s.g[a] - s.H[a, a + 1:jd] @ y[a + 1:]) / s.H[a, a]
+        yt = torch.as_tensor(y, dtype=torch.complex128, device=s.b.device)
+        coef = yt.to(s.b.dtype) if s.b.is_complex() else yt.real.to(s.b.dtype)
+        s.x = s.x + torch.tensordot(coef, s.V[:jd], dims=1)
+
+    active = start_cycle([(i, s) for i, s in enumerate(sys_) if not s.done])
+    while active:
+        # one Arnoldi step per active system, device only
+        cols = []
+        for i, s in active:
+            w = applies[i](s.V[s.j])
+            Vj = s.V[:s.j + 1].reshape(s.j + 1, -1)
+            wf = w.reshape(-1)
+            h = torch.conj(Vj) @ wf
+            wf = wf - h @ Vj
+            h2 = torch.conj(Vj) @ wf
+            wf = wf - h2 @ Vj
+            hn = torch.linalg.vector_norm(wf)
+            s.w = wf.reshape(s.b.shape)
+            cols.append(torch.cat([(h + h2).to(torch.complex128),
+                                   hn.to(torch.complex128).reshape(1)]))
+        host = torch.cat(cols).cpu().numpy()          # the one synchronisation
+        off = 0
+        next_active, cycle_over = [], []
+        for (i, s), c in zip(active, cols):
+            j = s.j
+            hcol = host[off:off + j + 2]
+            off += j + 2
+            hn = float(hcol[-1].real)
+            s.H[:j + 1, j] = hcol[:j + 1]
+            s.H[j + 1, j] = hn
+            if hn > 0:
+                s.V[j + 1] = s.w / hn
+            for a in range(j):
+                x0, x1 = s.H[a, j], s.H[a + 1, j]
+                s.H[a, j] = np.conj(s.cs[a]) * x0 + np.conj(s.sn[a]) * x1
+                s.H[a + 1, j] = -s.sn[a] * x0 + s.cs[a] * x1
+            x0, x1 = s.H[j, j], s.H[j + 1, j]
+            rho = np.hypot(abs(x0), abs(x1))
+            s.cs[j], s.sn[j] = (1.0, 0.0) if rho == 0 else (x0 / rho, x1 / rho)
+            s.H[j, j] = np.conj(s.cs[j]) * x0 + np.conj(s.sn[j]) * x1
+            s.H[j + 1, j] = 0.0
+            s.g[j + 1] = -s.sn[j] * s.g[j]
+            s.g[j] = np.conj(s.cs[j]) * s.g[j]
+            s.it += 1
+            s.j = j + 1
+            s.res.append(float(abs(s.g[j + 1]) / s.norm_b))
+            r = s.res
+            stop = (r[-1] < tol or hn == 0 or s.it >= s.maxiter or s.j == s.restart
+                    or (require_monotonicity and len(r) >= 2 and r[-1] > r[-2] * (1 + 1e-8))
+                    or (stall_iterations and len(r) > stall_iterations and
+                        r[-stall_iterations] < no_progress_factor * r[-1]))
+            (cycle_over if stop else next_active).append((i, s))
+        restart_list = []
+        for i, s in cycle_over:
+            finish_cycle(s, i)
+            r = s.res
+            if r[-1] < tol:
+                restart_list.append((i, s))       # confirm with the true residual
+                continue
+            if require_monotonicity and len(r) >= 2 and r[-1] > r[-2] * (1 + 1e-8):
+                s.done, s.state = True, "non-monotonic residuals"
+            elif stall_iterations and len(r) > stall_iterations and \
+                    r[-stall_iterations] < no_progress_factor * r[-1]:
+                s.done, s.state = True, "stalled"
+            elif s.it >= s.maxiter:
+                s.done, s.state = True, "max iterations"
+            else:
+                restart_list.append((i, s))
+        if restart_list:
+            next_active += start_cycle(restart_list)
+        active = next_active
+    results = []
+    for s in sys_:
+        ok = s.state == "success"
+        if not ok and hard_failure:
+            raise GMRESError(s.state)
+        results.append(GMRESResult(s.x, s.res, s.it, ok, s.state))
+    return results

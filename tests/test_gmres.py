@@ -57,3 +57,46 @@ def test_gmres_failure_states():
     res = gmres(lambda x: Pt @ x, torch.as_tensor(np.eye(n)[0].astype(complex)),
                 restart=5, tol=1e-10, hard_failure=False)
     assert not res.success and res.state in ("stalled", "max iterations")
+
+
+def test_gmres_many_matches_single_solves():
+    """Lockstep batch: each system converges like its own gmres call (same
+    iteration count) to the numpy solution; a zero right-hand side and a
+    complex system ride along."""
+    from pytential_amd.linalg import gmres, gmres_many
+    rng = np.random.default_rng(31)
+    ops, rhss, mats = [], [], []
+    for n, cplx in ((40, False), (17, True), (63, False), (5, False)):
+        A = 3 * np.eye(n) + rng.normal(size=(n, n)) / np.sqrt(n)
+        if cplx:
+            A = A + 0.5j * rng.normal(size=(n, n)) / np.sqrt(n)
+        At = torch.as_tensor(A)
+        mats.append(A)
+        ops.append(lambda x, At=At: At @ x)
+        b = rng.normal(size=n) + (1j * rng.normal(size=n) if cplx else 0)
+        rhss.append(torch.as_tensor(b))
+    rhss[3] = torch.zeros(5, dtype=torch.float64)
+    res = gmres_many(ops, rhss, tol=1e-11, restart=7, maxiter=300)
+    for A, b, op, r in zip(mats, rhss, ops, res):
+        assert r.success and r.state == "success"
+        np.testing.assert_allclose(r.solution.numpy(), np.linalg.solve(A, b.numpy()),
+                                   rtol=1e-9, atol=1e-10)
+        one = gmres(op, b, tol=1e-11, restart=7, maxiter=300)
+        assert one.iteration_count == r.iteration_count
+
+
+def test_gmres_many_failure_states():
+    from pytential_amd.linalg import GMRESError, gmres_many
+    rng = np.random.default_rng(32)
+    n = 30
+    good = 3 * np.eye(n) + 0.2 * rng.normal(size=(n, n))
+    # a rotation-like operator on which restarted GMRES(1) makes no progress
+    bad = np.roll(np.eye(n), 1, axis=0)
+    ops = [lambda x, A=torch.as_tensor(good): A @ x, lambda x, A=torch.as_tensor(bad): A @ x]
+    b = [torch.as_tensor(rng.normal(size=n)) for _ in range(2)]
+    res = gmres_many(ops, b, tol=1e-10, restart=1, maxiter=60)
+    assert res[0].success
+    assert not res[1].success and res[1].state in ("stalled", "max iterations",
+                                                   "non-monotonic residuals")
+    with pytest.raises(GMRESError):
+        gmres_many(ops, b, tol=1e-10, restart=1, maxiter=60, hard_failure=True)
