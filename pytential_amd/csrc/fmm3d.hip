@@ -364,6 +364,166 @@ __global__ void k3_p2l(KD kd, TD t, Flat fl, const int *ttp, const int *starts, 
     store_add<NRC>(kd, local + ecoef_off(kd, b, lv) + 2L * j * kd.nr, acc);
 }
 
+// P2M / P2L through LDS, one block per box, sources S at a time.  Phase A
+// (one lane per source) computes the source's harmonic table once, m >= 0
+// half: Y_n^m(s - c) for n <= p (multipoles) or T_n^m(s - c) for n <= p + 1
+// (locals, the dipole needs n + 1), at ht[s][n(n+1)/2 + m]; phase B (one lane
+// per coefficient (n, m)) sums q_s b_nm(s) over the chunk, with the negative
+// orders from (-1)^m conj and the dipole rules of p2e_basis.  O(p^2) work per
+// source instead of O(p^3) from one harmonic per thread and source.
+struct P2EShape3 {
+    int S;  // sources per chunk
+    int R;  // row stride (complex entries, odd) >= (pt+1)(pt+2)/2
+    __host__ __device__ int doubles(int nrc) const { return 2 * S * R + 2 * S * nrc + 3 * S + S; }
+};
+
+__device__ __forceinline__ int tri(int n, int m) { return n * (n + 1) / 2 + m; }
+
+template <int NRC, bool LOCAL>
+__global__ void __launch_bounds__(512) k3_p2e(KD kd, TD t, int lv, int i0, P2EShape3 sh,
+                                               const int *boxes, const int *starts,
+                                               const int *lists, const double *src, int ns,
+                                               const double *str, const double *nrm,
+                                               double *expn)
+{
+    extern __shared__ double lds[];
+    const int i = i0 + blockIdx.x;
+    const int Bx = boxes[i];
+    int e0 = 0, e1 = 1;
+    if (LOCAL) {
+        e0 = starts[i];
+        e1 = starts[i + 1];
+        if (e0 == e1) return;
+    }
+    const int p = kd.order[lv];
+    const int pt = LOCAL ? p + 1 : p;
+    const int S = sh.S, R = sh.R;
+    double *Ht = lds, *Qs = Ht + 2 * S * R, *Nu = Qs + 2 * S * NRC;
+    int *idx = (int *)(Nu + 3 * S);
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const bool dip = kd.dip;
+    const int rc = kd.rc, r0 = kd.r0, nr = kd.nr;
+    const v3 c = boxc(t, Bx);
+    const bool act = tid < (p + 1) * (p + 1);
+    int n = 0, m = 0;
+    if (act) nm_of(tid, n, m);
+    cplx acc[NRC];
+#pragma unroll
+    for (int q = 0; q < NRC; ++q) acc[q] = {0, 0};
+
+    // table entry (nn, mm) of staged source sl, any sign of mm, zero outside
+    auto hval = [&](int sl, int nn, int mm) -> cplx {
+        const int am = mm < 0 ? -mm : mm;
+        if (nn < 0 || am > nn) return {0, 0};
+        const cplx v = {Ht[2 * (sl * R + tri(nn, am))], Ht[2 * (sl * R + tri(nn, am)) + 1]};
+        return mm < 0 ? neg_m(v, am) : v;
+    };
+
+    auto flush = [&](int cnt) {
+        __syncthreads();
+        if (tid < cnt) {
+            const int si = idx[tid];
+            const v3 d = sub(ld3(src, ns, si), c);
+            double *h = Ht + 2 * tid * R;
+            const double r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+            if (LOCAL) {
+                const double ir2 = 1.0 / r2;
+                const cplx w = {d.x * ir2, d.y * ir2};
+                cplx tmm = {sqrt(ir2), 0};
+                for (int mm = 0; mm <= pt; ++mm) {
+                    if (mm > 0) tmm = (-(2.0 * mm - 1.0)) * (tmm * w);
+                    cplx a = {0, 0}, b = tmm;
+                    stc(h + 2 * tri(mm, mm), b);
+                    for (int nn = mm + 1; nn <= pt; ++nn) {
+                        const cplx cc = ir2 * (((2.0 * nn - 1.0) * d.z) * b -
+                                               ((double)(nn + mm - 1) * (nn - mm - 1)) * a);
+                        stc(h + 2 * tri(nn, mm), cc);
+                        a = b;
+                        b = cc;
+                    }
+                }
+            } else {
+                const cplx hw = {-0.5 * d.x, -0.5 * d.y};
+                cplx ymm = {1, 0};
+                for (int mm = 0; mm <= pt; ++mm) {
+                    if (mm > 0) ymm = (1.0 / mm) * (ymm * hw);
+                    cplx a = {0, 0}, b = ymm;
+                    stc(h + 2 * tri(mm, mm), b);
+                    for (int nn = mm + 1; nn <= pt; ++nn) {
+                        const cplx cc = (1.0 / ((double)(nn - mm) * (nn + mm))) *
+                                        (((2.0 * nn - 1.0) * d.z) * b - r2 * a);
+                        stc(h + 2 * tri(nn, mm), cc);
+                        a = b;
+                        b = cc;
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < NRC; ++q)
+                if (q < rc) {
+                    const double *pq = str + 2 * ((size_t)si * nr + r0 + q);
+                    Qs[2 * (tid * NRC + q)] = pq[0];
+                    Qs[2 * (tid * NRC + q) + 1] = pq[1];
+                }
+            if (dip) {
+                Nu[3 * tid] = nrm[si];
+                Nu[3 * tid + 1] = nrm[(size_t)ns + si];
+                Nu[3 * tid + 2] = nrm[2 * (size_t)ns + si];
+            }
+        }
+        __syncthreads();
+        if (act)
+            for (int sl = 0; sl < cnt; ++sl) {
+                cplx b;
+                if (!dip) {
+                    b = LOCAL ? hval(sl, n, m) : conj(hval(sl, n, m));
+                } else {
+                    const double nx = Nu[3 * sl], ny = Nu[3 * sl + 1], nz = Nu[3 * sl + 2];
+                    const cplx nmn = {nx, -ny}, npl = {nx, ny};
+                    if (LOCAL) {
+                        b = (-nz) * hval(sl, n + 1, m) +
+                            0.5 * (nmn * hval(sl, n + 1, m + 1) - npl * hval(sl, n + 1, m - 1));
+                    } else {
+                        b = conj(nz * hval(sl, n - 1, m) +
+                                 0.5 * (nmn * hval(sl, n - 1, m + 1) -
+                                        npl * hval(sl, n - 1, m - 1)));
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < NRC; ++q)
+                    if (q < rc)
+                        acc[q] += cplx{Qs[2 * (sl * NRC + q)], Qs[2 * (sl * NRC + q) + 1]} * b;
+            }
+    };
+
+    int fill = 0;
+    for (int e = e0; e < e1; ++e) {
+        const int Sb = LOCAL ? lists[e] : Bx;
+        int s = t.sstart[Sb];
+        const int s1 = s + t.scount[Sb];
+        while (s < s1) {
+            const int take = min(s1 - s, S - fill);
+            for (int j = tid; j < take; j += nth) idx[fill + j] = s + j;
+            fill += take;
+            s += take;
+            if (fill == S) {
+                flush(S);
+                fill = 0;
+            }
+        }
+    }
+    if (fill > 0) flush(fill);
+    if (act) {
+        double *o = expn + ecoef_off(kd, Bx, lv) + 2L * tid * nr;
+#pragma unroll
+        for (int q = 0; q < NRC; ++q)
+            if (q < rc) {
+                double *pq = o + 2 * (r0 + q);
+                stc(pq, LOCAL ? ldc(pq) + acc[q] : acc[q]);
+            }
+    }
+}
+
 // child offset index bits: bit d set = upper half in dimension d
 __device__ __forceinline__ const double *child_tab(const KD &kd, int ci)
 {
@@ -648,41 +808,60 @@ __device__ void qbx_store(const KD &kd, double *qexp, int c, const QAcc<PQ, NRC>
             }
 }
 
-constexpr int P2Q_LANES = 16;
+// P2QBXL over balanced work units (unit_plan over the global QBX centers of
+// each target box): the unit's TG centers take G lanes apiece; the box's
+// list-1 sources are staged through LDS in chunks (positions, normals,
+// strengths, coalesced) and each center's lanes split a chunk; the expansions
+// are reduced by shuffles within the G-lane groups.
+template <int NRC>
+struct P2QTile3 {
+    static constexpr int CH = NRC == 1 ? 256 : 64;
+};
 
-// P2QBXL: 16 lanes per global center over its list-1 sources
 template <int PQ, int NRC>
-__global__ void __launch_bounds__(256) k3_p2qbxl(KD kd, TD t, int ng, const int *gcent,
-                                                 const int *c2tb, const double *ctr, int nc,
-                                                 const int *l1s, const int *l1l,
-                                                 const double *src, int ns, const double *str,
-                                                 const double *nrm, double *qexp, int accumulate)
+__global__ void __launch_bounds__(64) k3_p2qbxl(KD kd, TD t, const int *ubox, const int *ut0,
+                                                const int *box_g, const int *gcs,
+                                                const int *gcl, const double *ctr, int nc,
+                                                const int *l1s, const int *l1l,
+                                                const double *src, int ns, const double *str,
+                                                const double *nrm, double *qexp, int accumulate)
 {
     constexpr int NQ = (PQ + 1) * (PQ + 1);
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int g = gid / P2Q_LANES, lane = gid % P2Q_LANES;
-    const bool active = g < ng;
+    constexpr int CH = P2QTile3<NRC>::CH;
+    __shared__ double sx[CH], sy[CH], sz[CH], snx[CH], sny[CH], snz[CH];
+    __shared__ double sq[CH * 2 * NRC];
+    const int u = blockIdx.x;
+    const int it = ubox[u];
+    const int G = box_g[it];
+    const int lane = threadIdx.x;
+    const int tsub = lane / G, sub = lane & (G - 1);
+    const int cl = ut0[u] + tsub;
+    const bool active = cl < gcs[it + 1] - gcs[it];
+    const bool dip = kd.dip;
+    const int rc = kd.rc, r0 = kd.r0, nr = kd.nr;
+    int c = 0;
+    v3 cc = {0, 0, 0};
+    if (active) {
+        c = gcl[gcs[it] + cl];
+        cc = ld3(ctr, nc, c);
+    }
     QAcc<PQ, NRC> A;
 #pragma unroll
     for (int q = 0; q < NRC; ++q)
 #pragma unroll
         for (int i = 0; i < NQ; ++i) A.v[q][i] = {0, 0};
-    int c = 0;
-    if (active) {
-        c = gcent[g];
-        const int it = c2tb[c];
-        const v3 cc = ld3(ctr, nc, c);
-        const int e0 = l1s[it], e1 = l1s[it + 1];
-        for (int e = e0; e < e1; ++e) {
-            const int Sb = l1l[e];
-            const int s0 = t.sstart[Sb], s1 = s0 + t.scount[Sb];
-            for (int s = s0 + lane; s < s1; s += P2Q_LANES) {
-                const v3 d = sub(ld3(src, ns, s), cc);
+
+    auto flush = [&](int n) {
+        __syncthreads();
+        if (active) {
+            for (int j = sub; j < n; j += G) {
+                const v3 d = {sx[j] - cc.x, sy[j] - cc.y, sz[j] - cc.z};
                 cplx qs[NRC];
 #pragma unroll
                 for (int q = 0; q < NRC; ++q)
-                    qs[q] = q < kd.rc ? ldq(kd, str, s, kd.r0 + q) : cplx{0, 0};
-                if (!kd.dip) {
+                    qs[q] = q < rc ? cplx{sq[2 * (j * NRC + q)], sq[2 * (j * NRC + q) + 1]}
+                                   : cplx{0, 0};
+                if (!dip) {
                     cplx T[NQ];
                     the_table<PQ>(d, T);
 #pragma unroll
@@ -693,13 +872,13 @@ __global__ void __launch_bounds__(256) k3_p2qbxl(KD kd, TD t, int ng, const int 
                     constexpr int P1 = PQ + 1;
                     cplx T[(P1 + 1) * (P1 + 1)];
                     the_table<P1>(d, T);
-                    const v3 nu = ld3(nrm, ns, s);
-                    const cplx nmn = {nu.x, -nu.y}, npl = {nu.x, nu.y};
+                    const cplx nmn = {snx[j], -sny[j]}, npl = {snx[j], sny[j]};
+                    const double nz = snz[j];
 #pragma unroll
                     for (int n = 0; n <= PQ; ++n)
 #pragma unroll
                         for (int m = -n; m <= n; ++m) {
-                            const cplx g2 = (-nu.z) * T[IDX(n + 1, m)] +
+                            const cplx g2 = (-nz) * T[IDX(n + 1, m)] +
                                             0.5 * (nmn * T[IDX(n + 1, m + 1)] -
                                                    npl * T[IDX(n + 1, m - 1)]);
 #pragma unroll
@@ -708,18 +887,53 @@ __global__ void __launch_bounds__(256) k3_p2qbxl(KD kd, TD t, int ng, const int 
                 }
             }
         }
+        __syncthreads();
+    };
+
+    int fill = 0;
+    const int e1 = l1s[it + 1];
+    for (int e = l1s[it]; e < e1; ++e) {
+        const int Sb = l1l[e];
+        int s = t.sstart[Sb];
+        const int s1 = s + t.scount[Sb];
+        while (s < s1) {
+            const int take = min(s1 - s, CH - fill);
+            for (int j = lane; j < take; j += 64) {
+                const int si = s + j, dd = fill + j;
+                sx[dd] = src[si];
+                sy[dd] = src[(size_t)ns + si];
+                sz[dd] = src[2 * (size_t)ns + si];
+                if (dip) {
+                    snx[dd] = nrm[si];
+                    sny[dd] = nrm[(size_t)ns + si];
+                    snz[dd] = nrm[2 * (size_t)ns + si];
+                }
+#pragma unroll
+                for (int q = 0; q < NRC; ++q)
+                    if (q < rc) {
+                        const double *pq = str + 2 * ((size_t)si * nr + r0 + q);
+                        sq[2 * (dd * NRC + q)] = pq[0];
+                        sq[2 * (dd * NRC + q) + 1] = pq[1];
+                    }
+            }
+            fill += take;
+            s += take;
+            if (fill == CH) {
+                flush(CH);
+                fill = 0;
+            }
+        }
     }
+    if (fill > 0) flush(fill);
 #pragma unroll
     for (int q = 0; q < NRC; ++q)
 #pragma unroll
-        for (int i = 0; i < NQ; ++i) {
-#pragma unroll
-            for (int m = P2Q_LANES / 2; m >= 1; m >>= 1) {
-                A.v[q][i].x += __shfl_xor(A.v[q][i].x, m, P2Q_LANES);
-                A.v[q][i].y += __shfl_xor(A.v[q][i].y, m, P2Q_LANES);
+        for (int i = 0; i < NQ; ++i)
+            for (int m = G / 2; m >= 1; m >>= 1) {
+                A.v[q][i].x += __shfl_xor(A.v[q][i].x, m);
+                A.v[q][i].y += __shfl_xor(A.v[q][i].y, m);
             }
-        }
-    if (active && lane == 0) qbx_store<PQ, NRC>(kd, qexp, c, A, accumulate);
+    if (active && sub == 0) qbx_store<PQ, NRC>(kd, qexp, c, A, accumulate);
 }
 
 constexpr int M2Q_LANES = 16;
@@ -1136,6 +1350,34 @@ int f3_ncoeff_qbx(const qbx_kernel_desc *d)
     return kd.ncq;
 }
 
+// Shape of k3_p2e at order p: S sources per chunk in 32 KiB of LDS, threads
+// covering max((p+1)^2, S) (at most 512).  False: the plain kernels.
+static bool p2e3_launch(int p, bool local, P2EShape3 *sh, int *threads, size_t *bytes)
+{
+    const int pt = local ? p + 1 : p;
+    P2EShape3 h;
+    h.R = ((pt + 1) * (pt + 2) / 2) | 1;
+    const long per = 8L * (2 * h.R + 2 * NRC_S + 3 + 1);
+    h.S = (int)std::min(64L, 32768L / per);
+    const int nout = (p + 1) * (p + 1);
+    if (h.S < 4 || nout > 512) return false;
+    *sh = h;
+    *threads = 64 * ((std::max(nout, h.S) + 63) / 64);
+    *bytes = 8UL * h.doubles(NRC_S);
+    return true;
+}
+
+static bool p2e3_all_levels(const KD &kd, int nlevels, bool local)
+{
+    for (int l = 0; l < nlevels; ++l) {
+        P2EShape3 sh;
+        int th;
+        size_t by;
+        if (!p2e3_launch(kd.order[l], local, &sh, &th, &by)) return false;
+    }
+    return true;
+}
+
 int f3_form_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
                        const qbx_kernel_desc *d, const double *str, const double *nrm,
                        double *mpole)
@@ -1145,6 +1387,26 @@ int f3_form_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_
     QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
     hipStream_t st = S(stream);
     QBX_TRY(zero_fill(mpole, (exp_bytes(kd)) / 8, st));
+    if (p2e3_all_levels(kd, T->nlevels, false)) {
+        const TD td = make_td(T);
+        for (int l = 0; l < T->nlevels; ++l) {
+            const int i0 = R->h_level_start_source_box_nrs[l];
+            const int i1 = R->h_level_start_source_box_nrs[l + 1];
+            if (i1 == i0) continue;
+            P2EShape3 sh;
+            int th;
+            size_t by;
+            p2e3_launch(kd.order[l], false, &sh, &th, &by);
+#define L3_P2MH(N)                                                                         \
+    hipLaunchKernelGGL((k3_p2e<N, false>), dim3(i1 - i0), dim3(th), by, st, kd, td, l, i0, sh, \
+                       R->source_boxes.p, nullptr, nullptr, T->sources.p, T->nsources, str,  \
+                       nrm, mpole)
+            FOR_RHS_CHUNKS3(kd, NRC_S, L3_P2MH);
+#undef L3_P2MH
+        }
+        QBX_HIP(hipGetLastError());
+        return QBX_OK;
+    }
     const Flat fl = make_flat(kd, R->h_level_start_source_box_nrs);
     const long nthr = fl.tstart[fl.nl];
     if (nthr > 0) {
@@ -1248,6 +1510,26 @@ int f3_form_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav
     QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
     hipStream_t st = S(stream);
     if (!accumulate) QBX_TRY(zero_fill(local, (exp_bytes(kd)) / 8, st));
+    if (R->n4 > 0 && p2e3_all_levels(kd, T->nlevels, true)) {
+        const TD td = make_td(T);
+        for (int l = 0; l < T->nlevels; ++l) {
+            const int i0 = R->h_level_start_ttp_box_nrs[l];
+            const int i1 = R->h_level_start_ttp_box_nrs[l + 1];
+            if (i1 == i0) continue;
+            P2EShape3 sh;
+            int th;
+            size_t by;
+            p2e3_launch(kd.order[l], true, &sh, &th, &by);
+#define L3_P2LH(N)                                                                         \
+    hipLaunchKernelGGL((k3_p2e<N, true>), dim3(i1 - i0), dim3(th), by, st, kd, td, l, i0, sh, \
+                       R->ttp_boxes.p, R->l4_starts.p, R->l4_lists.p, T->sources.p,         \
+                       T->nsources, str, nrm, local)
+            FOR_RHS_CHUNKS3(kd, NRC_S, L3_P2LH);
+#undef L3_P2LH
+        }
+        QBX_HIP(hipGetLastError());
+        return QBX_OK;
+    }
     const Flat fl = make_flat(kd, R->h_level_start_ttp_box_nrs);
     const long nthr = fl.tstart[fl.nl];
     if (nthr > 0 && R->n4 > 0) {
@@ -1296,14 +1578,15 @@ int f3_form_global_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T,
     hipStream_t st = S(stream);
     if (!accumulate)
         QBX_TRY(zero_fill(qexp, (sizeof(double) * 2 * (size_t)G->ncenters * kd.ncq * kd.nr) / 8, st));
-    const int ng = G->nglobal;
-    if (ng == 0) return QBX_OK;
+    const unit_plan &U = G->p2q_plan;
+    QBX_CHECK(G->trav == R, "P2QBXL: geometry was built for another traversal");
+    if (G->nglobal == 0 || U.nunits == 0) return QBX_OK;
     const TD td = make_td(T);
 #define L3_P2Q(N)                                                                          \
-    DISPATCH3_PQ_NR(kd.pq, N, k3_p2qbxl, dim3(cdiv((long)ng * P2Q_LANES, 256)), dim3(256), \
-                    0, st, kd, td, ng, G->global_qbx_centers.p, G->center_to_target_box.p, \
-                    G->centers.p, G->ncenters, R->l1_starts.p, R->l1_lists.p, T->sources.p, \
-                    T->nsources, str, nrm, qexp, accumulate)
+    DISPATCH3_PQ_NR(kd.pq, N, k3_p2qbxl, dim3(U.nunits), dim3(64), 0, st, kd, td,          \
+                    U.unit_box.p, U.unit_t0.p, U.box_g.p, G->gc_by_tbox_starts.p,           \
+                    G->gc_by_tbox.p, G->centers.p, G->ncenters, R->l1_starts.p,             \
+                    R->l1_lists.p, T->sources.p, T->nsources, str, nrm, qexp, accumulate)
     FOR_RHS_CHUNKS3(kd, NRC_Q, L3_P2Q);
 #undef L3_P2Q
     QBX_HIP(hipGetLastError());
