@@ -154,15 +154,17 @@ __device__ __forceinline__ cplx p2p_value_wave(const KD &kd, cplx s, cplx nu, cp
         v = !kd.dip ? cplx{-0.5 * log(r2s) * INV2PI, 0.0}
                     : cplx{(nu * cinv(ws)).x * INV2PI, 0.0};
     } else {
-        const double r = sqrt(r2s);
+        const double ir = 1.0 / sqrt(r2s);
+        const double r = r2s * ir;
         const double kr = kd.k * r;
+        const double ikr = ir / kd.k;
         cplx h0, h1;
         if (!kd.dip) {
-            hankel01_wave<true, false>(kr, tab, h0, h1);
+            hankel01_wave<true, false>(kr, ikr, tab, h0, h1);
             v = {-0.25 * h0.y, 0.25 * h0.x};
         } else {
-            hankel01_wave<false, true>(kr, tab, h0, h1);
-            const double proj = (ws.x * nu.x + ws.y * nu.y) / r;
+            hankel01_wave<false, true>(kr, ikr, tab, h0, h1);
+            const double proj = (ws.x * nu.x + ws.y * nu.y) * ir;
             const cplx hv = (kd.k * proj) * h1;
             v = {-0.25 * hv.y, 0.25 * hv.x};
         }
@@ -1109,7 +1111,7 @@ __device__ void eval_mpole_add(const KD &kd, const double *a, int p, cplx c, dou
 // reduced by shuffles in a fixed order.
 template <int NRC>
 struct P2PTile {
-    static constexpr int CH = NRC == 1 ? 256 : 64;
+    static constexpr int CH = NRC == 1 ? 128 : 64;
 };
 
 template <int NRC>
@@ -1347,13 +1349,14 @@ __device__ __forceinline__ void p2qbxl_pair(const KD &kd, cplx w, cplx nu, const
         }
         return;
     }
-    const double r = sqrt(abs2(w));
+    const double ir = 1.0 / sqrt(abs2(w));
+    const double r = abs2(w) * ir;
     const double x = kd.k * r;
-    const double ir = 1.0 / r;
+    const double ix = ir / kd.k;
     const cplx em = {w.x * ir, -w.y * ir};  // e^{-i theta}
     cplx H[PQ + 2];
-    hankel01_wave<true, true>(x, tab, H[0], H[1]);
-    const double tox = 2.0 / x;
+    hankel01_wave<true, true>(x, ix, tab, H[0], H[1]);
+    const double tox = 2.0 * ix;
 #pragma unroll
     for (int m = 2; m <= PQ + 1; ++m) H[m] = ((m - 1) * tox) * H[m - 1] - H[m - 2];
     // Gv[m + PQ + 1] = H_m e^{-i m theta}, m in [-(PQ+1), PQ+1]

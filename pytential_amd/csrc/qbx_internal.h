@@ -76,7 +76,7 @@ constexpr int MAX_LEVELS = 64;
 // walks more than about *_UNIT_ITER items, and so that boxes with few targets
 // still fill the wavefront.  A target belongs to exactly one unit, so units
 // write their targets' sums without atomics.
-constexpr int UNIT_ITER = 64;      // P2P: list-1 sources per lane
+constexpr int UNIT_ITER = 128;     // P2P: list-1 sources per lane
 constexpr int M2P_UNIT_ITER = 8;   // M2P: list-3 boxes per lane
 constexpr int P2Q_UNIT_ITER = 32;  // P2QBXL: list-1 sources per lane
 constexpr int M2Q_UNIT_ITER = 2;   // M2QBXL: list-3 boxes per lane

@@ -204,8 +204,8 @@ __device__ __noinline__ void hankel01_call(double x, cplx &h0, cplx &h1)
 constexpr int HANKEL_NSER = 18;
 
 template <int K, bool WANT0, bool WANT1>
-__device__ __forceinline__ void hankel01_series_tab(double x, const double *tab, cplx &h0,
-                                                    cplx &h1)
+__device__ __forceinline__ void hankel01_series_tab(double x, double ix, const double *tab,
+                                                    cplx &h0, cplx &h1)
 {
     const double t = 0.25 * x * x;
     double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
@@ -228,7 +228,7 @@ __device__ __forceinline__ void hankel01_series_tab(double x, const double *tab,
     if (WANT0) h0 = {a, TWO_OVER_PI * lg * a - ONE_OVER_PI * c};
     if (WANT1) {
         double j1v = 0.5 * x * b;
-        h1 = {j1v, -TWO_OVER_PI / x + TWO_OVER_PI * lg * j1v - 0.5 * x * ONE_OVER_PI * d};
+        h1 = {j1v, -TWO_OVER_PI * ix + TWO_OVER_PI * lg * j1v - 0.5 * x * ONE_OVER_PI * d};
     }
 }
 
@@ -245,9 +245,11 @@ __device__ __forceinline__ void hankel_table_fill(double *tab, int lane, int nla
 // The values of hankel01_fast with the series length chosen once per
 // wavefront: the lanes with x <= 4 share one series of the length the
 // largest of them needs (one pass instead of up to three divergent ones);
-// lanes beyond 4 take the device library in their own pass.
+// lanes beyond 4 take the device library in their own pass.  ix = 1 / x
+// (callers that have 1/r at hand save a division).
 template <bool WANT0, bool WANT1>
-__device__ __forceinline__ void hankel01_wave(double x, const double *tab, cplx &h0, cplx &h1)
+__device__ __forceinline__ void hankel01_wave(double x, double ix, const double *tab, cplx &h0,
+                                              cplx &h1)
 {
     const bool big = x > 4.0;
     if (__any(big)) {
@@ -256,11 +258,11 @@ __device__ __forceinline__ void hankel01_wave(double x, const double *tab, cplx 
     if (__any(!big)) {
         cplx a0 = {0, 0}, a1 = {0, 0};
         if (__any(!big && x > 3.0))
-            hankel01_series_tab<18, WANT0, WANT1>(x, tab, a0, a1);
+            hankel01_series_tab<18, WANT0, WANT1>(x, ix, tab, a0, a1);
         else if (__any(!big && x > 1.5))
-            hankel01_series_tab<14, WANT0, WANT1>(x, tab, a0, a1);
+            hankel01_series_tab<14, WANT0, WANT1>(x, ix, tab, a0, a1);
         else
-            hankel01_series_tab<10, WANT0, WANT1>(x, tab, a0, a1);
+            hankel01_series_tab<10, WANT0, WANT1>(x, ix, tab, a0, a1);
         if (!big) {
             h0 = a0;
             h1 = a1;
