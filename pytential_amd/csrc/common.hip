@@ -192,6 +192,10 @@ int qbx_ctx_destroy(qbx_ctx *ctx)
 {
     if (!ctx) return QBX_OK;
     if (ctx->scratch) (void)hipFree(ctx->scratch);
+    for (hipEvent_t &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (hipStream_t &s : ctx->side)
+        if (s) (void)hipStreamDestroy(s);
     delete ctx;
     return QBX_OK;
 }

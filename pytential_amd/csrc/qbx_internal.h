@@ -103,6 +103,10 @@ struct qbx_ctx {
     // 3D Laplace unit-offset harmonic tables and the (nsep, pmax) they hold
     qbx::dbuf<double> ws_tab3_m2l, ws_tab3_ch;
     std::vector<double> tab3_key;
+    // side streams (default priority) and fork/join events of qbx_drive_fmm,
+    // created on its first call
+    hipStream_t side[2] = {nullptr, nullptr};
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
 struct qbx_tree {
