@@ -123,6 +123,22 @@ __global__ void k_zero(double *p, long n)
     for (long k = i; k < n; k += stride) p[k] = 0.0;
 }
 
+__global__ void k_add(double *dst, const double *src, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (long k = i; k < n; k += stride) dst[k] += src[k];
+}
+
+int add_into(double *dst, const double *src, long ndoubles, hipStream_t st)
+{
+    if (ndoubles <= 0) return QBX_OK;
+    const long blocks = std::min<long>(cdiv(ndoubles, 256), 8192);
+    hipLaunchKernelGGL(k_add, dim3((unsigned)blocks), dim3(256), 0, st, dst, src, ndoubles);
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
 int zero_fill(double *p, long ndoubles, hipStream_t st)
 {
     if (ndoubles <= 0) return QBX_OK;

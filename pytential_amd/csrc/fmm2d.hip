@@ -2417,6 +2417,7 @@ int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav 
     const long np = 2L * std::max(T->ntargets, 1) * nr;
     if (ctx->ws_mpole.n < nm) QBX_TRY(ctx->ws_mpole.alloc(nm));
     if (ctx->ws_local.n < nm) QBX_TRY(ctx->ws_local.alloc(nm));
+    if (ctx->ws_local2.n < nm) QBX_TRY(ctx->ws_local2.alloc(nm));
     if (ctx->ws_qbx.n < nq) QBX_TRY(ctx->ws_qbx.alloc(nq));
     if (ctx->ws_pot.n < np) QBX_TRY(ctx->ws_pot.alloc(np));
     double *M = ctx->ws_mpole.p, *L = ctx->ws_local.p, *Q = ctx->ws_qbx.p, *P = ctx->ws_pot.p;
@@ -2424,21 +2425,26 @@ int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav 
         for (hipStream_t &s : ctx->side) QBX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         for (hipEvent_t &e : ctx->ev) QBX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
-    // The stage DAG on three streams (the stage order of drive_fmm is kept
+    // The stage DAG on four streams (the stage order of drive_fmm is kept
     // within each stream; every array has one writer stream at a time):
-    //   main: P2M -> M2M -[eM]-> M2L -> P2L -> L2L -[eL]->     ... join -> finalize
+    //   main: P2M -> M2M -[eM]-> M2L -(wait eP) L += L4 -> L2L -[eL]-> ... join
+    //   s3:   P2L into L4 (its own local buffer) -[eP]->
     //   s1:   P2P, wait eM: M2P, wait eL: L2P          (non-QBX targets of P)
     //   s2:   P2QBXL, wait eM: M2QBXL, wait eL: L2QBXL, QBXL2P  (Q, QBX targets of P)
-    // The latency-bound level-by-level translations overlap the near field.
-    hipStream_t st = S(stream), s1 = ctx->side[0], s2 = ctx->side[1];
+    // so the latency-bound level-by-level translations overlap the near field.
+    hipStream_t st = S(stream), s1 = ctx->side[0], s2 = ctx->side[1], s3 = ctx->side[2];
     hipEvent_t e0 = ctx->ev[0], eM = ctx->ev[1], eL = ctx->ev[2], e1 = ctx->ev[3],
-               e2 = ctx->ev[4];
+               e2 = ctx->ev[4], eP = ctx->ev[5];
+    double *L4 = ctx->ws_local2.p;
     QBX_TRY(zero_fill(P, np, st));
     QBX_HIP(hipEventRecord(e0, st));
     QBX_HIP(hipStreamWaitEvent(s1, e0, 0));
     QBX_HIP(hipStreamWaitEvent(s2, e0, 0));
+    QBX_HIP(hipStreamWaitEvent(s3, e0, 0));
     QBX_TRY(qbx_eval_direct(ctx, s1, T, R, G, d, str, nrm, P, 1));
     QBX_TRY(qbx_form_global_qbx_locals(ctx, s2, T, R, G, d, str, nrm, Q, 0));
+    QBX_TRY(qbx_form_locals(ctx, s3, T, R, d, str, nrm, L4, 0));
+    QBX_HIP(hipEventRecord(eP, s3));
     QBX_TRY(qbx_form_multipoles(ctx, st, T, R, d, str, nrm, M));
     QBX_TRY(qbx_coarsen_multipoles(ctx, st, T, R, d, M));
     QBX_HIP(hipEventRecord(eM, st));
@@ -2447,7 +2453,8 @@ int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav 
     QBX_TRY(qbx_eval_multipoles(ctx, s1, T, R, G, d, M, P, 1));
     QBX_TRY(qbx_translate_box_multipoles_to_qbx_local(ctx, s2, T, R, G, d, M, Q, 1));
     QBX_TRY(qbx_multipole_to_local(ctx, st, T, R, d, M, L));
-    QBX_TRY(qbx_form_locals(ctx, st, T, R, d, str, nrm, L, 1));
+    QBX_HIP(hipStreamWaitEvent(st, eP, 0));
+    QBX_TRY(add_into(L, L4, nexp * 2, st));
     QBX_TRY(qbx_refine_locals(ctx, st, T, R, d, L));
     QBX_HIP(hipEventRecord(eL, st));
     QBX_HIP(hipStreamWaitEvent(s1, eL, 0));

@@ -96,7 +96,7 @@ struct qbx_ctx {
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
     // FMM workspaces for qbx_drive_fmm
-    qbx::dbuf<double> ws_mpole, ws_local, ws_qbx, ws_pot;
+    qbx::dbuf<double> ws_mpole, ws_local, ws_local2, ws_qbx, ws_pot;
     // Helmholtz translation tables and the (tree, k, orders) they were built for
     qbx::dbuf<double> ws_tab_m2l, ws_tab_mm;
     std::vector<double> tab_key;
@@ -105,7 +105,7 @@ struct qbx_ctx {
     std::vector<double> tab3_key;
     // side streams (default priority) and fork/join events of qbx_drive_fmm,
     // created on its first call
-    hipStream_t side[2] = {nullptr, nullptr};
+    hipStream_t side[3] = {nullptr, nullptr, nullptr};
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
@@ -171,6 +171,8 @@ int compact_indices(qbx_ctx *ctx, hipStream_t st, const int *flags, int n, dbuf<
                     int *count);
 // stream-ordered zero fill of n doubles by a kernel (safe inside graph capture)
 int zero_fill(double *p, long ndoubles, hipStream_t st);
+// dst[i] += src[i] for n doubles, stream-ordered
+int add_into(double *dst, const double *src, long ndoubles, hipStream_t st);
 // work units of a target-box list (starts/lists over the traversal's target
 // boxes) for the geometry's non-QBX targets, or its global QBX centers
 // (centers); work per entry is the entry box's source count (by_sources) or one
