@@ -844,8 +844,9 @@ __global__ void k_l2l(KD kd, TD t, const double *B, int lv, int i0, int i1, cons
 // reads only LDS (the a_n are broadcast, the rows contiguous across m).  The
 // per-level launches at coarse levels have few boxes and long rows, where
 // the plain one-thread-per-coefficient kernels wait on serial global loads.
-// (M2L keeps the plain kernel: its ~75 list-2 boxes per target box keep the
-// lanes busy from L1 and it measured faster than staging each box.)
+// (M2L keeps the plain flat kernel, one thread per (box, coefficient): a
+// block per box with the list-2 coefficient vectors staged in LDS, and a
+// register-blocked variant over outputs, both measured slower.)
 // -------------------------------------------------------------------------
 constexpr int HT_MAX_THREADS = 256;
 constexpr long HT_LDS_BUDGET = 32768;  // bytes of LDS per block for staged sources

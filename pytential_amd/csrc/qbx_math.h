@@ -101,41 +101,55 @@ __device__ __forceinline__ double miller_j(double x, int nmax, Consume consume, 
 //   Y1 = -2/(pi x) + (2/pi) ln(x/2) J1 - (x/(2 pi)) sum D_k t^k,
 //   A_k = (-1)^k/(k!)^2,  B_k = (-1)^k/(k!(k+1)!),
 //   C_k = (-1)^k 2 psi(k+1)/(k!)^2,  D_k = (-1)^k (psi(k+1)+psi(k+2))/(k!(k+1)!)
-// (coefficients generated at 40 digits).  Relative error of H0 and H1 against
-// scipy: 10 terms < 1.1e-15 for x <= 1.5, 14 terms < 9e-16 for x <= 3, 18
-// terms < 1.8e-15 for x <= 4; beyond, the device library.
+// (coefficients generated at 50 digits, psi(k+1) = H_k - gamma).  Relative
+// error of H0 and H1 against scipy: 10 terms < 1.1e-15 for x <= 1.5, 14 terms
+// < 9e-16 for x <= 3, 18 terms < 2.7e-15 for x <= 4, 22 terms < 6.2e-15 for
+// x <= 5, 26 terms < 1.4e-14 for x <= 6 (cancellation grows with x);
+// hankel01_fast switches to the device library above 4, hankel01_wave above 6.
 // ---------------------------------------------------------------------------
-static __constant__ double c_ser_A[18] = {
+static __constant__ double c_ser_A[26] = {
     1.0, -1.0, 0.25,
     -0.027777777777777776, 0.001736111111111111, -6.944444444444444e-05,
     1.9290123456790124e-06, -3.936759889140842e-08, 6.151187326782565e-10,
     -7.594058428126624e-12, 7.594058428126623e-14, -6.276081345559193e-16,
     4.358389823304995e-18, -2.5789288895295828e-20, 1.3157800456783586e-22,
     -5.8479113141260385e-25, 2.2843403570804838e-27, -7.904291893012054e-30,
+    2.4395962632753253e-32, -6.757884385804225e-35, 1.6894710964510564e-37,
+    -3.8310002187098785e-40, 7.915289708078262e-43, -1.4962740468957016e-45,
+    2.5976979980828152e-48, -4.156316796932504e-51,
 };
-static __constant__ double c_ser_B[18] = {
+static __constant__ double c_ser_B[26] = {
     1.0, -0.5, 0.08333333333333333,
     -0.006944444444444444, 0.00034722222222222224, -1.1574074074074073e-05,
     2.755731922398589e-07, -4.920949861426052e-09, 6.834652585313961e-11,
     -7.594058428126623e-13, 6.903689480115112e-15, -5.230067787965994e-17,
     3.352607556388458e-19, -1.842092063949702e-21, 8.771866971189057e-24,
     -3.654944571328774e-26, 1.3437296218120491e-28, -4.391273273895586e-31,
+    1.2839980333028028e-33, -3.3789421929021126e-36, 8.045100459290744e-39,
+    -1.7413637357772174e-41, 3.4414303078601135e-44, -6.234475195398757e-47,
+    1.0390791992331261e-49, -1.5985833834355786e-52,
 };
-static __constant__ double c_ser_C[18] = {
+static __constant__ double c_ser_C[26] = {
     -1.1544313298030657, -0.8455686701969343, 0.46139216754923357,
     -0.0697843149128778, 0.0052295752376104185, -0.00023696078728219452,
     7.2252482063984075e-06, -1.5870193042608417e-07, 2.6334973460771295e-09,
     -3.419988145411616e-11, 3.571869313974148e-13, -3.0660686806416945e-15,
     2.2018541919451488e-17, -1.3425481319958094e-19, 7.03770394527961e-22,
     -3.205840570979285e-24, 1.2808357275022893e-26, -4.5249492039395507e-29,
+    1.4236958856226736e-31, -4.0148915409660105e-34, 1.0206175962060132e-36,
+    -2.350811331348287e-39, 4.929005384429404e-42, -9.447701305456722e-45,
+    1.6618734044035933e-47, -2.692247981421209e-50,
 };
-static __constant__ double c_ser_D[18] = {
+static __constant__ double c_ser_D[26] = {
     -0.15443132980306573, -0.6727843350984671, 0.18157516696085563,
     -0.019182189839330562, 0.001115359491966528, -4.142247689271143e-05,
     1.071545914091181e-06, -2.045286003593878e-08, 3.002048746589188e-10,
     -3.495928729692882e-12, 3.309914735250272e-14, -2.5986411321011286e-16,
     1.7195232826992565e-18, -9.721207518823618e-21, 4.750281743327667e-23,
     -2.0264937604328578e-25, 7.613370727767116e-28, -2.5382566314880593e-30,
+    7.560715083977377e-33, -2.0243404814475158e-35, 4.898393793644304e-38,
+    -1.0764658948663905e-40, 2.1580085597860893e-43, -3.962519190587796e-46,
+    6.689056785583698e-49, -1.0416283904829096e-51,
 };
 
 template <int K, bool WANT0, bool WANT1>
@@ -179,29 +193,45 @@ __device__ __forceinline__ void hankel01_fast(double x, cplx &h0, cplx &h1)
     }
 }
 
-// H_0, H_1 for x > 4 from the device library, kept out of line so that its
+// H_0, H_1 from the device library, kept out of line so that its
 // coefficients are materialised inside the call rather than held in
-// registers across a caller's loop
+// registers across a caller's loop; returned by value (in registers: a
+// by-reference result would go through scratch memory)
+struct H01 {
+    cplx h0, h1;
+};
+
 template <bool WANT0, bool WANT1>
-__device__ __noinline__ void hankel01_libm(double x, cplx &h0, cplx &h1)
+__device__ __noinline__ H01 hankel01_libm(double x)
 {
-    if (WANT0) h0 = {j0(x), y0(x)};
-    if (WANT1) h1 = {j1(x), y1(x)};
+    H01 r = {{0, 0}, {0, 0}};
+    if (WANT0) r.h0 = {j0(x), y0(x)};
+    if (WANT1) r.h1 = {j1(x), y1(x)};
+    return r;
 }
 
 // hankel01_fast out of line: for callers that evaluate it once per outer
 // iteration, so that neither the series coefficients nor the device library's
 // are held in registers across the caller's loops
-__device__ __noinline__ void hankel01_call(double x, cplx &h0, cplx &h1)
+__device__ __noinline__ H01 hankel01_call_v(double x)
 {
-    hankel01_fast<true, true>(x, h0, h1);
+    H01 r;
+    hankel01_fast<true, true>(x, r.h0, r.h1);
+    return r;
+}
+
+__device__ __forceinline__ void hankel01_call(double x, cplx &h0, cplx &h1)
+{
+    const H01 r = hankel01_call_v(x);
+    h0 = r.h0;
+    h1 = r.h1;
 }
 
 // The series of hankel01_series with its coefficients read from tab, laid
 // out [k][A_k, C_k, B_k, D_k] so that each step reads its pair with one
 // 16-byte load (typically a copy in LDS, so that a loop around it does not
 // pin the coefficients in registers)
-constexpr int HANKEL_NSER = 18;
+constexpr int HANKEL_NSER = 26;
 
 template <int K, bool WANT0, bool WANT1>
 __device__ __forceinline__ void hankel01_series_tab(double x, double ix, const double *tab,
@@ -242,22 +272,30 @@ __device__ __forceinline__ void hankel_table_fill(double *tab, int lane, int nla
     }
 }
 
-// The values of hankel01_fast with the series length chosen once per
-// wavefront: the lanes with x <= 4 share one series of the length the
-// largest of them needs (one pass instead of up to three divergent ones);
-// lanes beyond 4 take the device library in their own pass.  ix = 1 / x
-// (callers that have 1/r at hand save a division).
+// H_0, H_1 with the series length chosen once per wavefront: the lanes with
+// x <= 6 share one series of the length the largest of them needs (one pass
+// instead of several divergent ones); lanes beyond 6 take the device library
+// in their own pass.  ix = 1 / x (callers that have 1/r at hand save a
+// division).
 template <bool WANT0, bool WANT1>
 __device__ __forceinline__ void hankel01_wave(double x, double ix, const double *tab, cplx &h0,
                                               cplx &h1)
 {
-    const bool big = x > 4.0;
+    const bool big = x > 6.0;
     if (__any(big)) {
-        if (big) hankel01_libm<WANT0, WANT1>(x, h0, h1);
+        if (big) {
+            const H01 r = hankel01_libm<WANT0, WANT1>(x);
+            h0 = r.h0;
+            h1 = r.h1;
+        }
     }
     if (__any(!big)) {
         cplx a0 = {0, 0}, a1 = {0, 0};
-        if (__any(!big && x > 3.0))
+        if (__any(!big && x > 5.0))
+            hankel01_series_tab<26, WANT0, WANT1>(x, ix, tab, a0, a1);
+        else if (__any(!big && x > 4.0))
+            hankel01_series_tab<22, WANT0, WANT1>(x, ix, tab, a0, a1);
+        else if (__any(!big && x > 3.0))
             hankel01_series_tab<18, WANT0, WANT1>(x, ix, tab, a0, a1);
         else if (__any(!big && x > 1.5))
             hankel01_series_tab<14, WANT0, WANT1>(x, ix, tab, a0, a1);
