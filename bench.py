@@ -53,18 +53,21 @@ def log(*a):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_helmholtz2d_starfish.json")
 
 
-def measured_traffic(stage):
-    """HBM bytes per launch of a stage's kernel from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary of this workload (scaled by the copy
-    calibration recorded there), or None when the summary lacks the stage."""
+def profiled_stage(stage):
+    """The committed rocprofv3 record of a stage's kernel on this workload:
+    HBM bytes per evaluation from FETCH_SIZE / WRITE_SIZE (scaled by the copy
+    calibration recorded there) and the hardware FP64 flop count from
+    SQ_INSTS_VALU_FLOPS_FP64; None when the summary lacks the stage."""
     try:
         with open(TRAFFIC_FILE) as f:
-            rec = json.load(f)["stages"].get(stage)
+            return json.load(f)["stages"].get(stage)
     except (OSError, ValueError, KeyError):
         return None
-    if not rec:
-        return None
-    return int(rec["read_bytes"] + rec["write_bytes"])
+
+
+def measured_traffic(stage):
+    rec = profiled_stage(stage)
+    return None if not rec else int(rec["read_bytes"] + rec["write_bytes"])
 
 
 def build_problem(grid_n, nelements):
@@ -843,6 +846,13 @@ def main():
               "traffic": measured_traffic(dominant),
               "flops_per_launch": int(flops[dominant]),
               "avg_launch_ms": round(ps[dominant], 4)}
+        rec = profiled_stage(dominant)
+        if rec and rec.get("hw_fp64_flops"):
+            # the same launch time against the FP64 flops the hardware counted
+            hw = rec["hw_fp64_flops"] / (ps[dominant] * 1e-3) / 1e12
+            rl["hw_fp64_flops_per_launch"] = int(rec["hw_fp64_flops"])
+            rl["hw_achieved"] = round(hw, 3)
+            rl["hw_frac"] = round(hw / FP64_PEAK_TFLOPS, 4)
     stage_rl = {k: {"ms": round(ps[k], 4), "gflop": round(flops[k] / 1e9, 3),
                     "tflops": round(flops[k] / (ps[k] * 1e-3) / 1e12, 3),
                     "frac_fp64": round(flops[k] / (ps[k] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4)}
