@@ -808,6 +808,25 @@ def main():
             f"{ {k: round(v, 4) for k, v in main_run['per_stage_ms'].items()} }")
         return
     cfg_run = run_gpu(args, cfg, config_orders, f"p{cfg.fmm_order}")
+    # accuracy on every target: the GPU direct QBX (fmm_order=False) of the
+    # same discretization, untimed
+    import torch
+
+    from pytential_amd.qbx import QBXLayerPotentialSource
+    srcd = QBXLayerPotentialSource(cfg.discr, fine_order=cfg.discr.fine_order,
+                                   qbx_order=cfg.qbx_order, fmm_order=False)
+    t_d = time.perf_counter()
+    dref, = srcd.exec_compute_potential(cfg.kernel, cfg.density,
+                                        [(cfg.targets, cfg.target_sides)], k=cfg.k,
+                                        source_kind=cfg.source_kind)
+    torch.cuda.synchronize()
+    t_d = time.perf_counter() - t_d
+    dref = dref.cpu().numpy()
+    err_all = {key: float(np.max(np.abs(run["result"] - dref)) / np.max(np.abs(dref)))
+               for key, run in (("tol", main_run), ("config_order", cfg_run))}
+    log(f"GPU direct QBX on all {cfg.ntargets} targets ({t_d:.2f}s incl. geometry): "
+        f"FMM rel err {err_all}")
+    del srcd
     batched = run_batched(args, cfg, main_run["src"], main_run["geo"], args.nrhs)
 
     rng = np.random.default_rng(11)
@@ -874,11 +893,14 @@ def main():
             "ncenters": 2 * cfg.discr.ndofs, "fmm_orders_per_level": main_run["orders"],
             "tol": args.tol, "seed": cfg.seed},
         "rel_err_vs_direct_qbx": err["tol"],
+        "rel_err_vs_gpu_direct_qbx_all_targets": err_all["tol"],
+        "gpu_direct_qbx_s": round(t_d, 3),
         "at_config_order": {
             "fmm_order": cfg.fmm_order,
             "value": round(cfg.ntargets / cfg_run["step_fused_s"], 1),
             "ms_per_step": round(cfg_run["step_fused_s"] * 1e3, 4),
-            "rel_err_vs_direct_qbx": err["config_order"]},
+            "rel_err_vs_direct_qbx": err["config_order"],
+            "rel_err_vs_gpu_direct_qbx_all_targets": err_all["config_order"]},
         "batched": {
             "nrhs": batched["nrhs"],
             "value": round(cfg.ntargets * batched["nrhs"] / batched["step_s"], 1),

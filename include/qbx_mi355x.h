@@ -250,6 +250,19 @@ int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
                   const qbx_kernel_desc *kd, const double *strengths_tree,
                   const double *normals_tree, double *out_user);
 
+/* ---- direct QBX (no FMM) ---------------------------------------------------
+ * Replaces exec_compute_potential_insn_direct (pytential/qbx/__init__.py:
+ * 866-1068) with LayerPotentialOnTargetAndCenterSubset (qbx/direct.py:41-154):
+ * the evaluation of fmm_order=False, and the accuracy oracle at full size.
+ * Every global QBX center's order-p_q expansion is formed from all sources
+ * and evaluated at its targets; every non-QBX target sums all sources.  2D
+ * Laplace / Helmholtz; same inputs and output layout as the FMM stages
+ * (strengths_tree, normals_tree, pot [ntargets][nrhs] in tree order; finish
+ * with qbx_finalize_potentials).  The level orders of kd are not used. */
+int qbx_direct_qbx(qbx_ctx *ctx, void *stream, const qbx_tree *tree, const qbx_geo *geo,
+                   const qbx_kernel_desc *kd, const double *strengths_tree,
+                   const double *normals_tree, double *pot);
+
 /* ---- target-specific QBX (3D) ---------------------------------------------
  * Replaces eval_target_specific_qbx_locals (pytential/qbx/target_specific/
  * impl.pyx:566-785, called from pytential/qbx/fmmlib.py:587-639): for each

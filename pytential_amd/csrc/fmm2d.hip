@@ -1501,6 +1501,146 @@ __global__ void __launch_bounds__(64) k_p2qbxl(KD kd, TD t, const int *ubox, con
     if (active && sub == 0) qbx_store<PQ, NRC>(kd, qexp, c, A, accumulate);
 }
 
+// Direct QBX (no FMM; pytential/qbx/__init__.py:866-1068 with
+// qbx/direct.py:41-154): every global QBX center's expansion from ALL
+// sources, and every non-QBX target's potential summed over ALL sources.
+// The sources pass through LDS in chunks of DQ_CH shared by the block: 4
+// centers x 16 lanes (P2QBXL) or 64 targets x 1 lane (P2P) per block.
+constexpr int DQ_CH = 256;
+
+template <int PQ, int NRC>
+__global__ void __launch_bounds__(64) k_direct_p2qbxl(KD kd, int ng, const int *gcent,
+                                                      const double *ctr, const double *rad,
+                                                      int nc, const double *src, int ns,
+                                                      const double *str, const double *nrm,
+                                                      double *qexp)
+{
+    __shared__ double sx[DQ_CH], sy[DQ_CH], snx[DQ_CH], sny[DQ_CH];
+    __shared__ double sq[DQ_CH * 2 * NRC];
+    __shared__ __align__(16) double htab[4 * HANKEL_NSER];
+    const int lane = threadIdx.x;
+    const int g = blockIdx.x * 4 + lane / 16, sub = lane % 16;
+    const bool active = g < ng;
+    const bool dip = kd.dip;
+    const int rc = kd.rc, r0 = kd.r0, nr = kd.nr;
+    int c = 0;
+    cplx cc = {0, 0};
+    double R = 1.0;
+    if (active) {
+        c = gcent[g];
+        cc = ldpt(ctr, nc, c);
+        R = rad[c];
+    }
+    hankel_table_fill(htab, lane, 64);
+    QAcc<PQ, NRC> A;
+    qacc_zero<PQ, NRC>(A);
+    for (int b = 0; b < ns; b += DQ_CH) {
+        const int nb = min(DQ_CH, ns - b);
+        __syncthreads();
+        for (int j = lane; j < nb; j += 64) {
+            const int si = b + j;
+            sx[j] = src[si];
+            sy[j] = src[(size_t)ns + si];
+            if (dip) {
+                snx[j] = nrm[si];
+                sny[j] = nrm[(size_t)ns + si];
+            }
+#pragma unroll
+            for (int q = 0; q < NRC; ++q)
+                if (q < rc) {
+                    const double *pq = str + 2 * ((size_t)si * nr + r0 + q);
+                    sq[2 * (j * NRC + q)] = pq[0];
+                    sq[2 * (j * NRC + q) + 1] = pq[1];
+                }
+        }
+        __syncthreads();
+        if (active)
+            for (int j = sub; j < nb; j += 16) {
+                int toff = 0;
+                asm volatile("" : "+s"(toff));
+                cplx qs[NRC];
+#pragma unroll
+                for (int q = 0; q < NRC; ++q)
+                    qs[q] = q < rc ? cplx{sq[2 * (j * NRC + q)], sq[2 * (j * NRC + q) + 1]}
+                                   : cplx{0, 0};
+                const cplx nu = dip ? cplx{snx[j], sny[j]} : cplx{0, 0};
+                p2qbxl_pair<PQ, NRC>(kd, cplx{sx[j], sy[j]} - cc, nu, qs, R, htab + toff, A);
+            }
+    }
+#pragma unroll
+    for (int q = 0; q < NRC; ++q)
+#pragma unroll
+        for (int i = 0; i < 2 * PQ + 1; ++i)
+#pragma unroll
+            for (int m = 8; m >= 1; m >>= 1) {
+                A.v[q][i].x += __shfl_xor(A.v[q][i].x, m, 16);
+                A.v[q][i].y += __shfl_xor(A.v[q][i].y, m, 16);
+            }
+    if (active && sub == 0) qbx_store<PQ, NRC>(kd, qexp, c, A, 0);
+}
+
+template <int NRC>
+__global__ void __launch_bounds__(64) k_direct_p2p(KD kd, int nf, const int *unfilt,
+                                                   const double *tgt, int nt, const double *src,
+                                                   int ns, const double *str, const double *nrm,
+                                                   double *pot)
+{
+    __shared__ double sx[DQ_CH], sy[DQ_CH], snx[DQ_CH], sny[DQ_CH];
+    __shared__ double sq[DQ_CH * 2 * NRC];
+    __shared__ __align__(16) double htab[4 * HANKEL_NSER];
+    const int lane = threadIdx.x;
+    const int f = blockIdx.x * 64 + lane;
+    const bool active = f < nf;
+    const bool dip = kd.dip;
+    const int rc = kd.rc, r0 = kd.r0, nr = kd.nr;
+    int ti = 0;
+    cplx tp = {0, 0};
+    if (active) {
+        ti = unfilt[f];
+        tp = ldpt(tgt, nt, ti);
+    }
+    hankel_table_fill(htab, lane, 64);
+    cplx acc[NRC];
+#pragma unroll
+    for (int q = 0; q < NRC; ++q) acc[q] = {0, 0};
+    for (int b = 0; b < ns; b += DQ_CH) {
+        const int nb = min(DQ_CH, ns - b);
+        __syncthreads();
+        for (int j = lane; j < nb; j += 64) {
+            const int si = b + j;
+            sx[j] = src[si];
+            sy[j] = src[(size_t)ns + si];
+            if (dip) {
+                snx[j] = nrm[si];
+                sny[j] = nrm[(size_t)ns + si];
+            }
+#pragma unroll
+            for (int q = 0; q < NRC; ++q)
+                if (q < rc) {
+                    const double *pq = str + 2 * ((size_t)si * nr + r0 + q);
+                    sq[2 * (j * NRC + q)] = pq[0];
+                    sq[2 * (j * NRC + q) + 1] = pq[1];
+                }
+        }
+        __syncthreads();
+        if (active)
+            for (int j = 0; j < nb; ++j) {
+                int toff = 0;
+                asm volatile("" : "+s"(toff));
+                const cplx nu = dip ? cplx{snx[j], sny[j]} : cplx{0, 0};
+                const cplx g = p2p_value_wave(kd, cplx{sx[j], sy[j]}, nu, tp, htab + toff);
+#pragma unroll
+                for (int q = 0; q < NRC; ++q)
+                    if (q < rc) acc[q] += cplx{sq[2 * (j * NRC + q)], sq[2 * (j * NRC + q) + 1]} * g;
+            }
+    }
+    if (active) {
+#pragma unroll
+        for (int q = 0; q < NRC; ++q)
+            if (q < rc) stc(pot + 2 * ((size_t)ti * nr + r0 + q), acc[q]);
+    }
+}
+
 // M2QBXL over balanced work units (see unit_plan): the unit's TG global
 // centers of one target box take G lanes apiece that split the box's list-3
 // boxes; shuffle reduction, then the expansions are added.
@@ -2386,6 +2526,43 @@ int qbx_eval_qbx_expansions(qbx_ctx *ctx, void *stream, const qbx_tree *T, const
                        G->centers.p, G->radii.p, G->ncenters, qexp, pot)
         FOR_RHS_CHUNKS(kd, NRC_S, L_Q2P);
 #undef L_Q2P
+    }
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
+int qbx_direct_qbx(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_geo *G,
+                   const qbx_kernel_desc *d, const double *str, const double *nrm, double *pot)
+{
+    QBX_CHECK(ctx && T && G && d && str && pot, "qbx_direct_qbx: null argument");
+    QBX_CHECK(d->kernel == QBX_LAPLACE2D || d->kernel == QBX_HELMHOLTZ2D,
+              "qbx_direct_qbx: 2D Laplace or Helmholtz only");
+    KD kd;
+    QBX_TRY(make_kd(d, T, &kd));
+    QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
+    hipStream_t st = S(stream);
+    const int nr = kd.nr;
+    QBX_TRY(zero_fill(pot, 2L * T->ntargets * nr, st));
+    if (G->nfiltered > 0) {
+#define L_DP2P(N)                                                                          \
+    hipLaunchKernelGGL(k_direct_p2p<N>, dim3(cdiv(G->nfiltered, 64)), dim3(64), 0, st, kd, \
+                       G->nfiltered, G->nqbtl_unfiltered.p, T->targets.p, T->ntargets,     \
+                       T->sources.p, T->nsources, str, nrm, pot)
+        FOR_RHS_CHUNKS(kd, NRC_S, L_DP2P);
+#undef L_DP2P
+    }
+    if (G->nglobal > 0) {
+        const long nq = 2L * std::max(G->ncenters, 1) * kd.ncq * nr;
+        if (ctx->ws_qbx.n < nq) QBX_TRY(ctx->ws_qbx.alloc(nq));
+        double *Q = ctx->ws_qbx.p;
+        QBX_TRY(zero_fill(Q, nq, st));
+#define L_DP2Q(N)                                                                          \
+    DISPATCH_PQ_NR(kd.pq, N, k_direct_p2qbxl, dim3(cdiv(G->nglobal, 4)), dim3(64), 0, st,  \
+                   kd, G->nglobal, G->global_qbx_centers.p, G->centers.p, G->radii.p,      \
+                   G->ncenters, T->sources.p, T->nsources, str, nrm, Q)
+        FOR_RHS_CHUNKS(kd, NRC_Q, L_DP2Q);
+#undef L_DP2Q
+        QBX_TRY(qbx_eval_qbx_expansions(ctx, stream, T, G, d, Q, pot));
     }
     QBX_HIP(hipGetLastError());
     return QBX_OK;

@@ -279,6 +279,20 @@ def drive_fmm(wrangler: QBXExpansionWrangler, strengths, normals):
     return w.finalize_potentials(pot)
 
 
+def direct_qbx(wrangler: QBXExpansionWrangler, strengths, normals):
+    """Direct QBX, no FMM (pytential/qbx/__init__.py:866-1068, the evaluation
+    of ``fmm_order=False``): every QBX center's expansion from all sources,
+    every other target summed over all sources.  Returns the potential at the
+    user targets in user order.  2D kernels."""
+    w = wrangler
+    if w.dim != 2:
+        raise NotImplementedError("direct QBX on the GPU covers the 2D kernels")
+    pot = w.full_output_zeros()
+    call("qbx_direct_qbx", w.ctx.h, _stream(), w.tree.h, w.geo.h, ct.byref(w.kd),
+         _p(strengths), _p(normals), _p(pot))
+    return w.finalize_potentials(pot)
+
+
 def drive_fmm_fused(wrangler: QBXExpansionWrangler, strengths, normals, out=None):
     """Same stage sequence from the single ``qbx_drive_fmm`` entry point."""
     w = wrangler

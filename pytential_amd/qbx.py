@@ -57,9 +57,11 @@ class QBXLayerPotentialSource:
         if (fmm_order is not None) == (fmm_level_to_order is not None):
             raise TypeError(
                 "must specify exactly one of 'fmm_order' or 'fmm_level_to_order'.")
-        if fmm_order is False or fmm_level_to_order is False:
-            raise NotImplementedError(
-                "direct (fmm_order=False) evaluation is the CPU oracle's job here")
+        # fmm_order=False: direct QBX, no FMM (qbx/__init__.py:866-1068); the
+        # tree is still built for target association, its orders are unused
+        self._direct = fmm_order is False or fmm_level_to_order is False
+        if self._direct:
+            fmm_order, fmm_level_to_order = None, (lambda kernel, kernel_args, tree, level: 1)
         if not _expansions_in_tree_have_extent:
             raise NotImplementedError("only trees with center extent are supported")
         if fmm_level_to_order is None:
@@ -173,7 +175,9 @@ class QBXLayerPotentialSource:
         qd = self.quad_density(density)
         normals = self.dev["quad_normals"] if source_kind == "D" else None
         strengths, nrm = w.reorder_sources(qd, self.dev["quad_weights"], normals)
-        if fused and not w.using_tsqbx:
+        if self._direct:
+            pot = _fmm.direct_qbx(w, strengths, nrm)
+        elif fused and not w.using_tsqbx:
             pot = _fmm.drive_fmm_fused(w, strengths, nrm)
         else:
             pot = _fmm.drive_fmm(w, strengths, nrm)
