@@ -936,6 +936,144 @@ __global__ void __launch_bounds__(64) k3_p2qbxl(KD kd, TD t, const int *ubox, co
     if (active && sub == 0) qbx_store<PQ, NRC>(kd, qexp, c, A, accumulate);
 }
 
+// Batched P2QBXL (many right-hand sides), a complex contraction per target
+// box: for one group of up to GC_GROUP global centers of a box and MR_RB
+// right-hand sides, the box's list-1 sources are taken SC at a time; phase A
+// builds the basis of every (center, source) pair once into LDS,
+// A[k][row] with row = (center, coefficient), and stages the chunk's
+// strengths B[k][rhs]; phase B accumulates C[row][rhs] += sum_k A[k][row]
+// B[k][rhs] with each thread holding RPT rows x 4 right-hand sides in
+// registers.  The basis is computed once for all right-hand sides instead of
+// once per register chunk of them.
+constexpr int MR_RB = 32;
+
+template <int PQ>
+struct MRShape {
+    static constexpr int NQ = (PQ + 1) * (PQ + 1);
+    static constexpr int R = GC_GROUP * NQ;
+    static constexpr int RPT = (R + 31) / 32;
+    static constexpr int SC = PQ <= 3 ? 32 : 16;
+};
+
+template <int PQ>
+__global__ void __launch_bounds__(256) k3_p2qbxl_mr(KD kd, TD t, const int *ubox,
+                                                    const int *uc0, const int *gcs,
+                                                    const int *gcl, const double *ctr, int nc,
+                                                    const int *l1s, const int *l1l,
+                                                    const double *src, int ns, const double *str,
+                                                    const double *nrm, double *qexp,
+                                                    int accumulate)
+{
+    using SH = MRShape<PQ>;
+    constexpr int NQ = SH::NQ, R = SH::R, RPT = SH::RPT, SC = SH::SC;
+    __shared__ double As[SC * R * 2];
+    __shared__ double Bs[SC * MR_RB * 2];
+    __shared__ int sidx[SC];
+    const int u = blockIdx.x;
+    const int it = ubox[u];
+    const int c0 = uc0[u];
+    const int ncen = min(GC_GROUP, gcs[it + 1] - gcs[it] - c0);
+    const int nr = kd.nr;
+    const int rb0 = blockIdx.y * MR_RB;
+    const int nrb = min(MR_RB, nr - rb0);
+    const int tid = threadIdx.x;
+    const int rt = tid >> 3, ct = tid & 7;
+    const bool dip = kd.dip;
+    cplx C[RPT][4];
+#pragma unroll
+    for (int j = 0; j < RPT; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) C[j][q] = {0, 0};
+
+    auto flush = [&](int n) {
+        __syncthreads();
+        for (int pr = tid; pr < GC_GROUP * n; pr += 256) {
+            const int ci = pr / n, k = pr - ci * n;
+            if (ci >= ncen) continue;
+            const int si = sidx[k];
+            const v3 cc = ld3(ctr, nc, gcl[gcs[it] + c0 + ci]);
+            const v3 d = sub(ld3(src, ns, si), cc);
+            double *a = As + 2 * ((size_t)k * R + ci * NQ);
+            if (!dip) {
+                cplx T[NQ];
+                the_table<PQ>(d, T);
+#pragma unroll
+                for (int i = 0; i < NQ; ++i) stc(a + 2 * i, T[i]);
+            } else {
+                constexpr int P1 = PQ + 1;
+                cplx T[(P1 + 1) * (P1 + 1)];
+                the_table<P1>(d, T);
+                const v3 nu = ld3(nrm, ns, si);
+                const cplx nmn = {nu.x, -nu.y}, npl = {nu.x, nu.y};
+#pragma unroll
+                for (int nn = 0; nn <= PQ; ++nn)
+#pragma unroll
+                    for (int m = -nn; m <= nn; ++m)
+                        stc(a + 2 * IDX(nn, m),
+                            (-nu.z) * T[IDX(nn + 1, m)] +
+                                0.5 * (nmn * T[IDX(nn + 1, m + 1)] - npl * T[IDX(nn + 1, m - 1)]));
+            }
+        }
+        for (int e = tid; e < n * nrb; e += 256) {
+            const int k = e / nrb, r = e - k * nrb;
+            const double *pq = str + 2 * ((size_t)sidx[k] * nr + rb0 + r);
+            Bs[2 * (k * MR_RB + r)] = pq[0];
+            Bs[2 * (k * MR_RB + r) + 1] = pq[1];
+        }
+        __syncthreads();
+        for (int k = 0; k < n; ++k) {
+            cplx b[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                b[q] = {Bs[2 * (k * MR_RB + 4 * ct + q)], Bs[2 * (k * MR_RB + 4 * ct + q) + 1]};
+#pragma unroll
+            for (int j = 0; j < RPT; ++j) {
+                const int row = rt + 32 * j;
+                if (row < R) {
+                    const cplx a = {As[2 * (k * R + row)], As[2 * (k * R + row) + 1]};
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) C[j][q] += a * b[q];
+                }
+            }
+        }
+    };
+
+    int fill = 0;
+    const int e1 = l1s[it + 1];
+    for (int e = l1s[it]; e < e1; ++e) {
+        const int Sb = l1l[e];
+        int s = t.sstart[Sb];
+        const int s1 = s + t.scount[Sb];
+        while (s < s1) {
+            const int take = min(s1 - s, SC - fill);
+            for (int j = tid; j < take; j += 256) sidx[fill + j] = s + j;
+            fill += take;
+            s += take;
+            if (fill == SC) {
+                flush(SC);
+                fill = 0;
+            }
+        }
+    }
+    if (fill > 0) flush(fill);
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+        const int row = rt + 32 * j;
+        if (row >= R) continue;
+        const int ci = row / NQ, i = row - ci * NQ;
+        if (ci >= ncen) continue;
+        const int c = gcl[gcs[it] + c0 + ci];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int r = 4 * ct + q;
+            if (r < nrb) {
+                double *pq = qexp + 2 * (((size_t)c * NQ + i) * nr + rb0 + r);
+                stc(pq, accumulate ? ldc(pq) + C[j][q] : C[j][q]);
+            }
+        }
+    }
+}
+
 constexpr int M2Q_LANES = 16;
 constexpr int M2Q_CENTERS = 256 / M2Q_LANES;
 
@@ -1582,6 +1720,26 @@ int f3_form_global_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T,
     QBX_CHECK(G->trav == R, "P2QBXL: geometry was built for another traversal");
     if (G->nglobal == 0 || U.nunits == 0) return QBX_OK;
     const TD td = make_td(T);
+    if (kd.nr >= 8 && kd.pq <= 5 && G->gc_group_plan.nunits > 0) {
+        const unit_plan &V = G->gc_group_plan;
+        const dim3 grid(V.nunits, cdiv(kd.nr, MR_RB));
+#define L3_MR(PQV)                                                                         \
+    hipLaunchKernelGGL(k3_p2qbxl_mr<PQV>, grid, dim3(256), 0, st, kd, td, V.unit_box.p,    \
+                       V.unit_t0.p, G->gc_by_tbox_starts.p, G->gc_by_tbox.p, G->centers.p, \
+                       G->ncenters, R->l1_starts.p, R->l1_lists.p, T->sources.p,            \
+                       T->nsources, str, nrm, qexp, accumulate)
+        switch (kd.pq) {
+        case 0: L3_MR(0); break;
+        case 1: L3_MR(1); break;
+        case 2: L3_MR(2); break;
+        case 3: L3_MR(3); break;
+        case 4: L3_MR(4); break;
+        default: L3_MR(5); break;
+        }
+#undef L3_MR
+        QBX_HIP(hipGetLastError());
+        return QBX_OK;
+    }
 #define L3_P2Q(N)                                                                          \
     DISPATCH3_PQ_NR(kd.pq, N, k3_p2qbxl, dim3(U.nunits), dim3(64), 0, st, kd, td,          \
                     U.unit_box.p, U.unit_t0.p, U.box_g.p, G->gc_by_tbox_starts.p,           \

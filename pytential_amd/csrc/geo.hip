@@ -164,6 +164,27 @@ static int build_gc_by_tbox(qbx_ctx *ctx, hipStream_t st, const qbx_trav *R, qbx
     }
     if (ntb > 0) QBX_TRY(exclusive_scan(ctx, st, counts.p, G->gc_by_tbox_starts.p, ntb, nullptr));
     QBX_HIP(hipStreamSynchronize(st));
+    // groups of up to GC_GROUP consecutive centers of one target box (the
+    // rows of the batched P2QBXL contraction)
+    std::vector<int> hs(ntb + 1, 0), ub, ut;
+    if (ntb > 0)
+        QBX_HIP(hipMemcpy(hs.data(), G->gc_by_tbox_starts.p, sizeof(int) * (ntb + 1),
+                          hipMemcpyDeviceToHost));
+    for (int it = 0; it < ntb; ++it)
+        for (int c0 = 0; c0 < hs[it + 1] - hs[it]; c0 += GC_GROUP) {
+            ub.push_back(it);
+            ut.push_back(c0);
+        }
+    unit_plan &U = G->gc_group_plan;
+    U.nunits = (int)ub.size();
+    QBX_TRY(U.unit_box.alloc(std::max(U.nunits, 1)));
+    QBX_TRY(U.unit_t0.alloc(std::max(U.nunits, 1)));
+    if (U.nunits > 0) {
+        QBX_HIP(hipMemcpy(U.unit_box.p, ub.data(), sizeof(int) * U.nunits,
+                          hipMemcpyHostToDevice));
+        QBX_HIP(hipMemcpy(U.unit_t0.p, ut.data(), sizeof(int) * U.nunits,
+                          hipMemcpyHostToDevice));
+    }
     return QBX_OK;
 }
 

@@ -80,6 +80,7 @@ constexpr int UNIT_ITER = 128;     // P2P: list-1 sources per lane
 constexpr int M2P_UNIT_ITER = 8;   // M2P: list-3 boxes per lane
 constexpr int P2Q_UNIT_ITER = 32;  // P2QBXL: list-1 sources per lane
 constexpr int M2Q_UNIT_ITER = 2;   // M2QBXL: list-3 boxes per lane
+constexpr int GC_GROUP = 4;        // centers per block of the batched 3D P2QBXL
 
 struct unit_plan {
     int nunits = 0;
@@ -156,6 +157,8 @@ struct qbx_geo {
     // and M2P (list 3) over non-QBX targets, P2QBXL and M2QBXL over centers
     const qbx_trav *trav = nullptr;
     qbx::unit_plan p2p_plan, m2p_plan, p2q_plan, m2q_plan;
+    // groups of qbx::GC_GROUP consecutive global centers of one target box
+    qbx::unit_plan gc_group_plan;
 };
 
 namespace qbx {

@@ -95,3 +95,25 @@ def test_batched_3d_matches_single():
             err = _rel(multi[:, r], _np(one))
             print(f"[batched] 3D {kind} rhs {r}: rel diff {err:.2e}", flush=True)
             assert err < REL_TOL
+
+
+def test_batched_3d_contraction_matches_single():
+    """37 densities (a partial second block of 32 right-hand sides) through
+    the batched P2QBXL contraction, S and D, against one-density runs."""
+    from pytential_amd.qbx import QBXLayerPotentialSource
+    from pytential_amd.surface import icosphere
+    discr = icosphere(2, order=4, fine_order=5)
+    src = QBXLayerPotentialSource(discr, fine_order=5, qbx_order=3, fmm_order=8)
+    rng = np.random.default_rng(9)
+    nrhs = 37
+    dens = rng.normal(size=(discr.ndofs, nrhs))
+    print(f"\n[batched] laplace3d contraction nrhs={nrhs} ndofs={discr.ndofs}", flush=True)
+    for kind in ("S", "D"):
+        multi, = src.exec_compute_potential("laplace", dens, [("self", +1)], source_kind=kind)
+        multi = _np(multi)
+        for r in (0, 18, 31, 32, 36):
+            one, = src.exec_compute_potential("laplace", dens[:, r], [("self", +1)],
+                                              source_kind=kind)
+            err = _rel(multi[:, r], _np(one))
+            print(f"[batched] 3D contraction {kind} rhs {r}: rel diff {err:.2e}", flush=True)
+            assert err < REL_TOL
