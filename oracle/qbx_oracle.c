@@ -505,7 +505,6 @@ int orc_traverse(const orc_tree *t, int n, int targets_have_extent,
 #pragma omp for schedule(dynamic, 16)
         for (int it = 0; it < tb.n; ++it) {
             int B = out->target_boxes[it];
-            int lb = t->box_levels[B];
             /* (a) leaves not smaller than B */
             int anc[64], na = 0;
             for (int a = B; a >= 0; a = t->box_parent_ids[a]) anc[na++] = a;
