@@ -256,9 +256,11 @@ int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
  * the evaluation of fmm_order=False, and the accuracy oracle at full size.
  * Every global QBX center's order-p_q expansion is formed from all sources
  * and evaluated at its targets; every non-QBX target sums all sources.  2D
- * Laplace / Helmholtz; same inputs and output layout as the FMM stages
- * (strengths_tree, normals_tree, pot [ntargets][nrhs] in tree order; finish
- * with qbx_finalize_potentials).  The level orders of kd are not used. */
+ * Laplace / Helmholtz (nrhs right-hand sides) and 3D Laplace / Helmholtz (one
+ * right-hand side; the QBX targets through the target-specific form, which
+ * equals the expansion pipeline); same inputs and output layout as the FMM
+ * stages (strengths_tree, normals_tree, pot [ntargets][nrhs] in tree order;
+ * finish with qbx_finalize_potentials).  The level orders of kd are unused. */
 int qbx_direct_qbx(qbx_ctx *ctx, void *stream, const qbx_tree *tree, const qbx_geo *geo,
                    const qbx_kernel_desc *kd, const double *strengths_tree,
                    const double *normals_tree, double *pot);

@@ -2535,8 +2535,11 @@ int qbx_direct_qbx(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_geo 
                    const qbx_kernel_desc *d, const double *str, const double *nrm, double *pot)
 {
     QBX_CHECK(ctx && T && G && d && str && pot, "qbx_direct_qbx: null argument");
-    QBX_CHECK(d->kernel == QBX_LAPLACE2D || d->kernel == QBX_HELMHOLTZ2D,
-              "qbx_direct_qbx: 2D Laplace or Helmholtz only");
+    if (d->kernel == QBX_LAPLACE3D || d->kernel == QBX_HELMHOLTZ3D) {
+        QBX_CHECK(d->nrhs <= 1, "qbx_direct_qbx: 3D evaluates one density at a time");
+        return direct_qbx3d(ctx, S(stream), T, G, d->kernel, d->source_kind, d->k, d->qbx_order,
+                            str, nrm, pot);
+    }
     KD kd;
     QBX_TRY(make_kd(d, T, &kd));
     QBX_CHECK(!kd.dip || nrm, "double layer needs normals");

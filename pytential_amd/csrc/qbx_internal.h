@@ -183,6 +183,12 @@ int build_unit_plan(hipStream_t st, const qbx_tree *T, const qbx_trav *R, const 
                     const int *starts, const int *lists, bool by_sources, bool centers,
                     int iter, unit_plan &plan);
 
+// direct 3D QBX (tsqbx.hip): target-specific sums over all sources at QBX
+// targets, direct sums elsewhere; pot complex [ntargets] tree order, scaled
+int direct_qbx3d(qbx_ctx *ctx, hipStream_t st, const qbx_tree *T, const qbx_geo *G, int kernel,
+                 int kind, double k, int qbx_order, const double *str, const double *nrm,
+                 double *pot);
+
 // 3D Laplace stages (fmm3d.hip), same contracts as the C-ABI entry points
 long f3_expansion_size(const qbx_kernel_desc *d, const qbx_tree *T);
 int f3_expansion_level_offsets(const qbx_kernel_desc *d, const qbx_tree *T, long *offsets);

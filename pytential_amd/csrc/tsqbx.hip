@@ -315,6 +315,95 @@ static int launch_p(int order, dim3 grid, hipStream_t st, double kre, double kim
     return QBX_OK;
 }
 
+// one virtual list-1 box holding every source, for the direct (no FMM) form:
+// l1s[i] = i, l1l[i] = 0 for each target box, box 0 = sources [0, ns)
+__global__ void k_virtual_lists(int ntb, int ns, int *buf)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= ntb) buf[i] = i;
+    if (i < ntb) buf[ntb + 1 + i] = 0;
+    if (i == 0) {
+        buf[2 * ntb + 1] = 0;
+        buf[2 * ntb + 2] = ns;
+    }
+}
+
+__global__ void k_scale_c(long n, double s, double *a)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 2 * n) a[i] *= s;
+}
+
+// Direct 3D sums for the targets that need no QBX: one thread per target,
+// sources through LDS.  G = e^{ikr}/(4 pi r) (k = 0: Laplace); DIP: its
+// derivative along the source normal, n.(t - s) (1 - ikr) e^{ikr}/(4 pi r^3).
+constexpr int D3_TILE = 256;
+
+template <bool DIP>
+__global__ void __launch_bounds__(256) k3_direct_p2p(double kre, double kim, int nf,
+                                                     const int *unfilt, const double *tgt,
+                                                     int nt, const double *src, int ns,
+                                                     const double *q, const double *nrm,
+                                                     double *pot)
+{
+    __shared__ double sx[D3_TILE], sy[D3_TILE], sz[D3_TILE], sqr[D3_TILE], sqi[D3_TILE];
+    __shared__ double snx[DIP ? D3_TILE : 1], sny[DIP ? D3_TILE : 1], snz[DIP ? D3_TILE : 1];
+    const int f = blockIdx.x * 256 + threadIdx.x;
+    const bool act = f < nf;
+    int ti = 0;
+    double tx = 0, ty = 0, tz = 0;
+    if (act) {
+        ti = unfilt[f];
+        tx = tgt[ti];
+        ty = tgt[(size_t)nt + ti];
+        tz = tgt[2 * (size_t)nt + ti];
+    }
+    const zc ik = {-kim, kre};
+    double ar = 0, ai = 0;
+    for (int b = 0; b < ns; b += D3_TILE) {
+        const int nb = min(D3_TILE, ns - b);
+        __syncthreads();
+        for (int j = threadIdx.x; j < nb; j += 256) {
+            const int s = b + j;
+            sx[j] = src[s];
+            sy[j] = src[(size_t)ns + s];
+            sz[j] = src[2 * (size_t)ns + s];
+            sqr[j] = q[2 * (size_t)s];
+            sqi[j] = q[2 * (size_t)s + 1];
+            if (DIP) {
+                snx[j] = nrm[s];
+                sny[j] = nrm[(size_t)ns + s];
+                snz[j] = nrm[2 * (size_t)ns + s];
+            }
+        }
+        __syncthreads();
+        if (act)
+            for (int j = 0; j < nb; ++j) {
+                const double dx = tx - sx[j], dy = ty - sy[j], dz = tz - sz[j];
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 == 0.0) continue;
+                const double r = sqrt(r2), ir = 1.0 / r;
+                // e^{ikr} = e^{-kim r} (cos kre r + i sin kre r)
+                double sn, cs;
+                sincos(kre * r, &sn, &cs);
+                const double dec = kim == 0.0 ? 1.0 : exp(-kim * r);
+                zc g = {dec * cs * ir, dec * sn * ir};
+                if (DIP) {
+                    const double proj = (dx * snx[j] + dy * sny[j] + dz * snz[j]) * ir * ir;
+                    const zc fac = {1.0 - r * ik.x, -r * ik.y};
+                    g = (proj * fac) * g;
+                }
+                ar += g.x * sqr[j] - g.y * sqi[j];
+                ai += g.x * sqi[j] + g.y * sqr[j];
+            }
+    }
+    if (act) {
+        constexpr double INV4PI_ = 0.07957747154594766788;
+        pot[2 * (size_t)ti] = ar * INV4PI_;
+        pot[2 * (size_t)ti + 1] = ai * INV4PI_;
+    }
+}
+
 }  // namespace qbx
 
 using namespace qbx;
@@ -355,3 +444,61 @@ int qbx_eval_target_specific_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tr
 }
 
 }  // extern "C"
+
+namespace qbx {
+
+// Direct 3D QBX: the target-specific sums over every source at QBX targets
+// (exactly the order-p_q QBX expansion of all sources, evaluated), direct
+// sums at the other targets.  pot complex [ntargets] tree order, scaled
+// (1/(4 pi) included).  kind: QBX_SOURCE_S or QBX_SOURCE_D.
+int direct_qbx3d(qbx_ctx *ctx, hipStream_t st, const qbx_tree *T, const qbx_geo *G, int kernel,
+                 int kind, double k, int qbx_order, const double *str, const double *nrm,
+                 double *pot)
+{
+    QBX_CHECK(T->dim == 3, "direct 3D QBX needs a 3D tree");
+    QBX_CHECK(qbx_order >= 0 && qbx_order <= 12, "direct 3D QBX: order must be in [0, 12]");
+    QBX_CHECK(kind == QBX_SOURCE_S || nrm, "double layer needs normals");
+    QBX_TRY(zero_fill(pot, 2L * T->ntargets, st));
+    const double kk = kernel == QBX_HELMHOLTZ3D ? k : 0.0;
+    if (G->nctt > 0) {
+        // center_to_target_box indexes the target boxes of the geometry's traversal
+        QBX_CHECK(G->trav, "direct 3D QBX: geometry without a traversal");
+        const int ntb = G->trav->ntarget_boxes;
+        void *scr;
+        QBX_TRY(scratch_reserve(ctx, sizeof(int) * (2L * ntb + 3), &scr));
+        int *vl = (int *)scr;
+        hipLaunchKernelGGL(k_virtual_lists, dim3(cdiv(ntb + 1, 256)), dim3(256), 0, st, ntb,
+                           T->nsources, vl);
+        const dim3 grid(cdiv((long)G->nctt * TSQ_LANES, 256));
+        const int op = kind == QBX_SOURCE_S ? 0 : 1;
+#define DQ3_ARGS                                                                          \
+    qbx_order, grid, st, kk, 0.0, G->nctt, G->ctt_lists.p, G->tree_ttc.p,                \
+        G->center_to_target_box.p, G->centers.p, G->ncenters, T->targets.p, T->ntargets, \
+        T->sources.p, T->nsources, vl, vl + ntb + 1, vl + 2 * ntb + 1, vl + 2 * ntb + 2, \
+        str, nrm, pot, nullptr
+        int rc;
+        if (kernel == QBX_LAPLACE3D)
+            rc = op == 0 ? launch_p<0, 0>(DQ3_ARGS) : launch_p<0, 1>(DQ3_ARGS);
+        else
+            rc = op == 0 ? launch_p<1, 0>(DQ3_ARGS) : launch_p<1, 1>(DQ3_ARGS);
+        QBX_TRY(rc);
+#undef DQ3_ARGS
+        constexpr double INV4PI_ = 0.07957747154594766788;
+        hipLaunchKernelGGL(k_scale_c, dim3(cdiv(2L * T->ntargets, 256)), dim3(256), 0, st,
+                           (long)T->ntargets, INV4PI_, pot);
+    }
+    if (G->nfiltered > 0) {
+        if (kind == QBX_SOURCE_S)
+            hipLaunchKernelGGL(k3_direct_p2p<false>, dim3(cdiv(G->nfiltered, 256)), dim3(256), 0,
+                               st, kk, 0.0, G->nfiltered, G->nqbtl_unfiltered.p, T->targets.p,
+                               T->ntargets, T->sources.p, T->nsources, str, nrm, pot);
+        else
+            hipLaunchKernelGGL(k3_direct_p2p<true>, dim3(cdiv(G->nfiltered, 256)), dim3(256), 0,
+                               st, kk, 0.0, G->nfiltered, G->nqbtl_unfiltered.p, T->targets.p,
+                               T->ntargets, T->sources.p, T->nsources, str, nrm, pot);
+    }
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
+}  // namespace qbx

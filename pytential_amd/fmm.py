@@ -283,10 +283,10 @@ def direct_qbx(wrangler: QBXExpansionWrangler, strengths, normals):
     """Direct QBX, no FMM (pytential/qbx/__init__.py:866-1068, the evaluation
     of ``fmm_order=False``): every QBX center's expansion from all sources,
     every other target summed over all sources.  Returns the potential at the
-    user targets in user order.  2D kernels."""
+    user targets in user order.  3D kernels take one density at a time."""
     w = wrangler
-    if w.dim != 2:
-        raise NotImplementedError("direct QBX on the GPU covers the 2D kernels")
+    if w.dim == 3 and w.nrhs != 1:
+        raise NotImplementedError("direct 3D QBX evaluates one density at a time")
     pot = w.full_output_zeros()
     call("qbx_direct_qbx", w.ctx.h, _stream(), w.tree.h, w.geo.h, ct.byref(w.kd),
          _p(strengths), _p(normals), _p(pot))
