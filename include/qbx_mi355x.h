@@ -265,6 +265,22 @@ int qbx_direct_qbx(qbx_ctx *ctx, void *stream, const qbx_tree *tree, const qbx_g
                    const qbx_kernel_desc *kd, const double *strengths_tree,
                    const double *normals_tree, double *pot);
 
+/* ---- point-to-point sums over every source ---------------------------------
+ * Replaces sumpy's P2P as UnregularizedLayerPotentialSource.
+ * exec_compute_potential_insn_direct calls it (pytential/unregularized.py:
+ * 154-198: every target sums every source, no tree): pot[t] = sum_s G(t, s)
+ * q_s, G the kernel of kd (its source derivative along the normal for
+ * QBX_SOURCE_D) in the same units as the FMM stages.  Plain (user) order in
+ * and out: sources [dim][nsources], normals [dim][nsources] (double layer
+ * only), strengths complex [nsources][nrhs], targets [dim][ntargets], pot
+ * complex [ntargets][nrhs] (written, not accumulated).  3D kernels take one
+ * right-hand side.  A coincident source and target contribute zero.  kd's
+ * qbx_order and level orders are unused (level_orders must still name at
+ * least one valid order). */
+int qbx_p2p_all(qbx_ctx *ctx, void *stream, const qbx_kernel_desc *kd, int nsources,
+                const double *sources, const double *normals, const double *strengths,
+                int ntargets, const double *targets, double *pot);
+
 /* ---- target-specific QBX (3D) ---------------------------------------------
  * Replaces eval_target_specific_qbx_locals (pytential/qbx/target_specific/
  * impl.pyx:566-785, called from pytential/qbx/fmmlib.py:587-639): for each

@@ -1596,7 +1596,7 @@ __global__ void __launch_bounds__(64) k_direct_p2p(KD kd, int nf, const int *unf
     int ti = 0;
     cplx tp = {0, 0};
     if (active) {
-        ti = unfilt[f];
+        ti = unfilt ? unfilt[f] : f;
         tp = ldpt(tgt, nt, ti);
     }
     hankel_table_fill(htab, lane, 64);
@@ -2567,6 +2567,41 @@ int qbx_direct_qbx(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_geo 
 #undef L_DP2Q
         QBX_TRY(qbx_eval_qbx_expansions(ctx, stream, T, G, d, Q, pot));
     }
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
+int qbx_p2p_all(qbx_ctx *ctx, void *stream, const qbx_kernel_desc *d, int nsources,
+                const double *sources, const double *normals, const double *str, int ntargets,
+                const double *targets, double *pot)
+{
+    QBX_CHECK(ctx && d && pot, "qbx_p2p_all: null argument");
+    QBX_CHECK(nsources >= 0 && ntargets >= 0, "qbx_p2p_all: negative size");
+    QBX_CHECK(nsources == 0 || (sources && str), "qbx_p2p_all: null sources");
+    QBX_CHECK(ntargets == 0 || targets, "qbx_p2p_all: null targets");
+    hipStream_t st = S(stream);
+    if (d->kernel == QBX_LAPLACE3D || d->kernel == QBX_HELMHOLTZ3D) {
+        QBX_CHECK(d->nrhs <= 1, "qbx_p2p_all: 3D kernels take one density at a time");
+        QBX_CHECK(d->source_kind == QBX_SOURCE_S || d->source_kind == QBX_SOURCE_D,
+                  "unsupported source kind %d", d->source_kind);
+        QBX_CHECK(d->source_kind == QBX_SOURCE_S || nsources == 0 || normals,
+                  "double layer needs normals");
+        if (ntargets == 0) return QBX_OK;
+        return direct_p2p3d(st, d->kernel == QBX_HELMHOLTZ3D ? d->k : 0.0,
+                            d->source_kind == QBX_SOURCE_D, ntargets, nullptr, targets, ntargets,
+                            sources, nsources, str, normals, pot);
+    }
+    KD kd;
+    QBX_TRY(make_kd(d, nullptr, &kd));
+    QBX_CHECK(!kd.dip || nsources == 0 || normals, "double layer needs normals");
+    if (ntargets == 0) return QBX_OK;
+    // every target is written (zero when there are no sources)
+#define L_AP2P(N)                                                                          \
+    hipLaunchKernelGGL(k_direct_p2p<N>, dim3(cdiv(ntargets, 64)), dim3(64), 0, st, kd,     \
+                       ntargets, nullptr, targets, ntargets, sources, nsources, str,      \
+                       normals, pot)
+    FOR_RHS_CHUNKS(kd, NRC_S, L_AP2P);
+#undef L_AP2P
     QBX_HIP(hipGetLastError());
     return QBX_OK;
 }

@@ -185,6 +185,9 @@ int build_unit_plan(hipStream_t st, const qbx_tree *T, const qbx_trav *R, const 
 
 // direct 3D QBX (tsqbx.hip): target-specific sums over all sources at QBX
 // targets, direct sums elsewhere; pot complex [ntargets] tree order, scaled
+int direct_p2p3d(hipStream_t st, double k, bool dip, int nf, const int *unfilt,
+                 const double *tgt, int nt, const double *src, int ns, const double *str,
+                 const double *nrm, double *pot);
 int direct_qbx3d(qbx_ctx *ctx, hipStream_t st, const qbx_tree *T, const qbx_geo *G, int kernel,
                  int kind, double k, int qbx_order, const double *str, const double *nrm,
                  double *pot);

@@ -353,7 +353,7 @@ __global__ void __launch_bounds__(256) k3_direct_p2p(double kre, double kim, int
     int ti = 0;
     double tx = 0, ty = 0, tz = 0;
     if (act) {
-        ti = unfilt[f];
+        ti = unfilt ? unfilt[f] : f;
         tx = tgt[ti];
         ty = tgt[(size_t)nt + ti];
         tz = tgt[2 * (size_t)nt + ti];
@@ -447,6 +447,23 @@ int qbx_eval_target_specific_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tr
 
 namespace qbx {
 
+// Direct 3D point sums: pot[ti] = sum_s G(t_ti, s) q_s for the nf targets
+// ti = unfilt[f] (unfilt null: ti = f), written, not accumulated.
+int direct_p2p3d(hipStream_t st, double k, bool dip, int nf, const int *unfilt,
+                 const double *tgt, int nt, const double *src, int ns, const double *str,
+                 const double *nrm, double *pot)
+{
+    if (nf == 0) return QBX_OK;
+    if (dip)
+        hipLaunchKernelGGL(k3_direct_p2p<true>, dim3(cdiv(nf, 256)), dim3(256), 0, st, k, 0.0, nf,
+                           unfilt, tgt, nt, src, ns, str, nrm, pot);
+    else
+        hipLaunchKernelGGL(k3_direct_p2p<false>, dim3(cdiv(nf, 256)), dim3(256), 0, st, k, 0.0,
+                           nf, unfilt, tgt, nt, src, ns, str, nrm, pot);
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
 // Direct 3D QBX: the target-specific sums over every source at QBX targets
 // (exactly the order-p_q QBX expansion of all sources, evaluated), direct
 // sums at the other targets.  pot complex [ntargets] tree order, scaled
@@ -487,16 +504,10 @@ int direct_qbx3d(qbx_ctx *ctx, hipStream_t st, const qbx_tree *T, const qbx_geo 
         hipLaunchKernelGGL(k_scale_c, dim3(cdiv(2L * T->ntargets, 256)), dim3(256), 0, st,
                            (long)T->ntargets, INV4PI_, pot);
     }
-    if (G->nfiltered > 0) {
-        if (kind == QBX_SOURCE_S)
-            hipLaunchKernelGGL(k3_direct_p2p<false>, dim3(cdiv(G->nfiltered, 256)), dim3(256), 0,
-                               st, kk, 0.0, G->nfiltered, G->nqbtl_unfiltered.p, T->targets.p,
-                               T->ntargets, T->sources.p, T->nsources, str, nrm, pot);
-        else
-            hipLaunchKernelGGL(k3_direct_p2p<true>, dim3(cdiv(G->nfiltered, 256)), dim3(256), 0,
-                               st, kk, 0.0, G->nfiltered, G->nqbtl_unfiltered.p, T->targets.p,
-                               T->ntargets, T->sources.p, T->nsources, str, nrm, pot);
-    }
+    if (G->nfiltered > 0)
+        QBX_TRY(direct_p2p3d(st, kk, kind == QBX_SOURCE_D, G->nfiltered, G->nqbtl_unfiltered.p,
+                             T->targets.p, T->ntargets, T->sources.p, T->nsources, str, nrm,
+                             pot));
     QBX_HIP(hipGetLastError());
     return QBX_OK;
 }
