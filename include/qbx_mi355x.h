@@ -281,6 +281,30 @@ int qbx_p2p_all(qbx_ctx *ctx, void *stream, const qbx_kernel_desc *kd, int nsour
                 const double *sources, const double *normals, const double *strengths,
                 int ntargets, const double *targets, double *pot);
 
+/* ---- dense layer-potential matrices ----------------------------------------
+ * Replaces MatrixBuilder.map_int_g (pytential/symbolic/matrix.py:451-522, the
+ * QBX matrix of sumpy's LayerPotentialMatrixGenerator with the quadrature
+ * weights and the stage-1 -> quad-stage-2 resampling of map_interpolation,
+ * :401-449) and, with centers == NULL, P2PMatrixBuilder (matrix.py:530-600):
+ *   mat[t][e*n1 + j] = sum_i K(t, s_{e,i}) weights[e*nf + i] upsample[i][j]
+ * K(t, s): centers given -- the order-qbx_order QBX expansion of a unit
+ * source at s about target t's center (centers[:, t], radius radii[t]; the
+ * center of the side the caller chose), evaluated at t; 3D through the
+ * target-specific form, which equals it; centers NULL -- the kernel itself
+ * (coincident points zero).  QBX_SOURCE_D: the source-normal derivative.
+ * Units as the FMM stages.  sources / normals [dim][nelements*nf]
+ * (quad-stage-2 nodes, element-major), weights [nelements*nf] (area element
+ * times rule weight), upsample [nf][n1] row-major (NULL: identity, nf == n1,
+ * which gives the matrix over the fine nodes), targets / centers [dim]
+ * [ntargets], mat complex128 rows of leading dimension ld >= nelements*n1.
+ * nf <= 256 and nf*n1 <= 4096.  kd's level orders are unused (2D: they must
+ * still name at least one valid order). */
+int qbx_lpot_matrix(qbx_ctx *ctx, void *stream, const qbx_kernel_desc *kd, int nelements,
+                    int n1, int nf, const double *upsample, const double *sources,
+                    const double *normals, const double *weights, int ntargets,
+                    const double *targets, const double *centers, const double *radii,
+                    long ld, double *mat);
+
 /* ---- target-specific QBX (3D) ---------------------------------------------
  * Replaces eval_target_specific_qbx_locals (pytential/qbx/target_specific/
  * impl.pyx:566-785, called from pytential/qbx/fmmlib.py:587-639): for each

@@ -1641,6 +1641,138 @@ __global__ void __launch_bounds__(64) k_direct_p2p(KD kd, int nf, const int *unf
     }
 }
 
+// ---- dense layer-potential matrices -------------------------------------
+// Local-expansion basis at the target of a row: Laplace B_l = ((t - c)/R)^l,
+// l = 0..PQ; Helmholtz B_{n+PQ} = J_n(k rho) e^{i n theta}, n = -PQ..PQ, with
+// t - c = rho e^{i theta} (the same Miller recurrence as eval_local_n).
+template <int PQ>
+__device__ void local_basis_2d(const KD &kd, cplx c, double R, cplx tp, cplx *B)
+{
+    if (kd.kernel == QBX_LAPLACE2D) {
+        const cplx x = (1.0 / R) * (tp - c);
+        cplx pw = {1, 0};
+        for (int l = 0; l <= PQ; ++l) {
+            B[l] = pw;
+            pw = pw * x;
+        }
+        return;
+    }
+    for (int m = 0; m <= 2 * PQ; ++m) B[m] = {0, 0};
+    const cplx d = tp - c;
+    const double rho = sqrt(abs2(d));
+    const double x = kd.k * rho;
+    if (x < 1e-300) {
+        B[PQ] = {1, 0};
+        return;
+    }
+    const cplx e1 = {d.x / rho, d.y / rho};
+    const cplx em1 = conj(e1);
+    cplx e = cpow_int(e1, PQ);
+    const double norm = miller_j(
+        x, PQ,
+        [&](int n, double f) {
+            B[PQ + n] = f * e;
+            if (n > 0) {
+                cplx fc = f * conj(e);
+                if (n & 1) fc = -fc;
+                B[PQ - n] = fc;
+            }
+            e = e * em1;
+        },
+        [&](double sc) {
+            for (int m = 0; m <= 2 * PQ; ++m) B[m] = sc * B[m];
+        });
+    for (int m = 0; m <= 2 * PQ; ++m) B[m] = norm * B[m];
+}
+
+// The order-PQ QBX expansion of a unit source at w = s - c (normal nu) about
+// the center, evaluated against the row's basis B: one entry of the QBX
+// layer-potential matrix (P2QBXL of one source, then QBXL2P).
+template <int PQ>
+__device__ __forceinline__ cplx qbx_pair_value(const KD &kd, cplx w, cplx nu, double R,
+                                               const double *tab, const cplx *B)
+{
+    QAcc<PQ, 1> A;
+    qacc_zero(A);
+    const cplx one[1] = {{1.0, 0.0}};
+    p2qbxl_pair<PQ, 1>(kd, w, nu, one, R, tab, A);
+    cplx phi = {0, 0};
+    if (kd.kernel == QBX_LAPLACE2D) {
+#pragma unroll
+        for (int l = 0; l <= PQ; ++l) phi += A.v[0][l] * B[l];
+        return {-phi.x * INV2PI, 0.0};
+    }
+#pragma unroll
+    for (int n = 0; n <= 2 * PQ; ++n) phi += A.v[0][n] * B[n];
+    return {-0.25 * phi.y, 0.25 * phi.x};
+}
+
+// One block per target row t:  mat[t][e n1 + j] = sum_i K(t, s_ei) w_ei U[i][j]
+// with K the QBX entry about the row's center (PQ >= 0) or the kernel itself
+// (PQ < 0).  The fine sources of LM_THREADS / nf consecutive elements take one
+// thread each; their weighted values pass through LDS into the per-element
+// contraction with U, whose outputs are consecutive in the row (coalesced).
+template <int PQ>
+__global__ void __launch_bounds__(LM_THREADS)
+k_lpot_matrix_2d(KD kd, int nel, int n1, int nf, const double *U, const double *src,
+                 const double *nrm, const double *w, int nt, const double *tgt,
+                 const double *ctr, const double *rad, long ld, double *mat)
+{
+    __shared__ double kv[2 * LM_THREADS];
+    __shared__ double Us[LM_UMAX];
+    __shared__ __align__(16) double htab[4 * HANKEL_NSER];
+    __shared__ cplx B[PQ >= 0 ? 2 * PQ + 1 : 1];
+    const int t = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int ns = nel * nf;
+    const cplx tp = ldpt(tgt, nt, t);
+    cplx cc = {0, 0};
+    double R = 1.0;
+    if (PQ >= 0) {
+        cc = ldpt(ctr, nt, t);
+        R = rad[t];
+    }
+    for (int i = tid; i < nf * n1; i += LM_THREADS)
+        Us[i] = U ? U[i] : (i / n1 == i % n1 ? 1.0 : 0.0);
+    hankel_table_fill(htab, tid, LM_THREADS);
+    if (PQ >= 0 && tid == 0) local_basis_2d<(PQ >= 0 ? PQ : 0)>(kd, cc, R, tp, B);
+    __syncthreads();
+    const int epb = LM_THREADS / nf;
+    const int le = tid / nf, fi = tid % nf;
+    for (int e0 = 0; e0 < nel; e0 += epb) {
+        const int e = e0 + le;
+        cplx v = {0, 0};
+        if (le < epb && e < nel) {
+            const int s = e * nf + fi;
+            const cplx sp = ldpt(src, ns, s);
+            const cplx nu = kd.dip ? ldpt(nrm, ns, s) : cplx{0, 0};
+            int toff = 0;
+            asm volatile("" : "+s"(toff));
+            if constexpr (PQ < 0)
+                v = p2p_value_wave(kd, sp, nu, tp, htab + toff);
+            else
+                v = qbx_pair_value<PQ>(kd, sp - cc, nu, R, htab + toff, B);
+            v = w[s] * v;
+        }
+        kv[2 * tid] = v.x;
+        kv[2 * tid + 1] = v.y;
+        __syncthreads();
+        const int ne = min(epb, nel - e0);
+        for (int o = tid; o < ne * n1; o += LM_THREADS) {
+            const int le2 = o / n1, j = o % n1;
+            const double *kr = kv + 2 * le2 * nf;
+            cplx acc = {0, 0};
+            for (int ii = 0; ii < nf; ++ii) {
+                const double u = Us[ii * n1 + j];
+                acc.x += u * kr[2 * ii];
+                acc.y += u * kr[2 * ii + 1];
+            }
+            stc(mat + 2 * ((size_t)t * ld + (size_t)(e0 + le2) * n1 + j), acc);
+        }
+        __syncthreads();
+    }
+}
+
 // M2QBXL over balanced work units (see unit_plan): the unit's TG global
 // centers of one target box take G lanes apiece that split the box's list-3
 // boxes; shuffle reduction, then the expansions are added.
@@ -2091,6 +2223,33 @@ static Flat make_flat(const KD &kd, const std::vector<int> &level_starts, bool h
     }
 
 static inline long exp_bytes(const KD &kd) { return 16L * kd.lcoef[kd.nlevels] * kd.nr; }
+
+int lpot_matrix_2d(hipStream_t st, const qbx_kernel_desc *d, int nel, int n1, int nf,
+                   const double *U, const double *src, const double *nrm, const double *w,
+                   int nt, const double *tgt, const double *ctr, const double *rad, long ld,
+                   double *mat)
+{
+    KD kd;
+    QBX_TRY(make_kd(d, nullptr, &kd));
+    QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
+    const dim3 grid(nt), blk(LM_THREADS);
+#define LM_ARGS kd, nel, n1, nf, U, src, nrm, w, nt, tgt, ctr, rad, ld, mat
+    if (!ctr) {
+        hipLaunchKernelGGL(k_lpot_matrix_2d<-1>, grid, blk, 0, st, LM_ARGS);
+    } else {
+        switch (kd.pq) {
+#define LM_CASE(PP) \
+    case PP: hipLaunchKernelGGL(k_lpot_matrix_2d<PP>, grid, blk, 0, st, LM_ARGS); break;
+            LM_CASE(0) LM_CASE(1) LM_CASE(2) LM_CASE(3) LM_CASE(4) LM_CASE(5) LM_CASE(6)
+            LM_CASE(7) LM_CASE(8) LM_CASE(9) LM_CASE(10) LM_CASE(11) LM_CASE(12)
+#undef LM_CASE
+        default: set_error("qbx_order out of range"); return QBX_ERR_INVALID;
+        }
+    }
+#undef LM_ARGS
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
 
 }  // namespace qbx
 
@@ -2604,6 +2763,30 @@ int qbx_p2p_all(qbx_ctx *ctx, void *stream, const qbx_kernel_desc *d, int nsourc
 #undef L_AP2P
     QBX_HIP(hipGetLastError());
     return QBX_OK;
+}
+
+int qbx_lpot_matrix(qbx_ctx *ctx, void *stream, const qbx_kernel_desc *d, int nelements,
+                    int n1, int nf, const double *upsample, const double *sources,
+                    const double *normals, const double *weights, int ntargets,
+                    const double *targets, const double *centers, const double *radii, long ld,
+                    double *mat)
+{
+    QBX_CHECK(ctx && d && mat, "qbx_lpot_matrix: null argument");
+    QBX_CHECK(nelements >= 0 && ntargets >= 0, "qbx_lpot_matrix: negative size");
+    QBX_CHECK(n1 >= 1 && nf >= 1 && nf <= LM_THREADS && (long)nf * n1 <= LM_UMAX,
+              "qbx_lpot_matrix: element sizes n1=%d nf=%d outside the supported range", n1, nf);
+    QBX_CHECK(upsample || nf == n1, "qbx_lpot_matrix: no upsampling matrix needs nf == n1");
+    QBX_CHECK(ld >= (long)nelements * n1, "qbx_lpot_matrix: ld below the column count");
+    QBX_CHECK(nelements == 0 || (sources && weights), "qbx_lpot_matrix: null sources");
+    QBX_CHECK(ntargets == 0 || targets, "qbx_lpot_matrix: null targets");
+    QBX_CHECK(!centers || radii, "qbx_lpot_matrix: centers need radii");
+    if (ntargets == 0 || nelements == 0) return QBX_OK;
+    hipStream_t st = S(stream);
+    if (d->kernel == QBX_LAPLACE3D || d->kernel == QBX_HELMHOLTZ3D)
+        return lpot_matrix_3d(st, d, nelements, n1, nf, upsample, sources, normals, weights,
+                              ntargets, targets, centers, radii, ld, mat);
+    return lpot_matrix_2d(st, d, nelements, n1, nf, upsample, sources, normals, weights,
+                          ntargets, targets, centers, radii, ld, mat);
 }
 
 int qbx_finalize_potentials(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_geo *G,

@@ -185,6 +185,20 @@ int build_unit_plan(hipStream_t st, const qbx_tree *T, const qbx_trav *R, const 
 
 // direct 3D QBX (tsqbx.hip): target-specific sums over all sources at QBX
 // targets, direct sums elsewhere; pot complex [ntargets] tree order, scaled
+// dense layer-potential matrices (fmm2d.hip, tsqbx.hip): one block of
+// LM_THREADS per target row; the upsampling matrix (nf x n1 doubles, at most
+// LM_UMAX) is staged in LDS; nf <= LM_THREADS.  Contract: qbx_lpot_matrix.
+constexpr int LM_THREADS = 256;
+constexpr int LM_UMAX = 4096;
+int lpot_matrix_2d(hipStream_t st, const qbx_kernel_desc *d, int nel, int n1, int nf,
+                   const double *U, const double *src, const double *nrm, const double *w,
+                   int nt, const double *tgt, const double *ctr, const double *rad, long ld,
+                   double *mat);
+int lpot_matrix_3d(hipStream_t st, const qbx_kernel_desc *d, int nel, int n1, int nf,
+                   const double *U, const double *src, const double *nrm, const double *w,
+                   int nt, const double *tgt, const double *ctr, const double *rad, long ld,
+                   double *mat);
+
 int direct_p2p3d(hipStream_t st, double k, bool dip, int nf, const int *unfilt,
                  const double *tgt, int nt, const double *src, int ns, const double *str,
                  const double *nrm, double *pot);

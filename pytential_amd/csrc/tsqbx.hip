@@ -100,6 +100,109 @@ __device__ void sph_j(zc z, zc *j, zc *jd)
     }
 }
 
+// P_n(ct) and P_n'(ct) from the three-term recurrences; n = 0, 1, 2, ... in
+// order, (Pm2, Pm1, dPm2, dPm1) the running state initialised to (1, ct, 0, 1)
+__device__ __forceinline__ void legendre_step(int n, double ct, double &Pm2, double &Pm1,
+                                              double &dPm2, double &dPm1, double &Pn,
+                                              double &dPn)
+{
+    if (n == 0) {
+        Pn = 1.0;
+        dPn = 0.0;
+    } else if (n == 1) {
+        Pn = ct;
+        dPn = 1.0;
+    } else {
+        Pn = ((2.0 * n - 1.0) * ct * Pm1 - (n - 1.0) * Pm2) / n;
+        dPn = ((2.0 * n - 1.0) * (Pm1 + ct * dPm1) - (n - 1.0) * dPm2) / n;
+        Pm2 = Pm1;
+        Pm1 = Pn;
+        dPm2 = dPm1;
+        dPm1 = dPn;
+    }
+}
+
+// h_n(z) for n = 0, 1, 2, ... in order by the forward recurrence
+// h_n = (2n-1)/z h_{n-1} - h_{n-2}; (hm2, hm1) the running state (h0, h1)
+__device__ __forceinline__ zc hankel_step(int n, zc iz, zc h0, zc h1, zc &hm2, zc &hm1)
+{
+    if (n == 0) return h0;
+    if (n == 1) return h1;
+    const zc hn = ((2.0 * n - 1.0) * iz) * hm1 - hm2;
+    hm2 = hm1;
+    hm1 = hn;
+    return hn;
+}
+
+// One source's term of the order-P target-specific expansion about a center,
+// unit strength, fmmlib scaling (4 pi G): tc = t - c (|tc| = rt), sc = s - c
+// (|sc| = rs), ct = cos of the angle between them; jv = j_n(k rt) (Helmholtz).
+//   OP 0 (S):  Laplace sum_n rt^n / rs^{n+1} P_n(ct)
+//              Helmholtz i k sum_n (2n+1) j_n(k rt) h_n(k rs) P_n(ct)
+//   OP 1 (D):  nu . grad_s of the S term
+template <int KERNEL, int OP, int P>
+__device__ __forceinline__ zc tsq_pair_sd(zc k, const double *tc, double rt, const zc *jv,
+                                          const double *sc, double rs, double ct,
+                                          const double *nu)
+{
+    double Pm2 = 1.0, Pm1 = ct, dPm2 = 0.0, dPm1 = 1.0;
+    if (KERNEL == 0) {
+        if (OP == 0) {
+            double r = 1.0 / rs, acc = 0.0;
+#pragma unroll
+            for (int n = 0; n <= P; ++n) {
+                double Pn, dPn;
+                legendre_step(n, ct, Pm2, Pm1, dPm2, dPm1, Pn, dPn);
+                acc += r * Pn;
+                r *= rt / rs;
+            }
+            return {acc, 0.0};
+        }
+        double gv[3] = {0, 0, 0};
+        double Rn = 1.0 / (rs * rs);
+#pragma unroll
+        for (int n = 0; n <= P; ++n) {
+            double Pn, dPn;
+            legendre_step(n, ct, Pm2, Pm1, dPm2, dPm1, Pn, dPn);
+            for (int m = 0; m < 3; ++m)
+                gv[m] += Rn * (-(n + 1.0) * (sc[m] / rs) * Pn +
+                               (tc[m] / rt - ct * sc[m] / rs) * dPn);
+            Rn *= rt / rs;
+        }
+        return {nu[0] * gv[0] + nu[1] * gv[1] + nu[2] * gv[2], 0.0};
+    }
+    // spherical Hankel h_n(k rho_s) by forward recurrence
+    const zc z = rs * k;
+    const zc iz = zinv(z);
+    const zc h0 = zexp_i(z) * zinv(zc{-z.y, z.x});  // e^{iz} / (iz)
+    const zc h1 = h0 * (iz - zc{0, 1});
+    zc hm2 = h0, hm1 = h1;
+    zc acc = {0, 0};
+    zc av[3] = {{0, 0}, {0, 0}, {0, 0}};
+#pragma unroll
+    for (int n = 0; n <= P; ++n) {
+        double Pn, dPn;
+        legendre_step(n, ct, Pm2, Pm1, dPm2, dPm1, Pn, dPn);
+        const zc hprev = n == 1 ? h0 : hm1;
+        const zc hn = hankel_step(n, iz, h0, h1, hm2, hm1);
+        const double w = 2.0 * n + 1.0;
+        if (OP == 0) {
+            acc = acc + (w * Pn) * (jv[n] * hn);
+        } else {
+            // h_n' = h_{n-1} - (n+1)/z h_n,  h_0' = -h_1
+            const zc hd = n == 0 ? zc{-h1.x, -h1.y} : hprev - ((n + 1.0) * iz) * hn;
+            const zc a = (w * Pn) * (k * hd);
+            const zc b = (w * dPn) * hn;
+            for (int m = 0; m < 3; ++m)
+                av[m] = av[m] + (1.0 / rs) * (jv[n] * (sc[m] * a +
+                                               (tc[m] / rt - ct * sc[m] / rs) * b));
+        }
+    }
+    const zc ik = {-k.y, k.x};
+    if (OP == 0) return ik * acc;
+    return ik * (nu[0] * av[0] + nu[1] * av[1] + nu[2] * av[2]);
+}
+
 template <int KERNEL, int OP, int P>
 __global__ void __launch_bounds__(256) k_tsqbx(double kre, double kim, int nctt,
                                                const int *ctt_lists, const int *tree_ttc,
@@ -135,136 +238,53 @@ __global__ void __launch_bounds__(256) k_tsqbx(double kre, double kim, int nctt,
                 const double rs = sqrt(sc[0] * sc[0] + sc[1] * sc[1] + sc[2] * sc[2]);
                 const double ct = (tc[0] * sc[0] + tc[1] * sc[1] + tc[2] * sc[2]) / (rt * rs);
                 const zc qs = {q[2 * s], q[2 * s + 1]};
-                // Legendre recurrence state
-                double Pm2 = 1.0, Pm1 = ct, dPm2 = 0.0, dPm1 = 1.0;
-                if (KERNEL == 0) {
-                    if (OP == 0) {
-                        double r = 1.0 / rs, acc = 0.0;
+                if (OP != 2) {
+                    double nu[3] = {0, 0, 0};
+                    if (OP == 1) {
+                        nu[0] = dip[s];
+                        nu[1] = dip[(size_t)ns + s];
+                        nu[2] = dip[2 * (size_t)ns + s];
+                    }
+                    res = res + tsq_pair_sd<KERNEL, OP, P>(k, tc, rt, jv, sc, rs, ct, nu) * qs;
+                } else if (KERNEL == 0) {
+                    double Pm2 = 1.0, Pm1 = ct, dPm2 = 0.0, dPm1 = 1.0;
+                    double gv[3] = {0, 0, 0};
+                    double Rn = 1.0 / (rs * rs);
 #pragma unroll
-                        for (int n = 0; n <= P; ++n) {
-                            double Pn;
-                            if (n == 0) Pn = 1.0;
-                            else if (n == 1) Pn = ct;
-                            else {
-                                Pn = ((2.0 * n - 1.0) * ct * Pm1 - (n - 1.0) * Pm2) / n;
-                                Pm2 = Pm1;
-                                Pm1 = Pn;
-                            }
-                            acc += r * Pn;
-                            r *= rt / rs;
-                        }
-                        res = res + acc * qs;
-                    } else {
-                        double gv[3] = {0, 0, 0};
-                        double Rn = 1.0 / (rs * rs);
-#pragma unroll
-                        for (int n = 0; n <= P; ++n) {
-                            double Pn, dPn;
-                            if (n == 0) {
-                                Pn = 1.0;
-                                dPn = 0.0;
-                            } else if (n == 1) {
-                                Pn = ct;
-                                dPn = 1.0;
-                            } else {
-                                Pn = ((2.0 * n - 1.0) * ct * Pm1 - (n - 1.0) * Pm2) / n;
-                                dPn = ((2.0 * n - 1.0) * (Pm1 + ct * dPm1) - (n - 1.0) * dPm2) / n;
-                                Pm2 = Pm1;
-                                Pm1 = Pn;
-                                dPm2 = dPm1;
-                                dPm1 = dPn;
-                            }
-                            if (OP == 2) {
-                                if (n >= 1) {
-                                    for (int m = 0; m < 3; ++m)
-                                        gv[m] += Rn * (n * (tc[m] / rt) * Pn +
-                                                       (sc[m] / rs - ct * tc[m] / rt) * dPn);
-                                    Rn *= rt / rs;
-                                }
-                            } else {
-                                for (int m = 0; m < 3; ++m)
-                                    gv[m] += Rn * (-(n + 1.0) * (sc[m] / rs) * Pn +
-                                                   (tc[m] / rt - ct * sc[m] / rs) * dPn);
-                                Rn *= rt / rs;
-                            }
-                        }
-                        if (OP == 2) {
-                            for (int m = 0; m < 3; ++m) g[m] = g[m] + gv[m] * qs;
-                        } else {
-                            const double dv = dip[s] * gv[0] + dip[(size_t)ns + s] * gv[1] +
-                                              dip[2 * (size_t)ns + s] * gv[2];
-                            res = res + dv * qs;
+                    for (int n = 0; n <= P; ++n) {
+                        double Pn, dPn;
+                        legendre_step(n, ct, Pm2, Pm1, dPm2, dPm1, Pn, dPn);
+                        if (n >= 1) {
+                            for (int m = 0; m < 3; ++m)
+                                gv[m] += Rn * (n * (tc[m] / rt) * Pn +
+                                               (sc[m] / rs - ct * tc[m] / rt) * dPn);
+                            Rn *= rt / rs;
                         }
                     }
+                    for (int m = 0; m < 3; ++m) g[m] = g[m] + gv[m] * qs;
                 } else {
                     // spherical Hankel h_n(k rho_s) by forward recurrence
+                    double Pm2 = 1.0, Pm1 = ct, dPm2 = 0.0, dPm1 = 1.0;
                     const zc z = rs * k;
                     const zc iz = zinv(z);
-                    const zc ei = zexp_i(z);
-                    const zc h0 = ei * zinv(zc{-z.y, z.x});  // e^{iz} / (iz)
+                    const zc h0 = zexp_i(z) * zinv(zc{-z.y, z.x});  // e^{iz} / (iz)
                     const zc h1 = h0 * (iz - zc{0, 1});
                     zc hm2 = h0, hm1 = h1;
-                    zc acc = {0, 0};
                     zc av[3] = {{0, 0}, {0, 0}, {0, 0}};
 #pragma unroll
                     for (int n = 0; n <= P; ++n) {
                         double Pn, dPn;
-                        if (n == 0) {
-                            Pn = 1.0;
-                            dPn = 0.0;
-                        } else if (n == 1) {
-                            Pn = ct;
-                            dPn = 1.0;
-                        } else {
-                            Pn = ((2.0 * n - 1.0) * ct * Pm1 - (n - 1.0) * Pm2) / n;
-                            dPn = ((2.0 * n - 1.0) * (Pm1 + ct * dPm1) - (n - 1.0) * dPm2) / n;
-                            Pm2 = Pm1;
-                            Pm1 = Pn;
-                            dPm2 = dPm1;
-                            dPm1 = dPn;
-                        }
-                        zc hn, hprev;
-                        if (n == 0) {
-                            hn = h0;
-                            hprev = {0, 0};
-                        } else if (n == 1) {
-                            hn = h1;
-                            hprev = h0;
-                        } else {
-                            hn = ((2.0 * n - 1.0) * iz) * hm1 - hm2;
-                            hprev = hm1;
-                            hm2 = hm1;
-                            hm1 = hn;
-                        }
+                        legendre_step(n, ct, Pm2, Pm1, dPm2, dPm1, Pn, dPn);
+                        const zc hn = hankel_step(n, iz, h0, h1, hm2, hm1);
                         const double w = 2.0 * n + 1.0;
-                        if (OP == 0) {
-                            acc = acc + (w * Pn) * (jv[n] * hn);
-                        } else if (OP == 2) {
-                            const zc a = (w * Pn) * (k * jdv[n]);
-                            const zc b = (w * dPn) * jv[n];
-                            for (int m = 0; m < 3; ++m)
-                                av[m] = av[m] + (1.0 / rt) * (hn * (tc[m] * a +
-                                                       (sc[m] / rs - ct * tc[m] / rt) * b));
-                        } else {
-                            // h_n' = h_{n-1} - (n+1)/z h_n,  h_0' = -h_1
-                            const zc hd = n == 0 ? zc{-h1.x, -h1.y} : hprev - ((n + 1.0) * iz) * hn;
-                            const zc a = (w * Pn) * (k * hd);
-                            const zc b = (w * dPn) * hn;
-                            for (int m = 0; m < 3; ++m)
-                                av[m] = av[m] + (1.0 / rs) * (jv[n] * (sc[m] * a +
-                                                       (tc[m] / rt - ct * sc[m] / rs) * b));
-                        }
+                        const zc a = (w * Pn) * (k * jdv[n]);
+                        const zc b = (w * dPn) * jv[n];
+                        for (int m = 0; m < 3; ++m)
+                            av[m] = av[m] + (1.0 / rt) * (hn * (tc[m] * a +
+                                                   (sc[m] / rs - ct * tc[m] / rt) * b));
                     }
                     const zc ikq = zc{-k.y, k.x} * qs;  // i k q
-                    if (OP == 0) {
-                        res = res + ikq * acc;
-                    } else if (OP == 2) {
-                        for (int m = 0; m < 3; ++m) g[m] = g[m] + ikq * av[m];
-                    } else {
-                        const zc dv = dip[s] * av[0] + dip[(size_t)ns + s] * av[1] +
-                                      dip[2 * (size_t)ns + s] * av[2];
-                        res = res + ikq * dv;
-                    }
+                    for (int m = 0; m < 3; ++m) g[m] = g[m] + ikq * av[m];
                 }
             }
         }
@@ -311,6 +331,123 @@ static int launch_p(int order, dim3 grid, hipStream_t st, double kre, double kim
         return QBX_ERR_INVALID;
     }
 #undef TSQ_CASE
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
+// One block per target row t of a dense layer-potential matrix:
+//   mat[t][e n1 + j] = sum_i K(t, s_ei) w_ei U[i][j]
+// K the order-P target-specific entry about the row's center (the QBX
+// expansion of a unit source evaluated at t) or, for P < 0, the kernel
+// itself; both with the 1/(4 pi) of G.  Layout and contraction as in the 2D
+// kernel: the fine sources of LM_THREADS / nf elements one per thread, their
+// weighted values through LDS into the per-element contraction with U.
+template <int KERNEL, int OP, int P>
+__global__ void __launch_bounds__(LM_THREADS)
+k_lpot_matrix_3d(double kre, int nel, int n1, int nf, const double *U, const double *src,
+                 const double *nrm, const double *w, int nt, const double *tgt,
+                 const double *ctr, long ld, double *mat)
+{
+    constexpr int PJ = P >= 0 ? P : 0;
+    constexpr double INV4PI_ = 0.07957747154594766788;
+    __shared__ double kv[2 * LM_THREADS];
+    __shared__ double Us[LM_UMAX];
+    const int t = blockIdx.x, tid = threadIdx.x, ns = nel * nf;
+    const zc k = {kre, 0.0};
+    const double tp[3] = {tgt[t], tgt[(size_t)nt + t], tgt[2 * (size_t)nt + t]};
+    double tc[3] = {0, 0, 0}, cv[3] = {0, 0, 0}, rt = 0.0;
+    zc jv[PJ + 1], jdv[PJ + 1];
+    if (P >= 0) {
+        for (int m = 0; m < 3; ++m) {
+            cv[m] = ctr[(size_t)m * nt + t];
+            tc[m] = tp[m] - cv[m];
+        }
+        rt = sqrt(tc[0] * tc[0] + tc[1] * tc[1] + tc[2] * tc[2]);
+        if (KERNEL == 1) sph_j<PJ>(rt * k, jv, jdv);
+    }
+    for (int i = tid; i < nf * n1; i += LM_THREADS)
+        Us[i] = U ? U[i] : (i / n1 == i % n1 ? 1.0 : 0.0);
+    __syncthreads();
+    const int epb = LM_THREADS / nf;
+    const int le = tid / nf, fi = tid % nf;
+    for (int e0 = 0; e0 < nel; e0 += epb) {
+        const int e = e0 + le;
+        zc v = {0, 0};
+        if (le < epb && e < nel) {
+            const int s = e * nf + fi;
+            const double sp[3] = {src[s], src[(size_t)ns + s], src[2 * (size_t)ns + s]};
+            double nu[3] = {0, 0, 0};
+            if (OP == 1)
+                for (int m = 0; m < 3; ++m) nu[m] = nrm[(size_t)m * ns + s];
+            if constexpr (P < 0) {
+                const double d[3] = {tp[0] - sp[0], tp[1] - sp[1], tp[2] - sp[2]};
+                const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                if (r2 > 0.0) {
+                    const double r = sqrt(r2), ir = 1.0 / r;
+                    double sn, cs;
+                    sincos(kre * r, &sn, &cs);
+                    v = {cs * ir, sn * ir};
+                    if (OP == 1) {
+                        const double proj = (d[0] * nu[0] + d[1] * nu[1] + d[2] * nu[2]) * ir * ir;
+                        v = (proj * zc{1.0, -kre * r}) * v;
+                    }
+                }
+            } else {
+                double sc[3];
+                for (int m = 0; m < 3; ++m) sc[m] = sp[m] - cv[m];
+                const double rs = sqrt(sc[0] * sc[0] + sc[1] * sc[1] + sc[2] * sc[2]);
+                const double ct = (tc[0] * sc[0] + tc[1] * sc[1] + tc[2] * sc[2]) / (rt * rs);
+                v = tsq_pair_sd<KERNEL, OP, PJ>(k, tc, rt, jv, sc, rs, ct, nu);
+            }
+            v = (w[s] * INV4PI_) * v;
+        }
+        kv[2 * tid] = v.x;
+        kv[2 * tid + 1] = v.y;
+        __syncthreads();
+        const int ne = min(epb, nel - e0);
+        for (int o = tid; o < ne * n1; o += LM_THREADS) {
+            const int le2 = o / n1, j = o % n1;
+            const double *kr = kv + 2 * le2 * nf;
+            double ax = 0.0, ay = 0.0;
+            for (int ii = 0; ii < nf; ++ii) {
+                const double u = Us[ii * n1 + j];
+                ax += u * kr[2 * ii];
+                ay += u * kr[2 * ii + 1];
+            }
+            const size_t off = 2 * ((size_t)t * ld + (size_t)(e0 + le2) * n1 + j);
+            mat[off] = ax;
+            mat[off + 1] = ay;
+        }
+        __syncthreads();
+    }
+}
+
+template <int KERNEL, int OP>
+static int launch_lpot_matrix_3d(int order, bool qbx, hipStream_t st, double k, int nel, int n1,
+                                 int nf, const double *U, const double *src, const double *nrm,
+                                 const double *w, int nt, const double *tgt, const double *ctr,
+                                 long ld, double *mat)
+{
+    const dim3 grid(nt), blk(LM_THREADS);
+#define LM3_ARGS k, nel, n1, nf, U, src, nrm, w, nt, tgt, ctr, ld, mat
+    if (!qbx) {
+        hipLaunchKernelGGL((k_lpot_matrix_3d<KERNEL, OP, -1>), grid, blk, 0, st, LM3_ARGS);
+    } else {
+        switch (order) {
+#define LM3_CASE(PP)                                                                          \
+    case PP:                                                                                  \
+        hipLaunchKernelGGL((k_lpot_matrix_3d<KERNEL, OP, PP>), grid, blk, 0, st, LM3_ARGS); \
+        break;
+            LM3_CASE(0) LM3_CASE(1) LM3_CASE(2) LM3_CASE(3) LM3_CASE(4) LM3_CASE(5)
+            LM3_CASE(6) LM3_CASE(7) LM3_CASE(8) LM3_CASE(9) LM3_CASE(10) LM3_CASE(11)
+            LM3_CASE(12)
+#undef LM3_CASE
+        default:
+            set_error("qbx_order must be in [0, 12]");
+            return QBX_ERR_INVALID;
+        }
+    }
+#undef LM3_ARGS
     QBX_HIP(hipGetLastError());
     return QBX_OK;
 }
@@ -446,6 +583,26 @@ int qbx_eval_target_specific_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tr
 }  // extern "C"
 
 namespace qbx {
+
+int lpot_matrix_3d(hipStream_t st, const qbx_kernel_desc *d, int nel, int n1, int nf,
+                   const double *U, const double *src, const double *nrm, const double *w,
+                   int nt, const double *tgt, const double *ctr, const double *rad, long ld,
+                   double *mat)
+{
+    (void)rad;  // the target-specific form needs no expansion radius
+    QBX_CHECK(d->source_kind == QBX_SOURCE_S || d->source_kind == QBX_SOURCE_D,
+              "unsupported source kind %d", d->source_kind);
+    const bool dip = d->source_kind == QBX_SOURCE_D;
+    QBX_CHECK(!dip || nrm, "double layer needs normals");
+    const double k = d->kernel == QBX_HELMHOLTZ3D ? d->k : 0.0;
+    const bool qbx = ctr != nullptr;
+#define LM3(KN, OPV) \
+    launch_lpot_matrix_3d<KN, OPV>(d->qbx_order, qbx, st, k, nel, n1, nf, U, src, nrm, w, nt, \
+                                   tgt, ctr, ld, mat)
+    if (d->kernel == QBX_LAPLACE3D) return dip ? LM3(0, 1) : LM3(0, 0);
+    return dip ? LM3(1, 1) : LM3(1, 0);
+#undef LM3
+}
 
 // Direct 3D point sums: pot[ti] = sum_s G(t_ti, s) q_s for the nf targets
 // ti = unfilt[f] (unfilt null: ti = f), written, not accumulated.
