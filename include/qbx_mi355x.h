@@ -281,6 +281,32 @@ int qbx_p2p_all(qbx_ctx *ctx, void *stream, const qbx_kernel_desc *kd, int nsour
                 const double *sources, const double *normals, const double *strengths,
                 int ntargets, const double *targets, double *pot);
 
+/* ---- geometry of the discretization nodes ----------------------------------
+ * Replaces the geometry primitives the QBX source evaluates
+ * (pytential/symbolic/primitives.py:1150-1447: parametrization derivative,
+ * normal, area element, mapping stretch factor, expansion_radii,
+ * expansion_centers, weights_and_area_elements; used by
+ * pytential/qbx/geometry.py:466-499 and qbx/__init__.py:1075-1096), from the
+ * mesh's nodal coordinates el_nodes [dim][nelements*n1] (element-major) and
+ * the element's polynomial interpolant:
+ *   interp [m][n1] (NULL: identity, m == n1) -- values at the m output nodes,
+ *   deriv [dim-1][m][n1] -- reference derivatives at the output nodes,
+ *   ref_weights [m] -- reference quadrature weights (needed for weights),
+ *   stretch_map [(dim-1)^2] row-major -- E with stretch = the largest singular
+ *     value of J E (curves on [-1, 1]: E = 2; triangles in unit coordinates:
+ *     E = 2 x the equilateral-to-unit map),
+ *   radius_factor -- expansion radius = factor x stretch (0.5 curves, 0.25
+ *     surfaces).
+ * Outputs (NULL skips): nodes, normals [dim][nelements*m]; weights
+ * (area element x reference weight), radii [nelements*m]; centers
+ * [dim][2*nelements*m] interleaved (c-, c+) per node.  Curves: normal =
+ * (dy, -dx) / |.|; surfaces: (J_0 x J_1) / |.| (the element's orientation). */
+int qbx_element_geometry(qbx_ctx *ctx, void *stream, int dim, int nelements, int n1,
+                         const double *el_nodes, int m, const double *interp,
+                         const double *deriv, const double *ref_weights,
+                         const double *stretch_map, double radius_factor, double *nodes,
+                         double *normals, double *weights, double *radii, double *centers);
+
 /* ---- dense layer-potential matrices ----------------------------------------
  * Replaces MatrixBuilder.map_int_g (pytential/symbolic/matrix.py:451-522, the
  * QBX matrix of sumpy's LayerPotentialMatrixGenerator with the quadrature
