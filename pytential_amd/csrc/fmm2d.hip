@@ -771,13 +771,29 @@ __global__ void k_m2m(KD kd, TD t, const double *B, int lvp, int i0, int i1, con
     store_add<NRC>(kd, mpole + ecoef_off(kd, P, lvp) + 2L * m * kd.nr, acc);
 }
 
+// M2L with one block per target-or-target-parent box (fl: one entry per box):
+// thread t takes output coefficient j = t % ncl and every S-th list-2 entry
+// from t / ncl (S = M2L_THREADS / ncl groups), so a box's entries are split
+// S ways instead of one thread walking all of them -- the long serial
+// chains of the coarse levels (ncl ~ 60-150 outputs, ~27 entries each) were
+// the tail of the one-thread-per-coefficient kernel.  The groups' partial
+// sums are added in LDS in group order (deterministic), then stored.
+constexpr int M2L_THREADS = 256;
+
 template <int NRC>
-__global__ void k_m2l(KD kd, TD t, const double *B, Flat fl, const int *ttp, const int *starts,
-                      const int *lists, const double *mpole, double *local)
+__global__ void __launch_bounds__(M2L_THREADS)
+k_m2l_blk(KD kd, TD t, const double *B, Flat fl, const int *ttp, const int *starts,
+          const int *lists, const double *mpole, double *local)
 {
-    int lv, i, j;
-    if (!flat_locate(fl, blockIdx.x * (long)blockDim.x + threadIdx.x, lv, i, j)) return;
-    if (starts[i] == starts[i + 1]) return;
+    __shared__ cplx red[M2L_THREADS * NRC];
+    int lv, i, unused;
+    if (!flat_locate(fl, blockIdx.x, lv, i, unused)) return;
+    const int e0 = starts[i], e1 = starts[i + 1];
+    if (e0 == e1) return;
+    const int ncl = kd.ncl[lv];
+    const int S = M2L_THREADS / ncl;
+    const int tid = threadIdx.x;
+    const int g = tid / ncl, j = tid % ncl;
     const int P = ttp[i];
     const int p = kd.order[lv];
     const bool helm = kd.kernel == QBX_HELMHOLTZ2D;
@@ -788,19 +804,33 @@ __global__ void k_m2l(KD kd, TD t, const double *B, Flat fl, const int *ttp, con
     cplx acc[NRC];
 #pragma unroll
     for (int q = 0; q < NRC; ++q) acc[q] = {0, 0};
-    for (int e = starts[i]; e < starts[i + 1]; ++e) {
-        const int A = lists[e];
-        const double *a = mpole + ecoef_off(kd, A, lv);
-        if (!helm) {
-            lap_m2l<NRC>(kd, B, a, p, m, boxc(t, A) - cp, kd.rscale[lv], kd.rscale[lv], acc);
-        } else {
-            const int dx = Ix - t.coords[A], dy = Iy - t.coords[(size_t)t.nboxes + A];
-            const int cls = (dx + kd.m2l_w) * (2 * kd.m2l_w + 1) + (dy + kd.m2l_w);
-            const double *row = kd.tab_m2l + 2 * (kd.m2l_off[lv] + (long)cls * (2 * L + 1) + L);
-            helm_row_dot<NRC>(kd, a, p, m, row, false, acc);
+    if (g < S) {
+        for (int e = e0 + g; e < e1; e += S) {
+            const int A = lists[e];
+            const double *a = mpole + ecoef_off(kd, A, lv);
+            if (!helm) {
+                lap_m2l<NRC>(kd, B, a, p, m, boxc(t, A) - cp, kd.rscale[lv], kd.rscale[lv],
+                             acc);
+            } else {
+                const int dx = Ix - t.coords[A], dy = Iy - t.coords[(size_t)t.nboxes + A];
+                const int cls = (dx + kd.m2l_w) * (2 * kd.m2l_w + 1) + (dy + kd.m2l_w);
+                const double *row =
+                    kd.tab_m2l + 2 * (kd.m2l_off[lv] + (long)cls * (2 * L + 1) + L);
+                helm_row_dot<NRC>(kd, a, p, m, row, false, acc);
+            }
+        }
+        if (g > 0) {
+#pragma unroll
+            for (int q = 0; q < NRC; ++q) red[(tid - ncl) * NRC + q] = acc[q];
         }
     }
-    store_add<NRC>(kd, local + ecoef_off(kd, P, lv) + 2L * m * kd.nr, acc);
+    __syncthreads();
+    if (g == 0) {
+        for (int h = 1; h < S; ++h)
+#pragma unroll
+            for (int q = 0; q < NRC; ++q) acc[q] += red[((h - 1) * ncl + j) * NRC + q];
+        store_add<NRC>(kd, local + ecoef_off(kd, P, lv) + 2L * m * kd.nr, acc);
+    }
 }
 
 template <int NRC>
@@ -844,9 +874,9 @@ __global__ void k_l2l(KD kd, TD t, const double *B, int lv, int i0, int i1, cons
 // reads only LDS (the a_n are broadcast, the rows contiguous across m).  The
 // per-level launches at coarse levels have few boxes and long rows, where
 // the plain one-thread-per-coefficient kernels wait on serial global loads.
-// (M2L keeps the plain flat kernel, one thread per (box, coefficient): a
-// block per box with the list-2 coefficient vectors staged in LDS, and a
-// register-blocked variant over outputs, both measured slower.)
+// (M2L takes a block per box with its list-2 entries split across thread
+// groups, k_m2l_blk; staging the entries' coefficient vectors in LDS and
+// register blocking over outputs both measured slower.)
 // -------------------------------------------------------------------------
 constexpr int HT_MAX_THREADS = 256;
 constexpr long HT_LDS_BUDGET = 32768;  // bytes of LDS per block for staged sources
@@ -2477,11 +2507,19 @@ int qbx_multipole_to_local(qbx_ctx *ctx, void *stream, const qbx_tree *T, const 
     QBX_TRY(zero_fill(local, (exp_bytes(kd)) / 8, st));
     if (R->n2 == 0) return QBX_OK;
     const TD td = make_td(T);
-    const Flat fl = make_flat(kd, R->h_level_start_ttp_box_nrs, false);
-    const long nthr = fl.tstart[fl.nl];
-    if (nthr > 0) {
+    // one block per target-or-target-parent box
+    Flat fl = make_flat(kd, R->h_level_start_ttp_box_nrs, false);
+    long nb = 0;
+    for (int l = 0; l < fl.nl; ++l) {
+        QBX_CHECK(kd.ncl[l] <= M2L_THREADS, "M2L: %d coefficients exceed one block", kd.ncl[l]);
+        fl.tstart[l] = nb;
+        fl.nper[l] = 1;
+        nb += R->h_level_start_ttp_box_nrs[l + 1] - R->h_level_start_ttp_box_nrs[l];
+    }
+    fl.tstart[fl.nl] = nb;
+    if (nb > 0) {
 #define L_M2L(N)                                                                           \
-    hipLaunchKernelGGL(k_m2l<N>, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, td, B, fl,   \
+    hipLaunchKernelGGL(k_m2l_blk<N>, dim3(nb), dim3(M2L_THREADS), 0, st, kd, td, B, fl,    \
                        R->ttp_boxes.p, R->l2_starts.p, R->l2_lists.p, mpole, local)
         FOR_RHS_CHUNKS(kd, NRC_S, L_M2L);
 #undef L_M2L
