@@ -34,6 +34,8 @@
 //  M2QBXL          4 lanes per QBX center over its list-3 boxes
 //  L2QBXL          one thread per QBX center
 //  QBXL2P          one thread per (center, target) pair
+#include <algorithm>
+
 #include "qbx_internal.h"
 #include "qbx_math.h"
 
@@ -2254,6 +2256,69 @@ static Flat make_flat(const KD &kd, const std::vector<int> &level_starts, bool h
 
 static inline long exp_bytes(const KD &kd) { return 16L * kd.lcoef[kd.nlevels] * kd.nr; }
 
+// Helmholtz P2M (LDS kernel) of the source boxes at level l; the levels are
+// independent (each writes its own boxes)
+static void p2m_level_h(hipStream_t st, KD &kd, const TD &td, const qbx_tree *T,
+                        const qbx_trav *R, int l, const double *str, const double *nrm,
+                        double *mpole)
+{
+    const int i0 = R->h_level_start_source_box_nrs[l];
+    const int i1 = R->h_level_start_source_box_nrs[l + 1];
+    if (i1 == i0) return;
+    P2EShape sh;
+    int th;
+    size_t by;
+    p2e_launch(kd, kd.order[l], &sh, &th, &by);
+#define L_P2MH(N)                                                                          \
+    hipLaunchKernelGGL((k_p2e_h<N, false>), dim3(i1 - i0), dim3(th), by, st, kd, td, l, i0, sh, \
+                       R->source_boxes.p, nullptr, nullptr, T->sources.p, T->nsources, str,  \
+                       nrm, mpole)
+    FOR_RHS_CHUNKS(kd, NRC_S, L_P2MH);
+#undef L_P2MH
+}
+
+// P2M with the levels spread over the streams sts[0..nst) (zeroing on
+// sts[0] first, every stream joined back into sts[0] at the end): the
+// per-level launches are small and independent, so they run side by side
+// instead of one after another.  False when the LDS kernel does not apply
+// (the caller then runs qbx_form_multipoles).
+static int p2m_multistream(qbx_ctx *ctx, hipStream_t *sts, int nst, hipEvent_t *evs,
+                           const qbx_tree *T, const qbx_trav *R, const qbx_kernel_desc *d,
+                           const double *str, const double *nrm, double *mpole, bool *done)
+{
+    *done = false;
+    if (!d || d->kernel != QBX_HELMHOLTZ2D) return QBX_OK;
+    KD kd;
+    QBX_TRY(make_kd(d, T, &kd));
+    QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
+    for (int l = 0; l < T->nlevels; ++l) {
+        P2EShape sh;
+        int th;
+        size_t by;
+        if (!p2e_launch(kd, kd.order[l], &sh, &th, &by)) return QBX_OK;
+    }
+    const TD td = make_td(T);
+    QBX_TRY(zero_fill(mpole, (exp_bytes(kd)) / 8, sts[0]));
+    QBX_HIP(hipEventRecord(evs[0], sts[0]));
+    for (int k = 1; k < nst; ++k) QBX_HIP(hipStreamWaitEvent(sts[k], evs[0], 0));
+    // the busiest levels first, each to the next stream
+    std::vector<std::pair<long, int>> order;
+    for (int l = 0; l < T->nlevels; ++l) {
+        const long nb = R->h_level_start_source_box_nrs[l + 1] - R->h_level_start_source_box_nrs[l];
+        if (nb > 0) order.push_back({-nb * (2L * kd.order[l] + 1), l});
+    }
+    std::sort(order.begin(), order.end());
+    for (size_t k = 0; k < order.size(); ++k)
+        p2m_level_h(sts[k % nst], kd, td, T, R, order[k].second, str, nrm, mpole);
+    QBX_HIP(hipGetLastError());
+    for (int k = 1; k < nst; ++k) {
+        QBX_HIP(hipEventRecord(evs[k], sts[k]));
+        QBX_HIP(hipStreamWaitEvent(sts[0], evs[k], 0));
+    }
+    *done = true;
+    return QBX_OK;
+}
+
 int lpot_matrix_2d(hipStream_t st, const qbx_kernel_desc *d, int nel, int n1, int nf,
                    const double *U, const double *src, const double *nrm, const double *w,
                    int nt, const double *tgt, const double *ctr, const double *rad, long ld,
@@ -2335,21 +2400,7 @@ int qbx_form_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx
         }
         if (ok) {
             const TD td = make_td(T);
-            for (int l = 0; l < T->nlevels; ++l) {
-                const int i0 = R->h_level_start_source_box_nrs[l];
-                const int i1 = R->h_level_start_source_box_nrs[l + 1];
-                if (i1 == i0) continue;
-                P2EShape sh;
-                int th;
-                size_t by;
-                p2e_launch(kd, kd.order[l], &sh, &th, &by);
-#define L_P2MH(N)                                                                          \
-    hipLaunchKernelGGL((k_p2e_h<N, false>), dim3(i1 - i0), dim3(th), by, st, kd, td, l, i0, sh, \
-                       R->source_boxes.p, nullptr, nullptr, T->sources.p, T->nsources, str,  \
-                       nrm, mpole)
-                FOR_RHS_CHUNKS(kd, NRC_S, L_P2MH);
-#undef L_P2MH
-            }
+            for (int l = 0; l < T->nlevels; ++l) p2m_level_h(st, kd, td, T, R, l, str, nrm, mpole);
             QBX_HIP(hipGetLastError());
             return QBX_OK;
         }
@@ -2871,13 +2922,20 @@ int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav 
     //         (Q, QBX targets of P)
     // so the latency-bound level-by-level translations overlap the near field;
     // P2M, the head of the chain, runs before the fork without competition
-    // (measured 4% faster than forking first).
+    // (measured 4% faster than forking first), its independent per-level
+    // launches spread over the four streams.
     hipStream_t st = S(stream), s1 = ctx->side[0], s2 = ctx->side[1], s3 = ctx->side[2];
     hipEvent_t e0 = ctx->ev[0], eM = ctx->ev[1], eL = ctx->ev[2], e1 = ctx->ev[3],
                e2 = ctx->ev[4], eP = ctx->ev[5];
     double *L4 = ctx->ws_local2.p;
     QBX_TRY(zero_fill(P, np, st));
-    QBX_TRY(qbx_form_multipoles(ctx, st, T, R, d, str, nrm, M));
+    {
+        hipStream_t sts[4] = {st, s1, s2, s3};
+        hipEvent_t evs[4] = {e0, e1, e2, eP};
+        bool done;
+        QBX_TRY(p2m_multistream(ctx, sts, 4, evs, T, R, d, str, nrm, M, &done));
+        if (!done) QBX_TRY(qbx_form_multipoles(ctx, st, T, R, d, str, nrm, M));
+    }
     QBX_HIP(hipEventRecord(e0, st));
     QBX_HIP(hipStreamWaitEvent(s1, e0, 0));
     QBX_HIP(hipStreamWaitEvent(s2, e0, 0));
