@@ -307,6 +307,37 @@ int qbx_element_geometry(qbx_ctx *ctx, void *stream, int dim, int nelements, int
                          const double *stretch_map, double radius_factor, double *nodes,
                          double *normals, double *weights, double *radii, double *centers);
 
+/* ---- refinement checkers ---------------------------------------------------
+ * Replace RefinerWrangler.check_expansion_disks_undisturbed_by_sources and
+ * check_sufficient_source_quadrature_resolution (pytential/qbx/refinement.py:
+ * 155-281, 363-501), the ball queries of _refine_qbx_stage1 / _stage2
+ * (:667-884).  Every ball is tested against every point (the same answer as
+ * the reference's area query, without a tree).  Flags are only ever set to 1
+ * (clear them first); *found becomes 1 when any flag is set.
+ *
+ * qbx_check_expansion_disks_undisturbed: condition 1.  nodes [dim][nnodes]
+ * (stage 1), element_to_node_starts [nelements+1], centers [dim][2*nnodes]
+ * interleaved (c-, c+) per node, danger_radii [2*nnodes] (the expansion radius
+ * of the center's node); flags the element of every center with a node at
+ * distance <= (1 - tolerance) * danger radius.
+ *
+ * qbx_check_source_quadrature_resolution: condition 2.  sources
+ * [dim][nsources] (stage-2 nodes), element_to_source_starts [nelements+1],
+ * danger_radii_by_element [nelements] (0.75 x the element's largest stage-2
+ * expansion radius), centers [dim][ncenters] (stage 1); flags the element of
+ * every source with a center at distance <= its element's danger radius. */
+int qbx_check_expansion_disks_undisturbed(qbx_ctx *ctx, void *stream, int dim, int nelements,
+                                          const int *element_to_node_starts, int nnodes,
+                                          const double *nodes, const double *centers,
+                                          const double *danger_radii, double tolerance,
+                                          int *refine_flags, int *found);
+int qbx_check_source_quadrature_resolution(qbx_ctx *ctx, void *stream, int dim, int nelements,
+                                           const int *element_to_source_starts, int nsources,
+                                           const double *sources,
+                                           const double *danger_radii_by_element,
+                                           int ncenters, const double *centers,
+                                           int *refine_flags, int *found);
+
 /* ---- dense layer-potential matrices ----------------------------------------
  * Replaces MatrixBuilder.map_int_g (pytential/symbolic/matrix.py:451-522, the
  * QBX matrix of sumpy's LayerPotentialMatrixGenerator with the quadrature

@@ -84,6 +84,8 @@ _SIGS = {
     "qbx_p2p_all": [P, P, ct.POINTER(KernelDesc), I, P, P, P, I, P, P],
     "qbx_lpot_matrix": [P, P, ct.POINTER(KernelDesc), I, I, I, P, P, P, P, I, P, P, P, L, P],
     "qbx_element_geometry": [P, P, I, I, I, P, I, P, P, P, P, D, P, P, P, P, P],
+    "qbx_check_expansion_disks_undisturbed": [P, P, I, I, P, I, P, P, P, D, P, P],
+    "qbx_check_source_quadrature_resolution": [P, P, I, I, P, I, P, P, I, P, P, P],
     "qbx_helmholtz3d_p2p": [P, P, I, P, P, I, P, D, P, P],
     "qbx_helmholtz3d_p2p_adjoint": [P, P, I, P, I, P, D, P, P],
     "qbx_upsample_adjoint": [P, P, I, I, I, I, P, P, P],
