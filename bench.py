@@ -578,6 +578,33 @@ def main_ellipsoid3d(args):
     e_fd_v = abs(an_v - fd_v) / abs(fd_v)
     log(f"[ellipsoid3d] {dt * 1e3:.2f} ms/step, fwd {t_fwd:.2f} ms adj {t_adj:.2f} ms; "
         f"checks: grad vs dense^H {e_grad:.2e}, FD k {e_fd_k:.2e}, FD dir {e_fd_v:.2e}")
+    # the dense check at its stated size: G_k W U of the 1280-triangle
+    # coarsening at all 4096 targets, built on the GPU (qbx_lpot_matrix, no
+    # quad-stage-2 intermediate); its conjugate transpose against the adjoint
+    # kernels' gradient, and sampled rows against the oracle's dense operator
+    from pytential_amd.matrix import build_matrix
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    Ad = build_matrix(srcc, "helmholtz", k0, "S", qbx_forced_limit=None, targets=tg)
+    torch.cuda.synchronize()
+    t_mat = time.perf_counter() - t3
+    s4 = torch.tensor(sc, dtype=torch.complex128, device="cuda", requires_grad=True)
+    phi4 = single_layer_far(srcc, s4, k0, tgd)
+    (g4,) = torch.autograd.grad(phi4, s4, grad_outputs=g)
+    e_dense_adj = float((g4 - Ad.conj().T @ g).abs().max() / (Ad.conj().T @ g).abs().max())
+    phi4 = phi4.detach()
+    e_dense_fwd = float((phi4 - Ad @ s4.detach()).abs().max() / phi4.abs().max())
+    rows = rng.choice(4096, 64, replace=False)
+    Ao = oracle.helmholtz3d_far_operator(dc, tg[:, rows], k0)
+    e_dense_rows = float(np.max(np.abs(Ad[rows].cpu().numpy() - Ao)) / np.max(np.abs(Ao)))
+    dense = {"shape": list(Ad.shape), "gbytes": round(Ad.numel() * 16 / 1e9, 3),
+             "build_ms": round(t_mat * 1e3, 2),
+             "rows_vs_oracle_dense": e_dense_rows, "A_sigma_vs_forward": e_dense_fwd,
+             "AH_g_vs_adjoint_kernels": e_dense_adj}
+    log(f"[ellipsoid3d] dense G_k W U {tuple(Ad.shape)} ({dense['gbytes']} GB) built in "
+        f"{t_mat * 1e3:.1f} ms; rows vs oracle {e_dense_rows:.2e}, A sigma vs forward "
+        f"{e_dense_fwd:.2e}, A^H g vs adjoint {e_dense_adj:.2e}")
+    del Ad
     cpu = None
     if not args.no_cpu:
         # the oracle's dense rows for a sample of targets on the host cores
@@ -609,7 +636,8 @@ def main_ellipsoid3d(args):
         "kernel_ms": {"forward": round(t_fwd, 4), "adjoint": round(t_adj, 4)},
         "checks_1280_triangles": {"grad_sigma_vs_dense_adjoint": e_grad,
                                   "dL_dk_vs_central_diff": e_fd_k,
-                                  "dL_dsigma_dir_vs_central_diff": e_fd_v},
+                                  "dL_dsigma_dir_vs_central_diff": e_fd_v,
+                                  "gpu_dense_matrix_4096_targets": dense},
         "roofline": {"bound": "mfma", "compute": "fp64", "kernel": dom,
                      "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
