@@ -1910,8 +1910,12 @@ __global__ void __launch_bounds__(64) k_m2qbxl(KD kd, TD t, const double *B, con
     if (active && sub == 0) qbx_store<PQ, NRC>(kd, qexp, c, A, 1);
 }
 
+// (64-thread blocks declared: the default bound of 1024 threads capped the
+// kernel at 128 VGPRs, and with 4 right-hand sides it spilled 676 bytes per
+// lane -- 2.4x slower)
 template <int PQ, int NRC>
-__global__ void k_l2qbxl(KD kd, TD t, const double *B, int ng, const int *gcent, const int *c2tb,
+__global__ void __launch_bounds__(64)
+k_l2qbxl(KD kd, TD t, const double *B, int ng, const int *gcent, const int *c2tb,
                          const int *tboxes, const double *ctr, const double *rad, int nc,
                          const double *local, double *qexp, int accumulate)
 {
