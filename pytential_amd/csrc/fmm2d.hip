@@ -835,6 +835,196 @@ k_m2l_blk(KD kd, TD t, const double *B, Flat fl, const int *ttp, const int *star
     }
 }
 
+// ---- M2L on the FP64 matrix cores (Helmholtz, many right-hand sides) ------
+// For a target box P and its list-2 boxes A_e:  L_P[m][r] += sum_e sum_n
+// T_e[m][n] M_Ae[n][r], T_e[m][n] = row_e[n - m] (Toeplitz, from the
+// translation table), r over up to 16 right-hand sides: per entry a complex
+// (ncl x ncl) x (ncl x 16) product, tiled for v_mfma_f64_16x16x4f64 (lane i
+// holds A[i%16][k0 + i/16] and B[k0 + i/16][i%16]; D register j of lane i is
+// D[4j + i/16][i%16]).  The complex product is four real MFMAs (A_im negated
+// for the real part).  One block per box; its four waves take every fourth
+// entry, each stages the entry's row window in LDS (zero outside the row) and
+// reads the multipole coefficients straight from HBM (16 right-hand sides of
+// a coefficient are contiguous); the waves' partial sums are added in LDS in
+// wave order, then stored.  Levels with ncl = 2p + 1 <= M2LM_ROWS * 16.
+typedef double m2l_d4 __attribute__((ext_vector_type(4)));
+constexpr int M2LM_ROWS = 3;                  // row tiles of 16 (ncl <= 48)
+constexpr int M2LM_WIN = M2LM_ROWS * 16 + 48; // staged row window (complex)
+
+__global__ void __launch_bounds__(256)
+k_m2l_mfma(KD kd, TD t, Flat fl, const int *ttp, const int *starts, const int *lists,
+           const double *mpole, double *local)
+{
+    __shared__ double win[4][2 * M2LM_WIN];
+    __shared__ m2l_d4 red[3][M2LM_ROWS][2][64];
+    int lv, i, unused;
+    if (!flat_locate(fl, blockIdx.x, lv, i, unused)) return;
+    const int e0 = starts[i], e1 = starts[i + 1];
+    if (e0 == e1) return;
+    const int p = kd.order[lv];
+    const int ncl = kd.ncl[lv];
+    const int RT = (ncl + 15) / 16, KS = (ncl + 3) / 4;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int P = ttp[i];
+    const int Ix = t.coords[P], Iy = t.coords[(size_t)t.nboxes + P];
+    const int L = 2 * p;
+    const int rc = kd.rc, r0 = kd.r0;
+    const int col = lane & 15, kr = lane >> 4;
+    // window index w <-> row offset d = w - (RT * 16 - 1): n - m in [-(16 RT - 1), 4 KS - 1]
+    const int dlo = -(RT * 16 - 1), nwin = RT * 16 + KS * 4 - 1;
+    m2l_d4 cre[M2LM_ROWS], cim[M2LM_ROWS];
+#pragma unroll
+    for (int rt = 0; rt < M2LM_ROWS; ++rt) cre[rt] = cim[rt] = m2l_d4{0, 0, 0, 0};
+    double *wn = win[wv];
+    for (int e = e0 + wv; e < e1; e += 4) {
+        const int A = lists[e];
+        const int dx = Ix - t.coords[A], dy = Iy - t.coords[(size_t)t.nboxes + A];
+        const int cls = (dx + kd.m2l_w) * (2 * kd.m2l_w + 1) + (dy + kd.m2l_w);
+        const double *row = kd.tab_m2l + 2 * (kd.m2l_off[lv] + (long)cls * (2 * L + 1) + L);
+        const double *a = mpole + ecoef_off(kd, A, lv);
+        for (int w = lane; w < nwin; w += 64) {
+            const int d = w + dlo;
+            const bool in = d >= -L && d <= L;
+            wn[2 * w] = in ? row[2 * d] : 0.0;
+            wn[2 * w + 1] = in ? row[2 * d + 1] : 0.0;
+        }
+        // the window is written and read by this wave only: LDS keeps one
+        // wave's accesses in order; the barrier keeps the compiler's too
+        __builtin_amdgcn_wave_barrier();
+        for (int ks = 0; ks < KS; ++ks) {
+            const int n = ks * 4 + kr;  // 0-based coefficient of the source
+            double bre = 0.0, bim = 0.0;
+            if (n < ncl && col < rc) {
+                const cplx b = ldcn(kd, a, n - p, r0 + col);
+                bre = b.x;
+                bim = b.y;
+            }
+#pragma unroll
+            for (int rt = 0; rt < M2LM_ROWS; ++rt) {
+                if (rt < RT) {
+                    const int m = rt * 16 + col;  // A row of this lane
+                    const int w = n - m - dlo;
+                    const double are = wn[2 * w], aim = wn[2 * w + 1];
+                    cre[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bre, cre[rt], 0, 0, 0);
+                    cre[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(-aim, bim, cre[rt], 0, 0, 0);
+                    cim[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bim, cim[rt], 0, 0, 0);
+                    cim[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, bre, cim[rt], 0, 0, 0);
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (wv > 0) {
+#pragma unroll
+        for (int rt = 0; rt < M2LM_ROWS; ++rt) {
+            red[wv - 1][rt][0][lane] = cre[rt];
+            red[wv - 1][rt][1][lane] = cim[rt];
+        }
+    }
+    __syncthreads();
+    if (wv == 0) {
+        for (int h = 0; h < 3; ++h)
+#pragma unroll
+            for (int rt = 0; rt < M2LM_ROWS; ++rt) {
+                cre[rt] += red[h][rt][0][lane];
+                cim[rt] += red[h][rt][1][lane];
+            }
+        double *o = local + ecoef_off(kd, P, lv);
+#pragma unroll
+        for (int rt = 0; rt < M2LM_ROWS; ++rt)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int m = rt * 16 + 4 * j + kr;  // D row of register j
+                if (rt < RT && m < ncl && col < rc) {
+                    double *q = o + 2 * ((long)(m - p) * kd.nr + r0 + col);
+                    q[0] += cre[rt][j];
+                    q[1] += cim[rt][j];
+                }
+            }
+    }
+}
+
+// The same product for the coarse levels (48 < ncl <= 256): the block walks
+// the entries together, stages each entry's row window in LDS once, and wave
+// w computes the row tiles w, w + 4, ... (at most M2LM_RTW each) over all
+// entries, so no partial sums need combining.
+constexpr int M2LM_RTW = 4;
+constexpr int M2LM_WIN2 = 16 * 4 * M2LM_RTW + 256;
+
+__global__ void __launch_bounds__(256)
+k_m2l_mfma_rows(KD kd, TD t, Flat fl, const int *ttp, const int *starts, const int *lists,
+                const double *mpole, double *local)
+{
+    __shared__ double win[2 * M2LM_WIN2];
+    int lv, i, unused;
+    if (!flat_locate(fl, blockIdx.x, lv, i, unused)) return;
+    const int e0 = starts[i], e1 = starts[i + 1];
+    if (e0 == e1) return;
+    const int p = kd.order[lv];
+    const int ncl = kd.ncl[lv];
+    const int RT = (ncl + 15) / 16, KS = (ncl + 3) / 4;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int P = ttp[i];
+    const int Ix = t.coords[P], Iy = t.coords[(size_t)t.nboxes + P];
+    const int L = 2 * p;
+    const int rc = kd.rc, r0 = kd.r0;
+    const int col = lane & 15, kr = lane >> 4;
+    const int dlo = -(RT * 16 - 1), nwin = RT * 16 + KS * 4 - 1;
+    m2l_d4 cre[M2LM_RTW], cim[M2LM_RTW];
+#pragma unroll
+    for (int q = 0; q < M2LM_RTW; ++q) cre[q] = cim[q] = m2l_d4{0, 0, 0, 0};
+    for (int e = e0; e < e1; ++e) {
+        const int A = lists[e];
+        const int dx = Ix - t.coords[A], dy = Iy - t.coords[(size_t)t.nboxes + A];
+        const int cls = (dx + kd.m2l_w) * (2 * kd.m2l_w + 1) + (dy + kd.m2l_w);
+        const double *row = kd.tab_m2l + 2 * (kd.m2l_off[lv] + (long)cls * (2 * L + 1) + L);
+        const double *a = mpole + ecoef_off(kd, A, lv);
+        __syncthreads();
+        for (int w = threadIdx.x; w < nwin; w += 256) {
+            const int d = w + dlo;
+            const bool in = d >= -L && d <= L;
+            win[2 * w] = in ? row[2 * d] : 0.0;
+            win[2 * w + 1] = in ? row[2 * d + 1] : 0.0;
+        }
+        __syncthreads();
+        for (int ks = 0; ks < KS; ++ks) {
+            const int n = ks * 4 + kr;
+            double bre = 0.0, bim = 0.0;
+            if (n < ncl && col < rc) {
+                const cplx b = ldcn(kd, a, n - p, r0 + col);
+                bre = b.x;
+                bim = b.y;
+            }
+#pragma unroll
+            for (int q = 0; q < M2LM_RTW; ++q) {
+                const int rt = wv + 4 * q;
+                if (rt < RT) {
+                    const int w = n - (rt * 16 + col) - dlo;
+                    const double are = win[2 * w], aim = win[2 * w + 1];
+                    cre[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bre, cre[q], 0, 0, 0);
+                    cre[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-aim, bim, cre[q], 0, 0, 0);
+                    cim[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bim, cim[q], 0, 0, 0);
+                    cim[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, bre, cim[q], 0, 0, 0);
+                }
+            }
+        }
+    }
+    double *o = local + ecoef_off(kd, P, lv);
+#pragma unroll
+    for (int q = 0; q < M2LM_RTW; ++q) {
+        const int rt = wv + 4 * q;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int m = rt * 16 + 4 * j + kr;
+            if (rt < RT && m < ncl && col < rc) {
+                double *dst = o + 2 * ((long)(m - p) * kd.nr + r0 + col);
+                dst[0] += cre[q][j];
+                dst[1] += cim[q][j];
+            }
+        }
+    }
+}
+
 template <int NRC>
 __global__ void k_l2l(KD kd, TD t, const double *B, int lv, int i0, int i1, const int *ttp,
                       double *local)
@@ -2562,16 +2752,48 @@ int qbx_multipole_to_local(qbx_ctx *ctx, void *stream, const qbx_tree *T, const 
     QBX_TRY(zero_fill(local, (exp_bytes(kd)) / 8, st));
     if (R->n2 == 0) return QBX_OK;
     const TD td = make_td(T);
-    // one block per target-or-target-parent box
-    Flat fl = make_flat(kd, R->h_level_start_ttp_box_nrs, false);
-    long nb = 0;
-    for (int l = 0; l < fl.nl; ++l) {
-        QBX_CHECK(kd.ncl[l] <= M2L_THREADS, "M2L: %d coefficients exceed one block", kd.ncl[l]);
-        fl.tstart[l] = nb;
-        fl.nper[l] = 1;
-        nb += R->h_level_start_ttp_box_nrs[l + 1] - R->h_level_start_ttp_box_nrs[l];
+    // many right-hand sides (Helmholtz): on the matrix cores, 16 right-hand
+    // sides per launch -- levels with 2p + 1 <= 48 splitting the entries over
+    // the waves, coarser ones (<= 256) splitting the row tiles
+    const bool mfma = kd.kernel == QBX_HELMHOLTZ2D && kd.nr >= 4;
+    // path of level l: 0 vector kernel, 1 k_m2l_mfma, 2 k_m2l_mfma_rows
+    auto path = [&](int l) {
+        if (!mfma) return 0;
+        if (kd.ncl[l] <= M2LM_ROWS * 16) return 1;
+        return kd.ncl[l] <= 16 * 4 * M2LM_RTW ? 2 : 0;
+    };
+    auto box_flat = [&](int want, long *count) {
+        Flat f = make_flat(kd, R->h_level_start_ttp_box_nrs, false);
+        long n = 0;
+        for (int l = 0; l < f.nl; ++l) {
+            f.tstart[l] = n;
+            f.nper[l] = 1;
+            if (path(l) == want)
+                n += R->h_level_start_ttp_box_nrs[l + 1] - R->h_level_start_ttp_box_nrs[l];
+        }
+        f.tstart[f.nl] = n;
+        *count = n;
+        return f;
+    };
+    if (mfma) {
+        long nb1, nb2;
+        const Flat f1 = box_flat(1, &nb1), f2 = box_flat(2, &nb2);
+        for (int r0 = 0; r0 < kd.nr; r0 += 16) {
+            kd.r0 = r0;
+            kd.rc = std::min(16, kd.nr - r0);
+            if (nb2 > 0)
+                hipLaunchKernelGGL(k_m2l_mfma_rows, dim3(nb2), dim3(256), 0, st, kd, td, f2,
+                                   R->ttp_boxes.p, R->l2_starts.p, R->l2_lists.p, mpole, local);
+            if (nb1 > 0)
+                hipLaunchKernelGGL(k_m2l_mfma, dim3(nb1), dim3(256), 0, st, kd, td, f1,
+                                   R->ttp_boxes.p, R->l2_starts.p, R->l2_lists.p, mpole, local);
+        }
     }
-    fl.tstart[fl.nl] = nb;
+    // one block per target-or-target-parent box
+    for (int l = 0; l < kd.nlevels; ++l)
+        QBX_CHECK(kd.ncl[l] <= M2L_THREADS, "M2L: %d coefficients exceed one block", kd.ncl[l]);
+    long nb;
+    const Flat fl = box_flat(0, &nb);
     if (nb > 0) {
 #define L_M2L(N)                                                                           \
     hipLaunchKernelGGL(k_m2l_blk<N>, dim3(nb), dim3(M2L_THREADS), 0, st, kd, td, B, fl,    \

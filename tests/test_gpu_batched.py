@@ -61,6 +61,34 @@ def test_batched_2d_matches_single(name, nrhs):
             assert err < REL_TOL
 
 
+def test_batched_helmholtz2d_mixed_orders_matches_single():
+    """Helmholtz with 20 densities (right-hand-side chunks of 16 and 4 on the
+    matrix-core M2L) and orders 30 at the coarse levels (2p + 1 > 48: those
+    levels take the vector M2L) and 12 below."""
+    from pytential_amd.qbx import QBXLayerPotentialSource
+    cfg = _cfg("helmholtz2d")
+    nrhs = 20
+
+    def orders(kernel, kernel_args, tree, level):
+        return 30 if level <= 2 else 12
+
+    src = QBXLayerPotentialSource(cfg.discr, fine_order=cfg.discr.fine_order,
+                                  qbx_order=cfg.qbx_order, fmm_level_to_order=orders)
+    rng = np.random.default_rng(5)
+    n = cfg.discr.ndofs
+    dens = rng.normal(size=(n, nrhs)) + 1j * rng.normal(size=(n, nrhs))
+    tg = [(cfg.targets, cfg.target_sides)]
+    multi, = src.exec_compute_potential(cfg.kernel, dens, tg, k=cfg.k,
+                                        source_kind=cfg.source_kind)
+    multi = _np(multi)
+    for r in (0, 15, 16, 19):
+        one, = src.exec_compute_potential(cfg.kernel, dens[:, r], tg, k=cfg.k,
+                                          source_kind=cfg.source_kind)
+        err = _rel(multi[:, r], _np(one))
+        print(f"[batched] mixed orders, 20 densities, rhs {r}: rel diff {err:.2e}", flush=True)
+        assert err < REL_TOL
+
+
 def test_laplace2d_complex_density_splits():
     from pytential_amd.qbx import QBXLayerPotentialSource
     cfg = _cfg("laplace2d")
