@@ -1025,6 +1025,96 @@ k_m2l_mfma_rows(KD kd, TD t, Flat fl, const int *ttp, const int *starts, const i
     }
 }
 
+// Helmholtz M2M (MODE 0: parents at level lv from their children) and L2L
+// (MODE 1: boxes at level lv from their parent) on the matrix cores, up to 16
+// right-hand sides: out[m][r] += sum_n row[n - m] in[n][r] (L2L: times
+// (-1)^(n-m)), the rows from the shift table, n over the input order, m over
+// the output order.  Same tiling as k_m2l_mfma_rows: the block walks the
+// inputs (the nonempty children, or the parent), stages each row window in
+// LDS once, wave w computes the row tiles w, w + 4, ...
+template <int MODE>
+__global__ void __launch_bounds__(256)
+k_shift_mfma(KD kd, TD t, int lv, int i0, const int *boxes, double *expn)
+{
+    __shared__ double win[2 * M2LM_WIN2];
+    const int Bx = boxes[i0 + blockIdx.x];
+    const int lin = MODE == 0 ? lv + 1 : lv - 1;
+    const int pin = kd.order[lin], pout = kd.order[lv];
+    const int nin = 2 * pin + 1, nout = 2 * pout + 1;
+    const int RT = (nout + 15) / 16, KS = (nin + 3) / 4;
+    const int L = pin + pout;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int rc = kd.rc, r0 = kd.r0;
+    const int col = lane & 15, kr = lane >> 4;
+    // window index w <-> d = n - m (0-based n, m): d in [-(16 RT - 1), 4 KS - 1];
+    // the table offset of (n, m) is (n - pin) - (m - pout) = d + pout - pin
+    const int dlo = -(RT * 16 - 1), nwin = RT * 16 + KS * 4 - 1;
+    m2l_d4 cre[M2LM_RTW], cim[M2LM_RTW];
+#pragma unroll
+    for (int q = 0; q < M2LM_RTW; ++q) cre[q] = cim[q] = m2l_d4{0, 0, 0, 0};
+    const int nsrc = MODE == 0 ? 4 : 1;
+    for (int c = 0; c < nsrc; ++c) {
+        int IN, cls;
+        if (MODE == 0) {
+            IN = t.child[(size_t)c * t.nboxes + Bx];
+            if (IN < 0 || t.scumul[IN] == 0) continue;  // uniform over the block
+            cls = c;
+        } else {
+            IN = t.parent[Bx];
+            if (IN < 0) return;
+            cls = (t.coords[Bx] & 1) | ((t.coords[(size_t)t.nboxes + Bx] & 1) << 1);
+        }
+        const double *row =
+            kd.tab_mm + 2 * (kd.mm_off[MODE == 0 ? lin : lv] + (long)cls * (2 * L + 1) + L);
+        const double *a = expn + ecoef_off(kd, IN, lin);
+        __syncthreads();
+        for (int w = threadIdx.x; w < nwin; w += 256) {
+            const int d = w + dlo + pout - pin;
+            const bool in = d >= -L && d <= L;
+            double sg = 1.0;
+            if (MODE == 1 && (d & 1)) sg = -1.0;
+            win[2 * w] = in ? sg * row[2 * d] : 0.0;
+            win[2 * w + 1] = in ? sg * row[2 * d + 1] : 0.0;
+        }
+        __syncthreads();
+        for (int ks = 0; ks < KS; ++ks) {
+            const int n = ks * 4 + kr;
+            double bre = 0.0, bim = 0.0;
+            if (n < nin && col < rc) {
+                const cplx b = ldcn(kd, a, n - pin, r0 + col);
+                bre = b.x;
+                bim = b.y;
+            }
+#pragma unroll
+            for (int q = 0; q < M2LM_RTW; ++q) {
+                const int rt = wv + 4 * q;
+                if (rt < RT) {
+                    const int w = n - (rt * 16 + col) - dlo;
+                    const double are = win[2 * w], aim = win[2 * w + 1];
+                    cre[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bre, cre[q], 0, 0, 0);
+                    cre[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-aim, bim, cre[q], 0, 0, 0);
+                    cim[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bim, cim[q], 0, 0, 0);
+                    cim[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, bre, cim[q], 0, 0, 0);
+                }
+            }
+        }
+    }
+    double *o = expn + ecoef_off(kd, Bx, lv);
+#pragma unroll
+    for (int q = 0; q < M2LM_RTW; ++q) {
+        const int rt = wv + 4 * q;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int m = rt * 16 + 4 * j + kr;
+            if (rt < RT && m < nout && col < rc) {
+                double *dst = o + 2 * ((long)(m - pout) * kd.nr + r0 + col);
+                dst[0] += cre[q][j];
+                dst[1] += cim[q][j];
+            }
+        }
+    }
+}
+
 template <int NRC>
 __global__ void k_l2l(KD kd, TD t, const double *B, int lv, int i0, int i1, const int *ttp,
                       double *local)
@@ -2625,11 +2715,22 @@ int qbx_coarsen_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const 
     hipStream_t st = S(stream);
     QBX_TRY(ensure_tables(ctx, st, T, &kd));
     const TD td = make_td(T);
+    bool mfma = kd.kernel == QBX_HELMHOLTZ2D && kd.nr >= 4;
+    for (int l = 0; l < T->nlevels; ++l) mfma = mfma && 2 * kd.order[l] + 1 <= 16 * 4 * M2LM_RTW;
     for (int l = T->nlevels - 2; l >= 0; --l) {
         const int i0 = R->h_level_start_source_parent_box_nrs[l];
         const int i1 = R->h_level_start_source_parent_box_nrs[l + 1];
         const long nthr = (long)(i1 - i0) * kd.ncl[l];
         if (nthr == 0) continue;
+        if (mfma) {
+            for (int r0 = 0; r0 < kd.nr; r0 += 16) {
+                kd.r0 = r0;
+                kd.rc = std::min(16, kd.nr - r0);
+                hipLaunchKernelGGL(k_shift_mfma<0>, dim3(i1 - i0), dim3(256), 0, st, kd, td, l,
+                                   i0, R->source_parent_boxes.p, mpole);
+            }
+            continue;
+        }
         HtLaunch hl;
         if (ht_launch(kd, kd.order[l], kd.order[l + 1], kd.order[l] + kd.order[l + 1], &hl)) {
 #define L_M2MH(N)                                                                          \
@@ -2873,10 +2974,21 @@ int qbx_refine_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_t
     hipStream_t st = S(stream);
     QBX_TRY(ensure_tables(ctx, st, T, &kd));
     const TD td = make_td(T);
+    bool mfma = kd.kernel == QBX_HELMHOLTZ2D && kd.nr >= 4;
+    for (int l = 0; l < T->nlevels; ++l) mfma = mfma && 2 * kd.order[l] + 1 <= 16 * 4 * M2LM_RTW;
     for (int l = 1; l < T->nlevels; ++l) {
         const int i0 = R->h_level_start_ttp_box_nrs[l], i1 = R->h_level_start_ttp_box_nrs[l + 1];
         const long nthr = (long)(i1 - i0) * kd.ncl[l];
         if (nthr == 0) continue;
+        if (mfma) {
+            for (int r0 = 0; r0 < kd.nr; r0 += 16) {
+                kd.r0 = r0;
+                kd.rc = std::min(16, kd.nr - r0);
+                hipLaunchKernelGGL(k_shift_mfma<1>, dim3(i1 - i0), dim3(256), 0, st, kd, td, l,
+                                   i0, R->ttp_boxes.p, local);
+            }
+            continue;
+        }
         HtLaunch hl;
         if (ht_launch(kd, kd.order[l], kd.order[l - 1], kd.order[l] + kd.order[l - 1], &hl)) {
 #define L_L2LH(N)                                                                          \
