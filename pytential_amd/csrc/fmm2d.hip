@@ -2085,6 +2085,146 @@ k_lpot_matrix_2d(KD kd, int nel, int n1, int nf, const double *U, const double *
     }
 }
 
+// P2QBXL for many right-hand sides (Helmholtz, qbx_order <= 5) on the
+// matrix cores.  A block takes a group of up to GC_GROUP centers of one target
+// box (gc_group_plan) and walks the box's list-1 sources in chunks of PQM_CH:
+// the threads evaluate each (center, source) pair's unit-strength expansion
+// once (p2qbxl_pair) into LDS -- rows (center, coefficient), columns sources:
+// the A operand -- next to the chunk's strengths for up to 16 right-hand sides
+// (the B operand), and waves 0..RT-1 accumulate the complex product of their
+// 16-row tile with v_mfma_f64_16x16x4f64.  The expansion work is done once per
+// pair instead of once per right-hand-side chunk, and the sums over sources
+// become matrix products.  Results are added to qexp (zeroed by the caller).
+constexpr int PQM_CH = 48;
+
+template <int PQ>
+__global__ void __launch_bounds__(256)
+k_p2qbxl_mfma(KD kd, TD t, const int *ubox, const int *ut0, const int *gcs, const int *gcl,
+              const double *ctr, const double *rad, int nc, const int *l1s, const int *l1l,
+              const double *src, int ns, const double *str, const double *nrm, double *qexp)
+{
+    constexpr int NQ = 2 * PQ + 1;
+    constexpr int ROWS = GC_GROUP * NQ;
+    constexpr int RT = (ROWS + 15) / 16;
+    __shared__ double Ab[2][RT * 16][PQM_CH];
+    __shared__ double Sb[2][PQM_CH][16];
+    __shared__ double sx[PQM_CH], sy[PQM_CH], snx[PQM_CH], sny[PQM_CH];
+    __shared__ __align__(16) double htab[4 * HANKEL_NSER];
+    const int u = blockIdx.x, it = ubox[u], c0 = ut0[u];
+    const int ncen = min(GC_GROUP, gcs[it + 1] - gcs[it] - c0);
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int col = lane & 15, kr = lane >> 4;
+    const int rc = kd.rc, r0 = kd.r0, nr = kd.nr;
+    const bool dip = kd.dip;
+    // the pair this thread evaluates: center ci, chunk slot j
+    const int ci = tid / PQM_CH, jslot = tid % PQM_CH;
+    const bool has_c = ci < ncen;
+    cplx cc = {0, 0};
+    double R = 1.0;
+    if (has_c) {
+        const int c = gcl[gcs[it] + c0 + ci];
+        cc = ldpt(ctr, nc, c);
+        R = rad[c];
+    }
+    hankel_table_fill(htab, tid, 256);
+    m2l_d4 cre = {0, 0, 0, 0}, cim = {0, 0, 0, 0};
+
+    auto flush = [&](int n) {
+        __syncthreads();  // sx.. and Sb of the chunk written
+        if (ci < GC_GROUP) {
+            QAcc<PQ, 1> A;
+            qacc_zero<PQ, 1>(A);
+            if (has_c && jslot < n) {
+                int toff = 0;
+                asm volatile("" : "+s"(toff));
+                const cplx one[1] = {{1.0, 0.0}};
+                const cplx nu = dip ? cplx{snx[jslot], sny[jslot]} : cplx{0, 0};
+                p2qbxl_pair<PQ, 1>(kd, cplx{sx[jslot], sy[jslot]} - cc, nu, one, R, htab + toff,
+                                   A);
+            }
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) {
+                Ab[0][ci * NQ + i][jslot] = A.v[0][i].x;
+                Ab[1][ci * NQ + i][jslot] = A.v[0][i].y;
+            }
+        }
+        __syncthreads();  // basis complete
+        if (wv < RT) {
+#pragma unroll 4
+            for (int ks = 0; ks < PQM_CH / 4; ++ks) {
+                const int k = ks * 4 + kr;
+                const double are = Ab[0][wv * 16 + col][k], aim = Ab[1][wv * 16 + col][k];
+                const double bre = Sb[0][k][col], bim = Sb[1][k][col];
+                cre = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bre, cre, 0, 0, 0);
+                cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-aim, bim, cre, 0, 0, 0);
+                cim = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bim, cim, 0, 0, 0);
+                cim = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, bre, cim, 0, 0, 0);
+            }
+        }
+        __syncthreads();  // chunk consumed
+    };
+
+    int fill = 0;
+    const int e1 = l1s[it + 1];
+    for (int e = l1s[it]; e < e1; ++e) {
+        const int Sbx = l1l[e];
+        int s = t.sstart[Sbx];
+        const int s1 = s + t.scount[Sbx];
+        while (s < s1) {
+            const int take = min(s1 - s, PQM_CH - fill);
+            for (int j = tid; j < take; j += 256) {
+                const int si = s + j, d = fill + j;
+                sx[d] = src[si];
+                sy[d] = src[(size_t)ns + si];
+                if (dip) {
+                    snx[d] = nrm[si];
+                    sny[d] = nrm[(size_t)ns + si];
+                }
+            }
+            for (int q = tid; q < take * 16; q += 256) {
+                const int j = q >> 4, r = q & 15;
+                const int si = s + j, d = fill + j;
+                double vr = 0.0, vi = 0.0;
+                if (r < rc) {
+                    const double *pq = str + 2 * ((size_t)si * nr + r0 + r);
+                    vr = pq[0];
+                    vi = pq[1];
+                }
+                Sb[0][d][r] = vr;
+                Sb[1][d][r] = vi;
+            }
+            fill += take;
+            s += take;
+            if (fill == PQM_CH) {
+                flush(PQM_CH);
+                fill = 0;
+            }
+        }
+    }
+    if (fill > 0) {
+        // zero the unused source slots' strengths (their basis is zeroed in flush)
+        for (int q = tid; q < (PQM_CH - fill) * 16; q += 256) {
+            const int d = fill + (q >> 4), r = q & 15;
+            Sb[0][d][r] = 0.0;
+            Sb[1][d][r] = 0.0;
+        }
+        flush(fill);
+    }
+    if (wv < RT) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int row = wv * 16 + 4 * j + kr;
+            const int cj = row / NQ, i = row % NQ;
+            if (cj < ncen && col < rc) {
+                const int c = gcl[gcs[it] + c0 + cj];
+                double *o = qexp + 2 * (((size_t)c * kd.ncq + i) * nr + r0 + col);
+                o[0] += cre[j];
+                o[1] += cim[j];
+            }
+        }
+    }
+}
+
 // M2QBXL over balanced work units (see unit_plan): the unit's TG global
 // centers of one target box take G lanes apiece that split the box's list-3
 // boxes; shuffle reduction, then the expansions are added.
@@ -3025,6 +3165,29 @@ int qbx_form_global_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, co
     QBX_CHECK(G->trav == R, "P2QBXL: geometry was built for another traversal");
     if (G->nglobal == 0 || U.nunits == 0) return QBX_OK;
     const TD td = make_td(T);
+    const unit_plan &V = G->gc_group_plan;
+    if (kd.kernel == QBX_HELMHOLTZ2D && kd.nr >= 4 && kd.pq <= 5 && V.nunits > 0) {
+        for (int r0 = 0; r0 < kd.nr; r0 += 16) {
+            kd.r0 = r0;
+            kd.rc = std::min(16, kd.nr - r0);
+#define L_P2QM(PQV)                                                                        \
+    hipLaunchKernelGGL(k_p2qbxl_mfma<PQV>, dim3(V.nunits), dim3(256), 0, st, kd, td,       \
+                       V.unit_box.p, V.unit_t0.p, G->gc_by_tbox_starts.p, G->gc_by_tbox.p,  \
+                       G->centers.p, G->radii.p, G->ncenters, R->l1_starts.p,               \
+                       R->l1_lists.p, T->sources.p, T->nsources, str, nrm, qexp)
+            switch (kd.pq) {
+            case 0: L_P2QM(0); break;
+            case 1: L_P2QM(1); break;
+            case 2: L_P2QM(2); break;
+            case 3: L_P2QM(3); break;
+            case 4: L_P2QM(4); break;
+            default: L_P2QM(5); break;
+            }
+#undef L_P2QM
+        }
+        QBX_HIP(hipGetLastError());
+        return QBX_OK;
+    }
 #define L_P2Q(N)                                                                           \
     DISPATCH_PQ_NR(kd.pq, N, k_p2qbxl, dim3(U.nunits), dim3(64), 0, st, kd, td,            \
                    U.unit_box.p, U.unit_t0.p, U.box_g.p, G->gc_by_tbox_starts.p,            \
