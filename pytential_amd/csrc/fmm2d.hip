@@ -2225,6 +2225,116 @@ k_p2qbxl_mfma(KD kd, TD t, const int *ubox, const int *ut0, const int *gcs, cons
     }
 }
 
+// M2QBXL for many right-hand sides (Helmholtz, qbx_order <= 5) on the
+// matrix cores: for a group of up to GC_GROUP centers of one target box and
+// each box D of the target box's list 3, Q_c[m][r] += sum_n g_c(n - m)
+// M_D[n][r] with the Graf sequence g_c(d) = H_d(k rho) e^{i d theta} (d >= 0),
+// (-1)^d H_d e^{-i d theta} at -d (rho e^{i theta} = c - c_D; the same terms
+// as k_m2qbxl).  One thread per center runs its Hankel recurrence into LDS,
+// the rows (center, coefficient) form the A operand, the multipole
+// coefficients of up to 16 right-hand sides the B operand.
+constexpr int M2QM_G = 2 * (MAXP + 5) + 1;  // g length bound (p <= MAXP, p_q <= 5)
+
+template <int PQ>
+__global__ void __launch_bounds__(256)
+k_m2qbxl_mfma(KD kd, TD t, const int *ubox, const int *ut0, const int *gcs, const int *gcl,
+              const double *ctr, int nc, const int *l3s, const int *l3l, const double *mpole,
+              double *qexp)
+{
+    constexpr int NQ = 2 * PQ + 1;
+    constexpr int ROWS = GC_GROUP * NQ;
+    constexpr int RT = (ROWS + 15) / 16;
+    __shared__ double gs[GC_GROUP][2 * M2QM_G];
+    const int u = blockIdx.x, it = ubox[u], c0 = ut0[u];
+    const int ncen = min(GC_GROUP, gcs[it + 1] - gcs[it] - c0);
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int col = lane & 15, kr = lane >> 4;
+    const int rc = kd.rc, r0 = kd.r0, nr = kd.nr;
+    // this lane's A row (wave wv < RT)
+    const int row = wv * 16 + col;
+    const int rci = row / NQ, rm = row % NQ - PQ;
+    const bool rvalid = wv < RT && rci < ncen;
+    cplx cc = {0, 0};
+    if (tid < ncen) cc = ldpt(ctr, nc, gcl[gcs[it] + c0 + tid]);
+    m2l_d4 cre = {0, 0, 0, 0}, cim = {0, 0, 0, 0};
+    for (int e = l3s[it]; e < l3s[it + 1]; ++e) {
+        const int D = l3l[e];
+        const int lv = t.level[D];
+        const int p1 = kd.order[lv];
+        const int lmax = p1 + PQ;
+        const int ncl = 2 * p1 + 1;
+        const double *a = mpole + ecoef_off(kd, D, lv);
+        __syncthreads();  // previous g consumed
+        if (tid < ncen) {
+            double *g = gs[tid];
+            const cplx d = cc - boxc(t, D);
+            const double rho = sqrt(abs2(d));
+            const double x = kd.k * rho;
+            const cplx e1 = {d.x / rho, d.y / rho};
+            cplx ep = {1, 0};
+            cplx hm1, hm;
+            hankel01_call(x, hm1, hm);
+            const double tox = 2.0 / x;
+            cplx h = hm1;
+            for (int l = 0; l <= lmax; ++l) {
+                if (l == 1) {
+                    h = hm;
+                } else if (l >= 2) {
+                    h = ((l - 1) * tox) * hm - hm1;
+                    hm1 = hm;
+                    hm = h;
+                }
+                const cplx he = h * ep;
+                cplx hc = h * conj(ep);
+                if (l & 1) hc = -hc;
+                g[2 * (lmax + l)] = he.x;
+                g[2 * (lmax + l) + 1] = he.y;
+                if (l > 0) {
+                    g[2 * (lmax - l)] = hc.x;
+                    g[2 * (lmax - l) + 1] = hc.y;
+                }
+                ep = ep * e1;
+            }
+        }
+        __syncthreads();  // g ready
+        if (wv < RT) {
+            const int KS = (ncl + 3) / 4;
+            for (int ks = 0; ks < KS; ++ks) {
+                const int k = ks * 4 + kr;  // 0-based coefficient n = k - p1
+                double bre = 0.0, bim = 0.0;
+                if (k < ncl && col < rc) {
+                    const cplx b = ldcn(kd, a, k - p1, r0 + col);
+                    bre = b.x;
+                    bim = b.y;
+                }
+                double are = 0.0, aim = 0.0;
+                if (rvalid && k < ncl) {
+                    const int dd = (k - p1) - rm + lmax;  // d = n - m in [0, 2 lmax]
+                    are = gs[rci][2 * dd];
+                    aim = gs[rci][2 * dd + 1];
+                }
+                cre = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bre, cre, 0, 0, 0);
+                cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-aim, bim, cre, 0, 0, 0);
+                cim = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bim, cim, 0, 0, 0);
+                cim = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, bre, cim, 0, 0, 0);
+            }
+        }
+    }
+    if (wv < RT) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int orow = wv * 16 + 4 * j + kr;
+            const int cj = orow / NQ, i = orow % NQ;
+            if (cj < ncen && col < rc) {
+                const int c = gcl[gcs[it] + c0 + cj];
+                double *o = qexp + 2 * (((size_t)c * kd.ncq + i) * nr + r0 + col);
+                o[0] += cre[j];
+                o[1] += cim[j];
+            }
+        }
+    }
+}
+
 // M2QBXL over balanced work units (see unit_plan): the unit's TG global
 // centers of one target box take G lanes apiece that split the box's list-3
 // boxes; shuffle reduction, then the expansions are added.
@@ -3219,6 +3329,31 @@ int qbx_translate_box_multipoles_to_qbx_local(qbx_ctx *ctx, void *stream, const 
     QBX_CHECK(G->trav == R, "M2QBXL: geometry was built for another traversal");
     if (G->nglobal == 0 || R->n3 == 0 || U.nunits == 0) return QBX_OK;
     const TD td = make_td(T);
+    const unit_plan &V = G->gc_group_plan;
+    // (also for one right-hand side: the Graf sequence built once per (center,
+    // box) pair for four centers beats the per-thread form even with 15 of 16
+    // columns idle)
+    if (kd.kernel == QBX_HELMHOLTZ2D && kd.pq <= 5 && V.nunits > 0) {
+        for (int r0 = 0; r0 < kd.nr; r0 += 16) {
+            kd.r0 = r0;
+            kd.rc = std::min(16, kd.nr - r0);
+#define L_M2QM(PQV)                                                                        \
+    hipLaunchKernelGGL(k_m2qbxl_mfma<PQV>, dim3(V.nunits), dim3(256), 0, st, kd, td,       \
+                       V.unit_box.p, V.unit_t0.p, G->gc_by_tbox_starts.p, G->gc_by_tbox.p,  \
+                       G->centers.p, G->ncenters, R->l3_starts.p, R->l3_lists.p, mpole, qexp)
+            switch (kd.pq) {
+            case 0: L_M2QM(0); break;
+            case 1: L_M2QM(1); break;
+            case 2: L_M2QM(2); break;
+            case 3: L_M2QM(3); break;
+            case 4: L_M2QM(4); break;
+            default: L_M2QM(5); break;
+            }
+#undef L_M2QM
+        }
+        QBX_HIP(hipGetLastError());
+        return QBX_OK;
+    }
 #define L_M2Q(N)                                                                           \
     DISPATCH_PQ_NR(kd.pq, N, k_m2qbxl, dim3(U.nunits), dim3(64), 0, st, kd, td, B,         \
                    U.unit_box.p, U.unit_t0.p, U.box_g.p, G->gc_by_tbox_starts.p,            \
