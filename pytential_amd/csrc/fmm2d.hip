@@ -2335,6 +2335,109 @@ k_m2qbxl_mfma(KD kd, TD t, const int *ubox, const int *ut0, const int *gcs, cons
     }
 }
 
+// L2QBXL for many right-hand sides (Helmholtz, qbx_order <= 5) on the
+// matrix cores: the centers of a group share their target box's local
+// expansion L_B; Q_c[m][r] += sum_n g_c(n - m) L_B[n][r] with g_c(d) =
+// J_d(k rho) e^{i d theta} (d >= 0), (-1)^d J_d e^{-i d theta} at -d
+// (rho e^{i theta} = c - c_B; Miller's recurrence, as k_l2qbxl).  One thread
+// per center writes its normalised sequence into LDS (the A operand rows).
+template <int PQ>
+__global__ void __launch_bounds__(256)
+k_l2qbxl_mfma(KD kd, TD t, const int *ubox, const int *ut0, const int *gcs, const int *gcl,
+              const int *tboxes, const double *ctr, int nc, const double *local, double *qexp)
+{
+    constexpr int NQ = 2 * PQ + 1;
+    constexpr int ROWS = GC_GROUP * NQ;
+    constexpr int RT = (ROWS + 15) / 16;
+    __shared__ double gs[GC_GROUP][2 * M2QM_G];
+    const int u = blockIdx.x, it = ubox[u], c0 = ut0[u];
+    const int ncen = min(GC_GROUP, gcs[it + 1] - gcs[it] - c0);
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int col = lane & 15, kr = lane >> 4;
+    const int rc = kd.rc, r0 = kd.r0, nr = kd.nr;
+    const int Bx = tboxes[it];
+    const int lv = t.level[Bx];
+    const int p1 = kd.order[lv];
+    const int lmax = p1 + PQ;
+    const int ncl = 2 * p1 + 1;
+    const double *a = local + ecoef_off(kd, Bx, lv);
+    if (tid < ncen) {
+        double *g = gs[tid];
+        for (int w = 0; w <= 2 * lmax; ++w) g[2 * w] = g[2 * w + 1] = 0.0;
+        const cplx d = ldpt(ctr, nc, gcl[gcs[it] + c0 + tid]) - boxc(t, Bx);
+        const double rho = sqrt(abs2(d));
+        const double x = kd.k * rho;
+        if (x < 1e-300) {
+            g[2 * lmax] = 1.0;
+        } else {
+            const cplx e1 = {d.x / rho, d.y / rho};
+            const cplx em1 = conj(e1);
+            cplx e = cpow_int(e1, lmax);
+            const double norm = miller_j(
+                x, lmax,
+                [&](int l, double f) {
+                    const cplx fe = f * e;
+                    cplx fc = f * conj(e);
+                    if (l & 1) fc = -fc;
+                    g[2 * (lmax + l)] = fe.x;
+                    g[2 * (lmax + l) + 1] = fe.y;
+                    if (l > 0) {
+                        g[2 * (lmax - l)] = fc.x;
+                        g[2 * (lmax - l) + 1] = fc.y;
+                    }
+                    e = e * em1;
+                },
+                [&](double sc) {
+                    for (int w = 0; w <= 2 * lmax; ++w) {
+                        g[2 * w] *= sc;
+                        g[2 * w + 1] *= sc;
+                    }
+                });
+            for (int w = 0; w <= 2 * lmax; ++w) {
+                g[2 * w] *= norm;
+                g[2 * w + 1] *= norm;
+            }
+        }
+    }
+    __syncthreads();
+    if (wv >= RT) return;
+    const int row = wv * 16 + col;
+    const int rci = row / NQ, rm = row % NQ - PQ;
+    const bool rvalid = rci < ncen;
+    m2l_d4 cre = {0, 0, 0, 0}, cim = {0, 0, 0, 0};
+    const int KS = (ncl + 3) / 4;
+    for (int ks = 0; ks < KS; ++ks) {
+        const int k = ks * 4 + kr;
+        double bre = 0.0, bim = 0.0;
+        if (k < ncl && col < rc) {
+            const cplx b = ldcn(kd, a, k - p1, r0 + col);
+            bre = b.x;
+            bim = b.y;
+        }
+        double are = 0.0, aim = 0.0;
+        if (rvalid && k < ncl) {
+            const int dd = (k - p1) - rm + lmax;
+            are = gs[rci][2 * dd];
+            aim = gs[rci][2 * dd + 1];
+        }
+        cre = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bre, cre, 0, 0, 0);
+        cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-aim, bim, cre, 0, 0, 0);
+        cim = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bim, cim, 0, 0, 0);
+        cim = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, bre, cim, 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int orow = wv * 16 + 4 * j + kr;
+        const int cj = orow / NQ, i = orow % NQ;
+        if (cj < ncen && col < rc) {
+            const int c = gcl[gcs[it] + c0 + cj];
+            double *o = qexp + 2 * (((size_t)c * kd.ncq + i) * nr + r0 + col);
+            o[0] += cre[j];
+            o[1] += cim[j];
+        }
+    }
+}
+
 // M2QBXL over balanced work units (see unit_plan): the unit's TG global
 // centers of one target box take G lanes apiece that split the box's list-3
 // boxes; shuffle reduction, then the expansions are added.
@@ -3384,6 +3487,28 @@ int qbx_translate_box_local_to_qbx_local(qbx_ctx *ctx, void *stream, const qbx_t
     const int ng = G->nglobal;
     if (ng == 0) return QBX_OK;
     const TD td = make_td(T);
+    const unit_plan &V = G->gc_group_plan;
+    if (kd.kernel == QBX_HELMHOLTZ2D && kd.nr >= 4 && kd.pq <= 5 && V.nunits > 0) {
+        for (int r0 = 0; r0 < kd.nr; r0 += 16) {
+            kd.r0 = r0;
+            kd.rc = std::min(16, kd.nr - r0);
+#define L_L2QM(PQV)                                                                        \
+    hipLaunchKernelGGL(k_l2qbxl_mfma<PQV>, dim3(V.nunits), dim3(256), 0, st, kd, td,       \
+                       V.unit_box.p, V.unit_t0.p, G->gc_by_tbox_starts.p, G->gc_by_tbox.p,  \
+                       R->target_boxes.p, G->centers.p, G->ncenters, local, qexp)
+            switch (kd.pq) {
+            case 0: L_L2QM(0); break;
+            case 1: L_L2QM(1); break;
+            case 2: L_L2QM(2); break;
+            case 3: L_L2QM(3); break;
+            case 4: L_L2QM(4); break;
+            default: L_L2QM(5); break;
+            }
+#undef L_L2QM
+        }
+        QBX_HIP(hipGetLastError());
+        return QBX_OK;
+    }
 #define L_L2Q(N)                                                                           \
     DISPATCH_PQ_NR(kd.pq, N, k_l2qbxl, dim3(cdiv(ng, 64)), dim3(64), 0, st, kd, td, B, ng,  \
                    G->global_qbx_centers.p, G->center_to_target_box.p, R->target_boxes.p,  \
