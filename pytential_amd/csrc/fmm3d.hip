@@ -705,6 +705,125 @@ __global__ void __launch_bounds__(M2L_THREADS) k3_m2l(KD kd, TD t, const int *tt
     }
 }
 
+// M2L for many right-hand sides on the FP64 matrix cores: per target box
+// out(k,l)[r] = sum_(n,m) T(n+k, m+l) h^-n a(n,m)[r] (the sum of k3_m2l), a
+// complex product of the (k,l) x (n,m) operator -- entries gathered from the
+// entry's unit-offset table, staged in LDS -- with the multipole coefficients
+// of up to 16 right-hand sides, on v_mfma_f64_16x16x4f64 (lane i holds
+// A[i%16][k0 + i/16] and B[k0 + i/16][i%16]; D register j of lane i is
+// D[4j + i/16][i%16]).  Wave w takes the 16-row tiles w, w + 4, ... over all
+// entries; the epilogue scales by (-1)^k h^(-k-1) as k3_m2l.
+typedef double f3_d4 __attribute__((ext_vector_type(4)));
+constexpr int M2L3M_RTW = 4;             // row tiles per wave: (p + 1)^2 <= 256
+constexpr int M2L3M_T = 2 * 15 + 1;      // table side for p <= 15
+
+__global__ void __launch_bounds__(256)
+k3_m2l_mfma(KD kd, TD t, Flat fl, const int *ttp, const int *starts, const int *lists,
+            const double *mpole, double *local)
+{
+    __shared__ double Ts[2 * M2L3M_T * M2L3M_T];
+    __shared__ short nof[256], mof[256];
+    __shared__ double ihp[32];
+    int lv, i, unused;
+    if (!flat_locate(fl, blockIdx.x, lv, i, unused)) return;
+    const int e0 = starts[i], e1 = starts[i + 1];
+    if (e0 == e1) return;
+    const int Bx = ttp[i];
+    const int p = kd.order[lv];
+    const int nc = kd.ncl[lv];
+    const int RT = (nc + 15) / 16, KS = (nc + 3) / 4;
+    const int tnl = (2 * p + 1) * (2 * p + 1);  // table entries used: IDX(N, M), N <= 2p
+    const double ih = 1.0 / kd.h[lv];
+    const int W = kd.W, S1 = 2 * W + 1, tn = kd.tabn;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int col = lane & 15, kr = lane >> 4;
+    const int rc = kd.rc, r0 = kd.r0;
+    for (int q = tid; q < 256; q += 256) {
+        int n = (int)sqrt((double)q);
+        while (n * n > q) --n;
+        while ((n + 1) * (n + 1) <= q) ++n;
+        nof[q] = (short)n;
+        mof[q] = (short)(q - n * n - n);
+    }
+    if (tid < 32) {
+        double s = 1.0;
+        for (int k = 0; k < tid; ++k) s *= ih;
+        ihp[tid] = s;
+    }
+    __syncthreads();  // nof, mof, ihp (the epilogue reads them even with no entry)
+    const int bx = t.coords[Bx], by = t.coords[(size_t)t.nboxes + Bx],
+              bz = t.coords[2 * (size_t)t.nboxes + Bx];
+    f3_d4 cre[M2L3M_RTW], cim[M2L3M_RTW];
+#pragma unroll
+    for (int q = 0; q < M2L3M_RTW; ++q) cre[q] = cim[q] = f3_d4{0, 0, 0, 0};
+    for (int e = e0; e < e1; ++e) {
+        const int A = lists[e];
+        const int dx = bx - t.coords[A] + W;
+        const int dy = by - t.coords[(size_t)t.nboxes + A] + W;
+        const int dz = bz - t.coords[2 * (size_t)t.nboxes + A] + W;
+        if ((unsigned)dx >= (unsigned)S1 || (unsigned)dy >= (unsigned)S1 ||
+            (unsigned)dz >= (unsigned)S1)
+            continue;  // uniform over the block
+        const double *a = mpole + ecoef_off(kd, A, lv);
+        const double *T = kd.tab_m2l + 2L * ((dx * S1 + dy) * S1 + dz) * tn * tn;
+        __syncthreads();  // previous table consumed (and nof/ihp written)
+        for (int q = tid; q < tnl; q += 256) {
+            Ts[2 * q] = T[2 * q];
+            Ts[2 * q + 1] = T[2 * q + 1];
+        }
+        __syncthreads();
+        for (int ks = 0; ks < KS; ++ks) {
+            const int kq = ks * 4 + kr;  // (n, m) of the B row / A column
+            double bre = 0.0, bim = 0.0;
+            int n = 0, m = 0;
+            if (kq < nc) {
+                n = nof[kq];
+                m = mof[kq];
+                if (col < rc) {
+                    const cplx b = ldcn(kd, a, kq, r0 + col);
+                    bre = ihp[n] * b.x;
+                    bim = ihp[n] * b.y;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < M2L3M_RTW; ++q) {
+                const int rt = wv + 4 * q;
+                if (rt < RT) {
+                    const int row = rt * 16 + col;
+                    double are = 0.0, aim = 0.0;
+                    if (row < nc && kq < nc) {
+                        const int N = n + nof[row], M = m + mof[row];
+                        if (M >= -N && M <= N) {
+                            are = Ts[2 * IDX(N, M)];
+                            aim = Ts[2 * IDX(N, M) + 1];
+                        }
+                    }
+                    cre[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bre, cre[q], 0, 0, 0);
+                    cre[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-aim, bim, cre[q], 0, 0, 0);
+                    cim[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bim, cim[q], 0, 0, 0);
+                    cim[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, bre, cim[q], 0, 0, 0);
+                }
+            }
+        }
+    }
+    double *o = local + ecoef_off(kd, Bx, lv);
+#pragma unroll
+    for (int q = 0; q < M2L3M_RTW; ++q) {
+        const int rt = wv + 4 * q;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int row = rt * 16 + 4 * j + kr;
+            if (rt < RT && row < nc && col < rc) {
+                const int k = nof[row];
+                const double sc = ((k & 1) ? -1.0 : 1.0) * ihp[k + 1];
+                double *dst = o + 2 * ((long)row * kd.nr + r0 + col);
+                dst[0] += sc * cre[q][j];
+                dst[1] += sc * cim[q][j];
+            }
+        }
+    }
+}
+
 // P2P (MODE 0, list 1), M2P (1, list 3), L2P (2) for non-QBX targets
 template <int MODE, int NRC>
 __global__ void __launch_bounds__(64) k3_eval_targets(KD kd, TD t, int ntb, const int *tboxes,
@@ -1625,7 +1744,26 @@ int f3_multipole_to_local(qbx_ctx *ctx, void *stream, const qbx_tree *T, const q
     QBX_TRY(ensure_tables(ctx, &kd));
     hipStream_t st = S(stream);
     QBX_TRY(zero_fill(local, (exp_bytes(kd)) / 8, st));
-    if (R->nttp_boxes > 0 && R->n2 > 0) {
+    if (R->nttp_boxes > 0 && R->n2 > 0 && kd.nr >= 4 && kd.pmax <= 15) {
+        // many right-hand sides: on the matrix cores, 16 per launch
+        const TD td = make_td(T);
+        Flat fl;
+        fl.nl = kd.nlevels;
+        long nb = 0;
+        for (int l = 0; l < kd.nlevels; ++l) {
+            fl.tstart[l] = nb;
+            fl.istart[l] = R->h_level_start_ttp_box_nrs[l];
+            fl.nper[l] = 1;
+            nb += R->h_level_start_ttp_box_nrs[l + 1] - R->h_level_start_ttp_box_nrs[l];
+        }
+        fl.tstart[kd.nlevels] = nb;
+        for (int r0 = 0; r0 < kd.nr; r0 += 16) {
+            kd.r0 = r0;
+            kd.rc = std::min(16, kd.nr - r0);
+            hipLaunchKernelGGL(k3_m2l_mfma, dim3(nb), dim3(256), 0, st, kd, td, fl,
+                               R->ttp_boxes.p, R->l2_starts.p, R->l2_lists.p, mpole, local);
+        }
+    } else if (R->nttp_boxes > 0 && R->n2 > 0) {
         const TD td = make_td(T);
         const int no = ((kd.pmax + 1) * (kd.pmax + 1) + M2L_THREADS - 1) / M2L_THREADS;
 #define L3_M2L(N)                                                                          \
