@@ -64,6 +64,9 @@ constexpr double INV4PI = 0.079577471545947667884;
 
 __host__ __device__ constexpr int IDX(int n, int m) { return n * n + n + m; }
 
+// accumulator of v_mfma_f64_16x16x4f64 (4 doubles per lane)
+typedef double f3_d4 __attribute__((ext_vector_type(4)));
+
 struct KD {
     int dip, pq, nlevels, ncq, nsep, W, pmax, tabn;   // tabn = 2 pmax + 1
     int nr, r0, rc;            // right-hand sides; the chunk of this launch
@@ -713,7 +716,6 @@ __global__ void __launch_bounds__(M2L_THREADS) k3_m2l(KD kd, TD t, const int *tt
 // A[i%16][k0 + i/16] and B[k0 + i/16][i%16]; D register j of lane i is
 // D[4j + i/16][i%16]).  Wave w takes the 16-row tiles w, w + 4, ... over all
 // entries; the epilogue scales by (-1)^k h^(-k-1) as k3_m2l.
-typedef double f3_d4 __attribute__((ext_vector_type(4)));
 constexpr int M2L3M_RTW = 4;             // row tiles per wave: (p + 1)^2 <= 256
 constexpr int M2L3M_T = 2 * 15 + 1;      // table side for p <= 15
 
@@ -1061,16 +1063,15 @@ __global__ void __launch_bounds__(64) k3_p2qbxl(KD kd, TD t, const int *ubox, co
 // builds the basis of every (center, source) pair once into LDS,
 // A[k][row] with row = (center, coefficient), and stages the chunk's
 // strengths B[k][rhs]; phase B accumulates C[row][rhs] += sum_k A[k][row]
-// B[k][rhs] with each thread holding RPT rows x 4 right-hand sides in
-// registers.  The basis is computed once for all right-hand sides instead of
-// once per register chunk of them.
+// B[k][rhs] on the FP64 matrix cores (v_mfma_f64_16x16x4f64, 16 x 16 output
+// tiles, the complex product as four real ones).  The basis is computed once
+// for all right-hand sides instead of once per register chunk of them.
 constexpr int MR_RB = 32;
 
 template <int PQ>
 struct MRShape {
     static constexpr int NQ = (PQ + 1) * (PQ + 1);
     static constexpr int R = GC_GROUP * NQ;
-    static constexpr int RPT = (R + 31) / 32;
     static constexpr int SC = PQ <= 3 ? 32 : 16;
 };
 
@@ -1084,7 +1085,7 @@ __global__ void __launch_bounds__(256) k3_p2qbxl_mr(KD kd, TD t, const int *ubox
                                                     int accumulate)
 {
     using SH = MRShape<PQ>;
-    constexpr int NQ = SH::NQ, R = SH::R, RPT = SH::RPT, SC = SH::SC;
+    constexpr int NQ = SH::NQ, R = SH::R, SC = SH::SC;
     __shared__ double As[SC * R * 2];
     __shared__ double Bs[SC * MR_RB * 2];
     __shared__ int sidx[SC];
@@ -1096,13 +1097,17 @@ __global__ void __launch_bounds__(256) k3_p2qbxl_mr(KD kd, TD t, const int *ubox
     const int rb0 = blockIdx.y * MR_RB;
     const int nrb = min(MR_RB, nr - rb0);
     const int tid = threadIdx.x;
-    const int rt = tid >> 3, ct = tid & 7;
+    const int wv = tid >> 6, lane = tid & 63;
+    const int col = lane & 15, kr = lane >> 4;
     const bool dip = kd.dip;
-    cplx C[RPT][4];
+    // phase B on the matrix cores: output tiles (16 rows x 16 right-hand
+    // sides); wave w takes the row tiles w, w + 4, ... for both column tiles
+    constexpr int RT = (R + 15) / 16, RTW = (RT + 3) / 4;
+    f3_d4 cre[RTW][2], cim[RTW][2];
 #pragma unroll
-    for (int j = 0; j < RPT; ++j)
+    for (int j = 0; j < RTW; ++j)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) C[j][q] = {0, 0};
+        for (int h = 0; h < 2; ++h) cre[j][h] = cim[j][h] = f3_d4{0, 0, 0, 0};
 
     auto flush = [&](int n) {
         __syncthreads();
@@ -1140,18 +1145,36 @@ __global__ void __launch_bounds__(256) k3_p2qbxl_mr(KD kd, TD t, const int *ubox
             Bs[2 * (k * MR_RB + r) + 1] = pq[1];
         }
         __syncthreads();
-        for (int k = 0; k < n; ++k) {
-            cplx b[4];
+        // C[row][rhs] += sum_k A[k][row] B[k][rhs]: lane holds A[row0 + col][k]
+        // and B[k][16 h + col], k = 4 ks + kr; slots k >= n hold stale data
+        // and are masked (rows and columns past the valid ones only reach
+        // outputs that are not stored)
+        for (int ks = 0; ks < (SC + 3) / 4; ++ks) {
+            const int k = ks * 4 + kr;
+            const bool kin = k < n;
+            double bre[2], bim[2];
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                b[q] = {Bs[2 * (k * MR_RB + 4 * ct + q)], Bs[2 * (k * MR_RB + 4 * ct + q) + 1]};
+            for (int h = 0; h < 2; ++h) {
+                bre[h] = kin ? Bs[2 * (k * MR_RB + 16 * h + col)] : 0.0;
+                bim[h] = kin ? Bs[2 * (k * MR_RB + 16 * h + col) + 1] : 0.0;
+            }
 #pragma unroll
-            for (int j = 0; j < RPT; ++j) {
-                const int row = rt + 32 * j;
-                if (row < R) {
-                    const cplx a = {As[2 * (k * R + row)], As[2 * (k * R + row) + 1]};
+            for (int j = 0; j < RTW; ++j) {
+                const int rtile = wv + 4 * j;
+                if (rtile < RT) {
+                    const int row = rtile * 16 + col;
+                    double are = 0.0, aim = 0.0;
+                    if (kin && row < R) {
+                        are = As[2 * (k * R + row)];
+                        aim = As[2 * (k * R + row) + 1];
+                    }
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) C[j][q] += a * b[q];
+                    for (int h = 0; h < 2; ++h) {
+                        cre[j][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bre[h], cre[j][h], 0, 0, 0);
+                        cre[j][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(-aim, bim[h], cre[j][h], 0, 0, 0);
+                        cim[j][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bim[h], cim[j][h], 0, 0, 0);
+                        cim[j][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, bre[h], cim[j][h], 0, 0, 0);
+                    }
                 }
             }
         }
@@ -1176,20 +1199,22 @@ __global__ void __launch_bounds__(256) k3_p2qbxl_mr(KD kd, TD t, const int *ubox
     }
     if (fill > 0) flush(fill);
 #pragma unroll
-    for (int j = 0; j < RPT; ++j) {
-        const int row = rt + 32 * j;
-        if (row >= R) continue;
-        const int ci = row / NQ, i = row - ci * NQ;
-        if (ci >= ncen) continue;
-        const int c = gcl[gcs[it] + c0 + ci];
+    for (int j = 0; j < RTW; ++j) {
+        const int rtile = wv + 4 * j;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int r = 4 * ct + q;
-            if (r < nrb) {
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int row = rtile * 16 + 4 * g + kr;  // D row of register g
+                const int r = 16 * h + col;
+                if (rtile >= RT || row >= R || r >= nrb) continue;
+                const int ci = row / NQ, i = row - ci * NQ;
+                if (ci >= ncen) continue;
+                const int c = gcl[gcs[it] + c0 + ci];
+                const cplx v = {cre[j][h][g], cim[j][h][g]};
                 double *pq = qexp + 2 * (((size_t)c * NQ + i) * nr + rb0 + r);
-                stc(pq, accumulate ? ldc(pq) + C[j][q] : C[j][q]);
+                stc(pq, accumulate ? ldc(pq) + v : v);
             }
-        }
     }
 }
 
