@@ -382,6 +382,47 @@ struct P2EShape3 {
 
 __device__ __forceinline__ int tri(int n, int m) { return n * (n + 1) / 2 + m; }
 
+// the source's table for k3_p2e: h[tri(n, m)], m >= 0, n <= pt, of d = source -
+// center -- irregular harmonics (LOCAL) or regular ones (multipoles), by the
+// recurrences in m and n
+template <bool LOCAL>
+__device__ __forceinline__ void p2e3_table(v3 d, int pt, double *h)
+{
+    const double r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+    if (LOCAL) {
+        const double ir2 = 1.0 / r2;
+        const cplx w = {d.x * ir2, d.y * ir2};
+        cplx tmm = {sqrt(ir2), 0};
+        for (int mm = 0; mm <= pt; ++mm) {
+            if (mm > 0) tmm = (-(2.0 * mm - 1.0)) * (tmm * w);
+            cplx a = {0, 0}, b = tmm;
+            stc(h + 2 * tri(mm, mm), b);
+            for (int nn = mm + 1; nn <= pt; ++nn) {
+                const cplx cc = ir2 * (((2.0 * nn - 1.0) * d.z) * b -
+                                       ((double)(nn + mm - 1) * (nn - mm - 1)) * a);
+                stc(h + 2 * tri(nn, mm), cc);
+                a = b;
+                b = cc;
+            }
+        }
+    } else {
+        const cplx hw = {-0.5 * d.x, -0.5 * d.y};
+        cplx ymm = {1, 0};
+        for (int mm = 0; mm <= pt; ++mm) {
+            if (mm > 0) ymm = (1.0 / mm) * (ymm * hw);
+            cplx a = {0, 0}, b = ymm;
+            stc(h + 2 * tri(mm, mm), b);
+            for (int nn = mm + 1; nn <= pt; ++nn) {
+                const cplx cc = (1.0 / ((double)(nn - mm) * (nn + mm))) *
+                                (((2.0 * nn - 1.0) * d.z) * b - r2 * a);
+                stc(h + 2 * tri(nn, mm), cc);
+                a = b;
+                b = cc;
+            }
+        }
+    }
+}
+
 template <int NRC, bool LOCAL>
 __global__ void __launch_bounds__(512) k3_p2e(KD kd, TD t, int lv, int i0, P2EShape3 sh,
                                                const int *boxes, const int *starts,
@@ -426,41 +467,7 @@ __global__ void __launch_bounds__(512) k3_p2e(KD kd, TD t, int lv, int i0, P2ESh
         __syncthreads();
         if (tid < cnt) {
             const int si = idx[tid];
-            const v3 d = sub(ld3(src, ns, si), c);
-            double *h = Ht + 2 * tid * R;
-            const double r2 = d.x * d.x + d.y * d.y + d.z * d.z;
-            if (LOCAL) {
-                const double ir2 = 1.0 / r2;
-                const cplx w = {d.x * ir2, d.y * ir2};
-                cplx tmm = {sqrt(ir2), 0};
-                for (int mm = 0; mm <= pt; ++mm) {
-                    if (mm > 0) tmm = (-(2.0 * mm - 1.0)) * (tmm * w);
-                    cplx a = {0, 0}, b = tmm;
-                    stc(h + 2 * tri(mm, mm), b);
-                    for (int nn = mm + 1; nn <= pt; ++nn) {
-                        const cplx cc = ir2 * (((2.0 * nn - 1.0) * d.z) * b -
-                                               ((double)(nn + mm - 1) * (nn - mm - 1)) * a);
-                        stc(h + 2 * tri(nn, mm), cc);
-                        a = b;
-                        b = cc;
-                    }
-                }
-            } else {
-                const cplx hw = {-0.5 * d.x, -0.5 * d.y};
-                cplx ymm = {1, 0};
-                for (int mm = 0; mm <= pt; ++mm) {
-                    if (mm > 0) ymm = (1.0 / mm) * (ymm * hw);
-                    cplx a = {0, 0}, b = ymm;
-                    stc(h + 2 * tri(mm, mm), b);
-                    for (int nn = mm + 1; nn <= pt; ++nn) {
-                        const cplx cc = (1.0 / ((double)(nn - mm) * (nn + mm))) *
-                                        (((2.0 * nn - 1.0) * d.z) * b - r2 * a);
-                        stc(h + 2 * tri(nn, mm), cc);
-                        a = b;
-                        b = cc;
-                    }
-                }
-            }
+            p2e3_table<LOCAL>(sub(ld3(src, ns, si), c), pt, Ht + 2 * tid * R);
 #pragma unroll
             for (int q = 0; q < NRC; ++q)
                 if (q < rc) {
@@ -524,6 +531,151 @@ __global__ void __launch_bounds__(512) k3_p2e(KD kd, TD t, int lv, int i0, P2ESh
                 double *pq = o + 2 * (r0 + q);
                 stc(pq, LOCAL ? ldc(pq) + acc[q] : acc[q]);
             }
+    }
+}
+
+// P2M / P2L for many right-hand sides on the FP64 matrix cores: the sources
+// of the box (P2M) or of its list-4 boxes (P2L) are staged P2E3M_S at a time
+// with their tables (p2e3_table) and the strengths of up to 16 right-hand
+// sides; out[(n,m)][r] (+)= sum_s b_(n,m)(s) q_s[r] with b as in k3_p2e
+// (single or double layer) is a complex product: rows the coefficients
+// (wave w takes the 16-row tiles w, w + 4, ...), k the staged sources.
+constexpr int P2E3M_S = 32;
+constexpr int P2E3M_RTW = 4;  // (p + 1)^2 <= 256
+
+template <bool LOCAL>
+__global__ void __launch_bounds__(256)
+k3_p2e_mfma(KD kd, TD t, int lv, int i0, int R, const int *boxes, const int *starts,
+            const int *lists, const double *src, int ns, const double *str, const double *nrm,
+            double *expn)
+{
+    extern __shared__ double lds[];
+    double *Ht = lds, *Qs = Ht + 2 * P2E3M_S * R, *Nu = Qs + 2 * P2E3M_S * 16;
+    int *idx = (int *)(Nu + 3 * P2E3M_S);
+    __shared__ short nof[256], mof[256];
+    const int i = i0 + blockIdx.x;
+    const int Bx = boxes[i];
+    int e0 = 0, e1 = 1;
+    if (LOCAL) {
+        e0 = starts[i];
+        e1 = starts[i + 1];
+        if (e0 == e1) return;
+    }
+    const int p = kd.order[lv];
+    const int pt = LOCAL ? p + 1 : p;
+    const int nout = (p + 1) * (p + 1);
+    const int RT = (nout + 15) / 16;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int col = lane & 15, kr = lane >> 4;
+    const bool dip = kd.dip;
+    const int rc = kd.rc, r0 = kd.r0, nr = kd.nr;
+    const v3 c = boxc(t, Bx);
+    {
+        int n = (int)sqrt((double)tid);
+        while (n * n > tid) --n;
+        while ((n + 1) * (n + 1) <= tid) ++n;
+        nof[tid] = (short)n;
+        mof[tid] = (short)(tid - n * n - n);
+    }
+    f3_d4 cre[P2E3M_RTW], cim[P2E3M_RTW];
+#pragma unroll
+    for (int q = 0; q < P2E3M_RTW; ++q) cre[q] = cim[q] = f3_d4{0, 0, 0, 0};
+
+    auto hval = [&](int sl, int nn, int mm) -> cplx {
+        const int am = mm < 0 ? -mm : mm;
+        if (nn < 0 || am > nn) return {0, 0};
+        const cplx v = {Ht[2 * (sl * R + tri(nn, am))], Ht[2 * (sl * R + tri(nn, am)) + 1]};
+        return mm < 0 ? neg_m(v, am) : v;
+    };
+
+    auto flush = [&](int cnt) {
+        __syncthreads();
+        if (tid < cnt) {
+            const int si = idx[tid];
+            p2e3_table<LOCAL>(sub(ld3(src, ns, si), c), pt, Ht + 2 * tid * R);
+            if (dip) {
+                Nu[3 * tid] = nrm[si];
+                Nu[3 * tid + 1] = nrm[(size_t)ns + si];
+                Nu[3 * tid + 2] = nrm[2 * (size_t)ns + si];
+            }
+        }
+        for (int e = tid; e < P2E3M_S * 16; e += 256) {
+            const int k = e >> 4, r = e & 15;
+            double vr = 0.0, vi = 0.0;
+            if (k < cnt && r < rc) {
+                const double *pq = str + 2 * ((size_t)idx[k] * nr + r0 + r);
+                vr = pq[0];
+                vi = pq[1];
+            }
+            Qs[2 * e] = vr;
+            Qs[2 * e + 1] = vi;
+        }
+        __syncthreads();
+        for (int ks = 0; ks < P2E3M_S / 4; ++ks) {
+            const int sl = ks * 4 + kr;
+            const double bre = Qs[2 * (sl * 16 + col)], bim = Qs[2 * (sl * 16 + col) + 1];
+#pragma unroll
+            for (int q = 0; q < P2E3M_RTW; ++q) {
+                const int rt = wv + 4 * q;
+                if (rt < RT) {
+                    const int row = rt * 16 + col;
+                    cplx a = {0, 0};
+                    if (row < nout && sl < cnt) {
+                        const int n = nof[row], m = mof[row];
+                        if (!dip) {
+                            a = LOCAL ? hval(sl, n, m) : conj(hval(sl, n, m));
+                        } else {
+                            const double nx = Nu[3 * sl], ny = Nu[3 * sl + 1], nz = Nu[3 * sl + 2];
+                            const cplx nmn = {nx, -ny}, npl = {nx, ny};
+                            if (LOCAL)
+                                a = (-nz) * hval(sl, n + 1, m) +
+                                    0.5 * (nmn * hval(sl, n + 1, m + 1) -
+                                           npl * hval(sl, n + 1, m - 1));
+                            else
+                                a = conj(nz * hval(sl, n - 1, m) +
+                                         0.5 * (nmn * hval(sl, n - 1, m + 1) -
+                                                npl * hval(sl, n - 1, m - 1)));
+                        }
+                    }
+                    cre[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bre, cre[q], 0, 0, 0);
+                    cre[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, bim, cre[q], 0, 0, 0);
+                    cim[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bim, cim[q], 0, 0, 0);
+                    cim[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, bre, cim[q], 0, 0, 0);
+                }
+            }
+        }
+    };
+
+    int fill = 0;
+    for (int e = e0; e < e1; ++e) {
+        const int Sb = LOCAL ? lists[e] : Bx;
+        int s = t.sstart[Sb];
+        const int s1 = s + t.scount[Sb];
+        while (s < s1) {
+            const int take = min(s1 - s, P2E3M_S - fill);
+            for (int j = tid; j < take; j += 256) idx[fill + j] = s + j;
+            fill += take;
+            s += take;
+            if (fill == P2E3M_S) {
+                flush(P2E3M_S);
+                fill = 0;
+            }
+        }
+    }
+    if (fill > 0) flush(fill);
+    double *o = expn + ecoef_off(kd, Bx, lv);
+#pragma unroll
+    for (int q = 0; q < P2E3M_RTW; ++q) {
+        const int rt = wv + 4 * q;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int row = rt * 16 + 4 * j + kr;
+            if (rt < RT && row < nout && col < rc) {
+                double *pq = o + 2 * ((long)row * nr + r0 + col);
+                const cplx v = {cre[q][j], cim[q][j]};
+                stc(pq, LOCAL ? ldc(pq) + v : v);
+            }
+        }
     }
 }
 
@@ -1649,6 +1801,12 @@ static bool p2e3_launch(int p, bool local, P2EShape3 *sh, int *threads, size_t *
     return true;
 }
 
+// dynamic LDS of k3_p2e_mfma for table row stride R (complex)
+static size_t p2e3m_bytes(int R)
+{
+    return 8UL * (2 * P2E3M_S * R + 2 * P2E3M_S * 16 + 3 * P2E3M_S) + 4UL * P2E3M_S;
+}
+
 static bool p2e3_all_levels(const KD &kd, int nlevels, bool local)
 {
     for (int l = 0; l < nlevels; ++l) {
@@ -1679,6 +1837,17 @@ int f3_form_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_
             int th;
             size_t by;
             p2e3_launch(kd.order[l], false, &sh, &th, &by);
+            if (kd.nr >= 4 && (kd.order[l] + 1) * (kd.order[l] + 1) <= 16 * 4 * P2E3M_RTW) {
+                for (int r0 = 0; r0 < kd.nr; r0 += 16) {
+                    kd.r0 = r0;
+                    kd.rc = std::min(16, kd.nr - r0);
+                    hipLaunchKernelGGL(k3_p2e_mfma<false>, dim3(i1 - i0), dim3(256),
+                                       p2e3m_bytes(sh.R), st, kd, td, l, i0, sh.R,
+                                       R->source_boxes.p, nullptr, nullptr, T->sources.p,
+                                       T->nsources, str, nrm, mpole);
+                }
+                continue;
+            }
 #define L3_P2MH(N)                                                                         \
     hipLaunchKernelGGL((k3_p2e<N, false>), dim3(i1 - i0), dim3(th), by, st, kd, td, l, i0, sh, \
                        R->source_boxes.p, nullptr, nullptr, T->sources.p, T->nsources, str,  \
@@ -1821,6 +1990,17 @@ int f3_form_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav
             int th;
             size_t by;
             p2e3_launch(kd.order[l], true, &sh, &th, &by);
+            if (kd.nr >= 4 && (kd.order[l] + 1) * (kd.order[l] + 1) <= 16 * 4 * P2E3M_RTW) {
+                for (int r0 = 0; r0 < kd.nr; r0 += 16) {
+                    kd.r0 = r0;
+                    kd.rc = std::min(16, kd.nr - r0);
+                    hipLaunchKernelGGL(k3_p2e_mfma<true>, dim3(i1 - i0), dim3(256),
+                                       p2e3m_bytes(sh.R), st, kd, td, l, i0, sh.R,
+                                       R->ttp_boxes.p, R->l4_starts.p, R->l4_lists.p,
+                                       T->sources.p, T->nsources, str, nrm, local);
+                }
+                continue;
+            }
 #define L3_P2LH(N)                                                                         \
     hipLaunchKernelGGL((k3_p2e<N, true>), dim3(i1 - i0), dim3(th), by, st, kd, td, l, i0, sh, \
                        R->ttp_boxes.p, R->l4_starts.p, R->l4_lists.p, T->sources.p,         \
