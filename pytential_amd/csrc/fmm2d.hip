@@ -3056,11 +3056,16 @@ int qbx_form_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx
     return QBX_OK;
 }
 
-int qbx_coarsen_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
-                           const qbx_kernel_desc *d, double *mpole)
+// M2M into the parent levels lmin .. nlevels - 2 (lmin = 0: every level, as
+// the reference's coarsen_multipoles; the fused driver passes the coarsest
+// level whose multipoles a later stage reads)
+static int coarsen_multipoles_from(qbx_ctx *ctx, void *stream, const qbx_tree *T,
+                                   const qbx_trav *R, const qbx_kernel_desc *d, double *mpole,
+                                   int lmin)
 {
     QBX_CHECK(ctx && T && R && mpole, "qbx_coarsen_multipoles: null argument");
-    if (d && d->kernel == QBX_LAPLACE3D) return f3_coarsen_multipoles(ctx, stream, T, R, d, mpole);
+    if (d && d->kernel == QBX_LAPLACE3D)
+        return f3_coarsen_multipoles(ctx, stream, T, R, d, mpole, lmin);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd, R->nsep));
     const double *B;
@@ -3070,7 +3075,7 @@ int qbx_coarsen_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const 
     const TD td = make_td(T);
     bool mfma = kd.kernel == QBX_HELMHOLTZ2D && kd.nr >= 4;
     for (int l = 0; l < T->nlevels; ++l) mfma = mfma && 2 * kd.order[l] + 1 <= 16 * 4 * M2LM_RTW;
-    for (int l = T->nlevels - 2; l >= 0; --l) {
+    for (int l = T->nlevels - 2; l >= std::max(lmin, 0); --l) {
         const int i0 = R->h_level_start_source_parent_box_nrs[l];
         const int i1 = R->h_level_start_source_parent_box_nrs[l + 1];
         const long nthr = (long)(i1 - i0) * kd.ncl[l];
@@ -3104,6 +3109,12 @@ int qbx_coarsen_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const 
     }
     QBX_HIP(hipGetLastError());
     return QBX_OK;
+}
+
+int qbx_coarsen_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                           const qbx_kernel_desc *d, double *mpole)
+{
+    return coarsen_multipoles_from(ctx, stream, T, R, d, mpole, 0);
 }
 
 static int eval_targets(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
@@ -3315,11 +3326,15 @@ int qbx_form_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_tra
     return QBX_OK;
 }
 
-int qbx_refine_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
-                      const qbx_kernel_desc *d, double *local)
+// L2L from the parent levels lmin .. nlevels - 2 (lmin = 0: every level, as
+// the reference's refine_locals; the fused driver passes the coarsest level
+// with a nonzero local, above which the shifts would add zeros)
+static int refine_locals_from(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                              const qbx_kernel_desc *d, double *local, int lmin)
 {
     QBX_CHECK(ctx && T && R && local, "qbx_refine_locals: null argument");
-    if (d && d->kernel == QBX_LAPLACE3D) return f3_refine_locals(ctx, stream, T, R, d, local);
+    if (d && d->kernel == QBX_LAPLACE3D)
+        return f3_refine_locals(ctx, stream, T, R, d, local, lmin);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd, R->nsep));
     const double *B;
@@ -3329,7 +3344,7 @@ int qbx_refine_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_t
     const TD td = make_td(T);
     bool mfma = kd.kernel == QBX_HELMHOLTZ2D && kd.nr >= 4;
     for (int l = 0; l < T->nlevels; ++l) mfma = mfma && 2 * kd.order[l] + 1 <= 16 * 4 * M2LM_RTW;
-    for (int l = 1; l < T->nlevels; ++l) {
+    for (int l = std::max(lmin, 0) + 1; l < T->nlevels; ++l) {
         const int i0 = R->h_level_start_ttp_box_nrs[l], i1 = R->h_level_start_ttp_box_nrs[l + 1];
         const long nthr = (long)(i1 - i0) * kd.ncl[l];
         if (nthr == 0) continue;
@@ -3359,6 +3374,12 @@ int qbx_refine_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_t
     }
     QBX_HIP(hipGetLastError());
     return QBX_OK;
+}
+
+int qbx_refine_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                      const qbx_kernel_desc *d, double *local)
+{
+    return refine_locals_from(ctx, stream, T, R, d, local, 0);
 }
 
 int qbx_form_global_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
@@ -3705,7 +3726,7 @@ int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav 
     QBX_TRY(qbx_form_global_qbx_locals(ctx, s2, T, R, G, d, str, nrm, Q, 0));
     QBX_TRY(qbx_form_locals(ctx, s3, T, R, d, str, nrm, L4, 0));
     QBX_HIP(hipEventRecord(eP, s3));
-    QBX_TRY(qbx_coarsen_multipoles(ctx, st, T, R, d, M));
+    QBX_TRY(coarsen_multipoles_from(ctx, st, T, R, d, M, R->mpole_min_level));
     QBX_HIP(hipEventRecord(eM, st));
     QBX_HIP(hipStreamWaitEvent(s1, eM, 0));
     QBX_HIP(hipStreamWaitEvent(s2, eM, 0));
@@ -3714,7 +3735,7 @@ int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav 
     QBX_TRY(qbx_multipole_to_local(ctx, st, T, R, d, M, L));
     QBX_HIP(hipStreamWaitEvent(st, eP, 0));
     QBX_TRY(add_into(L, L4, nexp * 2, st));
-    QBX_TRY(qbx_refine_locals(ctx, st, T, R, d, L));
+    QBX_TRY(refine_locals_from(ctx, st, T, R, d, L, R->local_min_level));
     QBX_HIP(hipEventRecord(eL, st));
     QBX_HIP(hipStreamWaitEvent(s1, eL, 0));
     QBX_HIP(hipStreamWaitEvent(s2, eL, 0));

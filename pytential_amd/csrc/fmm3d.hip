@@ -1873,14 +1873,14 @@ int f3_form_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_
 }
 
 int f3_coarsen_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
-                          const qbx_kernel_desc *d, double *mpole)
+                          const qbx_kernel_desc *d, double *mpole, int lmin)
 {
     KD kd;
     QBX_TRY(make_kd(d, T, &kd, R->nsep));
     QBX_TRY(ensure_tables(ctx, &kd));
     hipStream_t st = S(stream);
     const TD td = make_td(T);
-    for (int l = T->nlevels - 2; l >= 0; --l) {
+    for (int l = T->nlevels - 2; l >= std::max(lmin, 0); --l) {
         const int i0 = R->h_level_start_source_parent_box_nrs[l];
         const int i1 = R->h_level_start_source_parent_box_nrs[l + 1];
         const long nthr = (long)(i1 - i0) * kd.ncl[l];
@@ -2027,14 +2027,14 @@ int f3_form_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav
 }
 
 int f3_refine_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
-                     const qbx_kernel_desc *d, double *local)
+                     const qbx_kernel_desc *d, double *local, int lmin)
 {
     KD kd;
     QBX_TRY(make_kd(d, T, &kd, R->nsep));
     QBX_TRY(ensure_tables(ctx, &kd));
     hipStream_t st = S(stream);
     const TD td = make_td(T);
-    for (int l = 1; l < T->nlevels; ++l) {
+    for (int l = std::max(lmin, 0) + 1; l < T->nlevels; ++l) {
         const int i0 = R->h_level_start_ttp_box_nrs[l], i1 = R->h_level_start_ttp_box_nrs[l + 1];
         const long nthr = (long)(i1 - i0) * kd.ncl[l];
         if (nthr > 0) {

@@ -139,6 +139,10 @@ struct qbx_trav {
         l4_starts, l4_lists;
     qbx::dbuf<int> box_to_ttp;        // box -> ttp index or -1
     qbx::dbuf<int> box_to_target_box; // box -> target box index or -1
+    // coarsest level of a box in list 2 or list 3 (the multipoles anything
+    // reads) and of a box with a nonempty list 2 or list 4 (the coarsest
+    // nonzero local); nlevels when there is none
+    int mpole_min_level = 0, local_min_level = 0;
 };
 
 struct qbx_geo {
@@ -213,8 +217,9 @@ int f3_ncoeff_qbx(const qbx_kernel_desc *d);
 int f3_form_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
                        const qbx_kernel_desc *d, const double *str, const double *nrm,
                        double *mpole);
+// M2M into parent levels >= lmin, L2L from parent levels >= lmin (0: all)
 int f3_coarsen_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
-                          const qbx_kernel_desc *d, double *mpole);
+                          const qbx_kernel_desc *d, double *mpole, int lmin = 0);
 int f3_eval_targets(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
                     const qbx_geo *G, const qbx_kernel_desc *d, int mode, const double *str,
                     const double *nrm, const double *expn, double *pot, int accumulate);
@@ -224,7 +229,7 @@ int f3_form_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav
                    const qbx_kernel_desc *d, const double *str, const double *nrm, double *local,
                    int accumulate);
 int f3_refine_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
-                     const qbx_kernel_desc *d, double *local);
+                     const qbx_kernel_desc *d, double *local, int lmin = 0);
 int f3_form_global_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T,
                               const qbx_trav *R, const qbx_geo *G, const qbx_kernel_desc *d,
                               const double *str, const double *nrm, double *qexp, int accumulate);

@@ -264,6 +264,34 @@ __global__ void k_inverse_map(int n, const int *boxes, int *map)
     if (i < n) map[boxes[i]] = i;
 }
 
+// Coarsest box level, into *out: with starts == nullptr over the n list
+// entries (boxes named by entries[i]), otherwise over the boxes[i] whose list
+// starts[i] .. starts[i + 1] is nonempty.  Minimum per wavefront, then one
+// atomic per wavefront.
+__global__ void __launch_bounds__(256) k_min_level(int n, const int *entries, const int *starts,
+                                                   const int *boxes, const int *level, int *out)
+{
+    int m = INT_MAX;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x) {
+        if (!starts)
+            m = min(m, level[entries[i]]);
+        else if (starts[i + 1] > starts[i])
+            m = min(m, level[boxes[i]]);
+    }
+    for (int o = 32; o > 0; o >>= 1) m = min(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0 && m != INT_MAX) atomicMin(out, m);
+}
+
+static void min_level(hipStream_t st, int n, const int *entries, const int *starts,
+                      const int *boxes, const int *level, int *out)
+{
+    if (n <= 0) return;
+    const int nb = (int)std::min<long>(cdiv(n, 256), 512);
+    hipLaunchKernelGGL(k_min_level, dim3(nb), dim3(256), 0, st, n, entries, starts, boxes, level,
+                       out);
+}
+
 static int level_starts(hipStream_t st, const qbx_tree *T, const dbuf<int> &boxes, int n,
                         dbuf<int> &d_out, std::vector<int> &h_out)
 {
@@ -342,9 +370,12 @@ static int traverse(qbx_ctx *ctx, hipStream_t st, const qbx_tree *T, int n, int 
                            R->ntarget_boxes, R->target_boxes.p, R->box_to_target_box.p);
     QBX_HIP(hipGetLastError());
 
+    // err[0]: stack overflow flag; err[1], err[2]: the coarsest levels of
+    // qbx_trav::mpole_min_level and local_min_level
     dbuf<int> err;
-    QBX_TRY(err.alloc(1));
-    QBX_HIP(hipMemsetAsync(err.p, 0, sizeof(int), st));
+    QBX_TRY(err.alloc(3));
+    const int err_init[3] = {0, T->nlevels, T->nlevels};
+    QBX_HIP(hipMemcpyAsync(err.p, err_init, sizeof(err_init), hipMemcpyHostToDevice, st));
     // lists 1 and 3
     {
         int ntb = R->ntarget_boxes;
@@ -395,10 +426,19 @@ static int traverse(qbx_ctx *ctx, hipStream_t st, const qbx_tree *T, int n, int 
         }
         QBX_HIP(hipGetLastError());
     }
-    int herr = 0;
-    QBX_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    min_level(st, R->n2, R->l2_lists.p, nullptr, nullptr, T->box_levels.p, err.p + 1);
+    min_level(st, R->n3, R->l3_lists.p, nullptr, nullptr, T->box_levels.p, err.p + 1);
+    min_level(st, R->nttp_boxes, nullptr, R->l2_starts.p, R->ttp_boxes.p, T->box_levels.p,
+              err.p + 2);
+    min_level(st, R->nttp_boxes, nullptr, R->l4_starts.p, R->ttp_boxes.p, T->box_levels.p,
+              err.p + 2);
+    QBX_HIP(hipGetLastError());
+    int herr[3] = {0, 0, 0};
+    QBX_HIP(hipMemcpyAsync(herr, err.p, sizeof(herr), hipMemcpyDeviceToHost, st));
     QBX_HIP(hipStreamSynchronize(st));
-    QBX_CHECK(herr == 0, "qbx_traverse: traversal stack overflow");
+    QBX_CHECK(herr[0] == 0, "qbx_traverse: traversal stack overflow");
+    R->mpole_min_level = herr[1];
+    R->local_min_level = herr[2];
     return QBX_OK;
 }
 
