@@ -2636,6 +2636,80 @@ k_l2qbxl(KD kd, TD t, const double *B, int ng, const int *gcent, const int *c2tb
     qbx_store<PQ, NRC>(kd, qexp, c, A, accumulate);
 }
 
+// Helmholtz L2QBXL for one right-hand side with LS lanes per center: lane
+// `sub` keeps the outputs m = -PQ + sub + LS i.  Each lane runs the center's
+// Miller recurrence itself (a few flops per step) and adds its outputs in the
+// order of k_l2qbxl, so the sums are the same; the split gives the launch LS
+// times the wavefronts of the one-thread-per-center kernel, which on its own
+// (one wavefront per 64 centers, a long serial chain each) was latency-bound.
+template <int PQ, int LS>
+__global__ void __launch_bounds__(256)
+k_l2qbxl_split(KD kd, TD t, int ng, const int *gcent, const int *c2tb, const int *tboxes,
+               const double *ctr, int nc, const double *local, double *qexp)
+{
+    constexpr int MI = (2 * PQ + 1 + LS - 1) / LS;
+    const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const int g = (int)(gid / LS), sub = (int)(gid % LS);
+    if (g >= ng) return;
+    const int c = gcent[g];
+    const int Bx = tboxes[c2tb[c]];
+    const int lv = t.level[Bx];
+    const int p1 = kd.order[lv];
+    const cplx cc = ldpt(ctr, nc, c);
+    const double *a = local + ecoef_off(kd, Bx, lv);
+    const cplx d = cc - boxc(t, Bx);
+    const double rho = sqrt(abs2(d));
+    const double x = kd.k * rho;
+    cplx A[MI];
+#pragma unroll
+    for (int i = 0; i < MI; ++i) A[i] = {0, 0};
+    if (x < 1e-300) {
+#pragma unroll
+        for (int i = 0; i < MI; ++i) {
+            const int m = -PQ + sub + LS * i;
+            if (m <= PQ && abs(m) <= p1) A[i] = ldcn(kd, a, m, 0);
+        }
+    } else {
+        const cplx e1 = {d.x / rho, d.y / rho};
+        const cplx em1 = conj(e1);
+        cplx e = cpow_int(e1, p1 + PQ);
+        const double norm = miller_j(
+            x, p1 + PQ,
+            [&](int l, double f) {
+                const cplx fe = f * e;
+                cplx fc = f * conj(e);
+                if (l & 1) fc = -fc;
+#pragma unroll
+                for (int i = 0; i < MI; ++i) {
+                    const int m = -PQ + sub + LS * i;
+                    const int n1 = m + l, n2 = m - l;
+                    if (m <= PQ) {
+                        if (n1 >= -p1 && n1 <= p1) A[i] += ldcn(kd, a, n1, 0) * fe;
+                        if (l > 0 && n2 >= -p1 && n2 <= p1) A[i] += ldcn(kd, a, n2, 0) * fc;
+                    }
+                }
+                e = e * em1;
+            },
+            [&](double sc) {
+#pragma unroll
+                for (int i = 0; i < MI; ++i) A[i] = sc * A[i];
+            });
+#pragma unroll
+        for (int i = 0; i < MI; ++i) A[i] = norm * A[i];
+    }
+    double *o = qexp + 2 * (size_t)c * kd.ncq;
+#pragma unroll
+    for (int i = 0; i < MI; ++i) {
+        const int m = -PQ + sub + LS * i;
+        if (m <= PQ) {
+            double *pm = o + 2 * (m + PQ);
+            stc(pm, A[i] + ldc(pm));
+        }
+    }
+}
+
+constexpr int L2QS_LANES = 4;
+
 // QBXL2P: one thread per (center, target) entry of center_to_tree_targets
 template <int NRC>
 __global__ void k_qbxl2p(KD kd, int nctt, const int *ctt_lists, const int *tree_ttc,
@@ -3527,6 +3601,31 @@ int qbx_translate_box_local_to_qbx_local(qbx_ctx *ctx, void *stream, const qbx_t
             }
 #undef L_L2QM
         }
+        QBX_HIP(hipGetLastError());
+        return QBX_OK;
+    }
+    if (kd.kernel == QBX_HELMHOLTZ2D && kd.nr == 1 && kd.pq <= 12) {
+#define L_L2QS(PQV)                                                                        \
+    hipLaunchKernelGGL((k_l2qbxl_split<PQV, L2QS_LANES>), dim3(cdiv((long)ng * L2QS_LANES, 256)), \
+                       dim3(256), 0, st, kd, td, ng, G->global_qbx_centers.p,               \
+                       G->center_to_target_box.p, R->target_boxes.p, G->centers.p,          \
+                       G->ncenters, local, qexp)
+        switch (kd.pq) {
+        case 0: L_L2QS(0); break;
+        case 1: L_L2QS(1); break;
+        case 2: L_L2QS(2); break;
+        case 3: L_L2QS(3); break;
+        case 4: L_L2QS(4); break;
+        case 5: L_L2QS(5); break;
+        case 6: L_L2QS(6); break;
+        case 7: L_L2QS(7); break;
+        case 8: L_L2QS(8); break;
+        case 9: L_L2QS(9); break;
+        case 10: L_L2QS(10); break;
+        case 11: L_L2QS(11); break;
+        default: L_L2QS(12); break;
+        }
+#undef L_L2QS
         QBX_HIP(hipGetLastError());
         return QBX_OK;
     }
