@@ -211,3 +211,39 @@ def test_association_failure_parity():
     ref = problems.refine_until_associated(cfg, assoc_status_and_flags)
     ref2 = problems.refine_until_associated(cfg, oracle.assoc_status_and_flags)
     assert np.array_equal(ref.discr.t_edges, ref2.discr.t_edges)
+
+
+@pytest.mark.parametrize("qbx_order", [2, 8])
+def test_helmholtz_qbx_locals_other_orders(qbx_order):
+    """2D Helmholtz D at QBX orders other than the default 5: P2QBXL + M2QBXL
+    + L2QBXL (for one density L2QBXL splits each center over four lanes) and
+    the single-call driver (which skips the coarse M2M/L2L levels no stage
+    reads) against the oracle."""
+    import oracle
+    from oracle import GeometryData
+
+    from pytential_amd import fmm as F
+    from pytential_amd import problems
+    from pytential_amd.qbx import QBXLayerPotentialSource
+    cfg = problems.config2(nelements=128, grid_n=48, fmm_order=12, qbx_order=qbx_order)
+    cfg = problems.refine_until_associated(cfg, oracle.assoc_status_and_flags)
+    print(f"\n[case] helmholtz-D qbx_order={qbx_order}: ndofs={cfg.discr.ndofs}", flush=True)
+    src = QBXLayerPotentialSource(cfg.discr, fine_order=cfg.discr.fine_order,
+                                  qbx_order=qbx_order, fmm_order=cfg.fmm_order)
+    geo = src.qbx_fmm_geometry_data(((cfg.targets, cfg.target_sides),))
+    ref = GeometryData(cfg.discr, cfg.targets, cfg.target_sides)
+    w = src.make_wrangler(geo, cfg.kernel, cfg.k, cfg.source_kind)
+    pot_ref, st = ref.fmm(cfg.kernel, cfg.k, cfg.source_kind, qbx_order, list(w._lo),
+                          cfg.discr.strengths(cfg.density), return_stages=True)
+    qd = src.quad_density(cfg.density)
+    strengths, nrm = w.reorder_sources(qd, src.dev["quad_weights"], src.dev["quad_normals"])
+    mpole = w.coarsen_multipoles(w.form_multipoles(strengths, nrm))
+    local = w.multipole_to_local(mpole)
+    w.form_locals(strengths, nrm, local)
+    w.refine_locals(local)
+    qexp = w.form_global_qbx_locals(strengths, nrm)
+    w.translate_box_multipoles_to_qbx_local(mpole, qexp)
+    w.translate_box_local_to_qbx_local(local, qexp)
+    assert _rel(_np(qexp), st["qbx"]) < REL_TOL, "P2QBXL+M2QBXL+L2QBXL"
+    fused = F.drive_fmm_fused(w, strengths, nrm)
+    assert _rel(_np(fused), pot_ref) < REL_TOL, "drive_fmm_fused"
