@@ -3796,15 +3796,20 @@ int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav 
     }
     // The stage DAG on four streams (the stage order of drive_fmm is kept
     // within each stream; every array has one writer stream at a time):
-    //   main: P2M -[e0]-> M2M -> M2L -[eM]-> (wait eP) L += L4 -> L2L -[eL]-> ... join
-    //   s3:   wait e0: P2L into L4 (its own local buffer) -[eP]->
-    //   s1:   wait e0: P2P, wait eM: M2P, wait eL: L2P   (non-QBX targets of P)
-    //   s2:   wait e0: P2QBXL, wait eM: M2QBXL, wait eL: L2QBXL, QBXL2P
+    //   main: zero P, M; P2M levels -(join)-> M2M -> M2L -[eM]-> (wait eP) L += L4
+    //         -> L2L -[eL]-> ... join
+    //   s3:   P2M levels, P2L into L4 (its own local buffer) -[eP]->
+    //   s1:   P2M levels, P2P, wait eM: M2P, wait eL: L2P  (non-QBX targets of P)
+    //   s2:   P2M levels, P2QBXL, wait eM: M2QBXL, wait eL: L2QBXL, QBXL2P
     //         (Q, QBX targets of P)
+    // (the side streams start after the zero fills; without the multistream
+    // P2M they wait for the whole P2M instead)
     // so the latency-bound level-by-level translations overlap the near field;
-    // P2M, the head of the chain, runs before the fork without competition
-    // (measured 4% faster than forking first), its independent per-level
-    // launches spread over the four streams.  M2P and M2QBXL, which need only
+    // P2M, the head of the chain, is issued before the near field (measured
+    // 4% faster than forking first), its independent per-level launches
+    // spread over the four streams, and each side stream goes on to its
+    // near-field stage when its own P2M levels are done (3% faster than
+    // waiting for the whole P2M: the join and fork cost two event hops).  M2P and M2QBXL, which need only
     // M2M, start after M2L: M2L then has the chip to itself and they fill the
     // idle slots of the L2L launches (measured 4% faster than starting them
     // beside M2L).
@@ -3813,17 +3818,22 @@ int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav 
                e2 = ctx->ev[4], eP = ctx->ev[5];
     double *L4 = ctx->ws_local2.p;
     QBX_TRY(zero_fill(P, np, st));
+    bool done;
     {
         hipStream_t sts[4] = {st, s1, s2, s3};
         hipEvent_t evs[4] = {e0, e1, e2, eP};
-        bool done;
         QBX_TRY(p2m_multistream(ctx, sts, 4, evs, T, R, d, str, nrm, M, &done));
         if (!done) QBX_TRY(qbx_form_multipoles(ctx, st, T, R, d, str, nrm, M));
     }
-    QBX_HIP(hipEventRecord(e0, st));
-    QBX_HIP(hipStreamWaitEvent(s1, e0, 0));
-    QBX_HIP(hipStreamWaitEvent(s2, e0, 0));
-    QBX_HIP(hipStreamWaitEvent(s3, e0, 0));
+    // after the multistream P2M each side stream has already waited for the
+    // zero fills and runs its near-field stage (which reads no multipole)
+    // right after its own P2M levels; only M2M waits for the join
+    if (!done) {
+        QBX_HIP(hipEventRecord(e0, st));
+        QBX_HIP(hipStreamWaitEvent(s1, e0, 0));
+        QBX_HIP(hipStreamWaitEvent(s2, e0, 0));
+        QBX_HIP(hipStreamWaitEvent(s3, e0, 0));
+    }
     QBX_TRY(qbx_eval_direct(ctx, s1, T, R, G, d, str, nrm, P, 1));
     QBX_TRY(qbx_form_global_qbx_locals(ctx, s2, T, R, G, d, str, nrm, Q, 0));
     QBX_TRY(qbx_form_locals(ctx, s3, T, R, d, str, nrm, L4, 0));
