@@ -3012,15 +3012,22 @@ static int p2m_multistream(qbx_ctx *ctx, hipStream_t *sts, int nst, hipEvent_t *
     QBX_TRY(zero_fill(mpole, (exp_bytes(kd)) / 8, sts[0]));
     QBX_HIP(hipEventRecord(evs[0], sts[0]));
     for (int k = 1; k < nst; ++k) QBX_HIP(hipStreamWaitEvent(sts[k], evs[0], 0));
-    // the busiest levels first, each to the next stream
+    // the busiest levels first, each to the stream with the least work so far
+    // (boxes x coefficients): a small level queued behind a big one would
+    // start only when the near field, issued meanwhile on the other streams,
+    // has dispatched all its workgroups
     std::vector<std::pair<long, int>> order;
     for (int l = 0; l < T->nlevels; ++l) {
         const long nb = R->h_level_start_source_box_nrs[l + 1] - R->h_level_start_source_box_nrs[l];
         if (nb > 0) order.push_back({-nb * (2L * kd.order[l] + 1), l});
     }
     std::sort(order.begin(), order.end());
-    for (size_t k = 0; k < order.size(); ++k)
-        p2m_level_h(sts[k % nst], kd, td, T, R, order[k].second, str, nrm, mpole);
+    std::vector<long> load(nst, 0);
+    for (size_t k = 0; k < order.size(); ++k) {
+        const int j = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[j] -= order[k].first;
+        p2m_level_h(sts[j], kd, td, T, R, order[k].second, str, nrm, mpole);
+    }
     QBX_HIP(hipGetLastError());
     for (int k = 1; k < nst; ++k) {
         QBX_HIP(hipEventRecord(evs[k], sts[k]));
