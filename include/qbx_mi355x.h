@@ -242,9 +242,14 @@ int qbx_finalize_potentials(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
                             const qbx_geo *geo, int nrhs, const double *pot,
                             double *out_user);
 
-/* Whole drive_fmm (fmm.py:414-594) in one call: stages in reference order on
- * `stream`, workspaces owned by ctx.  strengths_tree/normals_tree from
- * qbx_reorder_sources; out_user [ntargets - ncenters][nrhs] complex. */
+/* Whole drive_fmm (fmm.py:414-594) in one call: the stages run on `stream`
+ * and three side streams owned by ctx, forked from and joined back into
+ * `stream` by events (the result is ordered on `stream`; concurrent calls on
+ * one ctx race); workspaces owned by ctx.  M2M and L2L skip the coarse levels
+ * whose multipoles no stage reads and whose locals are zero, so the
+ * potentials are those of the stage-by-stage sequence.  strengths_tree/
+ * normals_tree from qbx_reorder_sources; out_user [ntargets - ncenters][nrhs]
+ * complex. */
 int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
                   const qbx_trav *trav, const qbx_geo *geo,
                   const qbx_kernel_desc *kd, const double *strengths_tree,
