@@ -3804,11 +3804,13 @@ int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav 
     // The stage DAG on four streams (the stage order of drive_fmm is kept
     // within each stream; every array has one writer stream at a time):
     //   main: zero P, M; P2M levels -(join)-> M2M -> M2L -[eM]-> (wait eP) L += L4
-    //         -> L2L -[eL]-> ... join
+    //         -> L2L -[eL]-> (wait e2) L2QBXL, QBXL2P -> (wait e1) finalize
     //   s3:   P2M levels, P2L into L4 (its own local buffer) -[eP]->
-    //   s1:   P2M levels, P2P, wait eM: M2P, wait eL: L2P  (non-QBX targets of P)
-    //   s2:   P2M levels, P2QBXL, wait eM: M2QBXL, wait eL: L2QBXL, QBXL2P
-    //         (Q, QBX targets of P)
+    //   s1:   P2M levels, P2P, wait eM: M2P, wait eL: L2P -[e1]->  (non-QBX
+    //         targets of P)
+    //   s2:   P2M levels, P2QBXL, wait eM: M2QBXL -[e2]->  (Q)
+    // (QBX targets of P are written on main: L2QBXL and QBXL2P follow L2L
+    // there with no event hop)
     // (the side streams start after the zero fills; without the multistream
     // P2M they wait for the whole P2M instead)
     // so the latency-bound level-by-level translations overlap the near field;
@@ -3852,19 +3854,18 @@ int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav 
     QBX_HIP(hipStreamWaitEvent(s2, eM, 0));
     QBX_TRY(qbx_eval_multipoles(ctx, s1, T, R, G, d, M, P, 1));
     QBX_TRY(qbx_translate_box_multipoles_to_qbx_local(ctx, s2, T, R, G, d, M, Q, 1));
+    QBX_HIP(hipEventRecord(e2, s2));
     QBX_HIP(hipStreamWaitEvent(st, eP, 0));
     QBX_TRY(add_into(L, L4, nexp * 2, st));
     QBX_TRY(refine_locals_from(ctx, st, T, R, d, L, R->local_min_level));
     QBX_HIP(hipEventRecord(eL, st));
     QBX_HIP(hipStreamWaitEvent(s1, eL, 0));
-    QBX_HIP(hipStreamWaitEvent(s2, eL, 0));
     QBX_TRY(qbx_eval_locals(ctx, s1, T, R, G, d, L, P, 1));
-    QBX_TRY(qbx_translate_box_local_to_qbx_local(ctx, s2, T, R, G, d, L, Q, 1));
-    QBX_TRY(qbx_eval_qbx_expansions(ctx, s2, T, G, d, Q, P));
     QBX_HIP(hipEventRecord(e1, s1));
-    QBX_HIP(hipEventRecord(e2, s2));
-    QBX_HIP(hipStreamWaitEvent(st, e1, 0));
     QBX_HIP(hipStreamWaitEvent(st, e2, 0));
+    QBX_TRY(qbx_translate_box_local_to_qbx_local(ctx, st, T, R, G, d, L, Q, 1));
+    QBX_TRY(qbx_eval_qbx_expansions(ctx, st, T, G, d, Q, P));
+    QBX_HIP(hipStreamWaitEvent(st, e1, 0));
     QBX_TRY(qbx_finalize_potentials(ctx, stream, T, G, nr, P, out_user));
     return QBX_OK;
 }
