@@ -762,8 +762,8 @@ def main_dirichlet64(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--grid", type=int, default=512)
     ap.add_argument("--nelements", type=int, default=4096)
     ap.add_argument("--tol", type=float, default=1e-10)
