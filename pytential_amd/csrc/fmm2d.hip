@@ -94,6 +94,33 @@ __device__ __forceinline__ bool flat_locate(const Flat &f, long gid, int &lv, in
     return true;
 }
 
+// Blocks over the boxes of several levels, each level's range padded to a
+// multiple of the 8 XCDs.  The dispatcher deals blocks to the XCDs in turn
+// (block b to XCD b % 8), so renumbering within a level gives each XCD one
+// contiguous eighth of every level (a bijection whatever the dispatch
+// order): neighbouring boxes, which read the same list-2 multipoles, share
+// one L2, and every XCD still gets its share of each level's work.
+struct FlatX {
+    int nl;
+    long tstart[FML + 1];  // padded block offset of each level
+    int istart[FML];       // first box (index into the level-ordered boxes)
+    int count[FML];        // boxes of the level
+};
+
+__device__ __forceinline__ bool flatx_locate(const FlatX &f, long b, int &lv, int &i)
+{
+    if (b >= f.tstart[f.nl]) return false;
+    int l = 0;
+    while (b >= f.tstart[l + 1]) ++l;
+    const long r = b - f.tstart[l];
+    const long per = (f.tstart[l + 1] - f.tstart[l]) / 8;
+    const long k = (r % 8) * per + r / 8;
+    if (k >= f.count[l]) return false;
+    lv = l;
+    i = f.istart[l] + (int)k;
+    return true;
+}
+
 __device__ __forceinline__ cplx boxc(const TD &t, int b)
 {
     return {t.centers[b], t.centers[(size_t)t.nboxes + b]};
@@ -784,12 +811,12 @@ constexpr int M2L_THREADS = 256;
 
 template <int NRC>
 __global__ void __launch_bounds__(M2L_THREADS)
-k_m2l_blk(KD kd, TD t, const double *B, Flat fl, const int *ttp, const int *starts,
+k_m2l_blk(KD kd, TD t, const double *B, FlatX fl, const int *ttp, const int *starts,
           const int *lists, const double *mpole, double *local)
 {
     __shared__ cplx red[M2L_THREADS * NRC];
-    int lv, i, unused;
-    if (!flat_locate(fl, blockIdx.x, lv, i, unused)) return;
+    int lv, i;
+    if (!flatx_locate(fl, blockIdx.x, lv, i)) return;
     const int e0 = starts[i], e1 = starts[i + 1];
     if (e0 == e1) return;
     const int ncl = kd.ncl[lv];
@@ -3338,8 +3365,19 @@ int qbx_multipole_to_local(qbx_ctx *ctx, void *stream, const qbx_tree *T, const 
     // one block per target-or-target-parent box
     for (int l = 0; l < kd.nlevels; ++l)
         QBX_CHECK(kd.ncl[l] <= M2L_THREADS, "M2L: %d coefficients exceed one block", kd.ncl[l]);
-    long nb;
-    const Flat fl = box_flat(0, &nb);
+    FlatX fl;
+    memset(&fl, 0, sizeof(fl));
+    fl.nl = kd.nlevels;
+    long nb = 0;
+    for (int l = 0; l < kd.nlevels; ++l) {
+        fl.tstart[l] = nb;
+        fl.istart[l] = R->h_level_start_ttp_box_nrs[l];
+        fl.count[l] = path(l) == 0 ? R->h_level_start_ttp_box_nrs[l + 1] -
+                                         R->h_level_start_ttp_box_nrs[l]
+                                   : 0;
+        nb += (fl.count[l] + 7) / 8 * 8;
+    }
+    fl.tstart[kd.nlevels] = nb;
     if (nb > 0) {
 #define L_M2L(N)                                                                           \
     hipLaunchKernelGGL(k_m2l_blk<N>, dim3(nb), dim3(M2L_THREADS), 0, st, kd, td, B, fl,    \
