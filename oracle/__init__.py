@@ -354,7 +354,7 @@ class GeometryData:
 
     def __init__(self, discr, targets, target_sides, *, max_leaf_refine_weight=64,
                  stick_out_factor=0.5, well_sep_is_n_away=2,
-                 from_sep_smaller_min_nsources_cumul=15,
+                 from_sep_smaller_min_nsources_cumul=None,
                  target_association_tolerance=1.0e3 * 2.220446049250313e-16):
         import time
         t0 = time.perf_counter()
@@ -382,6 +382,10 @@ class GeometryData:
         t1 = time.perf_counter()
         self.tree = Tree(discr.quad_nodes, self.all_targets, radii_t,
                          None, tw, max_leaf_refine_weight, stick_out_factor)
+        if from_sep_smaller_min_nsources_cumul is None:
+            # qbx/__init__.py:303-310: 15 for curves, 30 for surfaces
+            from_sep_smaller_min_nsources_cumul = 15 if discr.nodes.shape[0] == 2 else 30
+        self.from_sep_smaller_min_nsources_cumul = from_sep_smaller_min_nsources_cumul
         self.trav = Traversal(self.tree, well_sep_is_n_away, True,
                               from_sep_smaller_min_nsources_cumul)
         t2 = time.perf_counter()

@@ -39,6 +39,14 @@ def fmm_level_to_order_for_tolerance(kernel, k, tol, root_extent, nlevels,
     return orders
 
 
+class _NotProvided:
+    def __repr__(self):
+        return "NOT_PROVIDED"
+
+
+NOT_PROVIDED = _NotProvided()
+
+
 class QBXLayerPotentialSource:
     """QBX layer potential source for a panel discretization of a curve (2D)
     or a triangulated surface (3D)."""
@@ -81,22 +89,91 @@ class QBXLayerPotentialSource:
             else _max_leaf_refine_weight)
         self._expansion_stick_out_factor = _expansion_stick_out_factor
         self._well_sep_is_n_away = _well_sep_is_n_away
+        # qbx/__init__.py:303-310: 15 for curves, 30 for surfaces
         self._from_sep_smaller_min_nsources_cumul = (
-            15 if _from_sep_smaller_min_nsources_cumul is None
+            (15 if self.ambient_dim == 2 else 30)
+            if _from_sep_smaller_min_nsources_cumul is None
             else _from_sep_smaller_min_nsources_cumul)
         self._box_extent_norm = "l2" if _box_extent_norm is None else _box_extent_norm
         self._use_target_specific_qbx = bool(_use_target_specific_qbx)
-        self.ctx = Context.get(device)
-        d = density_discr
-        self.dev = {
-            "nodes": as_dev(d.nodes), "normals": as_dev(d.normals),
-            "tunnel": as_dev(d.tunnel_radii()), "centers": as_dev(d.centers()),
-            "center_radii": as_dev(d.center_radii()),
-            "element_to_source_starts": as_dev(d.element_to_source_starts(), torch.int32),
-            "quad_nodes": as_dev(d.quad_nodes), "quad_normals": as_dev(d.quad_normals),
-            "quad_weights": as_dev(d.quad_weights), "upsample": as_dev(d.upsample),
-        }
+        self._device = device
+        self._ctx = None
+        self._dev = None
         self._geo_cache = {}
+
+    @property
+    def ctx(self):
+        """The library context of this source's device (created on first use,
+        so constructing a source needs no GPU)."""
+        if self._ctx is None:
+            self._ctx = Context.get(self._device)
+        return self._ctx
+
+    @property
+    def dev(self):
+        """Device copies of the discretization arrays (made on first use)."""
+        if self._dev is None:
+            d = self.density_discr
+            self._dev = {
+                "nodes": as_dev(d.nodes), "normals": as_dev(d.normals),
+                "tunnel": as_dev(d.tunnel_radii()), "centers": as_dev(d.centers()),
+                "center_radii": as_dev(d.center_radii()),
+                "element_to_source_starts": as_dev(d.element_to_source_starts(),
+                                                   torch.int32),
+                "quad_nodes": as_dev(d.quad_nodes), "quad_normals": as_dev(d.quad_normals),
+                "quad_weights": as_dev(d.quad_weights), "upsample": as_dev(d.upsample),
+            }
+        return self._dev
+
+    def copy(self, density_discr=None, fine_order=None, qbx_order=None,
+             fmm_order=NOT_PROVIDED, fmm_level_to_order=NOT_PROVIDED,
+             target_association_tolerance=NOT_PROVIDED,
+             _expansions_in_tree_have_extent=NOT_PROVIDED,
+             _expansion_stick_out_factor=NOT_PROVIDED, _max_leaf_refine_weight=None,
+             _box_extent_norm=None, _use_target_specific_qbx=NOT_PROVIDED):
+        """A new source with some parameters replaced
+        (pytential/qbx/__init__.py:377-478): arguments left out keep this
+        source's values; ``fmm_order`` and ``fmm_level_to_order`` are mutually
+        exclusive (TypeError), and without either the level-to-order map is
+        kept.  ``None`` keeps the value for the arguments whose ``None`` the
+        reference reads as "not given"; ``False``/``0`` are valid values of
+        the ``NOT_PROVIDED`` ones."""
+        if fmm_order is not NOT_PROVIDED and fmm_level_to_order is not NOT_PROVIDED:
+            raise TypeError("may not specify both 'fmm_order' and 'fmm_level_to_order'")
+        kwargs = {}
+        if fmm_order is not NOT_PROVIDED:
+            kwargs["fmm_order"] = fmm_order
+        elif fmm_level_to_order is not NOT_PROVIDED:
+            kwargs["fmm_level_to_order"] = fmm_level_to_order
+        elif self._direct:
+            kwargs["fmm_order"] = False
+        else:
+            kwargs["fmm_level_to_order"] = self.fmm_level_to_order
+        return type(self)(
+            density_discr or self.density_discr,
+            fine_order=fine_order if fine_order is not None else self.fine_order,
+            qbx_order=qbx_order if qbx_order is not None else self.qbx_order,
+            target_association_tolerance=(
+                self.target_association_tolerance
+                if target_association_tolerance is NOT_PROVIDED
+                else target_association_tolerance),
+            _expansions_in_tree_have_extent=(
+                True if _expansions_in_tree_have_extent is NOT_PROVIDED
+                else _expansions_in_tree_have_extent),
+            _expansion_stick_out_factor=(
+                self._expansion_stick_out_factor
+                if _expansion_stick_out_factor is NOT_PROVIDED
+                else _expansion_stick_out_factor),
+            _well_sep_is_n_away=self._well_sep_is_n_away,
+            _max_leaf_refine_weight=_max_leaf_refine_weight or self._max_leaf_refine_weight,
+            _box_extent_norm=_box_extent_norm or self._box_extent_norm,
+            _from_sep_smaller_min_nsources_cumul=self._from_sep_smaller_min_nsources_cumul,
+            _use_target_specific_qbx=(
+                self._use_target_specific_qbx
+                if _use_target_specific_qbx is NOT_PROVIDED
+                else _use_target_specific_qbx),
+            device=self._device,
+            **kwargs)
 
     def qbx_fmm_geometry_data(self, target_discrs_and_qbx_sides):
         key = tuple((id(t) if not isinstance(t, str) else t,

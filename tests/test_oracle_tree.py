@@ -1,10 +1,15 @@
 """Pin the oracle's tree build, traversal and target association (CPU).
 
-* tree: structural invariants, stick-out placement, refine-weight splitting
+* tree: structural invariants, stick-out placement, and the refine-weight
+  split rule both ways (a box with children weighs more than
+  max_leaf_refine_weight, a leaf no more unless none of its particles can
+  descend), in 2D and 3D, plus closed-form box counts on uniform grids where
+  boxes weigh exactly the limit (a `>=` for `>` or a weighted QBX target
+  changes the count)
 * traversal: every list re-derived by brute force over all box pairs from
-  the written definitions, plus the constant-one completeness check of the
-  reference's test_cost_model.py:462-643 (every target box reaches every
-  source exactly once through lists 1-4)
+  the written definitions, in 2D and 3D, plus the constant-one completeness
+  check of the reference's test_cost_model.py:462-643 (every target box
+  reaches every source exactly once through lists 1-4)
 * association: brute force == accelerated, the property checks of
   test_global_qbx.py:464-506 and the forced failure of :511-566
 """
@@ -15,11 +20,12 @@ import oracle as O
 from pytential_amd import problems
 
 
-def _small_geometry(grid_n=24, nel=24):
+def _small_geometry(grid_n=24, nel=24, mins=None):
     cfg = problems.config1(nelements=nel, grid_n=grid_n)
     cfg = problems.refine_until_associated(cfg, O.assoc_status_and_flags)
     return cfg, O.GeometryData(cfg.discr, cfg.targets, cfg.target_sides,
-                               max_leaf_refine_weight=16)
+                               max_leaf_refine_weight=16,
+                               from_sep_smaller_min_nsources_cumul=mins)
 
 
 @pytest.fixture(scope="module")
@@ -27,12 +33,51 @@ def geo(oracle_lib):
     return _small_geometry()
 
 
+def _small_geometry3d(mins=None):
+    """icosphere(1) (80 triangles): on-surface targets on the exterior side
+    plus a shell of off-surface points, a small leaf weight so the tree has
+    several levels."""
+    from pytential_amd.surface import icosphere
+    discr = icosphere(1, order=4, fine_order=5)
+    rng = np.random.default_rng(7)
+    off = rng.normal(size=(3, 200))
+    rad = np.concatenate([rng.uniform(0.3, 0.7, 100), rng.uniform(1.4, 1.9, 100)])
+    off *= rad / np.linalg.norm(off, axis=0)
+    tg = np.concatenate([discr.nodes, off], axis=1)
+    sides = np.concatenate([np.ones(discr.ndofs, np.int32), np.zeros(200, np.int32)])
+    g = O.GeometryData(discr, tg, sides, max_leaf_refine_weight=32,
+                       from_sep_smaller_min_nsources_cumul=mins)
+    assert not np.any(g.target_status == 1)
+    return None, g
+
+
+@pytest.fixture(scope="module")
+def geo3(oracle_lib):
+    return _small_geometry3d()
+
+
+@pytest.fixture(params=["geo", "geo3"])
+def anygeo(request):
+    return request.getfixturevalue(request.param)
+
+
+@pytest.fixture(scope="module", params=["2d", "2d-unmerged", "3d", "3d-unmerged"])
+def listgeo(request, oracle_lib):
+    """The default merge of small list-3 boxes into list 1 (15 for curves,
+    30 for surfaces) hides most of list 3; the unmerged variants (threshold
+    0) exercise the list-3 separation test on every box."""
+    mins = 0 if request.param.endswith("unmerged") else None
+    if request.param.startswith("2d"):
+        return _small_geometry(mins=mins)
+    return _small_geometry3d(mins=mins)
+
+
 def _children(t, b):
     return [c for c in t.box_child_ids[:, b] if c >= 0]
 
 
-def test_tree_structure(geo):
-    cfg, g = geo
+def test_tree_structure(anygeo):
+    cfg, g = anygeo
     t = g.tree
     nb = t.nboxes
     assert t.box_parent_ids[0] == -1 and t.box_levels[0] == 0
@@ -41,7 +86,7 @@ def test_tree_structure(geo):
         assert t.box_levels[b] == t.box_levels[p] + 1
         assert b in _children(t, p)
         cidx = list(t.box_child_ids[:, p]).index(b)
-        for d in range(2):
+        for d in range(t.dimensions):
             assert t.box_coords[d, b] == 2 * t.box_coords[d, p] + ((cidx >> d) & 1)
     # boxes numbered level by level
     assert np.all(np.diff(t.box_levels) >= 0)
@@ -61,39 +106,111 @@ def test_tree_structure(geo):
                 assert starts[c] == pos
                 pos += cumul[c]
         assert np.all(cover == 1)
-    # sources only in leaves, all leaves at or below weight limit unless unsplittable
+    # sources only in leaves
     for b in range(nb):
         if _children(t, b):
             assert t.box_source_counts_nonchild[b] == 0
 
 
-def test_tree_particle_positions(geo):
-    cfg, g = geo
+def _target_weights_tree_order(g):
+    """Refine weights of the tree's targets in tree order: 1 for centers and
+    for targets needing no QBX, 0 for QBX targets (qbx/geometry.py:589-598)."""
+    tw = np.concatenate([np.ones(g.ncenters, np.int64),
+                         (g.target_to_center == -1).astype(np.int64)])
+    return tw[g.tree.sorted_target_ids]
+
+
+def _stop_levels(t, radii, stick_out=0.5, L=None):
+    """Deepest level whose box (and every coarser one on the way) holds a
+    particle of radius r under the l2 stick-out rule
+    |x - c| + r <= (1 + s) sqrt(d)/2 h, computed from the box geometry."""
+    d = t.dimensions
+    L = L or (28 if d == 2 else 19)
+    x = t.targets
+    q = np.floor((x - t.bounding_box_min[:, None]) / t.root_extent * 2.0 ** L)
+    q = np.clip(q, 0, 2 ** L - 1).astype(np.int64)
+    stop = np.full(x.shape[1], L)
+    alive = radii > 0
+    stop[alive] = 0
+    K = (1 + stick_out) * np.sqrt(d) / 2
+    for lev in range(1, L + 1):
+        if not alive.any():
+            break
+        h = t.root_extent * 2.0 ** -lev
+        c = t.bounding_box_min[:, None] + ((q >> (L - lev)) + 0.5) * h
+        dist = np.sqrt(np.sum((x - c) ** 2, axis=0))
+        fits = (K * h - radii >= 0) & (dist <= K * h - radii)
+        grow = alive & fits
+        stop[grow] = lev
+        alive &= fits
+    return stop
+
+
+def test_tree_split_rule(anygeo):
+    """The split rule both ways: a box with children weighs more than the
+    leaf limit; a leaf weighs at most the limit unless every particle in it
+    is stuck at its level (a target whose disk fits no child); targets that
+    stay in a box with children are exactly the stuck ones."""
+    cfg, g = anygeo
     t = g.tree
-    h = t.root_extent * 2.0 ** -t.box_levels
-    lo = t.bounding_box_min[:, None] + t.box_coords * h
+    W = 16 if t.dimensions == 2 else 32
+    tw = _target_weights_tree_order(g)
+    radii = np.concatenate([g.radii, np.zeros(t.ntargets - g.ncenters)])[t.sorted_target_ids]
+    stop = _stop_levels(t, radii)
+    tcum = np.concatenate([[0], np.cumsum(tw)])
+    nsplit = nleaf = 0
     for b in range(t.nboxes):
-        s0, n = t.box_source_starts[b], t.box_source_counts_nonchild[b]
-        p = t.sources[:, s0:s0 + n]
-        assert np.all(p >= lo[:, b:b + 1] - 1e-14) and np.all(p <= lo[:, b:b + 1] + h[b] + 1e-14)
-    # centers stick out by at most the l2 spec: |x-c| + r <= 1.5 * sqrt(2)/2 * h
-    nc = g.ncenters
-    radii = np.concatenate([g.radii, np.zeros(t.ntargets - nc)])[t.sorted_target_ids]
-    for b in range(t.nboxes):
-        s0, n = t.box_target_starts[b], t.box_target_counts_nonchild[b]
-        if n == 0 or b == 0:
-            continue
-        x = t.targets[:, s0:s0 + n]
-        r = radii[s0:s0 + n]
-        dist = np.hypot(*(x - t.box_centers[:, b:b + 1]))
-        assert np.all(dist + r <= 1.5 * np.sqrt(2) / 2 * h[b] * (1 + 1e-12))
+        s0, nt = t.box_target_starts[b], t.box_target_counts_cumul[b]
+        weight = t.box_source_counts_cumul[b] + tcum[s0 + nt] - tcum[s0]
+        lev = t.box_levels[b]
+        own = slice(t.box_target_starts[b],
+                    t.box_target_starts[b] + t.box_target_counts_nonchild[b])
+        if _children(t, b):
+            nsplit += 1
+            assert weight > W, (b, weight)
+            assert np.all(stop[own] == lev), b
+        else:
+            nleaf += 1
+            assert np.all(stop[own] >= lev), b
+            if weight > W:
+                assert t.box_source_counts_nonchild[b] == 0
+                assert np.all(stop[own] == lev), (b, weight)
+    assert nsplit > 3 and nleaf > 3
+
+
+def _uniform_grid(dim, m):
+    g = np.arange(m) / (m - 1.0)
+    return np.stack([a.ravel() for a in np.meshgrid(*([g] * dim), indexing="ij")])
+
+
+@pytest.mark.parametrize("dim,m,W,counts", [
+    # 16 x 16 sources on [0,1]^2: each level-2 box holds exactly 16
+    (2, 16, 16, [1, 4, 16]), (2, 16, 15, [1, 4, 16, 64]), (2, 16, 64, [1, 4]),
+    # 8^3 sources: each level-2 box holds exactly 8
+    (3, 8, 8, [1, 8, 64]), (3, 8, 7, [1, 8, 64, 512]), (3, 8, 64, [1, 8])])
+def test_tree_split_closed_form(oracle_lib, dim, m, W, counts):
+    """Boxes of a uniform grid weigh exactly the limit at one level: they are
+    leaves (split only when weight > limit).  Zero-weight targets (QBX
+    targets) beside every source never cause a split; weight-one targets do."""
+    src = _uniform_grid(dim, m)
+    none = np.zeros((dim, 0))
+    t = O.Tree(src, none, None, None, None, W, 0.5)
+    assert list(np.bincount(t.box_levels)) == counts
+    # zero-weight targets at the sources: same tree
+    t0 = O.Tree(src, src.copy(), None, None, np.zeros(src.shape[1], np.int32), W, 0.5)
+    assert list(np.bincount(t0.box_levels)) == counts
+    # one weight-one target per source doubles every box's weight: leaves
+    # that now exceed the limit split once more
+    t1 = O.Tree(src, src.copy(), None, None, np.ones(src.shape[1], np.int32), W, 0.5)
+    leaf_weight = m ** dim // counts[-1]
+    assert t1.nlevels == len(counts) + (1 if 2 * leaf_weight > W else 0)
 
 
 def _gap(t, a, b, grow=0):
     la, lb = t.box_levels[a], t.box_levels[b]
     m = max(la, lb)
     g = 0
-    for d in range(2):
+    for d in range(t.dimensions):
         sa, sb = 2 ** (m - la), 2 ** (m - lb)
         alo, ahi = t.box_coords[d, a] * sa, (t.box_coords[d, a] + 1) * sa
         blo, bhi = (t.box_coords[d, b] - grow) * sb, (t.box_coords[d, b] + 1 + grow) * sb
@@ -123,12 +240,15 @@ def _csr(starts, lists, i):
     return sorted(lists[starts[i]:starts[i + 1]])
 
 
-def test_lists_match_brute_force_definitions(geo):
+def test_lists_match_brute_force_definitions(listgeo):
     """Re-derive lists 1-4 from their definitions by brute force (n = 2,
-    targets with extent, min_nsources_cumul = 15) and compare exactly."""
-    cfg, g = geo
+    targets with extent, min_nsources_cumul as the geometry's) and compare
+    exactly."""
+    cfg, g = listgeo
     t, tr = g.tree, g.trav
-    n, e, mins = 2, 1, 15
+    n, e, mins = 2, 1, g.from_sep_smaller_min_nsources_cumul
+    if mins == 0:
+        assert len(tr.from_sep_smaller_lists) > 20
     leaf = np.array([not _children(t, b) for b in range(t.nboxes)])
     srcleaf = leaf & (t.box_source_counts_nonchild > 0)
     for it, B in enumerate(tr.target_boxes):
