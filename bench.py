@@ -340,19 +340,8 @@ def cpu_baseline(args, cfg, orders, sample_idx):
 def sphere_densities(discr, nrhs, seed=1002):
     """nrhs densities: real spherical harmonics Y_lm (l <= 6, in order) plus
     N(0, 0.01) noise, seeded."""
-    from scipy.special import sph_harm_y
-    x = discr.nodes
-    r = np.linalg.norm(x, axis=0)
-    th, ph = np.arccos(np.clip(x[2] / r, -1, 1)), np.arctan2(x[1], x[0])
-    lm = [(l, m) for l in range(7) for m in range(-l, l + 1)]
-    rng = np.random.default_rng(seed)
-    cols = []
-    for i in range(nrhs):
-        l, m = lm[i % len(lm)]
-        y = sph_harm_y(l, abs(m), th, ph)
-        y = y.real if m >= 0 else y.imag
-        cols.append(y + 0.01 * rng.standard_normal(len(y)))
-    return np.stack(cols, axis=1)
+    from pytential_amd.problems import sphere_harmonic_densities
+    return sphere_harmonic_densities(discr.nodes, nrhs, seed=seed)[0]
 
 
 def main_sphere3d(args):
@@ -663,17 +652,13 @@ def main_dirichlet64(args):
     from pytential_amd.linalg import MatVecGraph, gmres_many
     from pytential_amd.pde import DirichletOperator
     from pytential_amd.qbx import QBXLayerPotentialSource
-    rng = np.random.default_rng(1003)
+    from pytential_amd.problems import dirichlet_curves
+    specs, rng = dirichlet_curves(args.ncurves, 1003)
     t0 = time.perf_counter()
     curves = []
-    for _ in range(args.ncurves):
-        arms = int(rng.integers(3, 10))
-        rad = float(rng.uniform(0.5, 2.0))
-        nel = int(round(np.exp(rng.uniform(np.log(256), np.log(4096)))))
-        phase = float(rng.uniform(0, 2 * np.pi))
-        ang = rng.uniform(0, 2 * np.pi)
-        x0 = 1.6 * rad * np.array([np.cos(ang), np.sin(ang)])
-        d = discretize_curve(starfish(arms, 0.25, rad, phase=phase), nel, 8, 16)
+    for sp in specs:
+        rad, nel, x0 = sp["radius"], sp["nelements"], sp["charge"]
+        d = discretize_curve(starfish(sp["arms"], 0.25, rad, phase=sp["phase"]), nel, 8, 16)
         src = QBXLayerPotentialSource(d, fine_order=16, qbx_order=5, fmm_order=15)
         op = DirichletOperator(src, loc_sign=-1)
         f = -np.log(np.hypot(d.nodes[0] - x0[0], d.nodes[1] - x0[1])) / (2 * np.pi)

@@ -50,6 +50,15 @@ typedef struct qbx_geo qbx_geo;
 /* ---- context / errors --------------------------------------------------- */
 int qbx_ctx_create(qbx_ctx **ctx, int device);
 int qbx_ctx_destroy(qbx_ctx *ctx);
+/* A context owns the FMM workspaces, translation tables and scratch that the
+ * stage entry points and qbx_drive_fmm use.  Once a HIP stream capture has
+ * recorded work that uses them, the context keeps them fixed: a later call
+ * that would reallocate or rebuild them fails (QBX_ERR_INVALID) instead of
+ * leaving the graph with freed memory or another operator's tables.  Give
+ * each captured operator its own context.  qbx_ctx_release_graphs tells the
+ * context that no graph captured through it is replayed any more.  (No
+ * reference counterpart: the reference has no graph capture.) */
+int qbx_ctx_release_graphs(qbx_ctx *ctx);
 const char *qbx_last_error(void);
 int qbx_version(void);
 

@@ -516,6 +516,36 @@ def helmholtz3d_far_operator(discr, targets, k):
     return (G * discr.quad_weights[None, :]) @ U
 
 
+def _upsample_blocks(discr, v):
+    """U v for nodal v [ndofs] -> quad-stage-2 values, element by element."""
+    ne, n1 = discr.nelements, discr.nodes_per_element
+    return (np.asarray(v).reshape(ne, n1) @ discr.upsample.T).reshape(-1)
+
+
+def helmholtz3d_far_apply(discr, targets, k, sigma, chunk=256):
+    """A sigma with A = G_k W U, the rows of G_k built ``chunk`` targets at a
+    time (the dense operator of ``helmholtz3d_far_operator`` without storing
+    it)."""
+    q = discr.quad_weights * _upsample_blocks(discr, sigma)
+    t = np.asarray(targets, np.float64)
+    out = np.empty(t.shape[1], np.complex128)
+    for a in range(0, t.shape[1], chunk):
+        out[a:a + chunk] = helmholtz3d_dense(discr.quad_nodes, t[:, a:a + chunk], k) @ q
+    return out
+
+
+def helmholtz3d_far_adjoint(discr, targets, k, y, chunk=256):
+    """A^H y = U^T W conj(G_k)^T y, with the rows of G_k built ``chunk``
+    targets at a time."""
+    t = np.asarray(targets, np.float64)
+    acc = np.zeros(discr.quad_nodes.shape[1], np.complex128)
+    for a in range(0, t.shape[1], chunk):
+        acc += helmholtz3d_dense(discr.quad_nodes, t[:, a:a + chunk], k).conj().T @ y[a:a + chunk]
+    acc *= discr.quad_weights
+    ne, n1 = discr.nelements, discr.nodes_per_element
+    return (acc.reshape(ne, -1) @ discr.upsample).reshape(ne * n1)
+
+
 def sphere_single_layer_exterior(l, k, a, r):
     """Exterior value of the Helmholtz single layer of Y_lm on the sphere of
     radius a at distance r > a: S_k Y_lm = i k a^2 j_l(k a) h_l(k r) Y_lm,

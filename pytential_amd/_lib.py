@@ -41,6 +41,7 @@ _lib = None
 _SIGS = {
     "qbx_ctx_create": [ct.POINTER(P), I],
     "qbx_ctx_destroy": [P],
+    "qbx_ctx_release_graphs": [P],
     "qbx_version": [],
     "qbx_tree_build": [P, P, I, I, P, I, P, P, P, P, I, D, I, ct.POINTER(P)],
     "qbx_tree_destroy": [P],

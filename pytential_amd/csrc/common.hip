@@ -2,8 +2,11 @@
 // small elementwise kernels at the boundary (source reorder, upsampling).
 #include <hipcub/hipcub.hpp>
 
+#include <array>
 #include <cstdarg>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 #include "qbx_internal.h"
 
@@ -21,9 +24,47 @@ void set_error(const char *fmt, ...)
     va_end(ap);
 }
 
-int scratch_reserve(qbx_ctx *ctx, size_t bytes, void **p)
+int side_streams(int device, hipStream_t out[3])
 {
+    static std::mutex mu;
+    static std::map<int, std::array<hipStream_t, 3>> pool;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = pool.find(device);
+    if (it == pool.end()) {
+        std::array<hipStream_t, 3> s{};
+        for (hipStream_t &x : s) QBX_HIP(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+        it = pool.emplace(device, s).first;
+    }
+    for (int i = 0; i < 3; ++i) out[i] = it->second[i];
+    return QBX_OK;
+}
+
+int note_capture(qbx_ctx *ctx, hipStream_t st)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    QBX_HIP(hipStreamIsCapturing(st, &cs));
+    if (cs != hipStreamCaptureStatusNone) ctx->graph_held = true;
+    return QBX_OK;
+}
+
+int may_replace(qbx_ctx *ctx, hipStream_t st, const char *what)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    QBX_HIP(hipStreamIsCapturing(st, &cs));
+    QBX_CHECK(!ctx->graph_held && cs == hipStreamCaptureStatusNone,
+              "%s: the context's %s would be reallocated or rebuilt, but a HIP graph "
+              "captured through this context holds the current ones; give each captured "
+              "operator its own context (qbx_ctx_create), or warm it up before capturing "
+              "and call qbx_ctx_release_graphs after destroying its graphs",
+              "qbx context", what);
+    return QBX_OK;
+}
+
+int scratch_reserve(qbx_ctx *ctx, hipStream_t st, size_t bytes, void **p)
+{
+    QBX_TRY(note_capture(ctx, st));
     if (bytes > ctx->scratch_bytes) {
+        QBX_TRY(may_replace(ctx, st, "scratch buffer"));
         if (ctx->scratch) {
             QBX_HIP(hipDeviceSynchronize());
             QBX_HIP(hipFree(ctx->scratch));
@@ -47,7 +88,7 @@ int exclusive_scan(qbx_ctx *ctx, hipStream_t st, const int *in, int *out, int n,
     size_t tb = 0;
     QBX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, n, st));
     void *tmp;
-    QBX_TRY(scratch_reserve(ctx, tb, &tmp));
+    QBX_TRY(scratch_reserve(ctx, st, tb, &tmp));
     QBX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb, in, out, n, st));
     // out[n] = out[n-1] + in[n-1]
     int last_out = 0, last_in = 0;
@@ -69,7 +110,7 @@ int sort_pairs_u64(qbx_ctx *ctx, hipStream_t st, const unsigned long long *kin,
     QBX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kin, kout, vin, vout, n, 0,
                                                 end_bit, st));
     void *tmp;
-    QBX_TRY(scratch_reserve(ctx, tb, &tmp));
+    QBX_TRY(scratch_reserve(ctx, st, tb, &tmp));
     QBX_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, kin, kout, vin, vout, n, 0, end_bit,
                                                 st));
     return QBX_OK;
@@ -83,7 +124,7 @@ int sort_pairs_i32(qbx_ctx *ctx, hipStream_t st, const int *kin, int *kout, cons
     QBX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kin, kout, vin, vout, n, 0,
                                                 end_bit, st));
     void *tmp;
-    QBX_TRY(scratch_reserve(ctx, tb, &tmp));
+    QBX_TRY(scratch_reserve(ctx, st, tb, &tmp));
     QBX_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tb, kin, kout, vin, vout, n, 0, end_bit,
                                                 st));
     return QBX_OK;
@@ -204,14 +245,19 @@ int qbx_ctx_create(qbx_ctx **ctx, int device)
     return QBX_OK;
 }
 
+int qbx_ctx_release_graphs(qbx_ctx *ctx)
+{
+    QBX_CHECK(ctx, "qbx_ctx_release_graphs: null context");
+    ctx->graph_held = false;
+    return QBX_OK;
+}
+
 int qbx_ctx_destroy(qbx_ctx *ctx)
 {
     if (!ctx) return QBX_OK;
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     for (hipEvent_t &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
-    for (hipStream_t &s : ctx->side)
-        if (s) (void)hipStreamDestroy(s);
     delete ctx;
     return QBX_OK;
 }

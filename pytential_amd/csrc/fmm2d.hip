@@ -2865,7 +2865,10 @@ static int ensure_tables(qbx_ctx *ctx, hipStream_t st, const qbx_tree *T, KD *kd
     const int ncls = (2 * kd->m2l_w + 1) * (2 * kd->m2l_w + 1);
     for (int l = 0; l < kd->nlevels; ++l) nm2l += (long)ncls * (4 * kd->order[l] + 1);
     for (int l = 1; l < kd->nlevels; ++l) nmm += 4L * (2 * (kd->order[l] + kd->order[l - 1]) + 1);
+    QBX_TRY(note_capture(ctx, st));
     if (key != ctx->tab_key) {
+        // a graph replays the tables it captured: never rebuild under one
+        QBX_TRY(may_replace(ctx, st, "Helmholtz translation tables"));
         if (ctx->ws_tab_m2l.n < 2 * nm2l) QBX_TRY(ctx->ws_tab_m2l.alloc(2 * nm2l));
         if (ctx->ws_tab_mm.n < 2 * std::max(nmm, 1L))
             QBX_TRY(ctx->ws_tab_mm.alloc(2 * std::max(nmm, 1L)));
@@ -3829,14 +3832,14 @@ int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav 
     const long nm = 2L * std::max(nexp, 1L);
     const long nq = 2L * std::max(G->ncenters, 1) * ncq * nr;
     const long np = 2L * std::max(T->ntargets, 1) * nr;
-    if (ctx->ws_mpole.n < nm) QBX_TRY(ctx->ws_mpole.alloc(nm));
-    if (ctx->ws_local.n < nm) QBX_TRY(ctx->ws_local.alloc(nm));
-    if (ctx->ws_local2.n < nm) QBX_TRY(ctx->ws_local2.alloc(nm));
-    if (ctx->ws_qbx.n < nq) QBX_TRY(ctx->ws_qbx.alloc(nq));
-    if (ctx->ws_pot.n < np) QBX_TRY(ctx->ws_pot.alloc(np));
+    QBX_TRY(ws_reserve(ctx, S(stream), ctx->ws_mpole, nm, "multipole workspace"));
+    QBX_TRY(ws_reserve(ctx, S(stream), ctx->ws_local, nm, "local workspace"));
+    QBX_TRY(ws_reserve(ctx, S(stream), ctx->ws_local2, nm, "list-4 local workspace"));
+    QBX_TRY(ws_reserve(ctx, S(stream), ctx->ws_qbx, nq, "QBX expansion workspace"));
+    QBX_TRY(ws_reserve(ctx, S(stream), ctx->ws_pot, np, "potential workspace"));
     double *M = ctx->ws_mpole.p, *L = ctx->ws_local.p, *Q = ctx->ws_qbx.p, *P = ctx->ws_pot.p;
     if (!ctx->side[0]) {
-        for (hipStream_t &s : ctx->side) QBX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        QBX_TRY(side_streams(ctx->device, ctx->side));
         for (hipEvent_t &e : ctx->ev) QBX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
     // The stage DAG on four streams (the stage order of drive_fmm is kept

@@ -1647,10 +1647,13 @@ static int make_kd(const qbx_kernel_desc *d, const qbx_tree *T, KD *kd, int nsep
 }
 
 // unit-offset tables, cached in the context per (nsep, pmax)
-static int ensure_tables(qbx_ctx *ctx, KD *kd)
+static int ensure_tables(qbx_ctx *ctx, hipStream_t st, KD *kd)
 {
     std::vector<double> key = {3.0, (double)kd->nsep, (double)kd->pmax};
+    QBX_TRY(note_capture(ctx, st));
     if (key != ctx->tab3_key) {
+        // a graph replays the tables it captured: never rebuild under one
+        QBX_TRY(may_replace(ctx, st, "3D translation tables"));
         const int W = kd->W, S1 = 2 * W + 1, tn = kd->tabn;
         const long ncls = (long)S1 * S1 * S1;
         std::vector<zc> tab((size_t)ncls * tn * tn, zc(0.0, 0.0));
@@ -1877,7 +1880,7 @@ int f3_coarsen_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const q
 {
     KD kd;
     QBX_TRY(make_kd(d, T, &kd, R->nsep));
-    QBX_TRY(ensure_tables(ctx, &kd));
+    QBX_TRY(ensure_tables(ctx, S(stream), &kd));
     hipStream_t st = S(stream);
     const TD td = make_td(T);
     for (int l = T->nlevels - 2; l >= std::max(lmin, 0); --l) {
@@ -1935,7 +1938,7 @@ int f3_multipole_to_local(qbx_ctx *ctx, void *stream, const qbx_tree *T, const q
 {
     KD kd;
     QBX_TRY(make_kd(d, T, &kd, R->nsep));
-    QBX_TRY(ensure_tables(ctx, &kd));
+    QBX_TRY(ensure_tables(ctx, S(stream), &kd));
     hipStream_t st = S(stream);
     QBX_TRY(zero_fill(local, (exp_bytes(kd)) / 8, st));
     if (R->nttp_boxes > 0 && R->n2 > 0 && kd.nr >= 4 && kd.pmax <= 15) {
@@ -2031,7 +2034,7 @@ int f3_refine_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_tr
 {
     KD kd;
     QBX_TRY(make_kd(d, T, &kd, R->nsep));
-    QBX_TRY(ensure_tables(ctx, &kd));
+    QBX_TRY(ensure_tables(ctx, S(stream), &kd));
     hipStream_t st = S(stream);
     const TD td = make_td(T);
     for (int l = std::max(lmin, 0) + 1; l < T->nlevels; ++l) {

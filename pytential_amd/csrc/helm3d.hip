@@ -181,7 +181,7 @@ int qbx_helmholtz3d_p2p(qbx_ctx *ctx, void *stream, int nsources, const double *
     nchunks = cdiv(nsources, chunk);
     const size_t pb = sizeof(double) * 2 * (size_t)nchunks * ntargets;
     void *scr;
-    QBX_TRY(scratch_reserve(ctx, dpot_dk ? 2 * pb : pb, &scr));
+    QBX_TRY(scratch_reserve(ctx, st, dpot_dk ? 2 * pb : pb, &scr));
     double *ppot = (double *)scr;
     double *pdk = dpot_dk ? ppot + 2 * (size_t)nchunks * ntargets : nullptr;
     hipLaunchKernelGGL(k_h3_p2p_partial, dim3(tiles, nchunks), dim3(H3_THREADS), 0, st, nsources,

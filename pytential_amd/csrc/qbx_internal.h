@@ -96,6 +96,11 @@ struct qbx_ctx {
     // stream-ordered scratch (grown on demand, never shrunk)
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    // set once a stream capture (HIP graph) has recorded kernels that use this
+    // context's scratch, workspaces or tables: from then on those buffers are
+    // neither reallocated nor rebuilt (a graph replays the raw pointers and
+    // table contents it captured); qbx_ctx_release_graphs clears it
+    bool graph_held = false;
     // FMM workspaces for qbx_drive_fmm
     qbx::dbuf<double> ws_mpole, ws_local, ws_local2, ws_qbx, ws_pot;
     // Helmholtz translation tables and the (tree, k, orders) they were built for
@@ -104,8 +109,9 @@ struct qbx_ctx {
     // 3D Laplace unit-offset harmonic tables and the (nsep, pmax) they hold
     qbx::dbuf<double> ws_tab3_m2l, ws_tab3_ch;
     std::vector<double> tab3_key;
-    // side streams (default priority) and fork/join events of qbx_drive_fmm,
-    // created on its first call
+    // side streams of qbx_drive_fmm (default priority; shared by every
+    // context on the device, see side_streams) and its fork/join events
+    // (this context's own), set up on its first call
     hipStream_t side[3] = {nullptr, nullptr, nullptr};
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 };
@@ -166,7 +172,26 @@ struct qbx_geo {
 };
 
 namespace qbx {
-int scratch_reserve(qbx_ctx *ctx, size_t bytes, void **p);
+int scratch_reserve(qbx_ctx *ctx, hipStream_t st, size_t bytes, void **p);
+// the three side streams of the device's fused drivers, created once per
+// device and kept until the process ends: contexts share them, so a process
+// with many contexts (one per captured operator) does not spread its
+// streams over the hardware queues the main stream uses
+int side_streams(int device, hipStream_t out[3]);
+// marks the context as held by a graph when st is capturing
+int note_capture(qbx_ctx *ctx, hipStream_t st);
+// QBX_OK when the context's buffers may be replaced (no graph holds them),
+// else an error naming `what`; call before every reallocation or rebuild
+int may_replace(qbx_ctx *ctx, hipStream_t st, const char *what);
+// grows a context workspace to at least n elements (never shrinks it)
+template <class T>
+int ws_reserve(qbx_ctx *ctx, hipStream_t st, dbuf<T> &b, long n, const char *what)
+{
+    QBX_TRY(note_capture(ctx, st));
+    if (b.n >= n) return QBX_OK;
+    QBX_TRY(may_replace(ctx, st, what));
+    return b.alloc(n);
+}
 // exclusive scan of int flags/counts (n items) into out (n+1 items), returns total via host copy
 int exclusive_scan(qbx_ctx *ctx, hipStream_t st, const int *in, int *out, int n, int *total);
 // stable sort pairs by 64-bit or 32-bit key

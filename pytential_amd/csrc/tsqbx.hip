@@ -639,7 +639,7 @@ int direct_qbx3d(qbx_ctx *ctx, hipStream_t st, const qbx_tree *T, const qbx_geo 
         QBX_CHECK(G->trav, "direct 3D QBX: geometry without a traversal");
         const int ntb = G->trav->ntarget_boxes;
         void *scr;
-        QBX_TRY(scratch_reserve(ctx, sizeof(int) * (2L * ntb + 3), &scr));
+        QBX_TRY(scratch_reserve(ctx, st, sizeof(int) * (2L * ntb + 3), &scr));
         int *vl = (int *)scr;
         hipLaunchKernelGGL(k_virtual_lists, dim3(cdiv(ntb + 1, 256)), dim3(256), 0, st, ntb,
                            T->nsources, vl);

@@ -25,12 +25,14 @@ class QBXExpansionWrangler:
     """Stages of the QBX FMM on one tree (pytential/qbx/fmm.py:118-396)."""
 
     def __init__(self, geo_data, kernel, k, source_kind, qbx_order, level_orders,
-                 output="pot", use_target_specific_qbx=False, nrhs=1):
+                 output="pot", use_target_specific_qbx=False, nrhs=1, ctx=None):
         self.geo_data = geo_data
         self.tree = geo_data.tree()
         self.traversal = geo_data.traversal()
         self.geo = geo_data._geo
-        self.ctx = geo_data.ctx
+        # the context whose workspaces and tables the stages use (the
+        # geometry's shared one unless the caller gives the wrangler its own)
+        self.ctx = geo_data.ctx if ctx is None else ctx
         self.kernel = kernel
         self.dim = self.tree.dimensions
         self.k = complex(k)

@@ -25,7 +25,12 @@ def _stream():
 
 
 class Context:
-    """One library context per device (owns FMM workspaces and scratch)."""
+    """A library context: owns the FMM workspaces, translation tables and
+    scratch of the calls made through it.  ``Context.get`` returns the one
+    shared per device; an operator captured into a HIP graph gets its own
+    (``Context(device)``), because a graph replays the workspace pointers and
+    table contents it captured and the library refuses to replace them once
+    captured."""
     _by_device: dict = {}
 
     def __init__(self, device=0):
@@ -41,6 +46,11 @@ class Context:
         if device not in cls._by_device:
             cls._by_device[device] = Context(device)
         return cls._by_device[device]
+
+    def release_graphs(self):
+        """No graph captured through this context is replayed any more: its
+        workspaces and tables may be replaced again."""
+        call("qbx_ctx_release_graphs", self.h)
 
     def __del__(self):
         try:

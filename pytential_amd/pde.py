@@ -47,11 +47,14 @@ class DirichletOperator:
         self.ones = kernel == "laplace" and loc_sign > 0
         # D at both one-sided limits in one evaluation
         self.geo_d = lpot_source.qbx_fmm_geometry_data((("self", +1), ("self", -1)))
-        self.w_d = lpot_source.make_wrangler(self.geo_d, kernel, k, "D")
+        # each wrangler owns its workspaces and tables, so matvecs of several
+        # operators can be captured into HIP graphs and replayed in any order
+        self.w_d = lpot_source.make_wrangler(self.geo_d, kernel, k, "D", private_context=True)
         self.w_s = None
         if self.alpha != 0:
             self.geo_s = lpot_source.qbx_fmm_geometry_data((("self", +1),))
-            self.w_s = lpot_source.make_wrangler(self.geo_s, kernel, k, "S")
+            self.w_s = lpot_source.make_wrangler(self.geo_s, kernel, k, "S",
+                                                 private_context=True)
         self.dtype = torch.float64 if (kernel == "laplace" and self.alpha == 0) \
             else torch.complex128
 

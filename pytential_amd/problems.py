@@ -99,3 +99,52 @@ def refine_until_associated(problem: QBXProblem, assoc_fn, max_iter: int = 8) ->
         discr, conn = discr.refined(flags)
         dens = conn(dens)
     raise RuntimeError("target association still fails after refinement")
+
+
+def sphere_harmonic_densities(nodes, nrhs, noise=0.01, seed=1002):
+    """Config-3 densities on surface nodes (3, n): real spherical harmonics
+    Y_lm (l <= 6, taken in order l = 0.., m = -l..l, cyclically) plus
+    N(0, noise^2) noise from ``default_rng(seed)``.  Returns (n, nrhs) and
+    the degree l of each column."""
+    from scipy.special import sph_harm_y
+    x = np.asarray(nodes)
+    r = np.linalg.norm(x, axis=0)
+    th, ph = np.arccos(np.clip(x[2] / r, -1, 1)), np.arctan2(x[1], x[0])
+    lm = [(l, m) for l in range(7) for m in range(-l, l + 1)]
+    rng = np.random.default_rng(seed)
+    cols, degs = [], []
+    for i in range(nrhs):
+        l, m = lm[i % len(lm)]
+        y = sph_harm_y(l, abs(m), th, ph)
+        y = y.real if m >= 0 else y.imag
+        cols.append(y + noise * rng.standard_normal(len(y)) if noise else y)
+        degs.append(l)
+    return np.stack(cols, axis=1), np.array(degs)
+
+
+def dirichlet_curves(ncurves=64, seed=1003):
+    """Config-4 curve specs: ``ncurves`` starfish (arms 3..9, radius 0.5..2,
+    amplitude 0.25, element counts log-uniform in 256..4096, random phase)
+    and one point charge outside each, from ``default_rng(seed)``.  Returns a
+    list of dicts (arms, radius, nelements, phase, charge) and the generator
+    (callers draw their check points from it)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(ncurves):
+        arms = int(rng.integers(3, 10))
+        rad = float(rng.uniform(0.5, 2.0))
+        nel = int(round(np.exp(rng.uniform(np.log(256), np.log(4096)))))
+        phase = float(rng.uniform(0, 2 * np.pi))
+        ang = rng.uniform(0, 2 * np.pi)
+        x0 = 1.6 * rad * np.array([np.cos(ang), np.sin(ang)])
+        out.append({"arms": arms, "radius": rad, "nelements": nel, "phase": phase,
+                    "charge": x0})
+    return out, rng
+
+
+def exterior_shell_targets(n, rng, rmin=2.0, rmax=3.0):
+    """Config-5 targets: n points with uniform directions and radius uniform
+    in [rmin, rmax]."""
+    v = rng.normal(size=(3, n))
+    v /= np.linalg.norm(v, axis=0)
+    return v * rng.uniform(rmin, rmax, n)

@@ -206,11 +206,15 @@ class QBXLayerPotentialSource:
              _p(self.dev["upsample"]), _p(dens.contiguous()), _p(out))
         return out
 
-    def make_wrangler(self, geo_data, kernel, k, source_kind, nrhs=1):
+    def make_wrangler(self, geo_data, kernel, k, source_kind, nrhs=1, private_context=False):
+        """The wrangler of one evaluation.  ``private_context`` gives it a
+        library context of its own (its own workspaces and translation
+        tables), as an operator captured into a HIP graph needs."""
         tree = geo_data.tree()
+        ctx = Context(self.ctx.device) if private_context else None
         return _fmm.QBXExpansionWrangler(
             geo_data, kernel, k, source_kind, self.qbx_order, self.level_orders(kernel, k, tree),
-            use_target_specific_qbx=self._use_target_specific_qbx, nrhs=nrhs)
+            use_target_specific_qbx=self._use_target_specific_qbx, nrhs=nrhs, ctx=ctx)
 
     def exec_compute_potential(self, kernel, density, targets_and_sides, k=0.0,
                                source_kind="S", fused=True):

@@ -64,3 +64,76 @@ def test_interior_dirichlet_point_charge():
     err = np.max(np.abs(u - exact)) / np.max(np.abs(exact))
     print(f"[gmres] interior field rel err {err:.2e}", flush=True)
     assert err < 1e-6
+
+
+def test_two_captured_helmholtz_operators_interleaved():
+    """Two Helmholtz Dirichlet operators (different curves, sizes and k, each
+    with a D and an S wrangler on their own trees) captured into HIP graphs
+    and replayed interleaved: every replay equals the eager matvec, so no
+    graph reads workspaces or translation tables another operator replaced."""
+    from pytential_amd.discretization import discretize_curve, starfish
+    from pytential_amd.linalg import MatVecGraph
+    from pytential_amd.pde import DirichletOperator
+    from pytential_amd.qbx import QBXLayerPotentialSource
+    ops = []
+    for arms, nel, k in ((5, 64, 3.0), (7, 160, 11.0)):
+        d = discretize_curve(starfish(arms, 0.25), nel, 8, 16)
+        s = QBXLayerPotentialSource(d, fine_order=16, qbx_order=5, fmm_order=15)
+        ops.append((d, DirichletOperator(s, loc_sign=-1, kernel="helmholtz", k=k)))
+    rng = np.random.default_rng(3)
+    graphs = [MatVecGraph(op.matvec, torch.as_tensor(
+        rng.normal(size=d.ndofs) + 0j, device="cuda")) for d, op in ops]
+    for rep in range(3):
+        for (d, op), mv in zip(ops, graphs):
+            x = torch.as_tensor(rng.normal(size=d.ndofs) + 1j * rng.normal(size=d.ndofs),
+                                device="cuda")
+            y_graph = mv(x)
+            torch.cuda.synchronize()
+            y_eager = op.matvec(x)
+            torch.cuda.synchronize()
+            err = float((y_graph - y_eager).abs().max() / y_eager.abs().max())
+            print(f"[graph2] replay {rep} n={d.ndofs}: rel diff {err:.2e}", flush=True)
+            assert err < 1e-13
+
+
+def test_shared_context_refuses_to_replace_captured_workspaces():
+    """A graph captured through the shared context pins its workspaces: a later
+    evaluation on that context that needs larger ones raises instead of
+    freeing memory the graph replays; after release_graphs it runs."""
+    from pytential_amd import fmm as F
+    from pytential_amd._lib import QBXError
+    from pytential_amd.discretization import discretize_curve, starfish
+    from pytential_amd.geometry import Context
+    from pytential_amd.linalg import MatVecGraph
+    from pytential_amd.qbx import QBXLayerPotentialSource
+    ctx = Context(torch.cuda.current_device())
+
+    def evaluator(nel):
+        d = discretize_curve(starfish(5, 0.25), nel, 8, 16)
+        s = QBXLayerPotentialSource(d, fine_order=16, qbx_order=5, fmm_order=10)
+        geo = s.qbx_fmm_geometry_data((("self", +1),))
+        w = F.QBXExpansionWrangler(geo, "laplace", 0.0, "S", 5,
+                                   s.level_orders("laplace", 0.0, geo.tree()), ctx=ctx)
+
+        def fn(x):
+            st, _ = w.reorder_sources(s.quad_density(x), s.dev["quad_weights"], None)
+            return F.drive_fmm_fused(w, st, None)
+        return d, fn
+
+    d1, f1 = evaluator(32)
+    d2, f2 = evaluator(256)
+    x1 = torch.ones(d1.ndofs, dtype=torch.float64, device="cuda")
+    x2 = torch.ones(d2.ndofs, dtype=torch.float64, device="cuda")
+    mv = MatVecGraph(f1, x1)
+    with pytest.raises(QBXError, match="captured"):
+        f2(x2)
+    torch.cuda.synchronize()
+    y1 = mv(x1)
+    ref1 = f1(x1)                       # same sizes: no replacement needed
+    torch.cuda.synchronize()
+    assert float((y1 - ref1).abs().max() / ref1.abs().max()) < 1e-13
+    del mv
+    ctx.release_graphs()
+    y2 = f2(x2)
+    torch.cuda.synchronize()
+    assert bool(torch.isfinite(y2.real).all())

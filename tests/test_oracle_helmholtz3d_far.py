@@ -67,3 +67,20 @@ def test_k_derivative_formula():
     r = np.sqrt(np.sum(dd * dd, axis=0))
     an = (1j * np.exp(1j * k * r) / (4 * np.pi)) @ q
     assert np.max(np.abs(fd - an)) / np.max(np.abs(an)) < 1e-8
+
+
+def test_chunked_apply_and_adjoint_equal_dense_operator():
+    """helmholtz3d_far_apply / _adjoint (rows built in chunks) are A sigma and
+    A^H y of the dense operator G_k W U."""
+    import oracle
+    from pytential_amd.surface import ellipsoid
+    d = ellipsoid(1, order=4, fine_order=5)
+    rng = np.random.default_rng(9)
+    tg = rng.normal(size=(3, 70)) * 3.0
+    A = oracle.helmholtz3d_far_operator(d, tg, 2.5)
+    s = rng.normal(size=d.ndofs) + 1j * rng.normal(size=d.ndofs)
+    y = rng.normal(size=70) + 1j * rng.normal(size=70)
+    a = oracle.helmholtz3d_far_apply(d, tg, 2.5, s, chunk=32)
+    b = oracle.helmholtz3d_far_adjoint(d, tg, 2.5, y, chunk=32)
+    assert np.max(np.abs(a - A @ s)) < 1e-13 * np.max(np.abs(A @ s))
+    assert np.max(np.abs(b - A.conj().T @ y)) < 1e-13 * np.max(np.abs(A.conj().T @ y))
