@@ -129,10 +129,11 @@ def _miller_steps(p):
     return m + 16 + int(np.sqrt(40.0 * m + 1.0))
 
 
-def stage_flops(counts, orders, pq):
+def stage_flops(counts, orders, pq, SPECIAL=1):
     """Algorithmic flops per stage launch (Helmholtz 2D, double layer).  A
-    Bessel/Hankel function value of order 0 or 1 counts SPECIAL flops, a
-    complex multiply-add 8, a complex multiply 6."""
+    Bessel/Hankel function value of order 0 or 1 (and atan2) counts SPECIAL
+    flops (1 in the measurement plan's convention, where every special
+    function is one flop), a complex multiply-add 8, a complex multiply 6."""
     o = np.asarray(orders, np.int64)
     lv = counts["levels"]
     nq = 2 * pq + 1
@@ -263,23 +264,31 @@ def run_gpu(args, cfg, orders_fn, label):
     dt = (time.perf_counter() - t0) / args.steps
     per_stage = {s: float(np.mean([e[i].elapsed_time(e[i + 1]) for e in evs]))
                  for i, s in enumerate(stages)}
-    # fused single-entry driver over the same work, for reference
-    qd = src.quad_density(dens)
-    st, nrm = w.reorder_sources(qd, src.dev["quad_weights"], src.dev["quad_normals"])
-    F.drive_fmm_fused(w, st, nrm)
+    # the single-call driver (the timed evaluation of the headline)
+    def fused_step():
+        qd = src.quad_density(dens)
+        st, nrm = w.reorder_sources(qd, src.dev["quad_weights"], src.dev["quad_normals"])
+        return F.drive_fmm_fused(w, st, nrm)
+
+    for _ in range(max(args.warmup, 1)):
+        fused_step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     for _ in range(args.steps):
-        qd = src.quad_density(dens)
-        st, nrm = w.reorder_sources(qd, src.dev["quad_weights"], src.dev["quad_normals"])
-        F.drive_fmm_fused(w, st, nrm)
+        fused = fused_step()
     torch.cuda.synchronize()
     dt_fused = (time.perf_counter() - t1) / args.steps
     counts = interaction_counts(geo)
     flops = stage_flops(counts, orders, cfg.qbx_order)
+    flops40 = stage_flops(counts, orders, cfg.qbx_order, SPECIAL=40)
+    staged = out.cpu().numpy()
+    fused = fused.cpu().numpy()
     return {"label": label, "orders": orders, "cold_s": cold, "step_s": dt,
             "step_fused_s": dt_fused, "per_stage_ms": per_stage, "counts": counts,
-            "flops": flops, "result": out.cpu().numpy(), "geo": geo, "src": src}
+            "flops": flops, "flops40": flops40, "result": fused, "result_stagewise": staged,
+            "fused_vs_stagewise": float(np.max(np.abs(fused - staged)) /
+                                        np.max(np.abs(staged))),
+            "geo": geo, "src": src}
 
 
 def run_batched(args, cfg, src, geo, nrhs):
@@ -314,11 +323,12 @@ def run_batched(args, cfg, src, geo, nrhs):
 
 
 def cpu_baseline(args, cfg, orders, sample_idx):
-    """The oracle on the host cores: one full evaluation (timed) plus the
-    direct-QBX accuracy reference on a sample of targets."""
+    """The oracle on the host cores: one full evaluation (timed, on the
+    process's CPU share and on one thread) plus the direct-QBX accuracy
+    reference on a sample of targets."""
     import oracle
     oracle.build(with_ref=False)
-    nthreads = args.cpu_threads or os.cpu_count()
+    nthreads = cpu_threads(args)
     g = oracle.GeometryData(cfg.discr, cfg.targets, cfg.target_sides)
     st = cfg.discr.strengths(cfg.density)
     t0 = time.perf_counter()
@@ -329,137 +339,293 @@ def cpu_baseline(args, cfg, orders, sample_idx):
     ref = g.direct(cfg.kernel, cfg.k, cfg.source_kind, cfg.qbx_order, st, sample_idx,
                    nthreads=nthreads)
     t_direct = time.perf_counter() - t0
+    one = None
+    if not args.no_cpu_1thread:
+        t0 = time.perf_counter()
+        g.fmm(cfg.kernel, cfg.k, cfg.source_kind, cfg.qbx_order, orders, st, nthreads=1)
+        t1 = time.perf_counter() - t0
+        one = {"value": round(cfg.ntargets / t1, 1), "unit": "points/s", "cores": 1,
+               "seconds": round(t1, 2)}
     return {"value": cfg.ntargets / t_fmm, "unit": "points/s", "cores": nthreads,
             "kind": "port",
             "sample": (f"one full evaluation of the same workload ({cfg.ntargets} targets, "
-                       f"same tree/lists/orders), {t_fmm:.2f}s; direct-QBX reference on "
-                       f"{len(sample_idx)} sampled targets, {t_direct:.2f}s"),
+                       f"same tree/lists/orders) through the oracle (C + OpenMP), "
+                       f"{t_fmm:.2f}s; direct-QBX reference on {len(sample_idx)} sampled "
+                       f"targets, {t_direct:.2f}s"),
+            "host": host_cpu(), "one_thread": one,
             "fmm_result_sample": pot[sample_idx], "direct_sample": ref, "t_fmm": t_fmm}
 
 
-def sphere_densities(discr, nrhs, seed=1002):
-    """nrhs densities: real spherical harmonics Y_lm (l <= 6, in order) plus
-    N(0, 0.01) noise, seeded."""
-    from pytential_amd.problems import sphere_harmonic_densities
-    return sphere_harmonic_densities(discr.nodes, nrhs, seed=seed)[0]
+def host_cpu():
+    """The host's CPU model, logical CPU count and the CPUs this process may
+    use (lscpu's "Model name", nproc)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable}
 
 
-def main_sphere3d(args):
-    """3D Laplace single layer on the unit sphere, many densities in one
-    evaluation: icosahedron refined nsub times, order-4 triangles, qbx_order 3,
-    fmm_order 8, on-surface targets (side +1)."""
+def cpu_threads(args):
+    """Threads for the oracle's OpenMP legs: --cpu-threads, else this process's
+    CPU share (at most 16, the host share of one GPU)."""
+    if args.cpu_threads:
+        return args.cpu_threads
+    share = host_cpu()["usable_cpus"] or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        share = min(share, int(env))
+    return max(1, min(share, 16))
+
+
+def stage_flops_3d(counts, orders, pq, nrhs):
+    """Algorithmic flops per stage launch for 3D Laplace S with nrhs real
+    densities, in the per-unit convention of the measurement plan: +,-,* 1,
+    FMA 2, each sqrt/division/special function 1; a complex multiply-add 8, a
+    complex-by-real one 4.  ncq = (pq+1)^2 QBX coefficients, ncf = (p+1)^2
+    box coefficients at the box's level."""
+    o = np.asarray(orders, np.int64)
+    ncq = (pq + 1) ** 2
+    lv = counts["levels"]
+    # P2QBXL: the pair's basis (r, angles, the harmonic recurrence: ~4 ncq +
+    # 20) once, then a complex-by-real MAC per coefficient and density
+    p2qbxl = counts["p2qbxl_pairs"] * (4 * ncq + 20 + 4 * ncq * nrhs)
+    # M2L: dense (ncf x ncf) complex translation per (box, list-2 box) and density
+    ttc, l2s = counts["ttp"], counts["l2s"]
+    ncf_t = (o[lv[ttc]] + 1) ** 2
+    m2l = int(np.sum(np.diff(l2s) * ncf_t * ncf_t * 8 * nrhs))
+    # M2QBXL: dense (ncq x ncf) complex per (center, list-3 box) and density
+    m2q_lv = counts["m2qbxl_levels"]
+    c2tb, l3s = counts["c2tb"], counts["l3s"]
+    m2q = 0
+    if len(m2q_lv):
+        csum = np.concatenate([[0], np.cumsum(8 * ncq * (o[m2q_lv] + 1) ** 2 * nrhs)])
+        m2q = int(np.sum(csum[l3s[c2tb + 1]] - csum[l3s[c2tb]]))
+    return {"p2qbxl": int(p2qbxl), "m2l": m2l, "m2qbxl": m2q}
+
+
+def run_sphere3d(args):
+    """Config 3: 3D Laplace single layer on the unit sphere, nrhs3d densities
+    in one evaluation: icosahedron refined nsub times, order-4 triangles,
+    qbx_order 3, fmm_order 8 (and the order that reaches 1e-10), on-surface
+    targets (side +1).  Returns the result dict."""
     import torch
 
     from pytential_amd import fmm as F
     from pytential_amd.qbx import QBXLayerPotentialSource
     from pytential_amd.surface import icosphere
+    from pytential_amd.problems import sphere_harmonic_densities
     t0 = time.perf_counter()
     discr = icosphere(args.nsub, order=4, fine_order=args.fine3d)
-    dens_np = sphere_densities(discr, args.nrhs3d)
-    log(f"sphere: {discr.nelements} triangles, ndofs={discr.ndofs} "
+    nrhs = args.nrhs3d
+    dens_np, degs = sphere_harmonic_densities(discr.nodes, nrhs)
+    log(f"[sphere3d] {discr.nelements} triangles, ndofs={discr.ndofs} "
         f"nsources={discr.nsources} ({time.perf_counter() - t0:.1f}s)")
-    src = QBXLayerPotentialSource(discr, fine_order=args.fine3d, qbx_order=3, fmm_order=8)
-    geo = src.qbx_fmm_geometry_data((("self", +1),))
+    dens = torch.as_tensor(dens_np, dtype=torch.float64, device="cuda")
+    steps = args.steps3d
+
+    def make(order):
+        src = QBXLayerPotentialSource(discr, fine_order=args.fine3d, qbx_order=3,
+                                      fmm_order=order)
+        geo = src.qbx_fmm_geometry_data((("self", +1),))
+        return src, geo
+
+    def stepper(src, w, d):
+        def step():
+            qd = src.quad_density(d)
+            st, _ = w.reorder_sources(qd, src.dev["quad_weights"], None)
+            return F.drive_fmm_fused(w, st, None)
+        return step
+
+    def timed(step, nsteps, warm=1):
+        for _ in range(warm):
+            step()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(nsteps):
+            out = step()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / nsteps, out
+
+    src, geo = make(8)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     geo.build_all()
     torch.cuda.synchronize()
     cold = time.perf_counter() - t0
     tree = geo.tree()
-    log(f"geometry {cold:.3f}s nboxes={tree.nboxes} nlevels={tree.nlevels}")
-    nrhs = args.nrhs3d
-    dens = torch.as_tensor(dens_np, dtype=torch.float64, device="cuda")
+    log(f"[sphere3d] geometry {cold:.3f}s nboxes={tree.nboxes} nlevels={tree.nlevels}")
     w = src.make_wrangler(geo, "laplace", 0.0, "S", nrhs)
-
-    def step():
-        qd = src.quad_density(dens)
-        st, _ = w.reorder_sources(qd, src.dev["quad_weights"], None)
-        return F.drive_fmm_fused(w, st, None)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        out = step()
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.steps
+    dt, out = timed(stepper(src, w, dens), steps)
     res = out.real.cpu().numpy()
-    # plain: one density per evaluation
+    # stage by stage on one stream, HIP events between stages: per-stage times
+    # and the roofline of the dominant kernel
+    stages = ["p2m", "m2m", "p2p", "m2l", "m2p", "p2l", "l2l", "l2p", "p2qbxl", "m2qbxl",
+              "l2qbxl", "qbxl2p"]
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(stages) + 1)]
+    qd = src.quad_density(dens)
+    st, _ = w.reorder_sources(qd, src.dev["quad_weights"], None)
+    per_stage = {s: 0.0 for s in stages}
+    for rep in range(2):
+        ev[0].record()
+        m = w.form_multipoles(st, None)
+        ev[1].record()
+        w.coarsen_multipoles(m)
+        ev[2].record()
+        pot = w.eval_direct(st, None)
+        ev[3].record()
+        loc = w.multipole_to_local(m)
+        ev[4].record()
+        w.eval_multipoles(m, pot)
+        ev[5].record()
+        w.form_locals(st, None, loc)
+        ev[6].record()
+        w.refine_locals(loc)
+        ev[7].record()
+        w.eval_locals(loc, pot)
+        ev[8].record()
+        q = w.form_global_qbx_locals(st, None)
+        ev[9].record()
+        w.translate_box_multipoles_to_qbx_local(m, q)
+        ev[10].record()
+        w.translate_box_local_to_qbx_local(loc, q)
+        ev[11].record()
+        w.eval_qbx_expansions(q, pot)
+        ev[12].record()
+        torch.cuda.synchronize()
+        if rep == 1:
+            per_stage = {s: ev[i].elapsed_time(ev[i + 1]) for i, s in enumerate(stages)}
+    counts = interaction_counts(geo)
+    lo = list(map(int, w._lo))
+    flops = stage_flops_3d(counts, lo, 3, nrhs)
+    kernel_of = {"p2qbxl": "k3_p2qbxl_mr", "m2l": "k3_m2l_mfma", "m2qbxl": "k3_m2qbxl"}
+    dom = max(flops, key=lambda k: per_stage[k])
+    ach = flops[dom] / (per_stage[dom] * 1e-3) / 1e12
+    roofline = {"bound": "mfma" if dom in ("p2qbxl", "m2l") else "valu", "compute": "fp64",
+                "kernel": kernel_of[dom], "stage": dom, "achieved": round(ach, 3),
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                "flops_per_launch": int(flops[dom]),
+                "avg_launch_ms": round(per_stage[dom], 4),
+                "timing": "HIP events on the launch stream, stage-by-stage run of the same "
+                          "32-density evaluation"}
+    stage_rl = {k: {"ms": round(per_stage[k], 4), "gflop": round(flops[k] / 1e9, 3),
+                    "frac_fp64": round(flops[k] / (per_stage[k] * 1e-3) / 1e12 /
+                                       FP64_PEAK_TFLOPS, 4)} for k in flops}
+    log(f"[sphere3d] p=8, {nrhs} densities: {dt * 1e3:.1f} ms; stages "
+        f"{ {k: round(v, 2) for k, v in per_stage.items()} }")
+    # one density per evaluation
     w1 = src.make_wrangler(geo, "laplace", 0.0, "S", 1)
-    d1 = dens[:, 0].contiguous()
-
-    def step1():
-        qd = src.quad_density(d1)
-        st, _ = w1.reorder_sources(qd, src.dev["quad_weights"], None)
-        return F.drive_fmm_fused(w1, st, None)
-
-    step1()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
-        out1 = step1()
-    torch.cuda.synchronize()
-    dt1 = (time.perf_counter() - t1) / args.steps
-    # accuracy: S Y_00 = Y_00 on the unit sphere (the noise-free harmonic, one
-    # more untimed evaluation), and direct QBX on a sample
-    y00 = torch.full((discr.ndofs,), 0.5 / np.sqrt(np.pi), dtype=torch.float64, device="cuda")
-    qd = src.quad_density(y00)
-    st, _ = w1.reorder_sources(qd, src.dev["quad_weights"], None)
-    s_y00 = F.drive_fmm_fused(w1, st, None).real.cpu().numpy()
-    err_eig = float(np.max(np.abs(s_y00 - 0.5 / np.sqrt(np.pi))) / (0.5 / np.sqrt(np.pi)))
-    cpu = None
-    err_direct = None
-    if not args.no_cpu:
-        import oracle
-        oracle.build(with_ref=False)
-        nthreads = args.cpu_threads or os.cpu_count()
-        g = oracle.GeometryData(discr, discr.nodes, np.ones(discr.ndofs, np.int32),
-                                max_leaf_refine_weight=512)
-        rng = np.random.default_rng(12)
-        idx = np.sort(rng.choice(discr.ndofs, min(args.sample, discr.ndofs), replace=False))
-        st0 = discr.strengths(dens_np[:, 0])
-        ref = g.direct("laplace", 0.0, "S", 3, st0, idx, nthreads=nthreads).real
-        err_direct = float(np.max(np.abs(res[idx, 0] - ref)) / np.max(np.abs(ref)))
-        lo = list(map(int, w._lo))
-        t2 = time.perf_counter()
-        g.fmm("laplace", 0.0, "S", 3, lo, st0, nthreads=nthreads)
-        t_fmm = time.perf_counter() - t2
-        cpu = {"value": round(discr.ndofs / t_fmm, 1), "unit": "points*rhs/s",
-               "cores": nthreads, "kind": "port",
-               "sample": (f"one density (RHS 0) of the same workload through the oracle's "
-                          f"QBX-FMM, same tree/lists/orders, {t_fmm:.1f}s")}
-    line = {
-        "metric": METRIC,
-        "value": round(discr.ndofs * nrhs / dt, 1),
-        "unit": "points*rhs/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(dt * 1e3, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
+    dt1, out1 = timed(stepper(src, w1, dens[:, 0].contiguous()), steps)
+    # S Y_lm = Y_lm / (2l + 1) for the noise-free harmonics (untimed)
+    clean, _ = sphere_harmonic_densities(discr.nodes, nrhs, noise=0.0)
+    s_clean = stepper(src, w, torch.as_tensor(clean, device="cuda"))().real.cpu().numpy()
+    eig = {}
+    for c in range(nrhs):
+        e = float(np.max(np.abs(s_clean[:, c] - clean[:, c] / (2 * degs[c] + 1))) /
+                  np.max(np.abs(clean[:, c] / (2 * degs[c] + 1))))
+        eig[f"l={degs[c]}"] = max(eig.get(f"l={degs[c]}", 0.0), e)
+    result = {
+        "value": round(discr.ndofs * nrhs / dt, 1), "unit": "points*rhs/s",
+        "ms_per_step": round(dt * 1e3, 4), "steps": steps, "nrhs": nrhs,
+        "dtype": "f64",
         "config": {"workload": (f"3D Laplace S, unit sphere: icosahedron refined {args.nsub}x "
                                 f"({discr.nelements} triangles), order-4 nodes "
                                 f"({discr.ndofs}), quad {args.fine3d}x{args.fine3d} "
                                 f"collapsed Gauss ({discr.nsources} sources), qbx_order 3, "
-                                f"fmm_order 8, {nrhs} densities (Y_lm l<=6 + noise), "
-                                f"on-surface targets"),
-                   "nrhs": nrhs, "ntargets": discr.ndofs, "nboxes": tree.nboxes,
-                   "nlevels": tree.nlevels},
+                                f"fmm_order 8, {nrhs} densities (Y_lm l<=6 + N(0,0.01) "
+                                f"noise, seed 1002), on-surface targets"),
+                   "ntargets": discr.ndofs, "nboxes": tree.nboxes, "nlevels": tree.nlevels},
         "plain_one_density": {"value": round(discr.ndofs / dt1, 1), "unit": "points/s",
                               "ms_per_step": round(dt1 * 1e3, 4),
                               "rel_diff_vs_batched_rhs0": float(
                                   np.max(np.abs(out1.real.cpu().numpy() - res[:, 0])) /
                                   np.max(np.abs(res[:, 0])))},
-        "rel_err_vs_direct_qbx_rhs0": err_direct,
-        "rel_err_S_Y00_eigen": err_eig,
+        "S_Ylm_vs_Ylm_over_2l_plus_1_worst_rel_err": eig,
         "cold_geometry_s": round(cold, 4),
-        "roofline": None,
-        "cpu_baseline": cpu,
+        "per_stage_ms": {k: round(v, 4) for k, v in per_stage.items()},
+        "stage_roofline": stage_rl,
+        "roofline": roofline,
+        "cpu_baseline": None,
     }
+    if args.no_cpu:
+        return result
+    import oracle
+    oracle.build(with_ref=False)
+    nth = cpu_threads(args)
+    g = oracle.GeometryData(discr, discr.nodes, np.ones(discr.ndofs, np.int32),
+                            max_leaf_refine_weight=512)
+    rng = np.random.default_rng(12)
+    idx = np.sort(rng.choice(discr.ndofs, min(args.sample, discr.ndofs), replace=False))
+    st0 = discr.strengths(dens_np[:, 0])
+    t2 = time.perf_counter()
+    ref = g.direct("laplace", 0.0, "S", 3, st0, idx, nthreads=nth).real
+    t_dir = time.perf_counter() - t2
+    scale = np.max(np.abs(ref))
+    result["rel_err_vs_direct_qbx_rhs0"] = float(np.max(np.abs(res[idx, 0] - ref)) / scale)
+    # the per-level order that reaches 1e-10 against direct QBX (same sample)
+    sweep = {"8": result["rel_err_vs_direct_qbx_rhs0"]}
+    p_ok = None
+    for p in (10, 12, 13, 14):
+        sp, gp = make(p)
+        wp = sp.make_wrangler(gp, "laplace", 0.0, "S", 1)
+        o = stepper(sp, wp, dens[:, 0].contiguous())().real.cpu().numpy()
+        sweep[str(p)] = float(np.max(np.abs(o[idx] - ref)) / scale)
+        log(f"[sphere3d] fmm_order {p}: rel err vs direct QBX {sweep[str(p)]:.2e}")
+        if sweep[str(p)] <= 1e-10:
+            p_ok = p
+            break
+    result["rel_err_vs_direct_qbx_by_fmm_order"] = sweep
+    if p_ok is not None:
+        wq = sp.make_wrangler(gp, "laplace", 0.0, "S", nrhs)
+        dtq, _ = timed(stepper(sp, wq, dens), max(2, steps // 2))
+        result["at_1e-10_order"] = {"fmm_order": p_ok,
+                                    "value": round(discr.ndofs * nrhs / dtq, 1),
+                                    "unit": "points*rhs/s",
+                                    "ms_per_step": round(dtq * 1e3, 4),
+                                    "rel_err_vs_direct_qbx_rhs0": sweep[str(p_ok)]}
+        log(f"[sphere3d] p={p_ok}, {nrhs} densities: {dtq * 1e3:.1f} ms")
+    t2 = time.perf_counter()
+    g.fmm("laplace", 0.0, "S", 3, lo, st0, nthreads=nth)
+    t_fmm = time.perf_counter() - t2
+    one = None
+    if not args.no_cpu_1thread:
+        t2 = time.perf_counter()
+        g.fmm("laplace", 0.0, "S", 3, lo, st0, nthreads=1)
+        t1 = time.perf_counter() - t2
+        one = {"value": round(discr.ndofs / t1, 1), "unit": "points*rhs/s", "cores": 1,
+               "seconds": round(t1, 2)}
+    result["cpu_baseline"] = {
+        "value": round(discr.ndofs / t_fmm, 1), "unit": "points*rhs/s", "cores": nth,
+        "kind": "port",
+        "sample": (f"one density (RHS 0) of the same workload through the oracle's QBX-FMM "
+                   f"(C + OpenMP), same tree/lists/orders, {t_fmm:.2f}s; the direct-QBX "
+                   f"accuracy reference on {len(idx)} targets took {t_dir:.2f}s"),
+        "host": host_cpu(), "one_thread": one}
+    result["speedup_vs_cpu"] = round(result["value"] / result["cpu_baseline"]["value"], 1)
+    return result
+
+
+def main_sphere3d(args):
+    res = run_sphere3d(args)
+    line = {"metric": METRIC, "value": res.pop("value"), "unit": res.pop("unit"),
+            "n_gpus": 1, "steps": res.pop("steps"), "warmup": 1,
+            "ms_per_step": res.pop("ms_per_step"), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": res.pop("dtype"),
+            "data": "synthetic", **res}
     print(json.dumps(line), flush=True)
 
 
-def main_ellipsoid3d(args):
+def run_ellipsoid3d(args):
     """Loss and gradient of the 3D Helmholtz (k = 5) single layer on the
     ellipsoid with semi-axes (1, 0.75, 0.5) at 4096 seeded exterior targets
     (radius 2..3): L = ||S_k sigma - g||^2, g from a seeded point source inside;
@@ -529,13 +695,16 @@ def main_ellipsoid3d(args):
     e[2].record()
     torch.cuda.synchronize()
     t_fwd, t_adj = e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2])
-    # per pair: |t - s| (9), sincos, 1/r and the complex scale-and-add (12),
-    # and the k-derivative's complex add (4); the adjoint the same without it
-    flops_fwd = npairs * (9 + SPECIAL + 1 + 12 + 4)
-    flops_adj = npairs * (9 + SPECIAL + 1 + 12)
+    # per pair: |t - s|^2 (8), sqrt and 1/r (2), sincos (1, a special
+    # function), the complex scale-and-add (12) and the k-derivative's complex
+    # add (4); the adjoint the same without the last
+    flops_fwd = npairs * (8 + 2 + 1 + 12 + 4)
+    flops_adj = npairs * (8 + 2 + 1 + 12)
     dom, t_dom, f_dom = (("forward", t_fwd, flops_fwd) if t_fwd >= t_adj
                          else ("adjoint", t_adj, flops_adj))
     achieved = f_dom / (t_dom * 1e-3) / 1e12
+    # the same with the sincos counted as SPECIAL flops
+    achieved40 = (f_dom + npairs * (SPECIAL - 1)) / (t_dom * 1e-3) / 1e12
 
     # checks on the 1280-triangle coarsening against the dense operator
     dc = ellipsoid(3, order=4, fine_order=6)
@@ -606,7 +775,8 @@ def main_ellipsoid3d(args):
                "kind": "port",
                "sample": (f"forward only: {ns_t} of the 4096 targets against all "
                           f"{d.nsources} sources through the oracle's dense kernel rows "
-                          f"(numpy), {t_cpu:.2f}s")}
+                          f"(numpy, one thread), {t_cpu:.2f}s"),
+               "host": host_cpu()}
     line = {
         "metric": METRIC,
         "value": round(4096 / dt, 1),
@@ -627,17 +797,23 @@ def main_ellipsoid3d(args):
                                   "dL_dk_vs_central_diff": e_fd_k,
                                   "dL_dsigma_dir_vs_central_diff": e_fd_v,
                                   "gpu_dense_matrix_4096_targets": dense},
-        "roofline": {"bound": "mfma", "compute": "fp64", "kernel": dom,
+        "roofline": {"bound": "valu", "compute": "fp64",
+                     "kernel": "k_h3_p2p_partial" if dom == "forward" else "k_h3_p2p_adjoint",
                      "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                     "frac_special40": round(achieved40 / FP64_PEAK_TFLOPS, 4),
                      "traffic": None, "flops_per_launch": int(f_dom),
                      "avg_launch_ms": round(t_dom, 4)},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
+    return line
 
 
-def main_dirichlet64(args):
+def main_ellipsoid3d(args):
+    print(json.dumps(run_ellipsoid3d(args)), flush=True)
+
+
+def run_dirichlet64(args):
     """Config 4: ncurves independent starfish curves (arms 3..9, radius
     0.5..2, amplitude 0.25, element counts log-uniform in 256..4096, 8
     Gauss-Legendre nodes per element), the interior Laplace Dirichlet problem
@@ -741,7 +917,90 @@ def main_dirichlet64(args):
         "roofline": None,
         "cpu_baseline": None,
     }
-    print(json.dumps(line), flush=True)
+    return line
+
+
+def main_dirichlet64(args):
+    print(json.dumps(run_dirichlet64(args)), flush=True)
+
+
+def run_config1(args):
+    """Config 1, the reference's CPU-runnable case: 2D Laplace S on the 5-arm
+    starfish, 64 panels x 8 nodes, qbx_order 4, fmm_order 10, on-surface plus
+    a 128^2 grid; compared with direct QBX (fmm_order=False) on every target.
+    Returns the result dict."""
+    import torch
+
+    from pytential_amd import fmm as F
+    from pytential_amd import problems
+    from pytential_amd.geometry import assoc_status_and_flags
+    from pytential_amd.qbx import QBXLayerPotentialSource
+    cfg = problems.refine_until_associated(problems.config1(), assoc_status_and_flags)
+    tg = ((cfg.targets, cfg.target_sides),)
+    src = QBXLayerPotentialSource(cfg.discr, fine_order=16, qbx_order=4, fmm_order=10)
+    geo = src.qbx_fmm_geometry_data(tg)
+    geo.build_all()
+    w = src.make_wrangler(geo, "laplace", 0.0, "S")
+    dens = torch.as_tensor(cfg.density, dtype=torch.float64, device="cuda")
+
+    def step():
+        qd = src.quad_density(dens)
+        st, _ = w.reorder_sources(qd, src.dev["quad_weights"], None)
+        return F.drive_fmm_fused(w, st, None)
+
+    for _ in range(3):
+        step()
+    torch.cuda.synchronize()
+    n = max(args.steps, 20)
+    t0 = time.perf_counter()
+    for _ in range(n):
+        out = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / n
+    got = out.real.cpu().numpy()
+    srcd = QBXLayerPotentialSource(cfg.discr, fine_order=16, qbx_order=4, fmm_order=False)
+    dref, = srcd.exec_compute_potential("laplace", cfg.density, list(tg))
+    dref = dref.cpu().numpy()
+    res = {"value": round(cfg.ntargets / dt, 1), "unit": "points/s",
+           "ms_per_step": round(dt * 1e3, 4), "steps": n, "dtype": "f64",
+           "config": {"workload": (f"2D Laplace S, starfish 5 arms / 0.25, "
+                                   f"{cfg.discr.nelements} panels x 8 nodes, fine 16, "
+                                   f"qbx_order 4, fmm_order 10, 128^2 grid on [-2,2]^2 + "
+                                   f"on-surface targets, density sin 3t + 0.1 N(0,1)"),
+                      "ntargets": cfg.ntargets, "seed": cfg.seed},
+           "rel_err_vs_gpu_direct_qbx_all_targets": float(
+               np.max(np.abs(got - dref)) / np.max(np.abs(dref))),
+           "cpu_baseline": None}
+    if not args.no_cpu:
+        import oracle
+        oracle.build(with_ref=False)
+        g = oracle.GeometryData(cfg.discr, cfg.targets, cfg.target_sides)
+        st = cfg.discr.strengths(cfg.density)
+        lo = [10] * g.tree.nlevels
+        nth = cpu_threads(args)
+        times = {}
+        for th in (nth, 1):
+            reps = 0
+            t0 = time.perf_counter()
+            while reps < 3 or time.perf_counter() - t0 < 1.0:
+                f_ref, _ = g.fmm("laplace", 0.0, "S", 4, lo, st, nthreads=th)
+                reps += 1
+            times[th] = (time.perf_counter() - t0) / reps
+        d_ref = g.direct("laplace", 0.0, "S", 4, st, nthreads=nth).real
+        res["rel_err_vs_oracle_fmm_all_targets"] = float(
+            np.max(np.abs(got - f_ref.real)) / np.max(np.abs(f_ref)))
+        res["rel_err_vs_oracle_direct_qbx_all_targets"] = float(
+            np.max(np.abs(got - d_ref)) / np.max(np.abs(d_ref)))
+        res["cpu_baseline"] = {
+            "value": round(cfg.ntargets / times[nth], 1), "unit": "points/s", "cores": nth,
+            "kind": "port",
+            "sample": (f"the whole evaluation through the oracle (C + OpenMP), same "
+                       f"tree/lists/orders, {times[nth] * 1e3:.1f} ms per evaluation"),
+            "host": host_cpu(),
+            "one_thread": {"value": round(cfg.ntargets / times[1], 1), "unit": "points/s",
+                           "cores": 1, "seconds": round(times[1], 4)}}
+        res["speedup_vs_cpu"] = round(res["value"] / res["cpu_baseline"]["value"], 1)
+    return res
 
 
 def main():
@@ -753,18 +1012,24 @@ def main():
     ap.add_argument("--nelements", type=int, default=4096)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu-1thread", action="store_true",
+                    help="skip the one-thread oracle timings")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--sample", type=int, default=1000)
     ap.add_argument("--nrhs", type=int, default=8,
                     help="densities per evaluation for the batched figure")
     ap.add_argument("--workload", choices=["helmholtz2d", "sphere3d", "ellipsoid3d",
-                                           "dirichlet64"], default="helmholtz2d")
+                                           "dirichlet64", "laplace2d"],
+                    default="helmholtz2d")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="helmholtz2d without the other workloads' nested results")
     ap.add_argument("--ncurves", type=int, default=64, help="dirichlet64: curves")
     ap.add_argument("--nsub-ell", type=int, default=5,
                     help="ellipsoid3d: icosahedron refinements (5: 20480 triangles)")
     ap.add_argument("--nsub", type=int, default=5, help="sphere3d: icosahedron refinements")
     ap.add_argument("--fine3d", type=int, default=6, help="sphere3d: quad rule n x n")
     ap.add_argument("--nrhs3d", type=int, default=32, help="sphere3d: densities")
+    ap.add_argument("--steps3d", type=int, default=5, help="sphere3d: timed steps")
     ap.add_argument("--profile", action="store_true",
                     help="one configuration only (tolerance orders), no CPU leg: "
                          "the run profiled under rocprofv3")
@@ -772,9 +1037,8 @@ def main():
                     help="only copy a 512 MiB device buffer 5 times (HBM counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the QBX-FMM "
-              "path here is single-GPU, see DESIGN.md for how it would shard.",
-              file=sys.stderr)
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the QBX-FMM "
+              "path here is single-GPU.", file=sys.stderr)
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
@@ -800,6 +1064,12 @@ def main():
     if args.workload == "sphere3d":
         main_sphere3d(args)
         return
+    if args.workload == "laplace2d":
+        res = run_config1(args)
+        print(json.dumps({"metric": METRIC, "n_gpus": 1, "higher_is_better": True,
+                          "scaling": "weak", "vs_baseline": None, "data": "synthetic",
+                          **res}), flush=True)
+        return
     from pytential_amd.qbx import fmm_level_to_order_for_tolerance
 
     t0 = time.perf_counter()
@@ -823,8 +1093,6 @@ def main():
     cfg_run = run_gpu(args, cfg, config_orders, f"p{cfg.fmm_order}")
     # accuracy on every target: the GPU direct QBX (fmm_order=False) of the
     # same discretization, untimed
-    import torch
-
     from pytential_amd.qbx import QBXLayerPotentialSource
     srcd = QBXLayerPotentialSource(cfg.discr, fine_order=cfg.discr.fine_order,
                                    qbx_order=cfg.qbx_order, fmm_order=False)
@@ -862,22 +1130,32 @@ def main():
         log(f"errors: {err}")
 
     ps = main_run["per_stage_ms"]
-    flops = main_run["flops"]
-    # the dominant kernel: the slowest stage (every stage is one kernel launch,
-    # or one per level for the level-by-level translations)
+    flops, flops40 = main_run["flops"], main_run["flops40"]
+    # the dominant kernel: the slowest stage of the stage-by-stage run (every
+    # stage is one kernel launch, or one per level for the level-by-level
+    # translations), timed by HIP events on its launch stream
     dominant = max(ps, key=ps.get)
+    kernel_name = {"p2p": "k_p2p", "p2qbxl": "k_p2qbxl", "m2l": "k_m2l_blk",
+                   "m2m": "k_m2m_h", "p2l": "k_p2e_h", "p2m": "k_p2e_h",
+                   "l2l": "k_l2l_h", "m2p": "k_m2p", "m2qbxl": "k_m2qbxl_mfma"}
     rl = None
     if dominant in flops:
         achieved = flops[dominant] / (ps[dominant] * 1e-3) / 1e12
-        # compute-bound FP64 work; MI355X's FP64 vector and FP64 matrix peaks
-        # are the same 78.6 TFLOP/s, the figure this fraction is taken against
-        rl = {"bound": "mfma", "compute": "fp64", "kernel": dominant,
+        achieved40 = flops40[dominant] / (ps[dominant] * 1e-3) / 1e12
+        # FP64 compute; MI355X's FP64 vector and FP64 matrix peaks are the
+        # same 78.6 TFLOP/s, the figure this fraction is taken against
+        rl = {"bound": "mfma" if dominant == "m2qbxl" else "valu", "compute": "fp64",
+              "kernel": kernel_name.get(dominant, dominant), "stage": dominant,
               "achieved": round(achieved, 3),
               "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
               "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+              "frac_special40": round(achieved40 / FP64_PEAK_TFLOPS, 4),
               "traffic": measured_traffic(dominant),
               "flops_per_launch": int(flops[dominant]),
-              "avg_launch_ms": round(ps[dominant], 4)}
+              "flops_per_launch_special40": int(flops40[dominant]),
+              "avg_launch_ms": round(ps[dominant], 4),
+              "timing": "HIP events on the launch stream, stage-by-stage run of the same "
+                        "evaluation on one stream"}
         rec = profiled_stage(dominant)
         if rec and rec.get("hw_fp64_flops"):
             # the same launch time against the FP64 flops the hardware counted
@@ -886,8 +1164,10 @@ def main():
             rl["hw_achieved"] = round(hw, 3)
             rl["hw_frac"] = round(hw / FP64_PEAK_TFLOPS, 4)
     stage_rl = {k: {"ms": round(ps[k], 4), "gflop": round(flops[k] / 1e9, 3),
-                    "tflops": round(flops[k] / (ps[k] * 1e-3) / 1e12, 3),
-                    "frac_fp64": round(flops[k] / (ps[k] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4)}
+                    "gflop_special40": round(flops40[k] / 1e9, 3),
+                    "frac_fp64": round(flops[k] / (ps[k] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4),
+                    "frac_fp64_special40": round(flops40[k] / (ps[k] * 1e-3) / 1e12 /
+                                                 FP64_PEAK_TFLOPS, 4)}
                 for k in flops if ps.get(k, 0) > 0}
     line = {
         "metric": METRIC,
@@ -907,6 +1187,7 @@ def main():
             "tol": args.tol, "seed": cfg.seed},
         "rel_err_vs_direct_qbx": err["tol"],
         "rel_err_vs_gpu_direct_qbx_all_targets": err_all["tol"],
+        "fused_vs_stagewise_rel_diff": main_run["fused_vs_stagewise"],
         "gpu_direct_qbx_s": round(t_d, 3),
         "at_config_order": {
             "fmm_order": cfg.fmm_order,
@@ -929,11 +1210,28 @@ def main():
         "roofline": rl,
         "cpu_baseline": None if cpu is None else {
             k: (round(v, 1) if isinstance(v, float) else v) for k, v in cpu.items()
-            if k in ("value", "unit", "cores", "kind", "sample")},
+            if k in ("value", "unit", "cores", "kind", "sample", "host", "one_thread")},
     }
     if cpu is not None:
         line["gpu_vs_cpu_fmm_rel_diff"] = err["gpu_vs_cpu_fmm"]
         line["speedup_vs_cpu"] = round(line["value"] / cpu["value"], 2)
+    if not args.headline_only:
+        # the other workloads, each measured in this same run
+        others = {}
+        for name, fn, steps, warmup in (
+                ("laplace2d_starfish_c1", run_config1, args.steps, args.warmup),
+                ("laplace3d_sphere_32rhs_c3", run_sphere3d, args.steps, args.warmup),
+                ("dirichlet64_gmres_c4", run_dirichlet64, 2, 1),
+                ("helmholtz3d_ellipsoid_adjoint_c5", run_ellipsoid3d, 10, 2)):
+            t0 = time.perf_counter()
+            r = fn(argparse.Namespace(**{**vars(args), "steps": steps, "warmup": warmup}))
+            for k in ("metric", "n_gpus", "higher_is_better", "scaling", "vs_baseline",
+                      "data"):
+                r.pop(k, None)
+            r["wall_s"] = round(time.perf_counter() - t0, 1)
+            others[name] = r
+            log(f"[{name}] done in {r['wall_s']}s: {r.get('value')} {r.get('unit')}")
+        line["other_workloads"] = others
     print(json.dumps(line), flush=True)
 
 

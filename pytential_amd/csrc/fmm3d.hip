@@ -1371,7 +1371,6 @@ __global__ void __launch_bounds__(256) k3_p2qbxl_mr(KD kd, TD t, const int *ubox
 }
 
 constexpr int M2Q_LANES = 16;
-constexpr int M2Q_CENTERS = 256 / M2Q_LANES;
 
 // A lane group lives inside one wavefront: order its LDS writes before the
 // group's reads (and reads before the next overwrite) without a block barrier,
@@ -2110,12 +2109,19 @@ int f3_translate_box_multipoles_to_qbx_local(qbx_ctx *ctx, void *stream, const q
     const int ng = G->nglobal;
     if (ng == 0 || R->n3 == 0) return QBX_OK;
     const int pn = kd.pmax + kd.pq;
-    const size_t lds = sizeof(double) * 2 * (size_t)M2Q_CENTERS * (pn + 1) * (pn + 1);
+    // one harmonic table per 16-lane center group: 16 groups per block, or
+    // 8 / 4 for the orders whose tables would not fit 64 KiB of LDS
+    int block = 256;
+    size_t lds = sizeof(double) * 2 * (size_t)(block / M2Q_LANES) * (pn + 1) * (pn + 1);
+    while (lds > 64 * 1024 && block > 64) {
+        block /= 2;
+        lds /= 2;
+    }
     QBX_CHECK(lds <= 64 * 1024, "M2QBXL: orders too high for LDS staging (%d + %d)", kd.pmax,
               kd.pq);
     const TD td = make_td(T);
 #define L3_M2Q(N)                                                                          \
-    DISPATCH3_PQ_NR(kd.pq, N, k3_m2qbxl, dim3(cdiv((long)ng * M2Q_LANES, 256)), dim3(256), \
+    DISPATCH3_PQ_NR(kd.pq, N, k3_m2qbxl, dim3(cdiv((long)ng * M2Q_LANES, block)), dim3(block), \
                     lds, st, kd, td, ng, G->global_qbx_centers.p, G->center_to_target_box.p, \
                     G->centers.p, G->ncenters, R->l3_starts.p, R->l3_lists.p, mpole, qexp)
     FOR_RHS_CHUNKS3(kd, NRC_Q, L3_M2Q);

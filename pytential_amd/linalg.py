@@ -17,6 +17,7 @@ typically a captured-graph QBX-FMM matvec (``MatVecGraph``).
 """
 from __future__ import annotations
 
+import gc
 from dataclasses import dataclass
 from typing import Callable, Sequence
 
@@ -173,8 +174,19 @@ class MatVecGraph:
             fn(self._in)                        # warm-up: tables, workspaces
             torch.cuda.synchronize()
             self.graph = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(self.graph):
-                self._out = fn(self._in)
+            # Unreachable cycles holding library handles (a source and its
+            # cached geometry reference each other) free device memory in
+            # their finalizers; a free during the capture would invalidate
+            # it.  Collect them now and keep the collector off while capturing.
+            gc.collect()
+            was_enabled = gc.isenabled()
+            gc.disable()
+            try:
+                with torch.cuda.graph(self.graph):
+                    self._out = fn(self._in)
+            finally:
+                if was_enabled:
+                    gc.enable()
             torch.cuda.synchronize()
 
     def matvec(self, x):

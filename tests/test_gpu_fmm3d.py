@@ -32,6 +32,7 @@ CASES3 = {
     "S-p8-w64-f5": ("S", 8, 64, 5),
     "D-p8-w64-f5": ("D", 8, 64, 5),
     "S-p12-w512-f6": ("S", 12, 512, 6),
+    "S-p14-w512-f5": ("S", 14, 512, 5),
 }
 
 
