@@ -264,6 +264,24 @@ int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
                   const qbx_kernel_desc *kd, const double *strengths_tree,
                   const double *normals_tree, double *out_user);
 
+/* qbx_drive_fmm without allocation.  No reference counterpart: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * stage order and results as qbx_drive_fmm (2D Laplace and 2D Helmholtz
+ * only; other kernels return QBX_ERR_UNSUPPORTED), with every stage on the
+ * caller's stream and its workspace (expansions, potentials, Helmholtz
+ * translation tables) in one device array of QBX_NOALLOC_SCRATCH_BYTES per
+ * device that the library holds until the process ends; an evaluation that
+ * needs more returns QBX_ERR_INVALID.  It calls no hipMalloc or hipFree and
+ * creates no stream or event.  All calls on one device share the array and
+ * it keeps the last call's contents (each call clears what it needs): calls
+ * that overlap on one device race. */
+#define QBX_NOALLOC_SCRATCH_BYTES (256L << 20)
+int qbx_drive_fmm_noalloc(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
+                          const qbx_trav *trav, const qbx_geo *geo,
+                          const qbx_kernel_desc *kd, const double *strengths_tree,
+                          const double *normals_tree, double *out_user);
+
 /* ---- direct QBX (no FMM) ---------------------------------------------------
  * Replaces exec_compute_potential_insn_direct (pytential/qbx/__init__.py:
  * 866-1068) with LayerPotentialOnTargetAndCenterSubset (qbx/direct.py:41-154):

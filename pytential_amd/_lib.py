@@ -80,6 +80,7 @@ _SIGS = {
     "qbx_eval_qbx_expansions": [P, P, P, P, ct.POINTER(KernelDesc), P, P],
     "qbx_finalize_potentials": [P, P, P, P, I, P, P],
     "qbx_drive_fmm": [P, P, P, P, P, ct.POINTER(KernelDesc), P, P, P],
+    "qbx_drive_fmm_noalloc": [P, P, P, P, P, ct.POINTER(KernelDesc), P, P, P],
     "qbx_eval_target_specific_qbx_locals": [P, P, P, P, P, I, I, D, D, I, P, P, P, P],
     "qbx_direct_qbx": [P, P, P, P, ct.POINTER(KernelDesc), P, P, P],
     "qbx_p2p_all": [P, P, ct.POINTER(KernelDesc), I, P, P, P, I, P, P],

@@ -239,8 +239,11 @@ int qbx_ctx_create(qbx_ctx **ctx, int device)
 {
     QBX_CHECK(ctx, "qbx_ctx_create: null output");
     QBX_HIP(hipSetDevice(device));
+    double *scr = nullptr;
+    QBX_TRY(noalloc_scratch_address(device, &scr));
     qbx_ctx *c = new qbx_ctx();
     c->device = device;
+    c->noalloc_scratch = scr;
     *ctx = c;
     return QBX_OK;
 }

@@ -40,6 +40,7 @@
 #include "qbx_math.h"
 
 #include <cstring>
+#include <mutex>
 
 namespace qbx {
 
@@ -2772,25 +2773,43 @@ __global__ void k_finalize(int nt, int nc, int nr, const int *user_to_tree, cons
 // -------------------------------------------------------------------------
 // host side
 // -------------------------------------------------------------------------
-static double *g_binom_dev[64] = {nullptr};
+// Pascal's triangle to BINOM_N rows, generated at compile time by the
+// additions of the recurrence (IEEE round to nearest, as at run time) into an
+// initialised device array: no allocation, no copy, nothing to order against
+// the streams that read it.  The array has external linkage and is not const:
+// kernels reach it only through the pointer the host looks up, and the device
+// compiler drops an internal constant that no device code names (the
+// runtime's symbol lookup then aborts).
+struct BinomTab {
+    double v[BINOM_N * BINOM_N];
+};
+
+constexpr BinomTab make_binom()
+{
+    BinomTab t{};
+    for (int n = 0; n < BINOM_N; ++n) {
+        t.v[n * BINOM_N] = 1.0;
+        for (int k = 1; k <= n; ++k)
+            t.v[n * BINOM_N + k] = t.v[(n - 1) * BINOM_N + k - 1] +
+                                   (k < n ? t.v[(n - 1) * BINOM_N + k] : 0.0);
+    }
+    return t;
+}
+
+__device__ BinomTab qbx_binom_tab = make_binom();
+static const double *g_binom_dev[64] = {nullptr};
 
 static int binom_table(const double **out)
 {
+    static std::mutex mu;
     int dev = 0;
     QBX_HIP(hipGetDevice(&dev));
     QBX_CHECK(dev >= 0 && dev < 64, "device index out of range");
+    std::lock_guard<std::mutex> lock(mu);
     if (!g_binom_dev[dev]) {
-        std::vector<double> h((size_t)BINOM_N * BINOM_N, 0.0);
-        for (int n = 0; n < BINOM_N; ++n) {
-            h[(size_t)n * BINOM_N] = 1.0;
-            for (int k = 1; k <= n; ++k)
-                h[(size_t)n * BINOM_N + k] = h[(size_t)(n - 1) * BINOM_N + k - 1] +
-                                             (k < n ? h[(size_t)(n - 1) * BINOM_N + k] : 0.0);
-        }
-        double *p;
-        QBX_HIP(hipMalloc(&p, sizeof(double) * h.size()));
-        QBX_HIP(hipMemcpy(p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
-        g_binom_dev[dev] = p;
+        void *p = nullptr;
+        QBX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(qbx_binom_tab)));
+        g_binom_dev[dev] = (const double *)p;
     }
     *out = g_binom_dev[dev];
     return QBX_OK;
@@ -2865,6 +2884,20 @@ static int ensure_tables(qbx_ctx *ctx, hipStream_t st, const qbx_tree *T, KD *kd
     const int ncls = (2 * kd->m2l_w + 1) * (2 * kd->m2l_w + 1);
     for (int l = 0; l < kd->nlevels; ++l) nm2l += (long)ncls * (4 * kd->order[l] + 1);
     for (int l = 1; l < kd->nlevels; ++l) nmm += 4L * (2 * (kd->order[l] + kd->order[l - 1]) + 1);
+    const int ncls_ = (2 * kd->m2l_w + 1) * (2 * kd->m2l_w + 1);
+    if (ctx->tab_m2l_ext) {
+        // allocation-free call: build into its scratch every time (other
+        // calls on the device may have overwritten the array since)
+        hipLaunchKernelGGL(k_tab_m2l, dim3(cdiv(kd->nlevels * ncls_, 64)), dim3(64), 0, st, *kd,
+                           kd->nlevels, ctx->tab_m2l_ext);
+        if (kd->nlevels > 1)
+            hipLaunchKernelGGL(k_tab_mm, dim3(cdiv(4 * (kd->nlevels - 1), 64)), dim3(64), 0, st,
+                               *kd, kd->nlevels, ctx->tab_mm_ext);
+        QBX_HIP(hipGetLastError());
+        kd->tab_m2l = ctx->tab_m2l_ext;
+        kd->tab_mm = ctx->tab_mm_ext;
+        return QBX_OK;
+    }
     QBX_TRY(note_capture(ctx, st));
     if (key != ctx->tab_key) {
         // a graph replays the tables it captured: never rebuild under one
@@ -3733,7 +3766,7 @@ int qbx_direct_qbx(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_geo 
     }
     if (G->nglobal > 0) {
         const long nq = 2L * std::max(G->ncenters, 1) * kd.ncq * nr;
-        if (ctx->ws_qbx.n < nq) QBX_TRY(ctx->ws_qbx.alloc(nq));
+        QBX_TRY(ws_reserve(ctx, st, ctx->ws_qbx, nq, "QBX expansion workspace"));
         double *Q = ctx->ws_qbx.p;
         QBX_TRY(zero_fill(Q, nq, st));
 #define L_DP2Q(N)                                                                          \
@@ -3820,24 +3853,93 @@ int qbx_finalize_potentials(qbx_ctx *ctx, void *stream, const qbx_tree *T, const
     return QBX_OK;
 }
 
-int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
-                  const qbx_geo *G, const qbx_kernel_desc *d, const double *str,
-                  const double *nrm, double *out_user)
+}  // extern "C"
+
+// The allocation-free driver's workspace: one array per device, loaded with
+// the code object and kept until the process ends.  Host code never uses the
+// name as an address (on the host it is only a handle for the runtime):
+// noalloc_scratch_address looks up each device's own copy once, with that
+// device current, and the driver passes that address to its kernels.  Every
+// qbx_drive_fmm_noalloc call on a device shares the array, so calls that
+// overlap on one device race.
+static __device__ __align__(256) double scratch[QBX_NOALLOC_SCRATCH_BYTES / 8];
+
+namespace qbx {
+int noalloc_scratch_address(int device, double **out)
 {
-    QBX_CHECK(ctx && T && R && G && d && str && out_user, "qbx_drive_fmm: null argument");
-    const long nexp = qbx_expansion_size(d, T);
+    static std::mutex mu;
+    static double *addr[64] = {nullptr};
+    QBX_CHECK(device >= 0 && device < 64, "device index %d out of range", device);
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        int cur = 0;
+        QBX_HIP(hipGetDevice(&cur));
+        QBX_HIP(hipSetDevice(device));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        QBX_HIP(hipSetDevice(cur));
+        QBX_HIP(e);
+        QBX_CHECK(p, "hipGetSymbolAddress returned a null scratch address");
+        addr[device] = (double *)p;
+    }
+    *out = addr[device];
+    return QBX_OK;
+}
+}  // namespace qbx
+
+// Workspace of one fused evaluation: multipoles M, locals L, the list-4
+// locals L4, QBX expansions Q and tree-order potentials P.
+struct DriveBufs {
+    double *M, *L, *L4, *Q, *P;
+};
+
+struct DriveSizes {
+    long nexp, nm, nq, np;
+};
+
+static int drive_sizes(const qbx_tree *T, const qbx_geo *G, const qbx_kernel_desc *d,
+                       DriveSizes *z)
+{
+    z->nexp = qbx_expansion_size(d, T);
     const int ncq = qbx_ncoeff_qbx(d);
-    if (nexp < 0 || ncq < 0) return QBX_ERR_INVALID;
+    if (z->nexp < 0 || ncq < 0) return QBX_ERR_INVALID;
     const int nr = d->nrhs > 0 ? d->nrhs : 1;
-    const long nm = 2L * std::max(nexp, 1L);
-    const long nq = 2L * std::max(G->ncenters, 1) * ncq * nr;
-    const long np = 2L * std::max(T->ntargets, 1) * nr;
-    QBX_TRY(ws_reserve(ctx, S(stream), ctx->ws_mpole, nm, "multipole workspace"));
-    QBX_TRY(ws_reserve(ctx, S(stream), ctx->ws_local, nm, "local workspace"));
-    QBX_TRY(ws_reserve(ctx, S(stream), ctx->ws_local2, nm, "list-4 local workspace"));
-    QBX_TRY(ws_reserve(ctx, S(stream), ctx->ws_qbx, nq, "QBX expansion workspace"));
-    QBX_TRY(ws_reserve(ctx, S(stream), ctx->ws_pot, np, "potential workspace"));
-    double *M = ctx->ws_mpole.p, *L = ctx->ws_local.p, *Q = ctx->ws_qbx.p, *P = ctx->ws_pot.p;
+    z->nm = 2L * std::max(z->nexp, 1L);
+    z->nq = 2L * std::max(G->ncenters, 1) * ncq * nr;
+    z->np = 2L * std::max(T->ntargets, 1) * nr;
+    return QBX_OK;
+}
+
+// The stage DAG of qbx_drive_fmm.  multistream: on the caller's stream plus
+// the device's three side streams joined by this context's events (below);
+// otherwise every stage on the caller's stream in the order of drive_fmm.
+static int drive_impl(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                      const qbx_geo *G, const qbx_kernel_desc *d, const double *str,
+                      const double *nrm, double *out_user, const DriveSizes &z,
+                      const DriveBufs &b, bool multistream)
+{
+    const int nr = d->nrhs > 0 ? d->nrhs : 1;
+    const long nexp = z.nexp, np = z.np;
+    double *M = b.M, *L = b.L, *Q = b.Q, *P = b.P, *L4 = b.L4;
+    hipStream_t st = S(stream);
+    if (!multistream) {
+        QBX_TRY(zero_fill(P, np, st));
+        QBX_TRY(qbx_form_multipoles(ctx, st, T, R, d, str, nrm, M));
+        QBX_TRY(qbx_eval_direct(ctx, st, T, R, G, d, str, nrm, P, 1));
+        QBX_TRY(qbx_form_global_qbx_locals(ctx, st, T, R, G, d, str, nrm, Q, 0));
+        QBX_TRY(qbx_form_locals(ctx, st, T, R, d, str, nrm, L4, 0));
+        QBX_TRY(coarsen_multipoles_from(ctx, st, T, R, d, M, R->mpole_min_level));
+        QBX_TRY(qbx_multipole_to_local(ctx, st, T, R, d, M, L));
+        QBX_TRY(qbx_eval_multipoles(ctx, st, T, R, G, d, M, P, 1));
+        QBX_TRY(qbx_translate_box_multipoles_to_qbx_local(ctx, st, T, R, G, d, M, Q, 1));
+        QBX_TRY(add_into(L, L4, nexp * 2, st));
+        QBX_TRY(refine_locals_from(ctx, st, T, R, d, L, R->local_min_level));
+        QBX_TRY(qbx_eval_locals(ctx, st, T, R, G, d, L, P, 1));
+        QBX_TRY(qbx_translate_box_local_to_qbx_local(ctx, st, T, R, G, d, L, Q, 1));
+        QBX_TRY(qbx_eval_qbx_expansions(ctx, st, T, G, d, Q, P));
+        QBX_TRY(qbx_finalize_potentials(ctx, stream, T, G, nr, P, out_user));
+        return QBX_OK;
+    }
     if (!ctx->side[0]) {
         QBX_TRY(side_streams(ctx->device, ctx->side));
         for (hipEvent_t &e : ctx->ev) QBX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -3863,10 +3965,9 @@ int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav 
     // M2M, start after M2L: M2L then has the chip to itself and they fill the
     // idle slots of the L2L launches (measured 4% faster than starting them
     // beside M2L).
-    hipStream_t st = S(stream), s1 = ctx->side[0], s2 = ctx->side[1], s3 = ctx->side[2];
+    hipStream_t s1 = ctx->side[0], s2 = ctx->side[1], s3 = ctx->side[2];
     hipEvent_t e0 = ctx->ev[0], eM = ctx->ev[1], eL = ctx->ev[2], e1 = ctx->ev[3],
                e2 = ctx->ev[4], eP = ctx->ev[5];
-    double *L4 = ctx->ws_local2.p;
     QBX_TRY(zero_fill(P, np, st));
     bool done;
     {
@@ -3909,6 +4010,70 @@ int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav 
     QBX_HIP(hipStreamWaitEvent(st, e1, 0));
     QBX_TRY(qbx_finalize_potentials(ctx, stream, T, G, nr, P, out_user));
     return QBX_OK;
+}
+
+
+extern "C" {
+
+int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                  const qbx_geo *G, const qbx_kernel_desc *d, const double *str,
+                  const double *nrm, double *out_user)
+{
+    QBX_CHECK(ctx && T && R && G && d && str && out_user, "qbx_drive_fmm: null argument");
+    DriveSizes z;
+    QBX_TRY(drive_sizes(T, G, d, &z));
+    QBX_TRY(ws_reserve(ctx, S(stream), ctx->ws_mpole, z.nm, "multipole workspace"));
+    QBX_TRY(ws_reserve(ctx, S(stream), ctx->ws_local, z.nm, "local workspace"));
+    QBX_TRY(ws_reserve(ctx, S(stream), ctx->ws_local2, z.nm, "list-4 local workspace"));
+    QBX_TRY(ws_reserve(ctx, S(stream), ctx->ws_qbx, z.nq, "QBX expansion workspace"));
+    QBX_TRY(ws_reserve(ctx, S(stream), ctx->ws_pot, z.np, "potential workspace"));
+    const DriveBufs b{ctx->ws_mpole.p, ctx->ws_local.p, ctx->ws_local2.p, ctx->ws_qbx.p,
+                      ctx->ws_pot.p};
+    return drive_impl(ctx, stream, T, R, G, d, str, nrm, out_user, z, b, true);
+}
+
+int qbx_drive_fmm_noalloc(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                          const qbx_geo *G, const qbx_kernel_desc *d, const double *str,
+                          const double *nrm, double *out_user)
+{
+    QBX_CHECK(ctx && T && R && G && d && str && out_user,
+              "qbx_drive_fmm_noalloc: null argument");
+    QBX_CHECK(ctx->noalloc_scratch, "qbx_drive_fmm_noalloc: context has no scratch address");
+    if (d->kernel != QBX_LAPLACE2D && d->kernel != QBX_HELMHOLTZ2D) {
+        set_error("qbx_drive_fmm_noalloc: 2D Laplace and 2D Helmholtz only (got kernel %d)",
+                  d->kernel);
+        return QBX_ERR_UNSUPPORTED;
+    }
+    DriveSizes z;
+    QBX_TRY(drive_sizes(T, G, d, &z));
+    KD kd;
+    QBX_TRY(make_kd(d, T, &kd, R->nsep));
+    // Helmholtz translation tables (the sizes ensure_tables builds)
+    long ntab_m2l = 0, ntab_mm = 0;
+    if (kd.kernel == QBX_HELMHOLTZ2D) {
+        const int ncls = (2 * kd.m2l_w + 1) * (2 * kd.m2l_w + 1);
+        for (int l = 0; l < kd.nlevels; ++l) ntab_m2l += 2L * ncls * (4 * kd.order[l] + 1);
+        for (int l = 1; l < kd.nlevels; ++l)
+            ntab_mm += 8L * (2 * (kd.order[l] + kd.order[l - 1]) + 1);
+    }
+    auto up = [](long n) { return (n + 31) & ~31L; };   // 256-byte aligned pieces
+    const long need = 3 * up(z.nm) + up(z.nq) + up(z.np) + up(ntab_m2l) + up(std::max(ntab_mm, 1L));
+    QBX_CHECK(need <= QBX_NOALLOC_SCRATCH_BYTES / 8,
+              "qbx_drive_fmm_noalloc: this evaluation needs %ld bytes of scratch, more than the "
+              "library's %ld (QBX_NOALLOC_SCRATCH_BYTES); use qbx_drive_fmm",
+              need * 8, (long)QBX_NOALLOC_SCRATCH_BYTES);
+    double *p = ctx->noalloc_scratch;
+    DriveBufs b;
+    b.M = p; p += up(z.nm);
+    b.L = p; p += up(z.nm);
+    b.L4 = p; p += up(z.nm);
+    b.Q = p; p += up(z.nq);
+    b.P = p; p += up(z.np);
+    ctx->tab_m2l_ext = p; p += up(ntab_m2l);
+    ctx->tab_mm_ext = p;
+    const int rc = drive_impl(ctx, stream, T, R, G, d, str, nrm, out_user, z, b, false);
+    ctx->tab_m2l_ext = ctx->tab_mm_ext = nullptr;
+    return rc;
 }
 
 }  // extern "C"

@@ -106,6 +106,13 @@ struct qbx_ctx {
     // Helmholtz translation tables and the (tree, k, orders) they were built for
     qbx::dbuf<double> ws_tab_m2l, ws_tab_mm;
     std::vector<double> tab_key;
+    // set only for the duration of a qbx_drive_fmm_noalloc call: the tables
+    // are then built into this memory (inside the device scratch array) on
+    // every call instead of into ws_tab_*
+    double *tab_m2l_ext = nullptr, *tab_mm_ext = nullptr;
+    // this device's allocation-free scratch array (hipGetSymbolAddress at
+    // context creation), used only by qbx_drive_fmm_noalloc
+    double *noalloc_scratch = nullptr;
     // 3D Laplace unit-offset harmonic tables and the (nsep, pmax) they hold
     qbx::dbuf<double> ws_tab3_m2l, ws_tab3_ch;
     std::vector<double> tab3_key;
@@ -178,6 +185,9 @@ int scratch_reserve(qbx_ctx *ctx, hipStream_t st, size_t bytes, void **p);
 // with many contexts (one per captured operator) does not spread its
 // streams over the hardware queues the main stream uses
 int side_streams(int device, hipStream_t out[3]);
+// the device address of this device's allocation-free scratch array (looked
+// up once per device, with the device current)
+int noalloc_scratch_address(int device, double **out);
 // marks the context as held by a graph when st is capturing
 int note_capture(qbx_ctx *ctx, hipStream_t st);
 // QBX_OK when the context's buffers may be replaced (no graph holds them),

@@ -306,3 +306,16 @@ def drive_fmm_fused(wrangler: QBXExpansionWrangler, strengths, normals, out=None
     call("qbx_drive_fmm", w.ctx.h, _stream(), w.tree.h, w.traversal.h, w.geo.h,
          ct.byref(w.kd), _p(strengths), _p(normals), _p(out))
     return out
+
+
+def drive_fmm_noalloc(wrangler: QBXExpansionWrangler, strengths, normals, out):
+    """The same evaluation through ``qbx_drive_fmm_noalloc``: the library
+    allocates nothing (its workspace is a fixed device array shared by every
+    such call on the device); ``out`` ([n_user_targets] or
+    [n_user_targets, nrhs] complex128) is the caller's.  2D kernels only."""
+    w = wrangler
+    if w.using_tsqbx:
+        raise ValueError("the fused driver has no target-specific QBX stage; use drive_fmm")
+    call("qbx_drive_fmm_noalloc", w.ctx.h, _stream(), w.tree.h, w.traversal.h, w.geo.h,
+         ct.byref(w.kd), _p(strengths), _p(normals), _p(out))
+    return out
