@@ -156,6 +156,15 @@ typedef struct {
     int nlevels;
     const int *level_orders;/* host array [nlevels]: fmm_level_to_order */
     int nrhs;               /* right-hand sides per evaluation (0 is taken as 1) */
+    /* Target derivative, sumpy's DirectionalTargetDerivative as pytential's
+     * Sp / Dp apply it (symbolic/primitives.py:2303-2465, the target normal
+     * as the direction): 0 = the potential; 1 = its derivative along
+     * target_dir at each target (2D kernels; the FMM evaluation stages P2P,
+     * M2P, L2P, QBXL2P and direct QBX; the expansions are unchanged).
+     * target_dir: SoA doubles [2][ntargets] over the tree's targets in tree
+     * order (the centers' entries are not read). */
+    int target_deriv;
+    const double *target_dir;
 } qbx_kernel_desc;
 
 /* Box expansions (multipoles and locals) are level-compact: the boxes of

@@ -81,7 +81,8 @@ class OrcTrav(ct.Structure):
 
 class KParams(ct.Structure):
     _fields_ = [("kernel", ct.c_int), ("k", ct.c_double), ("source_kind", ct.c_int),
-                ("qbx_order", ct.c_int), ("level_orders", IP), ("nthreads", ct.c_int)]
+                ("qbx_order", ct.c_int), ("level_orders", IP), ("nthreads", ct.c_int),
+                ("target_deriv", ct.c_int), ("target_dir", P)]
 
 
 def _declare(L):
@@ -260,12 +261,16 @@ KERNEL_CODES = {("laplace", 2): 0, ("helmholtz", 2): 1, ("laplace", 3): 2}
 
 
 def kparams(kernel="laplace", k=0.0, source_kind="S", qbx_order=4,
-            level_orders=None, nthreads=0, dim=2):
+            level_orders=None, nthreads=0, dim=2, target_dir=None):
+    """``target_dir`` (dim, n), indexed like the targets of the call: evaluate
+    the derivative along it instead of the potential (2D)."""
     lo = _i32(level_orders if level_orders is not None else [0])
+    td = None if target_dir is None else _f64(target_dir)
     kp = KParams(KERNEL_CODES[(kernel, dim)], float(k),
                  0 if source_kind == "S" else 1, qbx_order,
-                 lo.ctypes.data_as(IP), nthreads)
+                 lo.ctypes.data_as(IP), nthreads, int(td is not None), _ptr(td))
     kp._lo = lo
+    kp._td = td
     return kp
 
 
@@ -392,9 +397,15 @@ class GeometryData:
         self.setup_times = {"target_assoc": t1 - t0, "tree+traversal": t2 - t1}
 
     def fmm(self, kernel, k, source_kind, qbx_order, level_orders, strengths,
-            nthreads=0, return_stages=False):
+            nthreads=0, return_stages=False, target_dir=None):
+        """``target_dir`` (dim, n_user_targets): the derivative along it at
+        each user target instead of the potential."""
         dim = self.discr.nodes.shape[0]
-        kp = kparams(kernel, k, source_kind, qbx_order, level_orders, nthreads, dim)
+        td = None
+        if target_dir is not None:
+            td = np.concatenate([np.zeros((dim, self.ncenters)),
+                                 np.asarray(target_dir, np.float64)], axis=1)
+        kp = kparams(kernel, k, source_kind, qbx_order, level_orders, nthreads, dim, td)
         if not return_stages:
             out, times = qbx_fmm(self.tree, self.trav, kp, self.discr.quad_nodes,
                                  self.discr.quad_normals, strengths, self.centers,
@@ -419,14 +430,17 @@ class GeometryData:
         return out[self.ncenters:], {"mpole": mp, "local": lc, "qbx": qx, "times": times}
 
     def direct(self, kernel, k, source_kind, qbx_order, strengths, target_idx=None,
-               nthreads=0):
-        kp = kparams(kernel, k, source_kind, qbx_order, [0], nthreads,
-                     self.discr.nodes.shape[0])
+               nthreads=0, target_dir=None):
         tg = self.user_targets
         ttc = self.target_to_center
+        td = None if target_dir is None else np.asarray(target_dir, np.float64)
         if target_idx is not None:
             tg = tg[:, target_idx]
             ttc = ttc[target_idx]
+            if td is not None:
+                td = td[:, target_idx]
+        kp = kparams(kernel, k, source_kind, qbx_order, [0], nthreads,
+                     self.discr.nodes.shape[0], td)
         return direct_qbx(kp, self.discr.quad_nodes, self.discr.quad_normals,
                           strengths, self.centers, self.radii, tg, ttc)
 

@@ -926,6 +926,38 @@ static void p2e_one(const orc_kparams *kp, int local, int p, cplx c, double R,
     }
 }
 
+/* E2P of the derivative along d (complex dx + i dy) at t.  Laplace: the
+ * potential is -Re F / (2 pi) with F analytic, the derivative -Re(F' d) /
+ * (2 pi), F' of the local sum a_l ((t - c)/R)^l and of the multipole
+ * a_0 log w + sum a_k (R/w)^k differentiated term by term.  Helmholtz: the
+ * derivative of Z_n(k rho) e^{in theta} along d is
+ * (k/2) (d Z_{n-1} e^{i(n-1) theta} - conj(d) Z_{n+1} e^{i(n+1) theta})
+ * (Graf's recurrences), summed over the coefficients. */
+static cplx e2p_dir(const orc_kparams *kp, int local, int p, cplx c, double R,
+                    const cplx *coef, cplx t, cplx d)
+{
+    if (kp->kernel == 0) {
+        cplx w = t - c, fp = 0;
+        if (local) {
+            cplx x = w / R, pw = 1.0;
+            for (int l = 1; l <= p; ++l) { fp += (double)l * coef[l] * pw / R; pw *= x; }
+        } else {
+            cplx x = R / w, pw = x;
+            fp = coef[0] / w;
+            for (int kk = 1; kk <= p; ++kk) { fp -= (double)kk * coef[kk] * pw / w; pw *= x; }
+        }
+        return -creal(fp * d) / (2 * PI);
+    } else {
+        cplx g[2 * 131 + 1];
+        int m = p + 1;
+        cyl_waves(!local, kp->k, t - c, m, g);
+        cplx phi = 0;
+        for (int n = -p; n <= p; ++n)
+            phi += coef[n + p] * 0.5 * kp->k * (d * g[n - 1 + m] - conj(d) * g[n + 1 + m]);
+        return 0.25 * JI * phi;
+    }
+}
+
 /* E2P: evaluate; returns the potential (kernel scaling included) */
 static cplx e2p_one(const orc_kparams *kp, int local, int p, cplx c, double R,
                     const cplx *coef, cplx t)
@@ -1014,6 +1046,31 @@ static void e2e_one(const orc_kparams *kp, int kind, int p1, int p2,
     }
 }
 
+/* P2P of the derivative along d at t (closed forms of the kernels'
+ * gradients in t; w = t - s, w^ = w / r):
+ *   Laplace S -Re(d / w) / (2 pi), D -Re(nu d / w^2) / (2 pi)
+ *   Helmholtz S -(i/4) k (d . w^) H1, D (i/4) k [H1 (d . nu - (d . w^)(nu . w^)) / r
+ *     + k (nu . w^)(d . w^) H1'(k r)], H1' = H0 - H1 / x */
+static cplx p2p_dir(const orc_kparams *kp, cplx s, cplx q, cplx nu, cplx t, cplx d)
+{
+    cplx w = t - s;
+    double r = cabs(w);
+    if (r == 0.0) return 0;
+    if (kp->kernel == 0) {
+        if (kp->source_kind == 0) return -creal(d / w) * q / (2 * PI);
+        return -creal(nu * d / (w * w)) * q / (2 * PI);
+    } else {
+        double kr = kp->k * r;
+        double pd = (creal(w) * creal(d) + cimag(w) * cimag(d)) / r;
+        cplx h0 = j0(kr) + JI * y0(kr), h1 = j1(kr) + JI * y1(kr);
+        if (kp->source_kind == 0) return -0.25 * JI * q * kp->k * pd * h1;
+        double pn = (creal(w) * creal(nu) + cimag(w) * cimag(nu)) / r;
+        double dn = creal(d) * creal(nu) + cimag(d) * cimag(nu);
+        cplx v = h1 * (dn - pd * pn) / r + kp->k * pn * pd * (h0 - h1 / kr);
+        return 0.25 * JI * q * kp->k * v;
+    }
+}
+
 static cplx p2p_one(const orc_kparams *kp, cplx s, cplx q, cplx nu, cplx t)
 {
     cplx w = t - s;
@@ -1028,6 +1085,24 @@ static cplx p2p_one(const orc_kparams *kp, cplx s, cplx q, cplx nu, cplx t)
         double proj = (creal(w) * creal(nu) + cimag(w) * cimag(nu)) / r;
         return 0.25 * JI * q * kp->k * (j1(kr) + JI * y1(kr)) * proj;
     }
+}
+
+/* the potential, or its derivative along d when kp->target_deriv */
+static inline cplx e2p_at(const orc_kparams *kp, int local, int p, cplx c, double R,
+                          const cplx *coef, cplx t, cplx d)
+{
+    return kp->target_deriv ? e2p_dir(kp, local, p, c, R, coef, t, d)
+                            : e2p_one(kp, local, p, c, R, coef, t);
+}
+
+static inline cplx p2p_at(const orc_kparams *kp, cplx s, cplx q, cplx nu, cplx t, cplx d)
+{
+    return kp->target_deriv ? p2p_dir(kp, s, q, nu, t, d) : p2p_one(kp, s, q, nu, t);
+}
+
+static inline cplx dir_at(const orc_kparams *kp, int n, int i)
+{
+    return kp->target_deriv ? cpt(kp->target_dir, n, i) : 0;
 }
 
 /* ------------------------------------------------------------------ */
@@ -1055,7 +1130,8 @@ int orc_e2p(const orc_kparams *kp, int local, int order, const double *c, double
     if (kp->kernel == 2) return orc_e2p3(local, order, c, coeffs, nt, tgt, pot);
     cplx cc = c[0] + JI * c[1];
     for (int t = 0; t < nt; ++t) {
-        cplx v = e2p_one(kp, local, order, cc, rscale, (const cplx *)coeffs, cpt(tgt, nt, t));
+        cplx v = e2p_at(kp, local, order, cc, rscale, (const cplx *)coeffs, cpt(tgt, nt, t),
+                        dir_at(kp, nt, t));
         pot[2 * t] = creal(v); pot[2 * t + 1] = cimag(v);
     }
     return 0;
@@ -1084,7 +1160,7 @@ int orc_p2p(const orc_kparams *kp, int ns, const double *src, const double *nrm,
         for (int s = 0; s < ns; ++s) {
             cplx q = str[2 * s] + JI * str[2 * s + 1];
             cplx nu = nrm ? cpt(nrm, ns, s) : 0;
-            acc += p2p_one(kp, cpt(src, ns, s), q, nu, tt);
+            acc += p2p_at(kp, cpt(src, ns, s), q, nu, tt, dir_at(kp, nt, t));
         }
         pot[2 * t] = creal(acc); pot[2 * t + 1] = cimag(acc);
     }
@@ -1160,10 +1236,12 @@ int orc_qbx_fmm_ex(const orc_tree *t, const orc_trav *tr, const orc_kparams *kp,
         NU[i] = dip ? cpt(normals, ns, u) : 0;
     }
     cplx *T = (cplx *)malloc(sizeof(cplx) * (nT ? nT : 1));
+    cplx *DIR = (cplx *)malloc(sizeof(cplx) * (nT ? nT : 1));
     int *tree_ttc = (int *)malloc(sizeof(int) * (nT ? nT : 1));
     for (int i = 0; i < nT; ++i) {
         int u = t->sorted_target_ids[i];
         T[i] = cpt(targets, nT, u);
+        DIR[i] = dir_at(kp, nT, u);
         tree_ttc[i] = u < ncenters ? -2 : uttc[u];  /* -2: a center, not evaluated */
     }
     int *tgt_box_of_box = (int *)malloc(sizeof(int) * nb);
@@ -1221,7 +1299,7 @@ int orc_qbx_fmm_ex(const orc_tree *t, const orc_trav *tr, const orc_kparams *kp,
             for (int j = tr->l1_starts[i]; j < tr->l1_starts[i + 1]; ++j) {
                 int Sb = tr->l1_lists[j];
                 int s0 = t->box_source_starts[Sb], s1 = s0 + t->box_source_counts_nonchild[Sb];
-                for (int s = s0; s < s1; ++s) acc += p2p_one(kp, S[s], Q[s], NU[s], T[ti]);
+                for (int s = s0; s < s1; ++s) acc += p2p_at(kp, S[s], Q[s], NU[s], T[ti], DIR[ti]);
             }
             pot[ti] += acc;
         }
@@ -1250,7 +1328,8 @@ int orc_qbx_fmm_ex(const orc_tree *t, const orc_trav *tr, const orc_kparams *kp,
             cplx acc = 0;
             for (int j = tr->l3_starts[i]; j < tr->l3_starts[i + 1]; ++j) {
                 int D = tr->l3_lists[j];
-                acc += e2p_one(kp, 0, PL(D), BOXC(D), rsc[LEV(D)], M + (size_t)D * ncf, T[ti]);
+                acc += e2p_at(kp, 0, PL(D), BOXC(D), rsc[LEV(D)], M + (size_t)D * ncf, T[ti],
+                              DIR[ti]);
             }
             pot[ti] += acc;
         }
@@ -1290,7 +1369,8 @@ int orc_qbx_fmm_ex(const orc_tree *t, const orc_trav *tr, const orc_kparams *kp,
         int t0i = t->box_target_starts[B], t1i = t0i + t->box_target_counts_nonchild[B];
         for (int ti = t0i; ti < t1i; ++ti) {
             if (tree_ttc[ti] != -1) continue;
-            pot[ti] += e2p_one(kp, 1, PL(B), BOXC(B), rsc[LEV(B)], Lc + (size_t)B * ncf, T[ti]);
+            pot[ti] += e2p_at(kp, 1, PL(B), BOXC(B), rsc[LEV(B)], Lc + (size_t)B * ncf, T[ti],
+                              DIR[ti]);
         }
     }
     t1 = omp_get_wtime(); tm[8] = t1 - t0; t0 = t1;
@@ -1350,7 +1430,8 @@ int orc_qbx_fmm_ex(const orc_tree *t, const orc_trav *tr, const orc_kparams *kp,
     for (int ti = 0; ti < nT; ++ti) {
         int c = tree_ttc[ti];
         if (c < 0) continue;
-        pot[ti] = e2p_one(kp, 1, pq, cpt(centers, ncenters, c), radii[c], QX + (size_t)c * ncq, T[ti]);
+        pot[ti] = e2p_at(kp, 1, pq, cpt(centers, ncenters, c), radii[c], QX + (size_t)c * ncq,
+                         T[ti], DIR[ti]);
     }
     t1 = omp_get_wtime(); tm[12] = t1 - t0; t0 = t1;
 
@@ -1378,7 +1459,7 @@ int orc_qbx_fmm_ex(const orc_tree *t, const orc_trav *tr, const orc_kparams *kp,
 #undef BOXC
 #undef LEV
 #undef PL
-    free(S); free(Q); free(NU); free(T); free(tree_ttc); free(tgt_box_of_box); free(ttp_of_box);
+    free(S); free(Q); free(NU); free(T); free(DIR); free(tree_ttc); free(tgt_box_of_box); free(ttp_of_box);
     free(rsc); free(M); free(Lc); free(pot); free(used); free(center_tbox); free(QX);
     return 0;
 }
@@ -1416,12 +1497,12 @@ int orc_direct_qbx(int dim, const orc_kparams *kp,
                 cplx nu = dip ? cpt(nrm, ns, s) : 0;
                 p2e_one(kp, 1, pq, cc, radii[c], cpt(src, ns, s), q, nu, coef);
             }
-            v = e2p_one(kp, 1, pq, cc, radii[c], coef, tt);
+            v = e2p_at(kp, 1, pq, cc, radii[c], coef, tt, dir_at(kp, nt, it));
         } else {
             for (int s = 0; s < ns; ++s) {
                 cplx q = str[2 * s] + JI * str[2 * s + 1];
                 cplx nu = dip ? cpt(nrm, ns, s) : 0;
-                v += p2p_one(kp, cpt(src, ns, s), q, nu, tt);
+                v += p2p_at(kp, cpt(src, ns, s), q, nu, tt, dir_at(kp, nt, it));
             }
         }
         result[2 * it] = creal(v); result[2 * it + 1] = cimag(v);

@@ -87,6 +87,11 @@ typedef struct {
     int qbx_order;
     const int *level_orders;  /* [nlevels] */
     int nthreads;          /* 0 = OpenMP default */
+    /* 1: evaluate the derivative along target_dir at each target instead of
+     * the potential (2D; target_dir SoA [2][n] indexed like the targets array
+     * of the call) */
+    int target_deriv;
+    const double *target_dir;
 } orc_kparams;
 
 /* Evaluate the QBX-FMM.  Geometry arrays are in user order; the

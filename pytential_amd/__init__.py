@@ -18,5 +18,5 @@ def __getattr__(name):
         return getattr(qbx, name)
     if name in ("bind", "sym"):
         from . import symbolic
-        return getattr(symbolic, name) if name == "bind" else symbolic
+        return getattr(symbolic, name)
     raise AttributeError(name)

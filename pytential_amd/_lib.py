@@ -27,7 +27,8 @@ IP = ct.POINTER(ct.c_int)
 
 class KernelDesc(ct.Structure):
     _fields_ = [("kernel", I), ("k", D), ("source_kind", I), ("qbx_order", I),
-                ("nlevels", I), ("level_orders", IP), ("nrhs", I)]
+                ("nlevels", I), ("level_orders", IP), ("nrhs", I),
+                ("target_deriv", I), ("target_dir", P)]
 
 
 class QBXError(RuntimeError):

@@ -66,6 +66,9 @@ struct KD {
     const double *tab_mm;      // Helmholtz M2M/L2L table (centred rows)
     long m2l_off[FML];
     long mm_off[FML];
+    int tder;                  // evaluate the derivative along tdir at the targets
+    int ntgt;                  // stride of tdir (the tree's targets)
+    const double *tdir;        // [2][ntgt] tree order
 };
 
 struct TD {
@@ -171,8 +174,17 @@ __device__ __forceinline__ cplx cpow_int(cplx z, int m)
 // and its dipole (i/4) k (w . nu / r) H1(k r), w = t - s.  The Hankel series
 // length is chosen per wavefront; coincident points give zero.
 // -------------------------------------------------------------------------
+//
+// DER: the derivative of that value along the target direction d instead
+// (complex d = dx + i dy; w^ = w / r):
+//   Laplace S  -Re(d / w) / (2 pi)          Laplace D  -Re(nu d / w^2) / (2 pi)
+//   Helmholtz S  -(i/4) k (d . w^) H1(k r)
+//   Helmholtz D  (i/4) k [H1(k r) (d . nu - (d . w^)(nu . w^)) / r
+//                         + k (nu . w^)(d . w^) (H0(k r) - H1(k r) / (k r))]
+// (H1' = H0 - H1 / x).
+template <bool DER = false>
 __device__ __forceinline__ cplx p2p_value_wave(const KD &kd, cplx s, cplx nu, cplx t,
-                                            const double *tab)
+                                            const double *tab, cplx dir = {0.0, 0.0})
 {
     const cplx w = t - s;
     const double r2 = abs2(w);
@@ -181,25 +193,53 @@ __device__ __forceinline__ cplx p2p_value_wave(const KD &kd, cplx s, cplx nu, cp
     const double r2s = zero ? 1.0 : r2;
     cplx v;
     if (kd.kernel == QBX_LAPLACE2D) {
-        v = !kd.dip ? cplx{-0.5 * log(r2s) * INV2PI, 0.0}
-                    : cplx{(nu * cinv(ws)).x * INV2PI, 0.0};
+        if (!DER) {
+            v = !kd.dip ? cplx{-0.5 * log(r2s) * INV2PI, 0.0}
+                        : cplx{(nu * cinv(ws)).x * INV2PI, 0.0};
+        } else {
+            const cplx iw = cinv(ws);
+            v = !kd.dip ? cplx{-(dir * iw).x * INV2PI, 0.0}
+                        : cplx{-((nu * dir) * (iw * iw)).x * INV2PI, 0.0};
+        }
     } else {
         const double ir = 1.0 / sqrt(r2s);
         const double r = r2s * ir;
         const double kr = kd.k * r;
         const double ikr = ir / kd.k;
         cplx h0, h1;
-        if (!kd.dip) {
-            hankel01_wave<true, false>(kr, ikr, tab, h0, h1);
-            v = {-0.25 * h0.y, 0.25 * h0.x};
+        if (!DER) {
+            if (!kd.dip) {
+                hankel01_wave<true, false>(kr, ikr, tab, h0, h1);
+                v = {-0.25 * h0.y, 0.25 * h0.x};
+            } else {
+                hankel01_wave<false, true>(kr, ikr, tab, h0, h1);
+                const double proj = (ws.x * nu.x + ws.y * nu.y) * ir;
+                const cplx hv = (kd.k * proj) * h1;
+                v = {-0.25 * hv.y, 0.25 * hv.x};
+            }
         } else {
-            hankel01_wave<false, true>(kr, ikr, tab, h0, h1);
-            const double proj = (ws.x * nu.x + ws.y * nu.y) * ir;
-            const cplx hv = (kd.k * proj) * h1;
-            v = {-0.25 * hv.y, 0.25 * hv.x};
+            const double pd = (ws.x * dir.x + ws.y * dir.y) * ir;
+            if (!kd.dip) {
+                hankel01_wave<false, true>(kr, ikr, tab, h0, h1);
+                const cplx hv = (-kd.k * pd) * h1;
+                v = {-0.25 * hv.y, 0.25 * hv.x};
+            } else {
+                hankel01_wave<true, true>(kr, ikr, tab, h0, h1);
+                const double pn = (ws.x * nu.x + ws.y * nu.y) * ir;
+                const double dn = dir.x * nu.x + dir.y * nu.y;
+                const cplx hv = kd.k * ((ir * (dn - pd * pn)) * h1 +
+                                        (kd.k * pn * pd) * (h0 - ikr * h1));
+                v = {-0.25 * hv.y, 0.25 * hv.x};
+            }
         }
     }
     return zero ? cplx{0.0, 0.0} : v;
+}
+
+// the target direction of tree target ti (zero when no derivative is taken)
+__device__ __forceinline__ cplx ldir(const KD &kd, int ti)
+{
+    return kd.tder ? cplx{kd.tdir[ti], kd.tdir[(size_t)kd.ntgt + ti]} : cplx{0.0, 0.0};
 }
 
 // -------------------------------------------------------------------------
@@ -1331,9 +1371,26 @@ __global__ void k_l2l_h(KD kd, TD t, int lv, int i0, const int *ttp, double *loc
 // (coefficient n, right-hand side r at a[2 (n nr + r)]); out[q] receives the
 // potential of right-hand side r0 + q
 // -------------------------------------------------------------------------
-template <int NRC>
+// DER: the derivative along the complex direction d instead.  Laplace: the
+// potential is -Re F / (2 pi) with F analytic, so the derivative is
+// -Re(F' d) / (2 pi).  Helmholtz: with D+- = d/dx +- i d/dy,
+// D+ [Z_n e^{in th}] = -k Z_{n+1} e^{i(n+1) th} and
+// D- [Z_n e^{in th}] = k Z_{n-1} e^{i(n-1) th}, so the derivative of
+// sum_n a_n Z_n e^{in th} along d is sum_m b_m Z_m e^{im th} with
+// b_m = (k/2) (d a_{m+1} - conj(d) a_{m-1}), |m| <= p + 1.
+template <int NRC, bool DER = false>
+__device__ __forceinline__ cplx dcoef(const KD &kd, const double *a, int p, int m, int r,
+                                      cplx dir)
+{
+    if (!DER) return ldcn(kd, a, m, r);
+    const cplx up = (m + 1 >= -p && m + 1 <= p) ? ldcn(kd, a, m + 1, r) : cplx{0, 0};
+    const cplx dn = (m - 1 >= -p && m - 1 <= p) ? ldcn(kd, a, m - 1, r) : cplx{0, 0};
+    return (0.5 * kd.k) * (dir * up - conj(dir) * dn);
+}
+
+template <int NRC, bool DER = false>
 __device__ void eval_local_n(const KD &kd, const double *a, int p, cplx c, double R, cplx tp,
-                             cplx *out)
+                             cplx *out, cplx dir = {0.0, 0.0})
 {
     const int r0 = kd.r0, rc = kd.rc;
     if (kd.kernel == QBX_LAPLACE2D) {
@@ -1341,14 +1398,26 @@ __device__ void eval_local_n(const KD &kd, const double *a, int p, cplx c, doubl
         cplx phi[NRC];
 #pragma unroll
         for (int q = 0; q < NRC; ++q) phi[q] = {0, 0};
-        for (int l = p; l >= 0; --l)
+        if (!DER) {
+            for (int l = p; l >= 0; --l)
 #pragma unroll
-            for (int q = 0; q < NRC; ++q)
-                if (q < rc) phi[q] = phi[q] * x + ldcn(kd, a, l, r0 + q);
+                for (int q = 0; q < NRC; ++q)
+                    if (q < rc) phi[q] = phi[q] * x + ldcn(kd, a, l, r0 + q);
 #pragma unroll
-        for (int q = 0; q < NRC; ++q) out[q] = {-phi[q].x * INV2PI, 0.0};
+            for (int q = 0; q < NRC; ++q) out[q] = {-phi[q].x * INV2PI, 0.0};
+        } else {
+            // F' = sum_l l a_l x^(l-1) / R
+            for (int l = p; l >= 1; --l)
+#pragma unroll
+                for (int q = 0; q < NRC; ++q)
+                    if (q < rc) phi[q] = phi[q] * x + (double)l * ldcn(kd, a, l, r0 + q);
+            const cplx dr = (1.0 / R) * dir;
+#pragma unroll
+            for (int q = 0; q < NRC; ++q) out[q] = {-(phi[q] * dr).x * INV2PI, 0.0};
+        }
         return;
     }
+    const int pt = DER ? p + 1 : p;
     const cplx d = tp - c;
     const double rho = sqrt(abs2(d));
     const double x = kd.k * rho;
@@ -1359,13 +1428,13 @@ __device__ void eval_local_n(const KD &kd, const double *a, int p, cplx c, doubl
     if (x < 1e-300) {
 #pragma unroll
         for (int q = 0; q < NRC; ++q)
-            if (q < rc) acc[q] = ldcn(kd, a, 0, r0 + q);
+            if (q < rc) acc[q] = dcoef<NRC, DER>(kd, a, p, 0, r0 + q, dir);
     } else {
         const cplx e1 = {d.x / rho, d.y / rho};
         const cplx em1 = conj(e1);
-        cplx e = cpow_int(e1, p);
+        cplx e = cpow_int(e1, pt);
         norm = miller_j(
-            x, p,
+            x, pt,
             [&](int n, double f) {
                 const cplx fe = f * e;
                 cplx fc = f * conj(e);
@@ -1373,8 +1442,8 @@ __device__ void eval_local_n(const KD &kd, const double *a, int p, cplx c, doubl
 #pragma unroll
                 for (int q = 0; q < NRC; ++q)
                     if (q < rc) {
-                        acc[q] += ldcn(kd, a, n, r0 + q) * fe;
-                        if (n > 0) acc[q] += ldcn(kd, a, -n, r0 + q) * fc;
+                        acc[q] += dcoef<NRC, DER>(kd, a, p, n, r0 + q, dir) * fe;
+                        if (n > 0) acc[q] += dcoef<NRC, DER>(kd, a, p, -n, r0 + q, dir) * fc;
                     }
                 e = e * em1;
             },
@@ -1390,30 +1459,47 @@ __device__ void eval_local_n(const KD &kd, const double *a, int p, cplx c, doubl
     }
 }
 
-template <int NRC>
+// DER: the derivative along d (see eval_local_n): Laplace
+// F' = (a_0 - sum_k k a_k (R/w)^k) / w; Helmholtz the b_m of the H_m series.
+template <int NRC, bool DER = false>
 __device__ void eval_mpole_add(const KD &kd, const double *a, int p, cplx c, double R, cplx tp,
-                               cplx *out)
+                               cplx *out, cplx dir = {0.0, 0.0})
 {
     const int r0 = kd.r0, rc = kd.rc;
     const cplx w = tp - c;
     if (kd.kernel == QBX_LAPLACE2D) {
         const cplx x = R * cinv(w);
-        const cplx lw = clogz(w);
         cplx phi[NRC];
 #pragma unroll
         for (int q = 0; q < NRC; ++q) phi[q] = {0, 0};
-        for (int k = p; k >= 1; --k)
+        if (!DER) {
+            const cplx lw = clogz(w);
+            for (int k = p; k >= 1; --k)
+#pragma unroll
+                for (int q = 0; q < NRC; ++q)
+                    if (q < rc) phi[q] = (phi[q] + ldcn(kd, a, k, r0 + q)) * x;
 #pragma unroll
             for (int q = 0; q < NRC; ++q)
-                if (q < rc) phi[q] = (phi[q] + ldcn(kd, a, k, r0 + q)) * x;
+                if (q < rc) {
+                    const cplx v = phi[q] + ldcn(kd, a, 0, r0 + q) * lw;
+                    out[q] += cplx{-v.x * INV2PI, 0.0};
+                }
+        } else {
+            for (int k = p; k >= 1; --k)
 #pragma unroll
-        for (int q = 0; q < NRC; ++q)
-            if (q < rc) {
-                const cplx v = phi[q] + ldcn(kd, a, 0, r0 + q) * lw;
-                out[q] += cplx{-v.x * INV2PI, 0.0};
-            }
+                for (int q = 0; q < NRC; ++q)
+                    if (q < rc) phi[q] = (phi[q] + (double)k * ldcn(kd, a, k, r0 + q)) * x;
+            const cplx dw = dir * cinv(w);
+#pragma unroll
+            for (int q = 0; q < NRC; ++q)
+                if (q < rc) {
+                    const cplx v = (ldcn(kd, a, 0, r0 + q) - phi[q]) * dw;
+                    out[q] += cplx{-v.x * INV2PI, 0.0};
+                }
+        }
         return;
     }
+    const int pt = DER ? p + 1 : p;
     const double rho = sqrt(abs2(w));
     const double x = kd.k * rho;
     const cplx e1 = {w.x / rho, w.y / rho};
@@ -1423,8 +1509,9 @@ __device__ void eval_mpole_add(const KD &kd, const double *a, int p, cplx c, dou
     const double tox = 2.0 / x;
     cplx acc[NRC];
 #pragma unroll
-    for (int q = 0; q < NRC; ++q) acc[q] = q < rc ? ldcn(kd, a, 0, r0 + q) * hm1 : cplx{0, 0};
-    for (int n = 1; n <= p; ++n) {
+    for (int q = 0; q < NRC; ++q)
+        acc[q] = q < rc ? dcoef<NRC, DER>(kd, a, p, 0, r0 + q, dir) * hm1 : cplx{0, 0};
+    for (int n = 1; n <= pt; ++n) {
         cplx h;
         if (n == 1) {
             h = hm;
@@ -1439,7 +1526,9 @@ __device__ void eval_mpole_add(const KD &kd, const double *a, int p, cplx c, dou
         if (n & 1) hc = -hc;
 #pragma unroll
         for (int q = 0; q < NRC; ++q)
-            if (q < rc) acc[q] += ldcn(kd, a, n, r0 + q) * he + ldcn(kd, a, -n, r0 + q) * hc;
+            if (q < rc)
+                acc[q] += dcoef<NRC, DER>(kd, a, p, n, r0 + q, dir) * he +
+                          dcoef<NRC, DER>(kd, a, p, -n, r0 + q, dir) * hc;
     }
 #pragma unroll
     for (int q = 0; q < NRC; ++q) out[q] += cplx{-0.25 * acc[q].y, 0.25 * acc[q].x};
@@ -1455,7 +1544,7 @@ struct P2PTile {
     static constexpr int CH = NRC == 1 ? 128 : 64;
 };
 
-template <int NRC>
+template <int NRC, bool DER = false>
 __global__ void __launch_bounds__(64) k_p2p(KD kd, TD t, const int *ubox, const int *ut0,
                                             const int *box_g, const int *tboxes,
                                             const int *starts, const int *lists,
@@ -1479,10 +1568,11 @@ __global__ void __launch_bounds__(64) k_p2p(KD kd, TD t, const int *ubox, const 
     const bool dip = kd.dip;
     const int rc = kd.rc, r0 = kd.r0, nr = kd.nr;
     int ti = 0;
-    cplx tp = {0, 0};
+    cplx tp = {0, 0}, dir = {0, 0};
     if (active) {
         ti = unfilt[fstart[B] + tl];
         tp = ldpt(tgt, nt, ti);
+        if (DER) dir = ldir(kd, ti);
     }
     hankel_table_fill(htab, lane, 64);
     cplx acc[NRC];
@@ -1499,7 +1589,8 @@ __global__ void __launch_bounds__(64) k_p2p(KD kd, TD t, const int *ubox, const 
                 int toff = 0;
                 asm volatile("" : "+s"(toff));
                 const cplx nu = dip ? cplx{snx[j], sny[j]} : cplx{0, 0};
-                const cplx g = p2p_value_wave(kd, cplx{sx[j], sy[j]}, nu, tp, htab + toff);
+                const cplx g = p2p_value_wave<DER>(kd, cplx{sx[j], sy[j]}, nu, tp, htab + toff,
+                                                   dir);
 #pragma unroll
                 for (int q = 0; q < NRC; ++q)
                     if (q < rc) acc[q] += cplx{sq[2 * (j * NRC + q)], sq[2 * (j * NRC + q) + 1]} * g;
@@ -1552,7 +1643,7 @@ __global__ void __launch_bounds__(64) k_p2p(KD kd, TD t, const int *ubox, const 
 
 // M2P over balanced work units (see unit_plan): the unit's TG targets take
 // G lanes apiece that split the box's list-3 boxes; shuffle reduction.
-template <int NRC>
+template <int NRC, bool DER = false>
 __global__ void __launch_bounds__(64) k_m2p(KD kd, TD t, const int *ubox, const int *ut0,
                                             const int *box_g, const int *tboxes,
                                             const int *starts, const int *lists,
@@ -1575,11 +1666,12 @@ __global__ void __launch_bounds__(64) k_m2p(KD kd, TD t, const int *ubox, const 
     if (active) {
         ti = unfilt[fstart[B] + tl];
         const cplx tp = ldpt(tgt, nt, ti);
+        const cplx dir = DER ? ldir(kd, ti) : cplx{0, 0};
         for (int e = starts[it] + sub; e < starts[it + 1]; e += G) {
             const int D = lists[e];
             const int lv = t.level[D];
-            eval_mpole_add<NRC>(kd, mpole + ecoef_off(kd, D, lv), kd.order[lv], boxc(t, D),
-                                kd.rscale[lv], tp, acc);
+            eval_mpole_add<NRC, DER>(kd, mpole + ecoef_off(kd, D, lv), kd.order[lv], boxc(t, D),
+                                     kd.rscale[lv], tp, acc, dir);
         }
     }
 #pragma unroll
@@ -1592,7 +1684,7 @@ __global__ void __launch_bounds__(64) k_m2p(KD kd, TD t, const int *ubox, const 
 }
 
 // L2P: one 64-lane block per target box, one lane per non-QBX target
-template <int MODE, int NRC>
+template <int MODE, int NRC, bool DER = false>
 __global__ void __launch_bounds__(64) k_eval_targets(KD kd, TD t, int ntb, const int *tboxes,
                                                      const int *starts, const int *lists,
                                                      const int *fstart, const int *fcount,
@@ -1608,8 +1700,9 @@ __global__ void __launch_bounds__(64) k_eval_targets(KD kd, TD t, int ntb, const
     for (int f = f0 + threadIdx.x; f < f0 + nf; f += 64) {
         const int ti = unfilt[f];
         cplx v[NRC];
-        eval_local_n<NRC>(kd, expn + ecoef_off(kd, B, lv), kd.order[lv], boxc(t, B),
-                          kd.rscale[lv], ldpt(tgt, nt, ti), v);
+        eval_local_n<NRC, DER>(kd, expn + ecoef_off(kd, B, lv), kd.order[lv], boxc(t, B),
+                               kd.rscale[lv], ldpt(tgt, nt, ti), v,
+                               DER ? ldir(kd, ti) : cplx{0, 0});
         store_add<NRC>(kd, pot + 2 * (size_t)ti * kd.nr, v);
     }
 }
@@ -1919,7 +2012,7 @@ __global__ void __launch_bounds__(64) k_direct_p2qbxl(KD kd, int ng, const int *
     if (active && sub == 0) qbx_store<PQ, NRC>(kd, qexp, c, A, 0);
 }
 
-template <int NRC>
+template <int NRC, bool DER = false>
 __global__ void __launch_bounds__(64) k_direct_p2p(KD kd, int nf, const int *unfilt,
                                                    const double *tgt, int nt, const double *src,
                                                    int ns, const double *str, const double *nrm,
@@ -1934,10 +2027,11 @@ __global__ void __launch_bounds__(64) k_direct_p2p(KD kd, int nf, const int *unf
     const bool dip = kd.dip;
     const int rc = kd.rc, r0 = kd.r0, nr = kd.nr;
     int ti = 0;
-    cplx tp = {0, 0};
+    cplx tp = {0, 0}, dir = {0, 0};
     if (active) {
         ti = unfilt ? unfilt[f] : f;
         tp = ldpt(tgt, nt, ti);
+        if (DER) dir = ldir(kd, ti);
     }
     hankel_table_fill(htab, lane, 64);
     cplx acc[NRC];
@@ -1968,7 +2062,8 @@ __global__ void __launch_bounds__(64) k_direct_p2p(KD kd, int nf, const int *unf
                 int toff = 0;
                 asm volatile("" : "+s"(toff));
                 const cplx nu = dip ? cplx{snx[j], sny[j]} : cplx{0, 0};
-                const cplx g = p2p_value_wave(kd, cplx{sx[j], sy[j]}, nu, tp, htab + toff);
+                const cplx g = p2p_value_wave<DER>(kd, cplx{sx[j], sy[j]}, nu, tp, htab + toff,
+                                                   dir);
 #pragma unroll
                 for (int q = 0; q < NRC; ++q)
                     if (q < rc) acc[q] += cplx{sq[2 * (j * NRC + q)], sq[2 * (j * NRC + q) + 1]} * g;
@@ -2739,7 +2834,7 @@ k_l2qbxl_split(KD kd, TD t, int ng, const int *gcent, const int *c2tb, const int
 constexpr int L2QS_LANES = 4;
 
 // QBXL2P: one thread per (center, target) entry of center_to_tree_targets
-template <int NRC>
+template <int NRC, bool DER = false>
 __global__ void k_qbxl2p(KD kd, int nctt, const int *ctt_lists, const int *tree_ttc,
                          const double *tgt, int nt, const double *ctr, const double *rad, int nc,
                          const double *qexp, double *pot)
@@ -2753,7 +2848,7 @@ __global__ void k_qbxl2p(KD kd, int nctt, const int *ctt_lists, const int *tree_
     const double *a = qexp + 2 * ((size_t)c * kd.ncq +
                                   (kd.kernel == QBX_HELMHOLTZ2D ? kd.pq : 0)) * kd.nr;
     cplx v[NRC];
-    eval_local_n<NRC>(kd, a, kd.pq, cc, rad[c], tp, v);
+    eval_local_n<NRC, DER>(kd, a, kd.pq, cc, rad[c], tp, v, DER ? ldir(kd, ti) : cplx{0, 0});
 #pragma unroll
     for (int q = 0; q < NRC; ++q)
         if (q < kd.rc) stc(pot + 2 * ((size_t)ti * kd.nr + kd.r0 + q), v[q]);
@@ -2833,7 +2928,12 @@ static int make_kd(const qbx_kernel_desc *d, const qbx_tree *T, KD *kd, int nsep
                   d->nlevels, T->nlevels);
     }
     if (d->kernel == QBX_HELMHOLTZ2D) QBX_CHECK(d->k > 0, "Helmholtz needs k > 0");
+    QBX_CHECK(d->target_deriv == 0 || d->target_deriv == 1, "target_deriv must be 0 or 1");
+    QBX_CHECK(!d->target_deriv || d->target_dir, "target_deriv needs target_dir");
     memset(kd, 0, sizeof(*kd));
+    kd->tder = d->target_deriv;
+    kd->tdir = d->target_dir;
+    kd->ntgt = T ? T->ntargets : 0;
     kd->kernel = d->kernel;
     kd->dip = d->source_kind == QBX_SOURCE_D;
     kd->pq = d->qbx_order;
@@ -3108,6 +3208,7 @@ int lpot_matrix_2d(hipStream_t st, const qbx_kernel_desc *d, int nel, int n1, in
     KD kd;
     QBX_TRY(make_kd(d, nullptr, &kd));
     QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
+    QBX_CHECK(!kd.tder, "qbx_lpot_matrix: target derivatives are not supported");
     const dim3 grid(nt), blk(LM_THREADS);
 #define LM_ARGS kd, nel, n1, nf, U, src, nrm, w, nt, tgt, ctr, rad, ld, mat
     if (!ctr) {
@@ -3283,13 +3384,20 @@ static int eval_targets(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx
         const unit_plan &U = G->m2p_plan;
         QBX_CHECK(G->trav == R, "qbx_eval_multipoles: geometry was built for another traversal");
         if (U.nunits == 0) return QBX_OK;
-#define M2P_LAUNCH(N)                                                                      \
-    hipLaunchKernelGGL((k_m2p<N>), dim3(U.nunits), dim3(64), 0, st, kd, td, U.unit_box.p,  \
-                       U.unit_t0.p, U.box_g.p, R->target_boxes.p, starts, lists,          \
-                       G->nqbtl_box_target_starts.p, G->nqbtl_box_target_counts_nonchild.p, \
-                       G->nqbtl_unfiltered.p, T->targets.p, T->ntargets, expn, pot)
+#define M2P_LAUNCH_D(N, D)                                                                 \
+    hipLaunchKernelGGL((k_m2p<N, D>), dim3(U.nunits), dim3(64), 0, st, kd, td,            \
+                       U.unit_box.p, U.unit_t0.p, U.box_g.p, R->target_boxes.p, starts,   \
+                       lists, G->nqbtl_box_target_starts.p,                              \
+                       G->nqbtl_box_target_counts_nonchild.p, G->nqbtl_unfiltered.p,      \
+                       T->targets.p, T->ntargets, expn, pot)
+#define M2P_LAUNCH(N)              \
+    if (kd.tder)                   \
+        M2P_LAUNCH_D(N, true);     \
+    else                           \
+        M2P_LAUNCH_D(N, false)
         FOR_RHS_CHUNKS(kd, NRC_S, M2P_LAUNCH);
 #undef M2P_LAUNCH
+#undef M2P_LAUNCH_D
         QBX_HIP(hipGetLastError());
         return QBX_OK;
     }
@@ -3297,23 +3405,33 @@ static int eval_targets(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx
         const unit_plan &U = G->p2p_plan;
         QBX_CHECK(G->trav == R, "qbx_eval_direct: geometry was built for another traversal");
         if (U.nunits == 0) return QBX_OK;
-#define P2P_LAUNCH(N)                                                                      \
-    hipLaunchKernelGGL((k_p2p<N>), dim3(U.nunits), dim3(64), 0, st, kd, td, U.unit_box.p,  \
-                       U.unit_t0.p, U.box_g.p, R->target_boxes.p, starts, lists,          \
-                       G->nqbtl_box_target_starts.p, G->nqbtl_box_target_counts_nonchild.p, \
-                       G->nqbtl_unfiltered.p, T->targets.p, T->ntargets, T->sources.p,    \
-                       T->nsources, str, nrm, pot)
+#define P2P_LAUNCH_D(N, D)                                                                 \
+    hipLaunchKernelGGL((k_p2p<N, D>), dim3(U.nunits), dim3(64), 0, st, kd, td,            \
+                       U.unit_box.p, U.unit_t0.p, U.box_g.p, R->target_boxes.p, starts,   \
+                       lists, G->nqbtl_box_target_starts.p,                              \
+                       G->nqbtl_box_target_counts_nonchild.p, G->nqbtl_unfiltered.p,      \
+                       T->targets.p, T->ntargets, T->sources.p, T->nsources, str, nrm, pot)
+#define P2P_LAUNCH(N)              \
+    if (kd.tder)                   \
+        P2P_LAUNCH_D(N, true);     \
+    else                           \
+        P2P_LAUNCH_D(N, false)
         FOR_RHS_CHUNKS(kd, NRC_S, P2P_LAUNCH);
 #undef P2P_LAUNCH
+#undef P2P_LAUNCH_D
         QBX_HIP(hipGetLastError());
         return QBX_OK;
     }
-#define EVAL_LAUNCH(M, N)                                                                  \
-    hipLaunchKernelGGL((k_eval_targets<M, N>), dim3(ntb), dim3(64), 0, st, kd, td, ntb,    \
+#define EVAL_LAUNCH(M, N, D)                                                               \
+    hipLaunchKernelGGL((k_eval_targets<M, N, D>), dim3(ntb), dim3(64), 0, st, kd, td, ntb, \
                        R->target_boxes.p, starts, lists, G->nqbtl_box_target_starts.p,    \
                        G->nqbtl_box_target_counts_nonchild.p, G->nqbtl_unfiltered.p,      \
                        T->targets.p, T->ntargets, expn, pot)
-#define EV2(N) EVAL_LAUNCH(2, N)
+#define EV2(N)                     \
+    if (kd.tder)                   \
+        EVAL_LAUNCH(2, N, true);   \
+    else                           \
+        EVAL_LAUNCH(2, N, false)
     FOR_RHS_CHUNKS(kd, NRC_S, EV2);
 #undef EV2
 #undef EVAL_LAUNCH
@@ -3730,12 +3848,18 @@ int qbx_eval_qbx_expansions(qbx_ctx *ctx, void *stream, const qbx_tree *T, const
     QBX_TRY(make_kd(d, T, &kd));
     hipStream_t st = S(stream);
     if (G->nctt > 0) {
-#define L_Q2P(N)                                                                           \
-    hipLaunchKernelGGL(k_qbxl2p<N>, dim3(cdiv(G->nctt, 64)), dim3(64), 0, st, kd, G->nctt, \
-                       G->ctt_lists.p, G->tree_ttc.p, T->targets.p, T->ntargets,           \
+#define L_Q2P_D(N, D)                                                                      \
+    hipLaunchKernelGGL((k_qbxl2p<N, D>), dim3(cdiv(G->nctt, 64)), dim3(64), 0, st, kd,     \
+                       G->nctt, G->ctt_lists.p, G->tree_ttc.p, T->targets.p, T->ntargets,  \
                        G->centers.p, G->radii.p, G->ncenters, qexp, pot)
+#define L_Q2P(N)                   \
+    if (kd.tder)                   \
+        L_Q2P_D(N, true);          \
+    else                           \
+        L_Q2P_D(N, false)
         FOR_RHS_CHUNKS(kd, NRC_S, L_Q2P);
 #undef L_Q2P
+#undef L_Q2P_D
     }
     QBX_HIP(hipGetLastError());
     return QBX_OK;
@@ -3747,6 +3871,7 @@ int qbx_direct_qbx(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_geo 
     QBX_CHECK(ctx && T && G && d && str && pot, "qbx_direct_qbx: null argument");
     if (d->kernel == QBX_LAPLACE3D || d->kernel == QBX_HELMHOLTZ3D) {
         QBX_CHECK(d->nrhs <= 1, "qbx_direct_qbx: 3D evaluates one density at a time");
+        QBX_CHECK(!d->target_deriv, "qbx_direct_qbx: 3D target derivatives are not supported");
         return direct_qbx3d(ctx, S(stream), T, G, d->kernel, d->source_kind, d->k, d->qbx_order,
                             str, nrm, pot);
     }
@@ -3757,12 +3882,18 @@ int qbx_direct_qbx(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_geo 
     const int nr = kd.nr;
     QBX_TRY(zero_fill(pot, 2L * T->ntargets * nr, st));
     if (G->nfiltered > 0) {
-#define L_DP2P(N)                                                                          \
-    hipLaunchKernelGGL(k_direct_p2p<N>, dim3(cdiv(G->nfiltered, 64)), dim3(64), 0, st, kd, \
-                       G->nfiltered, G->nqbtl_unfiltered.p, T->targets.p, T->ntargets,     \
+#define L_DP2P_D(N, D)                                                                     \
+    hipLaunchKernelGGL((k_direct_p2p<N, D>), dim3(cdiv(G->nfiltered, 64)), dim3(64), 0, st, \
+                       kd, G->nfiltered, G->nqbtl_unfiltered.p, T->targets.p, T->ntargets, \
                        T->sources.p, T->nsources, str, nrm, pot)
+#define L_DP2P(N)                  \
+    if (kd.tder)                   \
+        L_DP2P_D(N, true);         \
+    else                           \
+        L_DP2P_D(N, false)
         FOR_RHS_CHUNKS(kd, NRC_S, L_DP2P);
 #undef L_DP2P
+#undef L_DP2P_D
     }
     if (G->nglobal > 0) {
         const long nq = 2L * std::max(G->ncenters, 1) * kd.ncq * nr;
@@ -3804,6 +3935,7 @@ int qbx_p2p_all(qbx_ctx *ctx, void *stream, const qbx_kernel_desc *d, int nsourc
     KD kd;
     QBX_TRY(make_kd(d, nullptr, &kd));
     QBX_CHECK(!kd.dip || nsources == 0 || normals, "double layer needs normals");
+    QBX_CHECK(!kd.tder, "qbx_p2p_all: target derivatives are not supported");
     if (ntargets == 0) return QBX_OK;
     // every target is written (zero when there are no sources)
 #define L_AP2P(N)                                                                          \
@@ -3823,6 +3955,7 @@ int qbx_lpot_matrix(qbx_ctx *ctx, void *stream, const qbx_kernel_desc *d, int ne
                     double *mat)
 {
     QBX_CHECK(ctx && d && mat, "qbx_lpot_matrix: null argument");
+    QBX_CHECK(!d->target_deriv, "qbx_lpot_matrix: target derivatives are not supported");
     QBX_CHECK(nelements >= 0 && ntargets >= 0, "qbx_lpot_matrix: negative size");
     QBX_CHECK(n1 >= 1 && nf >= 1 && nf <= LM_THREADS && (long)nf * n1 <= LM_UMAX,
               "qbx_lpot_matrix: element sizes n1=%d nf=%d outside the supported range", n1, nf);

@@ -1599,6 +1599,8 @@ static void host_regular(int p, const double *x, zc *Y)
 static int make_kd(const qbx_kernel_desc *d, const qbx_tree *T, KD *kd, int nsep = 2)
 {
     QBX_CHECK(d, "null kernel description");
+    QBX_CHECK(!d->target_deriv, "3D stages: target derivatives are not supported (the "
+                                "target-specific stage gives 3D gradients)");
     QBX_CHECK(d->kernel == QBX_LAPLACE3D, "3D stages support QBX_LAPLACE3D only (got %d)",
               d->kernel);
     QBX_CHECK(d->source_kind == QBX_SOURCE_S || d->source_kind == QBX_SOURCE_D,

@@ -25,7 +25,12 @@ class QBXExpansionWrangler:
     """Stages of the QBX FMM on one tree (pytential/qbx/fmm.py:118-396)."""
 
     def __init__(self, geo_data, kernel, k, source_kind, qbx_order, level_orders,
-                 output="pot", use_target_specific_qbx=False, nrhs=1, ctx=None):
+                 output="pot", use_target_specific_qbx=False, nrhs=1, ctx=None,
+                 target_dirs=None):
+        """``target_dirs`` ([dim, n_user_targets], device): evaluate the
+        derivative along these directions at the user targets instead of the
+        potential (sumpy's DirectionalTargetDerivative; Sp and Dp take the
+        target normals), 2D only; the expansions are unchanged."""
         self.geo_data = geo_data
         self.tree = geo_data.tree()
         self.traversal = geo_data.traversal()
@@ -56,6 +61,22 @@ class QBXExpansionWrangler:
                              _lib.SOURCE_S if source_kind == "S" else _lib.SOURCE_D,
                              qbx_order, len(lo), lo.ctypes.data_as(ct.POINTER(ct.c_int)),
                              self.nrhs)
+        self.target_dirs_tree = None
+        if target_dirs is not None:
+            if self.dim != 2:
+                raise NotImplementedError("target derivatives through the expansion stages "
+                                          "are 2D only")
+            nc = geo_data.ncenters
+            td = torch.as_tensor(target_dirs, dtype=torch.float64, device="cuda")
+            if td.shape != (2, self.tree.ntargets - nc):
+                raise ValueError(f"target_dirs must be (2, {self.tree.ntargets - nc}), "
+                                 f"got {tuple(td.shape)}")
+            allv = torch.cat([torch.zeros((2, nc), dtype=torch.float64, device="cuda"), td],
+                             dim=1)
+            order = self.tree.sorted_target_ids.long()
+            self.target_dirs_tree = allv[:, order].contiguous()
+            self.kd.target_deriv = 1
+            self.kd.target_dir = self.target_dirs_tree.data_ptr()
         self.source_kind = source_kind
         self.qbx_order = qbx_order
         # TSQBX replaces P2QBXL's list-1 part (qbx/fmmlib.py:73-108, 295-298)

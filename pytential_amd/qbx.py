@@ -206,24 +206,32 @@ class QBXLayerPotentialSource:
              _p(self.dev["upsample"]), _p(dens.contiguous()), _p(out))
         return out
 
-    def make_wrangler(self, geo_data, kernel, k, source_kind, nrhs=1, private_context=False):
+    def make_wrangler(self, geo_data, kernel, k, source_kind, nrhs=1, private_context=False,
+                      target_dirs=None):
         """The wrangler of one evaluation.  ``private_context`` gives it a
         library context of its own (its own workspaces and translation
-        tables), as an operator captured into a HIP graph needs."""
+        tables), as an operator captured into a HIP graph needs;
+        ``target_dirs`` makes it evaluate target derivatives."""
         tree = geo_data.tree()
         ctx = Context(self.ctx.device) if private_context else None
         return _fmm.QBXExpansionWrangler(
             geo_data, kernel, k, source_kind, self.qbx_order, self.level_orders(kernel, k, tree),
-            use_target_specific_qbx=self._use_target_specific_qbx, nrhs=nrhs, ctx=ctx)
+            use_target_specific_qbx=self._use_target_specific_qbx, nrhs=nrhs, ctx=ctx,
+            target_dirs=target_dirs)
 
     def exec_compute_potential(self, kernel, density, targets_and_sides, k=0.0,
-                               source_kind="S", fused=True):
+                               source_kind="S", fused=True, target_derivative=False,
+                               target_dirs=None):
         """Evaluate ``S`` or ``D`` of ``density`` (stage-1 nodal values,
         ``[ndofs]`` or ``[ndofs, nrhs]`` for several densities in one
         evaluation) at each ``(targets, qbx_forced_limit)`` pair; ``targets``
         is ``"self"`` (the stage-1 nodes) or a (dim, n) array.  Returns one
         tensor per pair ([n] or [n, nrhs]).  Target-association failure raises
-        QBXTargetAssociationFailedError like the reference."""
+        QBXTargetAssociationFailedError like the reference.
+
+        ``target_derivative``: the derivative along a direction at each
+        target instead (Sp from S, Dp from D): the node normals for "self",
+        ``target_dirs[i]`` ((dim, n) per pair) otherwise (2D)."""
         if kernel not in ("laplace", "helmholtz"):
             raise ValueError(f"unknown kernel {kernel!r}")
         if source_kind not in ("S", "D"):
@@ -243,7 +251,8 @@ class QBXLayerPotentialSource:
             # and imaginary parts as two right-hand sides (linearity)
             two = torch.stack([dens.real, dens.imag], dim=-1)
             outs = self.exec_compute_potential(kernel, two.reshape(dens.shape[0], -1),
-                                               targets_and_sides, k, source_kind, fused)
+                                               targets_and_sides, k, source_kind, fused,
+                                               target_derivative, target_dirs)
             nr = dens.shape[1] if dens.dim() == 2 else 1
             res = []
             for o in outs:
@@ -252,7 +261,18 @@ class QBXLayerPotentialSource:
                 res.append(c if dens.dim() == 2 else c[:, 0])
             return res
         nrhs = 1 if dens.dim() == 1 else dens.shape[1]
-        w = self.make_wrangler(geo, kernel, k, source_kind, nrhs)
+        tdirs = None
+        if target_derivative:
+            parts = []
+            for i, (t, _) in enumerate(sides):
+                if isinstance(t, str) and t == "self":
+                    parts.append(self.dev["normals"])
+                elif target_dirs is None or target_dirs[i] is None:
+                    raise ValueError("target derivative at points targets needs target_dirs")
+                else:
+                    parts.append(as_dev(target_dirs[i]))
+            tdirs = torch.cat(parts, dim=1)
+        w = self.make_wrangler(geo, kernel, k, source_kind, nrhs, target_dirs=tdirs)
         qd = self.quad_density(density)
         normals = self.dev["quad_normals"] if source_kind == "D" else None
         strengths, nrm = w.reorder_sources(qd, self.dev["quad_weights"], normals)
