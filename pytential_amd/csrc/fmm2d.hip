@@ -41,6 +41,7 @@
 
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 
 namespace qbx {
 
@@ -57,6 +58,7 @@ struct KD {
     int nr;                    // right-hand sides (stride of the innermost axis)
     int r0, rc;                // the chunk of right-hand sides of this launch
     double k;
+    double ik;                 // 1 / k (Helmholtz)
     int order[FML];
     int ncl[FML];
     int lbox[FML + 1];
@@ -182,49 +184,63 @@ __device__ __forceinline__ cplx cpow_int(cplx z, int m)
 //   Helmholtz D  (i/4) k [H1(k r) (d . nu - (d . w^)(nu . w^)) / r
 //                         + k (nu . w^)(d . w^) (H0(k r) - H1(k r) / (k r))]
 // (H1' = H0 - H1 / x).
-template <bool DER = false>
-__device__ __forceinline__ cplx p2p_value_wave(const KD &kd, cplx s, cplx nu, cplx t,
-                                            const double *tab, cplx dir = {0.0, 0.0})
+// KN, DP: the kernel and source kind fixed at compile time (-1: read from kd);
+// SC: Hankel series coefficients as scalar operands (off: an opaque
+// wave-uniform zero, see hankel01_wave_sc) instead of from the LDS table tab.
+template <int KN, int DP, bool DER, bool SC>
+__device__ __forceinline__ cplx p2p_value_impl(const KD &kd, cplx s, cplx nu, cplx t,
+                                               const double *tab, int off, cplx dir)
 {
+    const int kern = KN >= 0 ? KN : kd.kernel;
+    const bool dip = DP >= 0 ? DP != 0 : kd.dip != 0;
     const cplx w = t - s;
     const double r2 = abs2(w);
     const bool zero = r2 == 0.0;
     const cplx ws = zero ? cplx{1.0, 0.0} : w;
     const double r2s = zero ? 1.0 : r2;
     cplx v;
-    if (kd.kernel == QBX_LAPLACE2D) {
+    if (kern == QBX_LAPLACE2D) {
         if (!DER) {
-            v = !kd.dip ? cplx{-0.5 * log(r2s) * INV2PI, 0.0}
-                        : cplx{(nu * cinv(ws)).x * INV2PI, 0.0};
+            v = !dip ? cplx{-0.5 * qbx_log(r2s) * INV2PI, 0.0}
+                     : cplx{(nu * cinv(ws)).x * INV2PI, 0.0};
         } else {
             const cplx iw = cinv(ws);
-            v = !kd.dip ? cplx{-(dir * iw).x * INV2PI, 0.0}
-                        : cplx{-((nu * dir) * (iw * iw)).x * INV2PI, 0.0};
+            v = !dip ? cplx{-(dir * iw).x * INV2PI, 0.0}
+                     : cplx{-((nu * dir) * (iw * iw)).x * INV2PI, 0.0};
         }
     } else {
-        const double ir = 1.0 / sqrt(r2s);
+        const double ir = qbx_rsqrt(r2s);
         const double r = r2s * ir;
         const double kr = kd.k * r;
-        const double ikr = ir / kd.k;
+        const double ikr = ir * kd.ik;
         cplx h0, h1;
+        auto hank = [&](auto w0, auto w1) {
+            constexpr bool W0 = decltype(w0)::value, W1 = decltype(w1)::value;
+            if (SC)
+                hankel01_wave_sc<W0, W1>(kr, ikr, off, h0, h1);
+            else
+                hankel01_wave<W0, W1>(kr, ikr, tab, h0, h1);
+        };
+        using T_ = std::true_type;
+        using F_ = std::false_type;
         if (!DER) {
-            if (!kd.dip) {
-                hankel01_wave<true, false>(kr, ikr, tab, h0, h1);
+            if (!dip) {
+                hank(T_{}, F_{});
                 v = {-0.25 * h0.y, 0.25 * h0.x};
             } else {
-                hankel01_wave<false, true>(kr, ikr, tab, h0, h1);
+                hank(F_{}, T_{});
                 const double proj = (ws.x * nu.x + ws.y * nu.y) * ir;
                 const cplx hv = (kd.k * proj) * h1;
                 v = {-0.25 * hv.y, 0.25 * hv.x};
             }
         } else {
             const double pd = (ws.x * dir.x + ws.y * dir.y) * ir;
-            if (!kd.dip) {
-                hankel01_wave<false, true>(kr, ikr, tab, h0, h1);
+            if (!dip) {
+                hank(F_{}, T_{});
                 const cplx hv = (-kd.k * pd) * h1;
                 v = {-0.25 * hv.y, 0.25 * hv.x};
             } else {
-                hankel01_wave<true, true>(kr, ikr, tab, h0, h1);
+                hank(T_{}, T_{});
                 const double pn = (ws.x * nu.x + ws.y * nu.y) * ir;
                 const double dn = dir.x * nu.x + dir.y * nu.y;
                 const cplx hv = kd.k * ((ir * (dn - pd * pn)) * h1 +
@@ -234,6 +250,13 @@ __device__ __forceinline__ cplx p2p_value_wave(const KD &kd, cplx s, cplx nu, cp
         }
     }
     return zero ? cplx{0.0, 0.0} : v;
+}
+
+template <bool DER = false>
+__device__ __forceinline__ cplx p2p_value_wave(const KD &kd, cplx s, cplx nu, cplx t,
+                                            const double *tab, cplx dir = {0.0, 0.0})
+{
+    return p2p_value_impl<-1, -1, DER, false>(kd, s, nu, t, tab, 0, dir);
 }
 
 // the target direction of tree target ti (zero when no derivative is taken)
@@ -1500,13 +1523,14 @@ __device__ void eval_mpole_add(const KD &kd, const double *a, int p, cplx c, dou
         return;
     }
     const int pt = DER ? p + 1 : p;
-    const double rho = sqrt(abs2(w));
+    const double iw = qbx_rsqrt(abs2(w));
+    const double rho = abs2(w) * iw;
     const double x = kd.k * rho;
-    const cplx e1 = {w.x / rho, w.y / rho};
+    const cplx e1 = {w.x * iw, w.y * iw};
     cplx e = {1, 0};
     cplx hm1, hm;
     hankel01_call(x, hm1, hm);
-    const double tox = 2.0 / x;
+    const double tox = 2.0 * iw * kd.ik;
     cplx acc[NRC];
 #pragma unroll
     for (int q = 0; q < NRC; ++q)
@@ -1544,7 +1568,7 @@ struct P2PTile {
     static constexpr int CH = NRC == 1 ? 128 : 64;
 };
 
-template <int NRC, bool DER = false>
+template <int NRC, bool DER, int KN, int DP>
 __global__ void __launch_bounds__(64) k_p2p(KD kd, TD t, const int *ubox, const int *ut0,
                                             const int *box_g, const int *tboxes,
                                             const int *starts, const int *lists,
@@ -1565,7 +1589,7 @@ __global__ void __launch_bounds__(64) k_p2p(KD kd, TD t, const int *ubox, const 
     const int tsub = lane / G, sub = lane & (G - 1);
     const int tl = ut0[u] + tsub;
     const bool active = tl < fcount[B];
-    const bool dip = kd.dip;
+    const bool dip = DP >= 0 ? DP != 0 : kd.dip != 0;
     const int rc = kd.rc, r0 = kd.r0, nr = kd.nr;
     int ti = 0;
     cplx tp = {0, 0}, dir = {0, 0};
@@ -1589,8 +1613,8 @@ __global__ void __launch_bounds__(64) k_p2p(KD kd, TD t, const int *ubox, const 
                 int toff = 0;
                 asm volatile("" : "+s"(toff));
                 const cplx nu = dip ? cplx{snx[j], sny[j]} : cplx{0, 0};
-                const cplx g = p2p_value_wave<DER>(kd, cplx{sx[j], sy[j]}, nu, tp, htab + toff,
-                                                   dir);
+                const cplx g = p2p_value_impl<KN, DP, DER, false>(
+                    kd, cplx{sx[j], sy[j]}, nu, tp, htab + toff, 0, dir);
 #pragma unroll
                 for (int q = 0; q < NRC; ++q)
                     if (q < rc) acc[q] += cplx{sq[2 * (j * NRC + q)], sq[2 * (j * NRC + q) + 1]} * g;
@@ -1783,10 +1807,10 @@ __device__ __forceinline__ void p2qbxl_pair(const KD &kd, cplx w, cplx nu, const
         }
         return;
     }
-    const double ir = 1.0 / sqrt(abs2(w));
+    const double ir = qbx_rsqrt(abs2(w));
     const double r = abs2(w) * ir;
     const double x = kd.k * r;
-    const double ix = ir / kd.k;
+    const double ix = ir * kd.ik;
     const cplx em = {w.x * ir, -w.y * ir};  // e^{-i theta}
     cplx H[PQ + 2];
     hankel01_wave<true, true>(x, ix, tab, H[0], H[1]);
@@ -2938,6 +2962,7 @@ static int make_kd(const qbx_kernel_desc *d, const qbx_tree *T, KD *kd, int nsep
     kd->dip = d->source_kind == QBX_SOURCE_D;
     kd->pq = d->qbx_order;
     kd->k = d->k;
+    kd->ik = d->k > 0 ? 1.0 / d->k : 0.0;
     kd->nsep = nsep;
     kd->m2l_w = 2 * nsep + 1;
     kd->nr = d->nrhs > 0 ? d->nrhs : 1;
@@ -3405,12 +3430,26 @@ static int eval_targets(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx
         const unit_plan &U = G->p2p_plan;
         QBX_CHECK(G->trav == R, "qbx_eval_direct: geometry was built for another traversal");
         if (U.nunits == 0) return QBX_OK;
-#define P2P_LAUNCH_D(N, D)                                                                 \
-    hipLaunchKernelGGL((k_p2p<N, D>), dim3(U.nunits), dim3(64), 0, st, kd, td,            \
+#define P2P_LAUNCH_K(N, D, KN, DP)                                                         \
+    hipLaunchKernelGGL((k_p2p<N, D, KN, DP>), dim3(U.nunits), dim3(64), 0, st, kd, td,    \
                        U.unit_box.p, U.unit_t0.p, U.box_g.p, R->target_boxes.p, starts,   \
                        lists, G->nqbtl_box_target_starts.p,                              \
                        G->nqbtl_box_target_counts_nonchild.p, G->nqbtl_unfiltered.p,      \
                        T->targets.p, T->ntargets, T->sources.p, T->nsources, str, nrm, pot)
+#define P2P_LAUNCH_D(N, D)                                                                 \
+    do {                                                                                   \
+        if (kd.kernel == QBX_HELMHOLTZ2D) {                                                \
+            if (kd.dip)                                                                    \
+                P2P_LAUNCH_K(N, D, QBX_HELMHOLTZ2D, 1);                                    \
+            else                                                                           \
+                P2P_LAUNCH_K(N, D, QBX_HELMHOLTZ2D, 0);                                    \
+        } else {                                                                           \
+            if (kd.dip)                                                                    \
+                P2P_LAUNCH_K(N, D, QBX_LAPLACE2D, 1);                                      \
+            else                                                                           \
+                P2P_LAUNCH_K(N, D, QBX_LAPLACE2D, 0);                                      \
+        }                                                                                  \
+    } while (0)
 #define P2P_LAUNCH(N)              \
     if (kd.tder)                   \
         P2P_LAUNCH_D(N, true);     \
@@ -3419,6 +3458,7 @@ static int eval_targets(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx
         FOR_RHS_CHUNKS(kd, NRC_S, P2P_LAUNCH);
 #undef P2P_LAUNCH
 #undef P2P_LAUNCH_D
+#undef P2P_LAUNCH_K
         QBX_HIP(hipGetLastError());
         return QBX_OK;
     }

@@ -1472,6 +1472,149 @@ __global__ void __launch_bounds__(256) k3_m2qbxl(KD kd, TD t, int ng, const int 
     }
 }
 
+// M2QBXL for many right-hand sides on the FP64 matrix cores: a block per
+// group of GC_GROUP global centers of one target box (gc_group_plan), wave w
+// takes center w of the group and the 32 right-hand sides of blockIdx.y.
+// Per list-3 box D of the target box the wave builds T_N^M(c - b_D),
+// N <= p_D + PQ, in its own LDS slice (as k3_m2qbxl, one column m per lane),
+// then the complex product out[(k,l)][r] += sum_(n,m) T(n+k, m+l)
+// a_D[(n,m)][r] runs as four real v_mfma_f64_16x16x4f64 per K-step of four
+// (n,m) (lane i holds A[i%16][k0 + i/16] and B[k0 + i/16][i%16]; D register
+// j of lane i is D[4j + i/16][i%16]): rows (k,l) in 16-row tiles, columns
+// two tiles of 16 right-hand sides, multipoles read straight from HBM (the
+// group's waves read the same box).  The epilogue applies (-1)^k and adds
+// into qexp.  Waves only touch their own LDS slice: no block barrier.
+constexpr int M2QM_RB = 32;  // right-hand sides per block (blockIdx.y)
+
+template <int PQ>
+__global__ void __launch_bounds__(256) k3_m2qbxl_mfma(KD kd, TD t, const int *ubox,
+                                                      const int *uc0, const int *gcs,
+                                                      const int *gcl, const double *ctr, int nc,
+                                                      const int *l3s, const int *l3l,
+                                                      const double *mpole, double *qexp)
+{
+    constexpr int NQ = (PQ + 1) * (PQ + 1);
+    constexpr int RT = (NQ + 15) / 16;
+    extern __shared__ double lds[];
+    const int u = blockIdx.x;
+    const int it = ubox[u];
+    const int c0 = uc0[u];
+    const int ncen = min(GC_GROUP, gcs[it + 1] - gcs[it] - c0);
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    if (wv >= ncen) return;  // no barriers below: idle waves leave at once
+    const int col = lane & 15, kr = lane >> 4;
+    const int nr = kd.nr;
+    const int rb0 = blockIdx.y * M2QM_RB;
+    const int nrb = min(M2QM_RB, nr - rb0);
+    const int c = gcl[gcs[it] + c0 + wv];
+    const v3 cc = ld3(ctr, nc, c);
+    const int pn = kd.pmax + PQ;
+    double *T = lds + 2L * wv * (pn + 1) * (pn + 1);
+    // the (k, l) of this lane's A rows, tile by tile
+    int rk[RT], rl[RT];
+#pragma unroll
+    for (int q = 0; q < RT; ++q) {
+        const int row = q * 16 + col;
+        if (row < NQ)
+            nm_of(row, rk[q], rl[q]);
+        else
+            rk[q] = rl[q] = 0;
+    }
+    f3_d4 cre[RT][2], cim[RT][2];
+#pragma unroll
+    for (int q = 0; q < RT; ++q)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) cre[q][h] = cim[q][h] = f3_d4{0, 0, 0, 0};
+    const int e0 = l3s[it], e1 = l3s[it + 1];
+    for (int e = e0; e < e1; ++e) {
+        const int D = l3l[e];
+        const int lv = t.level[D];
+        const int p1 = kd.order[lv];
+        const int N1 = p1 + PQ;
+        const int ncf = (p1 + 1) * (p1 + 1);
+        const v3 d = sub(cc, boxc(t, D));
+        const double r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+        const double ir2 = 1.0 / r2;
+        const cplx w = {d.x * ir2, d.y * ir2};
+        for (int m = lane; m <= N1; m += 64) {
+            cplx tmm = {sqrt(ir2), 0};
+            for (int i = 1; i <= m; ++i) tmm = (-(2.0 * i - 1.0)) * (tmm * w);
+            cplx a = {0, 0}, b = tmm;
+            for (int n = m; n <= N1; ++n) {
+                if (n > m) {
+                    const cplx cn = ir2 * (((2.0 * n - 1.0) * d.z) * b -
+                                           ((double)(n + m - 1) * (n - m - 1)) * a);
+                    a = b;
+                    b = cn;
+                }
+                stc(T + 2 * IDX(n, m), b);
+                if (m > 0) stc(T + 2 * IDX(n, -m), neg_m(b, m));
+            }
+        }
+        wave_sync();
+        const double *am = mpole + ecoef_off(kd, D, lv);
+        const int KS = (ncf + 3) / 4;
+        for (int ks = 0; ks < KS; ++ks) {
+            const int kq = ks * 4 + kr;  // (n, m) of the A column and B row
+            int n = 0, m = 0;
+            const bool kin = kq < ncf;
+            if (kin) nm_of(kq, n, m);
+            double bre[2], bim[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int rr = h * 16 + col;
+                bre[h] = bim[h] = 0.0;
+                if (kin && rr < nrb) {
+                    const cplx bv = ldcn(kd, am, kq, rb0 + rr);
+                    bre[h] = bv.x;
+                    bim[h] = bv.y;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < RT; ++q) {
+                const int row = q * 16 + col;
+                double are = 0.0, aim = 0.0;
+                if (kin && row < NQ) {
+                    const int N = n + rk[q], M = m + rl[q];
+                    if (M >= -N && M <= N) {
+                        are = T[2 * IDX(N, M)];
+                        aim = T[2 * IDX(N, M) + 1];
+                    }
+                }
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    cre[q][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bre[h], cre[q][h], 0, 0, 0);
+                    cre[q][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(-aim, bim[h], cre[q][h], 0, 0, 0);
+                    cim[q][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bim[h], cim[q][h], 0, 0, 0);
+                    cim[q][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, bre[h], cim[q][h], 0, 0, 0);
+                }
+            }
+        }
+        wave_sync();  // T read before the next box overwrites it
+    }
+    double *o = qexp + 2 * (size_t)c * NQ * nr;
+#pragma unroll
+    for (int q = 0; q < RT; ++q)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int row = q * 16 + 4 * j + kr;
+            if (row < NQ) {
+                int k, l;
+                nm_of(row, k, l);
+                const double sg = (k & 1) ? -1.0 : 1.0;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int rr = h * 16 + col;
+                    if (rr < nrb) {
+                        double *dst = o + 2 * ((long)row * nr + rb0 + rr);
+                        dst[0] += sg * cre[q][h][j];
+                        dst[1] += sg * cim[q][h][j];
+                    }
+                }
+            }
+        }
+}
+
 // L2QBXL: one thread per global center
 template <int PQ, int NRC>
 __global__ void k3_l2qbxl(KD kd, TD t, int ng, const int *gcent, const int *c2tb,
@@ -2111,6 +2254,32 @@ int f3_translate_box_multipoles_to_qbx_local(qbx_ctx *ctx, void *stream, const q
     const int ng = G->nglobal;
     if (ng == 0 || R->n3 == 0) return QBX_OK;
     const int pn = kd.pmax + kd.pq;
+    if (kd.nr >= 4 && kd.pq <= 5 && G->gc_group_plan.nunits > 0) {
+        // many right-hand sides: the matrix-core kernel, one harmonic table
+        // per wave (center)
+        QBX_CHECK(G->trav == R, "M2QBXL: geometry was built for another traversal");
+        const unit_plan &V = G->gc_group_plan;
+        const size_t lds4 = sizeof(double) * 2 * (size_t)GC_GROUP * (pn + 1) * (pn + 1);
+        QBX_CHECK(lds4 <= 64 * 1024, "M2QBXL: orders too high for LDS staging (%d + %d)",
+                  kd.pmax, kd.pq);
+        const TD td4 = make_td(T);
+        const dim3 grid(V.nunits, cdiv(kd.nr, M2QM_RB));
+#define L3_M2QM(PQV)                                                                       \
+    hipLaunchKernelGGL(k3_m2qbxl_mfma<PQV>, grid, dim3(256), lds4, st, kd, td4, V.unit_box.p, \
+                       V.unit_t0.p, G->gc_by_tbox_starts.p, G->gc_by_tbox.p, G->centers.p, \
+                       G->ncenters, R->l3_starts.p, R->l3_lists.p, mpole, qexp)
+        switch (kd.pq) {
+        case 0: L3_M2QM(0); break;
+        case 1: L3_M2QM(1); break;
+        case 2: L3_M2QM(2); break;
+        case 3: L3_M2QM(3); break;
+        case 4: L3_M2QM(4); break;
+        default: L3_M2QM(5); break;
+        }
+#undef L3_M2QM
+        QBX_HIP(hipGetLastError());
+        return QBX_OK;
+    }
     // one harmonic table per 16-lane center group: 16 groups per block, or
     // 8 / 4 for the orders whose tables would not fit 64 KiB of LDS
     int block = 256;

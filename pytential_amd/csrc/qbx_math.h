@@ -14,6 +14,71 @@
 
 namespace qbx {
 
+// ---- cheap elementary functions for the pair loops -------------------------
+// The device library's log (double-double internally), sqrt and IEEE
+// division are correctly rounded or nearly so and cost 30..90 instructions
+// each; the pair loops only need a few ulp.
+
+// 1 / sqrt(x) for positive normal x: the hardware estimate refined by two
+// Newton steps (each squares the relative error)
+__device__ __forceinline__ double qbx_rsqrt(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    double e = __builtin_fma(-h * y, y, 0.5);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-h * y, y, 0.5);
+    return __builtin_fma(y, e, y);
+}
+
+// 1 / x for normal x: the hardware estimate refined by two Newton steps
+__device__ __forceinline__ double qbx_rcp(double x)
+{
+    double y = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-x, y, 1.0);
+    return __builtin_fma(y, e, y);
+}
+
+// natural logarithm of a positive normal x, within a few ulp: x = m 2^e with
+// m in [sqrt(1/2), sqrt(2)), ln m = 2 atanh(s) with s = (m - 1)/(m + 1),
+// |s| <= 0.1716, summed as 2 s sum_{j<=11} s^(2j) / (2j + 1) (the first
+// dropped term is below 1e-18 relative), e ln 2 in two parts (Cody and
+// Waite's split of ln 2: the high part has trailing zeros, so e times it is
+// exact)
+__device__ __forceinline__ double qbx_log(double x)
+{
+    double m = __builtin_amdgcn_frexp_mant(x);   // [0.5, 1)
+    int e = __builtin_amdgcn_frexp_exp(x);
+    if (m < 0.70710678118654752440) {
+        m *= 2.0;
+        e -= 1;
+    }
+    const double num = m - 1.0, den = m + 1.0;
+    const double r = qbx_rcp(den);
+    double sv = num * r;
+    sv = __builtin_fma(r, __builtin_fma(-den, sv, num), sv);
+    const double s2 = sv * sv;
+    double p = 1.0 / 23.0;
+    p = __builtin_fma(p, s2, 1.0 / 21.0);
+    p = __builtin_fma(p, s2, 1.0 / 19.0);
+    p = __builtin_fma(p, s2, 1.0 / 17.0);
+    p = __builtin_fma(p, s2, 1.0 / 15.0);
+    p = __builtin_fma(p, s2, 1.0 / 13.0);
+    p = __builtin_fma(p, s2, 1.0 / 11.0);
+    p = __builtin_fma(p, s2, 1.0 / 9.0);
+    p = __builtin_fma(p, s2, 1.0 / 7.0);
+    p = __builtin_fma(p, s2, 1.0 / 5.0);
+    p = __builtin_fma(p, s2, 1.0 / 3.0);
+    const double t = (2.0 * sv) * s2 * p;      // 2 s (s^2/3 + s^4/5 + ...)
+    constexpr double LN2_HI = 6.93147180369123816490e-01;
+    constexpr double LN2_LO = 1.90821492927058770002e-10;
+    const double de = (double)e;
+    return __builtin_fma(de, LN2_HI, __builtin_fma(de, LN2_LO, t) + 2.0 * sv);
+}
+
+
 struct cplx {
     double x, y;
 };
@@ -38,7 +103,7 @@ __device__ __forceinline__ cplx conj(cplx a) { return {a.x, -a.y}; }
 __device__ __forceinline__ double abs2(cplx a) { return a.x * a.x + a.y * a.y; }
 __device__ __forceinline__ cplx cinv(cplx a)
 {
-    double d = 1.0 / abs2(a);
+    double d = qbx_rcp(abs2(a));
     return {a.x * d, -a.y * d};
 }
 __device__ __forceinline__ cplx cdivz(cplx a, cplx b) { return a * cinv(b); }
@@ -168,7 +233,7 @@ __device__ __forceinline__ void hankel01_series(double x, cplx &h0, cplx &h1)
             d = d * t + c_ser_D[k];
         }
     }
-    const double lg = log(0.5 * x);
+    const double lg = qbx_log(0.5 * x);
     constexpr double TWO_OVER_PI = 0.63661977236758134308;
     constexpr double ONE_OVER_PI = 0.31830988618379067154;
     if (WANT0) h0 = {a, TWO_OVER_PI * lg * a - ONE_OVER_PI * c};
@@ -252,7 +317,7 @@ __device__ __forceinline__ void hankel01_series_tab(double x, double ix, const d
             d = d * t + bd.y;
         }
     }
-    const double lg = log(0.5 * x);
+    const double lg = qbx_log(0.5 * x);
     constexpr double TWO_OVER_PI = 0.63661977236758134308;
     constexpr double ONE_OVER_PI = 0.31830988618379067154;
     if (WANT0) h0 = {a, TWO_OVER_PI * lg * a - ONE_OVER_PI * c};
@@ -301,6 +366,72 @@ __device__ __forceinline__ void hankel01_wave(double x, double ix, const double 
             hankel01_series_tab<14, WANT0, WANT1>(x, ix, tab, a0, a1);
         else
             hankel01_series_tab<10, WANT0, WANT1>(x, ix, tab, a0, a1);
+        if (!big) {
+            h0 = a0;
+            h1 = a1;
+        }
+    }
+}
+
+// The series of hankel01_series with its coefficients read from the
+// constant tables through an offset the compiler cannot see through (a
+// wave-uniform zero kept in a scalar register): the reads stay scalar loads
+// inside the caller's loop, served by the scalar cache, and feed the FMAs as
+// scalar operands -- no vector registers held across the loop and no LDS
+// traffic.
+template <int K, bool WANT0, bool WANT1>
+__device__ __forceinline__ void hankel01_series_sc(double x, double ix, int off, cplx &h0,
+                                                   cplx &h1)
+{
+    const double t = 0.25 * x * x;
+    double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+#pragma unroll
+    for (int k = K - 1; k >= 0; --k) {
+        if (WANT0) {
+            a = a * t + c_ser_A[k + off];
+            c = c * t + c_ser_C[k + off];
+        }
+        if (WANT1) {
+            b = b * t + c_ser_B[k + off];
+            d = d * t + c_ser_D[k + off];
+        }
+    }
+    const double lg = qbx_log(0.5 * x);
+    constexpr double TWO_OVER_PI = 0.63661977236758134308;
+    constexpr double ONE_OVER_PI = 0.31830988618379067154;
+    if (WANT0) h0 = {a, TWO_OVER_PI * lg * a - ONE_OVER_PI * c};
+    if (WANT1) {
+        double j1v = 0.5 * x * b;
+        h1 = {j1v, -TWO_OVER_PI * ix + TWO_OVER_PI * lg * j1v - 0.5 * x * ONE_OVER_PI * d};
+    }
+}
+
+// hankel01_wave with the scalar-operand series (off: a wave-uniform zero the
+// caller hides from the optimiser once per iteration)
+template <bool WANT0, bool WANT1>
+__device__ __forceinline__ void hankel01_wave_sc(double x, double ix, int off, cplx &h0,
+                                                 cplx &h1)
+{
+    const bool big = x > 6.0;
+    if (__any(big)) {
+        if (big) {
+            const H01 r = hankel01_libm<WANT0, WANT1>(x);
+            h0 = r.h0;
+            h1 = r.h1;
+        }
+    }
+    if (__any(!big)) {
+        cplx a0 = {0, 0}, a1 = {0, 0};
+        if (__any(!big && x > 5.0))
+            hankel01_series_sc<26, WANT0, WANT1>(x, ix, off, a0, a1);
+        else if (__any(!big && x > 4.0))
+            hankel01_series_sc<22, WANT0, WANT1>(x, ix, off, a0, a1);
+        else if (__any(!big && x > 3.0))
+            hankel01_series_sc<18, WANT0, WANT1>(x, ix, off, a0, a1);
+        else if (__any(!big && x > 1.5))
+            hankel01_series_sc<14, WANT0, WANT1>(x, ix, off, a0, a1);
+        else
+            hankel01_series_sc<10, WANT0, WANT1>(x, ix, off, a0, a1);
         if (!big) {
             h0 = a0;
             h1 = a1;
