@@ -599,6 +599,7 @@ k3_p2e_mfma(KD kd, TD t, int lv, int i0, int R, const int *boxes, const int *sta
                 Nu[3 * tid + 2] = nrm[2 * (size_t)ns + si];
             }
         }
+        int imag = 0;
         for (int e = tid; e < P2E3M_S * 16; e += 256) {
             const int k = e >> 4, r = e & 15;
             double vr = 0.0, vi = 0.0;
@@ -607,10 +608,13 @@ k3_p2e_mfma(KD kd, TD t, int lv, int i0, int R, const int *boxes, const int *sta
                 vr = pq[0];
                 vi = pq[1];
             }
+            imag |= vi != 0.0;
             Qs[2 * e] = vr;
             Qs[2 * e + 1] = vi;
         }
-        __syncthreads();
+        // real strengths (a real density): the products with the zero
+        // imaginary parts are skipped (block-uniform)
+        imag = __syncthreads_or(imag);
         for (int ks = 0; ks < P2E3M_S / 4; ++ks) {
             const int sl = ks * 4 + kr;
             const double bre = Qs[2 * (sl * 16 + col)], bim = Qs[2 * (sl * 16 + col) + 1];
@@ -638,8 +642,10 @@ k3_p2e_mfma(KD kd, TD t, int lv, int i0, int R, const int *boxes, const int *sta
                         }
                     }
                     cre[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bre, cre[q], 0, 0, 0);
-                    cre[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, bim, cre[q], 0, 0, 0);
-                    cim[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bim, cim[q], 0, 0, 0);
+                    if (imag) {
+                        cre[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, bim, cre[q], 0, 0, 0);
+                        cim[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bim, cim[q], 0, 0, 0);
+                    }
                     cim[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, bre, cim[q], 0, 0, 0);
                 }
             }
@@ -1290,13 +1296,17 @@ __global__ void __launch_bounds__(256) k3_p2qbxl_mr(KD kd, TD t, const int *ubox
                                 0.5 * (nmn * T[IDX(nn + 1, m + 1)] - npl * T[IDX(nn + 1, m - 1)]));
             }
         }
+        int imag = 0;
         for (int e = tid; e < n * nrb; e += 256) {
             const int k = e / nrb, r = e - k * nrb;
             const double *pq = str + 2 * ((size_t)sidx[k] * nr + rb0 + r);
             Bs[2 * (k * MR_RB + r)] = pq[0];
             Bs[2 * (k * MR_RB + r) + 1] = pq[1];
+            imag |= pq[1] != 0.0;
         }
-        __syncthreads();
+        // real strengths (a real density): the products with the zero
+        // imaginary parts are skipped (block-uniform)
+        imag = __syncthreads_or(imag);
         // C[row][rhs] += sum_k A[k][row] B[k][rhs]: lane holds A[row0 + col][k]
         // and B[k][16 h + col], k = 4 ks + kr; slots k >= n hold stale data
         // and are masked (rows and columns past the valid ones only reach
@@ -1323,8 +1333,10 @@ __global__ void __launch_bounds__(256) k3_p2qbxl_mr(KD kd, TD t, const int *ubox
 #pragma unroll
                     for (int h = 0; h < 2; ++h) {
                         cre[j][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bre[h], cre[j][h], 0, 0, 0);
-                        cre[j][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(-aim, bim[h], cre[j][h], 0, 0, 0);
-                        cim[j][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bim[h], cim[j][h], 0, 0, 0);
+                        if (imag) {
+                            cre[j][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(-aim, bim[h], cre[j][h], 0, 0, 0);
+                            cim[j][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bim[h], cim[j][h], 0, 0, 0);
+                        }
                         cim[j][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, bre[h], cim[j][h], 0, 0, 0);
                     }
                 }
@@ -1609,6 +1621,129 @@ __global__ void __launch_bounds__(256) k3_m2qbxl_mfma(KD kd, TD t, const int *ub
                         double *dst = o + 2 * ((long)row * nr + rb0 + rr);
                         dst[0] += sg * cre[q][h][j];
                         dst[1] += sg * cim[q][h][j];
+                    }
+                }
+            }
+        }
+}
+
+// L2QBXL for many right-hand sides on the FP64 matrix cores: a block per
+// group of GC_GROUP global centers of one target box (they share the box's
+// local expansion L), wave w takes center w and the 32 right-hand sides of
+// blockIdx.y.  The wave builds Y_j^i(c - b), j <= p (the recurrences of
+// k3_l2qbxl, one column i per lane, Y_j^-i = (-1)^i conj(Y_j^i)) in its LDS
+// slice; then out[(a,b)][r] += sum_(n,m) conj(Y_(n-a)^(m-b)) L[(n,m)][r]
+// (zero unless n >= a and |m - b| <= n - a) as four real
+// v_mfma_f64_16x16x4f64 per K-step (lane layout as k3_m2qbxl_mfma).
+template <int PQ>
+__global__ void __launch_bounds__(256) k3_l2qbxl_mfma(KD kd, TD t, const int *ubox,
+                                                      const int *uc0, const int *gcs,
+                                                      const int *gcl, const int *tboxes,
+                                                      const double *ctr, int nc,
+                                                      const double *local, double *qexp)
+{
+    constexpr int NQ = (PQ + 1) * (PQ + 1);
+    constexpr int RT = (NQ + 15) / 16;
+    extern __shared__ double lds[];
+    const int u = blockIdx.x;
+    const int it = ubox[u];
+    const int c0 = uc0[u];
+    const int ncen = min(GC_GROUP, gcs[it + 1] - gcs[it] - c0);
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    if (wv >= ncen) return;  // no barriers below: idle waves leave at once
+    const int col = lane & 15, kr = lane >> 4;
+    const int nr = kd.nr;
+    const int rb0 = blockIdx.y * M2QM_RB;
+    const int nrb = min(M2QM_RB, nr - rb0);
+    const int c = gcl[gcs[it] + c0 + wv];
+    const int Bx = tboxes[it];
+    const int lv = t.level[Bx];
+    const int p1 = kd.order[lv];
+    const int ncf = (p1 + 1) * (p1 + 1);
+    double *Y = lds + 2L * wv * (kd.pmax + 1) * (kd.pmax + 1);
+    const v3 d = sub(ld3(ctr, nc, c), boxc(t, Bx));
+    const double r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+    const cplx hw = {-0.5 * d.x, -0.5 * d.y};
+    for (int i = lane; i <= p1; i += 64) {
+        cplx ymm = {1, 0};
+        for (int q = 1; q <= i; ++q) ymm = (1.0 / q) * (ymm * hw);
+        cplx ya = {0, 0}, yb = ymm;
+        for (int j = i; j <= p1; ++j) {
+            if (j > i) {
+                const cplx cn = (1.0 / ((double)(j - i) * (j + i))) *
+                                (((2.0 * j - 1.0) * d.z) * yb - r2 * ya);
+                ya = yb;
+                yb = cn;
+            }
+            stc(Y + 2 * IDX(j, i), yb);
+            if (i > 0) stc(Y + 2 * IDX(j, -i), (i & 1) ? -conj(yb) : conj(yb));
+        }
+    }
+    wave_sync();
+    int ra[RT], rb[RT];
+#pragma unroll
+    for (int q = 0; q < RT; ++q) {
+        const int row = q * 16 + col;
+        if (row < NQ)
+            nm_of(row, ra[q], rb[q]);
+        else
+            ra[q] = rb[q] = 0;
+    }
+    f3_d4 cre[RT][2], cim[RT][2];
+#pragma unroll
+    for (int q = 0; q < RT; ++q)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) cre[q][h] = cim[q][h] = f3_d4{0, 0, 0, 0};
+    const double *L = local + ecoef_off(kd, Bx, lv);
+    const int KS = (ncf + 3) / 4;
+    for (int ks = 0; ks < KS; ++ks) {
+        const int kq = ks * 4 + kr;
+        int n = 0, m = 0;
+        const bool kin = kq < ncf;
+        if (kin) nm_of(kq, n, m);
+        double bre[2], bim[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int rr = h * 16 + col;
+            bre[h] = bim[h] = 0.0;
+            if (kin && rr < nrb) {
+                const cplx bv = ldcn(kd, L, kq, rb0 + rr);
+                bre[h] = bv.x;
+                bim[h] = bv.y;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < RT; ++q) {
+            const int row = q * 16 + col;
+            double are = 0.0, aim = 0.0;
+            const int j = n - ra[q], i = m - rb[q];
+            if (kin && row < NQ && j >= 0 && i >= -j && i <= j) {
+                are = Y[2 * IDX(j, i)];
+                aim = -Y[2 * IDX(j, i) + 1];  // conj
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                cre[q][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bre[h], cre[q][h], 0, 0, 0);
+                cre[q][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(-aim, bim[h], cre[q][h], 0, 0, 0);
+                cim[q][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bim[h], cim[q][h], 0, 0, 0);
+                cim[q][h] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, bre[h], cim[q][h], 0, 0, 0);
+            }
+        }
+    }
+    double *o = qexp + 2 * (size_t)c * NQ * nr;
+#pragma unroll
+    for (int q = 0; q < RT; ++q)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int row = q * 16 + 4 * j + kr;
+            if (row < NQ) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int rr = h * 16 + col;
+                    if (rr < nrb) {
+                        double *dst = o + 2 * ((long)row * nr + rb0 + rr);
+                        dst[0] += cre[q][h][j];
+                        dst[1] += cim[q][h][j];
                     }
                 }
             }
@@ -2314,6 +2449,29 @@ int f3_translate_box_local_to_qbx_local(qbx_ctx *ctx, void *stream, const qbx_tr
     const int ng = G->nglobal;
     if (ng == 0) return QBX_OK;
     const TD td = make_td(T);
+    if (kd.nr >= 4 && kd.pq <= 5 && G->gc_group_plan.nunits > 0) {
+        // many right-hand sides: the matrix-core kernel (a Y table per wave)
+        QBX_CHECK(G->trav == R, "L2QBXL: geometry was built for another traversal");
+        const unit_plan &V = G->gc_group_plan;
+        const size_t lds4 = sizeof(double) * 2 * (size_t)GC_GROUP * (kd.pmax + 1) * (kd.pmax + 1);
+        QBX_CHECK(lds4 <= 64 * 1024, "L2QBXL: order too high for LDS staging (%d)", kd.pmax);
+        const dim3 grid(V.nunits, cdiv(kd.nr, M2QM_RB));
+#define L3_L2QM(PQV)                                                                       \
+    hipLaunchKernelGGL(k3_l2qbxl_mfma<PQV>, grid, dim3(256), lds4, st, kd, td, V.unit_box.p, \
+                       V.unit_t0.p, G->gc_by_tbox_starts.p, G->gc_by_tbox.p,               \
+                       R->target_boxes.p, G->centers.p, G->ncenters, local, qexp)
+        switch (kd.pq) {
+        case 0: L3_L2QM(0); break;
+        case 1: L3_L2QM(1); break;
+        case 2: L3_L2QM(2); break;
+        case 3: L3_L2QM(3); break;
+        case 4: L3_L2QM(4); break;
+        default: L3_L2QM(5); break;
+        }
+#undef L3_L2QM
+        QBX_HIP(hipGetLastError());
+        return QBX_OK;
+    }
 #define L3_L2Q(N)                                                                          \
     DISPATCH3_PQ_NR(kd.pq, N, k3_l2qbxl, dim3(cdiv(ng, 64)), dim3(64), 0, st, kd, td, ng,  \
                     G->global_qbx_centers.p, G->center_to_target_box.p, R->target_boxes.p, \
