@@ -110,9 +110,12 @@ def interaction_counts(geo):
     n_l3 = np.diff(l3s)
     gc = h(geo.global_qbx_centers())
     c2tb = h(geo.qbx_center_to_target_box())[gc]
+    ctt_starts = h(geo.center_to_tree_targets()[0])
     return {
         "p2p_pairs": int(np.sum(nfilt_box * src_in_l1)),
         "p2qbxl_pairs": int(np.sum(src_in_l1[c2tb])),
+        # (QBX target, list-1 source of its center's box) pairs
+        "tsqbx_pairs": int(np.sum(np.diff(ctt_starts)[gc] * src_in_l1[c2tb])),
         "m2qbxl_levels": levels[l3l] if len(l3l) else np.zeros(0, np.int64),
         "l2qbxl": int(len(gc)),
         "qbxl2p": int(g.nctt),
@@ -399,6 +402,10 @@ def stage_flops_3d(counts, orders, pq, nrhs):
     # P2QBXL: the pair's basis (r, angles, the harmonic recurrence: ~4 ncq +
     # 20) once, then a complex-by-real MAC per coefficient and density
     p2qbxl = counts["p2qbxl_pairs"] * (4 * ncq + 20 + 4 * ncq * nrhs)
+    # target-specific sum: per (QBX target, source) pair the distances and the
+    # angle cosine (~20), the Legendre recurrence and the power series (~8 per
+    # order), then a real MAC per density
+    tsqbx = counts["tsqbx_pairs"] * (20 + 8 * (pq + 1) + 2 * nrhs)
     # M2L: dense (ncf x ncf) complex translation per (box, list-2 box) and density
     ttc, l2s = counts["ttp"], counts["l2s"]
     ncf_t = (o[lv[ttc]] + 1) ** 2
@@ -410,7 +417,7 @@ def stage_flops_3d(counts, orders, pq, nrhs):
     if len(m2q_lv):
         csum = np.concatenate([[0], np.cumsum(8 * ncq * (o[m2q_lv] + 1) ** 2 * nrhs)])
         m2q = int(np.sum(csum[l3s[c2tb + 1]] - csum[l3s[c2tb]]))
-    return {"p2qbxl": int(p2qbxl), "m2l": m2l, "m2qbxl": m2q}
+    return {"p2qbxl": int(p2qbxl), "tsqbx": int(tsqbx), "m2l": m2l, "m2qbxl": m2q}
 
 
 def run_sphere3d(args):
@@ -433,9 +440,12 @@ def run_sphere3d(args):
     dens = torch.as_tensor(dens_np, dtype=torch.float64, device="cuda")
     steps = args.steps3d
 
-    def make(order):
+    def make(order, tsq=True):
+        # the list-1 part of the QBX targets by the target-specific sum, as
+        # pytential's fmmlib backend does by default (_use_target_specific_qbx
+        # None: "use if possible"); the expansion pipeline is timed beside it
         src = QBXLayerPotentialSource(discr, fine_order=args.fine3d, qbx_order=3,
-                                      fmm_order=order)
+                                      fmm_order=order, _use_target_specific_qbx=tsq)
         geo = src.qbx_fmm_geometry_data((("self", +1),))
         return src, geo
 
@@ -470,7 +480,7 @@ def run_sphere3d(args):
     # stage by stage on one stream, HIP events between stages: per-stage times
     # and the roofline of the dominant kernel
     stages = ["p2m", "m2m", "p2p", "m2l", "m2p", "p2l", "l2l", "l2p", "p2qbxl", "m2qbxl",
-              "l2qbxl", "qbxl2p"]
+              "l2qbxl", "qbxl2p", "tsqbx"]
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(stages) + 1)]
     qd = src.quad_density(dens)
     st, _ = w.reorder_sources(qd, src.dev["quad_weights"], None)
@@ -501,16 +511,21 @@ def run_sphere3d(args):
         ev[11].record()
         w.eval_qbx_expansions(q, pot)
         ev[12].record()
+        pot.add_(w.eval_target_specific_qbx_locals(st, None))
+        ev[13].record()
         torch.cuda.synchronize()
         if rep == 1:
             per_stage = {s: ev[i].elapsed_time(ev[i + 1]) for i, s in enumerate(stages)}
     counts = interaction_counts(geo)
     lo = list(map(int, w._lo))
     flops = stage_flops_3d(counts, lo, 3, nrhs)
-    kernel_of = {"p2qbxl": "k3_p2qbxl_mr", "m2l": "k3_m2l_mfma", "m2qbxl": "k3_m2qbxl"}
+    # P2QBXL is the zero fill of the expansions here: the target-specific sum
+    # does its work
+    flops.pop("p2qbxl")
+    kernel_of = {"tsqbx": "k3_tsqbx_mr", "m2l": "k3_m2l_mfma", "m2qbxl": "k3_m2qbxl_mfma"}
     dom = max(flops, key=lambda k: per_stage[k])
     ach = flops[dom] / (per_stage[dom] * 1e-3) / 1e12
-    roofline = {"bound": "mfma" if dom in ("p2qbxl", "m2l") else "valu", "compute": "fp64",
+    roofline = {"bound": "mfma", "compute": "fp64",
                 "kernel": kernel_of[dom], "stage": dom, "achieved": round(ach, 3),
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": None,
@@ -523,6 +538,16 @@ def run_sphere3d(args):
                                        FP64_PEAK_TFLOPS, 4)} for k in flops}
     log(f"[sphere3d] p=8, {nrhs} densities: {dt * 1e3:.1f} ms; stages "
         f"{ {k: round(v, 2) for k, v in per_stage.items()} }")
+    # the expansion pipeline (P2QBXL + QBXL2P for the list-1 part) on the
+    # same tree: same result to rounding, for comparison
+    src_e, _ = make(8, tsq=False)
+    we = src_e.make_wrangler(geo, "laplace", 0.0, "S", nrhs)
+    dte, oute = timed(stepper(src, we, dens), steps)
+    expansion_pipeline = {"ms_per_step": round(dte * 1e3, 4),
+                          "rel_diff_vs_target_specific": float(
+                              np.max(np.abs(oute.real.cpu().numpy() - res)) /
+                              np.max(np.abs(res)))}
+    log(f"[sphere3d] expansion pipeline: {dte * 1e3:.1f} ms")
     # one density per evaluation
     w1 = src.make_wrangler(geo, "laplace", 0.0, "S", 1)
     dt1, out1 = timed(stepper(src, w1, dens[:, 0].contiguous()), steps)
@@ -543,7 +568,8 @@ def run_sphere3d(args):
                                 f"({discr.ndofs}), quad {args.fine3d}x{args.fine3d} "
                                 f"collapsed Gauss ({discr.nsources} sources), qbx_order 3, "
                                 f"fmm_order 8, {nrhs} densities (Y_lm l<=6 + N(0,0.01) "
-                                f"noise, seed 1002), on-surface targets"),
+                                f"noise, seed 1002), on-surface targets, target-specific "
+                                f"QBX for the list-1 part"),
                    "ntargets": discr.ndofs, "nboxes": tree.nboxes, "nlevels": tree.nlevels},
         "plain_one_density": {"value": round(discr.ndofs / dt1, 1), "unit": "points/s",
                               "ms_per_step": round(dt1 * 1e3, 4),
@@ -552,6 +578,7 @@ def run_sphere3d(args):
                                   np.max(np.abs(res[:, 0])))},
         "S_Ylm_vs_Ylm_over_2l_plus_1_worst_rel_err": eig,
         "cold_geometry_s": round(cold, 4),
+        "expansion_pipeline": expansion_pipeline,
         "per_stage_ms": {k: round(v, 4) for k, v in per_stage.items()},
         "stage_roofline": stage_rl,
         "roofline": roofline,

@@ -165,6 +165,11 @@ typedef struct {
      * order (the centers' entries are not read). */
     int target_deriv;
     const double *target_dir;
+    /* 3D Laplace in qbx_drive_fmm: 1 = the list-1 part of the QBX targets by
+     * the target-specific sum (qbx_eval_target_specific_qbx_locals_nrhs)
+     * instead of P2QBXL, as pytential's fmmlib backend does when
+     * _use_target_specific_qbx allows it (qbx/fmmlib.py:73-108) */
+    int target_specific_qbx;
 } qbx_kernel_desc;
 
 /* Box expansions (multipoles and locals) are level-compact: the boxes of
@@ -424,6 +429,18 @@ int qbx_eval_target_specific_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tr
                                         int qbx_order, const double *strengths_tree,
                                         const double *dipvec_tree, double *pot,
                                         double *grad);
+/* The same sum for nrhs densities in one call (3D Laplace, S or D): the
+ * kernel value of each (target, source) pair is computed once and applied to
+ * every density on the FP64 matrix cores.  strengths_tree [nsources][nrhs]
+ * complex, pot [ntargets][nrhs] tree order; accumulate: add into pot at the
+ * QBX targets (0: overwrite them).  Output unscaled like the above. */
+int qbx_eval_target_specific_qbx_locals_nrhs(qbx_ctx *ctx, void *stream,
+                                             const qbx_tree *tree, const qbx_trav *trav,
+                                             const qbx_geo *geo, int kernel, int op,
+                                             int qbx_order, int nrhs,
+                                             const double *strengths_tree,
+                                             const double *dipvec_tree, double *pot,
+                                             int accumulate);
 
 /* ---- differentiable 3D Helmholtz evaluation at smooth targets --------------
  * Targets far from the surface need no QBX: pytential evaluates them through

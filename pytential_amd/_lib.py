@@ -28,7 +28,7 @@ IP = ct.POINTER(ct.c_int)
 class KernelDesc(ct.Structure):
     _fields_ = [("kernel", I), ("k", D), ("source_kind", I), ("qbx_order", I),
                 ("nlevels", I), ("level_orders", IP), ("nrhs", I),
-                ("target_deriv", I), ("target_dir", P)]
+                ("target_deriv", I), ("target_dir", P), ("target_specific_qbx", I)]
 
 
 class QBXError(RuntimeError):
@@ -83,6 +83,7 @@ _SIGS = {
     "qbx_drive_fmm": [P, P, P, P, P, ct.POINTER(KernelDesc), P, P, P],
     "qbx_drive_fmm_noalloc": [P, P, P, P, P, ct.POINTER(KernelDesc), P, P, P],
     "qbx_eval_target_specific_qbx_locals": [P, P, P, P, P, I, I, D, D, I, P, P, P, P],
+    "qbx_eval_target_specific_qbx_locals_nrhs": [P, P, P, P, P, I, I, I, I, P, P, P, I],
     "qbx_direct_qbx": [P, P, P, P, ct.POINTER(KernelDesc), P, P, P],
     "qbx_p2p_all": [P, P, ct.POINTER(KernelDesc), I, P, P, P, I, P, P],
     "qbx_lpot_matrix": [P, P, ct.POINTER(KernelDesc), I, I, I, P, P, P, P, I, P, P, P, L, P],

@@ -4086,6 +4086,9 @@ static int drive_sizes(const qbx_tree *T, const qbx_geo *G, const qbx_kernel_des
 // The stage DAG of qbx_drive_fmm.  multistream: on the caller's stream plus
 // the device's three side streams joined by this context's events (below);
 // otherwise every stage on the caller's stream in the order of drive_fmm.
+// G = 1/(4 pi r) of the 3D potentials, applied to the target-specific sum
+constexpr double INV4PI_3D = 0.079577471545947667884;
+
 static int drive_impl(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
                       const qbx_geo *G, const qbx_kernel_desc *d, const double *str,
                       const double *nrm, double *out_user, const DriveSizes &z,
@@ -4095,11 +4098,18 @@ static int drive_impl(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_t
     const long nexp = z.nexp, np = z.np;
     double *M = b.M, *L = b.L, *Q = b.Q, *P = b.P, *L4 = b.L4;
     hipStream_t st = S(stream);
+    // 3D Laplace with the target-specific sum: P2QBXL leaves the expansions
+    // at zero (M2QBXL and L2QBXL add into them) and the list-1 part of the
+    // QBX targets is added after QBXL2P
+    const bool tsq = d->target_specific_qbx != 0;
     if (!multistream) {
         QBX_TRY(zero_fill(P, np, st));
         QBX_TRY(qbx_form_multipoles(ctx, st, T, R, d, str, nrm, M));
         QBX_TRY(qbx_eval_direct(ctx, st, T, R, G, d, str, nrm, P, 1));
-        QBX_TRY(qbx_form_global_qbx_locals(ctx, st, T, R, G, d, str, nrm, Q, 0));
+        if (tsq)
+            QBX_TRY(zero_fill(Q, z.nq, st));
+        else
+            QBX_TRY(qbx_form_global_qbx_locals(ctx, st, T, R, G, d, str, nrm, Q, 0));
         QBX_TRY(qbx_form_locals(ctx, st, T, R, d, str, nrm, L4, 0));
         QBX_TRY(coarsen_multipoles_from(ctx, st, T, R, d, M, R->mpole_min_level));
         QBX_TRY(qbx_multipole_to_local(ctx, st, T, R, d, M, L));
@@ -4110,6 +4120,10 @@ static int drive_impl(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_t
         QBX_TRY(qbx_eval_locals(ctx, st, T, R, G, d, L, P, 1));
         QBX_TRY(qbx_translate_box_local_to_qbx_local(ctx, st, T, R, G, d, L, Q, 1));
         QBX_TRY(qbx_eval_qbx_expansions(ctx, st, T, G, d, Q, P));
+        if (tsq)
+            QBX_TRY(tsqbx_nrhs(ctx, st, T, R, G, d->kernel,
+                               d->source_kind == QBX_SOURCE_D ? QBX_TSQBX_D : QBX_TSQBX_S,
+                               d->qbx_order, nr, str, nrm, INV4PI_3D, P, 1));
         QBX_TRY(qbx_finalize_potentials(ctx, stream, T, G, nr, P, out_user));
         return QBX_OK;
     }
@@ -4159,7 +4173,10 @@ static int drive_impl(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_t
         QBX_HIP(hipStreamWaitEvent(s3, e0, 0));
     }
     QBX_TRY(qbx_eval_direct(ctx, s1, T, R, G, d, str, nrm, P, 1));
-    QBX_TRY(qbx_form_global_qbx_locals(ctx, s2, T, R, G, d, str, nrm, Q, 0));
+    if (tsq)
+        QBX_TRY(zero_fill(Q, z.nq, s2));
+    else
+        QBX_TRY(qbx_form_global_qbx_locals(ctx, s2, T, R, G, d, str, nrm, Q, 0));
     QBX_TRY(qbx_form_locals(ctx, s3, T, R, d, str, nrm, L4, 0));
     QBX_HIP(hipEventRecord(eP, s3));
     QBX_TRY(coarsen_multipoles_from(ctx, st, T, R, d, M, R->mpole_min_level));
@@ -4180,6 +4197,10 @@ static int drive_impl(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_t
     QBX_HIP(hipStreamWaitEvent(st, e2, 0));
     QBX_TRY(qbx_translate_box_local_to_qbx_local(ctx, st, T, R, G, d, L, Q, 1));
     QBX_TRY(qbx_eval_qbx_expansions(ctx, st, T, G, d, Q, P));
+    if (tsq)
+        QBX_TRY(tsqbx_nrhs(ctx, st, T, R, G, d->kernel,
+                           d->source_kind == QBX_SOURCE_D ? QBX_TSQBX_D : QBX_TSQBX_S,
+                           d->qbx_order, nr, str, nrm, INV4PI_3D, P, 1));
     QBX_HIP(hipStreamWaitEvent(st, e1, 0));
     QBX_TRY(qbx_finalize_potentials(ctx, stream, T, G, nr, P, out_user));
     return QBX_OK;
@@ -4193,6 +4214,8 @@ int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav 
                   const double *nrm, double *out_user)
 {
     QBX_CHECK(ctx && T && R && G && d && str && out_user, "qbx_drive_fmm: null argument");
+    QBX_CHECK(!d->target_specific_qbx || d->kernel == QBX_LAPLACE3D,
+              "qbx_drive_fmm: the target-specific sum is for 3D Laplace");
     DriveSizes z;
     QBX_TRY(drive_sizes(T, G, d, &z));
     QBX_TRY(ws_reserve(ctx, S(stream), ctx->ws_mpole, z.nm, "multipole workspace"));

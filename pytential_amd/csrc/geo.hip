@@ -185,6 +185,44 @@ static int build_gc_by_tbox(qbx_ctx *ctx, hipStream_t st, const qbx_trav *R, qbx
         QBX_HIP(hipMemcpy(U.unit_t0.p, ut.data(), sizeof(int) * U.nunits,
                           hipMemcpyHostToDevice));
     }
+    // the QBX targets of the centers of each target box, in rows of TSQ_ROWS
+    // (the rows of the batched target-specific sum)
+    const int nc = G->ncenters;
+    std::vector<int> cs(nc + 1, 0), cl(std::max(G->nctt, 1)), gcl(std::max(hs[ntb], 1));
+    if (nc > 0)
+        QBX_HIP(hipMemcpy(cs.data(), G->ctt_starts.p, sizeof(int) * (nc + 1),
+                          hipMemcpyDeviceToHost));
+    if (G->nctt > 0)
+        QBX_HIP(hipMemcpy(cl.data(), G->ctt_lists.p, sizeof(int) * G->nctt,
+                          hipMemcpyDeviceToHost));
+    if (hs[ntb] > 0)
+        QBX_HIP(hipMemcpy(gcl.data(), G->gc_by_tbox.p, sizeof(int) * hs[ntb],
+                          hipMemcpyDeviceToHost));
+    std::vector<int> tb, rows;
+    for (int it = 0; it < ntb; ++it) {
+        int fill = 0;
+        for (int g = hs[it]; g < hs[it + 1]; ++g) {
+            const int c = gcl[g];
+            for (int j = cs[c]; j < cs[c + 1]; ++j) {
+                if (fill == 0) {
+                    tb.push_back(it);
+                    rows.insert(rows.end(), TSQ_ROWS, -1);
+                }
+                rows[rows.size() - TSQ_ROWS + fill] = cl[j];
+                fill = (fill + 1) % TSQ_ROWS;
+            }
+        }
+    }
+    unit_plan &V = G->tsq_plan;
+    V.nunits = (int)tb.size();
+    QBX_TRY(V.unit_box.alloc(std::max(V.nunits, 1)));
+    QBX_TRY(G->tsq_rows.alloc(std::max((long)rows.size(), 1L)));
+    if (V.nunits > 0) {
+        QBX_HIP(hipMemcpy(V.unit_box.p, tb.data(), sizeof(int) * V.nunits,
+                          hipMemcpyHostToDevice));
+        QBX_HIP(hipMemcpy(G->tsq_rows.p, rows.data(), sizeof(int) * rows.size(),
+                          hipMemcpyHostToDevice));
+    }
     return QBX_OK;
 }
 

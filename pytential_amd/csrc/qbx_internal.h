@@ -81,6 +81,11 @@ constexpr int M2P_UNIT_ITER = 8;   // M2P: list-3 boxes per lane
 constexpr int P2Q_UNIT_ITER = 32;  // P2QBXL: list-1 sources per lane
 constexpr int M2Q_UNIT_ITER = 2;   // M2QBXL: list-3 boxes per lane
 constexpr int GC_GROUP = 4;        // centers per block of the batched 3D P2QBXL
+constexpr int TSQ_ROWS = 16;       // QBX targets per block of the batched 3D TSQBX
+int tsqbx_nrhs(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+               const qbx_geo *G, int kernel, int op, int qbx_order, int nrhs,
+               const double *strengths_tree, const double *dipvec_tree, double scale,
+               double *pot, int accumulate);
 
 struct unit_plan {
     int nunits = 0;
@@ -176,6 +181,12 @@ struct qbx_geo {
     qbx::unit_plan p2p_plan, m2p_plan, p2q_plan, m2q_plan;
     // groups of qbx::GC_GROUP consecutive global centers of one target box
     qbx::unit_plan gc_group_plan;
+    // the QBX targets grouped for the many-density target-specific sum:
+    // unit u covers up to qbx::TSQ_ROWS (center, target) entries whose
+    // centers lie in one target box (unit_box[u]); tsq_rows[TSQ_ROWS u + j]
+    // is the tree target index of row j, or -1
+    qbx::unit_plan tsq_plan;
+    qbx::dbuf<int> tsq_rows;
 };
 
 namespace qbx {

@@ -310,6 +310,120 @@ __global__ void __launch_bounds__(256) k_tsqbx(double kre, double kim, int nctt,
     }
 }
 
+// The target-specific sum for many densities (3D Laplace S and D) on the
+// FP64 matrix cores: pot[t][r] (+)= sum_s G(t, s) q[s][r] with G the
+// unit-strength term of tsq_pair_sd -- real for Laplace, computed once per
+// (target, source) pair for all right-hand sides.  A block per unit of
+// tsq_plan: up to TSQ_ROWS QBX targets whose centers share target box it (the
+// A rows: lane i evaluates G for row i % 16 and source k0 + i / 16, exactly
+// the A operand of v_mfma_f64_16x16x4f64), K = the box's list-1 sources in
+// steps of four, shared round-robin by the four waves, columns two tiles of
+// 16 right-hand sides (blockIdx.y chunks of 32).  The waves' partial sums
+// are added in LDS in wave order and written once.
+typedef double tsq_d4 __attribute__((ext_vector_type(4)));
+constexpr int TSQM_RB = 32;
+
+template <int OP, int P>
+__global__ void __launch_bounds__(256) k3_tsqbx_mr(int nr, const int *ubox, const int *rows,
+                                                   const int *tree_ttc, const double *ctr,
+                                                   int nc, const double *tgt, int nt,
+                                                   const double *src, int ns,
+                                                   const int *l1s, const int *l1l,
+                                                   const int *sstart, const int *scount,
+                                                   const double *q, const double *dip,
+                                                   double scale, double *pot, int accumulate)
+{
+    __shared__ double part[4][TSQ_ROWS][TSQM_RB][2];
+    const int u = blockIdx.x;
+    const int it = ubox[u];
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int col = lane & 15, kr = lane >> 4;
+    const int rb0 = blockIdx.y * TSQM_RB;
+    const int nrb = min(TSQM_RB, nr - rb0);
+    // this lane's A row: target ti of center c
+    const int ti = rows[TSQ_ROWS * u + col];
+    const bool rin = ti >= 0;
+    double tc[3] = {0, 0, 0}, cc[3] = {0, 0, 0}, rt = 1.0;
+    if (rin) {
+        const int c = tree_ttc[ti];
+        for (int m = 0; m < 3; ++m) {
+            cc[m] = ctr[(size_t)m * nc + c];
+            tc[m] = tgt[(size_t)m * nt + ti] - cc[m];
+        }
+        rt = sqrt(tc[0] * tc[0] + tc[1] * tc[1] + tc[2] * tc[2]);
+    }
+    const zc k0 = {0, 0};
+    tsq_d4 cre[2], cim[2];
+    for (int h = 0; h < 2; ++h) cre[h] = cim[h] = tsq_d4{0, 0, 0, 0};
+    int step = 0;  // K-step counter over the whole list, dealt to the waves
+    for (int e = l1s[it]; e < l1s[it + 1]; ++e) {
+        const int sb = l1l[e];
+        const int s0 = sstart[sb], n = scount[sb];
+        const int nks = (n + 3) / 4;
+        // the K-steps of this box that fall to this wave
+        int ks = (wv - step) & 3;
+        for (; ks < nks; ks += 4) {
+            const int k = ks * 4 + kr;
+            const bool kin = k < n;
+            const int s = s0 + (kin ? k : 0);
+            double g = 0.0;
+            if (rin && kin) {
+                double sc[3], nu[3] = {0, 0, 0};
+                for (int m = 0; m < 3; ++m) sc[m] = src[(size_t)m * ns + s] - cc[m];
+                const double rs = sqrt(sc[0] * sc[0] + sc[1] * sc[1] + sc[2] * sc[2]);
+                const double ct = (tc[0] * sc[0] + tc[1] * sc[1] + tc[2] * sc[2]) / (rt * rs);
+                if (OP == 1)
+                    for (int m = 0; m < 3; ++m) nu[m] = dip[(size_t)m * ns + s];
+                g = tsq_pair_sd<0, OP, P>(k0, tc, rt, nullptr, sc, rs, ct, nu).x;
+            }
+            double bre[2] = {0, 0}, bim[2] = {0, 0};
+            for (int h = 0; h < 2; ++h) {
+                const int r = h * 16 + col;
+                if (kin && r < nrb) {
+                    bre[h] = q[2 * ((size_t)s * nr + rb0 + r)];
+                    bim[h] = q[2 * ((size_t)s * nr + rb0 + r) + 1];
+                }
+            }
+            // real strengths (the usual case) need no imaginary-part products;
+            // the test is wave-uniform
+            const bool anyim = __ballot(bim[0] != 0.0 || bim[1] != 0.0) != 0;
+            for (int h = 0; h < 2; ++h) {
+                cre[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(g, bre[h], cre[h], 0, 0, 0);
+                if (anyim)
+                    cim[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(g, bim[h], cim[h], 0, 0, 0);
+            }
+        }
+        step = (step + nks) & 3;
+    }
+    // D register j of lane i: row 4 j + i / 16, column i % 16
+    for (int h = 0; h < 2; ++h)
+        for (int j = 0; j < 4; ++j) {
+            part[wv][4 * j + kr][h * 16 + col][0] = cre[h][j];
+            part[wv][4 * j + kr][h * 16 + col][1] = cim[h][j];
+        }
+    __syncthreads();
+    for (int e = tid; e < TSQ_ROWS * TSQM_RB; e += 256) {
+        const int row = e / TSQM_RB, r = e % TSQM_RB;
+        const int tr = rows[TSQ_ROWS * u + row];
+        if (tr < 0 || r >= nrb) continue;
+        double vr = 0.0, vi = 0.0;
+        for (int w = 0; w < 4; ++w) {
+            vr += part[w][row][r][0];
+            vi += part[w][row][r][1];
+        }
+        vr *= scale;
+        vi *= scale;
+        double *dst = pot + 2 * ((size_t)tr * nr + rb0 + r);
+        if (accumulate) {
+            dst[0] += vr;
+            dst[1] += vi;
+        } else {
+            dst[0] = vr;
+            dst[1] = vi;
+        }
+    }
+}
+
 template <int KERNEL, int OP>
 static int launch_p(int order, dim3 grid, hipStream_t st, double kre, double kim, int nctt,
                     const int *a0, const int *a1, const int *a2, const double *ctr, int nc,
@@ -541,6 +655,61 @@ __global__ void __launch_bounds__(256) k3_direct_p2p(double kre, double kim, int
     }
 }
 
+// The many-density target-specific sum; scale multiplies the result (1 for
+// the public entry point's unscaled form, 1/(4 pi) inside the fused driver).
+int tsqbx_nrhs(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+               const qbx_geo *G, int kernel, int op, int qbx_order, int nrhs,
+               const double *strengths_tree, const double *dipvec_tree, double scale,
+               double *pot, int accumulate)
+{
+    QBX_CHECK(ctx && T && R && G && strengths_tree && pot, "TSQBX: null argument");
+    QBX_CHECK(T->dim == 3, "TSQBX is defined for 3D trees (got dim %d)", T->dim);
+    QBX_CHECK(kernel == QBX_LAPLACE3D, "many-density TSQBX: 3D Laplace only (got %d)", kernel);
+    QBX_CHECK(op == QBX_TSQBX_S || op == QBX_TSQBX_D, "many-density TSQBX: op S or D (got %d)",
+              op);
+    QBX_CHECK(op != QBX_TSQBX_D || dipvec_tree, "TSQBX double layer needs dipole vectors");
+    QBX_CHECK(qbx_order >= 0 && qbx_order <= 12, "TSQBX order must be in [0, 12]");
+    QBX_CHECK(nrhs >= 1, "TSQBX: nrhs must be >= 1");
+    QBX_CHECK(G->trav == R, "TSQBX: geometry was built for another traversal");
+    const unit_plan &V = G->tsq_plan;
+    if (V.nunits == 0) return QBX_OK;
+    hipStream_t st = S(stream);
+    const dim3 grid(V.nunits, cdiv(nrhs, TSQM_RB));
+#define TSQM_CASE(OPV, PP)                                                                  \
+    hipLaunchKernelGGL((k3_tsqbx_mr<OPV, PP>), grid, dim3(256), 0, st, nrhs, V.unit_box.p,  \
+                       G->tsq_rows.p, G->tree_ttc.p, G->centers.p, G->ncenters,            \
+                       T->targets.p, T->ntargets, T->sources.p, T->nsources,               \
+                       R->l1_starts.p, R->l1_lists.p, T->box_source_starts.p,              \
+                       T->box_source_counts_nonchild.p, strengths_tree, dipvec_tree, scale, \
+                       pot, accumulate);                                                         \
+    break;
+#define TSQM_OP(OPV)                  \
+    switch (qbx_order) {              \
+    case 0: TSQM_CASE(OPV, 0)         \
+    case 1: TSQM_CASE(OPV, 1)         \
+    case 2: TSQM_CASE(OPV, 2)         \
+    case 3: TSQM_CASE(OPV, 3)         \
+    case 4: TSQM_CASE(OPV, 4)         \
+    case 5: TSQM_CASE(OPV, 5)         \
+    case 6: TSQM_CASE(OPV, 6)         \
+    case 7: TSQM_CASE(OPV, 7)         \
+    case 8: TSQM_CASE(OPV, 8)         \
+    case 9: TSQM_CASE(OPV, 9)         \
+    case 10: TSQM_CASE(OPV, 10)       \
+    case 11: TSQM_CASE(OPV, 11)       \
+    default: TSQM_CASE(OPV, 12)       \
+    }
+    if (op == QBX_TSQBX_S) {
+        TSQM_OP(0)
+    } else {
+        TSQM_OP(1)
+    }
+#undef TSQM_OP
+#undef TSQM_CASE
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
 }  // namespace qbx
 
 using namespace qbx;
@@ -578,6 +747,17 @@ int qbx_eval_target_specific_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tr
     if (hk && op == QBX_TSQBX_D) return launch_p<1, 1>(TSQ_ARGS);
     return launch_p<1, 2>(TSQ_ARGS);
 #undef TSQ_ARGS
+}
+
+int qbx_eval_target_specific_qbx_locals_nrhs(qbx_ctx *ctx, void *stream, const qbx_tree *T,
+                                             const qbx_trav *R, const qbx_geo *G, int kernel,
+                                             int op, int qbx_order, int nrhs,
+                                             const double *strengths_tree,
+                                             const double *dipvec_tree, double *pot,
+                                             int accumulate)
+{
+    return qbx::tsqbx_nrhs(ctx, stream, T, R, G, kernel, op, qbx_order, nrhs, strengths_tree,
+                           dipvec_tree, 1.0, pot, accumulate);
 }
 
 }  // extern "C"

@@ -82,9 +82,12 @@ class QBXExpansionWrangler:
         # TSQBX replaces P2QBXL's list-1 part (qbx/fmmlib.py:73-108, 295-298)
         if use_target_specific_qbx and (self.dim != 3 or output != "pot"):
             raise ValueError("target-specific QBX needs a 3D geometry and potential output")
-        if use_target_specific_qbx and self.nrhs != 1:
-            raise NotImplementedError("target-specific QBX evaluates one density at a time")
+        if use_target_specific_qbx and self.nrhs != 1 and self.k != 0:
+            raise NotImplementedError("3D Helmholtz target-specific QBX evaluates one density "
+                                      "at a time")
         self.using_tsqbx = bool(use_target_specific_qbx)
+        # the fused driver runs the sum in place of P2QBXL (3D Laplace)
+        self.kd.target_specific_qbx = int(self.using_tsqbx and self.k == 0)
         self.expansion_size = None
         if self.kernel_code == _lib.HELMHOLTZ3D:
             # 3D Helmholtz: only the target-specific stage exists in the library
@@ -253,8 +256,21 @@ class QBXExpansionWrangler:
         Returns [ntargets] complex, or [3, ntargets] for the gradient."""
         if self.dim != 3:
             return self.full_output_zeros()
+        kernel = _lib.LAPLACE3D if self.k == 0 else _lib.HELMHOLTZ3D
         if self.nrhs != 1:
-            raise NotImplementedError("target-specific QBX evaluates one density at a time")
+            # many densities: each pair's kernel value once, applied to all of
+            # them on the matrix cores (3D Laplace potential)
+            if kernel != _lib.LAPLACE3D or self.output != "pot":
+                raise NotImplementedError("many-density target-specific QBX: 3D Laplace "
+                                          "potential only")
+            op = _lib.TSQBX_D if self.source_kind == "D" else _lib.TSQBX_S
+            if op == _lib.TSQBX_D and normals is None:
+                raise ValueError("Missing dipvec")
+            pot = self.full_output_zeros()
+            self._a("qbx_eval_target_specific_qbx_locals_nrhs", self.tree.h,
+                    self.traversal.h, self.geo.h, kernel, op, self.qbx_order, self.nrhs,
+                    _p(strengths), _p(normals), _p(pot), 0)
+            return pot.mul_(1.0 / (4 * np.pi))
         nt = self.tree.ntargets
         grad = pot = None
         if self.output == "grad":
@@ -265,7 +281,6 @@ class QBXExpansionWrangler:
             pot = self.full_output_zeros()
         if op == _lib.TSQBX_D and normals is None:
             raise ValueError("Missing dipvec")
-        kernel = _lib.LAPLACE3D if self.k == 0 else _lib.HELMHOLTZ3D
         self._a("qbx_eval_target_specific_qbx_locals", self.tree.h, self.traversal.h,
                 self.geo.h, kernel, op, self.k.real, self.k.imag, self.qbx_order,
                 _p(strengths), _p(normals), _p(pot), _p(grad))
@@ -317,10 +332,12 @@ def direct_qbx(wrangler: QBXExpansionWrangler, strengths, normals):
 
 
 def drive_fmm_fused(wrangler: QBXExpansionWrangler, strengths, normals, out=None):
-    """Same stage sequence from the single ``qbx_drive_fmm`` entry point."""
+    """Same stage sequence from the single ``qbx_drive_fmm`` entry point
+    (with target-specific QBX: 3D Laplace)."""
     w = wrangler
-    if w.using_tsqbx:
-        raise ValueError("the fused driver has no target-specific QBX stage; use drive_fmm")
+    if w.using_tsqbx and not w.kd.target_specific_qbx:
+        raise ValueError("the fused driver's target-specific QBX stage is 3D Laplace; "
+                         "use drive_fmm")
     nuser = w.tree.ntargets - w.geo_data.ncenters
     if out is None:
         out = torch.empty(w._rhs_shape(nuser), dtype=torch.complex128, device="cuda")

@@ -278,7 +278,7 @@ class QBXLayerPotentialSource:
         strengths, nrm = w.reorder_sources(qd, self.dev["quad_weights"], normals)
         if self._direct:
             pot = _fmm.direct_qbx(w, strengths, nrm)
-        elif fused and not w.using_tsqbx:
+        elif fused and (not w.using_tsqbx or w.kd.target_specific_qbx):
             pot = _fmm.drive_fmm_fused(w, strengths, nrm)
         else:
             pot = _fmm.drive_fmm(w, strengths, nrm)

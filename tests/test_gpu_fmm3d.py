@@ -5,7 +5,9 @@ against the CPU oracle on the same triangulated sphere.
   fused driver) to 1e-12 relative to the array's largest entry;
 * the FMM against direct QBX (error stated);
 * the target-specific pipeline equals the expansion pipeline, the identity the
-  reference asserts at 1e-13 (test/test_target_specific_qbx.py:204-214);
+  reference asserts at 1e-13 (test/test_target_specific_qbx.py:204-214), for
+  one density and, through the many-density sum and the fused driver, for 3
+  and 33 (complex) densities;
 * the sphere's single-layer eigenvalue S Y_3^2 = Y_3^2 / 7 on the surface.
 """
 import numpy as np
@@ -133,6 +135,46 @@ def test_tsqbx_pipeline_equals_expansion_pipeline(case3):
     err = _rel(b, a)
     print(f"[case3 {kind}] TSQBX vs P2QBXL pipeline rel diff = {err:.3e}", flush=True)
     assert err < 1e-12
+
+
+@pytest.mark.parametrize("nrhs,cplx", [(3, False), (33, True)])
+def test_tsqbx_many_densities(case3, nrhs, cplx):
+    """The many-density target-specific sum (one kernel value per pair, all
+    densities on the matrix cores; 33 crosses a 32-density chunk, complex
+    strengths take the imaginary-part products) equals the one-density sum
+    column by column, and the fused driver with it equals the stage-by-stage
+    driver and the expansion pipeline (P2QBXL), all to 1e-12."""
+    from pytential_amd import fmm as F
+    kind, discr, src, geo, ref, sigma, targets, pf, mlw = case3
+    rng = np.random.default_rng(77 + nrhs)
+    dens = rng.normal(size=(discr.ndofs, nrhs))
+    if cplx:
+        dens = dens + 1j * rng.normal(size=dens.shape)
+    lo = src.level_orders("laplace", 0.0, geo.tree())
+
+    def wr(tsq, n):
+        return F.QBXExpansionWrangler(geo, "laplace", 0.0, kind, 3, lo,
+                                      use_target_specific_qbx=tsq, nrhs=n)
+    w = wr(True, nrhs)
+    assert w.kd.target_specific_qbx == 1
+    normals = src.dev["quad_normals"] if kind == "D" else None
+    strengths, nrm = w.reorder_sources(src.quad_density(dens), src.dev["quad_weights"],
+                                       normals)
+    print(f"[case3 {kind}] many-density TSQBX nrhs={nrhs} complex={cplx}", flush=True)
+    many = _np(w.eval_target_specific_qbx_locals(strengths, nrm))
+    w1 = wr(True, 1)
+    for c in range(nrhs):
+        one = _np(w1.eval_target_specific_qbx_locals(strengths[:, c].contiguous(), nrm))
+        err = _rel(many[:, c], one)
+        assert err < REL_TOL, (c, err)
+    print(f"[case3 {kind}] many-density TSQBX = one-density sum per column", flush=True)
+    a = _np(F.drive_fmm_fused(w, strengths, nrm))
+    b = _np(F.drive_fmm(w, strengths, nrm))
+    c = _np(F.drive_fmm_fused(wr(False, nrhs), strengths, nrm))
+    e_ab, e_ac = _rel(a, b), _rel(a, c)
+    print(f"[case3 {kind}] fused TSQBX vs stage-by-stage {e_ab:.2e}, vs P2QBXL pipeline "
+          f"{e_ac:.2e}", flush=True)
+    assert e_ab < REL_TOL and e_ac < REL_TOL
 
 
 def test_sphere_eigenvalue_gpu():
