@@ -189,6 +189,32 @@ int zero_fill(double *p, long ndoubles, hipStream_t st)
     return QBX_OK;
 }
 
+__global__ void k_set_int(int *p, int v)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
+}
+
+// clears *flag when any imaginary part is nonzero (every writer stores 0)
+__global__ void k_imag_any(const double *z, long n, int *flag)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long stride = (long)gridDim.x * blockDim.x;
+    bool nz = false;
+    for (long k = i; k < n; k += stride) nz |= z[2 * k + 1] != 0.0;
+    if (__ballot(nz) != 0 && (threadIdx.x & 63) == 0) *flag = 0;
+}
+
+int strengths_all_real(const double *z, long n, int *flag, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, st, flag, 1);
+    if (n > 0) {
+        const long blocks = std::min<long>(cdiv(n, 256), 4096);
+        hipLaunchKernelGGL(k_imag_any, dim3((unsigned)blocks), dim3(256), 0, st, z, n, flag);
+    }
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
 // one thread per (tree source, right-hand side); density [nsources][nrhs]
 __global__ void k_reorder_sources(int ns, int nr, int dim, const int *user_source_ids,
                                   const double *density, const double *weights,

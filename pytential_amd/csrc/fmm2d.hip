@@ -4064,6 +4064,7 @@ int noalloc_scratch_address(int device, double **out)
 // locals L4, QBX expansions Q and tree-order potentials P.
 struct DriveBufs {
     double *M, *L, *L4, *Q, *P;
+    int *allreal = nullptr;  // 3D Laplace: the all-strengths-real flag
 };
 
 struct DriveSizes {
@@ -4102,6 +4103,14 @@ static int drive_impl(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_t
     // at zero (M2QBXL and L2QBXL add into them) and the list-1 part of the
     // QBX targets is added after QBXL2P
     const bool tsq = d->target_specific_qbx != 0;
+    // 3D Laplace: whether every strength is real (the M2L then forms half
+    // the local coefficients and mirrors the rest)
+    if (b.allreal) QBX_TRY(strengths_all_real(str, (long)T->nsources * nr, b.allreal, st));
+    auto m2l = [&](hipStream_t s) {
+        if (d->kernel == QBX_LAPLACE3D)
+            return f3_multipole_to_local(ctx, s, T, R, d, M, L, b.allreal);
+        return qbx_multipole_to_local(ctx, s, T, R, d, M, L);
+    };
     if (!multistream) {
         QBX_TRY(zero_fill(P, np, st));
         QBX_TRY(qbx_form_multipoles(ctx, st, T, R, d, str, nrm, M));
@@ -4112,7 +4121,7 @@ static int drive_impl(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_t
             QBX_TRY(qbx_form_global_qbx_locals(ctx, st, T, R, G, d, str, nrm, Q, 0));
         QBX_TRY(qbx_form_locals(ctx, st, T, R, d, str, nrm, L4, 0));
         QBX_TRY(coarsen_multipoles_from(ctx, st, T, R, d, M, R->mpole_min_level));
-        QBX_TRY(qbx_multipole_to_local(ctx, st, T, R, d, M, L));
+        QBX_TRY(m2l(st));
         QBX_TRY(qbx_eval_multipoles(ctx, st, T, R, G, d, M, P, 1));
         QBX_TRY(qbx_translate_box_multipoles_to_qbx_local(ctx, st, T, R, G, d, M, Q, 1));
         QBX_TRY(add_into(L, L4, nexp * 2, st));
@@ -4180,7 +4189,7 @@ static int drive_impl(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_t
     QBX_TRY(qbx_form_locals(ctx, s3, T, R, d, str, nrm, L4, 0));
     QBX_HIP(hipEventRecord(eP, s3));
     QBX_TRY(coarsen_multipoles_from(ctx, st, T, R, d, M, R->mpole_min_level));
-    QBX_TRY(qbx_multipole_to_local(ctx, st, T, R, d, M, L));
+    QBX_TRY(m2l(st));
     QBX_HIP(hipEventRecord(eM, st));
     QBX_HIP(hipStreamWaitEvent(s1, eM, 0));
     QBX_HIP(hipStreamWaitEvent(s2, eM, 0));
@@ -4223,8 +4232,12 @@ int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav 
     QBX_TRY(ws_reserve(ctx, S(stream), ctx->ws_local2, z.nm, "list-4 local workspace"));
     QBX_TRY(ws_reserve(ctx, S(stream), ctx->ws_qbx, z.nq, "QBX expansion workspace"));
     QBX_TRY(ws_reserve(ctx, S(stream), ctx->ws_pot, z.np, "potential workspace"));
-    const DriveBufs b{ctx->ws_mpole.p, ctx->ws_local.p, ctx->ws_local2.p, ctx->ws_qbx.p,
+    DriveBufs b{ctx->ws_mpole.p, ctx->ws_local.p, ctx->ws_local2.p, ctx->ws_qbx.p,
                       ctx->ws_pot.p};
+    if (d->kernel == QBX_LAPLACE3D) {
+        QBX_TRY(ws_reserve(ctx, S(stream), ctx->ws_allreal, 1, "strength flag"));
+        b.allreal = ctx->ws_allreal.p;
+    }
     return drive_impl(ctx, stream, T, R, G, d, str, nrm, out_user, z, b, true);
 }
 

@@ -879,11 +879,12 @@ constexpr int M2L3M_T = 2 * 15 + 1;      // table side for p <= 15
 
 __global__ void __launch_bounds__(256)
 k3_m2l_mfma(KD kd, TD t, Flat fl, const int *ttp, const int *starts, const int *lists,
-            const double *mpole, double *local)
+            const double *mpole, double *local, const int *allreal)
 {
     __shared__ double Ts[2 * M2L3M_T * M2L3M_T];
     __shared__ short nof[256], mof[256];
     __shared__ double ihp[32];
+    if (allreal && *allreal) return;  // k3_m2l_real's case
     int lv, i, unused;
     if (!flat_locate(fl, blockIdx.x, lv, i, unused)) return;
     const int e0 = starts[i], e1 = starts[i + 1];
@@ -979,6 +980,162 @@ k3_m2l_mfma(KD kd, TD t, Flat fl, const int *ttp, const int *starts, const int *
                 double *dst = o + 2 * ((long)row * kd.nr + r0 + col);
                 dst[0] += sc * cre[q][j];
                 dst[1] += sc * cim[q][j];
+            }
+        }
+    }
+}
+
+// M2L for many real densities on the FP64 matrix cores.  Real strengths give
+// M_n^{-m} = (-1)^m conj(M_n^m) and L_k^{-l} = (-1)^l conj(L_k^l), so only
+// the rows l >= 0 are formed, from the columns m >= 0: with i = (k, l),
+// j = (n, m) (0 <= l <= k, 0 <= m <= n), A1 = T(n+k, m+l) and A2 =
+// (-1)^m T(n+k, l-m) (0 for m = 0), out_i = sum_j A1 a_j + A2 conj(a_j), a
+// real-linear map
+//   [Re out]   [A1r + A2r   A2i - A1i] [Re a]
+//   [Im out] = [A1i + A2i   A1r - A2r] [Im a]
+// of (P+1)(P+2) rows and columns -- about a third of the real products of
+// the complex (P+1)^2 square of k3_m2l_mfma.  Rows and columns interleave
+// (re, im) of each coefficient (row 2i + pr, column 2j + pk); the A operand
+// is assembled from the entry's unit-offset table in LDS, the B operand is
+// read from the multipoles (scaled by h^-n).  A block per target box of one
+// level, 32 right-hand sides (two 16-column tiles); the K-steps of each
+// entry are dealt round-robin to the four waves, each wave holding every
+// row tile, and the waves' sums are added in LDS at the end.  The epilogue
+// scales by (-1)^k h^(-k-1) as k3_m2l and adds row (k, l) and its mirror
+// (k, -l).  Runs only when *allreal is set (k3_m2l_mfma takes the other case).
+template <int P>
+__global__ void __launch_bounds__(256)
+k3_m2l_real(KD kd, TD t, int lv, int i0, const int *ttp, const int *starts, const int *lists,
+            const double *mpole, double *local, const int *allreal)
+{
+    constexpr int R = (P + 1) * (P + 2) / 2;      // coefficients with m >= 0
+    constexpr int R2 = 2 * R;                     // real rows / columns
+    constexpr int RT = (R2 + 15) / 16;            // row tiles
+    constexpr int KS = (R2 + 3) / 4;              // K-steps
+    constexpr int TNL = (2 * P + 1) * (2 * P + 1);  // table entries IDX(N, M), N <= 2P
+    __shared__ double Ts[2 * TNL];
+    __shared__ double red[RT * 16][17];
+    __shared__ double ihp[P + 2];
+    if (*allreal == 0) return;
+    const int i = i0 + blockIdx.x;
+    const int e0 = starts[i], e1 = starts[i + 1];
+    if (e0 == e1) return;
+    const int Bx = ttp[i];
+    const double ih = 1.0 / kd.h[lv];
+    const int W = kd.W, S1 = 2 * W + 1, tn = kd.tabn;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int col = lane & 15, kr = lane >> 4;
+    const int rc = kd.rc, r0 = kd.r0;
+    if (tid < P + 2) {
+        double sc = 1.0;
+        for (int k = 0; k < tid; ++k) sc *= ih;
+        ihp[tid] = sc;
+    }
+    // this lane's A rows: (k, l, pr) of row rt * 16 + col, packed
+    int rowk[RT], rowl[RT], rowpr[RT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+        const int rho = rt * 16 + col;
+        const int ii = rho >> 1;
+        int k = 0;
+        while ((k + 1) * (k + 2) / 2 <= ii) ++k;
+        rowk[rt] = rho < R2 ? k : -1;
+        rowl[rt] = ii - k * (k + 1) / 2;
+        rowpr[rt] = rho & 1;
+    }
+    const int bx = t.coords[Bx], by = t.coords[(size_t)t.nboxes + Bx],
+              bz = t.coords[2 * (size_t)t.nboxes + Bx];
+    f3_d4 acc[RT][2];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) acc[rt][0] = acc[rt][1] = f3_d4{0, 0, 0, 0};
+    int step = 0;  // K-steps dealt so far (rotates the round-robin start)
+    for (int e = e0; e < e1; ++e) {
+        const int A = lists[e];
+        const int dx = bx - t.coords[A] + W;
+        const int dy = by - t.coords[(size_t)t.nboxes + A] + W;
+        const int dz = bz - t.coords[2 * (size_t)t.nboxes + A] + W;
+        if ((unsigned)dx >= (unsigned)S1 || (unsigned)dy >= (unsigned)S1 ||
+            (unsigned)dz >= (unsigned)S1)
+            continue;  // uniform over the block
+        const double *a = mpole + ecoef_off(kd, A, lv);
+        const double *T = kd.tab_m2l + 2L * ((dx * S1 + dy) * S1 + dz) * tn * tn;
+        __syncthreads();  // previous table consumed (and ihp written)
+        for (int q = tid; q < TNL; q += 256) {
+            Ts[2 * q] = T[2 * q];
+            Ts[2 * q + 1] = T[2 * q + 1];
+        }
+        __syncthreads();
+        for (int ks = (wv - step) & 3; ks < KS; ks += 4) {
+            const int kap = ks * 4 + kr;  // B row / A column
+            const int j = kap >> 1, pk = kap & 1;
+            int n = 0;
+            while ((n + 1) * (n + 2) / 2 <= j) ++n;
+            const int m = j - n * (n + 1) / 2;
+            const bool kin = kap < R2;
+            double b[2] = {0.0, 0.0};
+            if (kin) {
+                const long q = (long)(n * n + n + m) * kd.nr;
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    const int r = c * 16 + col;
+                    if (r < rc) b[c] = ihp[n] * a[2 * (q + r0 + r) + pk];
+                }
+            }
+            const double sm = (m == 0) ? 0.0 : ((m & 1) ? -1.0 : 1.0);
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) {
+                double v = 0.0;
+                if (kin && rowk[rt] >= 0) {
+                    const int N = n + rowk[rt], base = N * N + N, l = rowl[rt];
+                    const int pr = rowpr[rt];
+                    const int i1 = base + m + l, i2 = base + l - m;
+                    const bool same = pr == pk;
+                    const double c1 = same ? Ts[2 * i1] : Ts[2 * i1 + 1];
+                    const double c2 = same ? Ts[2 * i2] : Ts[2 * i2 + 1];
+                    v = c1 + (pk ? -sm : sm) * c2;
+                    if (pr == 0 && pk == 1) v = -v;
+                }
+                acc[rt][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(v, b[0], acc[rt][0], 0, 0, 0);
+                acc[rt][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(v, b[1], acc[rt][1], 0, 0, 0);
+            }
+        }
+        step = (step + KS) & 3;
+    }
+    // the waves' sums, one 16-column tile at a time: D register j of lane i
+    // is row 4 j + i / 16, column i % 16 of its tile
+    double *o = local + ecoef_off(kd, Bx, lv);
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        for (int w = 0; w < 4; ++w) {
+            __syncthreads();
+            if (wv == w) {
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        double &d = red[rt * 16 + 4 * j + kr][col];
+                        d = (w == 0 ? 0.0 : d) + acc[rt][c][j];
+                    }
+            }
+        }
+        __syncthreads();
+        for (int q = tid; q < R * 16; q += 256) {
+            const int ii = q >> 4, cc = q & 15;
+            const int r = c * 16 + cc;
+            if (r >= rc) continue;
+            int k = 0;
+            while ((k + 1) * (k + 2) / 2 <= ii) ++k;
+            const int l = ii - k * (k + 1) / 2;
+            const double sc = ((k & 1) ? -1.0 : 1.0) * ihp[k + 1];
+            const double re = sc * red[2 * ii][cc], im = sc * red[2 * ii + 1][cc];
+            double *dst = o + 2 * ((long)(k * k + k + l) * kd.nr + r0 + r);
+            dst[0] += re;
+            dst[1] += im;
+            if (l > 0) {
+                const double sl = (l & 1) ? -1.0 : 1.0;
+                double *dm = o + 2 * ((long)(k * k + k - l) * kd.nr + r0 + r);
+                dm[0] += sl * re;
+                dm[1] -= sl * im;
             }
         }
     }
@@ -2213,7 +2370,8 @@ int f3_eval_targets(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_tra
 }
 
 int f3_multipole_to_local(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
-                          const qbx_kernel_desc *d, const double *mpole, double *local)
+                          const qbx_kernel_desc *d, const double *mpole, double *local,
+                          const int *allreal)
 {
     KD kd;
     QBX_TRY(make_kd(d, T, &kd, R->nsep));
@@ -2237,7 +2395,36 @@ int f3_multipole_to_local(qbx_ctx *ctx, void *stream, const qbx_tree *T, const q
             kd.r0 = r0;
             kd.rc = std::min(16, kd.nr - r0);
             hipLaunchKernelGGL(k3_m2l_mfma, dim3(nb), dim3(256), 0, st, kd, td, fl,
-                               R->ttp_boxes.p, R->l2_starts.p, R->l2_lists.p, mpole, local);
+                               R->ttp_boxes.p, R->l2_starts.p, R->l2_lists.p, mpole, local,
+                               allreal);
+        }
+        // real strengths: the conjugate-symmetric kernel, level by level
+        // (its tile counts are fixed by the level's order), 32 right-hand
+        // sides per launch; the launches above then return at once
+        if (allreal) {
+            for (int l = 0; l < kd.nlevels; ++l) {
+                const int i0 = R->h_level_start_ttp_box_nrs[l];
+                const int nbl = R->h_level_start_ttp_box_nrs[l + 1] - i0;
+                if (nbl == 0) continue;
+                for (int r0 = 0; r0 < kd.nr; r0 += 32) {
+                    kd.r0 = r0;
+                    kd.rc = std::min(32, kd.nr - r0);
+#define M2LR_CASE(PP)                                                                      \
+    case PP:                                                                               \
+        hipLaunchKernelGGL(k3_m2l_real<PP>, dim3(nbl), dim3(256), 0, st, kd, td, l, i0,    \
+                           R->ttp_boxes.p, R->l2_starts.p, R->l2_lists.p, mpole, local,    \
+                           allreal);                                                       \
+        break;
+                    switch (kd.order[l]) {
+                        M2LR_CASE(0) M2LR_CASE(1) M2LR_CASE(2) M2LR_CASE(3) M2LR_CASE(4)
+                        M2LR_CASE(5) M2LR_CASE(6) M2LR_CASE(7) M2LR_CASE(8) M2LR_CASE(9)
+                        M2LR_CASE(10) M2LR_CASE(11) M2LR_CASE(12) M2LR_CASE(13)
+                        M2LR_CASE(14) M2LR_CASE(15)
+                    default: set_error("M2L: order out of range"); return QBX_ERR_INVALID;
+                    }
+#undef M2LR_CASE
+                }
+            }
         }
     } else if (R->nttp_boxes > 0 && R->n2 > 0) {
         const TD td = make_td(T);

@@ -120,6 +120,9 @@ struct qbx_ctx {
     double *noalloc_scratch = nullptr;
     // 3D Laplace unit-offset harmonic tables and the (nsep, pmax) they hold
     qbx::dbuf<double> ws_tab3_m2l, ws_tab3_ch;
+    // qbx_drive_fmm (3D Laplace): 1 when every strength of the call is real
+    // (the translations then use the conjugate symmetry of the expansions)
+    qbx::dbuf<int> ws_allreal;
     std::vector<double> tab3_key;
     // side streams of qbx_drive_fmm (default priority; shared by every
     // context on the device, see side_streams) and its fork/join events
@@ -269,8 +272,13 @@ int f3_coarsen_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const q
 int f3_eval_targets(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
                     const qbx_geo *G, const qbx_kernel_desc *d, int mode, const double *str,
                     const double *nrm, const double *expn, double *pot, int accumulate);
+// allreal: null, or a device flag that is 1 when every strength behind the
+// multipoles is real (then the conjugate-symmetric matrix-core kernel runs)
 int f3_multipole_to_local(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
-                          const qbx_kernel_desc *d, const double *mpole, double *local);
+                          const qbx_kernel_desc *d, const double *mpole, double *local,
+                          const int *allreal = nullptr);
+// *flag = 1 if every imaginary part of the n complex values z is zero, else 0
+int strengths_all_real(const double *z, long n, int *flag, hipStream_t st);
 int f3_form_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
                    const qbx_kernel_desc *d, const double *str, const double *nrm, double *local,
                    int accumulate);
