@@ -406,10 +406,12 @@ def stage_flops_3d(counts, orders, pq, nrhs):
     # angle cosine (~20), the Legendre recurrence and the power series (~8 per
     # order), then a real MAC per density
     tsqbx = counts["tsqbx_pairs"] * (20 + 8 * (pq + 1) + 2 * nrhs)
-    # M2L: dense (ncf x ncf) complex translation per (box, list-2 box) and density
+    # M2L for real densities: the conjugate-symmetric real-linear map of the
+    # m >= 0 coefficients, (p+1)(p+2) x (p+1)(p+2) real multiply-adds per
+    # (box, list-2 box) and density (the dense complex square is 3.2x that)
     ttc, l2s = counts["ttp"], counts["l2s"]
-    ncf_t = (o[lv[ttc]] + 1) ** 2
-    m2l = int(np.sum(np.diff(l2s) * ncf_t * ncf_t * 8 * nrhs))
+    nr2 = (o[lv[ttc]] + 1) * (o[lv[ttc]] + 2)
+    m2l = int(np.sum(np.diff(l2s) * nr2 * nr2 * 2 * nrhs))
     # M2QBXL: dense (ncq x ncf) complex per (center, list-3 box) and density
     m2q_lv = counts["m2qbxl_levels"]
     c2tb, l3s = counts["c2tb"], counts["l3s"]
@@ -476,6 +478,9 @@ def run_sphere3d(args):
     log(f"[sphere3d] geometry {cold:.3f}s nboxes={tree.nboxes} nlevels={tree.nlevels}")
     w = src.make_wrangler(geo, "laplace", 0.0, "S", nrhs)
     dt, out = timed(stepper(src, w, dens), steps)
+    # the stage-by-stage run below uses the stage entry points, told that the
+    # strengths are real (the fused driver decides that on the device)
+    w.kd.real_strengths = 1
     res = out.real.cpu().numpy()
     # stage by stage on one stream, HIP events between stages: per-stage times
     # and the roofline of the dominant kernel
@@ -522,7 +527,8 @@ def run_sphere3d(args):
     # P2QBXL is the zero fill of the expansions here: the target-specific sum
     # does its work
     flops.pop("p2qbxl")
-    kernel_of = {"tsqbx": "k3_tsqbx_mr", "m2l": "k3_m2l_mfma", "m2qbxl": "k3_m2qbxl_mfma"}
+    kernel_of = {"tsqbx": "k3_tsqbx_mr", "m2l": "k3_trans_real<0,8,2>",
+                 "m2qbxl": "k3_m2qbxl_mfma"}
     dom = max(flops, key=lambda k: per_stage[k])
     ach = flops[dom] / (per_stage[dom] * 1e-3) / 1e12
     roofline = {"bound": "mfma", "compute": "fp64",

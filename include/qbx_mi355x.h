@@ -170,6 +170,13 @@ typedef struct {
      * instead of P2QBXL, as pytential's fmmlib backend does when
      * _use_target_specific_qbx allows it (qbx/fmmlib.py:73-108) */
     int target_specific_qbx;
+    /* 3D Laplace, several densities: 1 = the caller states that every
+     * strength is real, so the box translations (M2M, M2L, L2L) use the
+     * conjugate symmetry of the expansions (half the coefficients, a third
+     * of the products) in the stage entry points too; qbx_drive_fmm finds
+     * this out on the device by itself.  Not a reference field: pytential's
+     * expansions are complex whatever the density. */
+    int real_strengths;
 } qbx_kernel_desc;
 
 /* Box expansions (multipoles and locals) are level-compact: the boxes of

@@ -28,7 +28,8 @@ IP = ct.POINTER(ct.c_int)
 class KernelDesc(ct.Structure):
     _fields_ = [("kernel", I), ("k", D), ("source_kind", I), ("qbx_order", I),
                 ("nlevels", I), ("level_orders", IP), ("nrhs", I),
-                ("target_deriv", I), ("target_dir", P), ("target_specific_qbx", I)]
+                ("target_deriv", I), ("target_dir", P), ("target_specific_qbx", I),
+                ("real_strengths", I)]
 
 
 class QBXError(RuntimeError):

@@ -4111,6 +4111,16 @@ static int drive_impl(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_t
             return f3_multipole_to_local(ctx, s, T, R, d, M, L, b.allreal);
         return qbx_multipole_to_local(ctx, s, T, R, d, M, L);
     };
+    auto m2m = [&](hipStream_t s) {
+        if (d->kernel == QBX_LAPLACE3D)
+            return f3_coarsen_multipoles(ctx, s, T, R, d, M, R->mpole_min_level, b.allreal);
+        return coarsen_multipoles_from(ctx, s, T, R, d, M, R->mpole_min_level);
+    };
+    auto l2l = [&](hipStream_t s) {
+        if (d->kernel == QBX_LAPLACE3D)
+            return f3_refine_locals(ctx, s, T, R, d, L, R->local_min_level, b.allreal);
+        return refine_locals_from(ctx, s, T, R, d, L, R->local_min_level);
+    };
     if (!multistream) {
         QBX_TRY(zero_fill(P, np, st));
         QBX_TRY(qbx_form_multipoles(ctx, st, T, R, d, str, nrm, M));
@@ -4120,12 +4130,12 @@ static int drive_impl(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_t
         else
             QBX_TRY(qbx_form_global_qbx_locals(ctx, st, T, R, G, d, str, nrm, Q, 0));
         QBX_TRY(qbx_form_locals(ctx, st, T, R, d, str, nrm, L4, 0));
-        QBX_TRY(coarsen_multipoles_from(ctx, st, T, R, d, M, R->mpole_min_level));
+        QBX_TRY(m2m(st));
         QBX_TRY(m2l(st));
         QBX_TRY(qbx_eval_multipoles(ctx, st, T, R, G, d, M, P, 1));
         QBX_TRY(qbx_translate_box_multipoles_to_qbx_local(ctx, st, T, R, G, d, M, Q, 1));
         QBX_TRY(add_into(L, L4, nexp * 2, st));
-        QBX_TRY(refine_locals_from(ctx, st, T, R, d, L, R->local_min_level));
+        QBX_TRY(l2l(st));
         QBX_TRY(qbx_eval_locals(ctx, st, T, R, G, d, L, P, 1));
         QBX_TRY(qbx_translate_box_local_to_qbx_local(ctx, st, T, R, G, d, L, Q, 1));
         QBX_TRY(qbx_eval_qbx_expansions(ctx, st, T, G, d, Q, P));
@@ -4188,7 +4198,7 @@ static int drive_impl(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_t
         QBX_TRY(qbx_form_global_qbx_locals(ctx, s2, T, R, G, d, str, nrm, Q, 0));
     QBX_TRY(qbx_form_locals(ctx, s3, T, R, d, str, nrm, L4, 0));
     QBX_HIP(hipEventRecord(eP, s3));
-    QBX_TRY(coarsen_multipoles_from(ctx, st, T, R, d, M, R->mpole_min_level));
+    QBX_TRY(m2m(st));
     QBX_TRY(m2l(st));
     QBX_HIP(hipEventRecord(eM, st));
     QBX_HIP(hipStreamWaitEvent(s1, eM, 0));
@@ -4198,7 +4208,7 @@ static int drive_impl(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_t
     QBX_HIP(hipEventRecord(e2, s2));
     QBX_HIP(hipStreamWaitEvent(st, eP, 0));
     QBX_TRY(add_into(L, L4, nexp * 2, st));
-    QBX_TRY(refine_locals_from(ctx, st, T, R, d, L, R->local_min_level));
+    QBX_TRY(l2l(st));
     QBX_HIP(hipEventRecord(eL, st));
     QBX_HIP(hipStreamWaitEvent(s1, eL, 0));
     QBX_TRY(qbx_eval_locals(ctx, s1, T, R, G, d, L, P, 1));

@@ -693,8 +693,10 @@ __device__ __forceinline__ const double *child_tab(const KD &kd, int ci)
 
 // M2M into parents of level lvp: thread per (parent, coefficient)
 template <int NRC>
-__global__ void k3_m2m(KD kd, TD t, int lvp, int i0, int i1, const int *spb, double *mpole)
+__global__ void k3_m2m(KD kd, TD t, int lvp, int i0, int i1, const int *spb, double *mpole,
+                       const int *allreal)
 {
+    if (allreal && *allreal) return;  // k3_trans_real's case
     const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int ncp = kd.ncl[lvp];
     const long nthr = (long)(i1 - i0) * ncp;
@@ -739,8 +741,10 @@ __global__ void k3_m2m(KD kd, TD t, int lvp, int i0, int i1, const int *spb, dou
 
 // L2L into ttp boxes of level lv: thread per (box, coefficient), accumulating
 template <int NRC>
-__global__ void k3_l2l(KD kd, TD t, int lv, int i0, int i1, const int *ttp, double *local)
+__global__ void k3_l2l(KD kd, TD t, int lv, int i0, int i1, const int *ttp, double *local,
+                       const int *allreal)
 {
+    if (allreal && *allreal) return;  // k3_trans_real's case
     const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int nc = kd.ncl[lv];
     const long nthr = (long)(i1 - i0) * nc;
@@ -884,7 +888,7 @@ k3_m2l_mfma(KD kd, TD t, Flat fl, const int *ttp, const int *starts, const int *
     __shared__ double Ts[2 * M2L3M_T * M2L3M_T];
     __shared__ short nof[256], mof[256];
     __shared__ double ihp[32];
-    if (allreal && *allreal) return;  // k3_m2l_real's case
+    if (allreal && *allreal) return;  // k3_trans_real's case
     int lv, i, unused;
     if (!flat_locate(fl, blockIdx.x, lv, i, unused)) return;
     const int e0 = starts[i], e1 = starts[i + 1];
@@ -985,127 +989,298 @@ k3_m2l_mfma(KD kd, TD t, Flat fl, const int *ttp, const int *starts, const int *
     }
 }
 
-// M2L for many real densities on the FP64 matrix cores.  Real strengths give
-// M_n^{-m} = (-1)^m conj(M_n^m) and L_k^{-l} = (-1)^l conj(L_k^l), so only
-// the rows l >= 0 are formed, from the columns m >= 0: with i = (k, l),
-// j = (n, m) (0 <= l <= k, 0 <= m <= n), A1 = T(n+k, m+l) and A2 =
-// (-1)^m T(n+k, l-m) (0 for m = 0), out_i = sum_j A1 a_j + A2 conj(a_j), a
-// real-linear map
+// Box translations for many real densities on the FP64 matrix cores.  Real
+// strengths give M_n^{-m} = (-1)^m conj(M_n^m) and L_n^{-m} = (-1)^m
+// conj(L_n^m) for every multipole and local, so a translation out = A in
+// forms only the output rows i = (k, l), l >= 0, from the input columns
+// j = (n, m), m >= 0: out_i = sum_j A1 a_j + A2 conj(a_j) with A1 = A(i, j)
+// and A2 = (-1)^m A(i, (n, -m)) (0 for m = 0), a real-linear map
 //   [Re out]   [A1r + A2r   A2i - A1i] [Re a]
 //   [Im out] = [A1i + A2i   A1r - A2r] [Im a]
 // of (P+1)(P+2) rows and columns -- about a third of the real products of
-// the complex (P+1)^2 square of k3_m2l_mfma.  Rows and columns interleave
-// (re, im) of each coefficient (row 2i + pr, column 2j + pk); the A operand
-// is assembled from the entry's unit-offset table in LDS, the B operand is
-// read from the multipoles (scaled by h^-n).  A block per target box of one
-// level, 32 right-hand sides (two 16-column tiles); the K-steps of each
-// entry are dealt round-robin to the four waves, each wave holding every
-// row tile, and the waves' sums are added in LDS at the end.  The epilogue
-// scales by (-1)^k h^(-k-1) as k3_m2l and adds row (k, l) and its mirror
-// (k, -l).  Runs only when *allreal is set (k3_m2l_mfma takes the other case).
-template <int P>
+// the complex (P+1)^2 square.  The operator of one entry, read from a table
+// staged in LDS:
+//   M2L (OPK 0)  A = T_{n+k}^{m+l}(delta), the unit-offset irregular
+//                harmonics of the list-2 offset; B scaled h^-n, output
+//                (-1)^k h^-(k+1); entries: the list-2 boxes
+//   M2M (OPK 1)  A = conj(Y_{k-n}^{l-m}(child offset)) for n <= k (unit
+//                child size); B scaled hc^-n, output hc^k; entries: the
+//                children, output the parent (stored)
+//   L2L (OPK 2)  A = conj(Y_{n-k}^{m-l}(child offset)) for n >= k; B (the
+//                parent's locals) scaled hc^n, output hc^-k; entry: the
+//                parent, output the box (added)
+// (row i = (k, l), column j = (n, m); for M2M and L2L the tiles on the zero
+// side of n = k are skipped).  Rows and columns interleave (re, im) of each
+// coefficient (row 2i + pr, column 2j + pk).  A block per output box of one
+// level and 16 NCT right-hand sides.  Per entry the table and the input
+// box's m >= 0 coefficients (scaled, split into re / im rows) are staged in
+// LDS; the next entry's are loaded into registers while the current one is
+// contracted.  The K-steps of each entry are dealt round-robin to the four
+// waves, each wave holding every row tile; the waves' sums are added in LDS
+// at the end and the epilogue writes row (k, l) and its mirror (k, -l).
+// With allreal given the kernel runs only when *allreal is set (the complex
+// kernels take the other case); without, the caller has stated it.
+constexpr int TR3_ECH = 256;  // entries whose offsets are staged at once
+
+__host__ __device__ constexpr int tri_n(int j)  // n of triangular index j
+{
+    int n = 0;
+    while ((n + 1) * (n + 2) / 2 <= j) ++n;
+    return n;
+}
+
+template <int OPK, int P, int NCT>
 __global__ void __launch_bounds__(256)
-k3_m2l_real(KD kd, TD t, int lv, int i0, const int *ttp, const int *starts, const int *lists,
-            const double *mpole, double *local, const int *allreal)
+k3_trans_real(KD kd, TD t, int lv, int i0, const int *boxes, const int *starts,
+              const int *lists, const double *in, double *out, const int *allreal)
 {
     constexpr int R = (P + 1) * (P + 2) / 2;      // coefficients with m >= 0
     constexpr int R2 = 2 * R;                     // real rows / columns
     constexpr int RT = (R2 + 15) / 16;            // row tiles
     constexpr int KS = (R2 + 3) / 4;              // K-steps
-    constexpr int TNL = (2 * P + 1) * (2 * P + 1);  // table entries IDX(N, M), N <= 2P
-    __shared__ double Ts[2 * TNL];
-    __shared__ double red[RT * 16][17];
-    __shared__ double ihp[P + 2];
-    if (*allreal == 0) return;
+    constexpr int RC = 16 * NCT;                  // right-hand sides per block
+    constexpr int TNL = OPK == 0 ? (2 * P + 1) * (2 * P + 1) : (P + 1) * (P + 1);
+    constexpr int NT = (2 * TNL + 255) / 256;       // table doubles per thread
+    constexpr int NB = (R * RC + 255) / 256;        // (coefficient, rhs) pairs per thread
+    // staging area, reused for the reduction at the end
+    constexpr int LDSN = (2 * TNL + R2 * RC) > RT * 16 * 17 ? (2 * TNL + R2 * RC)
+                                                            : RT * 16 * 17;
+    __shared__ double lds[LDSN];
+    __shared__ double bsc[P + 1], osc[P + 1];
+    __shared__ short jn[R];
+    __shared__ int etab[TR3_ECH];
+    __shared__ long eoff[TR3_ECH];
+    __shared__ int nent;
+    double *Ts = lds, *Bs = lds + 2 * TNL;
+    if (allreal && *allreal == 0) return;
     const int i = i0 + blockIdx.x;
-    const int e0 = starts[i], e1 = starts[i + 1];
-    if (e0 == e1) return;
-    const int Bx = ttp[i];
-    const double ih = 1.0 / kd.h[lv];
+    const int Bx = boxes[i];
+    // the entry range: list 2, the children, or the parent
+    int e0 = 0, e1 = 0;
+    if (OPK == 0) {
+        e0 = starts[i];
+        e1 = starts[i + 1];
+        if (e0 == e1) return;
+    } else if (OPK == 1) {
+        e1 = 8;
+    } else {
+        e1 = 1;
+    }
     const int W = kd.W, S1 = 2 * W + 1, tn = kd.tabn;
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int col = lane & 15, kr = lane >> 4;
-    const int rc = kd.rc, r0 = kd.r0;
-    if (tid < P + 2) {
-        double sc = 1.0;
-        for (int k = 0; k < tid; ++k) sc *= ih;
-        ihp[tid] = sc;
+    const int rc = kd.rc, r0 = kd.r0, nr = kd.nr;
+    // the level of the input boxes and the scalings
+    const int lin = OPK == 0 ? lv : (OPK == 1 ? lv + 1 : lv - 1);
+    if (tid <= P) {
+        const double h = OPK == 0 ? kd.h[lv] : (OPK == 1 ? kd.h[lv + 1] : kd.h[lv]);
+        const double ih = 1.0 / h;
+        double pw = 1.0, ipw = 1.0;
+        for (int k = 0; k < tid; ++k) {
+            pw *= h;
+            ipw *= ih;
+        }
+        if (OPK == 0) {
+            bsc[tid] = ipw;
+            osc[tid] = ((tid & 1) ? -ipw : ipw) * ih;
+        } else if (OPK == 1) {
+            bsc[tid] = ipw;
+            osc[tid] = pw;
+        } else {
+            bsc[tid] = pw;
+            osc[tid] = ipw;
+        }
     }
-    // this lane's A rows: (k, l, pr) of row rt * 16 + col, packed
+    for (int j = tid; j < R; j += 256) jn[j] = (short)tri_n(j);
+    // this lane's A rows: (k, l, pr) of row rt * 16 + col
     int rowk[RT], rowl[RT], rowpr[RT];
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt) {
         const int rho = rt * 16 + col;
         const int ii = rho >> 1;
-        int k = 0;
-        while ((k + 1) * (k + 2) / 2 <= ii) ++k;
+        const int k = tri_n(ii);
         rowk[rt] = rho < R2 ? k : -1;
         rowl[rt] = ii - k * (k + 1) / 2;
         rowpr[rt] = rho & 1;
     }
     const int bx = t.coords[Bx], by = t.coords[(size_t)t.nboxes + Bx],
               bz = t.coords[2 * (size_t)t.nboxes + Bx];
-    f3_d4 acc[RT][2];
+    // the staged data of an entry: table doubles and (re, im) of the B pairs;
+    // the entries' table and coefficient offsets come from LDS (filled per
+    // chunk of TR3_ECH entries), so the loads of the next entry depend on no
+    // global read and overlap the current entry's products
+    double tr[NT], br[NB][2];
+    auto fetch = [&](int ee) -> bool {
+        const int tix = etab[ee];
+        if (tix < 0) return false;  // uniform over the block
+        const double *T = OPK == 0 ? kd.tab_m2l + 2L * tix * tn * tn
+                                   : kd.tab_ch + 2L * tix * (kd.pmax + 1) * (kd.pmax + 1);
 #pragma unroll
-    for (int rt = 0; rt < RT; ++rt) acc[rt][0] = acc[rt][1] = f3_d4{0, 0, 0, 0};
+        for (int u = 0; u < NT; ++u) {
+            const int q = tid + 256 * u;
+            // the child table holds Y: stage conj(Y) (flip the imaginary parts)
+            tr[u] = q < 2 * TNL ? ((OPK != 0 && (q & 1)) ? -T[q] : T[q]) : 0.0;
+        }
+        const double *a = in + eoff[ee];
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const int f = tid + 256 * u;
+            const int j = f / RC, r = f % RC;
+            br[u][0] = br[u][1] = 0.0;
+            if (f < R * RC && r < rc) {
+                const int n = jn[j];
+                const int m = j - n * (n + 1) / 2;
+                const double *z = a + 2 * ((long)(n * n + n + m) * nr + r0 + r);
+                br[u][0] = z[0];
+                br[u][1] = z[1];
+            }
+        }
+        return true;
+    };
+    f3_d4 acc[RT][NCT];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int c = 0; c < NCT; ++c) acc[rt][c] = f3_d4{0, 0, 0, 0};
     int step = 0;  // K-steps dealt so far (rotates the round-robin start)
-    for (int e = e0; e < e1; ++e) {
-        const int A = lists[e];
-        const int dx = bx - t.coords[A] + W;
-        const int dy = by - t.coords[(size_t)t.nboxes + A] + W;
-        const int dz = bz - t.coords[2 * (size_t)t.nboxes + A] + W;
-        if ((unsigned)dx >= (unsigned)S1 || (unsigned)dy >= (unsigned)S1 ||
-            (unsigned)dz >= (unsigned)S1)
-            continue;  // uniform over the block
-        const double *a = mpole + ecoef_off(kd, A, lv);
-        const double *T = kd.tab_m2l + 2L * ((dx * S1 + dy) * S1 + dz) * tn * tn;
-        __syncthreads();  // previous table consumed (and ihp written)
-        for (int q = tid; q < TNL; q += 256) {
-            Ts[2 * q] = T[2 * q];
-            Ts[2 * q + 1] = T[2 * q + 1];
+    for (int cb = e0; cb < e1; cb += TR3_ECH) {
+        __syncthreads();  // the previous chunk's entries consumed (bsc, jn written)
+        if (OPK == 0) {
+            const int cn = min(TR3_ECH, e1 - cb);
+            for (int q = tid; q < cn; q += 256) {
+                const int A = lists[cb + q];
+                const int dx = bx - t.coords[A] + W;
+                const int dy = by - t.coords[(size_t)t.nboxes + A] + W;
+                const int dz = bz - t.coords[2 * (size_t)t.nboxes + A] + W;
+                const bool ok = (unsigned)dx < (unsigned)S1 && (unsigned)dy < (unsigned)S1 &&
+                                (unsigned)dz < (unsigned)S1;
+                etab[q] = ok ? (dx * S1 + dy) * S1 + dz : -1;
+                eoff[q] = ecoef_off(kd, A, lin);
+            }
+            if (tid == 0) nent = cn;
+        } else if (OPK == 1) {
+            // the nonempty children, octant ci = child slot ci
+            if (tid == 0) {
+                int c = 0;
+                for (int ci = 0; ci < 8; ++ci) {
+                    const int C = t.child[(size_t)ci * t.nboxes + Bx];
+                    if (C < 0 || t.scumul[C] == 0) continue;
+                    etab[c] = ci;
+                    eoff[c] = ecoef_off(kd, C, lin);
+                    ++c;
+                }
+                nent = c;
+            }
+        } else {
+            if (tid == 0) {
+                etab[0] = (bx & 1) | ((by & 1) << 1) | ((bz & 1) << 2);
+                eoff[0] = ecoef_off(kd, t.parent[Bx], lin);
+                nent = 1;
+            }
         }
         __syncthreads();
-        for (int ks = (wv - step) & 3; ks < KS; ks += 4) {
-            const int kap = ks * 4 + kr;  // B row / A column
-            const int j = kap >> 1, pk = kap & 1;
-            int n = 0;
-            while ((n + 1) * (n + 2) / 2 <= j) ++n;
-            const int m = j - n * (n + 1) / 2;
-            const bool kin = kap < R2;
-            double b[2] = {0.0, 0.0};
-            if (kin) {
-                const long q = (long)(n * n + n + m) * kd.nr;
+        const int cn = nent;
+        bool have = cn > 0 ? fetch(0) : false;
+        for (int ee = 0; ee < cn; ++ee) {
+            __syncthreads();  // the previous entry's staging consumed
+            if (have) {
 #pragma unroll
-                for (int c = 0; c < 2; ++c) {
-                    const int r = c * 16 + col;
-                    if (r < rc) b[c] = ihp[n] * a[2 * (q + r0 + r) + pk];
+                for (int u = 0; u < NT; ++u) {
+                    const int q = tid + 256 * u;
+                    if (q < 2 * TNL) Ts[q] = tr[u];
+                }
+#pragma unroll
+                for (int u = 0; u < NB; ++u) {
+                    const int f = tid + 256 * u;
+                    if (f < R * RC) {
+                        const int j = f / RC, r = f % RC;
+                        const double sc = bsc[jn[j]];
+                        Bs[(2 * j) * RC + r] = sc * br[u][0];
+                        Bs[(2 * j + 1) * RC + r] = sc * br[u][1];
+                    }
                 }
             }
-            const double sm = (m == 0) ? 0.0 : ((m & 1) ? -1.0 : 1.0);
+            const bool cur = have;
+            __syncthreads();
+            have = ee + 1 < cn ? fetch(ee + 1) : false;
+            if (!cur) continue;
+            for (int ks = (wv - step) & 3; ks < KS; ks += 4) {
+                const int kap = ks * 4 + kr;  // B row / A column
+                const int j = kap >> 1, pk = kap & 1;
+                const bool kin = kap < R2;
+                int n = 0, m = 0;
+                double b[NCT];
 #pragma unroll
-            for (int rt = 0; rt < RT; ++rt) {
-                double v = 0.0;
-                if (kin && rowk[rt] >= 0) {
-                    const int N = n + rowk[rt], base = N * N + N, l = rowl[rt];
-                    const int pr = rowpr[rt];
-                    const int i1 = base + m + l, i2 = base + l - m;
-                    const bool same = pr == pk;
-                    const double c1 = same ? Ts[2 * i1] : Ts[2 * i1 + 1];
-                    const double c2 = same ? Ts[2 * i2] : Ts[2 * i2 + 1];
-                    v = c1 + (pk ? -sm : sm) * c2;
-                    if (pr == 0 && pk == 1) v = -v;
+                for (int c = 0; c < NCT; ++c) b[c] = 0.0;
+                if (kin) {
+                    n = jn[j];
+                    m = j - n * (n + 1) / 2;
+#pragma unroll
+                    for (int c = 0; c < NCT; ++c) b[c] = Bs[kap * RC + c * 16 + col];
                 }
-                acc[rt][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(v, b[0], acc[rt][0], 0, 0, 0);
-                acc[rt][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(v, b[1], acc[rt][1], 0, 0, 0);
+                const double sm = (m == 0) ? 0.0 : ((m & 1) ? -1.0 : 1.0);
+                // the K-step's column orders (uniform): M2M and L2L skip the
+                // row tiles on the zero side of n = k
+                const int nlo = tri_n(min(2 * ks, R - 1)), nhi = tri_n(min(2 * ks + 1, R - 1));
+                // branch-free gather: every row tile's two table reads are
+                // issued together (invalid entries read 0 and are masked),
+                // then the matrix products follow
+                double c1[RT], c2[RT];
+                bool v1[RT], v2[RT];
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt) {
+                    const int k = rowk[rt], l = rowl[rt];
+                    const int off = rowpr[rt] == pk ? 0 : 1;
+                    int i1, i2;
+                    bool ok1, ok2;
+                    if (OPK == 0) {
+                        const int N = n + k, base = N * N + N;
+                        i1 = base + m + l;
+                        i2 = base + l - m;
+                        ok1 = ok2 = kin && k >= 0;
+                    } else if (OPK == 1) {
+                        const int K = k - n, base = K * K + K;
+                        i1 = base + l - m;
+                        i2 = base + l + m;
+                        ok1 = kin && k >= 0 && K >= 0 && l - m <= K && m - l <= K;
+                        ok2 = kin && k >= 0 && K >= 0 && l + m <= K;
+                    } else {
+                        const int K = n - k, base = K * K + K;
+                        i1 = base + m - l;
+                        i2 = base - m - l;
+                        ok1 = kin && k >= 0 && K >= 0 && m - l <= K && l - m <= K;
+                        ok2 = kin && k >= 0 && K >= 0 && m + l <= K;
+                    }
+                    v1[rt] = ok1;
+                    v2[rt] = ok2;
+                    c1[rt] = Ts[ok1 ? 2 * i1 + off : 0];
+                    c2[rt] = Ts[ok2 ? 2 * i2 + off : 0];
+                }
+                const double s2 = pk ? -sm : sm;
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt) {
+                    // row orders of the tile (compile-time)
+                    const int rklo = tri_n(min(8 * rt, R - 1)), rkhi = tri_n(min(8 * rt + 7, R - 1));
+                    const bool live = OPK == 0 || (OPK == 1 ? nlo <= rkhi : nhi >= rklo);
+                    double v = (v1[rt] ? c1[rt] : 0.0) + s2 * (v2[rt] ? c2[rt] : 0.0);
+                    v = (rowpr[rt] == 0 && pk == 1) ? -v : v;
+                    if (live) {
+#pragma unroll
+                        for (int c = 0; c < NCT; ++c)
+                            acc[rt][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(v, b[c], acc[rt][c],
+                                                                               0, 0, 0);
+                    }
+                }
             }
+            step = (step + KS) & 3;
         }
-        step = (step + KS) & 3;
     }
-    // the waves' sums, one 16-column tile at a time: D register j of lane i
-    // is row 4 j + i / 16, column i % 16 of its tile
-    double *o = local + ecoef_off(kd, Bx, lv);
+    // the waves' sums, one 16-column tile at a time (in the staging area):
+    // D register j of lane i is row 4 j + i / 16, column i % 16 of its tile
+    double(*red)[17] = reinterpret_cast<double(*)[17]>(lds);
+    double *o = out + ecoef_off(kd, Bx, lv);
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < NCT; ++c) {
         for (int w = 0; w < 4; ++w) {
             __syncthreads();
             if (wv == w) {
@@ -1123,19 +1298,27 @@ k3_m2l_real(KD kd, TD t, int lv, int i0, const int *ttp, const int *starts, cons
             const int ii = q >> 4, cc = q & 15;
             const int r = c * 16 + cc;
             if (r >= rc) continue;
-            int k = 0;
-            while ((k + 1) * (k + 2) / 2 <= ii) ++k;
+            const int k = jn[ii];
             const int l = ii - k * (k + 1) / 2;
-            const double sc = ((k & 1) ? -1.0 : 1.0) * ihp[k + 1];
+            const double sc = osc[k];
             const double re = sc * red[2 * ii][cc], im = sc * red[2 * ii + 1][cc];
-            double *dst = o + 2 * ((long)(k * k + k + l) * kd.nr + r0 + r);
-            dst[0] += re;
-            dst[1] += im;
-            if (l > 0) {
-                const double sl = (l & 1) ? -1.0 : 1.0;
-                double *dm = o + 2 * ((long)(k * k + k - l) * kd.nr + r0 + r);
-                dm[0] += sl * re;
-                dm[1] -= sl * im;
+            double *dst = o + 2 * ((long)(k * k + k + l) * nr + r0 + r);
+            double *dm = o + 2 * ((long)(k * k + k - l) * nr + r0 + r);
+            const double sl = (l & 1) ? -1.0 : 1.0;
+            if (OPK == 1) {
+                dst[0] = re;
+                dst[1] = im;
+                if (l > 0) {
+                    dm[0] = sl * re;
+                    dm[1] = -sl * im;
+                }
+            } else {
+                dst[0] += re;
+                dst[1] += im;
+                if (l > 0) {
+                    dm[0] += sl * re;
+                    dm[1] -= sl * im;
+                }
             }
         }
     }
@@ -2311,8 +2494,53 @@ int f3_form_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_
     return QBX_OK;
 }
 
+// Launch k3_trans_real<OPK, p> over nb output boxes of level l (orders p <= 15;
+// 32 right-hand sides per block up to order 9, 16 above).
+template <int OPK>
+static int launch_trans_real(hipStream_t st, KD kd, const TD &td, int l, int p, int i0,
+                             int nb, const int *boxes, const int *starts, const int *lists,
+                             const double *in, double *out, const int *allreal)
+{
+    const int rcb = p <= 9 ? 32 : 16;
+    for (int r0 = 0; r0 < kd.nr; r0 += rcb) {
+        kd.r0 = r0;
+        kd.rc = std::min(rcb, kd.nr - r0);
+#define TR_CASE(PP)                                                                        \
+    case PP:                                                                               \
+        hipLaunchKernelGGL((k3_trans_real<OPK, PP, (PP <= 9 ? 2 : 1)>), dim3(nb), dim3(256), \
+                           0, st, kd, td, l, i0, boxes, starts, lists, in, out, allreal);  \
+        break;
+        switch (p) {
+            TR_CASE(1) TR_CASE(2) TR_CASE(3) TR_CASE(4) TR_CASE(5) TR_CASE(6) TR_CASE(7)
+            TR_CASE(8) TR_CASE(9) TR_CASE(10) TR_CASE(11) TR_CASE(12) TR_CASE(13)
+            TR_CASE(14) TR_CASE(15)
+        default: set_error("real-strength translation: order %d out of range", p);
+            return QBX_ERR_INVALID;
+        }
+#undef TR_CASE
+    }
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
+// Which kernels run a box translation: the real-strength matrix-core kernel
+// (stated by the caller, or decided on the device by *flag), the complex one,
+// or both, each returning at once in the other's case.
+struct RealSel {
+    bool real, complex_;
+    const int *flag;  // passed to both kernels (null: no device decision)
+};
+
+static RealSel real_sel(const qbx_kernel_desc *d, const KD &kd, const int *flag, bool fits)
+{
+    const bool ok = fits && kd.nr >= 4;
+    if (ok && d->real_strengths) return {true, false, nullptr};
+    if (ok && flag) return {true, true, flag};
+    return {false, true, nullptr};
+}
+
 int f3_coarsen_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
-                          const qbx_kernel_desc *d, double *mpole, int lmin)
+                          const qbx_kernel_desc *d, double *mpole, int lmin, const int *allreal)
 {
     KD kd;
     QBX_TRY(make_kd(d, T, &kd, R->nsep));
@@ -2324,11 +2552,20 @@ int f3_coarsen_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const q
         const int i1 = R->h_level_start_source_parent_box_nrs[l + 1];
         const long nthr = (long)(i1 - i0) * kd.ncl[l];
         if (nthr > 0) {
+            // the real-strength kernel where parent and child orders agree
+            const int p = kd.order[l];
+            const RealSel rs = real_sel(d, kd, allreal, p <= 15 && kd.order[l + 1] == p);
+            if (rs.real)
+                QBX_TRY(launch_trans_real<1>(st, kd, td, l, p, i0, i1 - i0,
+                                             R->source_parent_boxes.p, nullptr, nullptr, mpole,
+                                             mpole, rs.flag));
+            if (rs.complex_) {
 #define L3_M2M(N)                                                                          \
     hipLaunchKernelGGL(k3_m2m<N>, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, td, l, i0,  \
-                       i1, R->source_parent_boxes.p, mpole)
-            FOR_RHS_CHUNKS3(kd, NRC_S, L3_M2M);
+                       i1, R->source_parent_boxes.p, mpole, rs.flag)
+                FOR_RHS_CHUNKS3(kd, NRC_S, L3_M2M);
 #undef L3_M2M
+            }
         }
     }
     QBX_HIP(hipGetLastError());
@@ -2378,52 +2615,39 @@ int f3_multipole_to_local(qbx_ctx *ctx, void *stream, const qbx_tree *T, const q
     QBX_TRY(ensure_tables(ctx, S(stream), &kd));
     hipStream_t st = S(stream);
     QBX_TRY(zero_fill(local, (exp_bytes(kd)) / 8, st));
+    const RealSel rs = real_sel(d, kd, allreal, kd.pmax <= 15);
     if (R->nttp_boxes > 0 && R->n2 > 0 && kd.nr >= 4 && kd.pmax <= 15) {
-        // many right-hand sides: on the matrix cores, 16 per launch
+        // many right-hand sides: on the matrix cores, 16 per launch (complex)
         const TD td = make_td(T);
-        Flat fl;
-        fl.nl = kd.nlevels;
-        long nb = 0;
-        for (int l = 0; l < kd.nlevels; ++l) {
-            fl.tstart[l] = nb;
-            fl.istart[l] = R->h_level_start_ttp_box_nrs[l];
-            fl.nper[l] = 1;
-            nb += R->h_level_start_ttp_box_nrs[l + 1] - R->h_level_start_ttp_box_nrs[l];
+        if (rs.complex_) {
+            Flat fl;
+            fl.nl = kd.nlevels;
+            long nb = 0;
+            for (int l = 0; l < kd.nlevels; ++l) {
+                fl.tstart[l] = nb;
+                fl.istart[l] = R->h_level_start_ttp_box_nrs[l];
+                fl.nper[l] = 1;
+                nb += R->h_level_start_ttp_box_nrs[l + 1] - R->h_level_start_ttp_box_nrs[l];
+            }
+            fl.tstart[kd.nlevels] = nb;
+            for (int r0 = 0; r0 < kd.nr; r0 += 16) {
+                kd.r0 = r0;
+                kd.rc = std::min(16, kd.nr - r0);
+                hipLaunchKernelGGL(k3_m2l_mfma, dim3(nb), dim3(256), 0, st, kd, td, fl,
+                                   R->ttp_boxes.p, R->l2_starts.p, R->l2_lists.p, mpole, local,
+                                   rs.flag);
+            }
         }
-        fl.tstart[kd.nlevels] = nb;
-        for (int r0 = 0; r0 < kd.nr; r0 += 16) {
-            kd.r0 = r0;
-            kd.rc = std::min(16, kd.nr - r0);
-            hipLaunchKernelGGL(k3_m2l_mfma, dim3(nb), dim3(256), 0, st, kd, td, fl,
-                               R->ttp_boxes.p, R->l2_starts.p, R->l2_lists.p, mpole, local,
-                               allreal);
-        }
-        // real strengths: the conjugate-symmetric kernel, level by level
-        // (its tile counts are fixed by the level's order), 32 right-hand
-        // sides per launch; the launches above then return at once
-        if (allreal) {
+        // real strengths: the conjugate-symmetric kernel, level by level (its
+        // tile counts are fixed by the level's order)
+        if (rs.real) {
             for (int l = 0; l < kd.nlevels; ++l) {
                 const int i0 = R->h_level_start_ttp_box_nrs[l];
                 const int nbl = R->h_level_start_ttp_box_nrs[l + 1] - i0;
                 if (nbl == 0) continue;
-                for (int r0 = 0; r0 < kd.nr; r0 += 32) {
-                    kd.r0 = r0;
-                    kd.rc = std::min(32, kd.nr - r0);
-#define M2LR_CASE(PP)                                                                      \
-    case PP:                                                                               \
-        hipLaunchKernelGGL(k3_m2l_real<PP>, dim3(nbl), dim3(256), 0, st, kd, td, l, i0,    \
-                           R->ttp_boxes.p, R->l2_starts.p, R->l2_lists.p, mpole, local,    \
-                           allreal);                                                       \
-        break;
-                    switch (kd.order[l]) {
-                        M2LR_CASE(0) M2LR_CASE(1) M2LR_CASE(2) M2LR_CASE(3) M2LR_CASE(4)
-                        M2LR_CASE(5) M2LR_CASE(6) M2LR_CASE(7) M2LR_CASE(8) M2LR_CASE(9)
-                        M2LR_CASE(10) M2LR_CASE(11) M2LR_CASE(12) M2LR_CASE(13)
-                        M2LR_CASE(14) M2LR_CASE(15)
-                    default: set_error("M2L: order out of range"); return QBX_ERR_INVALID;
-                    }
-#undef M2LR_CASE
-                }
+                QBX_TRY(launch_trans_real<0>(st, kd, td, l, kd.order[l], i0, nbl,
+                                             R->ttp_boxes.p, R->l2_starts.p, R->l2_lists.p,
+                                             mpole, local, rs.flag));
             }
         }
     } else if (R->nttp_boxes > 0 && R->n2 > 0) {
@@ -2496,7 +2720,7 @@ int f3_form_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav
 }
 
 int f3_refine_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
-                     const qbx_kernel_desc *d, double *local, int lmin)
+                     const qbx_kernel_desc *d, double *local, int lmin, const int *allreal)
 {
     KD kd;
     QBX_TRY(make_kd(d, T, &kd, R->nsep));
@@ -2507,11 +2731,18 @@ int f3_refine_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_tr
         const int i0 = R->h_level_start_ttp_box_nrs[l], i1 = R->h_level_start_ttp_box_nrs[l + 1];
         const long nthr = (long)(i1 - i0) * kd.ncl[l];
         if (nthr > 0) {
+            const int p = kd.order[l];
+            const RealSel rs = real_sel(d, kd, allreal, p <= 15 && kd.order[l - 1] == p);
+            if (rs.real)
+                QBX_TRY(launch_trans_real<2>(st, kd, td, l, p, i0, i1 - i0, R->ttp_boxes.p,
+                                             nullptr, nullptr, local, local, rs.flag));
+            if (rs.complex_) {
 #define L3_L2L(N)                                                                          \
     hipLaunchKernelGGL(k3_l2l<N>, dim3(cdiv(nthr, 128)), dim3(128), 0, st, kd, td, l, i0,  \
-                       i1, R->ttp_boxes.p, local)
-            FOR_RHS_CHUNKS3(kd, NRC_S, L3_L2L);
+                       i1, R->ttp_boxes.p, local, rs.flag)
+                FOR_RHS_CHUNKS3(kd, NRC_S, L3_L2L);
 #undef L3_L2L
+            }
         }
     }
     QBX_HIP(hipGetLastError());

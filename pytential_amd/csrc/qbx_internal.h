@@ -268,7 +268,8 @@ int f3_form_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_
                        double *mpole);
 // M2M into parent levels >= lmin, L2L from parent levels >= lmin (0: all)
 int f3_coarsen_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
-                          const qbx_kernel_desc *d, double *mpole, int lmin = 0);
+                          const qbx_kernel_desc *d, double *mpole, int lmin = 0,
+                          const int *allreal = nullptr);
 int f3_eval_targets(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
                     const qbx_geo *G, const qbx_kernel_desc *d, int mode, const double *str,
                     const double *nrm, const double *expn, double *pot, int accumulate);
@@ -283,7 +284,8 @@ int f3_form_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav
                    const qbx_kernel_desc *d, const double *str, const double *nrm, double *local,
                    int accumulate);
 int f3_refine_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
-                     const qbx_kernel_desc *d, double *local, int lmin = 0);
+                     const qbx_kernel_desc *d, double *local, int lmin = 0,
+                     const int *allreal = nullptr);
 int f3_form_global_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T,
                               const qbx_trav *R, const qbx_geo *G, const qbx_kernel_desc *d,
                               const double *str, const double *nrm, double *qexp, int accumulate);

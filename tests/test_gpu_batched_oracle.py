@@ -7,7 +7,9 @@ multipoles after P2M+M2M, locals after M2L+P2L+L2L, QBX expansions after
 P2QBXL+M2QBXL+L2QBXL, and the potentials, each to 1e-12 relative to the
 array's largest entry.  Right-hand-side counts leave partial chunks (20 =
 16 + 4 in 2D, 37 = 32 + 5 in 3D), and the compared columns include the first,
-one inside the last chunk and the last."""
+one inside the last chunk and the last.  The 3D cases run a second time with
+the stage entry points told that the strengths are real (k3_trans_real for
+M2M, M2L and L2L: half the coefficients, mirrored), orders 8, 12 and 15."""
 import numpy as np
 import pytest
 
@@ -29,10 +31,13 @@ def _rel(a, b):
 
 
 def _check_columns(name, src, geo, ref, kernel, k, kind, dens, cols, strengths_user_fn,
-                   qbx_order):
+                   qbx_order, real_strengths=False):
     from pytential_amd import fmm as F
     nrhs = dens.shape[1]
     w = src.make_wrangler(geo, kernel, k, kind, nrhs)
+    # the stage entry points told that the strengths are real (the
+    # conjugate-symmetric translation kernels; the fused driver finds it out)
+    w.kd.real_strengths = int(real_strengths)
     lo = list(w._lo)
     qd = src.quad_density(dens)
     normals = src.dev["quad_normals"] if kind == "D" else None
@@ -92,8 +97,11 @@ def test_batched_2d_stages_match_oracle(name, nrhs):
                    cfg.discr.strengths, cfg.qbx_order)
 
 
-@pytest.mark.parametrize("kind,pf,nrhs", [("S", 8, 37), ("D", 8, 12), ("S", 12, 16)])
-def test_batched_3d_stages_match_oracle(kind, pf, nrhs):
+@pytest.mark.parametrize("kind,pf,nrhs,real", [("S", 8, 37, False), ("D", 8, 12, False),
+                                               ("S", 12, 16, False), ("S", 8, 37, True),
+                                               ("D", 8, 12, True), ("S", 12, 16, True),
+                                               ("S", 15, 5, True)])
+def test_batched_3d_stages_match_oracle(kind, pf, nrhs, real):
     import oracle
 
     from pytential_amd.qbx import QBXLayerPotentialSource
@@ -115,5 +123,5 @@ def test_batched_3d_stages_match_oracle(kind, pf, nrhs):
     print(f"\n[batched-oracle] 3D {kind} p={pf}: ndofs={discr.ndofs} "
           f"nboxes={geo.tree().nboxes}", flush=True)
     cols = sorted({0, min(33, nrhs - 2), nrhs - 1})
-    _check_columns(f"laplace3d-{kind}-p{pf}", src, geo, ref, "laplace", 0.0, kind, dens,
-                   cols, discr.strengths, 3)
+    _check_columns(f"laplace3d-{kind}-p{pf}{'-real' if real else ''}", src, geo, ref,
+                   "laplace", 0.0, kind, dens, cols, discr.strengths, 3, real_strengths=real)
