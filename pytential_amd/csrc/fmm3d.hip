@@ -1038,6 +1038,10 @@ k3_trans_real(KD kd, TD t, int lv, int i0, const int *boxes, const int *starts,
     constexpr int R2 = 2 * R;                     // real rows / columns
     constexpr int RT = (R2 + 15) / 16;            // row tiles
     constexpr int KS = (R2 + 3) / 4;              // K-steps
+    // high orders split the row tiles over RG groups of waves (fewer
+    // accumulators per wave) and the K-steps over the KG waves of a group
+    constexpr int RG = (OPK == 0 && P >= 10) ? 2 : 1, KG = 4 / RG;
+    constexpr int RTW = (RT + RG - 1) / RG;       // row tiles per wave
     constexpr int RC = 16 * NCT;                  // right-hand sides per block
     constexpr int TNL = OPK == 0 ? (2 * P + 1) * (2 * P + 1) : (P + 1) * (P + 1);
     constexpr int NT = (2 * TNL + 255) / 256;       // table doubles per thread
@@ -1069,6 +1073,7 @@ k3_trans_real(KD kd, TD t, int lv, int i0, const int *boxes, const int *starts,
     const int W = kd.W, S1 = 2 * W + 1, tn = kd.tabn;
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int col = lane & 15, kr = lane >> 4;
+    const int rg = wv / KG, kg = wv % KG;
     const int rc = kd.rc, r0 = kd.r0, nr = kd.nr;
     // the level of the input boxes and the scalings
     const int lin = OPK == 0 ? lv : (OPK == 1 ? lv + 1 : lv - 1);
@@ -1092,11 +1097,11 @@ k3_trans_real(KD kd, TD t, int lv, int i0, const int *boxes, const int *starts,
         }
     }
     for (int j = tid; j < R; j += 256) jn[j] = (short)tri_n(j);
-    // this lane's A rows: (k, l, pr) of row rt * 16 + col
-    int rowk[RT], rowl[RT], rowpr[RT];
+    // this lane's A rows: (k, l, pr) of row (rg RTW + rt) 16 + col
+    int rowk[RTW], rowl[RTW], rowpr[RTW];
 #pragma unroll
-    for (int rt = 0; rt < RT; ++rt) {
-        const int rho = rt * 16 + col;
+    for (int rt = 0; rt < RTW; ++rt) {
+        const int rho = (rg * RTW + rt) * 16 + col;
         const int ii = rho >> 1;
         const int k = tri_n(ii);
         rowk[rt] = rho < R2 ? k : -1;
@@ -1137,9 +1142,9 @@ k3_trans_real(KD kd, TD t, int lv, int i0, const int *boxes, const int *starts,
         }
         return true;
     };
-    f3_d4 acc[RT][NCT];
+    f3_d4 acc[RTW][NCT];
 #pragma unroll
-    for (int rt = 0; rt < RT; ++rt)
+    for (int rt = 0; rt < RTW; ++rt)
 #pragma unroll
         for (int c = 0; c < NCT; ++c) acc[rt][c] = f3_d4{0, 0, 0, 0};
     int step = 0;  // K-steps dealt so far (rotates the round-robin start)
@@ -1204,7 +1209,7 @@ k3_trans_real(KD kd, TD t, int lv, int i0, const int *boxes, const int *starts,
             __syncthreads();
             have = ee + 1 < cn ? fetch(ee + 1) : false;
             if (!cur) continue;
-            for (int ks = (wv - step) & 3; ks < KS; ks += 4) {
+            for (int ks = (kg - step + KG * 4) % KG; ks < KS; ks += KG) {
                 const int kap = ks * 4 + kr;  // B row / A column
                 const int j = kap >> 1, pk = kap & 1;
                 const bool kin = kap < R2;
@@ -1225,10 +1230,10 @@ k3_trans_real(KD kd, TD t, int lv, int i0, const int *boxes, const int *starts,
                 // branch-free gather: every row tile's two table reads are
                 // issued together (invalid entries read 0 and are masked),
                 // then the matrix products follow
-                double c1[RT], c2[RT];
-                bool v1[RT], v2[RT];
+                double c1[RTW], c2[RTW];
+                bool v1[RTW], v2[RTW];
 #pragma unroll
-                for (int rt = 0; rt < RT; ++rt) {
+                for (int rt = 0; rt < RTW; ++rt) {
                     const int k = rowk[rt], l = rowl[rt];
                     const int off = rowpr[rt] == pk ? 0 : 1;
                     int i1, i2;
@@ -1258,10 +1263,13 @@ k3_trans_real(KD kd, TD t, int lv, int i0, const int *boxes, const int *starts,
                 }
                 const double s2 = pk ? -sm : sm;
 #pragma unroll
-                for (int rt = 0; rt < RT; ++rt) {
-                    // row orders of the tile (compile-time)
-                    const int rklo = tri_n(min(8 * rt, R - 1)), rkhi = tri_n(min(8 * rt + 7, R - 1));
-                    const bool live = OPK == 0 || (OPK == 1 ? nlo <= rkhi : nhi >= rklo);
+                for (int rt = 0; rt < RTW; ++rt) {
+                    // row orders of the tile
+                    const int rtg = rg * RTW + rt;
+                    const int rklo = tri_n(min(8 * rtg, R - 1)),
+                              rkhi = tri_n(min(8 * rtg + 7, R - 1));
+                    const bool live =
+                        rtg < RT && (OPK == 0 || (OPK == 1 ? nlo <= rkhi : nhi >= rklo));
                     double v = (v1[rt] ? c1[rt] : 0.0) + s2 * (v2[rt] ? c2[rt] : 0.0);
                     v = (rowpr[rt] == 0 && pk == 1) ? -v : v;
                     if (live) {
@@ -1272,25 +1280,30 @@ k3_trans_real(KD kd, TD t, int lv, int i0, const int *boxes, const int *starts,
                     }
                 }
             }
-            step = (step + KS) & 3;
+            step = (step + KS) % KG;
         }
     }
-    // the waves' sums, one 16-column tile at a time (in the staging area):
-    // D register j of lane i is row 4 j + i / 16, column i % 16 of its tile
+    // the sums of a row group's waves, one 16-column tile at a time (in the
+    // staging area): D register j of lane i is row 4 j + i / 16, column
+    // i % 16 of its tile
     double(*red)[17] = reinterpret_cast<double(*)[17]>(lds);
     double *o = out + ecoef_off(kd, Bx, lv);
 #pragma unroll
     for (int c = 0; c < NCT; ++c) {
-        for (int w = 0; w < 4; ++w) {
+        for (int w = 0; w < KG; ++w) {
             __syncthreads();
-            if (wv == w) {
+            if (kg == w) {
 #pragma unroll
-                for (int rt = 0; rt < RT; ++rt)
+                for (int rt = 0; rt < RTW; ++rt) {
+                    const int rtg = rg * RTW + rt;
+                    if (rtg < RT) {
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        double &d = red[rt * 16 + 4 * j + kr][col];
-                        d = (w == 0 ? 0.0 : d) + acc[rt][c][j];
+                        for (int j = 0; j < 4; ++j) {
+                            double &d = red[rtg * 16 + 4 * j + kr][col];
+                            d = (w == 0 ? 0.0 : d) + acc[rt][c][j];
+                        }
                     }
+                }
             }
         }
         __syncthreads();
@@ -2495,20 +2508,21 @@ int f3_form_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_
 }
 
 // Launch k3_trans_real<OPK, p> over nb output boxes of level l (orders p <= 15;
-// 32 right-hand sides per block up to order 9, 16 above).
+// 32 right-hand sides per block, 16 for M2M and L2L above order 9).
 template <int OPK>
 static int launch_trans_real(hipStream_t st, KD kd, const TD &td, int l, int p, int i0,
                              int nb, const int *boxes, const int *starts, const int *lists,
                              const double *in, double *out, const int *allreal)
 {
-    const int rcb = p <= 9 ? 32 : 16;
+    const int rcb = (OPK != 0 && p >= 10) ? 16 : 32;
     for (int r0 = 0; r0 < kd.nr; r0 += rcb) {
         kd.r0 = r0;
         kd.rc = std::min(rcb, kd.nr - r0);
 #define TR_CASE(PP)                                                                        \
     case PP:                                                                               \
-        hipLaunchKernelGGL((k3_trans_real<OPK, PP, (PP <= 9 ? 2 : 1)>), dim3(nb), dim3(256), \
-                           0, st, kd, td, l, i0, boxes, starts, lists, in, out, allreal);  \
+        hipLaunchKernelGGL((k3_trans_real<OPK, PP, (OPK != 0 && PP >= 10) ? 1 : 2>), dim3(nb), \
+                           dim3(256), 0, st, kd, td,                                       \
+                           l, i0, boxes, starts, lists, in, out, allreal);                 \
         break;
         switch (p) {
             TR_CASE(1) TR_CASE(2) TR_CASE(3) TR_CASE(4) TR_CASE(5) TR_CASE(6) TR_CASE(7)
