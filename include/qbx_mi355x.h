@@ -303,6 +303,45 @@ int qbx_drive_fmm_noalloc(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
                           const qbx_kernel_desc *kd, const double *strengths_tree,
                           const double *normals_tree, double *out_user);
 
+/* ---- adjoint QBX-FMM (2D, one density) --------------------------------------
+ * No reference counterpart: pytential differentiates only by differencing or
+ * through its dense matrix builder (pytential/symbolic/matrix.py:376-522),
+ * whose conjugate transpose this applies in O(N).  For the forward map
+ * out_user = A strengths_tree of qbx_drive_fmm (2D Laplace or Helmholtz, S or
+ * D, nrhs 1, no target derivative) it computes str_adj = A^H pot_adj_user
+ * through the transposed stages on the same tree, lists and orders (Laplace:
+ * the transpose of the real-linear map strengths -> real potentials).
+ * pot_adj_user [n_user_targets] complex, str_adj [nsources] complex, tree
+ * order (overwritten).  Workspaces are the context's. */
+int qbx_drive_fmm_adjoint(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
+                          const qbx_trav *trav, const qbx_geo *geo,
+                          const qbx_kernel_desc *kd, const double *pot_adj_user,
+                          const double *normals_tree, double *str_adj);
+
+/* One transposed stage of the above (for stage-by-stage checks): in is the
+ * adjoint of the forward stage's output, out that of its input; every stage
+ * adds into out except QBXL2P (overwrites the QBX expansions' adjoint) and
+ * L2L / M2M (update in in place, out unused).  Arrays as the forward stages
+ * (tree-order potentials [ntargets], level-compact expansions, QBX expansions
+ * [ncenters][ncoeff_q], tree-order strengths [nsources], all complex). */
+enum {
+    QBX_ADJ_QBXL2P = 0,  /* potentials -> QBX expansions */
+    QBX_ADJ_L2QBXL = 1,  /* QBX expansions -> box locals */
+    QBX_ADJ_M2QBXL = 2,  /* QBX expansions -> multipoles */
+    QBX_ADJ_P2QBXL = 3,  /* QBX expansions -> strengths */
+    QBX_ADJ_L2P = 4,     /* potentials -> box locals */
+    QBX_ADJ_M2P = 5,     /* potentials -> multipoles */
+    QBX_ADJ_P2P = 6,     /* potentials -> strengths */
+    QBX_ADJ_L2L = 7,     /* locals, child -> parent */
+    QBX_ADJ_M2L = 8,     /* locals -> multipoles */
+    QBX_ADJ_P2L = 9,     /* locals -> strengths */
+    QBX_ADJ_M2M = 10,    /* multipoles, parent -> child */
+    QBX_ADJ_P2M = 11     /* multipoles -> strengths */
+};
+int qbx_adjoint_stage(qbx_ctx *ctx, void *stream, const qbx_tree *tree, const qbx_trav *trav,
+                      const qbx_geo *geo, const qbx_kernel_desc *kd, int stage,
+                      const double *in, double *out, const double *normals_tree);
+
 /* ---- direct QBX (no FMM) ---------------------------------------------------
  * Replaces exec_compute_potential_insn_direct (pytential/qbx/__init__.py:
  * 866-1068) with LayerPotentialOnTargetAndCenterSubset (qbx/direct.py:41-154):

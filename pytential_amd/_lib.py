@@ -85,6 +85,8 @@ _SIGS = {
     "qbx_drive_fmm_noalloc": [P, P, P, P, P, ct.POINTER(KernelDesc), P, P, P],
     "qbx_eval_target_specific_qbx_locals": [P, P, P, P, P, I, I, D, D, I, P, P, P, P],
     "qbx_eval_target_specific_qbx_locals_nrhs": [P, P, P, P, P, I, I, I, I, P, P, P, I],
+    "qbx_drive_fmm_adjoint": [P, P, P, P, P, P, P, P, P],
+    "qbx_adjoint_stage": [P, P, P, P, P, P, I, P, P, P],
     "qbx_direct_qbx": [P, P, P, P, ct.POINTER(KernelDesc), P, P, P],
     "qbx_p2p_all": [P, P, ct.POINTER(KernelDesc), I, P, P, P, I, P, P],
     "qbx_lpot_matrix": [P, P, ct.POINTER(KernelDesc), I, I, I, P, P, P, P, I, P, P, P, L, P],

@@ -102,3 +102,47 @@ def single_layer_far(src, sigma, k, targets):
     sig = sigma if isinstance(sigma, torch.Tensor) else torch.as_tensor(sigma)
     sig = sig.to(device="cuda", dtype=torch.complex128)
     return HelmholtzSingleLayerFar.apply(sig, k, src, tg)
+
+
+class QBXLayerPotential2D(torch.autograd.Function):
+    """S or D of a stage-1 nodal density on a curve at (targets, side) pairs
+    through the QBX-FMM, differentiable in the density: the backward applies
+    the transposed QBX-FMM (qbx_drive_fmm_adjoint: QBXL2P^T, L2QBXL^T,
+    M2QBXL^T, P2QBXL^T, the point-FMM transposes), then the weights and the
+    interpolation transposed.  Laplace densities and potentials are real."""
+
+    @staticmethod
+    def forward(ctx, sigma, src, kernel, k, source_kind, targets_and_sides):
+        outs = src.exec_compute_potential(kernel, sigma, list(targets_and_sides), k=k,
+                                          source_kind=source_kind)
+        ctx.src, ctx.kernel, ctx.k = src, kernel, k
+        ctx.source_kind, ctx.pairs = source_kind, list(targets_and_sides)
+        ctx.sizes = [o.shape[0] for o in outs]
+        ctx.real = not torch.is_complex(sigma)
+        return torch.cat(outs)
+
+    @staticmethod
+    def backward(ctx, g):
+        if not ctx.needs_input_grad[0]:
+            return (None,) * 6
+        parts = list(torch.split(g.to(torch.complex128), ctx.sizes))
+        ga = ctx.src.exec_compute_potential_adjoint(ctx.kernel, parts, ctx.pairs, k=ctx.k,
+                                                    source_kind=ctx.source_kind)
+        return (ga.real if ctx.real else ga), None, None, None, None, None
+
+
+def layer_potential(src, kernel, sigma, targets_and_sides, k=0.0, source_kind="S"):
+    """``exec_compute_potential`` of one density, concatenated over the
+    target pairs, differentiable in ``sigma`` (2D QBX-FMM; see
+    QBXLayerPotential2D)."""
+    if src.ambient_dim != 2:
+        raise ValueError("layer_potential differentiates the 2D QBX-FMM")
+    sig = sigma if isinstance(sigma, torch.Tensor) else torch.as_tensor(sigma)
+    sig = sig.to(device="cuda")
+    if kernel == "laplace":
+        sig = sig.to(torch.float64)
+    else:
+        sig = sig.to(torch.complex128)
+    return QBXLayerPotential2D.apply(sig, src, kernel, float(k), source_kind,
+                                     tuple(targets_and_sides))
+

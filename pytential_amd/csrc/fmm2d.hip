@@ -4296,3 +4296,6 @@ int qbx_drive_fmm_noalloc(qbx_ctx *ctx, void *stream, const qbx_tree *T, const q
 }
 
 }  // extern "C"
+
+// the adjoint (conjugate transpose) of the stages above
+#include "adjoint2d.inc"

@@ -123,6 +123,9 @@ struct qbx_ctx {
     // qbx_drive_fmm (3D Laplace): 1 when every strength of the call is real
     // (the translations then use the conjugate symmetry of the expansions)
     qbx::dbuf<int> ws_allreal;
+    // qbx_drive_fmm_adjoint: the adjoints of the multipoles, locals, QBX
+    // expansions and tree-order potentials
+    qbx::dbuf<double> ws_adj_m, ws_adj_l, ws_adj_q, ws_adj_p;
     std::vector<double> tab3_key;
     // side streams of qbx_drive_fmm (default priority; shared by every
     // context on the device, see side_streams) and its fork/join events

@@ -295,6 +295,56 @@ class QBXExpansionWrangler:
         return out
 
 
+# the transposed stages of qbx_adjoint_stage (include/qbx_mi355x.h)
+ADJOINT_STAGES = {"eval_qbx_expansions": 0, "translate_box_local_to_qbx_local": 1,
+                  "translate_box_multipoles_to_qbx_local": 2, "form_global_qbx_locals": 3,
+                  "eval_locals": 4, "eval_multipoles": 5, "eval_direct": 6,
+                  "refine_locals": 7, "multipole_to_local": 8, "form_locals": 9,
+                  "coarsen_multipoles": 10, "form_multipoles": 11}
+
+
+def adjoint_stage(wrangler: QBXExpansionWrangler, stage, adj_out, adj_in=None, normals=None):
+    """The conjugate transpose of one wrangler stage (named as the forward
+    method): ``adj_out`` is the adjoint of the stage's output, the result
+    (added into ``adj_in`` when given) that of its input.  refine_locals and
+    coarsen_multipoles update ``adj_out`` in place and return it; the others
+    return the input adjoint (eval_qbx_expansions overwrites the QBX
+    expansions' adjoint).  2D, one density."""
+    w = wrangler
+    sid = ADJOINT_STAGES[stage]
+    adj_out = adj_out.to(torch.complex128).contiguous()
+    if stage in ("refine_locals", "coarsen_multipoles"):
+        call("qbx_adjoint_stage", w.ctx.h, _stream(), w.tree.h, w.traversal.h, w.geo.h,
+             ct.byref(w.kd), sid, _p(adj_out), None, _p(normals))
+        return adj_out
+    if adj_in is None:
+        if stage in ("eval_qbx_expansions",):
+            adj_in = w.qbx_local_expansion_zeros()
+        elif stage in ("translate_box_local_to_qbx_local", "eval_locals"):
+            adj_in = w.local_expansion_zeros()
+        elif stage in ("translate_box_multipoles_to_qbx_local", "eval_multipoles",
+                       "multipole_to_local"):
+            adj_in = w.multipole_expansion_zeros()
+        else:
+            adj_in = torch.zeros(w.tree.nsources, dtype=torch.complex128, device="cuda")
+    call("qbx_adjoint_stage", w.ctx.h, _stream(), w.tree.h, w.traversal.h, w.geo.h,
+         ct.byref(w.kd), sid, _p(adj_out), _p(adj_in), _p(normals))
+    return adj_in
+
+
+def drive_fmm_adjoint(wrangler: QBXExpansionWrangler, pot_adj, normals):
+    """The conjugate transpose of ``drive_fmm_fused``: for the forward map
+    strengths (tree order) -> potentials at the user targets, the adjoint of
+    the strengths from that of the potentials (``pot_adj`` [n_user_targets]
+    complex; Laplace: only its real part enters).  2D, one density."""
+    w = wrangler
+    out = torch.empty(w.tree.nsources, dtype=torch.complex128, device="cuda")
+    pa = pot_adj.to(torch.complex128).contiguous()
+    call("qbx_drive_fmm_adjoint", w.ctx.h, _stream(), w.tree.h, w.traversal.h, w.geo.h,
+         ct.byref(w.kd), _p(pa), _p(normals), _p(out))
+    return out
+
+
 def drive_fmm(wrangler: QBXExpansionWrangler, strengths, normals):
     """Stage-by-stage QBX FMM (pytential/qbx/fmm.py:414-594).  Returns the
     potential at the user targets, in user order."""

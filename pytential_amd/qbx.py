@@ -288,3 +288,43 @@ class QBXLayerPotentialSource:
         if kernel == "laplace":
             outs = [o.real for o in outs]
         return outs
+
+    def exec_compute_potential_adjoint(self, kernel, pot_adj, targets_and_sides, k=0.0,
+                                       source_kind="S"):
+        """The conjugate transpose of ``exec_compute_potential`` for one
+        density (2D, QBX-FMM): from the adjoints of the potentials (one
+        tensor per ``(targets, qbx_forced_limit)`` pair, as the forward
+        returns them) the adjoint of the stage-1 nodal density, U^T W A^H
+        (A the QBX-FMM of qbx_drive_fmm, W the quadrature weights, U the
+        stage-1 -> quad-stage-2 interpolation).  Laplace: the transpose of
+        the real forward map (pass real adjoints, take the real part)."""
+        from ._lib import call
+        from .geometry import _p, _stream
+        from . import fmm as _fmm
+        if self.ambient_dim != 2:
+            raise NotImplementedError("the adjoint QBX-FMM is 2D")
+        if self._direct:
+            raise NotImplementedError("the adjoint is that of the QBX-FMM (fmm_order given)")
+        sides = [(t, 0 if sd is None else sd) for t, sd in targets_and_sides]
+        geo = self.qbx_fmm_geometry_data(tuple(sides))
+        w = self.make_wrangler(geo, kernel, k, source_kind, 1)
+        d = self.density_discr
+        ga = torch.cat([as_dev(g, torch.complex128).reshape(-1) for g in pot_adj])
+        nuser = w.tree.ntargets - geo.ncenters
+        if ga.numel() != nuser:
+            raise ValueError(f"adjoint has {ga.numel()} values, the targets {nuser}")
+        nrm = None
+        if source_kind == "D":
+            zero = torch.zeros(d.nsources, dtype=torch.complex128, device="cuda")
+            _, nrm = w.reorder_sources(zero, self.dev["quad_weights"], self.dev["quad_normals"])
+        sa = _fmm.drive_fmm_adjoint(w, ga, nrm)
+        # tree -> user source order, the quadrature weights, the interpolation
+        uid = w.tree.user_source_ids.long()
+        qa = torch.empty_like(sa)
+        qa[uid] = sa
+        qa = (qa * self.dev["quad_weights"]).contiguous()
+        out = torch.empty(d.ndofs, dtype=torch.complex128, device="cuda")
+        call("qbx_upsample_adjoint", self.ctx.h, _stream(), d.nelements,
+             d.nodes_per_element, d.fine_nodes_per_element, 1, _p(self.dev["upsample"]),
+             _p(qa), _p(out))
+        return out

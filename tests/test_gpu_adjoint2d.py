@@ -1,0 +1,185 @@
+"""GPU: the adjoint (conjugate transpose) of the 2D QBX-FMM.
+
+* every transposed stage against its forward stage through the identity
+  Re <y', A x> = Re <A^H y', x> on random inputs (P2M, M2M, P2P, M2L, M2P,
+  P2L, L2L, L2P, P2QBXL, M2QBXL, L2QBXL, QBXL2P), 1e-12 relative to
+  |y'| |A x|;
+* the whole chain (qbx_drive_fmm_adjoint) against the fused forward the same
+  way: the adjoint is the exact transpose of the discrete forward operator;
+* the density adjoint (weights, interpolation) against the conjugate
+  transpose of the dense QBX matrix (qbx_lpot_matrix, on-surface targets),
+  to the FMM's agreement with that matrix;
+* the autograd function: the gradient of |S sigma - g|^2 against central
+  differences along random directions, 1e-7.
+Laplace and Helmholtz (k = 8), single and double layer.
+"""
+import functools
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+CASES = [("laplace", 0.0, "S"), ("laplace", 0.0, "D"), ("helmholtz", 8.0, "S"),
+         ("helmholtz", 8.0, "D")]
+
+
+def _dot(a, b):
+    return complex(torch.sum(torch.conj(a.reshape(-1)) * b.reshape(-1)))
+
+
+def _rnd(n, rng, real=False):
+    x = rng.normal(size=n)
+    if not real:
+        x = x + 1j * rng.normal(size=n)
+    return torch.as_tensor(x, dtype=torch.complex128, device="cuda")
+
+
+@functools.lru_cache(maxsize=1)
+def _config():
+    import oracle
+
+    from pytential_amd import problems
+    return problems.refine_until_associated(problems.config1(nelements=48, grid_n=24),
+                                            oracle.assoc_status_and_flags)
+
+
+def _setup(kernel, k, kind, fmm_order=12):
+    """The 5-arm starfish of config 1 (48 panels) with on-surface targets
+    (side +1) and a 24^2 grid on [-2, 2]^2 (grid points near the curve get
+    QBX centers, the rest the point FMM: P2P, M2P and L2P all act)."""
+    from pytential_amd.qbx import QBXLayerPotentialSource
+    cfg = _config()
+    d = cfg.discr
+    src = QBXLayerPotentialSource(d, fine_order=d.fine_order, qbx_order=4,
+                                  fmm_order=fmm_order)
+    pairs = [("self", +1), (cfg.targets[:, d.ndofs:], 0)]
+    return d, src, pairs
+
+
+@pytest.mark.parametrize("kernel,k,kind", CASES)
+def test_transposed_stages_dot_identity(kernel, k, kind):
+    from pytential_amd import fmm as F
+    d, src, pairs = _setup(kernel, k, kind)
+    geo = src.qbx_fmm_geometry_data(tuple(pairs))
+    w = src.make_wrangler(geo, kernel, k, kind, 1)
+    rng = np.random.default_rng(11)
+    ns = w.tree.nsources
+    _, nrm = w.reorder_sources(torch.zeros(ns, dtype=torch.complex128, device="cuda"),
+                               src.dev["quad_weights"],
+                               src.dev["quad_normals"] if kind == "D" else None)
+    nm = w.multipole_expansion_zeros().numel()
+    nq = w.qbx_local_expansion_zeros().numel()
+    npot = w.full_output_zeros().numel()
+    # forward stages as functions of their input
+    fw = {
+        "form_multipoles": (lambda x: w.form_multipoles(x, nrm), ns, nm),
+        "coarsen_multipoles": (lambda x: w.coarsen_multipoles(x.clone()), nm, nm),
+        "eval_direct": (lambda x: w.eval_direct(x, nrm), ns, npot),
+        "multipole_to_local": (lambda x: w.multipole_to_local(x), nm, nm),
+        "eval_multipoles": (lambda x: w.eval_multipoles(x), nm, npot),
+        "form_locals": (lambda x: w.form_locals(x, nrm), ns, nm),
+        "refine_locals": (lambda x: w.refine_locals(x.clone()), nm, nm),
+        "eval_locals": (lambda x: w.eval_locals(x), nm, npot),
+        "form_global_qbx_locals": (lambda x: w.form_global_qbx_locals(x, nrm), ns, nq),
+        "translate_box_multipoles_to_qbx_local":
+            (lambda x: w.translate_box_multipoles_to_qbx_local(x), nm, nq),
+        "translate_box_local_to_qbx_local":
+            (lambda x: w.translate_box_local_to_qbx_local(x), nm, nq),
+        "eval_qbx_expansions": (lambda x: w.eval_qbx_expansions(x), nq, npot),
+    }
+    for stage, (f, nin, nout) in fw.items():
+        x = _rnd(nin, rng)
+        if kernel == "laplace" and stage in ("form_multipoles", "form_locals",
+                                             "form_global_qbx_locals", "eval_direct"):
+            # the 2D Laplace stages take real strengths
+            x = x.real.to(torch.complex128)
+        ya = _rnd(nout, rng)
+        y = f(x).reshape(-1)
+        xa = F.adjoint_stage(w, stage, ya.clone(), normals=nrm).reshape(-1)
+        lhs = _dot(ya, y).real
+        rhs = _dot(xa, x).real
+        scale = float(torch.linalg.norm(ya) * torch.linalg.norm(y)) + 1e-300
+        err = abs(lhs - rhs) / scale
+        print(f"[adjoint] {kernel}-{kind} {stage}: Re<y',Ax> {lhs:+.6e} Re<A^H y',x> "
+              f"{rhs:+.6e} rel {err:.1e}", flush=True)
+        assert err < 1e-12, (stage, err)
+
+
+@pytest.mark.parametrize("kernel,k,kind", CASES)
+def test_adjoint_chain_is_exact_transpose(kernel, k, kind):
+    from pytential_amd import fmm as F
+    d, src, pairs = _setup(kernel, k, kind)
+    geo = src.qbx_fmm_geometry_data(tuple(pairs))
+    w = src.make_wrangler(geo, kernel, k, kind, 1)
+    rng = np.random.default_rng(12)
+    ns = w.tree.nsources
+    x = _rnd(ns, rng, real=kernel == "laplace")
+    _, nrm = w.reorder_sources(torch.zeros(ns, dtype=torch.complex128, device="cuda"),
+                               src.dev["quad_weights"],
+                               src.dev["quad_normals"] if kind == "D" else None)
+    y = F.drive_fmm_fused(w, x, nrm)
+    ya = _rnd(y.numel(), rng, real=kernel == "laplace")
+    xa = F.drive_fmm_adjoint(w, ya, nrm)
+    lhs = _dot(ya, y).real
+    rhs = _dot(xa, x).real
+    err = abs(lhs - rhs) / float(torch.linalg.norm(ya) * torch.linalg.norm(y))
+    print(f"[adjoint] {kernel}-{kind} chain: rel {err:.1e}", flush=True)
+    assert err < 1e-12
+
+
+@pytest.mark.parametrize("kernel,k,kind", CASES)
+def test_density_adjoint_vs_dense_matrix(kernel, k, kind):
+    from pytential_amd.matrix import build_matrix
+    d, src, _ = _setup(kernel, k, kind, fmm_order=20)
+    pairs = [("self", +1)]
+    rng = np.random.default_rng(13)
+    ga = rng.normal(size=d.ndofs) + (1j * rng.normal(size=d.ndofs) if kernel == "helmholtz"
+                                     else 0)
+    got = src.exec_compute_potential_adjoint(kernel, [ga], pairs, k=k, source_kind=kind)
+    A = build_matrix(src, kernel, k, kind, qbx_forced_limit=+1)
+    A = A.cpu().numpy()
+    ref = A.conj().T @ ga
+    got = got.cpu().numpy()
+    if kernel == "laplace":
+        got, ref = got.real, ref.real
+    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    print(f"[adjoint] {kernel}-{kind} density adjoint vs dense A^H: {err:.2e}", flush=True)
+    assert err < 1e-8
+
+
+@pytest.mark.parametrize("kernel,k,kind", CASES)
+def test_autograd_gradient_vs_central_differences(kernel, k, kind):
+    from pytential_amd.autodiff import layer_potential
+    d, src, pairs = _setup(kernel, k, kind, fmm_order=16)
+    rng = np.random.default_rng(14)
+    cplx = kernel == "helmholtz"
+
+    def rand(n):
+        v = rng.normal(size=n)
+        return v + 1j * rng.normal(size=n) if cplx else v
+    dt = torch.complex128 if cplx else torch.float64
+    sigma = torch.as_tensor(rand(d.ndofs), dtype=dt, device="cuda")
+    out0 = layer_potential(src, kernel, sigma, pairs, k=k, source_kind=kind)
+    g = torch.as_tensor(rand(out0.shape[0]), dtype=dt, device="cuda")
+
+    def loss(s):
+        r = layer_potential(src, kernel, s, pairs, k=k, source_kind=kind) - g
+        return torch.sum(torch.abs(r) ** 2)
+    s = sigma.clone().requires_grad_(True)
+    L = loss(s)
+    L.backward()
+    grad = s.grad.detach()
+    for trial in range(3):
+        v = torch.as_tensor(rand(d.ndofs), dtype=dt, device="cuda")
+        h = 1e-5
+        with torch.no_grad():
+            fd = (loss(sigma + h * v) - loss(sigma - h * v)) / (2 * h)
+        # directional derivative of a real loss: Re <grad, v>
+        an = torch.sum((torch.conj(grad) * v).real) if cplx else torch.sum(grad * v)
+        err = abs(float(an) - float(fd)) / abs(float(fd))
+        print(f"[adjoint] {kernel}-{kind} autograd vs central difference {trial}: "
+              f"{float(an):+.8e} {float(fd):+.8e} rel {err:.1e}", flush=True)
+        assert err < 1e-7
