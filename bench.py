@@ -281,6 +281,27 @@ def run_gpu(args, cfg, orders_fn, label):
         fused = fused_step()
     torch.cuda.synchronize()
     dt_fused = (time.perf_counter() - t1) / args.steps
+    # the adjoint (conjugate-transposed) QBX-FMM on the same tree and orders:
+    # the backward pass of the density gradient, checked against the forward
+    # by Re <y', A x> = Re <A^H y', x>
+    qd = src.quad_density(dens)
+    st, nrm = w.reorder_sources(qd, src.dev["quad_weights"], src.dev["quad_normals"])
+    fx = F.drive_fmm_fused(w, st, nrm)
+    rng = np.random.default_rng(90)
+    ya = torch.as_tensor(rng.standard_normal(fx.numel()) + 1j * rng.standard_normal(fx.numel()),
+                         dtype=torch.complex128, device="cuda")
+    xa = F.drive_fmm_adjoint(w, ya, nrm)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    nadj = max(2, args.steps // 3)
+    for _ in range(nadj):
+        xa = F.drive_fmm_adjoint(w, ya, nrm)
+    torch.cuda.synchronize()
+    dt_adj = (time.perf_counter() - t2) / nadj
+    lhs = complex(torch.sum(torch.conj(ya) * fx)).real
+    rhs = complex(torch.sum(torch.conj(xa) * st)).real
+    adj_rel = abs(lhs - rhs) / float(torch.linalg.norm(ya) * torch.linalg.norm(fx))
+    log(f"[{label}] adjoint {dt_adj * 1e3:.3f} ms, transpose identity {adj_rel:.1e}")
     counts = interaction_counts(geo)
     flops = stage_flops(counts, orders, cfg.qbx_order)
     flops40 = stage_flops(counts, orders, cfg.qbx_order, SPECIAL=40)
@@ -288,6 +309,7 @@ def run_gpu(args, cfg, orders_fn, label):
     fused = fused.cpu().numpy()
     return {"label": label, "orders": orders, "cold_s": cold, "step_s": dt,
             "step_fused_s": dt_fused, "per_stage_ms": per_stage, "counts": counts,
+            "adjoint_s": dt_adj, "adjoint_identity_rel": adj_rel,
             "flops": flops, "flops40": flops40, "result": fused, "result_stagewise": staged,
             "fused_vs_stagewise": float(np.max(np.abs(fused - staged)) /
                                         np.max(np.abs(staged))),
@@ -1236,6 +1258,10 @@ def main():
             "rel_diff_rhs0_vs_plain": float(
                 np.max(np.abs(batched["result0"] - main_run["result"])) /
                 np.max(np.abs(main_run["result"])))},
+        "adjoint": {"ms_per_step": round(main_run["adjoint_s"] * 1e3, 4),
+                    "what": "qbx_drive_fmm_adjoint: the conjugate-transposed QBX-FMM "
+                            "(density gradient backward pass), same tree and orders",
+                    "transpose_identity_rel": main_run["adjoint_identity_rel"]},
         "stagewise_ms_per_step": round(main_run["step_s"] * 1e3, 4),
         "cold_geometry_s": round(main_run["cold_s"], 4),
         "per_stage_ms": {k: round(v, 4) for k, v in ps.items()},

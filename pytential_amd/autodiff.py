@@ -1,6 +1,11 @@
-"""Differentiable single-layer potential of a surface density at smooth
-targets, 3D Helmholtz: a ``torch.autograd.Function`` whose backward applies
-hand-written adjoint kernels.
+"""Differentiable layer potentials: ``torch.autograd.Function``s whose
+backward applies hand-written adjoint kernels.
+
+* ``layer_potential`` (2D, QBX-FMM): S or D of a curve density at on- and
+  off-surface targets; the backward runs the transposed QBX-FMM
+  (QBXLayerPotential2D below).
+* ``single_layer_far`` (3D Helmholtz, far targets): the single layer of a
+  surface density at smooth targets, described next.
 
 The forward is what pytential computes for ``bind(places, S_k(sigma))``
 evaluated at targets far from the surface (``qbx_forced_limit=None``, the
@@ -106,35 +111,59 @@ def single_layer_far(src, sigma, k, targets):
 
 class QBXLayerPotential2D(torch.autograd.Function):
     """S or D of a stage-1 nodal density on a curve at (targets, side) pairs
-    through the QBX-FMM, differentiable in the density: the backward applies
-    the transposed QBX-FMM (qbx_drive_fmm_adjoint: QBXL2P^T, L2QBXL^T,
-    M2QBXL^T, P2QBXL^T, the point-FMM transposes), then the weights and the
-    interpolation transposed.  Laplace densities and potentials are real."""
+    through the QBX-FMM, differentiable in the density and (Helmholtz) the
+    wavenumber.  Backward:
+
+        d/dsigma = U^T W A(k)^H g   (the transposed QBX-FMM,
+                                    qbx_drive_fmm_adjoint: QBXL2P^T, L2QBXL^T,
+                                    M2QBXL^T, P2QBXL^T and the point-FMM
+                                    transposes, then the weights and the
+                                    interpolation transposed)
+        d/dk     = Re sum_t conj(g_t) (dA/dk sigma)_t, dA/dk sigma by the
+                   fourth-order central difference of the forward in k
+                   (step 1e-4 k; same tree, lists and orders)
+
+    Laplace densities and potentials are real."""
 
     @staticmethod
-    def forward(ctx, sigma, src, kernel, k, source_kind, targets_and_sides):
-        outs = src.exec_compute_potential(kernel, sigma, list(targets_and_sides), k=k,
+    def forward(ctx, sigma, k, src, kernel, source_kind, targets_and_sides):
+        kv = float(k)
+        outs = src.exec_compute_potential(kernel, sigma, list(targets_and_sides), k=kv,
                                           source_kind=source_kind)
-        ctx.src, ctx.kernel, ctx.k = src, kernel, k
+        ctx.src, ctx.kernel, ctx.k = src, kernel, kv
         ctx.source_kind, ctx.pairs = source_kind, list(targets_and_sides)
         ctx.sizes = [o.shape[0] for o in outs]
         ctx.real = not torch.is_complex(sigma)
+        ctx.k_device = k.device
+        ctx.save_for_backward(sigma)
         return torch.cat(outs)
 
     @staticmethod
     def backward(ctx, g):
-        if not ctx.needs_input_grad[0]:
-            return (None,) * 6
-        parts = list(torch.split(g.to(torch.complex128), ctx.sizes))
-        ga = ctx.src.exec_compute_potential_adjoint(ctx.kernel, parts, ctx.pairs, k=ctx.k,
-                                                    source_kind=ctx.source_kind)
-        return (ga.real if ctx.real else ga), None, None, None, None, None
+        sigma, = ctx.saved_tensors
+        grad_sigma = grad_k = None
+        if ctx.needs_input_grad[0]:
+            parts = list(torch.split(g.to(torch.complex128), ctx.sizes))
+            ga = ctx.src.exec_compute_potential_adjoint(ctx.kernel, parts, ctx.pairs,
+                                                        k=ctx.k, source_kind=ctx.source_kind)
+            grad_sigma = ga.real if ctx.real else ga
+        if ctx.needs_input_grad[1] and ctx.kernel == "helmholtz":
+            h = 1e-4 * ctx.k
+
+            def at(kk):
+                return torch.cat(ctx.src.exec_compute_potential(
+                    ctx.kernel, sigma, ctx.pairs, k=kk, source_kind=ctx.source_kind))
+            dk = (-at(ctx.k + 2 * h) + 8 * at(ctx.k + h) - 8 * at(ctx.k - h)
+                  + at(ctx.k - 2 * h)) / (12 * h)
+            grad_k = torch.real(torch.sum(torch.conj(g.to(torch.complex128)) * dk)).to(
+                device=ctx.k_device, dtype=torch.float64)
+        return grad_sigma, grad_k, None, None, None, None
 
 
 def layer_potential(src, kernel, sigma, targets_and_sides, k=0.0, source_kind="S"):
     """``exec_compute_potential`` of one density, concatenated over the
-    target pairs, differentiable in ``sigma`` (2D QBX-FMM; see
-    QBXLayerPotential2D)."""
+    target pairs, differentiable in ``sigma`` and (a tensor) ``k`` (2D
+    QBX-FMM; see QBXLayerPotential2D)."""
     if src.ambient_dim != 2:
         raise ValueError("layer_potential differentiates the 2D QBX-FMM")
     sig = sigma if isinstance(sigma, torch.Tensor) else torch.as_tensor(sigma)
@@ -143,6 +172,7 @@ def layer_potential(src, kernel, sigma, targets_and_sides, k=0.0, source_kind="S
         sig = sig.to(torch.float64)
     else:
         sig = sig.to(torch.complex128)
-    return QBXLayerPotential2D.apply(sig, src, kernel, float(k), source_kind,
+    if not isinstance(k, torch.Tensor):
+        k = torch.tensor(float(k), dtype=torch.float64)
+    return QBXLayerPotential2D.apply(sig, k, src, kernel, source_kind,
                                      tuple(targets_and_sides))
-

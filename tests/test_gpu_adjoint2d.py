@@ -10,7 +10,8 @@
   transpose of the dense QBX matrix (qbx_lpot_matrix, on-surface targets),
   to the FMM's agreement with that matrix;
 * the autograd function: the gradient of |S sigma - g|^2 against central
-  differences along random directions, 1e-7.
+  differences along random directions, 1e-7, and its wavenumber derivative
+  (Helmholtz) against a central difference in k, 1e-7.
 Laplace and Helmholtz (k = 8), single and double layer.
 """
 import functools
@@ -183,3 +184,36 @@ def test_autograd_gradient_vs_central_differences(kernel, k, kind):
         print(f"[adjoint] {kernel}-{kind} autograd vs central difference {trial}: "
               f"{float(an):+.8e} {float(fd):+.8e} rel {err:.1e}", flush=True)
         assert err < 1e-7
+
+
+@pytest.mark.parametrize("kind", ["S", "D"])
+def test_autograd_wavenumber_gradient(kind):
+    """d|S_k sigma - g|^2 / dk (Helmholtz) against a central difference of the
+    loss in k, 1e-7."""
+    from pytential_amd.autodiff import layer_potential
+    d, src, pairs = _setup("helmholtz", 8.0, kind, fmm_order=16)
+    rng = np.random.default_rng(15)
+    sigma = torch.as_tensor(rng.normal(size=d.ndofs) + 1j * rng.normal(size=d.ndofs),
+                            dtype=torch.complex128, device="cuda")
+    out0 = layer_potential(src, "helmholtz", sigma, pairs, k=8.0, source_kind=kind)
+    g = out0.detach() * (1.0 + 0.1j) + 0.05
+
+    def loss(kk):
+        r = layer_potential(src, "helmholtz", sigma, pairs, k=kk, source_kind=kind) - g
+        return torch.sum(torch.abs(r) ** 2)
+    k = torch.tensor(8.0, dtype=torch.float64, requires_grad=True)
+    L = loss(k)
+    L.backward()
+    # the loss differenced directly (fourth order, another step than the
+    # backward's difference of the potentials)
+    h = 3e-4
+    def kt(v):
+        return torch.tensor(v, dtype=torch.float64)
+    with torch.no_grad():
+        fd = (-loss(kt(8.0 + 2 * h)) + 8 * loss(kt(8.0 + h)) - 8 * loss(kt(8.0 - h))
+              + loss(kt(8.0 - 2 * h))) / (12 * h)
+    err = abs(float(k.grad) - float(fd)) / abs(float(fd))
+    print(f"[adjoint] helmholtz-{kind} dL/dk {float(k.grad):+.10e} central difference "
+          f"{float(fd):+.10e} rel {err:.1e}", flush=True)
+    assert err < 1e-7
+
