@@ -27,6 +27,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -381,6 +382,25 @@ def cpu_baseline(args, cfg, orders, sample_idx):
             "fmm_result_sample": pot[sample_idx], "direct_sample": ref, "t_fmm": t_fmm}
 
 
+_CEILING = None
+
+
+def fp64_ceiling():
+    """FP64 throughput this GPU reaches without memory traffic (the probe
+    pytential_amd/fp64_ceiling built beside the library): TFLOP/s of
+    v_mfma_f64_16x16x4f64 and of v_fma_f64, run once per bench process."""
+    global _CEILING
+    if _CEILING is None:
+        exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "pytential_amd",
+                           "fp64_ceiling")
+        try:
+            r = subprocess.run([exe], capture_output=True, text=True, timeout=60, check=True)
+            _CEILING = json.loads(r.stdout.strip().splitlines()[-1])
+        except (OSError, subprocess.SubprocessError, ValueError) as e:
+            _CEILING = {"error": str(e)}
+    return _CEILING
+
+
 def host_cpu():
     """The host's CPU model, logical CPU count and the CPUs this process may
     use (lscpu's "Model name", nproc)."""
@@ -609,7 +629,7 @@ def run_sphere3d(args):
         "expansion_pipeline": expansion_pipeline,
         "per_stage_ms": {k: round(v, 4) for k, v in per_stage.items()},
         "stage_roofline": stage_rl,
-        "roofline": roofline,
+        "roofline": {**roofline, "measured_ceiling": fp64_ceiling()},
         "cpu_baseline": None,
     }
     if args.no_cpu:
@@ -1266,7 +1286,7 @@ def main():
         "cold_geometry_s": round(main_run["cold_s"], 4),
         "per_stage_ms": {k: round(v, 4) for k, v in ps.items()},
         "stage_roofline": stage_rl,
-        "roofline": rl,
+        "roofline": {**rl, "measured_ceiling": fp64_ceiling()},
         "cpu_baseline": None if cpu is None else {
             k: (round(v, 1) if isinstance(v, float) else v) for k, v in cpu.items()
             if k in ("value", "unit", "cores", "kind", "sample", "host", "one_thread")},
