@@ -197,13 +197,36 @@ def stage_flops(counts, orders, pq, SPECIAL=1):
             "p2m": p2m, "m2m": int(m2m), "l2l": l2l, "m2p": m2p}
 
 
-def run_gpu(args, cfg, orders_fn, label):
+def run_gpu(args, cfg, orders_fn, label, tune=False):
     import torch
 
+    from pytential_amd import autotune
     from pytential_amd import fmm as F
+    from pytential_amd.geometry import Context
     from pytential_amd.qbx import QBXLayerPotentialSource
     src = QBXLayerPotentialSource(cfg.discr, fine_order=cfg.discr.fine_order,
                                   qbx_order=cfg.qbx_order, fmm_level_to_order=orders_fn)
+    tuned = None
+    if tune and args.tuning:
+        # fixed settings (a profiled run repeats the headline's tuned shapes
+        # without the search's trial launches in its trace)
+        src._ctx = Context(torch.cuda.current_device())
+        autotune.apply(src.ctx, json.loads(args.tuning))
+        tuned = {"settings": autotune.get(src.ctx), "ms": float("nan"),
+                 "default_ms": float("nan"), "trials": [], "search_s": 0.0}
+    elif tune:
+        # the work-unit shapes searched on this very evaluation (its own
+        # context, so the other workloads keep the defaults); the timed
+        # evaluation below runs on a geometry built with the result
+        src._ctx = Context(torch.cuda.current_device())
+        t0 = time.perf_counter()
+        tuned = autotune.autotune(src, [(cfg.targets, cfg.target_sides)], cfg.kernel, cfg.k,
+                                  cfg.source_kind, density=cfg.density, reps=5,
+                                  use_cache=False)
+        tuned["search_s"] = time.perf_counter() - t0
+        log(f"[{label}] autotune: {tuned['settings']} {tuned['ms']:.3f} ms (defaults "
+            f"{tuned['default_ms']:.3f} ms, {len(tuned['trials'])} trials, "
+            f"{tuned['search_s']:.1f}s)")
     geo = src.qbx_fmm_geometry_data(((cfg.targets, cfg.target_sides),))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -310,7 +333,7 @@ def run_gpu(args, cfg, orders_fn, label):
     fused = fused.cpu().numpy()
     return {"label": label, "orders": orders, "cold_s": cold, "step_s": dt,
             "step_fused_s": dt_fused, "per_stage_ms": per_stage, "counts": counts,
-            "adjoint_s": dt_adj, "adjoint_identity_rel": adj_rel,
+            "adjoint_s": dt_adj, "adjoint_identity_rel": adj_rel, "autotune": tuned,
             "flops": flops, "flops40": flops40, "result": fused, "result_stagewise": staged,
             "fused_vs_stagewise": float(np.max(np.abs(fused - staged)) /
                                         np.max(np.abs(staged))),
@@ -1087,6 +1110,11 @@ def main():
     ap.add_argument("--nelements", type=int, default=4096)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="time the headline with the default work-unit shapes")
+    ap.add_argument("--tuning", default=None,
+                    help="JSON dict of work-unit shapes (pytential_amd.autotune.KNOBS) "
+                         "for the headline instead of searching them")
     ap.add_argument("--no-cpu-1thread", action="store_true",
                     help="skip the one-thread oracle timings")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -1160,7 +1188,7 @@ def main():
     def config_orders(kernel, kernel_args, tree, level):
         return cfg.fmm_order
 
-    main_run = run_gpu(args, cfg, tol_orders, "tol")
+    main_run = run_gpu(args, cfg, tol_orders, "tol", tune=not args.no_autotune)
     if args.profile:
         log(f"profile run: {main_run['step_s'] * 1e3:.3f} ms/step, per stage "
             f"{ {k: round(v, 4) for k, v in main_run['per_stage_ms'].items()} }")
@@ -1278,6 +1306,15 @@ def main():
             "rel_diff_rhs0_vs_plain": float(
                 np.max(np.abs(batched["result0"] - main_run["result"])) /
                 np.max(np.abs(main_run["result"])))},
+        "autotune": None if main_run["autotune"] is None else {
+            "settings": main_run["autotune"]["settings"],
+            "tuned_ms": round(main_run["autotune"]["ms"], 4),
+            "default_ms": round(main_run["autotune"]["default_ms"], 4),
+            "trials": len(main_run["autotune"]["trials"]),
+            "search_s": round(main_run["autotune"]["search_s"], 2),
+            "what": "pytential_amd.autotune: coordinate descent over the work-unit shapes "
+                    "(QBX_TUNE_*), each candidate the median of 5 fused evaluations on a "
+                    "geometry built with it; the headline is timed with the result"},
         "adjoint": {"ms_per_step": round(main_run["adjoint_s"] * 1e3, 4),
                     "what": "qbx_drive_fmm_adjoint: the conjugate-transposed QBX-FMM "
                             "(density gradient backward pass), same tree and orders",

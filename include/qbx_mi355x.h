@@ -303,6 +303,25 @@ int qbx_drive_fmm_noalloc(qbx_ctx *ctx, void *stream, const qbx_tree *tree,
                           const qbx_kernel_desc *kd, const double *strengths_tree,
                           const double *normals_tree, double *out_user);
 
+/* ---- tuning knobs ------------------------------------------------------------
+ * No reference counterpart: the work-unit shapes and lane splits that the
+ * autotuner (pytential_amd/autotune.py) searches per workload.  Unit plans
+ * (lanes per target so that a lane walks about this many list items) take
+ * effect at the next qbx_geo_build on the context; the L2QBXL lane split at
+ * the next evaluation.  Every setting gives the same results up to the
+ * order of the last additions (the L2QBXL split: bit for bit). */
+enum {
+    QBX_TUNE_P2P_UNIT_ITER = 0,     /* P2P: list-1 sources per lane (16..1024) */
+    QBX_TUNE_M2P_UNIT_ITER = 1,     /* M2P: list-3 boxes per lane (1..64) */
+    QBX_TUNE_P2QBXL_UNIT_ITER = 2,  /* P2QBXL: list-1 sources per lane (4..512) */
+    QBX_TUNE_M2QBXL_UNIT_ITER = 3,  /* M2QBXL: list-3 boxes per lane (1..64) */
+    QBX_TUNE_L2QBXL_LANES = 4,      /* 2D Helmholtz one-density L2QBXL: lanes per
+                                       center (1, 2, 4, 8) */
+    QBX_TUNE_COUNT = 5
+};
+int qbx_ctx_set_tuning(qbx_ctx *ctx, int param, int value);
+int qbx_ctx_get_tuning(qbx_ctx *ctx, int param, int *value);
+
 /* ---- adjoint QBX-FMM (2D, one density) --------------------------------------
  * No reference counterpart: pytential differentiates only by differencing or
  * through its dense matrix builder (pytential/symbolic/matrix.py:376-522),

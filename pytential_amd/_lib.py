@@ -44,6 +44,8 @@ _SIGS = {
     "qbx_ctx_create": [ct.POINTER(P), I],
     "qbx_ctx_destroy": [P],
     "qbx_ctx_release_graphs": [P],
+    "qbx_ctx_set_tuning": [P, I, I],
+    "qbx_ctx_get_tuning": [P, I, IP],
     "qbx_version": [],
     "qbx_tree_build": [P, P, I, I, P, I, P, P, P, P, I, D, I, ct.POINTER(P)],
     "qbx_tree_destroy": [P],

@@ -274,6 +274,29 @@ int qbx_ctx_create(qbx_ctx **ctx, int device)
     return QBX_OK;
 }
 
+int qbx_ctx_set_tuning(qbx_ctx *ctx, int param, int value)
+{
+    QBX_CHECK(ctx, "qbx_ctx_set_tuning: null context");
+    QBX_CHECK(param >= 0 && param < QBX_TUNE_COUNT, "qbx_ctx_set_tuning: unknown knob %d",
+              param);
+    static const int lo[QBX_TUNE_COUNT] = {16, 1, 4, 1, 1};
+    static const int hi[QBX_TUNE_COUNT] = {1024, 64, 512, 64, 8};
+    QBX_CHECK(value >= lo[param] && value <= hi[param] && (value & (value - 1)) == 0,
+              "qbx_ctx_set_tuning: knob %d takes a power of two in [%d, %d] (got %d)", param,
+              lo[param], hi[param], value);
+    ctx->tune[param] = value;
+    return QBX_OK;
+}
+
+int qbx_ctx_get_tuning(qbx_ctx *ctx, int param, int *value)
+{
+    QBX_CHECK(ctx && value, "qbx_ctx_get_tuning: null argument");
+    QBX_CHECK(param >= 0 && param < QBX_TUNE_COUNT, "qbx_ctx_get_tuning: unknown knob %d",
+              param);
+    *value = ctx->tune[param];
+    return QBX_OK;
+}
+
 int qbx_ctx_release_graphs(qbx_ctx *ctx)
 {
     QBX_CHECK(ctx, "qbx_ctx_release_graphs: null context");

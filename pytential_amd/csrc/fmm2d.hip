@@ -2855,7 +2855,6 @@ k_l2qbxl_split(KD kd, TD t, int ng, const int *gcent, const int *c2tb, const int
     }
 }
 
-constexpr int L2QS_LANES = 4;
 
 // QBXL2P: one thread per (center, target) entry of center_to_tree_targets
 template <int NRC, bool DER = false>
@@ -3844,11 +3843,20 @@ int qbx_translate_box_local_to_qbx_local(qbx_ctx *ctx, void *stream, const qbx_t
         return QBX_OK;
     }
     if (kd.kernel == QBX_HELMHOLTZ2D && kd.nr == 1 && kd.pq <= 12) {
-#define L_L2QS(PQV)                                                                        \
-    hipLaunchKernelGGL((k_l2qbxl_split<PQV, L2QS_LANES>), dim3(cdiv((long)ng * L2QS_LANES, 256)), \
+        // lanes per center: the tuning knob (every choice gives the same sums)
+        const int lanes = ctx->tune[QBX_TUNE_L2QBXL_LANES];
+#define L_L2QS_LS(PQV, LSV)                                                                \
+    hipLaunchKernelGGL((k_l2qbxl_split<PQV, LSV>), dim3(cdiv((long)ng * LSV, 256)),        \
                        dim3(256), 0, st, kd, td, ng, G->global_qbx_centers.p,               \
                        G->center_to_target_box.p, R->target_boxes.p, G->centers.p,          \
                        G->ncenters, local, qexp)
+#define L_L2QS(PQV)                                        \
+    do {                                                   \
+        if (lanes == 1) L_L2QS_LS(PQV, 1);                 \
+        else if (lanes == 2) L_L2QS_LS(PQV, 2);            \
+        else if (lanes == 8) L_L2QS_LS(PQV, 8);            \
+        else L_L2QS_LS(PQV, 4);                            \
+    } while (0)
         switch (kd.pq) {
         case 0: L_L2QS(0); break;
         case 1: L_L2QS(1); break;
@@ -3864,6 +3872,7 @@ int qbx_translate_box_local_to_qbx_local(qbx_ctx *ctx, void *stream, const qbx_t
         case 11: L_L2QS(11); break;
         default: L_L2QS(12); break;
         }
+#undef L_L2QS_LS
 #undef L_L2QS
         QBX_HIP(hipGetLastError());
         return QBX_OK;

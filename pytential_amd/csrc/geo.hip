@@ -309,14 +309,15 @@ static int build_geo(qbx_ctx *ctx, hipStream_t st, const qbx_tree *T, const qbx_
     }
     G->trav = R;
     QBX_TRY(build_gc_by_tbox(ctx, st, R, G));
-    QBX_TRY(build_unit_plan(st, T, R, G, R->l1_starts.p, R->l1_lists.p, true, false, UNIT_ITER,
+    QBX_TRY(build_unit_plan(st, T, R, G, R->l1_starts.p, R->l1_lists.p, true, false,
+                            ctx->tune[QBX_TUNE_P2P_UNIT_ITER],
                             G->p2p_plan));
     QBX_TRY(build_unit_plan(st, T, R, G, R->l3_starts.p, R->l3_lists.p, false, false,
-                            M2P_UNIT_ITER, G->m2p_plan));
+                            ctx->tune[QBX_TUNE_M2P_UNIT_ITER], G->m2p_plan));
     QBX_TRY(build_unit_plan(st, T, R, G, R->l1_starts.p, R->l1_lists.p, true, true,
-                            P2Q_UNIT_ITER, G->p2q_plan));
+                            ctx->tune[QBX_TUNE_P2QBXL_UNIT_ITER], G->p2q_plan));
     QBX_TRY(build_unit_plan(st, T, R, G, R->l3_starts.p, R->l3_lists.p, false, true,
-                            M2Q_UNIT_ITER, G->m2q_plan));
+                            ctx->tune[QBX_TUNE_M2QBXL_UNIT_ITER], G->m2q_plan));
     QBX_HIP(hipStreamSynchronize(st));
     return QBX_OK;
 }

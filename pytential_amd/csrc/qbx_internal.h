@@ -126,6 +126,9 @@ struct qbx_ctx {
     // qbx_drive_fmm_adjoint: the adjoints of the multipoles, locals, QBX
     // expansions and tree-order potentials
     qbx::dbuf<double> ws_adj_m, ws_adj_l, ws_adj_q, ws_adj_p;
+    // tuning knobs (QBX_TUNE_*): unit-plan list items per lane, L2QBXL lanes
+    int tune[QBX_TUNE_COUNT] = {qbx::UNIT_ITER, qbx::M2P_UNIT_ITER, qbx::P2Q_UNIT_ITER,
+                                qbx::M2Q_UNIT_ITER, 4};
     std::vector<double> tab3_key;
     // side streams of qbx_drive_fmm (default priority; shared by every
     // context on the device, see side_streams) and its fork/join events
