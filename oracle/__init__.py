@@ -112,6 +112,8 @@ def _declare(L):
     L.orc_p2p.argtypes = [ct.POINTER(KParams), ct.c_int, P, P, P, ct.c_int, P, P]
     L.orc_bessel_jy.argtypes = [ct.c_int, ct.c_double, P, P]
     L.orc_sh3d.argtypes = [ct.c_int, P, P, P]
+    L.orc_gaunt3.argtypes = [ct.c_int] * 5
+    L.orc_gaunt3.restype = ct.c_double
     L.orc_tsqbx.argtypes = [ct.c_int, ct.c_int, ct.c_double, ct.c_double, ct.c_int,
                             ct.c_int, P, ct.c_int, P, ct.c_int, P,
                             ct.c_int, P, P, P, P, P, P, P, P, P, P, P, P]
@@ -257,7 +259,8 @@ def refine_flags_for_failed(src1, tunnel, el_starts, targets, status):
     return out
 
 
-KERNEL_CODES = {("laplace", 2): 0, ("helmholtz", 2): 1, ("laplace", 3): 2}
+KERNEL_CODES = {("laplace", 2): 0, ("helmholtz", 2): 1, ("laplace", 3): 2,
+                ("helmholtz", 3): 3}
 
 
 def kparams(kernel="laplace", k=0.0, source_kind="S", qbx_order=4,
@@ -341,6 +344,12 @@ def e2e(kp, kernel, kind, p1, p2, c1, r1, c2, r2, coeffs):
     L.orc_e2e(ct.byref(kp), E2E_KIND[kind], p1, p2, _ptr(_f64(c1)), r1,
               _ptr(_f64(c2)), r2, _ptr(cf), _ptr(out))
     return out
+
+
+def gaunt3(n, m, nu, mu, l):
+    """int Y_n^m conj(Y_nu^mu) conj(Y_l^(m-mu)) over the unit sphere
+    (orthonormal harmonics, Condon-Shortley phase; fmmh3d_oracle.c)."""
+    return lib().orc_gaunt3(n, m, nu, mu, l)
 
 
 def p2p(kp, sources, normals, strengths, targets):

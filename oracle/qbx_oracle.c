@@ -1112,6 +1112,7 @@ int orc_p2e(const orc_kparams *kp, int local, int order, const double *c, double
             int ns, const double *src, const double *nrm, const double *str, double *coeffs)
 {
     if (kp->kernel == 2) return orc_p2e3(kp, local, order, c, ns, src, nrm, str, coeffs);
+    if (kp->kernel == 3) return orc_p2eh3(kp, local, order, c, ns, src, nrm, str, coeffs);
     int nc = ncoeff_of(kp->kernel, order);
     cplx *cf = (cplx *)coeffs;
     for (int i = 0; i < nc; ++i) cf[i] = 0;
@@ -1128,6 +1129,7 @@ int orc_e2p(const orc_kparams *kp, int local, int order, const double *c, double
             const double *coeffs, int nt, const double *tgt, double *pot)
 {
     if (kp->kernel == 2) return orc_e2p3(local, order, c, coeffs, nt, tgt, pot);
+    if (kp->kernel == 3) return orc_e2ph3(kp, local, order, c, coeffs, nt, tgt, pot);
     cplx cc = c[0] + JI * c[1];
     for (int t = 0; t < nt; ++t) {
         cplx v = e2p_at(kp, local, order, cc, rscale, (const cplx *)coeffs, cpt(tgt, nt, t),
@@ -1142,6 +1144,7 @@ int orc_e2e(const orc_kparams *kp, int kind, int p1, int p2,
             const double *src, double *tgt)
 {
     if (kp->kernel == 2) return orc_e2e3(kind, p1, p2, c1, c2, src, tgt);
+    if (kp->kernel == 3) return orc_e2eh3(kp, kind, p1, p2, c1, c2, src, tgt);
     int nc = ncoeff_of(kp->kernel, p2);
     cplx *b = (cplx *)tgt;
     for (int i = 0; i < nc; ++i) b[i] = 0;
@@ -1154,6 +1157,7 @@ int orc_p2p(const orc_kparams *kp, int ns, const double *src, const double *nrm,
             const double *str, int nt, const double *tgt, double *pot)
 {
     if (kp->kernel == 2) return orc_p2p3(kp, ns, src, nrm, str, nt, tgt, pot);
+    if (kp->kernel == 3) return orc_p2ph3(kp, ns, src, nrm, str, nt, tgt, pot);
 #pragma omp parallel for schedule(static)
     for (int t = 0; t < nt; ++t) {
         cplx acc = 0, tt = cpt(tgt, nt, t);
@@ -1207,6 +1211,10 @@ int orc_qbx_fmm_ex(const orc_tree *t, const orc_trav *tr, const orc_kparams *kp,
                    const double *targets, const int *uttc, double *result, double *stage_times,
                    double *mpole_out, double *local_out, double *qexp_out)
 {
+    if (t->dim == 3 && kp->kernel == 3)
+        return orc_qbx_fmmh3d_ex(t, tr, kp, sources, normals, strengths, ncenters, centers,
+                                 radii, targets, uttc, result, stage_times, mpole_out,
+                                 local_out, qexp_out);
     if (t->dim == 3)
         return orc_qbx_fmm3d_ex(t, tr, kp, sources, normals, strengths, ncenters, centers,
                                 radii, targets, uttc, result, stage_times, mpole_out,
@@ -1475,6 +1483,9 @@ int orc_direct_qbx(int dim, const orc_kparams *kp,
 {
     if (dim == 3) {
         if (kp->nthreads > 0) omp_set_num_threads(kp->nthreads);
+        if (kp->kernel == 3)
+            return orc_direct_qbxh3d(kp, ns, src, nrm, str, ncenters, centers, nt, tgt, ttc,
+                                     result);
         return orc_direct_qbx3d(kp, ns, src, nrm, str, ncenters, centers, nt, tgt, ttc,
                                 result);
     }

@@ -81,7 +81,7 @@ int orc_refine_flags_for_failed(int dim, int nsrc1, const double *src1,
 
 /* QBX-FMM and direct QBX (2D Laplace / Helmholtz, S or D source kernel) */
 typedef struct {
-    int kernel;            /* 0 laplace2d, 1 helmholtz2d, 2 laplace3d */
+    int kernel;            /* 0 laplace2d, 1 helmholtz2d, 2 laplace3d, 3 helmholtz3d */
     double k;              /* helmholtz wave number (real) */
     int source_kind;       /* 0 single layer (charge), 1 double layer (dipole) */
     int qbx_order;
@@ -162,6 +162,32 @@ int orc_p2p3(const orc_kparams *kp, int nsources, const double *sources,
              int ntargets, const double *targets, double *pot);
 /* regular / irregular solid harmonics, interleaved complex [(p+1)^2] */
 void orc_sh3d(int p, const double *x, double *ups, double *the);
+
+/* 3D Helmholtz (fmmh3d_oracle.c, kernel code 3): same contracts, spherical
+ * waves R = j_n Y_n^m (locals), S = h_n Y_n^m (multipoles) */
+int orc_qbx_fmmh3d_ex(const orc_tree *t, const orc_trav *tr, const orc_kparams *kp,
+                      const double *sources, const double *normals, const double *strengths,
+                      int ncenters, const double *centers, const double *radii,
+                      const double *targets, const int *user_target_to_center,
+                      double *result, double *stage_times,
+                      double *mpole_out, double *local_out, double *qexp_out);
+int orc_direct_qbxh3d(const orc_kparams *kp, int nsources, const double *sources,
+                      const double *normals, const double *strengths, int ncenters,
+                      const double *centers, int ntargets, const double *targets,
+                      const int *target_to_center, double *result);
+int orc_p2eh3(const orc_kparams *kp, int local, int order, const double *c, int nsources,
+              const double *sources, const double *normals, const double *strengths,
+              double *coeffs);
+int orc_e2ph3(const orc_kparams *kp, int local, int order, const double *c,
+              const double *coeffs, int ntargets, const double *targets, double *pot);
+int orc_e2eh3(const orc_kparams *kp, int kind, int src_order, int tgt_order,
+              const double *csrc, const double *ctgt, const double *src_coeffs,
+              double *tgt_coeffs);
+int orc_p2ph3(const orc_kparams *kp, int nsources, const double *sources,
+              const double *normals, const double *strengths,
+              int ntargets, const double *targets, double *pot);
+/* Gaunt coefficient int Y_n^m conj(Y_nu^mu) conj(Y_l^(m-mu)) */
+double orc_gaunt3(int n, int m, int nu, int mu, int l);
 
 /* Bessel helpers (for tests): J_n, Y_n for n = 0..nmax at real x */
 void orc_bessel_jy(int nmax, double x, double *j, double *y);
