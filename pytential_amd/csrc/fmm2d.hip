@@ -3261,6 +3261,7 @@ extern "C" {
 long qbx_expansion_size(const qbx_kernel_desc *d, const qbx_tree *T)
 {
     if (d && d->kernel == QBX_LAPLACE3D) return f3_expansion_size(d, T);
+    if (d && d->kernel == QBX_HELMHOLTZ3D) return h3_expansion_size(d, T);
     KD kd;
     if (!T || make_kd(d, T, &kd) != QBX_OK) return -1;
     return kd.lcoef[kd.nlevels] * kd.nr;
@@ -3270,6 +3271,7 @@ int qbx_expansion_level_offsets(const qbx_kernel_desc *d, const qbx_tree *T, lon
 {
     QBX_CHECK(T && offsets, "qbx_expansion_level_offsets: null argument");
     if (d && d->kernel == QBX_LAPLACE3D) return f3_expansion_level_offsets(d, T, offsets);
+    if (d && d->kernel == QBX_HELMHOLTZ3D) return h3_expansion_level_offsets(d, T, offsets);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd));
     for (int l = 0; l <= kd.nlevels; ++l) offsets[l] = kd.lcoef[l] * kd.nr;
@@ -3279,6 +3281,7 @@ int qbx_expansion_level_offsets(const qbx_kernel_desc *d, const qbx_tree *T, lon
 int qbx_ncoeff_qbx(const qbx_kernel_desc *d)
 {
     if (d && d->kernel == QBX_LAPLACE3D) return f3_ncoeff_qbx(d);
+    if (d && d->kernel == QBX_HELMHOLTZ3D) return h3_ncoeff_qbx(d);
     KD kd;
     if (make_kd(d, nullptr, &kd) != QBX_OK) return -1;
     return kd.ncq;
@@ -3291,6 +3294,8 @@ int qbx_form_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx
     QBX_CHECK(ctx && T && R && str && mpole, "qbx_form_multipoles: null argument");
     if (d && d->kernel == QBX_LAPLACE3D)
         return f3_form_multipoles(ctx, stream, T, R, d, str, nrm, mpole);
+    if (d && d->kernel == QBX_HELMHOLTZ3D)
+        return h3_form_multipoles(ctx, stream, T, R, d, str, nrm, mpole);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd));
     QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
@@ -3335,6 +3340,8 @@ static int coarsen_multipoles_from(qbx_ctx *ctx, void *stream, const qbx_tree *T
     QBX_CHECK(ctx && T && R && mpole, "qbx_coarsen_multipoles: null argument");
     if (d && d->kernel == QBX_LAPLACE3D)
         return f3_coarsen_multipoles(ctx, stream, T, R, d, mpole, lmin);
+    if (d && d->kernel == QBX_HELMHOLTZ3D)
+        return h3_coarsen_multipoles(ctx, stream, T, R, d, mpole, lmin);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd, R->nsep));
     const double *B;
@@ -3394,6 +3401,8 @@ static int eval_targets(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx
     QBX_CHECK(ctx && T && R && G && pot, "stage: null argument");
     if (d && d->kernel == QBX_LAPLACE3D)
         return f3_eval_targets(ctx, stream, T, R, G, d, mode, str, nrm, expn, pot, accumulate);
+    if (d && d->kernel == QBX_HELMHOLTZ3D)
+        return h3_eval_targets(ctx, stream, T, R, G, d, mode, str, nrm, expn, pot, accumulate);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd));
     hipStream_t st = S(stream);
@@ -3509,6 +3518,8 @@ int qbx_multipole_to_local(qbx_ctx *ctx, void *stream, const qbx_tree *T, const 
     QBX_CHECK(ctx && T && R && mpole && local, "qbx_multipole_to_local: null argument");
     if (d && d->kernel == QBX_LAPLACE3D)
         return f3_multipole_to_local(ctx, stream, T, R, d, mpole, local);
+    if (d && d->kernel == QBX_HELMHOLTZ3D)
+        return h3_multipole_to_local(ctx, stream, T, R, d, mpole, local);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd, R->nsep));
     const double *B;
@@ -3589,6 +3600,8 @@ int qbx_form_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_tra
     QBX_CHECK(ctx && T && R && str && local, "qbx_form_locals: null argument");
     if (d && d->kernel == QBX_LAPLACE3D)
         return f3_form_locals(ctx, stream, T, R, d, str, nrm, local, accumulate);
+    if (d && d->kernel == QBX_HELMHOLTZ3D)
+        return h3_form_locals(ctx, stream, T, R, d, str, nrm, local, accumulate);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd));
     QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
@@ -3647,6 +3660,8 @@ static int refine_locals_from(qbx_ctx *ctx, void *stream, const qbx_tree *T, con
     QBX_CHECK(ctx && T && R && local, "qbx_refine_locals: null argument");
     if (d && d->kernel == QBX_LAPLACE3D)
         return f3_refine_locals(ctx, stream, T, R, d, local, lmin);
+    if (d && d->kernel == QBX_HELMHOLTZ3D)
+        return h3_refine_locals(ctx, stream, T, R, d, local, lmin);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd, R->nsep));
     const double *B;
@@ -3701,6 +3716,8 @@ int qbx_form_global_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, co
     QBX_CHECK(ctx && T && R && G && str && qexp, "qbx_form_global_qbx_locals: null argument");
     if (d && d->kernel == QBX_LAPLACE3D)
         return f3_form_global_qbx_locals(ctx, stream, T, R, G, d, str, nrm, qexp, accumulate);
+    if (d && d->kernel == QBX_HELMHOLTZ3D)
+        return h3_form_global_qbx_locals(ctx, stream, T, R, G, d, str, nrm, qexp, accumulate);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd));
     QBX_CHECK(!kd.dip || nrm, "double layer needs normals");
@@ -3753,6 +3770,9 @@ int qbx_translate_box_multipoles_to_qbx_local(qbx_ctx *ctx, void *stream, const 
     QBX_CHECK(ctx && T && R && G && mpole && qexp, "M2QBXL: null argument");
     if (d && d->kernel == QBX_LAPLACE3D)
         return f3_translate_box_multipoles_to_qbx_local(ctx, stream, T, R, G, d, mpole, qexp,
+                                                        accumulate);
+    if (d && d->kernel == QBX_HELMHOLTZ3D)
+        return h3_translate_box_multipoles_to_qbx_local(ctx, stream, T, R, G, d, mpole, qexp,
                                                         accumulate);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd));
@@ -3809,6 +3829,9 @@ int qbx_translate_box_local_to_qbx_local(qbx_ctx *ctx, void *stream, const qbx_t
     QBX_CHECK(ctx && T && R && G && local && qexp, "L2QBXL: null argument");
     if (d && d->kernel == QBX_LAPLACE3D)
         return f3_translate_box_local_to_qbx_local(ctx, stream, T, R, G, d, local, qexp,
+                                                   accumulate);
+    if (d && d->kernel == QBX_HELMHOLTZ3D)
+        return h3_translate_box_local_to_qbx_local(ctx, stream, T, R, G, d, local, qexp,
                                                    accumulate);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd));
@@ -3893,6 +3916,8 @@ int qbx_eval_qbx_expansions(qbx_ctx *ctx, void *stream, const qbx_tree *T, const
     QBX_CHECK(ctx && T && G && qexp && pot, "qbx_eval_qbx_expansions: null argument");
     if (d && d->kernel == QBX_LAPLACE3D)
         return f3_eval_qbx_expansions(ctx, stream, T, G, d, qexp, pot);
+    if (d && d->kernel == QBX_HELMHOLTZ3D)
+        return h3_eval_qbx_expansions(ctx, stream, T, G, d, qexp, pot);
     KD kd;
     QBX_TRY(make_kd(d, T, &kd));
     hipStream_t st = S(stream);
@@ -4123,11 +4148,15 @@ static int drive_impl(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_t
     auto m2m = [&](hipStream_t s) {
         if (d->kernel == QBX_LAPLACE3D)
             return f3_coarsen_multipoles(ctx, s, T, R, d, M, R->mpole_min_level, b.allreal);
+        if (d->kernel == QBX_HELMHOLTZ3D)
+            return h3_coarsen_multipoles(ctx, s, T, R, d, M, R->mpole_min_level);
         return coarsen_multipoles_from(ctx, s, T, R, d, M, R->mpole_min_level);
     };
     auto l2l = [&](hipStream_t s) {
         if (d->kernel == QBX_LAPLACE3D)
             return f3_refine_locals(ctx, s, T, R, d, L, R->local_min_level, b.allreal);
+        if (d->kernel == QBX_HELMHOLTZ3D)
+            return h3_refine_locals(ctx, s, T, R, d, L, R->local_min_level);
         return refine_locals_from(ctx, s, T, R, d, L, R->local_min_level);
     };
     if (!multistream) {

@@ -1,0 +1,1372 @@
+// QBX-FMM wrangler stages for 3D Helmholtz (pytential/qbx/fmm.py drive_fmm and
+// pytential/qbx/interactions.py), complex128, G(x, y) = e^{ik|x-y|} /
+// (4 pi |x-y|) with real k > 0.
+//
+// Expansions in spherical waves (coefficient (n, m) at n*n + n + m):
+//   Y_n^m     orthonormal spherical harmonics, Condon-Shortley phase,
+//             Y_n^{-m} = (-1)^m conj(Y_n^m)
+//   R_n^m(x) = j_n(k|x|) Y_n^m(x/|x|),  S_n^m(x) = h_n(k|x|) Y_n^m(x/|x|)
+//   multipole about c:  M_n^m = ik sum q (-1)^m R_n^{-m}(s - c),  phi = sum M S(x - c)
+//   local about c:      L_n^m = ik sum q (-1)^m S_n^{-m}(s - c),  phi = sum L R(x - c)
+//   translations, t = c2 - c1 (M2M and L2L with X = R, M2L / M2QBXL with S):
+//     b_{nu,mu} = sum_{n,m} C^X_{nu mu, n m}(t) a_{n,m},
+//     C^X_{nu mu, n m}(t) = 4 pi sum_l (-1)^((nu+l-n)/2) G(n,m; nu,mu; l,m-mu) X_l^{m-mu}(t)
+//   with the Gaunt coefficients G = int Y_n^m conj(Y_nu^mu) conj(Y_l^(m-mu))
+//   (l = |n-nu|, |n-nu|+2, ..., n+nu), computed on the device by
+//   Gauss-Legendre quadrature of normalised Legendre functions (exact: the
+//   integrands are polynomials of degree n + nu + l).
+//   Double layer (q nu . grad_s): the ladder relations
+//     d_z F_n^m = k (A_{n-1}^m F_{n-1}^m - A_n^m F_{n+1}^m),
+//     (d_x + i d_y) F_n^m = k (a F_{n+1}^{m+1} + b F_{n-1}^{m+1}),
+//     (d_x - i d_y) F_n^m = -k (c F_{n+1}^{m-1} + d F_{n-1}^{m-1})
+//   (coefficients in nu_grad below), F = R or S.
+//
+// Storage (HBM): box expansions level-compact like the 3D Laplace stages,
+// (p_l+1)^2 complex per box of level l (times nr), scaled per level by
+// s_l = min(1, k h_l) (h_l the box size): multipoles hold M_n^m / s^n,
+// locals L_n^m s^n, so that j_n and h_n of small arguments stay O(1) at
+// every order.  QBX expansions [ncenters][(p_q+1)^2][nr] are unscaled.
+//
+// Box translations use dense tables, one complex (p_in+1)^2 x (p_out+1)^2
+// matrix per level and offset class (8 child octants for M2M and L2L, the
+// (4 nsep + 3)^3 integer offsets for M2L), built on the device from the
+// Gaunt table and cached in the context per (k, tree extent, orders, nsep).
+// M2QBXL and L2QBXL translate to arbitrary centers: the matrix entries are
+// summed from the Gaunt table and X_l^la(t) on the fly.
+//
+// Thread mappings:
+//  P2M, P2L, P2QBXL  a block per output expansion (box or QBX center); each
+//                    chunk of sources is expanded into R or S waves in LDS
+//                    (a thread per (source, m) column), then a thread per
+//                    output coefficient sums the chunk
+//  M2M, M2L, L2L     a block per output box, a thread per output
+//                    coefficient; each entry's input expansion staged in
+//                    LDS, the table read row by row (coalesced)
+//  P2P, M2P, L2P     a 64-lane block per target box, lanes over targets
+//  M2QBXL, L2QBXL    a 64-lane block per QBX center; per input box X(t) in
+//                    LDS, lanes over (output coefficient, input slice)
+//  QBXL2P            a thread per (center, target) pair
+#include "qbx_internal.h"
+#include "qbx_math.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace qbx {
+namespace h3 {
+
+constexpr int MAXP = 20;    // box expansion orders
+constexpr int MAXPQ = 8;    // QBX expansion orders
+constexpr int FML = 21;     // levels of a 3D tree
+constexpr int NRC_B = 4;    // right-hand-side chunk of the box stages
+constexpr int NRC_E = 8;    // right-hand-side chunk of the evaluations
+constexpr int NRC_Q = 2;    // right-hand-side chunk of the QBX translations
+constexpr double FOURPI = 12.566370614359172954;
+constexpr double INV4PI = 0.079577471545947667884;
+constexpr long TAB_BYTES_MAX = 32L << 30;  // translation tables per context
+
+__host__ __device__ constexpr int IDX(int n, int m) { return n * n + n + m; }
+
+struct HD {
+    int dip, pq, nlevels, ncq, nsep, W, pmax, pg, nr, r0, rc;
+    double k;
+    int order[FML];
+    int ncl[FML];
+    int lbox[FML + 1];
+    long lcoef[FML + 1];
+    double h[FML];  // box size per level
+    double s[FML];  // expansion scale per level
+    const double *gaunt;  // [(pg+1)^2 (n,m)][(pg+1)^2 (nu,mu)][pg+1]
+    const double *tab;    // every translation table (complex)
+    long m2m_off[FML], l2l_off[FML], m2l_off[FML];  // complex offsets, -1: none
+};
+
+struct TD {
+    int nboxes;
+    const int *level, *parent, *child, *coords;
+    const double *centers;  // [3][nboxes]
+    const int *sstart, *scount, *scumul;
+};
+
+struct v3 {
+    double x, y, z;
+};
+__device__ __forceinline__ v3 ld3(const double *p, long n, long i)
+{
+    return {p[i], p[n + i], p[2 * n + i]};
+}
+__device__ __forceinline__ v3 sub(v3 a, v3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ v3 boxc(const TD &t, int b) { return ld3(t.centers, t.nboxes, b); }
+
+__device__ __forceinline__ long ecoef_off(const HD &hd, int b, int lv)
+{
+    return 2 * (hd.lcoef[lv] + (long)(b - hd.lbox[lv]) * hd.ncl[lv]) * hd.nr;
+}
+
+__device__ __forceinline__ void nm_of(int i, int &n, int &m)
+{
+    n = (int)sqrt((double)i);
+    while (n * n > i) --n;
+    while ((n + 1) * (n + 1) <= i) ++n;
+    m = i - n * n - n;
+}
+
+__device__ __forceinline__ cplx sgn_conj(cplx v, int am)  // (-1)^m conj(v)
+{
+    return (am & 1) ? cplx{-v.x, v.y} : cplx{v.x, -v.y};
+}
+
+// ---- special functions (real argument) ------------------------------------
+// j_n(x), n = 0..p (p >= 1 slots written), by Miller's backward recurrence
+// normalised by j_0 or j_1 (the larger)
+__device__ void sph_j_seq(int p, double x, double *j)
+{
+    const int pp = p < 1 ? 1 : p;
+    if (x < 1e-300) {
+        for (int n = 0; n <= pp; ++n) j[n] = n == 0 ? 1.0 : 0.0;
+        return;
+    }
+    const double mx = fmax((double)pp, x);
+    const int N = (int)mx + 30 + (int)sqrt(40.0 * mx);
+    const double ix = 1.0 / x;
+    double fp1 = 0.0, f = 1e-300;
+    for (int n = N; n >= 0; --n) {
+        if (n <= pp) j[n] = f;
+        const double fm1 = (2.0 * n + 1.0) * ix * f - fp1;
+        fp1 = f;
+        f = fm1;
+        if (fabs(f) > 1e200) {
+            f *= 1e-200;
+            fp1 *= 1e-200;
+            for (int q = n; q <= pp; ++q) j[q] *= 1e-200;
+        }
+    }
+    double sn, cs;
+    sincos(x, &sn, &cs);
+    const double j0 = sn * ix, j1 = (sn * ix - cs) * ix;
+    const double sc = fabs(j0) >= fabs(j1) ? j0 / j[0] : j1 / j[1];
+    for (int n = 0; n <= pp; ++n) j[n] *= sc;
+}
+
+// radial sequence into rad (complex, n = 0..p): j_n (sing 0) or
+// h_n = j_n + i y_n (sing 1, y_n by the forward recurrence); jtmp holds
+// p + 2 doubles
+__device__ void radial_seq(int sing, int p, double x, cplx *rad, double *jtmp)
+{
+    sph_j_seq(p, x, jtmp);
+    if (!sing) {
+        for (int n = 0; n <= p; ++n) rad[n] = {jtmp[n], 0.0};
+        return;
+    }
+    double sn, cs;
+    sincos(x, &sn, &cs);
+    const double ix = 1.0 / x;
+    double ym2 = -cs * ix, ym1 = (-cs * ix - sn) * ix;
+    for (int n = 0; n <= p; ++n) {
+        double y;
+        if (n == 0) {
+            y = ym2;
+        } else if (n == 1) {
+            y = ym1;
+        } else {
+            y = (2.0 * n - 1.0) * ix * ym1 - ym2;
+            ym2 = ym1;
+            ym1 = y;
+        }
+        rad[n] = {jtmp[n], y};
+    }
+}
+
+// Column m >= 0 of Y_n^m(d / |d|) for n = m..p: calls f(n, Y_n^m).
+// Q_m^m = -sqrt((2m+1)/(2m)) Q_{m-1}^{m-1} (Q_0^0 = 1/sqrt(4 pi)),
+// Q_{m+1}^m = sqrt(2m+3) z Q_m^m, Q_n^m = a (z Q_{n-1}^m - b Q_{n-2}^m),
+// Y_n^m = Q_n^m ((x + i y) / r)^m.
+template <class F>
+__device__ __forceinline__ void ylm_column(int m, int p, v3 d, F f)
+{
+    const double r = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+    const double ir = r > 0.0 ? 1.0 / r : 0.0;
+    const double z = r > 0.0 ? d.z * ir : 1.0;
+    const cplx w = {d.x * ir, d.y * ir};
+    double qmm = 0.28209479177387814347;  // 1 / sqrt(4 pi)
+    cplx wm = {1.0, 0.0};
+    for (int i = 1; i <= m; ++i) {
+        qmm *= -sqrt((2.0 * i + 1.0) / (2.0 * i));
+        wm = wm * w;
+    }
+    double a2 = 0.0, a1 = qmm;
+    f(m, qmm * wm);
+    for (int n = m + 1; n <= p; ++n) {
+        double q;
+        if (n == m + 1) {
+            q = sqrt(2.0 * m + 3.0) * z * qmm;
+        } else {
+            const double a = sqrt((4.0 * n * n - 1.0) / ((double)n * n - (double)m * m));
+            const double b = sqrt(((n - 1.0) * (n - 1.0) - (double)m * m) /
+                                  (4.0 * (n - 1.0) * (n - 1.0) - 1.0));
+            q = a * (z * a1 - b * a2);
+        }
+        f(n, q * wm);
+        a2 = a1;
+        a1 = q;
+    }
+}
+
+__device__ __forceinline__ cplx getf(const cplx *F, int p, int n, int m)
+{
+    return (n < 0 || n > p || m < -n || m > n) ? cplx{0.0, 0.0} : F[IDX(n, m)];
+}
+
+// nu . grad F_n^m from the ladder relations; F holds orders <= pf (>= n + 1)
+__device__ cplx nu_grad(const cplx *F, int pf, double k, int n, int m, v3 nu)
+{
+    const double dn = n;
+    const double An = sqrt(((dn + 1.0) * (dn + 1.0) - (double)m * m) /
+                           ((2.0 * dn + 1.0) * (2.0 * dn + 3.0)));
+    const double An1 = (n >= 1 && abs(m) <= n - 1)
+                           ? sqrt((dn * dn - (double)m * m) / ((2.0 * dn - 1.0) * (2.0 * dn + 1.0)))
+                           : 0.0;
+    const cplx dz = k * (An1 * getf(F, pf, n - 1, m) - An * getf(F, pf, n + 1, m));
+    const double a = sqrt((dn + m + 1.0) * (dn + m + 2.0) / ((2.0 * dn + 1.0) * (2.0 * dn + 3.0)));
+    const double b = n >= 1 ? sqrt(fmax(0.0, (dn - m) * (dn - m - 1.0)) /
+                                   ((2.0 * dn - 1.0) * (2.0 * dn + 1.0)))
+                            : 0.0;
+    const double c = sqrt((dn - m + 1.0) * (dn - m + 2.0) / ((2.0 * dn + 1.0) * (2.0 * dn + 3.0)));
+    const double d = n >= 1 ? sqrt(fmax(0.0, (dn + m) * (dn + m - 1.0)) /
+                                   ((2.0 * dn - 1.0) * (2.0 * dn + 1.0)))
+                            : 0.0;
+    const cplx dp = k * (a * getf(F, pf, n + 1, m + 1) + b * getf(F, pf, n - 1, m + 1));
+    const cplx dm = -k * (c * getf(F, pf, n + 1, m - 1) + d * getf(F, pf, n - 1, m - 1));
+    const cplx nm = {0.5 * nu.x, -0.5 * nu.y}, np = {0.5 * nu.x, 0.5 * nu.y};
+    return nu.z * dz + nm * dp + np * dm;
+}
+
+__device__ __forceinline__ double ipow(double s, int n)
+{
+    double r = 1.0;
+    for (int i = 0; i < n; ++i) r *= s;
+    return r;
+}
+
+// ---- Gaunt table ----------------------------------------------------------
+// g[(IDX(n,m) ncg + IDX(nu,mu)) (pg+1) + i] = G(n,m; nu,mu; |n-nu| + 2i, m-mu)
+// from 2 pi sum_q w_q Pb_n^m(x_q) Pb_nu^mu(x_q) Pb_l^(m-mu)(x_q), Pb the
+// normalised associated Legendre functions at the nodes ([nq][(2pg+1)^2]).
+__global__ void kh_gaunt(int pg, int nq, const double *wq, const double *pb, double *g)
+{
+    const int ncg = (pg + 1) * (pg + 1), L = 2 * pg, ncl = (L + 1) * (L + 1);
+    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (e >= (long)ncg * ncg) return;
+    const int j = (int)(e / ncg), i = (int)(e % ncg);
+    int n, m, nu, mu;
+    nm_of(j, n, m);
+    nm_of(i, nu, mu);
+    double *out = g + e * (pg + 1);
+    const int imax = min(n, nu);
+    for (int ii = 0; ii <= pg; ++ii) {
+        double acc = 0.0;
+        const int l = abs(n - nu) + 2 * ii;
+        if (ii <= imax && abs(m - mu) <= l) {
+            for (int q = 0; q < nq; ++q) {
+                const double *P = pb + (long)q * ncl;
+                acc += wq[q] * P[IDX(n, m)] * P[IDX(nu, mu)] * P[IDX(l, m - mu)];
+            }
+            acc *= 6.283185307179586477;
+        }
+        out[ii] = acc;
+    }
+}
+
+// ---- wave tables ----------------------------------------------------------
+// X_l^la(t), l <= L, into X (LDS, (L+1)^2 complex) by the block; rad and
+// jtmp scratch of L + 3 entries; sing 0: R, 1: S
+__device__ void waves_block(int sing, int L, double k, v3 t, cplx *X, cplx *rad, double *jtmp)
+{
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        const double r = sqrt(t.x * t.x + t.y * t.y + t.z * t.z);
+        radial_seq(sing, L, k * r, rad, jtmp);
+    }
+    __syncthreads();
+    for (int m = tid; m <= L; m += blockDim.x)
+        ylm_column(m, L, t, [&](int n, cplx y) {
+            X[IDX(n, m)] = rad[n] * y;
+            if (m > 0) X[IDX(n, -m)] = rad[n] * sgn_conj(y, m);
+        });
+    __syncthreads();
+}
+
+// Translation tables: tab[cls][j = (n,m) < ncin][i = (nu,mu) < ncout] =
+// C^X_{i,j}(t_cls) sout^nu / sin^n (the scales of the stored expansions).
+__global__ void __launch_bounds__(256) kh_build_tab(int sing, double k, int pin, int pout,
+                                                    double sin_, double sout,
+                                                    const double *tv, int pg,
+                                                    const double *gaunt, double *tab)
+{
+    __shared__ double Xs[2 * 41 * 41];
+    __shared__ double rads[2 * 44];
+    __shared__ double jt[44];
+    const int cls = blockIdx.x;
+    const v3 t = {tv[3 * cls], tv[3 * cls + 1], tv[3 * cls + 2]};
+    const int ncin = (pin + 1) * (pin + 1), ncout = (pout + 1) * (pout + 1);
+    double *T = tab + 2L * cls * ncin * ncout;
+    const bool zero = t.x == 0.0 && t.y == 0.0 && t.z == 0.0;
+    if (zero) {
+        for (long e = threadIdx.x; e < (long)ncin * ncout; e += blockDim.x) {
+            T[2 * e] = 0.0;
+            T[2 * e + 1] = 0.0;
+        }
+        return;
+    }
+    cplx *X = reinterpret_cast<cplx *>(Xs);
+    waves_block(sing, pin + pout, k, t, X, reinterpret_cast<cplx *>(rads), jt);
+    const int ncg = (pg + 1) * (pg + 1);
+    for (long e = threadIdx.x; e < (long)ncin * ncout; e += blockDim.x) {
+        const int j = (int)(e / ncout), i = (int)(e % ncout);
+        int n, m, nu, mu;
+        nm_of(j, n, m);
+        nm_of(i, nu, mu);
+        const double *g = gaunt + ((long)j * ncg + i) * (pg + 1);
+        cplx c = {0.0, 0.0};
+        const int imax = min(n, nu);
+        for (int ii = 0; ii <= imax; ++ii) {
+            const int l = abs(n - nu) + 2 * ii;
+            if (abs(m - mu) > l) continue;
+            const int ex = (nu + l - n) / 2;
+            c += ((ex & 1) ? -g[ii] : g[ii]) * X[IDX(l, m - mu)];
+        }
+        const double sc = FOURPI * ipow(sout, nu) / ipow(sin_, n);
+        T[2 * e] = sc * c.x;
+        T[2 * e + 1] = sc * c.y;
+    }
+}
+
+// ---- P2M, P2L, P2QBXL -----------------------------------------------------
+// MODE 0: P2M, items = source boxes, a box's own sources, out = multipoles;
+// MODE 1: P2L, items = target-or-target-parent boxes, list-4 sources;
+// MODE 2: P2QBXL, items = global QBX centers, list-1 sources of the
+// center's target box, out = QBX expansions (order pq, unscaled).
+// Dynamic LDS: CH sources x [(pf+1)^2 waves, pf+1 radial, rc strengths].
+template <int MODE, int NRC>
+__global__ void __launch_bounds__(512) kh_p2e(HD hd, TD t, int nitems, const int *items,
+                                             const int *starts, const int *lists,
+                                             const double *ctrs, int nctr, const int *c2tb,
+                                             const double *src, int ns, const double *str,
+                                             const double *nrm, int CH, int pfmax,
+                                             double *out)
+{
+    extern __shared__ double sh[];
+    const int it = blockIdx.x;
+    if (it >= nitems) return;
+    const int ncf = (pfmax + 1) * (pfmax + 1);
+    cplx *sF = reinterpret_cast<cplx *>(sh);                       // [CH][ncf]
+    cplx *srad = sF + (long)CH * ncf;                               // [CH][pfmax+1]
+    cplx *sq = srad + (long)CH * (pfmax + 1);                       // [CH][NRC]
+    double *snu = reinterpret_cast<double *>(sq + (long)CH * NRC);  // [CH][3]
+    double *sd = snu + 3 * CH;                                      // [CH][3]
+    double *jt = sd + 3 * CH;                                       // [CH][pfmax+3]
+    int B, p, e0, e1, lv = 0;
+    v3 c;
+    double *o;
+    double sc;
+    if (MODE == 2) {
+        const int ci = items[it];
+        const int tb = c2tb[ci];
+        c = ld3(ctrs, nctr, ci);
+        p = hd.pq;
+        e0 = starts[tb];
+        e1 = starts[tb + 1];
+        B = -1;
+        o = out + 2L * ci * hd.ncq * hd.nr;
+        sc = 1.0;
+    } else {
+        B = items[it];
+        lv = t.level[B];
+        c = boxc(t, B);
+        p = hd.order[lv];
+        if (MODE == 0) {
+            e0 = 0;
+            e1 = 1;
+        } else {
+            e0 = starts[it];
+            e1 = starts[it + 1];
+        }
+        o = out + ecoef_off(hd, B, lv);
+        sc = MODE == 0 ? 1.0 / hd.s[lv] : hd.s[lv];  // M / s^n, L s^n
+    }
+    const int pf = p + (hd.dip ? 1 : 0);
+    const int nco = (p + 1) * (p + 1);
+    const int sing = MODE == 0 ? 0 : 1;
+    const int tid = threadIdx.x;
+    int n_o = 0, m_o = 0;
+    if (tid < nco) nm_of(tid, n_o, m_o);
+    cplx acc[NRC];
+#pragma unroll
+    for (int q = 0; q < NRC; ++q) acc[q] = {0.0, 0.0};
+    const double k = hd.k;
+    // walk the source ranges in chunks of CH
+    int e = e0, s = 0, s1 = 0;
+    bool have = false;
+    auto next_range = [&]() {
+        while (e < e1) {
+            const int Sb = MODE == 0 ? B : lists[e];
+            ++e;
+            const int a = t.sstart[Sb], b = a + t.scount[Sb];
+            if (b > a) {
+                s = a;
+                s1 = b;
+                return true;
+            }
+        }
+        return false;
+    };
+    have = next_range();
+    while (have) {
+        // gather up to CH sources (uniform across the block)
+        __syncthreads();
+        int cnt = 0;
+        int gs[64];
+        while (have && cnt < CH) {
+            const int take = min(s1 - s, CH - cnt);
+            for (int q = 0; q < take; ++q) gs[cnt + q] = s + q;
+            cnt += take;
+            s += take;
+            if (s >= s1) have = next_range();
+        }
+        for (int q = tid; q < cnt; q += blockDim.x) {
+            const int si = gs[q];
+            const v3 d = sub(ld3(src, ns, si), c);
+            sd[3 * q] = d.x;
+            sd[3 * q + 1] = d.y;
+            sd[3 * q + 2] = d.z;
+            if (hd.dip) {
+                const v3 nu = ld3(nrm, ns, si);
+                snu[3 * q] = nu.x;
+                snu[3 * q + 1] = nu.y;
+                snu[3 * q + 2] = nu.z;
+            }
+            for (int r = 0; r < NRC; ++r) {
+                cplx qv = {0.0, 0.0};
+                if (r < hd.rc) {
+                    const double *pq_ = str + 2 * ((size_t)si * hd.nr + hd.r0 + r);
+                    qv = {pq_[0], pq_[1]};
+                }
+                sq[q * NRC + r] = qv;
+            }
+            const double rr = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+            radial_seq(sing, pf, k * rr, srad + (long)q * (pfmax + 1), jt + (long)q * (pfmax + 3));
+        }
+        __syncthreads();
+        for (int w = tid; w < cnt * (pf + 1); w += blockDim.x) {
+            const int q = w / (pf + 1), m = w % (pf + 1);
+            const v3 d = {sd[3 * q], sd[3 * q + 1], sd[3 * q + 2]};
+            cplx *F = sF + (long)q * ncf;
+            const cplx *rd = srad + (long)q * (pfmax + 1);
+            ylm_column(m, pf, d, [&](int n, cplx y) {
+                F[IDX(n, m)] = rd[n] * y;
+                if (m > 0) F[IDX(n, -m)] = rd[n] * sgn_conj(y, m);
+            });
+        }
+        __syncthreads();
+        if (tid < nco) {
+            for (int q = 0; q < cnt; ++q) {
+                const cplx *F = sF + (long)q * ncf;
+                cplx f;
+                if (hd.dip) {
+                    const v3 nu = {snu[3 * q], snu[3 * q + 1], snu[3 * q + 2]};
+                    f = nu_grad(F, pf, k, n_o, -m_o, nu);
+                } else {
+                    f = F[IDX(n_o, -m_o)];
+                }
+                if (m_o & 1) f = -f;
+#pragma unroll
+                for (int r = 0; r < NRC; ++r) acc[r] += sq[q * NRC + r] * f;
+            }
+        }
+    }
+    if (tid < nco) {
+        // ik, and the stored scale s^(-n) (multipole) or s^n (local)
+        const double scn = ipow(sc, n_o);
+        const cplx ik = {0.0, k * scn};
+#pragma unroll
+        for (int r = 0; r < NRC; ++r)
+            if (r < hd.rc) {
+                double *dst = o + 2 * ((long)tid * hd.nr + hd.r0 + r);
+                const cplx v = ik * acc[r];
+                dst[0] += v.x;
+                dst[1] += v.y;
+            }
+    }
+}
+
+// ---- M2M, M2L, L2L ---------------------------------------------------------
+// OP 0 M2M: boxes = source parents of level lv, entries = their children
+//           with sources (level lv + 1), table by child octant
+// OP 1 M2L: boxes = target-or-target-parent boxes [i0, i0 + nbox) of level
+//           lv, entries = list 2, table by integer offset class
+// OP 2 L2L: boxes = target-or-target-parent boxes of level lv >= 1, entry =
+//           the parent (level lv - 1), table by the box's octant
+// A thread per output coefficient; dynamic LDS: the entry's input
+// (ncin x NRC complex).
+template <int OP, int NRC>
+__global__ void __launch_bounds__(448) kh_trans(HD hd, TD t, int lv, int i0, int nbox,
+                                                const int *boxes, const int *starts,
+                                                const int *lists, const double *in, double *out)
+{
+    extern __shared__ double sh[];
+    cplx *a = reinterpret_cast<cplx *>(sh);
+    const int ib = blockIdx.x;
+    if (ib >= nbox) return;
+    const int i = i0 + ib;
+    const int B = boxes[i];
+    const int lvin = OP == 0 ? lv + 1 : OP == 2 ? lv - 1 : lv;
+    const int pin = hd.order[lvin], pout = hd.order[lv];
+    const int ncin = (pin + 1) * (pin + 1), ncout = (pout + 1) * (pout + 1);
+    const int tid = threadIdx.x;
+    const int S1 = 2 * hd.W + 1;
+    const int nb = t.nboxes;
+    const int bx = t.coords[B], by = t.coords[nb + B], bz = t.coords[2 * nb + B];
+    cplx acc[NRC];
+#pragma unroll
+    for (int r = 0; r < NRC; ++r) acc[r] = {0.0, 0.0};
+    int ne;
+    if (OP == 0) ne = 8;
+    else if (OP == 1) ne = starts[i + 1] - starts[i];
+    else ne = 1;
+    for (int e = 0; e < ne; ++e) {
+        int A, cls;
+        if (OP == 0) {
+            A = t.child[(long)e * nb + B];
+            if (A < 0 || t.scumul[A] == 0) continue;  // uniform over the block
+            cls = (t.coords[A] & 1) | ((t.coords[nb + A] & 1) << 1) |
+                  ((t.coords[2 * nb + A] & 1) << 2);
+        } else if (OP == 1) {
+            A = lists[starts[i] + e];
+            const int dx = bx - t.coords[A] + hd.W, dy = by - t.coords[nb + A] + hd.W,
+                      dz = bz - t.coords[2 * nb + A] + hd.W;
+            if ((unsigned)dx >= (unsigned)S1 || (unsigned)dy >= (unsigned)S1 ||
+                (unsigned)dz >= (unsigned)S1)
+                continue;
+            cls = (dx * S1 + dy) * S1 + dz;
+        } else {
+            A = t.parent[B];
+            if (A < 0) continue;
+            cls = (bx & 1) | ((by & 1) << 1) | ((bz & 1) << 2);
+        }
+        const long toff = OP == 0 ? hd.m2m_off[lv] : OP == 1 ? hd.m2l_off[lv] : hd.l2l_off[lv];
+        const double *T = hd.tab + 2 * (toff + (long)cls * ncin * ncout);
+        const double *src = in + ecoef_off(hd, A, lvin);
+        __syncthreads();
+        for (int w = tid; w < ncin * NRC; w += blockDim.x) {
+            const int j = w / NRC, r = w % NRC;
+            a[w] = r < hd.rc ? ldc(src + 2 * ((long)j * hd.nr + hd.r0 + r)) : cplx{0.0, 0.0};
+        }
+        __syncthreads();
+        if (tid < ncout) {
+            for (int j = 0; j < ncin; ++j) {
+                const cplx tv = ldc(T + 2 * ((long)j * ncout + tid));
+#pragma unroll
+                for (int r = 0; r < NRC; ++r) acc[r] += tv * a[j * NRC + r];
+            }
+        }
+    }
+    if (tid < ncout) {
+        double *o = out + ecoef_off(hd, B, lv);
+#pragma unroll
+        for (int r = 0; r < NRC; ++r)
+            if (r < hd.rc) {
+                double *dst = o + 2 * ((long)tid * hd.nr + hd.r0 + r);
+                dst[0] += acc[r].x;
+                dst[1] += acc[r].y;
+            }
+    }
+}
+
+// ---- P2P, M2P, L2P ---------------------------------------------------------
+__device__ __forceinline__ cplx p2p_val(double k, bool dip, v3 s, v3 nu, v3 x)
+{
+    const v3 d = sub(x, s);
+    const double r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+    if (r2 == 0.0) return {0.0, 0.0};
+    const double r = sqrt(r2), ir = 1.0 / r;
+    double sn, cs;
+    sincos(k * r, &sn, &cs);
+    const cplx e = {cs * ir * INV4PI, sn * ir * INV4PI};  // e^{ikr} / (4 pi r)
+    if (!dip) return e;
+    const double pr = (nu.x * d.x + nu.y * d.y + nu.z * d.z) * ir * ir;
+    return (e * cplx{1.0, -k * r}) * pr;
+}
+
+// sum over (n, m) of coef * rad[n] Y_n^m(d) (rad already scaled)
+template <int NRC>
+__device__ void sum_waves(const HD &hd, const double *coef, int p, v3 d, const cplx *rad,
+                          cplx *acc)
+{
+    for (int m = 0; m <= p; ++m)
+        ylm_column(m, p, d, [&](int n, cplx y) {
+            const cplx fp = rad[n] * y;
+            const cplx fm = rad[n] * sgn_conj(y, m);
+#pragma unroll
+            for (int r = 0; r < NRC; ++r)
+                if (r < hd.rc) {
+                    acc[r] += ldc(coef + 2 * ((long)IDX(n, m) * hd.nr + hd.r0 + r)) * fp;
+                    if (m > 0)
+                        acc[r] += ldc(coef + 2 * ((long)IDX(n, -m) * hd.nr + hd.r0 + r)) * fm;
+                }
+        });
+}
+
+template <int MODE, int NRC>
+__global__ void __launch_bounds__(64) kh_eval(HD hd, TD t, int ntb, const int *tboxes,
+                                              const int *starts, const int *lists,
+                                              const int *fstart, const int *fcount,
+                                              const int *unfilt, const double *tgt, int nt,
+                                              const double *src, int ns, const double *str,
+                                              const double *nrm, const double *expn,
+                                              double *pot)
+{
+    __shared__ double rads[64 * 2 * (MAXP + 1)];
+    __shared__ double jts[64 * (MAXP + 3)];
+    const int itb = blockIdx.x;
+    if (itb >= ntb) return;
+    const int B = tboxes[itb];
+    const int f0 = fstart[B], nf = fcount[B];
+    if (nf == 0) return;
+    const int lane = threadIdx.x;
+    cplx *rad = reinterpret_cast<cplx *>(rads) + lane * (MAXP + 1);
+    double *jt = jts + lane * (MAXP + 3);
+    const double k = hd.k;
+    for (int f = f0 + lane; f < f0 + nf; f += 64) {
+        const int ti = unfilt[f];
+        const v3 x = ld3(tgt, nt, ti);
+        cplx acc[NRC];
+#pragma unroll
+        for (int r = 0; r < NRC; ++r) acc[r] = {0.0, 0.0};
+        if (MODE == 0) {
+            for (int e = starts[itb]; e < starts[itb + 1]; ++e) {
+                const int Sb = lists[e];
+                const int s0 = t.sstart[Sb], s1 = s0 + t.scount[Sb];
+                for (int s = s0; s < s1; ++s) {
+                    const v3 nu = hd.dip ? ld3(nrm, ns, s) : v3{0, 0, 0};
+                    const cplx g = p2p_val(k, hd.dip != 0, ld3(src, ns, s), nu, x);
+#pragma unroll
+                    for (int r = 0; r < NRC; ++r)
+                        if (r < hd.rc)
+                            acc[r] += g * ldc(str + 2 * ((size_t)s * hd.nr + hd.r0 + r));
+                }
+            }
+        } else {
+            const int eb = MODE == 1 ? starts[itb] : 0;
+            const int ee = MODE == 1 ? starts[itb + 1] : 1;
+            for (int e = eb; e < ee; ++e) {
+                const int D = MODE == 1 ? lists[e] : B;
+                const int lv = t.level[D];
+                const int p = hd.order[lv];
+                const v3 d = sub(x, boxc(t, D));
+                const double rr = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+                radial_seq(MODE == 1 ? 1 : 0, p, k * rr, rad, jt);
+                // stored M / s^n: h_n s^n; stored L s^n: j_n / s^n
+                const double s = MODE == 1 ? hd.s[lv] : 1.0 / hd.s[lv];
+                double sp = 1.0;
+                for (int n = 0; n <= p; ++n) {
+                    rad[n] = sp * rad[n];
+                    sp *= s;
+                }
+                sum_waves<NRC>(hd, expn + ecoef_off(hd, D, lv), p, d, rad, acc);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < NRC; ++r)
+            if (r < hd.rc) {
+                double *dst = pot + 2 * ((size_t)ti * hd.nr + hd.r0 + r);
+                dst[0] += acc[r].x;
+                dst[1] += acc[r].y;
+            }
+    }
+}
+
+// ---- M2QBXL, L2QBXL --------------------------------------------------------
+// MODE 0: M2QBXL over list 3 of the center's target box (C^S);
+// MODE 1: L2QBXL from the target box's own local (C^R).
+template <int MODE, int NRC>
+__global__ void __launch_bounds__(64) kh_e2qbxl(HD hd, TD t, int ng, const int *gcent,
+                                                const int *c2tb, const int *tboxes,
+                                                const int *starts, const int *lists,
+                                                const double *ctrs, int nctr,
+                                                const double *expn, double *qexp)
+{
+    __shared__ double Xs[2 * (MAXP + MAXPQ + 1) * (MAXP + MAXPQ + 1)];
+    __shared__ double as[2 * (MAXP + 1) * (MAXP + 1) * NRC];
+    __shared__ double accs[2 * 96 * NRC];  // (output, slice) slots: nco * parts <= 81
+    __shared__ double rads[2 * (MAXP + MAXPQ + 3)];
+    __shared__ double jt[MAXP + MAXPQ + 4];
+    const int g = blockIdx.x;
+    if (g >= ng) return;
+    const int ci = gcent[g];
+    const int itb = c2tb[ci];
+    const v3 c = ld3(ctrs, nctr, ci);
+    cplx *X = reinterpret_cast<cplx *>(Xs);
+    cplx *a = reinterpret_cast<cplx *>(as);
+    cplx *sacc = reinterpret_cast<cplx *>(accs);
+    const int tid = threadIdx.x;
+    const int pq = hd.pq, nco = hd.ncq;
+    const int parts = nco >= 64 ? 1 : 64 / nco;
+    const int ncg = (hd.pg + 1) * (hd.pg + 1);
+    for (int w = tid; w < 96 * NRC; w += 64) sacc[w] = {0.0, 0.0};
+    const int eb = MODE == 0 ? starts[itb] : 0;
+    const int ee = MODE == 0 ? starts[itb + 1] : 1;
+    for (int e = eb; e < ee; ++e) {
+        const int D = MODE == 0 ? lists[e] : tboxes[itb];
+        const int lv = t.level[D];
+        const int p = hd.order[lv];
+        const int ncin = (p + 1) * (p + 1);
+        const v3 tv = sub(c, boxc(t, D));
+        __syncthreads();  // previous entry's X and a consumed
+        waves_block(MODE == 0 ? 1 : 0, p + pq, hd.k, tv, X, reinterpret_cast<cplx *>(rads), jt);
+        // the box's coefficients, unscaled (M = M~ s^n, L = L~ / s^n)
+        const double s = MODE == 0 ? hd.s[lv] : 1.0 / hd.s[lv];
+        const double *src = expn + ecoef_off(hd, D, lv);
+        for (int w = tid; w < ncin * NRC; w += 64) {
+            const int j = w / NRC, r = w % NRC;
+            int n, m;
+            nm_of(j, n, m);
+            a[w] = r < hd.rc ? ipow(s, n) * ldc(src + 2 * ((long)j * hd.nr + hd.r0 + r))
+                             : cplx{0.0, 0.0};
+        }
+        __syncthreads();
+        for (int w = tid; w < nco * parts; w += 64) {
+            const int o = w % nco, part = w / nco;
+            int nu, mu;
+            nm_of(o, nu, mu);
+            cplx acc[NRC];
+#pragma unroll
+            for (int r = 0; r < NRC; ++r) acc[r] = {0.0, 0.0};
+            for (int j = part; j < ncin; j += parts) {
+                int n, m;
+                nm_of(j, n, m);
+                const double *gg = hd.gaunt + ((long)j * ncg + o) * (hd.pg + 1);
+                cplx cf = {0.0, 0.0};
+                const int imax = min(n, nu);
+                for (int ii = 0; ii <= imax; ++ii) {
+                    const int l = abs(n - nu) + 2 * ii;
+                    if (abs(m - mu) > l) continue;
+                    const int ex = (nu + l - n) / 2;
+                    cf += ((ex & 1) ? -gg[ii] : gg[ii]) * X[IDX(l, m - mu)];
+                }
+#pragma unroll
+                for (int r = 0; r < NRC; ++r) acc[r] += cf * a[j * NRC + r];
+            }
+#pragma unroll
+            for (int r = 0; r < NRC; ++r) sacc[w * NRC + r] += acc[r];
+        }
+    }
+    __syncthreads();
+    for (int w = tid; w < nco * NRC; w += 64) {
+        const int o = w / NRC, r = w % NRC;
+        if (r >= hd.rc) continue;
+        cplx v = {0.0, 0.0};
+        for (int part = 0; part < parts; ++part) v += sacc[(part * nco + o) * NRC + r];
+        double *dst = qexp + 2 * (((size_t)ci * nco + o) * hd.nr + hd.r0 + r);
+        dst[0] += FOURPI * v.x;
+        dst[1] += FOURPI * v.y;
+    }
+}
+
+// ---- QBXL2P ---------------------------------------------------------------
+template <int NRC>
+__global__ void kh_qbxl2p(HD hd, int nctt, const int *ctt_lists, const int *tree_ttc,
+                          const double *tgt, int nt, const double *ctr, int nc,
+                          const double *qexp, double *pot)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nctt) return;
+    const int ti = ctt_lists[j];
+    const int c = tree_ttc[ti];
+    const v3 d = sub(ld3(tgt, nt, ti), ld3(ctr, nc, c));
+    cplx rad[MAXPQ + 1];
+    double jt[MAXPQ + 3];
+    const double rr = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+    sph_j_seq(hd.pq, hd.k * rr, jt);
+    for (int n = 0; n <= hd.pq; ++n) rad[n] = {jt[n], 0.0};
+    cplx acc[NRC];
+#pragma unroll
+    for (int r = 0; r < NRC; ++r) acc[r] = {0.0, 0.0};
+    sum_waves<NRC>(hd, qexp + 2 * (size_t)c * hd.ncq * hd.nr, hd.pq, d, rad, acc);
+#pragma unroll
+    for (int r = 0; r < NRC; ++r)
+        if (r < hd.rc) stc(pot + 2 * ((size_t)ti * hd.nr + hd.r0 + r), acc[r]);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static int make_hd(const qbx_kernel_desc *d, const qbx_tree *T, HD *hd, int nsep = 2)
+{
+    QBX_CHECK(d, "null kernel description");
+    QBX_CHECK(d->kernel == QBX_HELMHOLTZ3D, "3D Helmholtz stages: kernel %d", d->kernel);
+    QBX_CHECK(!d->target_deriv, "3D Helmholtz stages: target derivatives are not supported");
+    QBX_CHECK(d->k > 0.0 && std::isfinite(d->k), "3D Helmholtz stages need a real k > 0 (got %g)",
+              d->k);
+    QBX_CHECK(d->source_kind == QBX_SOURCE_S || d->source_kind == QBX_SOURCE_D,
+              "unsupported source kind %d", d->source_kind);
+    QBX_CHECK(d->qbx_order >= 0 && d->qbx_order <= MAXPQ, "3D qbx_order must be in [0, %d]",
+              MAXPQ);
+    QBX_CHECK(d->level_orders && d->nlevels >= 1, "level_orders missing");
+    QBX_CHECK(d->nrhs >= 0, "nrhs must be >= 0 (0 means 1)");
+    if (T) {
+        QBX_CHECK(T->dim == 3, "3D kernels need a 3D tree");
+        QBX_CHECK(T->nlevels <= FML, "tree has too many levels (%d)", T->nlevels);
+        QBX_CHECK(d->nlevels >= T->nlevels, "level_orders has %d entries, tree has %d levels",
+                  d->nlevels, T->nlevels);
+    }
+    memset(hd, 0, sizeof(*hd));
+    hd->dip = d->source_kind == QBX_SOURCE_D;
+    hd->pq = d->qbx_order;
+    hd->k = d->k;
+    hd->nsep = nsep;
+    hd->W = 2 * nsep + 1;
+    hd->nr = d->nrhs > 0 ? d->nrhs : 1;
+    hd->r0 = 0;
+    hd->rc = 1;
+    const int nl = T ? T->nlevels : std::min(d->nlevels, FML);
+    hd->nlevels = nl;
+    long coef = 0;
+    int pmax = 0;
+    for (int l = 0; l < nl; ++l) {
+        const int p = d->level_orders[l];
+        QBX_CHECK(p >= 1 && p <= MAXP, "3D fmm order %d at level %d outside [1, %d]", p, l, MAXP);
+        hd->order[l] = p;
+        pmax = std::max(pmax, p);
+        hd->ncl[l] = (p + 1) * (p + 1);
+        hd->h[l] = T ? ldexp(T->root_extent, -l) : 1.0;
+        hd->s[l] = std::min(1.0, hd->k * hd->h[l]);
+        hd->lbox[l] = T ? T->level_start_box_nrs[l] : 0;
+        hd->lcoef[l] = coef;
+        if (T)
+            coef += (long)(T->level_start_box_nrs[l + 1] - T->level_start_box_nrs[l]) * hd->ncl[l];
+        hd->m2m_off[l] = hd->l2l_off[l] = hd->m2l_off[l] = -1;
+    }
+    hd->lbox[nl] = T ? T->nboxes : 0;
+    hd->lcoef[nl] = coef;
+    hd->pmax = pmax;
+    hd->pg = std::max(pmax, hd->pq);
+    hd->ncq = (hd->pq + 1) * (hd->pq + 1);
+    return QBX_OK;
+}
+
+static TD make_td(const qbx_tree *T)
+{
+    TD t;
+    t.nboxes = T->nboxes;
+    t.level = T->box_levels.p;
+    t.parent = T->box_parent_ids.p;
+    t.child = T->box_child_ids.p;
+    t.coords = T->box_coords.p;
+    t.centers = T->box_centers.p;
+    t.sstart = T->box_source_starts.p;
+    t.scount = T->box_source_counts_nonchild.p;
+    t.scumul = T->box_source_counts_cumul.p;
+    return t;
+}
+
+static void gauss_legendre(int N, std::vector<double> &x, std::vector<double> &w)
+{
+    x.resize(N);
+    w.resize(N);
+    for (int i = 0; i < N; ++i) {
+        double z = std::cos(M_PI * (i + 0.75) / (N + 0.5)), dp = 1.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = z;
+            for (int n = 2; n <= N; ++n) {
+                const double p2 = ((2.0 * n - 1.0) * z * p1 - (n - 1.0) * p0) / n;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = N * (z * p1 - p0) / (z * z - 1.0);
+            const double dz = p1 / dp;
+            z -= dz;
+            if (std::fabs(dz) < 1e-16) break;
+        }
+        x[i] = z;
+        w[i] = 2.0 / ((1.0 - z * z) * dp * dp);
+    }
+}
+
+// normalised associated Legendre functions (Condon-Shortley) at x, all
+// |m| <= n <= L
+static void pbar_host(int L, double x, double *P)
+{
+    const double s = std::sqrt(std::max(0.0, 1.0 - x * x));
+    double qmm = 1.0 / std::sqrt(4.0 * M_PI), sm = 1.0;
+    for (int m = 0; m <= L; ++m) {
+        if (m > 0) {
+            qmm *= -std::sqrt((2.0 * m + 1.0) / (2.0 * m));
+            sm *= s;
+        }
+        double a2 = 0.0, a1 = qmm;
+        P[IDX(m, m)] = qmm * sm;
+        for (int n = m + 1; n <= L; ++n) {
+            double q;
+            if (n == m + 1) {
+                q = std::sqrt(2.0 * m + 3.0) * x * qmm;
+            } else {
+                const double a = std::sqrt((4.0 * n * n - 1.0) / ((double)n * n - (double)m * m));
+                const double b = std::sqrt(((n - 1.0) * (n - 1.0) - (double)m * m) /
+                                           (4.0 * (n - 1.0) * (n - 1.0) - 1.0));
+                q = a * (x * a1 - b * a2);
+            }
+            P[IDX(n, m)] = q * sm;
+            a2 = a1;
+            a1 = q;
+        }
+    }
+    for (int n = 1; n <= L; ++n)
+        for (int m = 1; m <= n; ++m) P[IDX(n, -m)] = ((m & 1) ? -1.0 : 1.0) * P[IDX(n, m)];
+}
+
+// The Gaunt table and the translation tables of this tree, kernel and
+// orders, cached in the context.
+static int ensure_tables(qbx_ctx *ctx, hipStream_t st, const qbx_tree *T, HD *hd,
+                         bool need_trans)
+{
+    QBX_TRY(note_capture(ctx, st));
+    // Gaunt table for degrees <= pg
+    if (ctx->h3_pg < hd->pg) {
+        QBX_TRY(may_replace(ctx, st, "3D Helmholtz Gaunt table"));
+        const int pg = std::max(hd->pg, 8), L = 2 * pg, nq = 2 * pg + 2;
+        const int ncg = (pg + 1) * (pg + 1), ncl = (L + 1) * (L + 1);
+        std::vector<double> xq, wq;
+        gauss_legendre(nq, xq, wq);
+        std::vector<double> pb((size_t)nq * ncl);
+        for (int q = 0; q < nq; ++q) pbar_host(L, xq[q], pb.data() + (size_t)q * ncl);
+        dbuf<double> dw, dp;
+        QBX_TRY(dw.alloc(nq));
+        QBX_TRY(dp.alloc((long)pb.size()));
+        QBX_HIP(hipMemcpy(dw.p, wq.data(), sizeof(double) * nq, hipMemcpyHostToDevice));
+        QBX_HIP(hipMemcpy(dp.p, pb.data(), sizeof(double) * pb.size(), hipMemcpyHostToDevice));
+        QBX_TRY(ctx->ws_h3_gaunt.alloc((long)ncg * ncg * (pg + 1)));
+        hipLaunchKernelGGL(kh_gaunt, dim3(cdiv((long)ncg * ncg, 256)), dim3(256), 0, st, pg, nq,
+                           dw.p, dp.p, ctx->ws_h3_gaunt.p);
+        QBX_HIP(hipGetLastError());
+        QBX_HIP(hipStreamSynchronize(st));  // before the staging buffers go
+        ctx->h3_pg = pg;
+        ctx->h3_key.clear();  // tables index the Gaunt table's layout
+    }
+    hd->pg = ctx->h3_pg;
+    hd->gaunt = ctx->ws_h3_gaunt.p;
+    if (!need_trans) return QBX_OK;
+    const int nl = hd->nlevels;
+    std::vector<double> key = {hd->k, T->root_extent, (double)hd->nsep, (double)nl,
+                               (double)hd->pg};
+    for (int l = 0; l < nl; ++l) key.push_back(hd->order[l]);
+    // table layout: M2M per parent level (8 classes), L2L per child level
+    // (8 classes), M2L per level with separated boxes ((2W+1)^3 classes)
+    const int S1 = 2 * hd->W + 1;
+    const long nm2l = (long)S1 * S1 * S1;
+    long off = 0;
+    long m2m[FML], l2l[FML], m2l[FML];
+    for (int l = 0; l < nl; ++l) {
+        m2m[l] = l2l[l] = m2l[l] = -1;
+        if (l + 1 < nl) {
+            m2m[l] = off;
+            off += 8L * hd->ncl[l + 1] * hd->ncl[l];
+        }
+        if (l >= 1) {
+            l2l[l] = off;
+            off += 8L * hd->ncl[l - 1] * hd->ncl[l];
+        }
+        if ((1L << l) > hd->nsep + 1) {
+            m2l[l] = off;
+            off += nm2l * hd->ncl[l] * hd->ncl[l];
+        }
+    }
+    QBX_CHECK(16L * off <= TAB_BYTES_MAX,
+              "3D Helmholtz translation tables would take %ld MiB (orders too high for this "
+              "tree depth)", (16L * off) >> 20);
+    if (key != ctx->h3_key) {
+        QBX_TRY(may_replace(ctx, st, "3D Helmholtz translation tables"));
+        QBX_TRY(ctx->ws_h3_tab.alloc(2 * std::max(off, 1L)));
+        // class offset vectors t = c2 - c1 per table
+        std::vector<double> tv;
+        std::vector<long> tvoff(3 * nl, -1);
+        for (int l = 0; l < nl; ++l) {
+            if (m2m[l] >= 0) {  // child (level l+1) -> parent: t = parent - child
+                tvoff[3 * l] = (long)tv.size() / 3;
+                const double hc = hd->h[l + 1];
+                for (int ci = 0; ci < 8; ++ci) {
+                    tv.push_back(-((ci & 1) ? 0.5 : -0.5) * hc);
+                    tv.push_back(-((ci & 2) ? 0.5 : -0.5) * hc);
+                    tv.push_back(-((ci & 4) ? 0.5 : -0.5) * hc);
+                }
+            }
+            if (l2l[l] >= 0) {  // parent -> child (level l): t = child - parent
+                tvoff[3 * l + 1] = (long)tv.size() / 3;
+                const double hc = hd->h[l];
+                for (int ci = 0; ci < 8; ++ci) {
+                    tv.push_back(((ci & 1) ? 0.5 : -0.5) * hc);
+                    tv.push_back(((ci & 2) ? 0.5 : -0.5) * hc);
+                    tv.push_back(((ci & 4) ? 0.5 : -0.5) * hc);
+                }
+            }
+            if (m2l[l] >= 0) {  // source -> target: t = (target - source) h
+                tvoff[3 * l + 2] = (long)tv.size() / 3;
+                const double h = hd->h[l];
+                for (int dx = -hd->W; dx <= hd->W; ++dx)
+                    for (int dy = -hd->W; dy <= hd->W; ++dy)
+                        for (int dz = -hd->W; dz <= hd->W; ++dz) {
+                            tv.push_back(dx * h);
+                            tv.push_back(dy * h);
+                            tv.push_back(dz * h);
+                        }
+            }
+        }
+        dbuf<double> dtv;
+        QBX_TRY(dtv.alloc(std::max<long>(1, (long)tv.size())));
+        if (!tv.empty())
+            QBX_HIP(hipMemcpy(dtv.p, tv.data(), sizeof(double) * tv.size(),
+                              hipMemcpyHostToDevice));
+        for (int l = 0; l < nl; ++l) {
+            const double k = hd->k;
+            if (m2m[l] >= 0)  // M~ = M / s^n: in scale 1/s_child, out 1/s_parent
+                hipLaunchKernelGGL(kh_build_tab, dim3(8), dim3(256), 0, st, 0, k, hd->order[l + 1],
+                                   hd->order[l], 1.0 / hd->s[l + 1], 1.0 / hd->s[l],
+                                   dtv.p + 3 * tvoff[3 * l], hd->pg, hd->gaunt,
+                                   ctx->ws_h3_tab.p + 2 * m2m[l]);
+            if (l2l[l] >= 0)  // L~ = L s^n: in scale s_parent, out s_child
+                hipLaunchKernelGGL(kh_build_tab, dim3(8), dim3(256), 0, st, 0, k, hd->order[l - 1],
+                                   hd->order[l], hd->s[l - 1], hd->s[l],
+                                   dtv.p + 3 * tvoff[3 * l + 1], hd->pg, hd->gaunt,
+                                   ctx->ws_h3_tab.p + 2 * l2l[l]);
+            if (m2l[l] >= 0)  // in 1/s, out s
+                hipLaunchKernelGGL(kh_build_tab, dim3(nm2l), dim3(256), 0, st, 1, k, hd->order[l],
+                                   hd->order[l], 1.0 / hd->s[l], hd->s[l],
+                                   dtv.p + 3 * tvoff[3 * l + 2], hd->pg, hd->gaunt,
+                                   ctx->ws_h3_tab.p + 2 * m2l[l]);
+        }
+        QBX_HIP(hipGetLastError());
+        QBX_HIP(hipStreamSynchronize(st));  // before the offset vectors go
+        ctx->h3_key = key;
+    }
+    for (int l = 0; l < nl; ++l) {
+        hd->m2m_off[l] = m2m[l];
+        hd->l2l_off[l] = l2l[l];
+        hd->m2l_off[l] = m2l[l];
+    }
+    hd->tab = ctx->ws_h3_tab.p;
+    return QBX_OK;
+}
+
+#define FOR_CHUNKS(hd, CH, LAUNCH)                                                       \
+    do {                                                                                 \
+        if ((hd).nr == 1) {                                                              \
+            (hd).r0 = 0;                                                                 \
+            (hd).rc = 1;                                                                 \
+            LAUNCH(1);                                                                   \
+        } else {                                                                         \
+            for (int r0_ = 0; r0_ < (hd).nr; r0_ += (CH)) {                             \
+                (hd).r0 = r0_;                                                           \
+                (hd).rc = std::min((CH), (hd).nr - r0_);                                 \
+                LAUNCH(CH);                                                              \
+            }                                                                            \
+        }                                                                                \
+    } while (0)
+
+static inline long exp_doubles(const HD &hd) { return 2L * hd.lcoef[hd.nlevels] * hd.nr; }
+
+// LDS bytes and chunk size of kh_p2e for orders up to pfmax
+static void p2e_shape(int pfmax, int nrc, int *CH, size_t *bytes)
+{
+    const long ncf = (long)(pfmax + 1) * (pfmax + 1);
+    const long per = 16 * ncf + 16L * (pfmax + 1) + 16L * nrc + 48 + 8L * (pfmax + 3);
+    int ch = (int)std::min(32L, (56L * 1024) / per);
+    ch = std::max(ch, 1);
+    *CH = ch;
+    *bytes = (size_t)(per * ch);
+}
+
+static int threads_for(int nco)
+{
+    return std::min(512, std::max(64, 64 * ((nco + 63) / 64)));
+}
+
+template <int MODE>
+static int launch_p2e(hipStream_t st, HD hd, const TD &td, int nitems, const int *items,
+                      const int *starts, const int *lists, const double *ctrs, int nctr,
+                      const int *c2tb, const double *src, int ns, const double *str,
+                      const double *nrm, int pmax_out, double *out)
+{
+    if (nitems == 0) return QBX_OK;
+    const int pfmax = pmax_out + (hd.dip ? 1 : 0);
+    const int nco = (pmax_out + 1) * (pmax_out + 1);
+    const int th = threads_for(nco);
+#define H3_P2E(N)                                                                          \
+    do {                                                                                   \
+        int CH;                                                                            \
+        size_t by;                                                                         \
+        p2e_shape(pfmax, N, &CH, &by);                                                     \
+        hipLaunchKernelGGL((kh_p2e<MODE, N>), dim3(nitems), dim3(th), by, st, hd, td,      \
+                           nitems, items, starts, lists, ctrs, nctr, c2tb, src, ns, str,   \
+                           nrm, CH, pfmax, out);                                           \
+    } while (0)
+    FOR_CHUNKS(hd, NRC_B, H3_P2E);
+#undef H3_P2E
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
+}  // namespace h3
+
+using namespace h3;
+
+long h3_expansion_size(const qbx_kernel_desc *d, const qbx_tree *T)
+{
+    HD hd;
+    if (!T || make_hd(d, T, &hd) != QBX_OK) return -1;
+    return hd.lcoef[hd.nlevels] * hd.nr;
+}
+
+int h3_expansion_level_offsets(const qbx_kernel_desc *d, const qbx_tree *T, long *offsets)
+{
+    HD hd;
+    QBX_TRY(make_hd(d, T, &hd));
+    for (int l = 0; l <= hd.nlevels; ++l) offsets[l] = hd.lcoef[l] * hd.nr;
+    return QBX_OK;
+}
+
+int h3_ncoeff_qbx(const qbx_kernel_desc *d)
+{
+    HD hd;
+    if (make_hd(d, nullptr, &hd) != QBX_OK) return -1;
+    return hd.ncq;
+}
+
+int h3_form_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                       const qbx_kernel_desc *d, const double *str, const double *nrm,
+                       double *mpole)
+{
+    HD hd;
+    QBX_TRY(make_hd(d, T, &hd));
+    QBX_CHECK(!hd.dip || nrm, "double layer needs normals");
+    hipStream_t st = S(stream);
+    QBX_TRY(zero_fill(mpole, exp_doubles(hd), st));
+    const TD td = make_td(T);
+    return launch_p2e<0>(st, hd, td, R->nsource_boxes, R->source_boxes.p, nullptr, nullptr,
+                         nullptr, 0, nullptr, T->sources.p, T->nsources, str, nrm, hd.pmax,
+                         mpole);
+}
+
+static int h3_trans_level(hipStream_t st, const HD &hd0, const TD &td, int op, int lv, int i0,
+                          int nbox, const int *boxes, const int *starts, const int *lists,
+                          const double *in, double *out)
+{
+    if (nbox <= 0) return QBX_OK;
+    HD hd = hd0;
+    const int lvin = op == 0 ? lv + 1 : op == 2 ? lv - 1 : lv;
+    const int ncin = hd.ncl[lvin], ncout = hd.ncl[lv];
+    const int th = std::min(448, std::max(64, 64 * ((ncout + 63) / 64)));
+#define H3_TR(N)                                                                           \
+    do {                                                                                   \
+        const size_t by = sizeof(double) * 2 * (size_t)ncin * (N);                         \
+        if (op == 0)                                                                       \
+            hipLaunchKernelGGL((kh_trans<0, N>), dim3(nbox), dim3(th), by, st, hd, td, lv, \
+                               i0, nbox, boxes, starts, lists, in, out);                   \
+        else if (op == 1)                                                                  \
+            hipLaunchKernelGGL((kh_trans<1, N>), dim3(nbox), dim3(th), by, st, hd, td, lv, \
+                               i0, nbox, boxes, starts, lists, in, out);                   \
+        else                                                                               \
+            hipLaunchKernelGGL((kh_trans<2, N>), dim3(nbox), dim3(th), by, st, hd, td, lv, \
+                               i0, nbox, boxes, starts, lists, in, out);                   \
+    } while (0)
+    FOR_CHUNKS(hd, NRC_B, H3_TR);
+#undef H3_TR
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
+int h3_coarsen_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                          const qbx_kernel_desc *d, double *mpole, int lmin)
+{
+    HD hd;
+    QBX_TRY(make_hd(d, T, &hd, R->nsep));
+    hipStream_t st = S(stream);
+    QBX_TRY(ensure_tables(ctx, st, T, &hd, true));
+    const TD td = make_td(T);
+    for (int l = T->nlevels - 2; l >= std::max(lmin, 0); --l) {
+        const int i0 = R->h_level_start_source_parent_box_nrs[l];
+        const int i1 = R->h_level_start_source_parent_box_nrs[l + 1];
+        QBX_TRY(h3_trans_level(st, hd, td, 0, l, i0, i1 - i0, R->source_parent_boxes.p, nullptr,
+                               nullptr, mpole, mpole));
+    }
+    return QBX_OK;
+}
+
+int h3_multipole_to_local(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                          const qbx_kernel_desc *d, const double *mpole, double *local)
+{
+    HD hd;
+    QBX_TRY(make_hd(d, T, &hd, R->nsep));
+    hipStream_t st = S(stream);
+    QBX_TRY(ensure_tables(ctx, st, T, &hd, true));
+    QBX_TRY(zero_fill(local, exp_doubles(hd), st));
+    if (R->n2 == 0) return QBX_OK;
+    const TD td = make_td(T);
+    for (int l = 0; l < T->nlevels; ++l) {
+        const int i0 = R->h_level_start_ttp_box_nrs[l], i1 = R->h_level_start_ttp_box_nrs[l + 1];
+        if (i1 == i0 || hd.m2l_off[l] < 0) continue;
+        QBX_TRY(h3_trans_level(st, hd, td, 1, l, i0, i1 - i0, R->ttp_boxes.p, R->l2_starts.p,
+                               R->l2_lists.p, mpole, local));
+    }
+    return QBX_OK;
+}
+
+int h3_refine_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                     const qbx_kernel_desc *d, double *local, int lmin)
+{
+    HD hd;
+    QBX_TRY(make_hd(d, T, &hd, R->nsep));
+    hipStream_t st = S(stream);
+    QBX_TRY(ensure_tables(ctx, st, T, &hd, true));
+    const TD td = make_td(T);
+    for (int l = std::max(lmin, 0) + 1; l < T->nlevels; ++l) {
+        const int i0 = R->h_level_start_ttp_box_nrs[l], i1 = R->h_level_start_ttp_box_nrs[l + 1];
+        QBX_TRY(h3_trans_level(st, hd, td, 2, l, i0, i1 - i0, R->ttp_boxes.p, nullptr, nullptr,
+                               local, local));
+    }
+    return QBX_OK;
+}
+
+int h3_form_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                   const qbx_kernel_desc *d, const double *str, const double *nrm, double *local,
+                   int accumulate)
+{
+    HD hd;
+    QBX_TRY(make_hd(d, T, &hd));
+    QBX_CHECK(!hd.dip || nrm, "double layer needs normals");
+    hipStream_t st = S(stream);
+    if (!accumulate) QBX_TRY(zero_fill(local, exp_doubles(hd), st));
+    if (R->n4 == 0 || R->nttp_boxes == 0) return QBX_OK;
+    const TD td = make_td(T);
+    return launch_p2e<1>(st, hd, td, R->nttp_boxes, R->ttp_boxes.p, R->l4_starts.p,
+                         R->l4_lists.p, nullptr, 0, nullptr, T->sources.p, T->nsources, str, nrm,
+                         hd.pmax, local);
+}
+
+int h3_eval_targets(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                    const qbx_geo *G, const qbx_kernel_desc *d, int mode, const double *str,
+                    const double *nrm, const double *expn, double *pot, int accumulate)
+{
+    HD hd;
+    QBX_TRY(make_hd(d, T, &hd));
+    QBX_CHECK(mode != 0 || !hd.dip || nrm, "double layer needs normals");
+    hipStream_t st = S(stream);
+    if (!accumulate)
+        QBX_TRY(zero_fill(pot, 2L * (long)T->ntargets * hd.nr, st));
+    const int ntb = R->ntarget_boxes;
+    if (ntb == 0 || G->nfiltered == 0) return QBX_OK;
+    const TD td = make_td(T);
+    const int *starts = mode == 1 ? R->l3_starts.p : R->l1_starts.p;
+    const int *lists = mode == 1 ? R->l3_lists.p : R->l1_lists.p;
+#define H3_EV(M, N)                                                                        \
+    hipLaunchKernelGGL((kh_eval<M, N>), dim3(ntb), dim3(64), 0, st, hd, td, ntb,           \
+                       R->target_boxes.p, starts, lists, G->nqbtl_box_target_starts.p,    \
+                       G->nqbtl_box_target_counts_nonchild.p, G->nqbtl_unfiltered.p,      \
+                       T->targets.p, T->ntargets, T->sources.p, T->nsources, str, nrm,    \
+                       expn, pot)
+#define H3_EV0(N) H3_EV(0, N)
+#define H3_EV1(N) H3_EV(1, N)
+#define H3_EV2(N) H3_EV(2, N)
+    if (mode == 0) FOR_CHUNKS(hd, NRC_E, H3_EV0);
+    else if (mode == 1) FOR_CHUNKS(hd, NRC_E, H3_EV1);
+    else FOR_CHUNKS(hd, NRC_E, H3_EV2);
+#undef H3_EV0
+#undef H3_EV1
+#undef H3_EV2
+#undef H3_EV
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
+int h3_form_global_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T,
+                              const qbx_trav *R, const qbx_geo *G, const qbx_kernel_desc *d,
+                              const double *str, const double *nrm, double *qexp, int accumulate)
+{
+    HD hd;
+    QBX_TRY(make_hd(d, T, &hd));
+    QBX_CHECK(!hd.dip || nrm, "double layer needs normals");
+    hipStream_t st = S(stream);
+    if (!accumulate)
+        QBX_TRY(zero_fill(qexp, 2L * G->ncenters * hd.ncq * hd.nr, st));
+    if (G->nglobal == 0) return QBX_OK;
+    QBX_CHECK(G->trav == R, "P2QBXL: geometry was built for another traversal");
+    const TD td = make_td(T);
+    return launch_p2e<2>(st, hd, td, G->nglobal, G->global_qbx_centers.p, R->l1_starts.p,
+                         R->l1_lists.p, G->centers.p, G->ncenters, G->center_to_target_box.p,
+                         T->sources.p, T->nsources, str, nrm, hd.pq, qexp);
+}
+
+static int h3_e2qbxl(qbx_ctx *ctx, hipStream_t st, const qbx_tree *T, const qbx_trav *R,
+                     const qbx_geo *G, const qbx_kernel_desc *d, int mode, const double *expn,
+                     double *qexp, int accumulate)
+{
+    HD hd;
+    QBX_TRY(make_hd(d, T, &hd));
+    QBX_TRY(ensure_tables(ctx, st, T, &hd, false));
+    if (!accumulate)
+        QBX_TRY(zero_fill(qexp, 2L * G->ncenters * hd.ncq * hd.nr, st));
+    const int ng = G->nglobal;
+    if (ng == 0 || (mode == 0 && R->n3 == 0)) return QBX_OK;
+    QBX_CHECK(G->trav == R, "M2QBXL / L2QBXL: geometry was built for another traversal");
+    const TD td = make_td(T);
+#define H3_EQ(M, N)                                                                        \
+    hipLaunchKernelGGL((kh_e2qbxl<M, N>), dim3(ng), dim3(64), 0, st, hd, td, ng,           \
+                       G->global_qbx_centers.p, G->center_to_target_box.p,                 \
+                       R->target_boxes.p, R->l3_starts.p, R->l3_lists.p, G->centers.p,     \
+                       G->ncenters, expn, qexp)
+#define H3_EQ0(N) H3_EQ(0, N)
+#define H3_EQ1(N) H3_EQ(1, N)
+    if (mode == 0) FOR_CHUNKS(hd, NRC_Q, H3_EQ0);
+    else FOR_CHUNKS(hd, NRC_Q, H3_EQ1);
+#undef H3_EQ0
+#undef H3_EQ1
+#undef H3_EQ
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
+int h3_translate_box_multipoles_to_qbx_local(qbx_ctx *ctx, void *stream, const qbx_tree *T,
+                                             const qbx_trav *R, const qbx_geo *G,
+                                             const qbx_kernel_desc *d, const double *mpole,
+                                             double *qexp, int accumulate)
+{
+    return h3_e2qbxl(ctx, S(stream), T, R, G, d, 0, mpole, qexp, accumulate);
+}
+
+int h3_translate_box_local_to_qbx_local(qbx_ctx *ctx, void *stream, const qbx_tree *T,
+                                        const qbx_trav *R, const qbx_geo *G,
+                                        const qbx_kernel_desc *d, const double *local,
+                                        double *qexp, int accumulate)
+{
+    return h3_e2qbxl(ctx, S(stream), T, R, G, d, 1, local, qexp, accumulate);
+}
+
+int h3_eval_qbx_expansions(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_geo *G,
+                           const qbx_kernel_desc *d, const double *qexp, double *pot)
+{
+    HD hd;
+    QBX_TRY(make_hd(d, T, &hd));
+    hipStream_t st = S(stream);
+    if (G->nctt > 0) {
+#define H3_Q2P(N)                                                                          \
+    hipLaunchKernelGGL(kh_qbxl2p<N>, dim3(cdiv(G->nctt, 64)), dim3(64), 0, st, hd, G->nctt, \
+                       G->ctt_lists.p, G->tree_ttc.p, T->targets.p, T->ntargets,            \
+                       G->centers.p, G->ncenters, qexp, pot)
+        FOR_CHUNKS(hd, NRC_E, H3_Q2P);
+#undef H3_Q2P
+    }
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
+}  // namespace qbx

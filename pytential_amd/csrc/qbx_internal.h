@@ -130,6 +130,11 @@ struct qbx_ctx {
     int tune[QBX_TUNE_COUNT] = {qbx::UNIT_ITER, qbx::M2P_UNIT_ITER, qbx::P2Q_UNIT_ITER,
                                 qbx::M2Q_UNIT_ITER, 4};
     std::vector<double> tab3_key;
+    // 3D Helmholtz: the Gaunt table (degrees <= h3_pg) and the translation
+    // tables of the (k, tree extent, nsep, orders) in h3_key
+    qbx::dbuf<double> ws_h3_gaunt, ws_h3_tab;
+    int h3_pg = -1;
+    std::vector<double> h3_key;
     // side streams of qbx_drive_fmm (default priority; shared by every
     // context on the device, see side_streams) and its fork/join events
     // (this context's own), set up on its first call
@@ -305,4 +310,38 @@ int f3_translate_box_local_to_qbx_local(qbx_ctx *ctx, void *stream, const qbx_tr
                                         double *qexp, int accumulate);
 int f3_eval_qbx_expansions(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_geo *G,
                            const qbx_kernel_desc *d, const double *qexp, double *pot);
+
+// 3D Helmholtz stages (fmmh3d.hip): the f3_* contracts for QBX_HELMHOLTZ3D
+long h3_expansion_size(const qbx_kernel_desc *d, const qbx_tree *T);
+int h3_expansion_level_offsets(const qbx_kernel_desc *d, const qbx_tree *T, long *offsets);
+int h3_ncoeff_qbx(const qbx_kernel_desc *d);
+int h3_form_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                       const qbx_kernel_desc *d, const double *str, const double *nrm,
+                       double *mpole);
+int h3_coarsen_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                          const qbx_kernel_desc *d, double *mpole, int lmin);
+int h3_eval_targets(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                    const qbx_geo *G, const qbx_kernel_desc *d, int mode, const double *str,
+                    const double *nrm, const double *expn, double *pot, int accumulate);
+int h3_multipole_to_local(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                          const qbx_kernel_desc *d, const double *mpole, double *local);
+int h3_form_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                   const qbx_kernel_desc *d, const double *str, const double *nrm, double *local,
+                   int accumulate);
+int h3_refine_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
+                     const qbx_kernel_desc *d, double *local, int lmin);
+int h3_form_global_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T,
+                              const qbx_trav *R, const qbx_geo *G, const qbx_kernel_desc *d,
+                              const double *str, const double *nrm, double *qexp, int accumulate);
+int h3_translate_box_multipoles_to_qbx_local(qbx_ctx *ctx, void *stream, const qbx_tree *T,
+                                             const qbx_trav *R, const qbx_geo *G,
+                                             const qbx_kernel_desc *d, const double *mpole,
+                                             double *qexp, int accumulate);
+int h3_translate_box_local_to_qbx_local(qbx_ctx *ctx, void *stream, const qbx_tree *T,
+                                        const qbx_trav *R, const qbx_geo *G,
+                                        const qbx_kernel_desc *d, const double *local,
+                                        double *qexp, int accumulate);
+int h3_eval_qbx_expansions(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_geo *G,
+                           const qbx_kernel_desc *d, const double *qexp, double *pot);
+
 }  // namespace qbx

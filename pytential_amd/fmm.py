@@ -88,10 +88,6 @@ class QBXExpansionWrangler:
         self.using_tsqbx = bool(use_target_specific_qbx)
         # the fused driver runs the sum in place of P2QBXL (3D Laplace)
         self.kd.target_specific_qbx = int(self.using_tsqbx and self.k == 0)
-        self.expansion_size = None
-        if self.kernel_code == _lib.HELMHOLTZ3D:
-            # 3D Helmholtz: only the target-specific stage exists in the library
-            return
         self.expansion_size = _lib.lib().qbx_expansion_size(ct.byref(self.kd), self.tree.h)
         if self.expansion_size < 0:
             raise _lib.QBXError(1, _lib.lib().qbx_last_error().decode())
