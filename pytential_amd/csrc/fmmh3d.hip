@@ -120,7 +120,8 @@ __device__ __forceinline__ cplx sgn_conj(cplx v, int am)  // (-1)^m conj(v)
 
 // ---- special functions (real argument) ------------------------------------
 // j_n(x), n = 0..p (p >= 1 slots written), by Miller's backward recurrence
-// normalised by j_0 or j_1 (the larger)
+// from N = max(p, x) + 10 + 4 sqrt(max(p, x)) (j_N / j_p below 1e-10 there,
+// so the start's error enters squared), normalised by j_0 or j_1 (the larger)
 __device__ void sph_j_seq(int p, double x, double *j)
 {
     const int pp = p < 1 ? 1 : p;
@@ -129,7 +130,7 @@ __device__ void sph_j_seq(int p, double x, double *j)
         return;
     }
     const double mx = fmax((double)pp, x);
-    const int N = (int)mx + 30 + (int)sqrt(40.0 * mx);
+    const int N = (int)mx + 10 + (int)(4.0 * sqrt(mx));
     const double ix = 1.0 / x;
     double fp1 = 0.0, f = 1e-300;
     for (int n = N; n >= 0; --n) {
@@ -220,7 +221,7 @@ __device__ __forceinline__ cplx getf(const cplx *F, int p, int n, int m)
 }
 
 // nu . grad F_n^m from the ladder relations; F holds orders <= pf (>= n + 1)
-__device__ cplx nu_grad(const cplx *F, int pf, double k, int n, int m, v3 nu)
+__device__ __forceinline__ cplx nu_grad(const cplx *F, int pf, double k, int n, int m, v3 nu)
 {
     const double dn = n;
     const double An = sqrt(((dn + 1.0) * (dn + 1.0) - (double)m * m) /
@@ -358,6 +359,7 @@ __global__ void __launch_bounds__(512) kh_p2e(HD hd, TD t, int nitems, const int
                                              double *out)
 {
     extern __shared__ double sh[];
+    __shared__ int gs[64];  // the chunk's source indices (CH <= 32)
     const int it = blockIdx.x;
     if (it >= nitems) return;
     const int ncf = (pfmax + 1) * (pfmax + 1);
@@ -425,16 +427,16 @@ __global__ void __launch_bounds__(512) kh_p2e(HD hd, TD t, int nitems, const int
     have = next_range();
     while (have) {
         // gather up to CH sources (uniform across the block)
-        __syncthreads();
+        __syncthreads();  // the previous chunk consumed
         int cnt = 0;
-        int gs[64];
         while (have && cnt < CH) {
             const int take = min(s1 - s, CH - cnt);
-            for (int q = 0; q < take; ++q) gs[cnt + q] = s + q;
+            for (int q = tid; q < take; q += blockDim.x) gs[cnt + q] = s + q;
             cnt += take;
             s += take;
             if (s >= s1) have = next_range();
         }
+        __syncthreads();
         for (int q = tid; q < cnt; q += blockDim.x) {
             const int si = gs[q];
             const v3 d = sub(ld3(src, ns, si), c);
@@ -498,6 +500,139 @@ __global__ void __launch_bounds__(512) kh_p2e(HD hd, TD t, int nitems, const int
                 dst[0] += v.x;
                 dst[1] += v.y;
             }
+    }
+}
+
+// P2QBXL for one density with the QBX expansion in registers (orders <= 5):
+// a wavefront per global QBX center, the center's list-1 sources dealt to
+// the lanes round-robin across the boxes.  Each lane forms a source's S
+// waves -- y_n by the forward recurrence, j_n by Miller's backward
+// recurrence from N = max(PF, x) + 10 + 4 sqrt(max(PF, x)) normalised by j_0
+// or j_1, Y by the column recurrences, all unrolled -- and adds
+// q (-1)^m S_n^{-m}(s - c) (or its nu-gradient) into its registers; the
+// lanes' sums are reduced by shuffles and lane 0 adds ik times them.
+template <int PQ, int DIP>
+__global__ void __launch_bounds__(64) kh_p2qbxl_reg(HD hd, TD t, int ng, const int *gcent,
+                                                    const int *c2tb, const int *starts,
+                                                    const int *lists, const double *ctrs,
+                                                    int nctr, const double *src, int ns,
+                                                    const double *str, const double *nrm,
+                                                    double *qexp)
+{
+    constexpr int PF = PQ + DIP;
+    constexpr int PJ = PF < 1 ? 1 : PF;
+    constexpr int NC = (PQ + 1) * (PQ + 1);
+    constexpr int NF = (PF + 1) * (PF + 1);
+    const int g = blockIdx.x;
+    if (g >= ng) return;
+    const int ci = gcent[g];
+    const int tb = c2tb[ci];
+    const v3 c = ld3(ctrs, nctr, ci);
+    const int lane = threadIdx.x;
+    const double k = hd.k;
+    cplx acc[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) acc[i] = {0.0, 0.0};
+    int base = 0;
+    for (int e = starts[tb]; e < starts[tb + 1]; ++e) {
+        const int Sb = lists[e];
+        const int s0 = t.sstart[Sb], cnt = t.scount[Sb];
+        for (int j = (lane - (base & 63) + 64) & 63; j < cnt; j += 64) {
+            const int si = s0 + j;
+            const v3 d = sub(ld3(src, ns, si), c);
+            const double r = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+            const double ir = 1.0 / r;
+            const double x = k * r, ix = 1.0 / x;
+            double sn, cs;
+            sincos(x, &sn, &cs);
+            double yv[PF + 1];
+            yv[0] = -cs * ix;
+            if (PF >= 1) yv[PF >= 1 ? 1 : 0] = (-cs * ix - sn) * ix;
+#pragma unroll
+            for (int n = 2; n <= PF; ++n) yv[n] = (2.0 * n - 1.0) * ix * yv[n - 1] - yv[n - 2];
+            const double mx = fmax((double)PJ, x);
+            const int N = (int)mx + 10 + (int)(4.0 * sqrt(mx));
+            double fp1 = 0.0, f = 1e-30;
+            for (int n = N; n > PJ; --n) {
+                const double fm1 = (2.0 * n + 1.0) * ix * f - fp1;
+                fp1 = f;
+                f = fm1;
+            }
+            double jv[PJ + 1];
+#pragma unroll
+            for (int n = PJ; n >= 0; --n) {
+                jv[n] = f;
+                const double fm1 = (2.0 * n + 1.0) * ix * f - fp1;
+                fp1 = f;
+                f = fm1;
+            }
+            const double j0 = sn * ix, j1 = (sn * ix - cs) * ix;
+            const double scl = fabs(j0) >= fabs(j1) ? j0 / jv[0] : j1 / jv[1];
+            cplx F[NF];
+            const double z = d.z * ir;
+            const cplx w = {d.x * ir, d.y * ir};
+            cplx wm = {1.0, 0.0};
+            double qmm = 0.28209479177387814347;
+#pragma unroll
+            for (int m = 0; m <= PF; ++m) {
+                if (m > 0) {
+                    qmm *= -sqrt((2.0 * m + 1.0) / (2.0 * m));
+                    wm = wm * w;
+                }
+                double a2 = 0.0, a1 = qmm;
+#pragma unroll
+                for (int n = m; n <= PF; ++n) {
+                    double q;
+                    if (n == m) {
+                        q = qmm;
+                    } else if (n == m + 1) {
+                        q = sqrt(2.0 * m + 3.0) * z * qmm;
+                    } else {
+                        const double a = sqrt((4.0 * n * n - 1.0) / ((double)n * n - (double)m * m));
+                        const double b = sqrt(((n - 1.0) * (n - 1.0) - (double)m * m) /
+                                              (4.0 * (n - 1.0) * (n - 1.0) - 1.0));
+                        q = a * (z * a1 - b * a2);
+                    }
+                    if (n > m) {
+                        a2 = a1;
+                        a1 = q;
+                    }
+                    const cplx hn = {scl * jv[n], yv[n]};
+                    const cplx y = q * wm;
+                    F[IDX(n, m)] = hn * y;
+                    if (m > 0) F[IDX(n, -m)] = hn * sgn_conj(y, m);
+                }
+            }
+            const cplx qs = ldc(str + 2 * ((size_t)si * hd.nr + hd.r0));
+            v3 nu = {0.0, 0.0, 0.0};
+            if (DIP) nu = ld3(nrm, ns, si);
+#pragma unroll
+            for (int n = 0; n <= PQ; ++n)
+#pragma unroll
+                for (int m = -n; m <= n; ++m) {
+                    cplx fv = DIP ? nu_grad(F, PF, k, n, -m, nu) : F[IDX(n, -m)];
+                    if (m & 1) fv = -fv;
+                    acc[IDX(n, m)] += qs * fv;
+                }
+        }
+        base += cnt;
+    }
+#pragma unroll
+    for (int i = 0; i < NC; ++i)
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            acc[i].x += __shfl_xor(acc[i].x, off);
+            acc[i].y += __shfl_xor(acc[i].y, off);
+        }
+    if (lane == 0) {
+        const cplx ik = {0.0, k};
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            double *dst = qexp + 2 * (((size_t)ci * NC + i) * hd.nr + hd.r0);
+            const cplx v = ik * acc[i];
+            dst[0] += v.x;
+            dst[1] += v.y;
+        }
     }
 }
 
@@ -1301,6 +1436,36 @@ int h3_form_global_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T,
     if (G->nglobal == 0) return QBX_OK;
     QBX_CHECK(G->trav == R, "P2QBXL: geometry was built for another traversal");
     const TD td = make_td(T);
+    if (hd.pq <= 5) {
+        // the expansion in registers, one density per launch
+        const int ng = G->nglobal;
+        for (int r0 = 0; r0 < hd.nr; ++r0) {
+            hd.r0 = r0;
+            hd.rc = 1;
+#define H3_PQR(PQV, DV)                                                                    \
+    hipLaunchKernelGGL((kh_p2qbxl_reg<PQV, DV>), dim3(ng), dim3(64), 0, st, hd, td, ng,    \
+                       G->global_qbx_centers.p, G->center_to_target_box.p, R->l1_starts.p, \
+                       R->l1_lists.p, G->centers.p, G->ncenters, T->sources.p, T->nsources, \
+                       str, nrm, qexp)
+#define H3_PQD(PQV)          \
+    if (hd.dip)              \
+        H3_PQR(PQV, 1);      \
+    else                     \
+        H3_PQR(PQV, 0)
+            switch (hd.pq) {
+            case 0: H3_PQD(0); break;
+            case 1: H3_PQD(1); break;
+            case 2: H3_PQD(2); break;
+            case 3: H3_PQD(3); break;
+            case 4: H3_PQD(4); break;
+            default: H3_PQD(5); break;
+            }
+#undef H3_PQD
+#undef H3_PQR
+        }
+        QBX_HIP(hipGetLastError());
+        return QBX_OK;
+    }
     return launch_p2e<2>(st, hd, td, G->nglobal, G->global_qbx_centers.p, R->l1_starts.p,
                          R->l1_lists.p, G->centers.p, G->ncenters, G->center_to_target_box.p,
                          T->sources.p, T->nsources, str, nrm, hd.pq, qexp);
