@@ -149,7 +149,8 @@ long qbx_geo_field_bytes(const qbx_geo *geo, const char *field);
 /* ---- QBX-FMM wrangler stages ------------------------------------------------
  * kernel description for all stages */
 typedef struct {
-    int kernel;             /* QBX_LAPLACE2D | QBX_HELMHOLTZ2D */
+    int kernel;             /* QBX_LAPLACE2D | QBX_HELMHOLTZ2D | QBX_LAPLACE3D |
+                               QBX_HELMHOLTZ3D (real k > 0, spherical-wave expansions) */
     double k;               /* Helmholtz wave number (real) */
     int source_kind;        /* QBX_SOURCE_S | QBX_SOURCE_D */
     int qbx_order;          /* p_q */
