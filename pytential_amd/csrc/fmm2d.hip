@@ -1035,13 +1035,184 @@ k_m2l_mfma(KD kd, TD t, Flat fl, const int *ttp, const int *starts, const int *l
     }
 }
 
+// One-density M2L grouped by translation class (Helmholtz, levels with
+// ncl = 2p + 1 <= 48).  All list-2 pairs of one level and integer offset
+// class share one Toeplitz operator T[m][n] = row[n - m], so a work item of up
+// to 16 such pairs is one complex (ncl x ncl) x (ncl x 16) product with the
+// pairs' source multipoles as the columns, on v_mfma_f64_16x16x4f64 (lane i
+// holds A[i%16][k0 + i/16] and B[k0 + i/16][i%16]; D register j of lane i is
+// D[4j + i/16][i%16]); a wavefront per item stages the row window in LDS.
+// Column c's product goes to the pair's scratch row (at yoff[first + c],
+// ncl complex); k_m2l_cls_sum then adds each box's rows in list order, so
+// the sums are deterministic.
+constexpr int M2LG_NP = 16;  // pairs per work item
+constexpr int M2LM_RTW = 4;  // row tiles per wave of the four-wave kernels (ncl <= 256)
+constexpr int M2LM_WIN2 = 16 * 4 * M2LM_RTW + 256;
+
+__global__ void __launch_bounds__(64)
+k_m2l_cls(KD kd, TD t, int nitems, const int *items, const int *pe, const int *yoff,
+          const int *lists, const double *mpole, double *Y)
+{
+    __shared__ double win[2 * (M2LM_ROWS * 16 + 48)];
+    const int it = blockIdx.x;
+    if (it >= nitems) return;
+    const int lv = items[4 * it], cls = items[4 * it + 1];
+    const int first = items[4 * it + 2], cnt = items[4 * it + 3];
+    const int p = kd.order[lv];
+    const int ncl = kd.ncl[lv];
+    const int RT = (ncl + 15) / 16, KS = (ncl + 3) / 4;
+    const int lane = threadIdx.x;
+    const int col = lane & 15, kr = lane >> 4;
+    const int L = 2 * p;
+    const double *row = kd.tab_m2l + 2 * (kd.m2l_off[lv] + (long)cls * (2 * L + 1) + L);
+    const int dlo = -(RT * 16 - 1), nwin = RT * 16 + KS * 4 - 1;
+    for (int w = lane; w < nwin; w += 64) {
+        const int d = w + dlo;
+        const bool in = d >= -L && d <= L;
+        win[2 * w] = in ? row[2 * d] : 0.0;
+        win[2 * w + 1] = in ? row[2 * d + 1] : 0.0;
+    }
+    __syncthreads();
+    // this lane's B column: the source multipole of pair `col`
+    const double *a = nullptr;
+    if (col < cnt) a = mpole + ecoef_off(kd, lists[pe[first + col]], lv);
+    m2l_d4 cre[M2LM_ROWS], cim[M2LM_ROWS];
+#pragma unroll
+    for (int rt = 0; rt < M2LM_ROWS; ++rt) cre[rt] = cim[rt] = m2l_d4{0, 0, 0, 0};
+    for (int ks = 0; ks < KS; ++ks) {
+        const int n = ks * 4 + kr;
+        double bre = 0.0, bim = 0.0;
+        if (a && n < ncl) {
+            const cplx b = ldcn(kd, a, n - p, kd.r0);
+            bre = b.x;
+            bim = b.y;
+        }
+#pragma unroll
+        for (int rt = 0; rt < M2LM_ROWS; ++rt) {
+            if (rt < RT) {
+                const int m = rt * 16 + col;
+                const int w = n - m - dlo;
+                const double are = win[2 * w], aim = win[2 * w + 1];
+                cre[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bre, cre[rt], 0, 0, 0);
+                cre[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(-aim, bim, cre[rt], 0, 0, 0);
+                cim[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bim, cim[rt], 0, 0, 0);
+                cim[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, bre, cim[rt], 0, 0, 0);
+            }
+        }
+    }
+    if (col < cnt) {
+        double *y = Y + 2L * yoff[first + col];
+#pragma unroll
+        for (int rt = 0; rt < M2LM_ROWS; ++rt)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int m = rt * 16 + 4 * j + kr;
+                if (rt < RT && m < ncl) {
+                    y[2 * m] = cre[rt][j];
+                    y[2 * m + 1] = cim[rt][j];
+                }
+            }
+    }
+}
+
+// k_m2l_cls for the coarse levels (48 < ncl <= 256): four waves share the
+// item's row window (staged once) and take the row tiles w, w + 4, ...
+__global__ void __launch_bounds__(256)
+k_m2l_cls_rows(KD kd, TD t, int nitems, const int *items, const int *pe, const int *yoff,
+               const int *lists, const double *mpole, double *Y)
+{
+    __shared__ double win[2 * M2LM_WIN2];
+    const int it = blockIdx.x;
+    if (it >= nitems) return;
+    const int lv = items[4 * it], cls = items[4 * it + 1];
+    const int first = items[4 * it + 2], cnt = items[4 * it + 3];
+    const int p = kd.order[lv];
+    const int ncl = kd.ncl[lv];
+    const int RT = (ncl + 15) / 16, KS = (ncl + 3) / 4;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = lane & 15, kr = lane >> 4;
+    const int L = 2 * p;
+    const double *row = kd.tab_m2l + 2 * (kd.m2l_off[lv] + (long)cls * (2 * L + 1) + L);
+    const int dlo = -(RT * 16 - 1), nwin = RT * 16 + KS * 4 - 1;
+    for (int w = threadIdx.x; w < nwin; w += 256) {
+        const int d = w + dlo;
+        const bool in = d >= -L && d <= L;
+        win[2 * w] = in ? row[2 * d] : 0.0;
+        win[2 * w + 1] = in ? row[2 * d + 1] : 0.0;
+    }
+    __syncthreads();
+    const double *a = nullptr;
+    if (col < cnt) a = mpole + ecoef_off(kd, lists[pe[first + col]], lv);
+    m2l_d4 cre[M2LM_RTW], cim[M2LM_RTW];
+#pragma unroll
+    for (int q = 0; q < M2LM_RTW; ++q) cre[q] = cim[q] = m2l_d4{0, 0, 0, 0};
+    for (int ks = 0; ks < KS; ++ks) {
+        const int n = ks * 4 + kr;
+        double bre = 0.0, bim = 0.0;
+        if (a && n < ncl) {
+            const cplx b = ldcn(kd, a, n - p, kd.r0);
+            bre = b.x;
+            bim = b.y;
+        }
+#pragma unroll
+        for (int q = 0; q < M2LM_RTW; ++q) {
+            const int rt = wv + 4 * q;
+            if (rt < RT) {
+                const int m = rt * 16 + col;
+                const int w = n - m - dlo;
+                const double are = win[2 * w], aim = win[2 * w + 1];
+                cre[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bre, cre[q], 0, 0, 0);
+                cre[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-aim, bim, cre[q], 0, 0, 0);
+                cim[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bim, cim[q], 0, 0, 0);
+                cim[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, bre, cim[q], 0, 0, 0);
+            }
+        }
+    }
+    if (col < cnt) {
+        double *y = Y + 2L * yoff[first + col];
+#pragma unroll
+        for (int q = 0; q < M2LM_RTW; ++q)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int rt = wv + 4 * q;
+                const int m = rt * 16 + 4 * j + kr;
+                if (rt < RT && m < ncl) {
+                    y[2 * m] = cre[q][j];
+                    y[2 * m + 1] = cim[q][j];
+                }
+            }
+    }
+}
+
+// the per-box sums of the grouped rows, in list-2 order: a 64-lane block per
+// target-or-target-parent box of a grouped level (levmask bit), lanes over
+// the output coefficients
+__global__ void __launch_bounds__(64)
+k_m2l_cls_sum(KD kd, TD t, int nttp, const int *ttp, const int *starts, const int *pos,
+              const int *yoff, unsigned levmask, const double *Y, double *local)
+{
+    const int i = blockIdx.x;
+    if (i >= nttp) return;
+    const int P = ttp[i];
+    const int lv = t.level[P];
+    if (!((levmask >> lv) & 1u)) return;
+    const int e0 = starts[i], e1 = starts[i + 1];
+    if (e0 == e1) return;
+    const int p = kd.order[lv], ncl = kd.ncl[lv];
+    double *o = local + ecoef_off(kd, P, lv);
+    for (int m = threadIdx.x; m < ncl; m += 64) {
+        cplx acc = {0.0, 0.0};
+        for (int e = e0; e < e1; ++e) acc += ldc(Y + 2 * ((long)yoff[pos[e]] + m));
+        double *q = o + 2 * ((long)(m - p) * kd.nr + kd.r0);
+        q[0] += acc.x;
+        q[1] += acc.y;
+    }
+}
+
 // The same product for the coarse levels (48 < ncl <= 256): the block walks
 // the entries together, stages each entry's row window in LDS once, and wave
 // w computes the row tiles w, w + 4, ... (at most M2LM_RTW each) over all
 // entries, so no partial sums need combining.
-constexpr int M2LM_RTW = 4;
-constexpr int M2LM_WIN2 = 16 * 4 * M2LM_RTW + 256;
-
 __global__ void __launch_bounds__(256)
 k_m2l_mfma_rows(KD kd, TD t, Flat fl, const int *ttp, const int *starts, const int *lists,
                 const double *mpole, double *local)
@@ -3252,6 +3423,94 @@ int lpot_matrix_2d(hipStream_t st, const qbx_kernel_desc *d, int nel, int n1, in
     return QBX_OK;
 }
 
+// The class grouping of k_m2l_cls for the levels `lev` of a traversal,
+// built on the host at the first call and kept with the traversal (pairs
+// sorted by (level, class, entry), so the items and sums are the same on
+// every call).
+static int m2l_groups(const qbx_tree *T, const qbx_trav *R, const KD &kd,
+                      const std::vector<int> &lev)
+{
+    if (R->m2lg_nitems >= 0 && R->m2lg_levels == lev) return QBX_OK;
+    const int nttp = R->nttp_boxes, n2 = R->n2, nb = T->nboxes;
+    std::vector<int> starts(nttp + 1), lists(std::max(n2, 1)), ttp(std::max(nttp, 1)),
+        coords(2L * nb);
+    QBX_HIP(hipDeviceSynchronize());
+    QBX_HIP(hipMemcpy(starts.data(), R->l2_starts.p, sizeof(int) * (nttp + 1),
+                      hipMemcpyDeviceToHost));
+    if (n2 > 0)
+        QBX_HIP(hipMemcpy(lists.data(), R->l2_lists.p, sizeof(int) * n2, hipMemcpyDeviceToHost));
+    if (nttp > 0)
+        QBX_HIP(hipMemcpy(ttp.data(), R->ttp_boxes.p, sizeof(int) * nttp, hipMemcpyDeviceToHost));
+    QBX_HIP(hipMemcpy(coords.data(), T->box_coords.p, sizeof(int) * 2L * nb,
+                      hipMemcpyDeviceToHost));
+    const int W = kd.m2l_w, S1 = 2 * W + 1;
+    std::vector<std::pair<long, int>> kv;
+    for (int l : lev)
+        for (int i = R->h_level_start_ttp_box_nrs[l]; i < R->h_level_start_ttp_box_nrs[l + 1];
+             ++i) {
+            const int P = ttp[i];
+            for (int e = starts[i]; e < starts[i + 1]; ++e) {
+                const int A = lists[e];
+                const int dx = coords[P] - coords[A], dy = coords[nb + P] - coords[nb + A];
+                QBX_CHECK(std::abs(dx) <= W && std::abs(dy) <= W,
+                          "M2L: list-2 offset (%d, %d) outside the class table", dx, dy);
+                kv.push_back({(long)l * S1 * S1 + (dx + W) * S1 + (dy + W), e});
+            }
+        }
+    std::sort(kv.begin(), kv.end());
+    const int np = (int)kv.size();
+    std::vector<int> pos(std::max(n2, 1), -1), eo(std::max(np, 1)), items;
+    for (int j = 0; j < np; ++j) {
+        pos[kv[j].second] = j;
+        eo[j] = kv[j].second;
+    }
+    for (int j = 0; j < np;) {
+        int j1 = j;
+        while (j1 < np && kv[j1].first == kv[j].first) ++j1;
+        const int l = (int)(kv[j].first / (S1 * S1)), cls = (int)(kv[j].first % (S1 * S1));
+        for (int a = j; a < j1; a += M2LG_NP) {
+            items.push_back(l);
+            items.push_back(cls);
+            items.push_back(a);
+            items.push_back(std::min(M2LG_NP, j1 - a));
+        }
+        j = j1;
+    }
+    // per-pair row offsets (rows of the pair's level length, in sorted order);
+    // items split into the one-wave (ncl <= 48) and four-wave lists
+    std::vector<int> yoff(std::max(np, 1));
+    long y = 0;
+    for (int j = 0; j < np; ++j) {
+        yoff[j] = (int)y;
+        y += kd.ncl[kv[j].first / (S1 * S1)];
+    }
+    QBX_CHECK(y < (1L << 31), "M2L: %ld grouped rows overflow the row offsets", y);
+    std::vector<int> small, big;
+    for (size_t q = 0; q < items.size(); q += 4) {
+        std::vector<int> &dst = kd.ncl[items[q]] <= M2LM_ROWS * 16 ? small : big;
+        dst.insert(dst.end(), items.begin() + q, items.begin() + q + 4);
+    }
+    const int ns = (int)small.size() / 4, nbg = (int)big.size() / 4;
+    small.insert(small.end(), big.begin(), big.end());
+    QBX_TRY(R->m2lg_e.alloc(std::max(np, 1)));
+    QBX_TRY(R->m2lg_pos.alloc(std::max(n2, 1)));
+    QBX_TRY(R->m2lg_yoff.alloc(std::max(np, 1)));
+    QBX_TRY(R->m2lg_items.alloc(std::max(4 * (ns + nbg), 4)));
+    QBX_HIP(hipMemcpy(R->m2lg_e.p, eo.data(), sizeof(int) * eo.size(), hipMemcpyHostToDevice));
+    QBX_HIP(hipMemcpy(R->m2lg_pos.p, pos.data(), sizeof(int) * pos.size(),
+                      hipMemcpyHostToDevice));
+    QBX_HIP(hipMemcpy(R->m2lg_yoff.p, yoff.data(), sizeof(int) * yoff.size(),
+                      hipMemcpyHostToDevice));
+    if (ns + nbg > 0)
+        QBX_HIP(hipMemcpy(R->m2lg_items.p, small.data(), sizeof(int) * small.size(),
+                          hipMemcpyHostToDevice));
+    R->m2lg_nitems = ns;
+    R->m2lg_nitems2 = nbg;
+    R->m2lg_nrows = y;
+    R->m2lg_levels = lev;
+    return QBX_OK;
+}
+
 }  // namespace qbx
 
 using namespace qbx;
@@ -3533,8 +3792,14 @@ int qbx_multipole_to_local(qbx_ctx *ctx, void *stream, const qbx_tree *T, const 
     // sides per launch -- levels with 2p + 1 <= 48 splitting the entries over
     // the waves, coarser ones (<= 256) splitting the row tiles
     const bool mfma = kd.kernel == QBX_HELMHOLTZ2D && kd.nr >= 4;
-    // path of level l: 0 vector kernel, 1 k_m2l_mfma, 2 k_m2l_mfma_rows
+    // one right-hand side (Helmholtz): the levels with 2p + 1 <= 48 grouped by
+    // translation class on the matrix cores (not in the allocation-free
+    // driver, whose workspace has no room for the per-pair rows)
+    const bool grp = kd.kernel == QBX_HELMHOLTZ2D && kd.nr == 1 && !ctx->tab_m2l_ext;
+    // path of level l: 0 vector kernel, 1 k_m2l_mfma, 2 k_m2l_mfma_rows,
+    // 3 k_m2l_cls, 4 k_m2l_cls_rows
     auto path = [&](int l) {
+        if (grp) return kd.ncl[l] <= M2LM_ROWS * 16 ? 3 : kd.ncl[l] <= 16 * 4 * M2LM_RTW ? 4 : 0;
         if (!mfma) return 0;
         if (kd.ncl[l] <= M2LM_ROWS * 16) return 1;
         return kd.ncl[l] <= 16 * 4 * M2LM_RTW ? 2 : 0;
@@ -3564,6 +3829,33 @@ int qbx_multipole_to_local(qbx_ctx *ctx, void *stream, const qbx_tree *T, const 
             if (nb1 > 0)
                 hipLaunchKernelGGL(k_m2l_mfma, dim3(nb1), dim3(256), 0, st, kd, td, f1,
                                    R->ttp_boxes.p, R->l2_starts.p, R->l2_lists.p, mpole, local);
+        }
+    }
+    if (grp) {
+        std::vector<int> lev;
+        unsigned mask = 0;
+        for (int l = 0; l < kd.nlevels; ++l)
+            if (path(l) >= 3) {
+                lev.push_back(l);
+                mask |= 1u << l;
+            }
+        QBX_TRY(m2l_groups(T, R, kd, lev));
+        if (R->m2lg_nitems + R->m2lg_nitems2 > 0) {
+            QBX_TRY(ws_reserve(ctx, st, ctx->ws_m2lg, 2L * R->m2lg_nrows,
+                               "class-grouped M2L rows"));
+            if (R->m2lg_nitems2 > 0)
+                hipLaunchKernelGGL(k_m2l_cls_rows, dim3(R->m2lg_nitems2), dim3(256), 0, st, kd,
+                                   td, R->m2lg_nitems2, R->m2lg_items.p + 4L * R->m2lg_nitems,
+                                   R->m2lg_e.p, R->m2lg_yoff.p, R->l2_lists.p, mpole,
+                                   ctx->ws_m2lg.p);
+            if (R->m2lg_nitems > 0)
+                hipLaunchKernelGGL(k_m2l_cls, dim3(R->m2lg_nitems), dim3(64), 0, st, kd, td,
+                                   R->m2lg_nitems, R->m2lg_items.p, R->m2lg_e.p,
+                                   R->m2lg_yoff.p, R->l2_lists.p, mpole, ctx->ws_m2lg.p);
+            hipLaunchKernelGGL(k_m2l_cls_sum, dim3(R->nttp_boxes), dim3(64), 0, st, kd, td,
+                               R->nttp_boxes, R->ttp_boxes.p, R->l2_starts.p, R->m2lg_pos.p,
+                               R->m2lg_yoff.p, mask, ctx->ws_m2lg.p, local);
+            QBX_HIP(hipGetLastError());
         }
     }
     // one block per target-or-target-parent box

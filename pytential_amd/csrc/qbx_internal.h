@@ -133,6 +133,9 @@ struct qbx_ctx {
     // 3D Helmholtz: the Gaunt table (degrees <= h3_pg) and the translation
     // tables of the (k, tree extent, nsep, orders) in h3_key
     qbx::dbuf<double> ws_h3_gaunt, ws_h3_tab;
+    // class-grouped one-density M2L: the per-entry products before the
+    // per-box sums
+    qbx::dbuf<double> ws_m2lg;
     int h3_pg = -1;
     std::vector<double> h3_key;
     // side streams of qbx_drive_fmm (default priority; shared by every
@@ -175,6 +178,17 @@ struct qbx_trav {
     // reads) and of a box with a nonempty list 2 or list 4 (the coarsest
     // nonzero local); nlevels when there is none
     int mpole_min_level = 0, local_min_level = 0;
+    // class-grouped one-density M2L (2D Helmholtz; built on first use): the
+    // list-2 entries of the grouped levels sorted by (level, offset class),
+    // each entry's sorted position (-1 for the other levels), and work items
+    // of up to 16 entries of one (level, class): {level, class, first, count},
+    // each entry's row offset in the scratch of products
+    mutable qbx::dbuf<int> m2lg_e, m2lg_pos, m2lg_yoff, m2lg_items;
+    // items of the one-wave kernel (ncl <= 48), then of the four-wave one;
+    // the pairs' rows (complex) in all
+    mutable int m2lg_nitems = -1, m2lg_nitems2 = 0;
+    mutable long m2lg_nrows = 0;
+    mutable std::vector<int> m2lg_levels;  // the grouped levels it was built for
 };
 
 struct qbx_geo {
