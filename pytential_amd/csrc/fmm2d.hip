@@ -1492,8 +1492,11 @@ __device__ __forceinline__ void ht_dot(int rc, const double *sa, int p1, int m,
 // M2M: one block per source parent box at level lvp; its children are staged
 // NS at a time (all four when they fit), so one round of global loads serves
 // several children
+// NG thread groups (blockDim >= NG (2 p2 + 1)) split the staged children;
+// their partial sums are added in LDS in group order (NG = 1: one group).
 template <int NRC>
-__global__ void k_m2m_h(KD kd, TD t, int lvp, int i0, int NS, const int *spb, double *mpole)
+__global__ void k_m2m_h(KD kd, TD t, int lvp, int i0, int NS, int NG, const int *spb,
+                        double *mpole)
 {
     extern __shared__ double lds[];
     const int P = spb[i0 + blockIdx.x];
@@ -1501,8 +1504,10 @@ __global__ void k_m2m_h(KD kd, TD t, int lvp, int i0, int NS, const int *spb, do
     const int na = 2 * p1 + 1, nrow = 2 * L + 1;
     const int per = ht_entry_doubles(NRC, p1, L);
     const int tid = threadIdx.x, nth = blockDim.x;
-    const bool act = tid < 2 * p2 + 1;
-    const int m = tid - p2;
+    const int nout = 2 * p2 + 1;
+    const int grp = tid / nout;
+    const bool act = grp < NG;
+    const int m = tid % nout - p2;
     int kids[4], nk = 0;
     for (int c = 0; c < 4; ++c) {
         const int C = t.child[(size_t)c * t.nboxes + P];
@@ -1524,12 +1529,25 @@ __global__ void k_m2m_h(KD kd, TD t, int lvp, int i0, int NS, const int *spb, do
         }
         __syncthreads();
         if (act)
-            for (int k = 0; k < kn; ++k) {
+            for (int k = grp; k < kn; k += NG) {
                 const double *sa = lds + (long)k * per;
                 ht_dot<NRC>(kd.rc, sa, p1, m, sa + 2 * NRC * na + 2 * L, false, acc);
             }
     }
-    if (act) store_add<NRC>(kd, mpole + ecoef_off(kd, P, lvp) + 2L * m * kd.nr, acc);
+    if (NG > 1) {
+        // the groups' partial sums through LDS (after the last chunk's reads)
+        __syncthreads();
+        cplx *red = reinterpret_cast<cplx *>(lds);
+        if (act && grp > 0)
+#pragma unroll
+            for (int q = 0; q < NRC; ++q) red[((grp - 1) * nout + (m + p2)) * NRC + q] = acc[q];
+        __syncthreads();
+        if (act && grp == 0)
+            for (int g = 1; g < NG; ++g)
+#pragma unroll
+                for (int q = 0; q < NRC; ++q) acc[q] += red[((g - 1) * nout + (m + p2)) * NRC + q];
+    }
+    if (act && grp == 0) store_add<NRC>(kd, mpole + ecoef_off(kd, P, lvp) + 2L * m * kd.nr, acc);
 }
 
 // L2L: one block per target-or-target-parent box at level lv
@@ -3626,11 +3644,21 @@ static int coarsen_multipoles_from(qbx_ctx *ctx, void *stream, const qbx_tree *T
         }
         HtLaunch hl;
         if (ht_launch(kd, kd.order[l], kd.order[l + 1], kd.order[l] + kd.order[l + 1], &hl)) {
+        // levels with few parents (the coarse, high-order ones): a thread
+        // group per staged child, so a block's four children run side by side
 #define L_M2MH(N)                                                                          \
     do {                                                                                   \
         const int ns_ = (int)std::max(1L, std::min(4L, 65536L / (long)ht_bytes<N>(hl)));   \
-        hipLaunchKernelGGL(k_m2m_h<N>, dim3(i1 - i0), dim3(hl.threads), ns_ * ht_bytes<N>(hl), \
-                           st, kd, td, l, i0, ns_, R->source_parent_boxes.p, mpole);        \
+        const int nout_ = 2 * kd.order[l] + 1;                                             \
+        int ng_ = 1, th_ = hl.threads;                                                     \
+        const size_t red_ = sizeof(cplx) * 3 * nout_ * (N);                                \
+        if (i1 - i0 <= 16 && ns_ == 4 && 4 * nout_ <= 1024 &&                              \
+            red_ <= (size_t)ns_ * ht_bytes<N>(hl)) {                                       \
+            ng_ = 4;                                                                       \
+            th_ = 64 * ((4 * nout_ + 63) / 64);                                            \
+        }                                                                                  \
+        hipLaunchKernelGGL(k_m2m_h<N>, dim3(i1 - i0), dim3(th_), ns_ * ht_bytes<N>(hl), st,  \
+                           kd, td, l, i0, ns_, ng_, R->source_parent_boxes.p, mpole);       \
     } while (0)
             FOR_RHS_CHUNKS(kd, NRC_S, L_M2MH);
 #undef L_M2MH
