@@ -1008,7 +1008,7 @@ def run_dirichlet64(args):
     import torch
 
     from pytential_amd.discretization import discretize_curve, starfish
-    from pytential_amd.linalg import MatVecGraph, gmres_many
+    from pytential_amd.linalg import GroupedMatVecGraphs, MatVecGraph, gmres_many
     from pytential_amd.pde import DirichletOperator
     from pytential_amd.qbx import QBXLayerPotentialSource
     from pytential_amd.problems import dirichlet_curves
@@ -1046,21 +1046,47 @@ def run_dirichlet64(args):
 
     ops = [counted(i, c["mv"]) for i, c in enumerate(curves)]
     rhss = [c["f"] for c in curves]
+    # the same solves with one graph per geometry, replayed one after another
     for _ in range(args.warmup):
         gmres_many(ops, rhss, tol=1e-10, restart=30, maxiter=200)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        gmres_many(ops, rhss, tol=1e-10, restart=30, maxiter=200)
+    torch.cuda.synchronize()
+    dt_seq = (time.perf_counter() - t2) / args.steps
+    t3 = time.perf_counter()
+    for c in curves:
+        c["mv"](c["f"])
+    torch.cuda.synchronize()
+    t_mv_seq = time.perf_counter() - t3
+    # all 64 operators in ONE graph on parallel branches: each lockstep
+    # iteration applies every pending operator with one replay
+    t4 = time.perf_counter()
+    bat = GroupedMatVecGraphs([c["op"].matvec for c in curves], [c["f"] for c in curves],
+                              contexts_of=lambda i: curves[i]["op"].library_contexts(),
+                              group=args.batch_group, nstreams=args.batch_streams)
+    torch.cuda.synchronize()
+    t_setup += time.perf_counter() - t4
+
+    def apply_many(pairs):
+        for i, _ in pairs:
+            counts[i] += 1
+        return bat.apply_many(pairs)
+    for _ in range(args.warmup):
+        gmres_many(ops, rhss, tol=1e-10, restart=30, maxiter=200, apply_many=apply_many)
     torch.cuda.synchronize()
     counts[:] = [0] * len(curves)
     t2 = time.perf_counter()
     for _ in range(args.steps):
-        res = gmres_many(ops, rhss, tol=1e-10, restart=30, maxiter=200)
+        res = gmres_many(ops, rhss, tol=1e-10, restart=30, maxiter=200, apply_many=apply_many)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t2) / args.steps
     nmv = sum(counts) / args.steps
     node_mv = sum(n * c for n, c in zip(ndofs, counts)) / args.steps
-    # one captured matvec per geometry, timed alone (the per-matvec figure)
+    # one replay of the batched graph: every geometry's matvec at once
     t3 = time.perf_counter()
-    for c in curves:
-        c["mv"](c["f"])
+    bat.apply_many([(i, c["f"]) for i, c in enumerate(curves)])
     torch.cuda.synchronize()
     t_mv_all = time.perf_counter() - t3
     errs = []
@@ -1087,7 +1113,9 @@ def run_dirichlet64(args):
                                 f"amplitude 0.25, 256..4096 elements log-uniform, 8 nodes "
                                 f"each, fine 16, qbx_order 5, fmm_order 15): interior Laplace "
                                 f"Dirichlet, 0.5u - D u, GMRES(30) to 1e-10 in lockstep, "
-                                f"HIP-graph matvecs"),
+                                f"every iteration's matvecs replays of HIP graphs holding "
+                                f"{args.batch_group} geometries each on {args.batch_streams} "
+                                f"parallel branches (only groups with unconverged systems)"),
                    "ncurves": len(curves), "nodes": int(sum(ndofs)),
                    "elements": int(sum(c["nel"] for c in curves)), "seed": 1003},
         "iterations": {"min": int(min(its)), "max": int(max(its)),
@@ -1095,6 +1123,11 @@ def run_dirichlet64(args):
         "states": states,
         "matvecs_per_step": nmv,
         "all_geometries_one_matvec_ms": round(t_mv_all * 1e3, 3),
+        "one_graph_per_geometry": {
+            "ms_per_step": round(dt_seq * 1e3, 4),
+            "all_geometries_one_matvec_ms": round(t_mv_seq * 1e3, 3),
+            "what": "the same solves with one HIP graph per geometry, replayed one after "
+                    "another"},
         "interior_rel_err_max": max(errs),
         "setup_s": round(t_setup, 3),
         "roofline": None,
@@ -1218,6 +1251,10 @@ def main():
     ap.add_argument("--fine3d", type=int, default=6, help="sphere3d: quad rule n x n")
     ap.add_argument("--nrhs3d", type=int, default=32, help="sphere3d: densities")
     ap.add_argument("--steps3d", type=int, default=5, help="sphere3d: timed steps")
+    ap.add_argument("--batch-streams", type=int, default=4,
+                    help="dirichlet64: parallel branches of each batched matvec graph")
+    ap.add_argument("--batch-group", type=int, default=4,
+                    help="dirichlet64: geometries per batched matvec graph")
     ap.add_argument("--profile", action="store_true",
                     help="one configuration only (tolerance orders), no CPU leg: "
                          "the run profiled under rocprofv3")

@@ -318,7 +318,12 @@ enum {
     QBX_TUNE_M2QBXL_UNIT_ITER = 3,  /* M2QBXL: list-3 boxes per lane (1..64) */
     QBX_TUNE_L2QBXL_LANES = 4,      /* 2D Helmholtz one-density L2QBXL: lanes per
                                        center (1, 2, 4, 8) */
-    QBX_TUNE_COUNT = 5
+    QBX_TUNE_DRIVE_STREAMS = 5,     /* qbx_drive_fmm: 4 = its stage DAG on the caller's
+                                       stream and three side streams (default); 1 = every
+                                       stage on the caller's stream, so that many contexts
+                                       can run side by side on streams of their own (one
+                                       graph holding several evaluations) */
+    QBX_TUNE_COUNT = 6
 };
 int qbx_ctx_set_tuning(qbx_ctx *ctx, int param, int value);
 int qbx_ctx_get_tuning(qbx_ctx *ctx, int param, int *value);

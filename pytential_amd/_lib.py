@@ -17,6 +17,8 @@ QBX_ERR_TARGET_ASSOC_FAILED = 3
 LAPLACE2D, HELMHOLTZ2D, LAPLACE3D, HELMHOLTZ3D = 0, 1, 2, 3
 TSQBX_S, TSQBX_D, TSQBX_SP = 0, 1, 2
 SOURCE_S, SOURCE_D = 0, 1
+# QBX_TUNE_DRIVE_STREAMS (include/qbx_mi355x.h): 1 = qbx_drive_fmm on the caller's stream only
+TUNE_DRIVE_STREAMS = 5
 
 P = ct.c_void_p
 I = ct.c_int

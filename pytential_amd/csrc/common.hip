@@ -279,9 +279,10 @@ int qbx_ctx_set_tuning(qbx_ctx *ctx, int param, int value)
     QBX_CHECK(ctx, "qbx_ctx_set_tuning: null context");
     QBX_CHECK(param >= 0 && param < QBX_TUNE_COUNT, "qbx_ctx_set_tuning: unknown knob %d",
               param);
-    static const int lo[QBX_TUNE_COUNT] = {16, 1, 4, 1, 1};
-    static const int hi[QBX_TUNE_COUNT] = {1024, 64, 512, 64, 8};
-    QBX_CHECK(value >= lo[param] && value <= hi[param] && (value & (value - 1)) == 0,
+    static const int lo[QBX_TUNE_COUNT] = {16, 1, 4, 1, 1, 1};
+    static const int hi[QBX_TUNE_COUNT] = {1024, 64, 512, 64, 8, 4};
+    QBX_CHECK(value >= lo[param] && value <= hi[param] && (value & (value - 1)) == 0 &&
+                  (param != QBX_TUNE_DRIVE_STREAMS || value != 2),
               "qbx_ctx_set_tuning: knob %d takes a power of two in [%d, %d] (got %d)", param,
               lo[param], hi[param], value);
     ctx->tune[param] = value;

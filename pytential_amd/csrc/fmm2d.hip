@@ -4606,7 +4606,8 @@ int qbx_drive_fmm(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav 
         QBX_TRY(ws_reserve(ctx, S(stream), ctx->ws_allreal, 1, "strength flag"));
         b.allreal = ctx->ws_allreal.p;
     }
-    return drive_impl(ctx, stream, T, R, G, d, str, nrm, out_user, z, b, true);
+    return drive_impl(ctx, stream, T, R, G, d, str, nrm, out_user, z, b,
+                      ctx->tune[QBX_TUNE_DRIVE_STREAMS] > 1);
 }
 
 int qbx_drive_fmm_noalloc(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,

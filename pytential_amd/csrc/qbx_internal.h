@@ -128,7 +128,7 @@ struct qbx_ctx {
     qbx::dbuf<double> ws_adj_m, ws_adj_l, ws_adj_q, ws_adj_p;
     // tuning knobs (QBX_TUNE_*): unit-plan list items per lane, L2QBXL lanes
     int tune[QBX_TUNE_COUNT] = {qbx::UNIT_ITER, qbx::M2P_UNIT_ITER, qbx::P2Q_UNIT_ITER,
-                                qbx::M2Q_UNIT_ITER, 4};
+                                qbx::M2Q_UNIT_ITER, 4, 4};
     std::vector<double> tab3_key;
     // 3D Helmholtz: the Gaunt table (degrees <= h3_pg) and the translation
     // tables of the (k, tree extent, nsep, orders) in h3_key

@@ -199,9 +199,92 @@ class MatVecGraph:
     __call__ = matvec
 
 
+class BatchedMatVecGraph:
+    """Several operators captured into ONE HIP graph, on parallel branches:
+    the capturing stream forks to ``nstreams`` side streams, operator i runs
+    on side stream i % nstreams, and the branches join again, so one replay
+    applies every operator and independent evaluations of small geometries
+    share the chip instead of running one after another.  Each operator must
+    be capturable (see MatVecGraph) and own its library contexts; those
+    contexts are set to run qbx_drive_fmm on their caller's stream alone
+    (QBX_TUNE_DRIVE_STREAMS = 1), so no two branches meet on the driver's
+    shared side streams.  ``apply_many([(i, x_i), ...])`` copies the given
+    inputs in, replays once and returns the outputs of those operators
+    (operators not named run on their previous input; their outputs are not
+    returned)."""
+
+    def __init__(self, fns, examples, contexts=(), nstreams=8):
+        from . import _lib
+        for ctx in contexts:
+            _lib.call("qbx_ctx_set_tuning", ctx.h, _lib.TUNE_DRIVE_STREAMS, 1)
+        self.fns = list(fns)
+        self._in = [e.clone() for e in examples]
+        for f, x in zip(self.fns, self._in):    # warm-up: tables, workspaces
+            f(x)
+        torch.cuda.synchronize()
+        self.streams = [torch.cuda.Stream() for _ in range(max(1, nstreams))]
+        self.graph = torch.cuda.CUDAGraph()
+        gc.collect()
+        was_enabled = gc.isenabled()
+        gc.disable()
+        try:
+            with torch.cuda.graph(self.graph):
+                cap = torch.cuda.current_stream()
+                for st in self.streams:
+                    st.wait_stream(cap)
+                outs = []
+                for i, (f, x) in enumerate(zip(self.fns, self._in)):
+                    with torch.cuda.stream(self.streams[i % len(self.streams)]):
+                        outs.append(f(x))
+                for st in self.streams:
+                    cap.wait_stream(st)
+        finally:
+            if was_enabled:
+                gc.enable()
+        self._out = outs
+        torch.cuda.synchronize()
+
+    def apply_many(self, pairs):
+        for i, x in pairs:
+            self._in[i].copy_(x)
+        self.graph.replay()
+        return [self._out[i].clone() for i, _ in pairs]
+
+    def matvec(self, i, x):
+        return self.apply_many([(i, x)])[0]
+
+
+class GroupedMatVecGraphs:
+    """Operators split into groups of ``group`` consecutive ones, each group a
+    BatchedMatVecGraph: ``apply_many`` replays only the groups holding a
+    requested operator, so systems that have converged stop costing their
+    group's replay once the whole group has."""
+
+    def __init__(self, fns, examples, contexts_of=None, group=8, nstreams=4):
+        fns, examples = list(fns), list(examples)
+        self.group = max(1, group)
+        self.batches = []
+        for g0 in range(0, len(fns), self.group):
+            idx = range(g0, min(len(fns), g0 + self.group))
+            ctxs = [c for i in idx for c in (contexts_of(i) if contexts_of else ())]
+            self.batches.append(BatchedMatVecGraph([fns[i] for i in idx],
+                                                   [examples[i] for i in idx], ctxs, nstreams))
+
+    def apply_many(self, pairs):
+        by = {}
+        for k, (i, x) in enumerate(pairs):
+            by.setdefault(i // self.group, []).append((k, i % self.group, x))
+        out = [None] * len(pairs)
+        for g, items in by.items():
+            ys = self.batches[g].apply_many([(j, x) for _, j, x in items])
+            for (k, _, _), y in zip(items, ys):
+                out[k] = y
+        return out
+
+
 def gmres_many(ops, rhss, restart=None, tol=None, maxiter=None, hard_failure=False,
                no_progress_factor=None, stall_iterations=None,
-               require_monotonicity=True) -> list[GMRESResult]:
+               require_monotonicity=True, apply_many=None) -> list[GMRESResult]:
     """Solve independent systems ``ops[i](x) = rhss[i]`` in lockstep: every
     iteration advances each unfinished system by one Arnoldi step (its
     operator application, classical Gram-Schmidt applied twice, all on the
@@ -210,7 +293,10 @@ def gmres_many(ops, rhss, restart=None, tol=None, maxiter=None, hard_failure=Fal
     iteration for the whole batch instead of once per system.  Restart,
     tolerance, stall and monotonicity rules and the outcome states are those
     of ``gmres`` (the reference's, gmres.py:352-413), applied per system;
-    ``hard_failure`` raises on the first failed system."""
+    ``hard_failure`` raises on the first failed system.  ``apply_many``
+    (e.g. ``BatchedMatVecGraph.apply_many``) applies the operators of a list
+    of ``(index, vector)`` pairs at once; given, each iteration's operator
+    applications are one call."""
     m = len(ops)
     if len(rhss) != m:
         raise ValueError("need one right-hand side per operator")
@@ -239,8 +325,15 @@ def gmres_many(ops, rhss, restart=None, tol=None, maxiter=None, hard_failure=Fal
             s.done, s.state = True, "success"
             s.res = [0.0]
 
+    def apply(pairs):
+        if apply_many is not None:
+            return apply_many(pairs) if pairs else []
+        return [applies[i](x) for i, x in pairs]
+
     def start_cycle(active):
-        rs = [s.b - applies[i](s.x) if s.it > 0 else s.b.clone() for i, s in active]
+        need = [(i, s.x) for i, s in active if s.it > 0]
+        ax = dict(zip([i for i, _ in need], apply(need)))
+        rs = [s.b - ax[i] if s.it > 0 else s.b.clone() for i, s in active]
         betas = torch.stack([torch.linalg.vector_norm(r) for r in rs]).cpu().numpy()
         out = []
         for (i, s), r, beta in zip(active, rs, betas):
@@ -277,8 +370,8 @@ def gmres_many(ops, rhss, restart=None, tol=None, maxiter=None, hard_failure=Fal
     while active:
         # one Arnoldi step per active system, device only
         cols = []
-        for i, s in active:
-            w = applies[i](s.V[s.j])
+        ws = apply([(i, s.V[s.j]) for i, s in active])
+        for (i, s), w in zip(active, ws):
             Vj = s.V[:s.j + 1].reshape(s.j + 1, -1)
             wf = w.reshape(-1)
             h = torch.conj(Vj) @ wf

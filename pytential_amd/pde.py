@@ -80,6 +80,11 @@ class DirichletOperator:
 
     __call__ = matvec
 
+    def library_contexts(self):
+        """The library contexts of this operator's evaluations (its own, see
+        private_context), e.g. to set their QBX_TUNE_* knobs."""
+        return [w.ctx for w in (self.w_d, self.w_s) if w is not None]
+
     @property
     def shape(self):
         return (self.n, self.n)

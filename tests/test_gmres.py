@@ -85,6 +85,33 @@ def test_gmres_many_matches_single_solves():
         assert one.iteration_count == r.iteration_count
 
 
+def test_gmres_many_apply_many_matches_per_system_calls():
+    """The batched application path (one call applying every pending
+    operator, as BatchedMatVecGraph does) gives the per-call path's
+    iterations and solutions exactly."""
+    from pytential_amd.linalg import gmres_many
+    rng = np.random.default_rng(32)
+    ops, rhss = [], []
+    for n in (30, 12, 45):
+        A = torch.as_tensor(2 * np.eye(n) + rng.normal(size=(n, n)) / np.sqrt(n))
+        ops.append(lambda x, A=A: A @ x)
+        rhss.append(torch.as_tensor(rng.normal(size=n)))
+    calls = []
+
+    def apply_many(pairs):
+        calls.append(len(pairs))
+        return [ops[i](x) for i, x in pairs]
+    a = gmres_many(ops, rhss, tol=1e-11, restart=5, maxiter=200)
+    b = gmres_many(ops, rhss, tol=1e-11, restart=5, maxiter=200, apply_many=apply_many)
+    for ra, rb in zip(a, b):
+        assert ra.state == rb.state == "success"
+        assert ra.iteration_count == rb.iteration_count
+        assert torch.equal(ra.solution, rb.solution)
+    # one call per lockstep iteration or restart, each for several systems
+    assert len(calls) < sum(r.iteration_count for r in a)
+    assert max(calls) == 3
+
+
 def test_gmres_many_failure_states():
     from pytential_amd.linalg import GMRESError, gmres_many
     rng = np.random.default_rng(32)
