@@ -328,12 +328,13 @@ enum {
 int qbx_ctx_set_tuning(qbx_ctx *ctx, int param, int value);
 int qbx_ctx_get_tuning(qbx_ctx *ctx, int param, int *value);
 
-/* ---- adjoint QBX-FMM (2D, one density) --------------------------------------
+/* ---- adjoint QBX-FMM (one density) ------------------------------------------
  * No reference counterpart: pytential differentiates only by differencing or
  * through its dense matrix builder (pytential/symbolic/matrix.py:376-522),
  * whose conjugate transpose this applies in O(N).  For the forward map
  * out_user = A strengths_tree of qbx_drive_fmm (2D Laplace or Helmholtz, S or
- * D, nrhs 1, no target derivative) it computes str_adj = A^H pot_adj_user
+ * D; 3D Helmholtz S; nrhs 1, no target derivative) it computes
+ * str_adj = A^H pot_adj_user
  * through the transposed stages on the same tree, lists and orders (Laplace:
  * the transpose of the real-linear map strengths -> real potentials).
  * pot_adj_user [n_user_targets] complex, str_adj [nsources] complex, tree

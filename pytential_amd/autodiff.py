@@ -1,9 +1,10 @@
 """Differentiable layer potentials: ``torch.autograd.Function``s whose
 backward applies hand-written adjoint kernels.
 
-* ``layer_potential`` (2D, QBX-FMM): S or D of a curve density at on- and
-  off-surface targets; the backward runs the transposed QBX-FMM
-  (QBXLayerPotential2D below).
+* ``layer_potential`` (QBX-FMM): S or D of a curve density (2D Laplace and
+  Helmholtz), or the 3D Helmholtz single layer of a surface density, at on-
+  and off-surface targets; the backward runs the transposed QBX-FMM
+  (QBXLayerPotential below).
 * ``single_layer_far`` (3D Helmholtz, far targets): the single layer of a
   surface density at smooth targets, described next.
 
@@ -109,10 +110,10 @@ def single_layer_far(src, sigma, k, targets):
     return HelmholtzSingleLayerFar.apply(sig, k, src, tg)
 
 
-class QBXLayerPotential2D(torch.autograd.Function):
-    """S or D of a stage-1 nodal density on a curve at (targets, side) pairs
-    through the QBX-FMM, differentiable in the density and (Helmholtz) the
-    wavenumber.  Backward:
+class QBXLayerPotential(torch.autograd.Function):
+    """S or D of a stage-1 nodal density on a curve (or S_k on a surface, 3D
+    Helmholtz) at (targets, side) pairs through the QBX-FMM, differentiable
+    in the density and (Helmholtz) the wavenumber.  Backward:
 
         d/dsigma = U^T W A(k)^H g   (the transposed QBX-FMM,
                                     qbx_drive_fmm_adjoint: QBXL2P^T, L2QBXL^T,
@@ -160,12 +161,17 @@ class QBXLayerPotential2D(torch.autograd.Function):
         return grad_sigma, grad_k, None, None, None, None
 
 
+QBXLayerPotential2D = QBXLayerPotential
+
+
 def layer_potential(src, kernel, sigma, targets_and_sides, k=0.0, source_kind="S"):
     """``exec_compute_potential`` of one density, concatenated over the
-    target pairs, differentiable in ``sigma`` and (a tensor) ``k`` (2D
-    QBX-FMM; see QBXLayerPotential2D)."""
-    if src.ambient_dim != 2:
-        raise ValueError("layer_potential differentiates the 2D QBX-FMM")
+    target pairs, differentiable in ``sigma`` and (a tensor) ``k`` (the
+    QBX-FMM of the 2D kernels or of the 3D Helmholtz single layer; see
+    QBXLayerPotential)."""
+    if src.ambient_dim == 3 and not (kernel == "helmholtz" and source_kind == "S"):
+        raise ValueError("layer_potential differentiates the 2D QBX-FMM and the 3D "
+                         "Helmholtz single layer")
     sig = sigma if isinstance(sigma, torch.Tensor) else torch.as_tensor(sigma)
     sig = sig.to(device="cuda")
     if kernel == "laplace":
@@ -174,5 +180,5 @@ def layer_potential(src, kernel, sigma, targets_and_sides, k=0.0, source_kind="S
         sig = sig.to(torch.complex128)
     if not isinstance(k, torch.Tensor):
         k = torch.tensor(float(k), dtype=torch.float64)
-    return QBXLayerPotential2D.apply(sig, k, src, kernel, source_kind,
+    return QBXLayerPotential.apply(sig, k, src, kernel, source_kind,
                                      tuple(targets_and_sides))

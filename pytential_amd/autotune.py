@@ -32,6 +32,12 @@ KNOBS = {
     "m2qbxl_unit_iter": (3, (1, 2, 4, 8)),
     "l2qbxl_lanes": (4, (1, 2, 4, 8)),
 }
+# settings that are not searched (the caller picks them): the stream layout
+# of qbx_drive_fmm, 4 (its stage DAG on four streams) or 1 (the caller's
+# stream only, for contexts that share a graph)
+SETTINGS = {
+    "drive_streams": 5,
+}
 
 
 def get(ctx):
@@ -47,9 +53,13 @@ def get(ctx):
 def apply(ctx, settings):
     """Set knob values (a dict name -> value) on a context."""
     for name, value in settings.items():
-        if name not in KNOBS:
+        if name in KNOBS:
+            pid = KNOBS[name][0]
+        elif name in SETTINGS:
+            pid = SETTINGS[name]
+        else:
             raise ValueError(f"unknown tuning knob {name!r}")
-        _lib.call("qbx_ctx_set_tuning", ctx.h, KNOBS[name][0], int(value))
+        _lib.call("qbx_ctx_set_tuning", ctx.h, pid, int(value))
 
 
 def signature(src, targets_and_sides, kernel, k, source_kind, nrhs):

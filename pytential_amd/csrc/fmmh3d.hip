@@ -934,6 +934,437 @@ __global__ void kh_qbxl2p(HD hd, int nctt, const int *ctt_lists, const int *tree
         if (r < hd.rc) stc(pot + 2 * ((size_t)ti * hd.nr + hd.r0 + r), acc[r]);
 }
 
+// ---- adjoint (conjugate transpose) of the point-FMM stages, single layer --
+// With the scaled storage of the forward (M~ = M / s^n, L~ = L s^n):
+//   P2M^T  q'_s += -ik sum_{n,m} R_n^m(s - c) s^-n M~'[n,m]
+//   P2L^T  q'_s += -ik sum_B sum conj(h_n) Y_n^m(s - c_B) s^n L~'_B[n,m]
+//   L2P^T  L~'[n,m] += sum_t (-1)^m R_n^{-m}(t - c) s^-n pot'_t
+//   M2P^T  M~'_D[n,m] += sum_t conj(h_n) (-1)^m Y_n^{-m}(t - c_D) s^n pot'_t
+//   M2M^T, M2L^T, L2L^T  in' += T^H out' with the forward's tables
+//   P2P^T  q'_s += sum_t conj(G(t, s)) pot'_t
+// Scatters (list 2, 3, 4 from the target side, L2L^T into parents, P2P^T)
+// add with atomics; the rest have one writer.
+
+__device__ __forceinline__ void atomic_addc(double *p, cplx v)
+{
+    atomicAdd(p, v.x);
+    atomicAdd(p + 1, v.y);
+}
+
+// P2M^T (LOCAL false: the box's own sources, coefficients M~') and P2L^T
+// (LOCAL true: the sources of the box's list-4 boxes, coefficients L~'):
+// a 64-lane block per box, lanes over the points
+template <bool LOCAL>
+__global__ void __launch_bounds__(64) kh_adj_p2e(HD hd, TD t, int nitems, const int *items,
+                                                 const int *starts, const int *lists,
+                                                 const double *src, int ns,
+                                                 const double *expn, double *sadj)
+{
+    __shared__ double rads[64 * 2 * (MAXP + 1)];
+    __shared__ double jts[64 * (MAXP + 3)];
+    const int it = blockIdx.x;
+    if (it >= nitems) return;
+    const int B = items[it];
+    const int lv = t.level[B];
+    const int p = hd.order[lv];
+    const v3 c = boxc(t, B);
+    const int lane = threadIdx.x;
+    cplx *rad = reinterpret_cast<cplx *>(rads) + lane * (MAXP + 1);
+    double *jt = jts + lane * (MAXP + 3);
+    const double *coef = expn + ecoef_off(hd, B, lv);
+    const double s = LOCAL ? hd.s[lv] : 1.0 / hd.s[lv];
+    const int e0 = LOCAL ? starts[it] : 0, e1 = LOCAL ? starts[it + 1] : 1;
+    for (int e = e0; e < e1; ++e) {
+        const int Sb = LOCAL ? lists[e] : B;
+        const int s0 = t.sstart[Sb], s1 = s0 + t.scount[Sb];
+        for (int si = s0 + lane; si < s1; si += 64) {
+            const v3 d = sub(ld3(src, ns, si), c);
+            const double rr = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+            radial_seq(LOCAL ? 1 : 0, p, hd.k * rr, rad, jt);
+            double sp = 1.0;
+            for (int n = 0; n <= p; ++n) {
+                rad[n] = sp * (LOCAL ? conj(rad[n]) : rad[n]);
+                sp *= s;
+            }
+            cplx acc[1] = {{0.0, 0.0}};
+            sum_waves<1>(hd, coef, p, d, rad, acc);
+            const cplx v = cplx{0.0, -hd.k} * acc[0];
+            if (LOCAL)
+                atomic_addc(sadj + 2 * (size_t)si, v);
+            else
+                stc(sadj + 2 * (size_t)si, ldc(sadj + 2 * (size_t)si) + v);
+        }
+    }
+}
+
+// L2P^T (LOCAL: the box's own local) and M2P^T (list-3 boxes): a block per
+// target box, a thread per coefficient of the output expansion; per entry
+// the waves about its center at the box's non-QBX targets staged in LDS in
+// chunks of 8 targets
+template <bool LOCAL>
+__global__ void __launch_bounds__(448) kh_adj_e2p(HD hd, TD t, int ntb, const int *tboxes,
+                                                  const int *starts, const int *lists,
+                                                  const int *fstart, const int *fcount,
+                                                  const int *unfilt, const double *tgt,
+                                                  int nt, const double *pot, double *xadj)
+{
+    constexpr int TC = 8;
+    __shared__ double Fs[2 * TC * (MAXP + 1) * (MAXP + 1)];
+    __shared__ double rads[2 * TC * (MAXP + 1)];
+    __shared__ double jts[TC * (MAXP + 3)];
+    __shared__ double ds[3 * TC];
+    __shared__ double gs[2 * TC];
+    const int itb = blockIdx.x;
+    if (itb >= ntb) return;
+    const int B = tboxes[itb];
+    const int f0 = fstart[B], nf = fcount[B];
+    if (nf == 0) return;
+    const int e0 = LOCAL ? 0 : starts[itb], e1 = LOCAL ? 1 : starts[itb + 1];
+    const int tid = threadIdx.x;
+    cplx *F = reinterpret_cast<cplx *>(Fs);
+    cplx *rad = reinterpret_cast<cplx *>(rads);
+    for (int e = e0; e < e1; ++e) {
+        const int D = LOCAL ? B : lists[e];
+        const int lv = t.level[D];
+        const int p = hd.order[lv];
+        const int nco = (p + 1) * (p + 1), ncf = (MAXP + 1) * (MAXP + 1);
+        const v3 c = boxc(t, D);
+        int n_o = 0, m_o = 0;
+        if (tid < nco) nm_of(tid, n_o, m_o);
+        cplx acc = {0.0, 0.0};
+        for (int c0 = 0; c0 < nf; c0 += TC) {
+            const int cnt = min(TC, nf - c0);
+            __syncthreads();  // the previous chunk consumed
+            if (tid < cnt) {
+                const int ti = unfilt[f0 + c0 + tid];
+                const v3 d = sub(ld3(tgt, nt, ti), c);
+                ds[3 * tid] = d.x;
+                ds[3 * tid + 1] = d.y;
+                ds[3 * tid + 2] = d.z;
+                gs[2 * tid] = pot[2 * (size_t)ti];
+                gs[2 * tid + 1] = pot[2 * (size_t)ti + 1];
+                const double rr = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+                cplx *rd = rad + tid * (MAXP + 1);
+                radial_seq(LOCAL ? 0 : 1, p, hd.k * rr, rd, jts + tid * (MAXP + 3));
+                if (!LOCAL)
+                    for (int n = 0; n <= p; ++n) rd[n] = conj(rd[n]);
+            }
+            __syncthreads();
+            for (int w = tid; w < cnt * (p + 1); w += blockDim.x) {
+                const int q = w / (p + 1), m = w % (p + 1);
+                const v3 d = {ds[3 * q], ds[3 * q + 1], ds[3 * q + 2]};
+                cplx *Fq = F + (long)q * ncf;
+                const cplx *rd = rad + q * (MAXP + 1);
+                ylm_column(m, p, d, [&](int n, cplx y) {
+                    Fq[IDX(n, m)] = rd[n] * y;
+                    if (m > 0) Fq[IDX(n, -m)] = rd[n] * sgn_conj(y, m);
+                });
+            }
+            __syncthreads();
+            if (tid < nco)
+                for (int q = 0; q < cnt; ++q) {
+                    cplx f = F[(long)q * ncf + IDX(n_o, -m_o)];
+                    if (m_o & 1) f = -f;
+                    acc += cplx{gs[2 * q], gs[2 * q + 1]} * f;
+                }
+        }
+        if (tid < nco) {
+            const double sc = ipow(LOCAL ? 1.0 / hd.s[lv] : hd.s[lv], n_o);
+            double *o = xadj + ecoef_off(hd, D, lv) + 2L * tid * hd.nr;
+            if (LOCAL)
+                stc(o, ldc(o) + sc * acc);
+            else
+                atomic_addc(o, sc * acc);
+        }
+    }
+}
+
+// M2M^T (OP 0: parents of level lv to their children), M2L^T (OP 1:
+// target-or-target-parent boxes of level lv to their list-2 boxes), L2L^T
+// (OP 2: boxes of level lv >= 1 to their parents): in'_j += sum_i
+// conj(T[j][i]) out'_i, a thread per input coefficient j, the output box's
+// adjoint staged in LDS
+template <int OP>
+__global__ void __launch_bounds__(448) kh_trans_adj(HD hd, TD t, int lv, int i0, int nbox,
+                                                    const int *boxes, const int *starts,
+                                                    const int *lists, const double *outa,
+                                                    double *ina)
+{
+    __shared__ double ys[2 * (MAXP + 1) * (MAXP + 1)];
+    cplx *y = reinterpret_cast<cplx *>(ys);
+    const int ib = blockIdx.x;
+    if (ib >= nbox) return;
+    const int i = i0 + ib;
+    const int B = boxes[i];
+    const int lvin = OP == 0 ? lv + 1 : OP == 2 ? lv - 1 : lv;
+    const int pin = hd.order[lvin], pout = hd.order[lv];
+    const int ncin = (pin + 1) * (pin + 1), ncout = (pout + 1) * (pout + 1);
+    const int tid = threadIdx.x;
+    const int S1 = 2 * hd.W + 1;
+    const int nb = t.nboxes;
+    const int bx = t.coords[B], by = t.coords[nb + B], bz = t.coords[2 * nb + B];
+    for (int w = tid; w < ncout; w += blockDim.x)
+        y[w] = ldc(outa + ecoef_off(hd, B, lv) + 2L * w * hd.nr);
+    __syncthreads();
+    int ne;
+    if (OP == 0) ne = 8;
+    else if (OP == 1) ne = starts[i + 1] - starts[i];
+    else ne = 1;
+    for (int e = 0; e < ne; ++e) {
+        int A, cls;
+        if (OP == 0) {
+            A = t.child[(long)e * nb + B];
+            if (A < 0 || t.scumul[A] == 0) continue;
+            cls = (t.coords[A] & 1) | ((t.coords[nb + A] & 1) << 1) |
+                  ((t.coords[2 * nb + A] & 1) << 2);
+        } else if (OP == 1) {
+            A = lists[starts[i] + e];
+            const int dx = bx - t.coords[A] + hd.W, dy = by - t.coords[nb + A] + hd.W,
+                      dz = bz - t.coords[2 * nb + A] + hd.W;
+            if ((unsigned)dx >= (unsigned)S1 || (unsigned)dy >= (unsigned)S1 ||
+                (unsigned)dz >= (unsigned)S1)
+                continue;
+            cls = (dx * S1 + dy) * S1 + dz;
+        } else {
+            A = t.parent[B];
+            if (A < 0) continue;
+            cls = (bx & 1) | ((by & 1) << 1) | ((bz & 1) << 2);
+        }
+        const long toff = OP == 0 ? hd.m2m_off[lv] : OP == 1 ? hd.m2l_off[lv] : hd.l2l_off[lv];
+        const double *T = hd.tab + 2 * (toff + (long)cls * ncin * ncout);
+        for (int j = tid; j < ncin; j += blockDim.x) {
+            cplx acc = {0.0, 0.0};
+            for (int q = 0; q < ncout; ++q) acc += conj(ldc(T + 2 * ((long)j * ncout + q))) * y[q];
+            double *o = ina + ecoef_off(hd, A, lvin) + 2L * j * hd.nr;
+            if (OP == 0)
+                stc(o, ldc(o) + acc);
+            else
+                atomic_addc(o, acc);
+        }
+    }
+}
+
+// P2P^T: a 64-lane block per target box, lanes over its list-1 sources,
+// each summing over the box's non-QBX targets; atomics into q'
+__global__ void __launch_bounds__(64) kh_adj_p2p(HD hd, TD t, int ntb, const int *tboxes,
+                                                 const int *starts, const int *lists,
+                                                 const int *fstart, const int *fcount,
+                                                 const int *unfilt, const double *tgt, int nt,
+                                                 const double *src, int ns, const double *pot,
+                                                 double *sadj)
+{
+    const int itb = blockIdx.x;
+    if (itb >= ntb) return;
+    const int B = tboxes[itb];
+    const int f0 = fstart[B], nf = fcount[B];
+    if (nf == 0) return;
+    for (int e = starts[itb]; e < starts[itb + 1]; ++e) {
+        const int Sb = lists[e];
+        const int s0 = t.sstart[Sb], s1 = s0 + t.scount[Sb];
+        for (int si = s0 + threadIdx.x; si < s1; si += 64) {
+            const v3 sp = ld3(src, ns, si);
+            cplx acc = {0.0, 0.0};
+            for (int f = f0; f < f0 + nf; ++f) {
+                const int ti = unfilt[f];
+                const cplx g = p2p_val(hd.k, false, sp, v3{0, 0, 0}, ld3(tgt, nt, ti));
+                acc += conj(g) * ldc(pot + 2 * (size_t)ti);
+            }
+            atomic_addc(sadj + 2 * (size_t)si, acc);
+        }
+    }
+}
+
+// ---- adjoint of the QBX stages (unscaled QBX expansions Lq_c) -------------
+//   QBXL2P^T  Lq'_c[n,m] = sum_{t of c} (-1)^m R_n^{-m}(t - c) pot'_t
+//   L2QBXL^T  L~'_B[j] += 4 pi s^-n sum_o conj(cf^R(j, o)) Lq'_c[o]
+//   M2QBXL^T  M~'_D[j] += 4 pi s^n sum_o conj(cf^S(j, o)) Lq'_c[o]
+//   P2QBXL^T  q'_s += -ik sum_c sum_o conj(h_nu) Y_nu^mu(s - c) Lq'_c[o]
+// with cf(j, o) the forward's Gaunt sum (kh_e2qbxl).
+
+// QBXL2P^T: a block per center (every center: those without targets get
+// zeros), a thread per QBX coefficient, the center's targets staged in LDS
+// in chunks of 16
+__global__ void __launch_bounds__(128) kh_adj_qbxl2p(HD hd, int nc, const int *cts,
+                                                     const int *ctl, const double *tgt, int nt,
+                                                     const double *ctr, const double *pot,
+                                                     double *qadj)
+{
+    constexpr int TC = 16;
+    constexpr int NCF = (MAXPQ + 1) * (MAXPQ + 1);
+    __shared__ double Fs[2 * TC * NCF];
+    __shared__ double rads[2 * TC * (MAXPQ + 1)];
+    __shared__ double jts[TC * (MAXPQ + 3)];
+    __shared__ double ds[3 * TC];
+    __shared__ double gs[2 * TC];
+    const int c = blockIdx.x;
+    if (c >= nc) return;
+    const int tid = threadIdx.x;
+    const int pq = hd.pq, nco = hd.ncq;
+    cplx *F = reinterpret_cast<cplx *>(Fs);
+    cplx *rad = reinterpret_cast<cplx *>(rads);
+    const v3 cc = ld3(ctr, nc, c);
+    int n_o = 0, m_o = 0;
+    if (tid < nco) nm_of(tid, n_o, m_o);
+    cplx acc = {0.0, 0.0};
+    const int t0 = cts[c], ntc = cts[c + 1] - t0;
+    for (int c0 = 0; c0 < ntc; c0 += TC) {
+        const int cnt = min(TC, ntc - c0);
+        __syncthreads();  // the previous chunk consumed
+        if (tid < cnt) {
+            const int ti = ctl[t0 + c0 + tid];
+            const v3 d = sub(ld3(tgt, nt, ti), cc);
+            ds[3 * tid] = d.x;
+            ds[3 * tid + 1] = d.y;
+            ds[3 * tid + 2] = d.z;
+            gs[2 * tid] = pot[2 * (size_t)ti];
+            gs[2 * tid + 1] = pot[2 * (size_t)ti + 1];
+            const double rr = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+            double *jt = jts + tid * (MAXPQ + 3);
+            sph_j_seq(pq, hd.k * rr, jt);
+            for (int n = 0; n <= pq; ++n) rad[tid * (MAXPQ + 1) + n] = {jt[n], 0.0};
+        }
+        __syncthreads();
+        for (int w = tid; w < cnt * (pq + 1); w += blockDim.x) {
+            const int q = w / (pq + 1), m = w % (pq + 1);
+            const v3 d = {ds[3 * q], ds[3 * q + 1], ds[3 * q + 2]};
+            cplx *Fq = F + q * NCF;
+            const cplx *rd = rad + q * (MAXPQ + 1);
+            ylm_column(m, pq, d, [&](int n, cplx y) {
+                Fq[IDX(n, m)] = rd[n] * y;
+                if (m > 0) Fq[IDX(n, -m)] = rd[n] * sgn_conj(y, m);
+            });
+        }
+        __syncthreads();
+        if (tid < nco)
+            for (int q = 0; q < cnt; ++q) {
+                cplx f = F[q * NCF + IDX(n_o, -m_o)];
+                if (m_o & 1) f = -f;
+                acc += cplx{gs[2 * q], gs[2 * q + 1]} * f;
+            }
+    }
+    if (tid < nco) stc(qadj + 2 * ((size_t)c * nco + tid), acc);
+}
+
+// L2QBXL^T (MODE 1: into the target box's own local) and M2QBXL^T (MODE 0:
+// into the list-3 boxes' multipoles, atomics): a block per target box, a
+// thread per box coefficient, per center the waves of c - c_D in LDS
+template <int MODE>
+__global__ void __launch_bounds__(448) kh_adj_e2qbxl(HD hd, TD t, int ntb, const int *tboxes,
+                                                     const int *starts, const int *lists,
+                                                     const int *gcs, const int *gcl,
+                                                     const double *ctrs, int nctr,
+                                                     const double *qadj, double *xadj)
+{
+    __shared__ double Xs[2 * (MAXP + MAXPQ + 1) * (MAXP + MAXPQ + 1)];
+    __shared__ double ys[2 * (MAXPQ + 1) * (MAXPQ + 1)];
+    __shared__ double rads[2 * (MAXP + MAXPQ + 3)];
+    __shared__ double jt[MAXP + MAXPQ + 4];
+    const int itb = blockIdx.x;
+    if (itb >= ntb) return;
+    const int g0 = gcs[itb], g1 = gcs[itb + 1];
+    if (g0 == g1) return;
+    cplx *X = reinterpret_cast<cplx *>(Xs);
+    cplx *y = reinterpret_cast<cplx *>(ys);
+    const int tid = threadIdx.x;
+    const int pq = hd.pq, nco = hd.ncq;
+    const int ncg = (hd.pg + 1) * (hd.pg + 1);
+    const int eb = MODE == 0 ? starts[itb] : 0;
+    const int ee = MODE == 0 ? starts[itb + 1] : 1;
+    for (int e = eb; e < ee; ++e) {
+        const int D = MODE == 0 ? lists[e] : tboxes[itb];
+        const int lv = t.level[D];
+        const int p = hd.order[lv];
+        const int ncin = (p + 1) * (p + 1);
+        const v3 cd = boxc(t, D);
+        int n = 0, m = 0;
+        if (tid < ncin) nm_of(tid, n, m);
+        cplx acc = {0.0, 0.0};
+        for (int g = g0; g < g1; ++g) {
+            const int ci = gcl[g];
+            // waves_block synchronises before it writes X: the previous
+            // center's X and y are consumed by then
+            waves_block(MODE == 0 ? 1 : 0, p + pq, hd.k, sub(ld3(ctrs, nctr, ci), cd), X,
+                        reinterpret_cast<cplx *>(rads), jt);
+            for (int w = tid; w < nco; w += blockDim.x)
+                y[w] = ldc(qadj + 2 * ((size_t)ci * nco + w));
+            __syncthreads();
+            if (tid < ncin) {
+                const double *gj = hd.gaunt + (long)tid * ncg * (hd.pg + 1);
+                for (int o = 0; o < nco; ++o) {
+                    int nu, mu;
+                    nm_of(o, nu, mu);
+                    const double *gg = gj + (long)o * (hd.pg + 1);
+                    cplx cf = {0.0, 0.0};
+                    const int imax = min(n, nu);
+                    for (int ii = 0; ii <= imax; ++ii) {
+                        const int l = abs(n - nu) + 2 * ii;
+                        if (abs(m - mu) > l) continue;
+                        const int ex = (nu + l - n) / 2;
+                        cf += ((ex & 1) ? -gg[ii] : gg[ii]) * X[IDX(l, m - mu)];
+                    }
+                    acc += conj(cf) * y[o];
+                }
+            }
+        }
+        if (tid < ncin) {
+            const double sc = FOURPI * ipow(MODE == 0 ? hd.s[lv] : 1.0 / hd.s[lv], n);
+            double *o = xadj + ecoef_off(hd, D, lv) + 2L * tid;
+            if (MODE == 0)
+                atomic_addc(o, sc * acc);
+            else
+                stc(o, ldc(o) + sc * acc);
+        }
+    }
+}
+
+// P2QBXL^T: a 64-lane block per target box, lanes over its list-1 sources,
+// each summing over the box's global centers; atomics into q'
+__global__ void __launch_bounds__(64) kh_adj_p2qbxl(HD hd, TD t, int ntb, const int *gcs,
+                                                    const int *gcl, const double *ctrs,
+                                                    int nctr, const int *starts,
+                                                    const int *lists, const double *src,
+                                                    int ns, const double *qadj, double *sadj)
+{
+    __shared__ double rads[64 * 2 * (MAXPQ + 1)];
+    __shared__ double jts[64 * (MAXPQ + 3)];
+    const int itb = blockIdx.x;
+    if (itb >= ntb) return;
+    const int g0 = gcs[itb], g1 = gcs[itb + 1];
+    if (g0 == g1) return;
+    const int lane = threadIdx.x;
+    cplx *rad = reinterpret_cast<cplx *>(rads) + lane * (MAXPQ + 1);
+    double *jt = jts + lane * (MAXPQ + 3);
+    const int pq = hd.pq;
+    for (int e = starts[itb]; e < starts[itb + 1]; ++e) {
+        const int Sb = lists[e];
+        const int s0 = t.sstart[Sb], s1 = s0 + t.scount[Sb];
+        for (int si = s0 + lane; si < s1; si += 64) {
+            const v3 sp = ld3(src, ns, si);
+            cplx acc[1] = {{0.0, 0.0}};
+            for (int g = g0; g < g1; ++g) {
+                const int ci = gcl[g];
+                const v3 d = sub(sp, ld3(ctrs, nctr, ci));
+                const double rr = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+                radial_seq(1, pq, hd.k * rr, rad, jt);
+                for (int n = 0; n <= pq; ++n) rad[n] = conj(rad[n]);
+                sum_waves<1>(hd, qadj + 2 * (size_t)ci * hd.ncq, pq, d, rad, acc);
+            }
+            atomic_addc(sadj + 2 * (size_t)si, cplx{0.0, -hd.k} * acc[0]);
+        }
+    }
+}
+
+// user-order potential adjoints to tree order (centers' entries stay zero)
+__global__ void kh_adj_scatter(int nt, int nc, const int *user_to_tree, const double *in,
+                               double *out)
+{
+    const int u = nc + blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nt) return;
+    const int ti = user_to_tree[u];
+    out[2 * (size_t)ti] = in[2 * (size_t)(u - nc)];
+    out[2 * (size_t)ti + 1] = in[2 * (size_t)(u - nc) + 1];
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -1531,6 +1962,178 @@ int h3_eval_qbx_expansions(qbx_ctx *ctx, void *stream, const qbx_tree *T, const 
 #undef H3_Q2P
     }
     QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
+static int adjoint_hd(qbx_ctx *ctx, hipStream_t st, const qbx_tree *T, const qbx_trav *R,
+                      const qbx_kernel_desc *d, HD *hd)
+{
+    QBX_CHECK(d && d->source_kind == QBX_SOURCE_S,
+              "the 3D Helmholtz adjoint QBX-FMM is for the single layer");
+    QBX_CHECK(d->nrhs <= 1, "the adjoint QBX-FMM takes one density (nrhs %d)", d->nrhs);
+    QBX_TRY(make_hd(d, T, hd, R->nsep));
+    QBX_TRY(ensure_tables(ctx, st, T, hd, true));
+    hd->r0 = 0;
+    hd->rc = 1;
+    return QBX_OK;
+}
+
+static int adjoint_launch(hipStream_t st, const qbx_tree *T, const qbx_trav *R,
+                          const qbx_geo *G, const HD &hd, int stage, const double *in,
+                          double *out)
+{
+    const TD td = make_td(T);
+    const int ntb = R->ntarget_boxes, nc = G->ncenters;
+    const bool qbx = G->nglobal > 0;
+    switch (stage) {
+    case QBX_ADJ_QBXL2P:
+        if (nc > 0)
+            hipLaunchKernelGGL(kh_adj_qbxl2p, dim3(nc), dim3(128), 0, st, hd, nc,
+                               G->ctt_starts.p, G->ctt_lists.p, T->targets.p, T->ntargets,
+                               G->centers.p, in, out);
+        break;
+    case QBX_ADJ_L2QBXL:
+        if (ntb > 0 && qbx)
+            hipLaunchKernelGGL(kh_adj_e2qbxl<1>, dim3(ntb), dim3(448), 0, st, hd, td, ntb,
+                               R->target_boxes.p, nullptr, nullptr, G->gc_by_tbox_starts.p,
+                               G->gc_by_tbox.p, G->centers.p, nc, in, out);
+        break;
+    case QBX_ADJ_M2QBXL:
+        if (ntb > 0 && qbx && R->n3 > 0)
+            hipLaunchKernelGGL(kh_adj_e2qbxl<0>, dim3(ntb), dim3(448), 0, st, hd, td, ntb,
+                               R->target_boxes.p, R->l3_starts.p, R->l3_lists.p,
+                               G->gc_by_tbox_starts.p, G->gc_by_tbox.p, G->centers.p, nc, in,
+                               out);
+        break;
+    case QBX_ADJ_P2QBXL:
+        if (ntb > 0 && qbx)
+            hipLaunchKernelGGL(kh_adj_p2qbxl, dim3(ntb), dim3(64), 0, st, hd, td, ntb,
+                               G->gc_by_tbox_starts.p, G->gc_by_tbox.p, G->centers.p, nc,
+                               R->l1_starts.p, R->l1_lists.p, T->sources.p, T->nsources, in,
+                               out);
+        break;
+    case QBX_ADJ_L2P:
+    case QBX_ADJ_M2P:
+        if (ntb > 0 && G->nfiltered > 0) {
+            auto kern = stage == QBX_ADJ_L2P ? kh_adj_e2p<true> : kh_adj_e2p<false>;
+            hipLaunchKernelGGL(kern, dim3(ntb), dim3(448), 0, st, hd, td, ntb, R->target_boxes.p,
+                               R->l3_starts.p, R->l3_lists.p, G->nqbtl_box_target_starts.p,
+                               G->nqbtl_box_target_counts_nonchild.p, G->nqbtl_unfiltered.p,
+                               T->targets.p, T->ntargets, in, out);
+        }
+        break;
+    case QBX_ADJ_P2P:
+        if (ntb > 0 && G->nfiltered > 0)
+            hipLaunchKernelGGL(kh_adj_p2p, dim3(ntb), dim3(64), 0, st, hd, td, ntb,
+                               R->target_boxes.p, R->l1_starts.p, R->l1_lists.p,
+                               G->nqbtl_box_target_starts.p,
+                               G->nqbtl_box_target_counts_nonchild.p, G->nqbtl_unfiltered.p,
+                               T->targets.p, T->ntargets, T->sources.p, T->nsources, in, out);
+        break;
+    case QBX_ADJ_L2L:
+        // out ignored: in (the locals' adjoint) is updated in place, finest
+        // level first
+        for (int l = T->nlevels - 1; l >= 1; --l) {
+            const int i0 = R->h_level_start_ttp_box_nrs[l], i1 = R->h_level_start_ttp_box_nrs[l + 1];
+            if (i1 > i0)
+                hipLaunchKernelGGL(kh_trans_adj<2>, dim3(i1 - i0), dim3(448), 0, st, hd, td, l, i0,
+                                   i1 - i0, R->ttp_boxes.p, nullptr, nullptr, in,
+                                   const_cast<double *>(in));
+        }
+        break;
+    case QBX_ADJ_M2L:
+        if (R->n2 > 0)
+            for (int l = 0; l < T->nlevels; ++l) {
+                const int i0 = R->h_level_start_ttp_box_nrs[l];
+                const int i1 = R->h_level_start_ttp_box_nrs[l + 1];
+                if (i1 > i0 && hd.m2l_off[l] >= 0)
+                    hipLaunchKernelGGL(kh_trans_adj<1>, dim3(i1 - i0), dim3(448), 0, st, hd, td,
+                                       l, i0, i1 - i0, R->ttp_boxes.p, R->l2_starts.p,
+                                       R->l2_lists.p, in, out);
+            }
+        break;
+    case QBX_ADJ_P2L:
+        if (R->nttp_boxes > 0 && R->n4 > 0)
+            hipLaunchKernelGGL(kh_adj_p2e<true>, dim3(R->nttp_boxes), dim3(64), 0, st, hd, td,
+                               R->nttp_boxes, R->ttp_boxes.p, R->l4_starts.p, R->l4_lists.p,
+                               T->sources.p, T->nsources, in, out);
+        break;
+    case QBX_ADJ_M2M:
+        // out ignored: in (the multipoles' adjoint) is updated in place,
+        // coarsest level first
+        for (int l = 0; l <= T->nlevels - 2; ++l) {
+            const int i0 = R->h_level_start_source_parent_box_nrs[l];
+            const int i1 = R->h_level_start_source_parent_box_nrs[l + 1];
+            if (i1 > i0)
+                hipLaunchKernelGGL(kh_trans_adj<0>, dim3(i1 - i0), dim3(448), 0, st, hd, td, l, i0,
+                                   i1 - i0, R->source_parent_boxes.p, nullptr, nullptr, in,
+                                   const_cast<double *>(in));
+        }
+        break;
+    case QBX_ADJ_P2M:
+        if (R->nsource_boxes > 0)
+            hipLaunchKernelGGL(kh_adj_p2e<false>, dim3(R->nsource_boxes), dim3(64), 0, st, hd,
+                               td, R->nsource_boxes, R->source_boxes.p, nullptr, nullptr,
+                               T->sources.p, T->nsources, in, out);
+        break;
+    default:
+        set_error("qbx_adjoint_stage: unknown stage %d", stage);
+        return QBX_ERR_INVALID;
+    }
+    QBX_HIP(hipGetLastError());
+    return QBX_OK;
+}
+
+int h3_adjoint_stage(qbx_ctx *ctx, hipStream_t st, const qbx_tree *T, const qbx_trav *R,
+                     const qbx_geo *G, const qbx_kernel_desc *d, int stage, const double *in,
+                     double *out)
+{
+    HD hd;
+    QBX_TRY(adjoint_hd(ctx, st, T, R, d, &hd));
+    return adjoint_launch(st, T, R, G, hd, stage, in, out);
+}
+
+int h3_drive_fmm_adjoint(qbx_ctx *ctx, hipStream_t st, const qbx_tree *T, const qbx_trav *R,
+                         const qbx_geo *G, const qbx_kernel_desc *d, const double *pot_adj_user,
+                         double *str_adj)
+{
+    HD hd;
+    QBX_TRY(adjoint_hd(ctx, st, T, R, d, &hd));
+    const long nm = 2L * std::max(hd.lcoef[hd.nlevels], 1L);
+    const long nq = 2L * std::max(G->ncenters, 1) * hd.ncq;
+    const long np = 2L * std::max(T->ntargets, 1);
+    QBX_TRY(ws_reserve(ctx, st, ctx->ws_adj_m, nm, "adjoint multipole workspace"));
+    QBX_TRY(ws_reserve(ctx, st, ctx->ws_adj_l, nm, "adjoint local workspace"));
+    QBX_TRY(ws_reserve(ctx, st, ctx->ws_adj_q, nq, "adjoint QBX workspace"));
+    QBX_TRY(ws_reserve(ctx, st, ctx->ws_adj_p, np, "adjoint potential workspace"));
+    double *Ma = ctx->ws_adj_m.p, *La = ctx->ws_adj_l.p, *Qa = ctx->ws_adj_q.p,
+           *Pa = ctx->ws_adj_p.p;
+    QBX_TRY(zero_fill(Ma, nm, st));
+    QBX_TRY(zero_fill(La, nm, st));
+    QBX_TRY(zero_fill(Pa, np, st));
+    QBX_TRY(zero_fill(str_adj, 2L * T->nsources, st));
+    const int nc = G->ncenters, nt = T->ntargets;
+    if (nt > nc)
+        hipLaunchKernelGGL(kh_adj_scatter, dim3(cdiv(nt - nc, 256)), dim3(256), 0, st, nt, nc,
+                           G->user_to_tree_target.p, pot_adj_user, Pa);
+    QBX_HIP(hipGetLastError());
+    auto run = [&](int stage, const double *in, double *out) {
+        return adjoint_launch(st, T, R, G, hd, stage, in, out);
+    };
+    // the forward's stages in reverse: the potentials' adjoint into the QBX
+    // and box locals, those down the tree, then into the multipoles and up
+    QBX_TRY(run(QBX_ADJ_QBXL2P, Pa, Qa));
+    QBX_TRY(run(QBX_ADJ_L2QBXL, Qa, La));
+    QBX_TRY(run(QBX_ADJ_L2P, Pa, La));
+    QBX_TRY(run(QBX_ADJ_L2L, La, nullptr));
+    QBX_TRY(run(QBX_ADJ_M2L, La, Ma));
+    QBX_TRY(run(QBX_ADJ_M2QBXL, Qa, Ma));
+    QBX_TRY(run(QBX_ADJ_M2P, Pa, Ma));
+    QBX_TRY(run(QBX_ADJ_P2L, La, str_adj));
+    QBX_TRY(run(QBX_ADJ_P2QBXL, Qa, str_adj));
+    QBX_TRY(run(QBX_ADJ_P2P, Pa, str_adj));
+    QBX_TRY(run(QBX_ADJ_M2M, Ma, nullptr));
+    QBX_TRY(run(QBX_ADJ_P2M, Ma, str_adj));
     return QBX_OK;
 }
 

@@ -357,5 +357,13 @@ int h3_translate_box_local_to_qbx_local(qbx_ctx *ctx, void *stream, const qbx_tr
                                         double *qexp, int accumulate);
 int h3_eval_qbx_expansions(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_geo *G,
                            const qbx_kernel_desc *d, const double *qexp, double *pot);
+// the conjugate-transposed stages and driver (single layer, one density);
+// arguments as qbx_adjoint_stage / qbx_drive_fmm_adjoint
+int h3_adjoint_stage(qbx_ctx *ctx, hipStream_t st, const qbx_tree *T, const qbx_trav *R,
+                     const qbx_geo *G, const qbx_kernel_desc *d, int stage, const double *in,
+                     double *out);
+int h3_drive_fmm_adjoint(qbx_ctx *ctx, hipStream_t st, const qbx_tree *T, const qbx_trav *R,
+                         const qbx_geo *G, const qbx_kernel_desc *d, const double *pot_adj_user,
+                         double *str_adj);
 
 }  // namespace qbx

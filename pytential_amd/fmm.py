@@ -305,7 +305,8 @@ def adjoint_stage(wrangler: QBXExpansionWrangler, stage, adj_out, adj_in=None, n
     (added into ``adj_in`` when given) that of its input.  refine_locals and
     coarsen_multipoles update ``adj_out`` in place and return it; the others
     return the input adjoint (eval_qbx_expansions overwrites the QBX
-    expansions' adjoint).  2D, one density."""
+    expansions' adjoint).  The 2D kernels and the 3D Helmholtz single layer,
+    one density."""
     w = wrangler
     sid = ADJOINT_STAGES[stage]
     adj_out = adj_out.to(torch.complex128).contiguous()
@@ -332,7 +333,8 @@ def drive_fmm_adjoint(wrangler: QBXExpansionWrangler, pot_adj, normals):
     """The conjugate transpose of ``drive_fmm_fused``: for the forward map
     strengths (tree order) -> potentials at the user targets, the adjoint of
     the strengths from that of the potentials (``pot_adj`` [n_user_targets]
-    complex; Laplace: only its real part enters).  2D, one density."""
+    complex; Laplace: only its real part enters).  The 2D kernels and the 3D
+    Helmholtz single layer, one density."""
     w = wrangler
     out = torch.empty(w.tree.nsources, dtype=torch.complex128, device="cuda")
     pa = pot_adj.to(torch.complex128).contiguous()

@@ -92,6 +92,29 @@ def build_matrix(source, kernel, k=0.0, source_kind="S", qbx_forced_limit=+1,
                 centers, radii)
 
 
+def associated_qbx_matrix(source, geo, kernel, k=0.0, source_kind="S"):
+    """Dense QBX matrix whose rows are the user targets of ``geo`` (a
+    QBXFMMGeometryData of ``source``) that the target association gave a
+    center, each through the QBX expansion about that center -- the operator
+    the QBX-FMM approximates at those targets (pytential/qbx/fmm.py:549-557
+    evaluates each target's expansion about its associated center).  Returns
+    (row indices into the user targets, matrix)."""
+    d = source.density_discr
+    dev = source.dev
+    nc = geo.ncenters
+    ttc = geo.user_target_to_center()[nc:].long()
+    rows = torch.nonzero(ttc >= 0).reshape(-1)
+    c = ttc[rows]
+    tg = geo.target_info()["targets"][:, nc:][:, rows].contiguous()
+    centers = geo.flat_centers()[:, c].contiguous()
+    radii = geo.flat_expansion_radii()[c].contiguous()
+    kd = _kernel_desc(source.ambient_dim, kernel, k, source_kind, source.qbx_order)
+    normals = dev["quad_normals"] if source_kind == "D" else None
+    return rows, _run(source, kd, d.nelements, d.nodes_per_element, d.fine_nodes_per_element,
+                      dev["upsample"], dev["quad_nodes"], normals, dev["quad_weights"], tg,
+                      centers, radii)
+
+
 def p2p_matrix(source, kernel, k=0.0, source_kind="S", targets="self", weighted=False):
     """Plain kernel matrix between the stage-1 nodes (columns) and ``targets``
     (rows; default the stage-1 nodes, whose coincident pairs are zero), times

@@ -292,7 +292,8 @@ class QBXLayerPotentialSource:
     def exec_compute_potential_adjoint(self, kernel, pot_adj, targets_and_sides, k=0.0,
                                        source_kind="S"):
         """The conjugate transpose of ``exec_compute_potential`` for one
-        density (2D, QBX-FMM): from the adjoints of the potentials (one
+        density (QBX-FMM; the 2D kernels, and in 3D the Helmholtz single
+        layer): from the adjoints of the potentials (one
         tensor per ``(targets, qbx_forced_limit)`` pair, as the forward
         returns them) the adjoint of the stage-1 nodal density, U^T W A^H
         (A the QBX-FMM of qbx_drive_fmm, W the quadrature weights, U the
@@ -301,8 +302,9 @@ class QBXLayerPotentialSource:
         from ._lib import call
         from .geometry import _p, _stream
         from . import fmm as _fmm
-        if self.ambient_dim != 2:
-            raise NotImplementedError("the adjoint QBX-FMM is 2D")
+        if self.ambient_dim != 2 and not (kernel == "helmholtz" and source_kind == "S"):
+            raise NotImplementedError("the adjoint QBX-FMM is for the 2D kernels and the 3D "
+                                      "Helmholtz single layer")
         if self._direct:
             raise NotImplementedError("the adjoint is that of the QBX-FMM (fmm_order given)")
         sides = [(t, 0 if sd is None else sd) for t, sd in targets_and_sides]

@@ -37,6 +37,7 @@ def test_knob_table_matches_header():
     enum = dict((name.lower(), int(v)) for name, v in
                 re.findall(r"QBX_TUNE_([A-Z0-9_]+) = (\d+)", hdr) if name != "COUNT")
     names = {n: pid for n, (pid, _) in autotune.KNOBS.items()}
+    names.update(autotune.SETTINGS)
     assert names == enum
     bounds = dict((m[0].lower(), (int(m[1]), int(m[2]))) for m in re.findall(
         r"QBX_TUNE_([A-Z0-9_]+) = \d+,\s*/\* [^(]*\((\d+)\.\.(\d+)\)", hdr))

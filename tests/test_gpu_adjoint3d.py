@@ -1,0 +1,205 @@
+"""GPU: the adjoint (conjugate transpose) of the 3D Helmholtz QBX-FMM, single
+layer, on a triangulated sphere (k = 1.7).
+
+* every transposed stage against its forward stage through the identity
+  Re <y', A x> = Re <A^H y', x> on random inputs (P2M, M2M, P2P, M2L, M2P,
+  P2L, L2L, L2P, P2QBXL, M2QBXL, L2QBXL, QBXL2P), 1e-12 relative to
+  |y'| |A x|;
+* the whole chain (qbx_drive_fmm_adjoint) against the fused forward the same
+  way;
+* the density adjoint (weights, interpolation) against the conjugate
+  transpose of the dense QBX matrix (qbx_lpot_matrix, on-surface targets,
+  each about its associated center), 1e-7, and no further from it than the
+  forward FMM is from the matrix itself;
+* the autograd function: the gradient of |S sigma - g|^2 against central
+  differences along random directions, 1e-7, and its wavenumber derivative
+  against a fourth-order difference of the loss in k, 1e-7.
+"""
+import functools
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+K = 1.7
+
+
+def _dot(a, b):
+    return complex(torch.sum(torch.conj(a.reshape(-1)) * b.reshape(-1)))
+
+
+def _rnd(n, rng):
+    x = rng.normal(size=n) + 1j * rng.normal(size=n)
+    return torch.as_tensor(x, dtype=torch.complex128, device="cuda")
+
+
+@functools.lru_cache(maxsize=2)
+def _discr(nsub):
+    from pytential_amd.surface import icosphere
+    return icosphere(nsub, order=4, fine_order=5)
+
+
+def _setup(nsub=2, fmm_order=10, mlw=64):
+    """icosphere(nsub) with on-surface targets (side +1, outside) and 400
+    off-surface points inside (radius 0.3..0.85) and outside (1.15..2): the
+    QBX stages act on the surface targets, P2P, M2P and L2P on the rest."""
+    from pytential_amd.qbx import QBXLayerPotentialSource
+    d = _discr(nsub)
+    rng = np.random.default_rng(1006)
+    v = rng.normal(size=(3, 400))
+    v /= np.linalg.norm(v, axis=0)
+    tgt = v * np.where(rng.uniform(size=400) < 0.5, rng.uniform(0.3, 0.85, 400),
+                       rng.uniform(1.15, 2.0, 400))
+    src = QBXLayerPotentialSource(d, fine_order=d.fine_order, qbx_order=3,
+                                  fmm_order=fmm_order, _max_leaf_refine_weight=mlw)
+    return d, src, [("self", +1), (tgt, 0)]
+
+
+def test_transposed_stages_dot_identity():
+    from pytential_amd import fmm as F
+    d, src, pairs = _setup()
+    geo = src.qbx_fmm_geometry_data(tuple(pairs))
+    w = src.make_wrangler(geo, "helmholtz", K, "S", 1)
+    print(f"\n[adjoint3d] icosphere(2): nsources={w.tree.nsources} nboxes={w.tree.nboxes} "
+          f"nlevels={w.tree.nlevels} ncenters={geo.ncenters}", flush=True)
+    assert w.tree.nlevels >= 4
+    rng = np.random.default_rng(21)
+    ns = w.tree.nsources
+    nm = w.multipole_expansion_zeros().numel()
+    nq = w.qbx_local_expansion_zeros().numel()
+    npot = w.full_output_zeros().numel()
+    fw = {
+        "form_multipoles": (lambda x: w.form_multipoles(x, None), ns, nm),
+        "coarsen_multipoles": (lambda x: w.coarsen_multipoles(x.clone()), nm, nm),
+        "eval_direct": (lambda x: w.eval_direct(x, None), ns, npot),
+        "multipole_to_local": (lambda x: w.multipole_to_local(x), nm, nm),
+        "eval_multipoles": (lambda x: w.eval_multipoles(x), nm, npot),
+        "form_locals": (lambda x: w.form_locals(x, None), ns, nm),
+        "refine_locals": (lambda x: w.refine_locals(x.clone()), nm, nm),
+        "eval_locals": (lambda x: w.eval_locals(x), nm, npot),
+        "form_global_qbx_locals": (lambda x: w.form_global_qbx_locals(x, None), ns, nq),
+        "translate_box_multipoles_to_qbx_local":
+            (lambda x: w.translate_box_multipoles_to_qbx_local(x), nm, nq),
+        "translate_box_local_to_qbx_local":
+            (lambda x: w.translate_box_local_to_qbx_local(x), nm, nq),
+        "eval_qbx_expansions": (lambda x: w.eval_qbx_expansions(x), nq, npot),
+    }
+    for stage, (f, nin, nout) in fw.items():
+        x = _rnd(nin, rng)
+        ya = _rnd(nout, rng)
+        y = f(x).reshape(-1)
+        xa = F.adjoint_stage(w, stage, ya.clone()).reshape(-1)
+        lhs = _dot(ya, y).real
+        rhs = _dot(xa, x).real
+        scale = float(torch.linalg.norm(ya) * torch.linalg.norm(y)) + 1e-300
+        err = abs(lhs - rhs) / scale
+        print(f"[adjoint3d] {stage}: Re<y',Ax> {lhs:+.6e} Re<A^H y',x> {rhs:+.6e} "
+              f"rel {err:.1e}", flush=True)
+        assert float(torch.linalg.norm(y)) > 0, stage
+        assert err < 1e-12, (stage, err)
+
+
+def test_adjoint_chain_is_exact_transpose():
+    from pytential_amd import fmm as F
+    d, src, pairs = _setup()
+    geo = src.qbx_fmm_geometry_data(tuple(pairs))
+    w = src.make_wrangler(geo, "helmholtz", K, "S", 1)
+    rng = np.random.default_rng(22)
+    x = _rnd(w.tree.nsources, rng)
+    y = F.drive_fmm_fused(w, x, None)
+    ya = _rnd(y.numel(), rng)
+    xa = F.drive_fmm_adjoint(w, ya, None)
+    lhs = _dot(ya, y).real
+    rhs = _dot(xa, x).real
+    err = abs(lhs - rhs) / float(torch.linalg.norm(ya) * torch.linalg.norm(y))
+    print(f"[adjoint3d] chain: rel {err:.1e}", flush=True)
+    assert err < 1e-12
+
+
+def test_density_adjoint_vs_dense_matrix():
+    """Against the dense QBX matrix with each on-surface target expanded
+    about the center the target association gave it (on a triangulated
+    surface that is not always the node's own center: coincident edge nodes
+    of neighbouring triangles share the nearest one)."""
+    from pytential_amd.matrix import associated_qbx_matrix
+    d, src, _ = _setup(nsub=1, fmm_order=16, mlw=32)
+    pairs = [("self", +1)]
+    geo = src.qbx_fmm_geometry_data(tuple(pairs))
+    rows, A = associated_qbx_matrix(src, geo, "helmholtz", K, "S")
+    assert rows.numel() == d.ndofs
+    A = A.cpu().numpy()
+    rng = np.random.default_rng(23)
+    ga = rng.normal(size=d.ndofs) + 1j * rng.normal(size=d.ndofs)
+    sig = rng.normal(size=d.ndofs) + 1j * rng.normal(size=d.ndofs)
+    got = src.exec_compute_potential_adjoint("helmholtz", [ga], pairs, k=K, source_kind="S")
+    ref = A.conj().T @ ga
+    err = np.max(np.abs(got.cpu().numpy() - ref)) / np.max(np.abs(ref))
+    fwd, = src.exec_compute_potential("helmholtz", sig, pairs, k=K, source_kind="S")
+    ref_f = A @ sig
+    err_f = np.max(np.abs(fwd.cpu().numpy() - ref_f)) / np.max(np.abs(ref_f))
+    print(f"[adjoint3d] density adjoint vs dense A^H: {err:.2e} (forward FMM vs dense A: "
+          f"{err_f:.2e})", flush=True)
+    assert err < 1e-7
+    assert err < 10 * err_f + 1e-12
+
+
+def test_autograd_gradient_vs_central_differences():
+    from pytential_amd.autodiff import layer_potential
+    d, src, pairs = _setup(fmm_order=12)
+    rng = np.random.default_rng(24)
+
+    def rand(n):
+        return torch.as_tensor(rng.normal(size=n) + 1j * rng.normal(size=n),
+                               dtype=torch.complex128, device="cuda")
+    sigma = rand(d.ndofs)
+    out0 = layer_potential(src, "helmholtz", sigma, pairs, k=K)
+    g = rand(out0.shape[0])
+
+    def loss(s):
+        r = layer_potential(src, "helmholtz", s, pairs, k=K) - g
+        return torch.sum(torch.abs(r) ** 2)
+    s = sigma.clone().requires_grad_(True)
+    L = loss(s)
+    L.backward()
+    grad = s.grad.detach()
+    for trial in range(3):
+        v = rand(d.ndofs)
+        h = 1e-5
+        with torch.no_grad():
+            fd = (loss(sigma + h * v) - loss(sigma - h * v)) / (2 * h)
+        an = torch.sum((torch.conj(grad) * v).real)
+        err = abs(float(an) - float(fd)) / abs(float(fd))
+        print(f"[adjoint3d] autograd vs central difference {trial}: {float(an):+.8e} "
+              f"{float(fd):+.8e} rel {err:.1e}", flush=True)
+        assert err < 1e-7
+
+
+def test_autograd_wavenumber_gradient():
+    from pytential_amd.autodiff import layer_potential
+    d, src, pairs = _setup(fmm_order=12)
+    rng = np.random.default_rng(25)
+    sigma = torch.as_tensor(rng.normal(size=d.ndofs) + 1j * rng.normal(size=d.ndofs),
+                            dtype=torch.complex128, device="cuda")
+    out0 = layer_potential(src, "helmholtz", sigma, pairs, k=K)
+    g = out0.detach() * (1.0 + 0.1j) + 0.05
+
+    def loss(kk):
+        r = layer_potential(src, "helmholtz", sigma, pairs, k=kk) - g
+        return torch.sum(torch.abs(r) ** 2)
+    k = torch.tensor(K, dtype=torch.float64, requires_grad=True)
+    L = loss(k)
+    L.backward()
+    h = 3e-4
+
+    def kt(v):
+        return torch.tensor(v, dtype=torch.float64)
+    with torch.no_grad():
+        fd = (-loss(kt(K + 2 * h)) + 8 * loss(kt(K + h)) - 8 * loss(kt(K - h))
+              + loss(kt(K - 2 * h))) / (12 * h)
+    err = abs(float(k.grad) - float(fd)) / abs(float(fd))
+    print(f"[adjoint3d] dL/dk {float(k.grad):+.10e} central difference {float(fd):+.10e} "
+          f"rel {err:.1e}", flush=True)
+    assert err < 1e-7
