@@ -861,6 +861,7 @@ def run_ellipsoid3d(args):
         f"{t_mat * 1e3:.1f} ms; rows vs oracle {e_dense_rows:.2e}, A sigma vs forward "
         f"{e_dense_fwd:.2e}, A^H g vs adjoint {e_dense_adj:.2e}")
     del Ad
+    fmm_grad = ellipsoid_fmm_gradient(src, sig0, k0, tgd, g, loss, gs, gk, args)
     qfmm = ellipsoid_qbx_fmm(d, k0, rng)
     cpu = None
     if not args.no_cpu:
@@ -904,9 +905,57 @@ def run_ellipsoid3d(args):
                      "traffic": None, "flops_per_launch": int(f_dom),
                      "avg_launch_ms": round(t_dom, 4)},
         "cpu_baseline": cpu,
+        "fmm_gradient": fmm_grad,
         "qbx_fmm": qfmm,
     }
     return line
+
+
+def ellipsoid_fmm_gradient(src, sig0, k0, tgd, g, loss_exact, gs_exact, gk_exact, args):
+    """The same loss and gradient through the QBX-FMM (fmm_order 10) instead of
+    the exact sums: forward drive_fmm, backward qbx_drive_fmm_adjoint for
+    dL/dsigma and a fourth-order difference of four more forwards in k for
+    dL/dk (layer_potential); timed per step and compared with the exact
+    path's loss and gradient."""
+    import torch
+
+    from pytential_amd.autodiff import layer_potential
+    pairs = [(tgd, 0)]
+
+    def step():
+        s = torch.tensor(sig0, dtype=torch.complex128, device="cuda", requires_grad=True)
+        k = torch.tensor(k0, dtype=torch.float64, requires_grad=True)
+        loss = torch.sum(torch.abs(layer_potential(src, "helmholtz", s, pairs, k=k) - g) ** 2)
+        loss.backward()
+        return loss, s.grad, k.grad
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nst = max(1, min(args.steps, 3))
+    for _ in range(nst):
+        loss, gsf, gkf = step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / nst * 1e3
+    # the dL/dsigma half alone (forward plus adjoint, no k-derivative)
+    t1 = time.perf_counter()
+    for _ in range(nst):
+        s = torch.tensor(sig0, dtype=torch.complex128, device="cuda", requires_grad=True)
+        L = torch.sum(torch.abs(layer_potential(src, "helmholtz", s, pairs, k=k0) - g) ** 2)
+        L.backward()
+    torch.cuda.synchronize()
+    ms_sig = (time.perf_counter() - t1) / nst * 1e3
+    e_loss = abs(float(loss) - float(loss_exact)) / abs(float(loss_exact))
+    e_gs = float((gsf - gs_exact).abs().max() / gs_exact.abs().max())
+    e_gk = abs(float(gkf) - float(gk_exact)) / abs(float(gk_exact))
+    log(f"[ellipsoid3d fmm-gradient] {ms:.2f} ms per step (sigma-gradient only "
+        f"{ms_sig:.2f} ms); vs exact sums: loss {e_loss:.2e}, dL/dsigma {e_gs:.2e}, "
+        f"dL/dk {e_gk:.2e}")
+    return {"what": "loss and gradient through the QBX-FMM (fmm_order 10): forward "
+                    "drive_fmm, dL/dsigma by qbx_drive_fmm_adjoint, dL/dk by a fourth-order "
+                    "difference of four forwards in k",
+            "ms_per_step": round(ms, 3), "ms_per_step_sigma_gradient_only": round(ms_sig, 3),
+            "steps": nst, "loss_vs_exact_sums": e_loss,
+            "grad_sigma_vs_exact_sums": e_gs, "grad_k_vs_exact_sums": e_gk}
 
 
 def ellipsoid_qbx_fmm(d, k0, rng, steps=5):
@@ -974,9 +1023,25 @@ def ellipsoid_qbx_fmm(d, k0, rng, steps=5):
     full_vs_sample = float((out[torch.as_tensor(idx, device="cuda")] - got).abs().max() /
                            got.abs().max())
     tree = geo.tree()
+    # the transposed QBX-FMM on the same geometry (QBXL2P^T ... P2M^T): its
+    # time and the identity Re <y', A x> = Re <A^H y', x>
+    ya = torch.as_tensor(rng.normal(size=out.numel()) + 1j * rng.normal(size=out.numel()),
+                         device="cuda")
+    xa = F.drive_fmm_adjoint(w, ya, None)
+    torch.cuda.synchronize()
+    ev[0].record()
+    for _ in range(steps):
+        xa = F.drive_fmm_adjoint(w, ya, None)
+    ev[1].record()
+    torch.cuda.synchronize()
+    ms_adj = ev[0].elapsed_time(ev[1]) / steps
+    lhs = complex(torch.sum(torch.conj(ya) * out)).real
+    rhs = complex(torch.sum(torch.conj(xa) * st)).real
+    e_tr = abs(lhs - rhs) / float(torch.linalg.norm(ya) * torch.linalg.norm(out))
     log(f"[ellipsoid3d qbx-fmm] {ms:.2f} ms per evaluation ({d.ndofs} targets, "
         f"{d.nsources} sources, {tree.nboxes} boxes, {tree.nlevels} levels), cold "
-        f"{t_cold:.2f}s, vs direct QBX {err:.2e}; stages {stages}")
+        f"{t_cold:.2f}s, vs direct QBX {err:.2e}; stages {stages}; adjoint {ms_adj:.2f} ms, "
+        f"transpose identity {e_tr:.1e}")
     return {"what": "3D Helmholtz QBX-FMM (spherical waves, qbx_order 3, fmm_order 10) "
                     "of S_k sigma at every surface node (exterior limit), fused driver",
             "ms_per_step": round(ms, 4), "steps": steps,
@@ -986,6 +1051,10 @@ def ellipsoid_qbx_fmm(d, k0, rng, steps=5):
             "rel_err_vs_gpu_direct_qbx_2000_nodes": err,
             "fused_vs_point_target_eval": full_vs_sample,
             "per_stage_ms": stages,
+            "adjoint": {"what": "qbx_drive_fmm_adjoint (the twelve transposed stages) "
+                                "of the same evaluation",
+                        "ms_per_step": round(ms_adj, 4),
+                        "transpose_identity_rel": e_tr},
             "m2l": {"list2_pairs": n2, "gflop": round(m2l_flops / 1e9, 2),
                     "tflops": round(m2l_flops / (stages["m2l"] * 1e-3) / 1e12, 3),
                     "frac_fp64": round(m2l_flops / (stages["m2l"] * 1e-3) / 1e12 /

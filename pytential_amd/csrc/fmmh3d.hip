@@ -503,6 +503,79 @@ __global__ void __launch_bounds__(512) kh_p2e(HD hd, TD t, int nitems, const int
     }
 }
 
+// S waves of one source about a center, in registers: F[IDX(n, m)] =
+// h_n(k r) Y_n^m(d) for n <= PF, d = s - c, r = |d| > 0.  y_n by the forward
+// recurrence, j_n by Miller's backward recurrence from N = max(PF, x) + 10 +
+// 4 sqrt(max(PF, x)) normalised by j_0 or j_1, Y by the column recurrences,
+// all unrolled.
+template <int PF>
+__device__ __forceinline__ void s_waves_reg(double k, v3 d, cplx *F)
+{
+    constexpr int PJ = PF < 1 ? 1 : PF;
+    const double r = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+    const double ir = 1.0 / r;
+    const double x = k * r, ix = 1.0 / x;
+    double sn, cs;
+    sincos(x, &sn, &cs);
+    double yv[PF + 1];
+    yv[0] = -cs * ix;
+    if (PF >= 1) yv[PF >= 1 ? 1 : 0] = (-cs * ix - sn) * ix;
+#pragma unroll
+    for (int n = 2; n <= PF; ++n) yv[n] = (2.0 * n - 1.0) * ix * yv[n - 1] - yv[n - 2];
+    const double mx = fmax((double)PJ, x);
+    const int N = (int)mx + 10 + (int)(4.0 * sqrt(mx));
+    double fp1 = 0.0, f = 1e-30;
+    for (int n = N; n > PJ; --n) {
+        const double fm1 = (2.0 * n + 1.0) * ix * f - fp1;
+        fp1 = f;
+        f = fm1;
+    }
+    double jv[PJ + 1];
+#pragma unroll
+    for (int n = PJ; n >= 0; --n) {
+        jv[n] = f;
+        const double fm1 = (2.0 * n + 1.0) * ix * f - fp1;
+        fp1 = f;
+        f = fm1;
+    }
+    const double j0 = sn * ix, j1 = (sn * ix - cs) * ix;
+    const double scl = fabs(j0) >= fabs(j1) ? j0 / jv[0] : j1 / jv[1];
+    const double z = d.z * ir;
+    const cplx w = {d.x * ir, d.y * ir};
+    cplx wm = {1.0, 0.0};
+    double qmm = 0.28209479177387814347;
+#pragma unroll
+    for (int m = 0; m <= PF; ++m) {
+        if (m > 0) {
+            qmm *= -sqrt((2.0 * m + 1.0) / (2.0 * m));
+            wm = wm * w;
+        }
+        double a2 = 0.0, a1 = qmm;
+#pragma unroll
+        for (int n = m; n <= PF; ++n) {
+            double q;
+            if (n == m) {
+                q = qmm;
+            } else if (n == m + 1) {
+                q = sqrt(2.0 * m + 3.0) * z * qmm;
+            } else {
+                const double a = sqrt((4.0 * n * n - 1.0) / ((double)n * n - (double)m * m));
+                const double b = sqrt(((n - 1.0) * (n - 1.0) - (double)m * m) /
+                                      (4.0 * (n - 1.0) * (n - 1.0) - 1.0));
+                q = a * (z * a1 - b * a2);
+            }
+            if (n > m) {
+                a2 = a1;
+                a1 = q;
+            }
+            const cplx hn = {scl * jv[n], yv[n]};
+            const cplx y = q * wm;
+            F[IDX(n, m)] = hn * y;
+            if (m > 0) F[IDX(n, -m)] = hn * sgn_conj(y, m);
+        }
+    }
+}
+
 // P2QBXL for one density with the QBX expansion in registers (orders <= 5):
 // a wavefront per global QBX center, the center's list-1 sources dealt to
 // the lanes round-robin across the boxes.  Each lane forms a source's S
@@ -520,7 +593,6 @@ __global__ void __launch_bounds__(64) kh_p2qbxl_reg(HD hd, TD t, int ng, const i
                                                     double *qexp)
 {
     constexpr int PF = PQ + DIP;
-    constexpr int PJ = PF < 1 ? 1 : PF;
     constexpr int NC = (PQ + 1) * (PQ + 1);
     constexpr int NF = (PF + 1) * (PF + 1);
     const int g = blockIdx.x;
@@ -540,69 +612,8 @@ __global__ void __launch_bounds__(64) kh_p2qbxl_reg(HD hd, TD t, int ng, const i
         for (int j = (lane - (base & 63) + 64) & 63; j < cnt; j += 64) {
             const int si = s0 + j;
             const v3 d = sub(ld3(src, ns, si), c);
-            const double r = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
-            const double ir = 1.0 / r;
-            const double x = k * r, ix = 1.0 / x;
-            double sn, cs;
-            sincos(x, &sn, &cs);
-            double yv[PF + 1];
-            yv[0] = -cs * ix;
-            if (PF >= 1) yv[PF >= 1 ? 1 : 0] = (-cs * ix - sn) * ix;
-#pragma unroll
-            for (int n = 2; n <= PF; ++n) yv[n] = (2.0 * n - 1.0) * ix * yv[n - 1] - yv[n - 2];
-            const double mx = fmax((double)PJ, x);
-            const int N = (int)mx + 10 + (int)(4.0 * sqrt(mx));
-            double fp1 = 0.0, f = 1e-30;
-            for (int n = N; n > PJ; --n) {
-                const double fm1 = (2.0 * n + 1.0) * ix * f - fp1;
-                fp1 = f;
-                f = fm1;
-            }
-            double jv[PJ + 1];
-#pragma unroll
-            for (int n = PJ; n >= 0; --n) {
-                jv[n] = f;
-                const double fm1 = (2.0 * n + 1.0) * ix * f - fp1;
-                fp1 = f;
-                f = fm1;
-            }
-            const double j0 = sn * ix, j1 = (sn * ix - cs) * ix;
-            const double scl = fabs(j0) >= fabs(j1) ? j0 / jv[0] : j1 / jv[1];
             cplx F[NF];
-            const double z = d.z * ir;
-            const cplx w = {d.x * ir, d.y * ir};
-            cplx wm = {1.0, 0.0};
-            double qmm = 0.28209479177387814347;
-#pragma unroll
-            for (int m = 0; m <= PF; ++m) {
-                if (m > 0) {
-                    qmm *= -sqrt((2.0 * m + 1.0) / (2.0 * m));
-                    wm = wm * w;
-                }
-                double a2 = 0.0, a1 = qmm;
-#pragma unroll
-                for (int n = m; n <= PF; ++n) {
-                    double q;
-                    if (n == m) {
-                        q = qmm;
-                    } else if (n == m + 1) {
-                        q = sqrt(2.0 * m + 3.0) * z * qmm;
-                    } else {
-                        const double a = sqrt((4.0 * n * n - 1.0) / ((double)n * n - (double)m * m));
-                        const double b = sqrt(((n - 1.0) * (n - 1.0) - (double)m * m) /
-                                              (4.0 * (n - 1.0) * (n - 1.0) - 1.0));
-                        q = a * (z * a1 - b * a2);
-                    }
-                    if (n > m) {
-                        a2 = a1;
-                        a1 = q;
-                    }
-                    const cplx hn = {scl * jv[n], yv[n]};
-                    const cplx y = q * wm;
-                    F[IDX(n, m)] = hn * y;
-                    if (m > 0) F[IDX(n, -m)] = hn * sgn_conj(y, m);
-                }
-            }
+            s_waves_reg<PF>(k, d, F);
             const cplx qs = ldc(str + 2 * ((size_t)si * hd.nr + hd.r0));
             v3 nu = {0.0, 0.0, 0.0};
             if (DIP) nu = ld3(nrm, ns, si);
@@ -945,6 +956,10 @@ __global__ void kh_qbxl2p(HD hd, int nctt, const int *ctt_lists, const int *tree
 // Scatters (list 2, 3, 4 from the target side, L2L^T into parents, P2P^T)
 // add with atomics; the rest have one writer.
 
+// shares (blockIdx.y) of the list-1 sources / list-3 boxes per center group
+// in P2QBXL^T and M2QBXL^T
+constexpr int ADJ_SHARES = 4;
+
 __device__ __forceinline__ void atomic_addc(double *p, cplx v)
 {
     atomicAdd(p, v.x);
@@ -1082,10 +1097,11 @@ __global__ void __launch_bounds__(448) kh_adj_e2p(HD hd, TD t, int ntb, const in
 // M2M^T (OP 0: parents of level lv to their children), M2L^T (OP 1:
 // target-or-target-parent boxes of level lv to their list-2 boxes), L2L^T
 // (OP 2: boxes of level lv >= 1 to their parents): in'_j += sum_i
-// conj(T[j][i]) out'_i, a thread per input coefficient j, the output box's
-// adjoint staged in LDS
+// conj(T[j][i]) out'_i.  The output box's adjoint staged in LDS; a
+// wavefront per table row j at a time, its lanes along the row (coalesced)
+// and a shuffle reduction.
 template <int OP>
-__global__ void __launch_bounds__(448) kh_trans_adj(HD hd, TD t, int lv, int i0, int nbox,
+__global__ void __launch_bounds__(256) kh_trans_adj(HD hd, TD t, int lv, int i0, int nbox,
                                                     const int *boxes, const int *starts,
                                                     const int *lists, const double *outa,
                                                     double *ina)
@@ -1099,7 +1115,7 @@ __global__ void __launch_bounds__(448) kh_trans_adj(HD hd, TD t, int lv, int i0,
     const int lvin = OP == 0 ? lv + 1 : OP == 2 ? lv - 1 : lv;
     const int pin = hd.order[lvin], pout = hd.order[lv];
     const int ncin = (pin + 1) * (pin + 1), ncout = (pout + 1) * (pout + 1);
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
     const int S1 = 2 * hd.W + 1;
     const int nb = t.nboxes;
     const int bx = t.coords[B], by = t.coords[nb + B], bz = t.coords[2 * nb + B];
@@ -1132,14 +1148,23 @@ __global__ void __launch_bounds__(448) kh_trans_adj(HD hd, TD t, int lv, int i0,
         }
         const long toff = OP == 0 ? hd.m2m_off[lv] : OP == 1 ? hd.m2l_off[lv] : hd.l2l_off[lv];
         const double *T = hd.tab + 2 * (toff + (long)cls * ncin * ncout);
-        for (int j = tid; j < ncin; j += blockDim.x) {
+        double *obase = ina + ecoef_off(hd, A, lvin);
+        for (int j = wv; j < ncin; j += nw) {
+            const double *row = T + 2L * j * ncout;
             cplx acc = {0.0, 0.0};
-            for (int q = 0; q < ncout; ++q) acc += conj(ldc(T + 2 * ((long)j * ncout + q))) * y[q];
-            double *o = ina + ecoef_off(hd, A, lvin) + 2L * j * hd.nr;
-            if (OP == 0)
-                stc(o, ldc(o) + acc);
-            else
-                atomic_addc(o, acc);
+            for (int q = lane; q < ncout; q += 64) acc += conj(ldc(row + 2 * q)) * y[q];
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                acc.x += __shfl_xor(acc.x, off);
+                acc.y += __shfl_xor(acc.y, off);
+            }
+            if (lane == 0) {
+                double *o = obase + 2L * j * hd.nr;
+                if (OP == 0)
+                    stc(o, ldc(o) + acc);
+                else
+                    atomic_addc(o, acc);
+            }
         }
     }
 }
@@ -1246,31 +1271,35 @@ __global__ void __launch_bounds__(128) kh_adj_qbxl2p(HD hd, int nc, const int *c
 }
 
 // L2QBXL^T (MODE 1: into the target box's own local) and M2QBXL^T (MODE 0:
-// into the list-3 boxes' multipoles, atomics): a block per target box, a
-// thread per box coefficient, per center the waves of c - c_D in LDS
+// into the list-3 boxes' multipoles): a block per (group of up to GC_GROUP
+// centers of one target box, share blockIdx.y of the list-3 entries), a
+// thread per box coefficient summing over the group's centers, per center
+// the waves of c - c_D in LDS; atomics into the box expansions
 template <int MODE>
-__global__ void __launch_bounds__(448) kh_adj_e2qbxl(HD hd, TD t, int ntb, const int *tboxes,
-                                                     const int *starts, const int *lists,
-                                                     const int *gcs, const int *gcl,
-                                                     const double *ctrs, int nctr,
-                                                     const double *qadj, double *xadj)
+__global__ void __launch_bounds__(448) kh_adj_e2qbxl(HD hd, TD t, int nunits,
+                                                     const int *unit_box, const int *unit_c0,
+                                                     const int *tboxes, const int *starts,
+                                                     const int *lists, const int *gcs,
+                                                     const int *gcl, const double *ctrs,
+                                                     int nctr, const double *qadj, double *xadj)
 {
     __shared__ double Xs[2 * (MAXP + MAXPQ + 1) * (MAXP + MAXPQ + 1)];
     __shared__ double ys[2 * (MAXPQ + 1) * (MAXPQ + 1)];
     __shared__ double rads[2 * (MAXP + MAXPQ + 3)];
     __shared__ double jt[MAXP + MAXPQ + 4];
-    const int itb = blockIdx.x;
-    if (itb >= ntb) return;
-    const int g0 = gcs[itb], g1 = gcs[itb + 1];
-    if (g0 == g1) return;
+    const int u = blockIdx.x;
+    if (u >= nunits) return;
+    const int itb = unit_box[u];
+    const int g0 = gcs[itb] + unit_c0[u], g1 = min(g0 + GC_GROUP, gcs[itb + 1]);
     cplx *X = reinterpret_cast<cplx *>(Xs);
     cplx *y = reinterpret_cast<cplx *>(ys);
     const int tid = threadIdx.x;
     const int pq = hd.pq, nco = hd.ncq;
     const int ncg = (hd.pg + 1) * (hd.pg + 1);
-    const int eb = MODE == 0 ? starts[itb] : 0;
-    const int ee = MODE == 0 ? starts[itb + 1] : 1;
-    for (int e = eb; e < ee; ++e) {
+    const int eb = MODE == 0 ? starts[itb] + (int)blockIdx.y : 0;
+    const int ee = MODE == 0 ? starts[itb + 1] : (blockIdx.y == 0 ? 1 : 0);
+    const int es = MODE == 0 ? (int)gridDim.y : 1;
+    for (int e = eb; e < ee; e += es) {
         const int D = MODE == 0 ? lists[e] : tboxes[itb];
         const int lv = t.level[D];
         const int p = hd.order[lv];
@@ -1308,49 +1337,89 @@ __global__ void __launch_bounds__(448) kh_adj_e2qbxl(HD hd, TD t, int ntb, const
         }
         if (tid < ncin) {
             const double sc = FOURPI * ipow(MODE == 0 ? hd.s[lv] : 1.0 / hd.s[lv], n);
-            double *o = xadj + ecoef_off(hd, D, lv) + 2L * tid;
-            if (MODE == 0)
-                atomic_addc(o, sc * acc);
-            else
-                stc(o, ldc(o) + sc * acc);
+            atomic_addc(xadj + ecoef_off(hd, D, lv) + 2L * tid, sc * acc);
         }
     }
 }
 
-// P2QBXL^T: a 64-lane block per target box, lanes over its list-1 sources,
-// each summing over the box's global centers; atomics into q'
-__global__ void __launch_bounds__(64) kh_adj_p2qbxl(HD hd, TD t, int ntb, const int *gcs,
+// P2QBXL^T: a 64-lane block per (group of up to GC_GROUP centers of one
+// target box, share blockIdx.y of its list-1 sources), lanes over the
+// sources, each summing over the group's centers; one atomic per source.
+// PQ >= 0: the S waves in registers (s_waves_reg); PQ = -1: any order, the
+// radial sequence in LDS.
+template <int PQ>
+__global__ void __launch_bounds__(64) kh_adj_p2qbxl(HD hd, TD t, int nunits, const int *unit_box,
+                                                    const int *unit_c0, const int *gcs,
                                                     const int *gcl, const double *ctrs,
                                                     int nctr, const int *starts,
                                                     const int *lists, const double *src,
                                                     int ns, const double *qadj, double *sadj)
 {
-    __shared__ double rads[64 * 2 * (MAXPQ + 1)];
-    __shared__ double jts[64 * (MAXPQ + 3)];
-    const int itb = blockIdx.x;
-    if (itb >= ntb) return;
-    const int g0 = gcs[itb], g1 = gcs[itb + 1];
-    if (g0 == g1) return;
+    constexpr int NCR = PQ >= 0 ? (PQ + 1) * (PQ + 1) : 1;
+    __shared__ double cq[2 * GC_GROUP * (MAXPQ + 1) * (MAXPQ + 1)];
+    __shared__ double cc[3 * GC_GROUP];
+    __shared__ double rads[PQ >= 0 ? 1 : 64 * 2 * (MAXPQ + 1)];
+    __shared__ double jts[PQ >= 0 ? 1 : 64 * (MAXPQ + 3)];
+    const int u = blockIdx.x;
+    if (u >= nunits) return;
+    const int itb = unit_box[u];
+    const int g0 = gcs[itb] + unit_c0[u], ng = min(GC_GROUP, gcs[itb + 1] - g0);
     const int lane = threadIdx.x;
-    cplx *rad = reinterpret_cast<cplx *>(rads) + lane * (MAXPQ + 1);
-    double *jt = jts + lane * (MAXPQ + 3);
-    const int pq = hd.pq;
+    const int pq = PQ >= 0 ? PQ : hd.pq, nco = hd.ncq;
+    cplx *Lq = reinterpret_cast<cplx *>(cq);
+    for (int w = lane; w < ng * nco; w += 64) {
+        const int ci = gcl[g0 + w / nco];
+        Lq[w] = ldc(qadj + 2 * ((size_t)ci * nco + w % nco));
+    }
+    if (lane < 3 * ng) {
+        const int ci = gcl[g0 + lane / 3];
+        cc[lane] = ctrs[(size_t)(lane % 3) * nctr + ci];
+    }
+    __syncthreads();
+    int base = 0;
+    const int ns_ = gridDim.y, sh = blockIdx.y;
     for (int e = starts[itb]; e < starts[itb + 1]; ++e) {
         const int Sb = lists[e];
-        const int s0 = t.sstart[Sb], s1 = s0 + t.scount[Sb];
-        for (int si = s0 + lane; si < s1; si += 64) {
+        const int s0 = t.sstart[Sb], cnt = t.scount[Sb];
+        // the box's sources, numbered across its list-1 boxes, dealt to the
+        // shares round-robin in runs of 64
+        for (int j0 = 0; j0 < cnt; j0 += 64) {
+            const int run = (base + j0) >> 6;
+            if (run % ns_ != sh) continue;
+            const int j = j0 + lane;
+            if (j >= cnt) continue;
+            const int si = s0 + j;
             const v3 sp = ld3(src, ns, si);
-            cplx acc[1] = {{0.0, 0.0}};
-            for (int g = g0; g < g1; ++g) {
-                const int ci = gcl[g];
-                const v3 d = sub(sp, ld3(ctrs, nctr, ci));
-                const double rr = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
-                radial_seq(1, pq, hd.k * rr, rad, jt);
-                for (int n = 0; n <= pq; ++n) rad[n] = conj(rad[n]);
-                sum_waves<1>(hd, qadj + 2 * (size_t)ci * hd.ncq, pq, d, rad, acc);
+            cplx acc = {0.0, 0.0};
+            for (int q = 0; q < ng; ++q) {
+                const v3 d = sub(sp, v3{cc[3 * q], cc[3 * q + 1], cc[3 * q + 2]});
+                const cplx *L = Lq + q * nco;
+                if constexpr (PQ >= 0) {
+                    cplx F[NCR];
+                    s_waves_reg<PQ>(hd.k, d, F);
+                    // sum conj(h_n) Y_n^m L = sum conj((-1)^m F[n, -m]) L
+#pragma unroll
+                    for (int nn = 0; nn <= PQ; ++nn)
+#pragma unroll
+                        for (int mm = -nn; mm <= nn; ++mm) {
+                            cplx f = conj(F[IDX(nn, -mm)]);
+                            if (mm & 1) f = -f;
+                            acc += f * L[IDX(nn, mm)];
+                        }
+                } else {
+                    cplx *rad = reinterpret_cast<cplx *>(rads) + lane * (MAXPQ + 1);
+                    double *jt = jts + lane * (MAXPQ + 3);
+                    const double rr = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+                    radial_seq(1, pq, hd.k * rr, rad, jt);
+                    for (int nn = 0; nn <= pq; ++nn) rad[nn] = conj(rad[nn]);
+                    cplx a1[1] = {{0.0, 0.0}};
+                    sum_waves<1>(hd, reinterpret_cast<const double *>(L), pq, d, rad, a1);
+                    acc += a1[0];
+                }
             }
-            atomic_addc(sadj + 2 * (size_t)si, cplx{0.0, -hd.k} * acc[0]);
+            atomic_addc(sadj + 2 * (size_t)si, cplx{0.0, -hd.k} * acc);
         }
+        base += cnt;
     }
 }
 
@@ -1984,7 +2053,10 @@ static int adjoint_launch(hipStream_t st, const qbx_tree *T, const qbx_trav *R,
 {
     const TD td = make_td(T);
     const int ntb = R->ntarget_boxes, nc = G->ncenters;
-    const bool qbx = G->nglobal > 0;
+    // groups of up to GC_GROUP global centers of one target box
+    const unit_plan &U = G->gc_group_plan;
+    int ncl_max = 1;
+    for (int l = 0; l < hd.nlevels; ++l) ncl_max = std::max(ncl_max, hd.ncl[l]);
     switch (stage) {
     case QBX_ADJ_QBXL2P:
         if (nc > 0)
@@ -1993,24 +2065,35 @@ static int adjoint_launch(hipStream_t st, const qbx_tree *T, const qbx_trav *R,
                                G->centers.p, in, out);
         break;
     case QBX_ADJ_L2QBXL:
-        if (ntb > 0 && qbx)
-            hipLaunchKernelGGL(kh_adj_e2qbxl<1>, dim3(ntb), dim3(448), 0, st, hd, td, ntb,
-                               R->target_boxes.p, nullptr, nullptr, G->gc_by_tbox_starts.p,
-                               G->gc_by_tbox.p, G->centers.p, nc, in, out);
-        break;
     case QBX_ADJ_M2QBXL:
-        if (ntb > 0 && qbx && R->n3 > 0)
-            hipLaunchKernelGGL(kh_adj_e2qbxl<0>, dim3(ntb), dim3(448), 0, st, hd, td, ntb,
-                               R->target_boxes.p, R->l3_starts.p, R->l3_lists.p,
-                               G->gc_by_tbox_starts.p, G->gc_by_tbox.p, G->centers.p, nc, in,
-                               out);
+        if (U.nunits > 0 && (stage == QBX_ADJ_L2QBXL || R->n3 > 0)) {
+            const int th = std::min(448, 64 * ((ncl_max + 63) / 64));
+            auto kern = stage == QBX_ADJ_L2QBXL ? kh_adj_e2qbxl<1> : kh_adj_e2qbxl<0>;
+            const int ny = stage == QBX_ADJ_L2QBXL ? 1 : ADJ_SHARES;
+            hipLaunchKernelGGL(kern, dim3(U.nunits, ny), dim3(th), 0, st, hd, td, U.nunits,
+                               U.unit_box.p, U.unit_t0.p, R->target_boxes.p, R->l3_starts.p,
+                               R->l3_lists.p, G->gc_by_tbox_starts.p, G->gc_by_tbox.p,
+                               G->centers.p, nc, in, out);
+        }
         break;
     case QBX_ADJ_P2QBXL:
-        if (ntb > 0 && qbx)
-            hipLaunchKernelGGL(kh_adj_p2qbxl, dim3(ntb), dim3(64), 0, st, hd, td, ntb,
-                               G->gc_by_tbox_starts.p, G->gc_by_tbox.p, G->centers.p, nc,
-                               R->l1_starts.p, R->l1_lists.p, T->sources.p, T->nsources, in,
-                               out);
+        if (U.nunits > 0) {
+#define H3_AP2Q(PQV)                                                                       \
+    hipLaunchKernelGGL(kh_adj_p2qbxl<PQV>, dim3(U.nunits, ADJ_SHARES), dim3(64), 0, st, hd, \
+                       td, U.nunits, U.unit_box.p, U.unit_t0.p, G->gc_by_tbox_starts.p,    \
+                       G->gc_by_tbox.p, G->centers.p, nc, R->l1_starts.p, R->l1_lists.p,   \
+                       T->sources.p, T->nsources, in, out)
+            switch (hd.pq) {
+            case 0: H3_AP2Q(0); break;
+            case 1: H3_AP2Q(1); break;
+            case 2: H3_AP2Q(2); break;
+            case 3: H3_AP2Q(3); break;
+            case 4: H3_AP2Q(4); break;
+            case 5: H3_AP2Q(5); break;
+            default: H3_AP2Q(-1); break;
+            }
+#undef H3_AP2Q
+        }
         break;
     case QBX_ADJ_L2P:
     case QBX_ADJ_M2P:
@@ -2036,7 +2119,7 @@ static int adjoint_launch(hipStream_t st, const qbx_tree *T, const qbx_trav *R,
         for (int l = T->nlevels - 1; l >= 1; --l) {
             const int i0 = R->h_level_start_ttp_box_nrs[l], i1 = R->h_level_start_ttp_box_nrs[l + 1];
             if (i1 > i0)
-                hipLaunchKernelGGL(kh_trans_adj<2>, dim3(i1 - i0), dim3(448), 0, st, hd, td, l, i0,
+                hipLaunchKernelGGL(kh_trans_adj<2>, dim3(i1 - i0), dim3(256), 0, st, hd, td, l, i0,
                                    i1 - i0, R->ttp_boxes.p, nullptr, nullptr, in,
                                    const_cast<double *>(in));
         }
@@ -2047,7 +2130,7 @@ static int adjoint_launch(hipStream_t st, const qbx_tree *T, const qbx_trav *R,
                 const int i0 = R->h_level_start_ttp_box_nrs[l];
                 const int i1 = R->h_level_start_ttp_box_nrs[l + 1];
                 if (i1 > i0 && hd.m2l_off[l] >= 0)
-                    hipLaunchKernelGGL(kh_trans_adj<1>, dim3(i1 - i0), dim3(448), 0, st, hd, td,
+                    hipLaunchKernelGGL(kh_trans_adj<1>, dim3(i1 - i0), dim3(256), 0, st, hd, td,
                                        l, i0, i1 - i0, R->ttp_boxes.p, R->l2_starts.p,
                                        R->l2_lists.p, in, out);
             }
@@ -2065,7 +2148,7 @@ static int adjoint_launch(hipStream_t st, const qbx_tree *T, const qbx_trav *R,
             const int i0 = R->h_level_start_source_parent_box_nrs[l];
             const int i1 = R->h_level_start_source_parent_box_nrs[l + 1];
             if (i1 > i0)
-                hipLaunchKernelGGL(kh_trans_adj<0>, dim3(i1 - i0), dim3(448), 0, st, hd, td, l, i0,
+                hipLaunchKernelGGL(kh_trans_adj<0>, dim3(i1 - i0), dim3(256), 0, st, hd, td, l, i0,
                                    i1 - i0, R->source_parent_boxes.p, nullptr, nullptr, in,
                                    const_cast<double *>(in));
         }

@@ -42,7 +42,7 @@ def _discr(nsub):
     return icosphere(nsub, order=4, fine_order=5)
 
 
-def _setup(nsub=2, fmm_order=10, mlw=64):
+def _setup(nsub=2, fmm_order=10, mlw=64, qbx_order=3):
     """icosphere(nsub) with on-surface targets (side +1, outside) and 400
     off-surface points inside (radius 0.3..0.85) and outside (1.15..2): the
     QBX stages act on the surface targets, P2P, M2P and L2P on the rest."""
@@ -53,17 +53,20 @@ def _setup(nsub=2, fmm_order=10, mlw=64):
     v /= np.linalg.norm(v, axis=0)
     tgt = v * np.where(rng.uniform(size=400) < 0.5, rng.uniform(0.3, 0.85, 400),
                        rng.uniform(1.15, 2.0, 400))
-    src = QBXLayerPotentialSource(d, fine_order=d.fine_order, qbx_order=3,
+    src = QBXLayerPotentialSource(d, fine_order=d.fine_order, qbx_order=qbx_order,
                                   fmm_order=fmm_order, _max_leaf_refine_weight=mlw)
     return d, src, [("self", +1), (tgt, 0)]
 
 
-def test_transposed_stages_dot_identity():
+@pytest.mark.parametrize("qbx_order", [3, 6])
+def test_transposed_stages_dot_identity(qbx_order):
+    """qbx_order 3: the register forms of P2QBXL and its transpose; 6: the
+    general-order forms."""
     from pytential_amd import fmm as F
-    d, src, pairs = _setup()
+    d, src, pairs = _setup(qbx_order=qbx_order)
     geo = src.qbx_fmm_geometry_data(tuple(pairs))
     w = src.make_wrangler(geo, "helmholtz", K, "S", 1)
-    print(f"\n[adjoint3d] icosphere(2): nsources={w.tree.nsources} nboxes={w.tree.nboxes} "
+    print(f"\n[adjoint3d] qbx_order {qbx_order} icosphere(2): nsources={w.tree.nsources} nboxes={w.tree.nboxes} "
           f"nlevels={w.tree.nlevels} ncenters={geo.ncenters}", flush=True)
     assert w.tree.nlevels >= 4
     rng = np.random.default_rng(21)
