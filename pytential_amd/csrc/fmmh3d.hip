@@ -98,6 +98,7 @@ __device__ __forceinline__ v3 ld3(const double *p, long n, long i)
     return {p[i], p[n + i], p[2 * n + i]};
 }
 __device__ __forceinline__ v3 sub(v3 a, v3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ v3 sub3(v3 a, v3 b) { return sub(a, b); }
 __device__ __forceinline__ v3 boxc(const TD &t, int b) { return ld3(t.centers, t.nboxes, b); }
 
 __device__ __forceinline__ long ecoef_off(const HD &hd, int b, int lv)
@@ -833,6 +834,89 @@ __global__ void __launch_bounds__(64) kh_eval(HD hd, TD t, int ntb, const int *t
     }
 }
 
+// P2P (MODE 0, list 1) and M2P (MODE 1, list 3) over balanced work units
+// (unit_plan: a 64-lane wavefront per unit, the unit's targets take G lanes
+// apiece, each lane walks every G-th source / list-3 entry) with the G
+// partial sums reduced by shuffles in a fixed order.  Same sums as kh_eval,
+// for boxes whose lists are long next to their few targets (far targets in
+// coarse leaves).
+template <int MODE, int NRC>
+__global__ void __launch_bounds__(64) kh_eval_u(HD hd, TD t, const int *ubox, const int *ut0,
+                                                const int *box_g, const int *tboxes,
+                                                const int *starts, const int *lists,
+                                                const int *fstart, const int *fcount,
+                                                const int *unfilt, const double *tgt, int nt,
+                                                const double *src, int ns, const double *str,
+                                                const double *nrm, const double *expn,
+                                                double *pot)
+{
+    __shared__ double rads[MODE == 1 ? 64 * 2 * (MAXP + 1) : 1];
+    __shared__ double jts[MODE == 1 ? 64 * (MAXP + 3) : 1];
+    const int u = blockIdx.x;
+    const int it = ubox[u];
+    const int B = tboxes[it];
+    const int G = box_g[it];
+    const int lane = threadIdx.x;
+    const int tsub = lane / G, sub = lane & (G - 1);
+    const int tl = ut0[u] + tsub;
+    const bool active = tl < fcount[B];
+    const double k = hd.k;
+    cplx acc[NRC];
+#pragma unroll
+    for (int r = 0; r < NRC; ++r) acc[r] = {0.0, 0.0};
+    int ti = 0;
+    if (active) {
+        ti = unfilt[fstart[B] + tl];
+        const v3 x = ld3(tgt, nt, ti);
+        if (MODE == 0) {
+            for (int e = starts[it]; e < starts[it + 1]; ++e) {
+                const int Sb = lists[e];
+                const int s0 = t.sstart[Sb], s1 = s0 + t.scount[Sb];
+                for (int s = s0 + sub; s < s1; s += G) {
+                    const v3 nu = hd.dip ? ld3(nrm, ns, s) : v3{0, 0, 0};
+                    const cplx g = p2p_val(k, hd.dip != 0, ld3(src, ns, s), nu, x);
+#pragma unroll
+                    for (int r = 0; r < NRC; ++r)
+                        if (r < hd.rc)
+                            acc[r] += g * ldc(str + 2 * ((size_t)s * hd.nr + hd.r0 + r));
+                }
+            }
+        } else {
+            cplx *rad = reinterpret_cast<cplx *>(rads) + lane * (MAXP + 1);
+            double *jt = jts + lane * (MAXP + 3);
+            for (int e = starts[it] + sub; e < starts[it + 1]; e += G) {
+                const int D = lists[e];
+                const int lv = t.level[D];
+                const int p = hd.order[lv];
+                const v3 d = sub3(x, boxc(t, D));
+                const double rr = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+                radial_seq(1, p, k * rr, rad, jt);
+                double sp = 1.0;
+                for (int n = 0; n <= p; ++n) {
+                    rad[n] = sp * rad[n];
+                    sp *= hd.s[lv];
+                }
+                sum_waves<NRC>(hd, expn + ecoef_off(hd, D, lv), p, d, rad, acc);
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NRC; ++r)
+        for (int off = G >> 1; off >= 1; off >>= 1) {
+            acc[r].x += __shfl_xor(acc[r].x, off);
+            acc[r].y += __shfl_xor(acc[r].y, off);
+        }
+    if (active && sub == 0) {
+#pragma unroll
+        for (int r = 0; r < NRC; ++r)
+            if (r < hd.rc) {
+                double *dst = pot + 2 * ((size_t)ti * hd.nr + hd.r0 + r);
+                dst[0] += acc[r].x;
+                dst[1] += acc[r].y;
+            }
+    }
+}
+
 // ---- M2QBXL, L2QBXL --------------------------------------------------------
 // MODE 0: M2QBXL over list 3 of the center's target box (C^S);
 // MODE 1: L2QBXL from the target box's own local (C^R).
@@ -1012,10 +1096,10 @@ __global__ void __launch_bounds__(64) kh_adj_p2e(HD hd, TD t, int nitems, const 
     }
 }
 
-// L2P^T (LOCAL: the box's own local) and M2P^T (list-3 boxes): a block per
-// target box, a thread per coefficient of the output expansion; per entry
-// the waves about its center at the box's non-QBX targets staged in LDS in
-// chunks of 8 targets
+// L2P^T (LOCAL: the box's own local; M2P^T uses kh_adj_m2p, a block per
+// list-3 entry): a block per target box, a thread per coefficient of the
+// output expansion; the waves about the center at the box's non-QBX targets
+// staged in LDS in chunks of 8 targets
 template <bool LOCAL>
 __global__ void __launch_bounds__(448) kh_adj_e2p(HD hd, TD t, int ntb, const int *tboxes,
                                                   const int *starts, const int *lists,
@@ -1169,34 +1253,137 @@ __global__ void __launch_bounds__(256) kh_trans_adj(HD hd, TD t, int lv, int i0,
     }
 }
 
-// P2P^T: a 64-lane block per target box, lanes over its list-1 sources,
-// each summing over the box's non-QBX targets; atomics into q'
-__global__ void __launch_bounds__(64) kh_adj_p2p(HD hd, TD t, int ntb, const int *tboxes,
+// the target box index of global list entry e (starts: CSR over target
+// boxes, ntb + 1 entries)
+__device__ __forceinline__ int entry_box(const int *starts, int ntb, int e)
+{
+    int lo = 0, hi = ntb;  // starts[lo] <= e < starts[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (starts[mid] <= e)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// P2P^T: a 64-lane block per list-1 entry (target box B, source box Sb),
+// lanes over Sb's sources, each summing over B's non-QBX targets; atomics
+// into q' (a source is in the list 1 of several target boxes)
+__global__ void __launch_bounds__(64) kh_adj_p2p(HD hd, TD t, int ntb, int n1, const int *tboxes,
                                                  const int *starts, const int *lists,
                                                  const int *fstart, const int *fcount,
                                                  const int *unfilt, const double *tgt, int nt,
                                                  const double *src, int ns, const double *pot,
                                                  double *sadj)
 {
-    const int itb = blockIdx.x;
-    if (itb >= ntb) return;
+    __shared__ double tx[3 * 64], tq[2 * 64];
+    const int e = blockIdx.x;
+    if (e >= n1) return;
+    const int itb = entry_box(starts, ntb, e);
     const int B = tboxes[itb];
     const int f0 = fstart[B], nf = fcount[B];
     if (nf == 0) return;
-    for (int e = starts[itb]; e < starts[itb + 1]; ++e) {
-        const int Sb = lists[e];
-        const int s0 = t.sstart[Sb], s1 = s0 + t.scount[Sb];
-        for (int si = s0 + threadIdx.x; si < s1; si += 64) {
-            const v3 sp = ld3(src, ns, si);
-            cplx acc = {0.0, 0.0};
-            for (int f = f0; f < f0 + nf; ++f) {
-                const int ti = unfilt[f];
-                const cplx g = p2p_val(hd.k, false, sp, v3{0, 0, 0}, ld3(tgt, nt, ti));
-                acc += conj(g) * ldc(pot + 2 * (size_t)ti);
+    const int Sb = lists[e];
+    const int s0 = t.sstart[Sb], s1 = s0 + t.scount[Sb];
+    const int lane = threadIdx.x;
+    for (int sb = s0; sb < s1; sb += 64) {
+        const int si = sb + lane;
+        const v3 sp = si < s1 ? ld3(src, ns, si) : v3{0, 0, 0};
+        cplx acc = {0.0, 0.0};
+        for (int c0 = 0; c0 < nf; c0 += 64) {
+            const int cnt = min(64, nf - c0);
+            __syncthreads();  // the previous chunk consumed
+            if (lane < cnt) {
+                const int ti = unfilt[f0 + c0 + lane];
+                tx[3 * lane] = tgt[ti];
+                tx[3 * lane + 1] = tgt[(size_t)nt + ti];
+                tx[3 * lane + 2] = tgt[2 * (size_t)nt + ti];
+                tq[2 * lane] = pot[2 * (size_t)ti];
+                tq[2 * lane + 1] = pot[2 * (size_t)ti + 1];
             }
-            atomic_addc(sadj + 2 * (size_t)si, acc);
+            __syncthreads();
+            if (si < s1)
+                for (int q = 0; q < cnt; ++q) {
+                    const cplx g = p2p_val(hd.k, false, sp, v3{0, 0, 0},
+                                           v3{tx[3 * q], tx[3 * q + 1], tx[3 * q + 2]});
+                    acc += conj(g) * cplx{tq[2 * q], tq[2 * q + 1]};
+                }
         }
+        if (si < s1) atomic_addc(sadj + 2 * (size_t)si, acc);
     }
+}
+
+// M2P^T: a block per list-3 entry (target box B, box D), a thread per
+// coefficient of D's multipole; the waves about D's center at B's non-QBX
+// targets staged in LDS in chunks of 8; atomics into M~'_D
+__global__ void __launch_bounds__(448) kh_adj_m2p(HD hd, TD t, int ntb, int n3, const int *tboxes,
+                                                  const int *starts, const int *lists,
+                                                  const int *fstart, const int *fcount,
+                                                  const int *unfilt, const double *tgt, int nt,
+                                                  const double *pot, double *madj)
+{
+    constexpr int TC = 8;
+    __shared__ double Fs[2 * TC * (MAXP + 1) * (MAXP + 1)];
+    __shared__ double rads[2 * TC * (MAXP + 1)];
+    __shared__ double jts[TC * (MAXP + 3)];
+    __shared__ double ds[3 * TC];
+    __shared__ double gs[2 * TC];
+    const int e = blockIdx.x;
+    if (e >= n3) return;
+    const int itb = entry_box(starts, ntb, e);
+    const int B = tboxes[itb];
+    const int f0 = fstart[B], nf = fcount[B];
+    if (nf == 0) return;
+    const int tid = threadIdx.x;
+    cplx *F = reinterpret_cast<cplx *>(Fs);
+    cplx *rad = reinterpret_cast<cplx *>(rads);
+    const int D = lists[e];
+    const int lv = t.level[D];
+    const int p = hd.order[lv];
+    const int nco = (p + 1) * (p + 1), ncf = (MAXP + 1) * (MAXP + 1);
+    const v3 c = boxc(t, D);
+    int n_o = 0, m_o = 0;
+    if (tid < nco) nm_of(tid, n_o, m_o);
+    cplx acc = {0.0, 0.0};
+    for (int c0 = 0; c0 < nf; c0 += TC) {
+        const int cnt = min(TC, nf - c0);
+        __syncthreads();  // the previous chunk consumed
+        if (tid < cnt) {
+            const int ti = unfilt[f0 + c0 + tid];
+            const v3 d = sub(ld3(tgt, nt, ti), c);
+            ds[3 * tid] = d.x;
+            ds[3 * tid + 1] = d.y;
+            ds[3 * tid + 2] = d.z;
+            gs[2 * tid] = pot[2 * (size_t)ti];
+            gs[2 * tid + 1] = pot[2 * (size_t)ti + 1];
+            const double rr = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+            cplx *rd = rad + tid * (MAXP + 1);
+            radial_seq(1, p, hd.k * rr, rd, jts + tid * (MAXP + 3));
+            for (int n = 0; n <= p; ++n) rd[n] = conj(rd[n]);
+        }
+        __syncthreads();
+        for (int w = tid; w < cnt * (p + 1); w += blockDim.x) {
+            const int q = w / (p + 1), m = w % (p + 1);
+            const v3 d = {ds[3 * q], ds[3 * q + 1], ds[3 * q + 2]};
+            cplx *Fq = F + (long)q * ncf;
+            const cplx *rd = rad + q * (MAXP + 1);
+            ylm_column(m, p, d, [&](int n, cplx y) {
+                Fq[IDX(n, m)] = rd[n] * y;
+                if (m > 0) Fq[IDX(n, -m)] = rd[n] * sgn_conj(y, m);
+            });
+        }
+        __syncthreads();
+        if (tid < nco)
+            for (int q = 0; q < cnt; ++q) {
+                cplx f = F[(long)q * ncf + IDX(n_o, -m_o)];
+                if (m_o & 1) f = -f;
+                acc += cplx{gs[2 * q], gs[2 * q + 1]} * f;
+            }
+    }
+    if (tid < nco)
+        atomic_addc(madj + ecoef_off(hd, D, lv) + 2L * tid * hd.nr, ipow(hd.s[lv], n_o) * acc);
 }
 
 // ---- adjoint of the QBX stages (unscaled QBX expansions Lq_c) -------------
@@ -1903,6 +2090,28 @@ int h3_eval_targets(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_tra
     const TD td = make_td(T);
     const int *starts = mode == 1 ? R->l3_starts.p : R->l1_starts.p;
     const int *lists = mode == 1 ? R->l3_lists.p : R->l1_lists.p;
+    if (mode != 2) {
+        // P2P and M2P over the geometry's balanced work units
+        QBX_CHECK(G->trav == R, "P2P / M2P: geometry was built for another traversal");
+        const unit_plan &U = mode == 0 ? G->p2p_plan : G->m2p_plan;
+        if (U.nunits == 0) return QBX_OK;
+#define H3_EU(M, N)                                                                        \
+    hipLaunchKernelGGL((kh_eval_u<M, N>), dim3(U.nunits), dim3(64), 0, st, hd, td,         \
+                       U.unit_box.p, U.unit_t0.p, U.box_g.p, R->target_boxes.p, starts,   \
+                       lists, G->nqbtl_box_target_starts.p,                              \
+                       G->nqbtl_box_target_counts_nonchild.p, G->nqbtl_unfiltered.p,      \
+                       T->targets.p, T->ntargets, T->sources.p, T->nsources, str, nrm, expn, \
+                       pot)
+#define H3_EU0(N) H3_EU(0, N)
+#define H3_EU1(N) H3_EU(1, N)
+        if (mode == 0) FOR_CHUNKS(hd, NRC_E, H3_EU0);
+        else FOR_CHUNKS(hd, NRC_E, H3_EU1);
+#undef H3_EU0
+#undef H3_EU1
+#undef H3_EU
+        QBX_HIP(hipGetLastError());
+        return QBX_OK;
+    }
 #define H3_EV(M, N)                                                                        \
     hipLaunchKernelGGL((kh_eval<M, N>), dim3(ntb), dim3(64), 0, st, hd, td, ntb,           \
                        R->target_boxes.p, starts, lists, G->nqbtl_box_target_starts.p,    \
@@ -2096,18 +2305,23 @@ static int adjoint_launch(hipStream_t st, const qbx_tree *T, const qbx_trav *R,
         }
         break;
     case QBX_ADJ_L2P:
-    case QBX_ADJ_M2P:
-        if (ntb > 0 && G->nfiltered > 0) {
-            auto kern = stage == QBX_ADJ_L2P ? kh_adj_e2p<true> : kh_adj_e2p<false>;
-            hipLaunchKernelGGL(kern, dim3(ntb), dim3(448), 0, st, hd, td, ntb, R->target_boxes.p,
-                               R->l3_starts.p, R->l3_lists.p, G->nqbtl_box_target_starts.p,
+        if (ntb > 0 && G->nfiltered > 0)
+            hipLaunchKernelGGL(kh_adj_e2p<true>, dim3(ntb), dim3(448), 0, st, hd, td, ntb,
+                               R->target_boxes.p, nullptr, nullptr, G->nqbtl_box_target_starts.p,
                                G->nqbtl_box_target_counts_nonchild.p, G->nqbtl_unfiltered.p,
                                T->targets.p, T->ntargets, in, out);
-        }
+        break;
+    case QBX_ADJ_M2P:
+        if (R->n3 > 0 && G->nfiltered > 0)
+            hipLaunchKernelGGL(kh_adj_m2p, dim3(R->n3), dim3(448), 0, st, hd, td, ntb, R->n3,
+                               R->target_boxes.p, R->l3_starts.p, R->l3_lists.p,
+                               G->nqbtl_box_target_starts.p,
+                               G->nqbtl_box_target_counts_nonchild.p, G->nqbtl_unfiltered.p,
+                               T->targets.p, T->ntargets, in, out);
         break;
     case QBX_ADJ_P2P:
-        if (ntb > 0 && G->nfiltered > 0)
-            hipLaunchKernelGGL(kh_adj_p2p, dim3(ntb), dim3(64), 0, st, hd, td, ntb,
+        if (R->n1 > 0 && G->nfiltered > 0)
+            hipLaunchKernelGGL(kh_adj_p2p, dim3(R->n1), dim3(64), 0, st, hd, td, ntb, R->n1,
                                R->target_boxes.p, R->l1_starts.p, R->l1_lists.p,
                                G->nqbtl_box_target_starts.p,
                                G->nqbtl_box_target_counts_nonchild.p, G->nqbtl_unfiltered.p,
