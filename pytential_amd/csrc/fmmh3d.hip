@@ -834,6 +834,101 @@ __global__ void __launch_bounds__(64) kh_eval(HD hd, TD t, int ntb, const int *t
     }
 }
 
+// ---- M2L grouped by translation class, on the FP64 matrix cores ----------
+// One density.  The list-2 pairs of the levels with M2L tables, sorted by
+// (level, offset class), in items of 16 pairs of one class: per item the
+// complex product Y[16 pairs][nc] = A[16][nc] T_cls[nc][nc] (A the pairs'
+// source multipoles M~, T the scaled table, so a row of Y is the pair's L~
+// contribution), tiled for v_mfma_f64_16x16x4f64 (lane i holds A[i%16][k0 +
+// i/16] and B[k0 + i/16][i%16]; D register j of lane i is D[4j + i/16][i%16];
+// the complex product is four real MFMAs, A_im negated for the real part).
+// The item's 16 source rows are staged in LDS; the block's four waves take
+// every fourth 16-column tile of T, so each item reads its table once.  The
+// rows go to per-pair scratch; kh_m2l_cls_sum adds each target box's rows in
+// list order (deterministic).
+typedef double h3_d4 __attribute__((ext_vector_type(4)));
+constexpr int H3G_NP = 16;  // pairs per item
+
+__global__ void __launch_bounds__(256) kh_m2l_cls(HD hd, TD t, int nitems, const int *items,
+                                                  const int *pe, const int *l2_lists,
+                                                  const int *yoff, const double *mpole,
+                                                  double *Y)
+{
+    extern __shared__ double sh[];
+    const int it = blockIdx.x;
+    if (it >= nitems) return;
+    const int lv = items[4 * it], cls = items[4 * it + 1], a0 = items[4 * it + 2],
+              cnt = items[4 * it + 3];
+    const int nc = hd.ncl[lv];
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, col = lane & 15,
+              kr = lane >> 4;
+    cplx *As = reinterpret_cast<cplx *>(sh);
+    for (int w = tid; w < H3G_NP * nc; w += blockDim.x) {
+        const int q = w / nc, j = w - q * nc;
+        cplx v = {0.0, 0.0};
+        if (q < cnt) v = ldc(mpole + ecoef_off(hd, l2_lists[pe[a0 + q]], lv) + 2L * j);
+        As[w] = v;
+    }
+    __syncthreads();
+    const double *T = hd.tab + 2 * (hd.m2l_off[lv] + (long)cls * nc * nc);
+    const int KS = (nc + 3) / 4, OT = (nc + 15) / 16;
+    const int nw = blockDim.x >> 6;
+    for (int ot = wv; ot < OT; ot += nw) {
+        const int o = ot * 16 + col;
+        h3_d4 cre = {0, 0, 0, 0}, cim = {0, 0, 0, 0};
+        for (int ks = 0; ks < KS; ++ks) {
+            const int k = ks * 4 + kr;
+            double are = 0.0, aim = 0.0, bre = 0.0, bim = 0.0;
+            if (k < nc) {
+                const cplx a = As[col * nc + k];
+                are = a.x;
+                aim = a.y;
+                if (o < nc) {
+                    const cplx b = ldc(T + 2 * ((long)k * nc + o));
+                    bre = b.x;
+                    bim = b.y;
+                }
+            }
+            cre = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bre, cre, 0, 0, 0);
+            cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-aim, bim, cre, 0, 0, 0);
+            cim = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bim, cim, 0, 0, 0);
+            cim = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, bre, cim, 0, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int q = 4 * j + kr;
+            if (q < cnt && o < nc)
+                stc(Y + 2 * ((long)yoff[a0 + q] + o), cplx{cre[j], cim[j]});
+        }
+    }
+}
+
+// the grouped rows of each target-or-target-parent box's list-2 pairs added
+// into its local, in list order (levels without M2L tables are skipped)
+__global__ void __launch_bounds__(128) kh_m2l_cls_sum(HD hd, TD t, int nttp, const int *ttp,
+                                                      const int *starts, const int *pos,
+                                                      const int *yoff, const double *Y,
+                                                      double *local)
+{
+    const int i = blockIdx.x;
+    if (i >= nttp) return;
+    const int e0 = starts[i], e1 = starts[i + 1];
+    if (e0 == e1) return;
+    const int B = ttp[i];
+    const int lv = t.level[B];
+    if (hd.m2l_off[lv] < 0) return;
+    const int nc = hd.ncl[lv];
+    double *o = local + ecoef_off(hd, B, lv);
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+        cplx acc = {0.0, 0.0};
+        for (int e = e0; e < e1; ++e) {
+            const int j = pos[e];
+            if (j >= 0) acc += ldc(Y + 2 * ((long)yoff[j] + c));
+        }
+        stc(o + 2L * c, ldc(o + 2L * c) + acc);
+    }
+}
+
 // P2P (MODE 0, list 1) and M2P (MODE 1, list 3) over balanced work units
 // (unit_plan: a 64-lane wavefront per unit, the unit's targets take G lanes
 // apiece, each lane walks every G-th source / list-3 entry) with the G
@@ -2024,6 +2119,91 @@ int h3_coarsen_multipoles(qbx_ctx *ctx, void *stream, const qbx_tree *T, const q
     return QBX_OK;
 }
 
+// the class-grouped list-2 pairs of the levels with M2L tables (kh_m2l_cls),
+// cached on the traversal: pe[j] the list entry of sorted pair j, pos[e] its
+// position (-1: level without a table), yoff[j] its scratch row, items
+// (level, class, first pair, count)
+static int h3_m2l_groups(const qbx_tree *T, const qbx_trav *R, const HD &hd)
+{
+    std::vector<int> lev = {-3};  // the 3D Helmholtz grouping (a 2D one is keyed apart)
+    for (int l = 0; l < hd.nlevels; ++l)
+        if (hd.m2l_off[l] >= 0) lev.push_back(l * 1000 + hd.ncl[l]);
+    if (R->m2lg_nitems >= 0 && R->m2lg_levels == lev) return QBX_OK;
+    const int nttp = R->nttp_boxes, n2 = R->n2, nb = T->nboxes;
+    std::vector<int> starts(nttp + 1), lists(std::max(n2, 1)), ttp(std::max(nttp, 1)),
+        coords(3L * nb);
+    QBX_HIP(hipDeviceSynchronize());
+    QBX_HIP(hipMemcpy(starts.data(), R->l2_starts.p, sizeof(int) * (nttp + 1),
+                      hipMemcpyDeviceToHost));
+    if (n2 > 0)
+        QBX_HIP(hipMemcpy(lists.data(), R->l2_lists.p, sizeof(int) * n2, hipMemcpyDeviceToHost));
+    if (nttp > 0)
+        QBX_HIP(hipMemcpy(ttp.data(), R->ttp_boxes.p, sizeof(int) * nttp, hipMemcpyDeviceToHost));
+    QBX_HIP(hipMemcpy(coords.data(), T->box_coords.p, sizeof(int) * 3L * nb,
+                      hipMemcpyDeviceToHost));
+    const int W = hd.W, S1 = 2 * W + 1, S3 = S1 * S1 * S1;
+    std::vector<std::pair<long, int>> kv;
+    for (int l = 0; l < T->nlevels; ++l) {
+        if (hd.m2l_off[l] < 0) continue;
+        for (int i = R->h_level_start_ttp_box_nrs[l]; i < R->h_level_start_ttp_box_nrs[l + 1];
+             ++i) {
+            const int P = ttp[i];
+            for (int e = starts[i]; e < starts[i + 1]; ++e) {
+                const int A = lists[e];
+                const int dx = coords[P] - coords[A], dy = coords[nb + P] - coords[nb + A],
+                          dz = coords[2L * nb + P] - coords[2L * nb + A];
+                QBX_CHECK(std::abs(dx) <= W && std::abs(dy) <= W && std::abs(dz) <= W,
+                          "M2L: list-2 offset (%d, %d, %d) outside the class table", dx, dy, dz);
+                kv.push_back({(long)l * S3 + ((dx + W) * S1 + (dy + W)) * S1 + (dz + W), e});
+            }
+        }
+    }
+    std::sort(kv.begin(), kv.end());
+    const int np = (int)kv.size();
+    std::vector<int> pos(std::max(n2, 1), -1), eo(std::max(np, 1)), items;
+    for (int j = 0; j < np; ++j) {
+        pos[kv[j].second] = j;
+        eo[j] = kv[j].second;
+    }
+    for (int j = 0; j < np;) {
+        int j1 = j;
+        while (j1 < np && kv[j1].first == kv[j].first) ++j1;
+        const int l = (int)(kv[j].first / S3), cls = (int)(kv[j].first % S3);
+        for (int a = j; a < j1; a += H3G_NP) {
+            items.push_back(l);
+            items.push_back(cls);
+            items.push_back(a);
+            items.push_back(std::min(H3G_NP, j1 - a));
+        }
+        j = j1;
+    }
+    std::vector<int> yoff(std::max(np, 1));
+    long y = 0;
+    for (int j = 0; j < np; ++j) {
+        yoff[j] = (int)y;
+        y += hd.ncl[kv[j].first / S3];
+    }
+    QBX_CHECK(y < (1L << 31), "M2L: %ld grouped rows overflow the row offsets", y);
+    const int ni = (int)items.size() / 4;
+    QBX_TRY(R->m2lg_e.alloc(std::max(np, 1)));
+    QBX_TRY(R->m2lg_pos.alloc(std::max(n2, 1)));
+    QBX_TRY(R->m2lg_yoff.alloc(std::max(np, 1)));
+    QBX_TRY(R->m2lg_items.alloc(std::max(4 * ni, 4)));
+    QBX_HIP(hipMemcpy(R->m2lg_e.p, eo.data(), sizeof(int) * eo.size(), hipMemcpyHostToDevice));
+    QBX_HIP(hipMemcpy(R->m2lg_pos.p, pos.data(), sizeof(int) * pos.size(),
+                      hipMemcpyHostToDevice));
+    QBX_HIP(hipMemcpy(R->m2lg_yoff.p, yoff.data(), sizeof(int) * yoff.size(),
+                      hipMemcpyHostToDevice));
+    if (ni > 0)
+        QBX_HIP(hipMemcpy(R->m2lg_items.p, items.data(), sizeof(int) * items.size(),
+                          hipMemcpyHostToDevice));
+    R->m2lg_nitems = ni;
+    R->m2lg_nitems2 = 0;
+    R->m2lg_nrows = y;
+    R->m2lg_levels = lev;
+    return QBX_OK;
+}
+
 int h3_multipole_to_local(qbx_ctx *ctx, void *stream, const qbx_tree *T, const qbx_trav *R,
                           const qbx_kernel_desc *d, const double *mpole, double *local)
 {
@@ -2034,6 +2214,26 @@ int h3_multipole_to_local(qbx_ctx *ctx, void *stream, const qbx_tree *T, const q
     QBX_TRY(zero_fill(local, exp_doubles(hd), st));
     if (R->n2 == 0) return QBX_OK;
     const TD td = make_td(T);
+    if (hd.nr == 1) {
+        // one density: class-grouped on the matrix cores
+        QBX_TRY(h3_m2l_groups(T, R, hd));
+        if (R->m2lg_nitems > 0) {
+            QBX_TRY(ws_reserve(ctx, st, ctx->ws_m2lg, 2 * std::max(R->m2lg_nrows, 1L),
+                               "grouped M2L rows"));
+            int ncmax = 1;
+            for (int l = 0; l < hd.nlevels; ++l)
+                if (hd.m2l_off[l] >= 0) ncmax = std::max(ncmax, hd.ncl[l]);
+            const size_t by = sizeof(double) * 2 * H3G_NP * (size_t)ncmax;
+            hipLaunchKernelGGL(kh_m2l_cls, dim3(R->m2lg_nitems), dim3(256), by, st, hd, td,
+                               R->m2lg_nitems, R->m2lg_items.p, R->m2lg_e.p, R->l2_lists.p,
+                               R->m2lg_yoff.p, mpole, ctx->ws_m2lg.p);
+            hipLaunchKernelGGL(kh_m2l_cls_sum, dim3(R->nttp_boxes), dim3(128), 0, st, hd, td,
+                               R->nttp_boxes, R->ttp_boxes.p, R->l2_starts.p, R->m2lg_pos.p,
+                               R->m2lg_yoff.p, ctx->ws_m2lg.p, local);
+            QBX_HIP(hipGetLastError());
+        }
+        return QBX_OK;
+    }
     for (int l = 0; l < T->nlevels; ++l) {
         const int i0 = R->h_level_start_ttp_box_nrs[l], i1 = R->h_level_start_ttp_box_nrs[l + 1];
         if (i1 == i0 || hd.m2l_off[l] < 0) continue;
