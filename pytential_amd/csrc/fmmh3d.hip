@@ -1273,8 +1273,9 @@ __global__ void __launch_bounds__(448) kh_adj_e2p(HD hd, TD t, int ntb, const in
     }
 }
 
-// M2M^T (OP 0: parents of level lv to their children), M2L^T (OP 1:
-// target-or-target-parent boxes of level lv to their list-2 boxes), L2L^T
+// M2M^T (OP 0: parents of level lv to their children), M2L^T (OP 1, per
+// target box: target-or-target-parent boxes of level lv to their list-2
+// boxes; the adjoint driver uses the class-grouped kh_m2l_cls_adj), L2L^T
 // (OP 2: boxes of level lv >= 1 to their parents): in'_j += sum_i
 // conj(T[j][i]) out'_i.  The output box's adjoint staged in LDS; a
 // wavefront per table row j at a time, its lanes along the row (coalesced)
@@ -1702,6 +1703,69 @@ __global__ void __launch_bounds__(64) kh_adj_p2qbxl(HD hd, TD t, int nunits, con
             atomic_addc(sadj + 2 * (size_t)si, cplx{0.0, -hd.k} * acc);
         }
         base += cnt;
+    }
+}
+
+// M2L^T on the same items: Y[16 pairs][nc] = A[16][nc] B with A the pairs'
+// target-box local adjoints L~' and B[k][o] = conj(T_cls[o][k]); each row
+// added into its source box's multipole adjoint with atomics
+__global__ void __launch_bounds__(256) kh_m2l_cls_adj(HD hd, TD t, int nitems, const int *items,
+                                                      const int *pe, const int *l2_starts,
+                                                      const int *l2_lists, int nttp,
+                                                      const int *ttp, const double *ladj,
+                                                      double *madj)
+{
+    extern __shared__ double sh[];
+    __shared__ int srcbox[H3G_NP];
+    const int it = blockIdx.x;
+    if (it >= nitems) return;
+    const int lv = items[4 * it], cls = items[4 * it + 1], a0 = items[4 * it + 2],
+              cnt = items[4 * it + 3];
+    const int nc = hd.ncl[lv];
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, col = lane & 15,
+              kr = lane >> 4;
+    cplx *As = reinterpret_cast<cplx *>(sh);
+    if (tid < H3G_NP) srcbox[tid] = tid < cnt ? l2_lists[pe[a0 + tid]] : -1;
+    for (int w = tid; w < H3G_NP * nc; w += blockDim.x) {
+        const int q = w / nc, j = w - q * nc;
+        cplx v = {0.0, 0.0};
+        if (q < cnt) {
+            const int B = ttp[entry_box(l2_starts, nttp, pe[a0 + q])];
+            v = ldc(ladj + ecoef_off(hd, B, lv) + 2L * j);
+        }
+        As[w] = v;
+    }
+    __syncthreads();
+    const double *T = hd.tab + 2 * (hd.m2l_off[lv] + (long)cls * nc * nc);
+    const int KS = (nc + 3) / 4, OT = (nc + 15) / 16;
+    const int nw = blockDim.x >> 6;
+    for (int ot = wv; ot < OT; ot += nw) {
+        const int o = ot * 16 + col;
+        h3_d4 cre = {0, 0, 0, 0}, cim = {0, 0, 0, 0};
+        for (int ks = 0; ks < KS; ++ks) {
+            const int k = ks * 4 + kr;
+            double are = 0.0, aim = 0.0, bre = 0.0, bim = 0.0;
+            if (k < nc) {
+                const cplx a = As[col * nc + k];
+                are = a.x;
+                aim = a.y;
+                if (o < nc) {
+                    const cplx b = ldc(T + 2 * ((long)o * nc + k));
+                    bre = b.x;
+                    bim = -b.y;
+                }
+            }
+            cre = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bre, cre, 0, 0, 0);
+            cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-aim, bim, cre, 0, 0, 0);
+            cim = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bim, cim, 0, 0, 0);
+            cim = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, bre, cim, 0, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int q = 4 * j + kr;
+            if (q < cnt && o < nc)
+                atomic_addc(madj + ecoef_off(hd, srcbox[q], lv) + 2L * o, cplx{cre[j], cim[j]});
+        }
     }
 }
 
@@ -2539,15 +2603,17 @@ static int adjoint_launch(hipStream_t st, const qbx_tree *T, const qbx_trav *R,
         }
         break;
     case QBX_ADJ_M2L:
-        if (R->n2 > 0)
-            for (int l = 0; l < T->nlevels; ++l) {
-                const int i0 = R->h_level_start_ttp_box_nrs[l];
-                const int i1 = R->h_level_start_ttp_box_nrs[l + 1];
-                if (i1 > i0 && hd.m2l_off[l] >= 0)
-                    hipLaunchKernelGGL(kh_trans_adj<1>, dim3(i1 - i0), dim3(256), 0, st, hd, td,
-                                       l, i0, i1 - i0, R->ttp_boxes.p, R->l2_starts.p,
-                                       R->l2_lists.p, in, out);
+        if (R->n2 > 0) {
+            // class-grouped on the matrix cores (the forward's items)
+            QBX_TRY(h3_m2l_groups(T, R, hd));
+            if (R->m2lg_nitems > 0) {
+                const size_t by = sizeof(double) * 2 * H3G_NP * (size_t)ncl_max;
+                hipLaunchKernelGGL(kh_m2l_cls_adj, dim3(R->m2lg_nitems), dim3(256), by, st, hd,
+                                   td, R->m2lg_nitems, R->m2lg_items.p, R->m2lg_e.p,
+                                   R->l2_starts.p, R->l2_lists.p, R->nttp_boxes,
+                                   R->ttp_boxes.p, in, out);
             }
+        }
         break;
     case QBX_ADJ_P2L:
         if (R->nttp_boxes > 0 && R->n4 > 0)
