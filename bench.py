@@ -1062,6 +1062,15 @@ def ellipsoid_qbx_fmm(d, k0, rng, steps=5):
 
 
 def main_ellipsoid3d(args):
+    if args.ell_qbx_fmm_only:
+        # the QBX-FMM of S_k sigma at the surface nodes and its adjoint alone
+        # (for profiles): no loss/gradient step, dense checks or CPU leg
+        from pytential_amd.surface import ellipsoid
+        d = ellipsoid(args.nsub_ell, order=4, fine_order=6)
+        res = ellipsoid_qbx_fmm(d, 5.0, np.random.default_rng(1004), steps=args.steps)
+        print(json.dumps({"metric": METRIC, "n_gpus": 1, "data": "synthetic", **res}),
+              flush=True)
+        return
     print(json.dumps(run_ellipsoid3d(args)), flush=True)
 
 
@@ -1316,6 +1325,8 @@ def main():
     ap.add_argument("--ncurves", type=int, default=64, help="dirichlet64: curves")
     ap.add_argument("--nsub-ell", type=int, default=5,
                     help="ellipsoid3d: icosahedron refinements (5: 20480 triangles)")
+    ap.add_argument("--ell-qbx-fmm-only", action="store_true",
+                    help="ellipsoid3d: only the surface QBX-FMM and its adjoint")
     ap.add_argument("--nsub", type=int, default=5, help="sphere3d: icosahedron refinements")
     ap.add_argument("--fine3d", type=int, default=6, help="sphere3d: quad rule n x n")
     ap.add_argument("--nrhs3d", type=int, default=32, help="sphere3d: densities")
