@@ -1022,19 +1022,22 @@ __global__ void __launch_bounds__(64) kh_e2qbxl(HD hd, TD t, int ng, const int *
                                                 const double *ctrs, int nctr,
                                                 const double *expn, double *qexp)
 {
-    __shared__ double Xs[2 * (MAXP + MAXPQ + 1) * (MAXP + MAXPQ + 1)];
-    __shared__ double as[2 * (MAXP + 1) * (MAXP + 1) * NRC];
-    __shared__ double accs[2 * 96 * NRC];  // (output, slice) slots: nco * parts <= 81
-    __shared__ double rads[2 * (MAXP + MAXPQ + 3)];
-    __shared__ double jt[MAXP + MAXPQ + 4];
+    // LDS sized for this launch's orders (e2qbxl_lds): X (L+1)^2, the box
+    // coefficients (p+1)^2 NRC, the (output, slice) sums 96 NRC and the
+    // radial sequence, L = max p + p_q -- a few KB, so that many blocks
+    // share a CU while one lane computes a radial sequence
+    extern __shared__ double sh[];
+    const int Lm = hd.pmax + hd.pq;
+    cplx *X = reinterpret_cast<cplx *>(sh);
+    cplx *a = X + (Lm + 1) * (Lm + 1);
+    cplx *sacc = a + (hd.pmax + 1) * (hd.pmax + 1) * NRC;
+    double *rads = reinterpret_cast<double *>(sacc + 96 * NRC);  // Lm + 3 complex
+    double *jt = rads + 2 * (Lm + 3);                             // Lm + 4
     const int g = blockIdx.x;
     if (g >= ng) return;
     const int ci = gcent[g];
     const int itb = c2tb[ci];
     const v3 c = ld3(ctrs, nctr, ci);
-    cplx *X = reinterpret_cast<cplx *>(Xs);
-    cplx *a = reinterpret_cast<cplx *>(as);
-    cplx *sacc = reinterpret_cast<cplx *>(accs);
     const int tid = threadIdx.x;
     const int pq = hd.pq, nco = hd.ncq;
     const int parts = nco >= 64 ? 1 : 64 / nco;
@@ -2056,12 +2059,14 @@ static int ensure_tables(qbx_ctx *ctx, hipStream_t st, const qbx_tree *T, HD *hd
 
 static inline long exp_doubles(const HD &hd) { return 2L * hd.lcoef[hd.nlevels] * hd.nr; }
 
-// LDS bytes and chunk size of kh_p2e for orders up to pfmax
+// LDS bytes and chunk size of kh_p2e for orders up to pfmax: chunks within
+// P2E_LDS_BUDGET KiB, so that several blocks share a CU
+constexpr long P2E_LDS_BUDGET = 20;
 static void p2e_shape(int pfmax, int nrc, int *CH, size_t *bytes)
 {
     const long ncf = (long)(pfmax + 1) * (pfmax + 1);
     const long per = 16 * ncf + 16L * (pfmax + 1) + 16L * nrc + 48 + 8L * (pfmax + 3);
-    int ch = (int)std::min(32L, (56L * 1024) / per);
+    int ch = (int)std::min(32L, (P2E_LDS_BUDGET * 1024) / per);
     ch = std::max(ch, 1);
     *CH = ch;
     *bytes = (size_t)(per * ch);
@@ -2444,6 +2449,15 @@ int h3_form_global_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T,
                          T->sources.p, T->nsources, str, nrm, hd.pq, qexp);
 }
 
+// dynamic LDS bytes of kh_e2qbxl
+static size_t e2qbxl_lds(const HD &hd, int nrc)
+{
+    const long Lm = hd.pmax + hd.pq;
+    const long cpl = (Lm + 1) * (Lm + 1) + (long)(hd.pmax + 1) * (hd.pmax + 1) * nrc + 96L * nrc +
+                     (Lm + 3);
+    return (size_t)(16 * cpl + 8 * (Lm + 4));
+}
+
 static int h3_e2qbxl(qbx_ctx *ctx, hipStream_t st, const qbx_tree *T, const qbx_trav *R,
                      const qbx_geo *G, const qbx_kernel_desc *d, int mode, const double *expn,
                      double *qexp, int accumulate)
@@ -2458,7 +2472,7 @@ static int h3_e2qbxl(qbx_ctx *ctx, hipStream_t st, const qbx_tree *T, const qbx_
     QBX_CHECK(G->trav == R, "M2QBXL / L2QBXL: geometry was built for another traversal");
     const TD td = make_td(T);
 #define H3_EQ(M, N)                                                                        \
-    hipLaunchKernelGGL((kh_e2qbxl<M, N>), dim3(ng), dim3(64), 0, st, hd, td, ng,           \
+    hipLaunchKernelGGL((kh_e2qbxl<M, N>), dim3(ng), dim3(64), e2qbxl_lds(hd, N), st, hd, td, ng, \
                        G->global_qbx_centers.p, G->center_to_target_box.p,                 \
                        R->target_boxes.p, R->l3_starts.p, R->l3_lists.p, G->centers.p,     \
                        G->ncenters, expn, qexp)
