@@ -1963,15 +1963,18 @@ __device__ void qbx_store(const KD &kd, double *qexp, int c, const QAcc<PQ, NRC>
 // strengths qs) to the order-PQ expansion A about a center of radius R.  No
 // atan2/sincos: the phase e^{-i theta} is conj(w)/|w|.  tab: the Hankel
 // series coefficients (hankel01_wave).
-template <int PQ, int NRC>
+// KN, DP: the kernel and source kind fixed at compile time (-1: read from kd)
+template <int PQ, int NRC, int KN = -1, int DP = -1>
 __device__ __forceinline__ void p2qbxl_pair(const KD &kd, cplx w, cplx nu, const cplx *qs,
                                             double R, const double *tab, QAcc<PQ, NRC> &A)
 {
-    if (kd.kernel == QBX_LAPLACE2D) {
+    const int kern = KN >= 0 ? KN : kd.kernel;
+    const bool dip = DP >= 0 ? DP != 0 : kd.dip != 0;
+    if (kern == QBX_LAPLACE2D) {
         const cplx iw = cinv(w);
         const cplx tt = R * iw;
         cplx pw = {1, 0};
-        if (!kd.dip) {
+        if (!dip) {
             const cplx lg = clogz(-w);
 #pragma unroll
             for (int q = 0; q < NRC; ++q) A.v[q][0] += qs[q] * lg;
@@ -2016,7 +2019,7 @@ __device__ __forceinline__ void p2qbxl_pair(const KD &kd, cplx w, cplx nu, const
         Gv[PQ + 1 - m] = (m & 1) ? -gm : gm;
         e = e * em;
     }
-    if (!kd.dip) {
+    if (!dip) {
 #pragma unroll
         for (int n = -PQ; n <= PQ; ++n)
 #pragma unroll
@@ -2056,7 +2059,7 @@ struct P2QTile {
     static constexpr int CH = NRC == 1 ? 256 : 64;
 };
 
-template <int PQ, int NRC>
+template <int PQ, int NRC, int KN = -1, int DP = -1>
 __global__ void __launch_bounds__(64) k_p2qbxl(KD kd, TD t, const int *ubox, const int *ut0,
                                                const int *box_g, const int *gcs,
                                                const int *gcl, const double *ctr,
@@ -2076,7 +2079,7 @@ __global__ void __launch_bounds__(64) k_p2qbxl(KD kd, TD t, const int *ubox, con
     const int tsub = lane / G, sub = lane & (G - 1);
     const int cl = ut0[u] + tsub;
     const bool active = cl < gcs[it + 1] - gcs[it];
-    const bool dip = kd.dip;
+    const bool dip = DP >= 0 ? DP != 0 : kd.dip != 0;
     const int rc = kd.rc, r0 = kd.r0, nr = kd.nr;
     int c = 0;
     cplx cc = {0, 0};
@@ -2104,7 +2107,8 @@ __global__ void __launch_bounds__(64) k_p2qbxl(KD kd, TD t, const int *ubox, con
                     qs[q] = q < rc ? cplx{sq[2 * (j * NRC + q)], sq[2 * (j * NRC + q) + 1]}
                                    : cplx{0, 0};
                 const cplx nu = dip ? cplx{snx[j], sny[j]} : cplx{0, 0};
-                p2qbxl_pair<PQ, NRC>(kd, cplx{sx[j], sy[j]} - cc, nu, qs, R, htab + toff, A);
+                p2qbxl_pair<PQ, NRC, KN, DP>(kd, cplx{sx[j], sy[j]} - cc, nu, qs, R,
+                                             htab + toff, A);
             }
         }
         __syncthreads();
@@ -4068,6 +4072,42 @@ int qbx_form_global_qbx_locals(qbx_ctx *ctx, void *stream, const qbx_tree *T, co
             }
 #undef L_P2QM
         }
+        QBX_HIP(hipGetLastError());
+        return QBX_OK;
+    }
+    if (kd.kernel == QBX_HELMHOLTZ2D && kd.nr == 1) {
+        // one Helmholtz density: the kernel and source kind fixed at compile time
+#define L_P2QH(PQV, DV)                                                                    \
+    hipLaunchKernelGGL((k_p2qbxl<PQV, 1, QBX_HELMHOLTZ2D, DV>), dim3(U.nunits), dim3(64), 0, \
+                       st, kd, td, U.unit_box.p, U.unit_t0.p, U.box_g.p,                   \
+                       G->gc_by_tbox_starts.p, G->gc_by_tbox.p, G->centers.p, G->radii.p,  \
+                       G->ncenters, R->l1_starts.p, R->l1_lists.p, T->sources.p,           \
+                       T->nsources, str, nrm, qexp, accumulate)
+#define L_P2QHD(PQV)          \
+    if (kd.dip)               \
+        L_P2QH(PQV, 1);       \
+    else                      \
+        L_P2QH(PQV, 0)
+        kd.r0 = 0;
+        kd.rc = 1;
+        switch (kd.pq) {
+        case 0: L_P2QHD(0); break;
+        case 1: L_P2QHD(1); break;
+        case 2: L_P2QHD(2); break;
+        case 3: L_P2QHD(3); break;
+        case 4: L_P2QHD(4); break;
+        case 5: L_P2QHD(5); break;
+        case 6: L_P2QHD(6); break;
+        case 7: L_P2QHD(7); break;
+        case 8: L_P2QHD(8); break;
+        case 9: L_P2QHD(9); break;
+        case 10: L_P2QHD(10); break;
+        case 11: L_P2QHD(11); break;
+        case 12: L_P2QHD(12); break;
+        default: set_error("qbx_order out of range"); return QBX_ERR_INVALID;
+        }
+#undef L_P2QHD
+#undef L_P2QH
         QBX_HIP(hipGetLastError());
         return QBX_OK;
     }
