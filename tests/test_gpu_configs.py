@@ -21,7 +21,8 @@ the CPU oracle and the closed forms.
   exterior targets: the forward against the oracle's dense rows; on the
   1280-triangle coarsening at all 4096 targets the gradient against the
   conjugate transpose of the oracle's dense operator and against central
-  differences (h = 1e-6) to 1e-7.
+  differences (h = 1e-6) to 1e-7; the same through the QBX-FMM and its
+  adjoint (the dense transpose to the FMM's accuracy, differences to 1e-7).
 """
 import os
 
@@ -288,3 +289,57 @@ def test_config5_full_size():
           flush=True)
     assert e_adj < 1e-12
     assert e_k < 1e-7 and e_v < 1e-7
+
+
+def test_config5_qbx_fmm_gradient():
+    """Configuration 5's loss and gradient through the QBX-FMM (fmm_order 10)
+    and its hand-written adjoint (layer_potential) instead of the exact sums,
+    on the 1280-triangle coarsening at all 4096 targets: dL/dsigma against
+    the oracle's dense conjugate transpose (to the FMM's accuracy), and both
+    gradients against central differences of the same FMM loss, 1e-7."""
+    import oracle
+
+    from pytential_amd.autodiff import layer_potential
+    from pytential_amd.problems import exterior_shell_targets
+    from pytential_amd.qbx import QBXLayerPotentialSource
+    from pytential_amd.surface import ellipsoid
+    k0 = 5.0
+    rng = np.random.default_rng(1004)
+    x0 = rng.uniform(-0.3, 0.3, 3) * np.array([1.0, 0.75, 0.5])
+    tg = exterior_shell_targets(4096, rng)
+    rp = np.linalg.norm(tg - x0[:, None], axis=0)
+    gt = torch.as_tensor(np.exp(1j * k0 * rp) / (4 * np.pi * rp), device="cuda")
+    dc = ellipsoid(3, order=4, fine_order=6)
+    srcc = QBXLayerPotentialSource(dc, fine_order=6, qbx_order=3, fmm_order=10)
+    tgd = torch.as_tensor(tg, device="cuda")
+    pairs = [(tgd, 0)]
+    sc = rng.normal(size=dc.ndofs) + 1j * rng.normal(size=dc.ndofs)
+
+    def loss_of(sv, kv):
+        return torch.sum(torch.abs(layer_potential(srcc, "helmholtz", sv, pairs, k=kv) - gt) ** 2)
+    s_t = torch.tensor(sc, dtype=torch.complex128, device="cuda", requires_grad=True)
+    k_t = torch.tensor(k0, dtype=torch.float64, requires_grad=True)
+    loss = loss_of(s_t, k_t)
+    loss.backward()
+    r = oracle.helmholtz3d_far_apply(dc, tg, k0, sc) - _np(gt)
+    grad_ref = oracle.helmholtz3d_far_adjoint(dc, tg, k0, 2 * r)
+    e_adj = _rel(_np(s_t.grad), grad_ref)
+
+    def lossf(sv, kv):
+        with torch.no_grad():
+            sv = torch.as_tensor(sv, device="cuda")
+            return float(loss_of(sv, torch.tensor(kv, dtype=torch.float64)))
+
+    h = 1e-6
+    fd_k = (lossf(sc, k0 + h) - lossf(sc, k0 - h)) / (2 * h)
+    v = rng.normal(size=dc.ndofs) + 1j * rng.normal(size=dc.ndofs)
+    fd_v = (lossf(sc + h * v, k0) - lossf(sc - h * v, k0)) / (2 * h)
+    an_v = float(np.real(np.vdot(_np(s_t.grad), v)))
+    e_k = abs(float(k_t.grad) - fd_k) / abs(fd_k)
+    e_v = abs(an_v - fd_v) / abs(fd_v)
+    print(f"\n[c5 fmm] 1280 triangles x 4096 targets: dL/dsigma (adjoint QBX-FMM) vs dense^H "
+          f"{e_adj:.2e}; central differences: dL/dk {e_k:.2e}, directional dL/dsigma "
+          f"{e_v:.2e}", flush=True)
+    assert e_adj < 1e-6
+    assert e_k < 1e-7 and e_v < 1e-7
+
