@@ -15,6 +15,7 @@
 // order (deterministic, no atomics).  Adjoint: one thread per source, the
 // targets staged through LDS.  Coincident points contribute zero.
 #include "qbx_internal.h"
+#include "qbx_math.h"
 
 namespace qbx {
 
@@ -51,13 +52,15 @@ k_h3_p2p_partial(int ns, const double *src, const double *q, int nt, const doubl
                 const double dx = tx - sx[j], dy = ty - sy[j], dz = tz - sz[j];
                 const double r2 = dx * dx + dy * dy + dz * dz;
                 if (r2 == 0.0) continue;
-                const double r = sqrt(r2);
+                // 1/r and r from the refined hardware estimate; sin and cos
+                // by qbx_sincos (a few ulp, no library call)
+                const double ir = qbx_rsqrt(r2);
+                const double r = r2 * ir;
                 double sn, cs;
-                sincos(k * r, &sn, &cs);
+                qbx_sincos(k * r, &sn, &cs);
                 // e^{ikr} q
                 const double er = cs * sqr[j] - sn * sqi[j];
                 const double ei = cs * sqi[j] + sn * sqr[j];
-                const double ir = 1.0 / r;
                 pr += er * ir;
                 pi += ei * ir;
                 // i e^{ikr} q
@@ -116,10 +119,10 @@ k_h3_p2p_adjoint(int ns, const double *src, int nt, const double *tgt, double k,
                 const double dx = tx_[j] - sx, dy = ty_[j] - sy, dz = tz_[j] - sz;
                 const double r2 = dx * dx + dy * dy + dz * dz;
                 if (r2 == 0.0) continue;
-                const double r = sqrt(r2);
+                const double ir = qbx_rsqrt(r2);
+                const double r = r2 * ir;
                 double sn, cs;
-                sincos(k * r, &sn, &cs);
-                const double ir = 1.0 / r;
+                qbx_sincos(k * r, &sn, &cs);
                 // e^{-ikr} u / r
                 vr += (cs * ur[j] + sn * ui[j]) * ir;
                 vi += (cs * ui[j] - sn * ur[j]) * ir;
