@@ -132,7 +132,7 @@ __device__ void sph_j_seq(int p, double x, double *j)
     }
     const double mx = fmax((double)pp, x);
     const int N = (int)mx + 10 + (int)(4.0 * sqrt(mx));
-    const double ix = 1.0 / x;
+    const double ix = qbx_rcp(x);
     double fp1 = 0.0, f = 1e-300;
     for (int n = N; n >= 0; --n) {
         if (n <= pp) j[n] = f;
@@ -146,7 +146,7 @@ __device__ void sph_j_seq(int p, double x, double *j)
         }
     }
     double sn, cs;
-    sincos(x, &sn, &cs);
+    qbx_sincos(x, &sn, &cs);
     const double j0 = sn * ix, j1 = (sn * ix - cs) * ix;
     const double sc = fabs(j0) >= fabs(j1) ? j0 / j[0] : j1 / j[1];
     for (int n = 0; n <= pp; ++n) j[n] *= sc;
@@ -163,8 +163,8 @@ __device__ void radial_seq(int sing, int p, double x, cplx *rad, double *jtmp)
         return;
     }
     double sn, cs;
-    sincos(x, &sn, &cs);
-    const double ix = 1.0 / x;
+    qbx_sincos(x, &sn, &cs);
+    const double ix = qbx_rcp(x);
     double ym2 = -cs * ix, ym1 = (-cs * ix - sn) * ix;
     for (int n = 0; n <= p; ++n) {
         double y;
@@ -183,32 +183,38 @@ __device__ void radial_seq(int sing, int p, double x, cplx *rad, double *jtmp)
 
 // Column m >= 0 of Y_n^m(d / |d|) for n = m..p: calls f(n, Y_n^m).
 // Q_m^m = -sqrt((2m+1)/(2m)) Q_{m-1}^{m-1} (Q_0^0 = 1/sqrt(4 pi)),
-// Q_{m+1}^m = sqrt(2m+3) z Q_m^m, Q_n^m = a (z Q_{n-1}^m - b Q_{n-2}^m),
-// Y_n^m = Q_n^m ((x + i y) / r)^m.
+// Q_{m+1}^m = sqrt(2m+3) z Q_m^m, Q_n^m = a_n (z Q_{n-1}^m - b_n Q_{n-2}^m),
+// a_n = sqrt((4n^2 - 1) / (n^2 - m^2)), b_n = 1 / a_{n-1};
+// Y_n^m = Q_n^m ((x + i y) / r)^m.  sqrt(A / B) = A / sqrt(A B) and its
+// inverse B / sqrt(A B) from one refined reciprocal square root per step
+// (no division, no library square root).
 template <class F>
 __device__ __forceinline__ void ylm_column(int m, int p, v3 d, F f)
 {
-    const double r = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
-    const double ir = r > 0.0 ? 1.0 / r : 0.0;
-    const double z = r > 0.0 ? d.z * ir : 1.0;
+    const double r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+    const double ir = r2 > 0.0 ? qbx_rsqrt(r2) : 0.0;
+    const double z = r2 > 0.0 ? d.z * ir : 1.0;
     const cplx w = {d.x * ir, d.y * ir};
     double qmm = 0.28209479177387814347;  // 1 / sqrt(4 pi)
     cplx wm = {1.0, 0.0};
     for (int i = 1; i <= m; ++i) {
-        qmm *= -sqrt((2.0 * i + 1.0) / (2.0 * i));
+        const double A = 2.0 * i + 1.0, B = 2.0 * i;
+        qmm *= -A * qbx_rsqrt(A * B);
         wm = wm * w;
     }
-    double a2 = 0.0, a1 = qmm;
+    double a2 = 0.0, a1 = qmm, binv = 0.0;
     f(m, qmm * wm);
     for (int n = m + 1; n <= p; ++n) {
         double q;
         if (n == m + 1) {
-            q = sqrt(2.0 * m + 3.0) * z * qmm;
+            const double rs = qbx_rsqrt(2.0 * m + 3.0);
+            q = (2.0 * m + 3.0) * rs * z * qmm;
+            binv = rs;
         } else {
-            const double a = sqrt((4.0 * n * n - 1.0) / ((double)n * n - (double)m * m));
-            const double b = sqrt(((n - 1.0) * (n - 1.0) - (double)m * m) /
-                                  (4.0 * (n - 1.0) * (n - 1.0) - 1.0));
-            q = a * (z * a1 - b * a2);
+            const double A = 4.0 * n * n - 1.0, B = (double)n * n - (double)m * m;
+            const double rs = qbx_rsqrt(A * B);
+            q = (A * rs) * (z * a1 - binv * a2);
+            binv = B * rs;
         }
         f(n, q * wm);
         a2 = a1;
@@ -513,11 +519,12 @@ template <int PF>
 __device__ __forceinline__ void s_waves_reg(double k, v3 d, cplx *F)
 {
     constexpr int PJ = PF < 1 ? 1 : PF;
-    const double r = sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
-    const double ir = 1.0 / r;
-    const double x = k * r, ix = 1.0 / x;
+    const double r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+    const double ir = qbx_rsqrt(r2);
+    const double r = r2 * ir;
+    const double x = k * r, ix = qbx_rcp(x);
     double sn, cs;
-    sincos(x, &sn, &cs);
+    qbx_sincos(x, &sn, &cs);
     double yv[PF + 1];
     yv[0] = -cs * ix;
     if (PF >= 1) yv[PF >= 1 ? 1 : 0] = (-cs * ix - sn) * ix;
@@ -737,9 +744,9 @@ __device__ __forceinline__ cplx p2p_val(double k, bool dip, v3 s, v3 nu, v3 x)
     const v3 d = sub(x, s);
     const double r2 = d.x * d.x + d.y * d.y + d.z * d.z;
     if (r2 == 0.0) return {0.0, 0.0};
-    const double r = sqrt(r2), ir = 1.0 / r;
+    const double ir = qbx_rsqrt(r2), r = r2 * ir;
     double sn, cs;
-    sincos(k * r, &sn, &cs);
+    qbx_sincos(k * r, &sn, &cs);
     const cplx e = {cs * ir * INV4PI, sn * ir * INV4PI};  // e^{ikr} / (4 pi r)
     if (!dip) return e;
     const double pr = (nu.x * d.x + nu.y * d.y + nu.z * d.z) * ir * ir;
