@@ -1109,6 +1109,175 @@ __global__ void __launch_bounds__(64) kh_e2qbxl(HD hd, TD t, int ng, const int *
     }
 }
 
+// shares (blockIdx.y) of the list-1 sources / list-3 boxes per center group
+// in M2QBXL, P2QBXL^T and M2QBXL^T
+constexpr int ADJ_SHARES = 4;
+
+__device__ __forceinline__ void atomic_addc(double *p, cplx v)
+{
+    atomicAdd(p, v.x);
+    atomicAdd(p + 1, v.y);
+}
+
+// ---- M2QBXL for one density, p_q <= 3, as a product on the matrix cores -----
+// The M2QBXL sum of kh_e2qbxl regrouped by the waves' index q = (l, m'):
+//   Q_c[o] = sum_D sum_q X_q(c - c_D) Bm_D[q][o],
+//   Bm_D[q = (l, m')][o = (nu, mu)] = 4 pi sum_{j = (n, m' + mu)}
+//       (-1)^((nu + l - n)/2) G(j, o, (l - |n - nu|)/2) s^n M~_D[j]
+// (the terms with l = |n - nu| + 2 ii, ii <= min(n, nu)).  Bm depends on the
+// box alone (kh_m2qb_mat, once per box and evaluation); the sum over q is,
+// per group of centers and list-3 box, a product of the centers' waves
+// X[centers][q] with Bm_D[q][16] on v_mfma_f64_16x16x4f64.
+__global__ void __launch_bounds__(256) kh_m2qb_mat(HD hd, TD t, int nqmax, const double *mpole,
+                                                   double *bm)
+{
+    __shared__ double as[2 * (MAXP + 1) * (MAXP + 1)];
+    const int D = blockIdx.x;
+    if (D >= t.nboxes) return;
+    if (t.scumul[D] == 0) {  // no multipole: a zero matrix
+        double *z = bm + 2L * D * nqmax * 16;
+        for (int w = threadIdx.x; w < 2 * nqmax * 16; w += blockDim.x) z[w] = 0.0;
+        return;
+    }
+    const int lv = t.level[D];
+    const int p = hd.order[lv];
+    const int ncin = (p + 1) * (p + 1);
+    const int pq = hd.pq, nco = hd.ncq;
+    const int L = p + pq, nq = (L + 1) * (L + 1);
+    const int ncg = (hd.pg + 1) * (hd.pg + 1);
+    cplx *a = reinterpret_cast<cplx *>(as);
+    const double s = hd.s[lv];
+    const double *src = mpole + ecoef_off(hd, D, lv);
+    for (int j = threadIdx.x; j < ncin; j += blockDim.x) {
+        int n, m;
+        nm_of(j, n, m);
+        a[j] = ipow(s, n) * ldc(src + 2L * j);
+    }
+    __syncthreads();
+    cplx *out = reinterpret_cast<cplx *>(bm) + (long)D * nqmax * 16;
+    for (int w = threadIdx.x; w < nqmax * 16; w += blockDim.x) {
+        const int q = w >> 4, o = w & 15;
+        cplx acc = {0.0, 0.0};
+        if (q < nq && o < nco) {
+            int l, mq, nu, mu;
+            nm_of(q, l, mq);
+            nm_of(o, nu, mu);
+            const int m = mq + mu;
+            const int n0 = max(max(abs(m), l - nu), nu - l), n1 = min(p, l + nu);
+            for (int n = n0; n <= n1; ++n) {
+                if ((n + nu + l) & 1) continue;
+                const int ii = (l - abs(n - nu)) >> 1;
+                if (ii > min(n, nu)) continue;
+                const int j = IDX(n, m);
+                const double g = hd.gaunt[((long)j * ncg + o) * (hd.pg + 1) + ii];
+                const int ex = (nu + l - n) / 2;
+                acc += ((ex & 1) ? -g : g) * a[j];
+            }
+            acc = FOURPI * acc;
+        }
+        out[w] = acc;
+    }
+}
+
+// a block (four waves) per (group of up to GC_GROUP global centers of one
+// target box, share blockIdx.y of its list-3 boxes); per list-3 box D the
+// radial sequences (a lane per center) and the Y columns (a lane per
+// (center, m)) give the group's waves X[centers][nq] in LDS (rows past the
+// group unused), the waves split the K = nq steps of the 16 x 16 product with
+// Bm_D read from HBM, their partial tiles are added in LDS in wave order and
+// the shares' sums added into the QBX expansions with atomics
+__global__ void __launch_bounds__(256) kh_m2qbxl_mm(HD hd, TD t, int nunits, const int *ubox,
+                                                    const int *uc0, const int *gcs,
+                                                    const int *gcl, const int *l3s,
+                                                    const int *l3l, const double *ctrs, int nctr,
+                                                    int nqmax, const double *bm, double *qexp)
+{
+    extern __shared__ double sh[];
+    const int u = blockIdx.x;
+    if (u >= nunits) return;
+    const int itb = ubox[u];
+    const int c0 = gcs[itb] + uc0[u], nct = min(GC_GROUP, gcs[itb + 1] - c0);
+    const int e0 = l3s[itb], e1 = l3s[itb + 1];
+    if (nct <= 0 || e0 + (int)blockIdx.y >= e1) return;
+    const int Lm = hd.pmax + hd.pq;
+    const int nco = hd.ncq;
+    h3_d4 *red = reinterpret_cast<h3_d4 *>(sh);                    // [3][2][64]
+    cplx *X = reinterpret_cast<cplx *>(red + 3 * 2 * 64);           // [GC_GROUP][nqmax]
+    cplx *rad = X + GC_GROUP * nqmax;                               // [GC_GROUP][Lm + 3]
+    double *jt = reinterpret_cast<double *>(rad + GC_GROUP * (Lm + 3));  // [GC_GROUP][Lm + 4]
+    __shared__ double cxs[GC_GROUP * 3];
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, col = lane & 15,
+              kr = lane >> 4;
+    if (tid < GC_GROUP * 3) {
+        const int q = tid / 3, dd = tid % 3;
+        cxs[tid] = q < nct ? ctrs[(size_t)dd * nctr + gcl[c0 + q]] : 0.0;
+    }
+    h3_d4 cre = {0, 0, 0, 0}, cim = {0, 0, 0, 0};
+    for (int e = e0 + blockIdx.y; e < e1; e += gridDim.y) {
+        const int D = l3l[e];
+        const int lv = t.level[D];
+        const int L = hd.order[lv] + hd.pq, nq = (L + 1) * (L + 1);
+        const v3 cd = boxc(t, D);
+        __syncthreads();  // the previous entry's X consumed, cxs written
+        if (tid < nct) {
+            const v3 dv = sub(v3{cxs[3 * tid], cxs[3 * tid + 1], cxs[3 * tid + 2]}, cd);
+            const double rr = sqrt(dv.x * dv.x + dv.y * dv.y + dv.z * dv.z);
+            radial_seq(1, L, hd.k * rr, rad + tid * (Lm + 3), jt + tid * (Lm + 4));
+        }
+        __syncthreads();
+        for (int w = tid; w < nct * (L + 1); w += blockDim.x) {
+            const int q = w / (L + 1), m = w % (L + 1);
+            const v3 dv = sub(v3{cxs[3 * q], cxs[3 * q + 1], cxs[3 * q + 2]}, cd);
+            const cplx *rd = rad + q * (Lm + 3);
+            cplx *Xq = X + (long)q * nqmax;
+            ylm_column(m, L, dv, [&](int n, cplx y) {
+                Xq[IDX(n, m)] = rd[n] * y;
+                if (m > 0) Xq[IDX(n, -m)] = rd[n] * sgn_conj(y, m);
+            });
+        }
+        __syncthreads();
+        const double *B = bm + 2L * (long)D * nqmax * 16;
+        const int KS = (nq + 3) / 4;
+        for (int ks = wv; ks < KS; ks += 4) {
+            const int k = ks * 4 + kr;
+            double are = 0.0, aim = 0.0, bre = 0.0, bim = 0.0;
+            if (k < nq) {
+                if (col < nct) {
+                    const cplx a = X[(long)col * nqmax + k];
+                    are = a.x;
+                    aim = a.y;
+                }
+                const cplx b = ldc(B + 2L * (k * 16 + col));
+                bre = b.x;
+                bim = b.y;
+            }
+            cre = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bre, cre, 0, 0, 0);
+            cre = __builtin_amdgcn_mfma_f64_16x16x4f64(-aim, bim, cre, 0, 0, 0);
+            cim = __builtin_amdgcn_mfma_f64_16x16x4f64(are, bim, cim, 0, 0, 0);
+            cim = __builtin_amdgcn_mfma_f64_16x16x4f64(aim, bre, cim, 0, 0, 0);
+        }
+    }
+    // the four waves' partial tiles: D register j of lane i is the entry
+    // (center 4j + i/16, coefficient i%16)
+    if (wv > 0) {
+        red[((wv - 1) * 2 + 0) * 64 + lane] = cre;
+        red[((wv - 1) * 2 + 1) * 64 + lane] = cim;
+    }
+    __syncthreads();
+    if (wv == 0) {
+        for (int h = 0; h < 3; ++h) {
+            cre += red[(h * 2 + 0) * 64 + lane];
+            cim += red[(h * 2 + 1) * 64 + lane];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int q = 4 * j + kr;
+            if (q < nct && col < nco)
+                atomic_addc(qexp + 2 * ((size_t)gcl[c0 + q] * nco + col), cplx{cre[j], cim[j]});
+        }
+    }
+}
+
 // ---- QBXL2P ---------------------------------------------------------------
 template <int NRC>
 __global__ void kh_qbxl2p(HD hd, int nctt, const int *ctt_lists, const int *tree_ttc,
@@ -1144,16 +1313,6 @@ __global__ void kh_qbxl2p(HD hd, int nctt, const int *ctt_lists, const int *tree
 //   P2P^T  q'_s += sum_t conj(G(t, s)) pot'_t
 // Scatters (list 2, 3, 4 from the target side, L2L^T into parents, P2P^T)
 // add with atomics; the rest have one writer.
-
-// shares (blockIdx.y) of the list-1 sources / list-3 boxes per center group
-// in P2QBXL^T and M2QBXL^T
-constexpr int ADJ_SHARES = 4;
-
-__device__ __forceinline__ void atomic_addc(double *p, cplx v)
-{
-    atomicAdd(p, v.x);
-    atomicAdd(p + 1, v.y);
-}
 
 // P2M^T (LOCAL false: the box's own sources, coefficients M~') and P2L^T
 // (LOCAL true: the sources of the box's list-4 boxes, coefficients L~'):
@@ -2478,6 +2637,25 @@ static int h3_e2qbxl(qbx_ctx *ctx, hipStream_t st, const qbx_tree *T, const qbx_
     if (ng == 0 || (mode == 0 && R->n3 == 0)) return QBX_OK;
     QBX_CHECK(G->trav == R, "M2QBXL / L2QBXL: geometry was built for another traversal");
     const TD td = make_td(T);
+    const int Lm = hd.pmax + hd.pq, nqmax = (Lm + 1) * (Lm + 1);
+    const size_t mm_by = 32 * 3 * 2 * 64 + 16 * ((long)GC_GROUP * nqmax + GC_GROUP * (Lm + 3)) +
+                         8L * GC_GROUP * (Lm + 4);
+    const unit_plan &U = G->gc_group_plan;
+    if (mode == 0 && hd.nr == 1 && hd.ncq <= 16 && U.nunits > 0 && mm_by <= 64 * 1024) {
+        // one density, p_q <= 3 (and the waves' tile within 96 KiB of LDS):
+        // the per-box matrices, then the products
+        QBX_TRY(ws_reserve(ctx, st, ctx->ws_h3_m2qb, 2L * T->nboxes * nqmax * 16,
+                           "M2QBXL box matrices"));
+        hipLaunchKernelGGL(kh_m2qb_mat, dim3(T->nboxes), dim3(256), 0, st, hd, td, nqmax, expn,
+                           ctx->ws_h3_m2qb.p);
+        const size_t by = mm_by;
+        hipLaunchKernelGGL(kh_m2qbxl_mm, dim3(U.nunits, ADJ_SHARES), dim3(256), by, st, hd, td,
+                           U.nunits, U.unit_box.p, U.unit_t0.p, G->gc_by_tbox_starts.p,
+                           G->gc_by_tbox.p, R->l3_starts.p, R->l3_lists.p, G->centers.p,
+                           G->ncenters, nqmax, ctx->ws_h3_m2qb.p, qexp);
+        QBX_HIP(hipGetLastError());
+        return QBX_OK;
+    }
 #define H3_EQ(M, N)                                                                        \
     hipLaunchKernelGGL((kh_e2qbxl<M, N>), dim3(ng), dim3(64), e2qbxl_lds(hd, N), st, hd, td, ng, \
                        G->global_qbx_centers.p, G->center_to_target_box.p,                 \

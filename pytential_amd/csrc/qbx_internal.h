@@ -136,6 +136,7 @@ struct qbx_ctx {
     // class-grouped one-density M2L: the per-entry products before the
     // per-box sums
     qbx::dbuf<double> ws_m2lg;
+    qbx::dbuf<double> ws_h3_m2qb;  // 3D Helmholtz M2QBXL: per-box contraction matrices
     int h3_pg = -1;
     std::vector<double> h3_key;
     // side streams of qbx_drive_fmm (default priority; shared by every
