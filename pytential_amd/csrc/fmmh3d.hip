@@ -1794,6 +1794,107 @@ __global__ void __launch_bounds__(448) kh_adj_e2qbxl(HD hd, TD t, int nunits,
     }
 }
 
+// L2QBXL^T / M2QBXL^T with the group's centers summed before the Gaunt
+// contraction (p_q <= 3): per list-3 box D (or the own box) the waves of all
+// the group's centers in LDS (a lane per center for the radial sequences, a
+// lane per (center, m) for the Y columns), Z[q][o] = sum_c conj(X_c[q])
+// Lq'_c[o] in LDS, then a thread per box coefficient j: sum_o sum_ii
+// (-1)^ex G(j, o, ii) Z[(l, m - mu)][o] -- one contraction per group and box
+// instead of one per center.  Dynamic LDS: X [GC_GROUP][nqmax], Z [nqmax][16],
+// the radial sequences.
+template <int MODE>
+__global__ void __launch_bounds__(448) kh_adj_e2qbxl_z(HD hd, TD t, int nunits,
+                                                       const int *unit_box, const int *unit_c0,
+                                                       const int *tboxes, const int *starts,
+                                                       const int *lists, const int *gcs,
+                                                       const int *gcl, const double *ctrs,
+                                                       int nctr, int nqmax, const double *qadj,
+                                                       double *xadj)
+{
+    extern __shared__ double sh[];
+    const int Lm = hd.pmax + hd.pq;
+    cplx *X = reinterpret_cast<cplx *>(sh);                          // [GC_GROUP][nqmax]
+    cplx *Z = X + GC_GROUP * nqmax;                                   // [nqmax][16]
+    cplx *rad = Z + 16 * nqmax;                                       // [GC_GROUP][Lm + 3]
+    double *jt = reinterpret_cast<double *>(rad + GC_GROUP * (Lm + 3));  // [GC_GROUP][Lm + 4]
+    __shared__ double cxs[GC_GROUP * 3];
+    __shared__ cplx ys[GC_GROUP * 16];
+    const int u = blockIdx.x;
+    if (u >= nunits) return;
+    const int itb = unit_box[u];
+    const int c0 = gcs[itb] + unit_c0[u], nct = min(GC_GROUP, gcs[itb + 1] - c0);
+    if (nct <= 0) return;
+    const int tid = threadIdx.x;
+    const int pq = hd.pq, nco = hd.ncq;
+    const int ncg = (hd.pg + 1) * (hd.pg + 1);
+    if (tid < GC_GROUP * 3) {
+        const int q = tid / 3, dd = tid % 3;
+        cxs[tid] = q < nct ? ctrs[(size_t)dd * nctr + gcl[c0 + q]] : 0.0;
+    }
+    if (tid < GC_GROUP * 16) {
+        const int q = tid >> 4, o = tid & 15;
+        ys[tid] = q < nct && o < nco ? ldc(qadj + 2 * ((size_t)gcl[c0 + q] * nco + o))
+                                     : cplx{0.0, 0.0};
+    }
+    const int eb = MODE == 0 ? starts[itb] + (int)blockIdx.y : 0;
+    const int ee = MODE == 0 ? starts[itb + 1] : (blockIdx.y == 0 ? 1 : 0);
+    const int es = MODE == 0 ? (int)gridDim.y : 1;
+    for (int e = eb; e < ee; e += es) {
+        const int D = MODE == 0 ? lists[e] : tboxes[itb];
+        const int lv = t.level[D];
+        const int p = hd.order[lv];
+        const int L = p + pq, nq = (L + 1) * (L + 1);
+        const int ncin = (p + 1) * (p + 1);
+        const v3 cd = boxc(t, D);
+        __syncthreads();  // the previous box's Z consumed; cxs, ys written
+        if (tid < nct) {
+            const v3 dv = sub(v3{cxs[3 * tid], cxs[3 * tid + 1], cxs[3 * tid + 2]}, cd);
+            const double rr = sqrt(dv.x * dv.x + dv.y * dv.y + dv.z * dv.z);
+            radial_seq(MODE == 0 ? 1 : 0, L, hd.k * rr, rad + tid * (Lm + 3),
+                       jt + tid * (Lm + 4));
+        }
+        __syncthreads();
+        for (int w = tid; w < nct * (L + 1); w += blockDim.x) {
+            const int q = w / (L + 1), m = w % (L + 1);
+            const v3 dv = sub(v3{cxs[3 * q], cxs[3 * q + 1], cxs[3 * q + 2]}, cd);
+            const cplx *rd = rad + q * (Lm + 3);
+            cplx *Xq = X + (long)q * nqmax;
+            ylm_column(m, L, dv, [&](int n, cplx y) {
+                Xq[IDX(n, m)] = rd[n] * y;
+                if (m > 0) Xq[IDX(n, -m)] = rd[n] * sgn_conj(y, m);
+            });
+        }
+        __syncthreads();
+        for (int w = tid; w < nq * 16; w += blockDim.x) {
+            const int q = w >> 4, o = w & 15;
+            cplx z = {0.0, 0.0};
+            for (int c = 0; c < nct; ++c) z += conj(X[(long)c * nqmax + q]) * ys[c * 16 + o];
+            Z[w] = z;
+        }
+        __syncthreads();
+        if (tid < ncin) {
+            int n, m;
+            nm_of(tid, n, m);
+            const double *gj = hd.gaunt + (long)tid * ncg * (hd.pg + 1);
+            cplx acc = {0.0, 0.0};
+            for (int o = 0; o < nco; ++o) {
+                int nu, mu;
+                nm_of(o, nu, mu);
+                const double *gg = gj + (long)o * (hd.pg + 1);
+                const int imax = min(n, nu);
+                for (int ii = 0; ii <= imax; ++ii) {
+                    const int l = abs(n - nu) + 2 * ii;
+                    if (abs(m - mu) > l) continue;
+                    const int ex = (nu + l - n) / 2;
+                    acc += ((ex & 1) ? -gg[ii] : gg[ii]) * Z[IDX(l, m - mu) * 16 + o];
+                }
+            }
+            const double sc = FOURPI * ipow(MODE == 0 ? hd.s[lv] : 1.0 / hd.s[lv], n);
+            atomic_addc(xadj + ecoef_off(hd, D, lv) + 2L * tid, sc * acc);
+        }
+    }
+}
+
 // P2QBXL^T: a 64-lane block per (group of up to GC_GROUP centers of one
 // target box, share blockIdx.y of its list-1 sources), lanes over the
 // sources, each summing over the group's centers; one atomic per source.
@@ -2740,8 +2841,20 @@ static int adjoint_launch(hipStream_t st, const qbx_tree *T, const qbx_trav *R,
     case QBX_ADJ_M2QBXL:
         if (U.nunits > 0 && (stage == QBX_ADJ_L2QBXL || R->n3 > 0)) {
             const int th = std::min(448, 64 * ((ncl_max + 63) / 64));
-            auto kern = stage == QBX_ADJ_L2QBXL ? kh_adj_e2qbxl<1> : kh_adj_e2qbxl<0>;
             const int ny = stage == QBX_ADJ_L2QBXL ? 1 : ADJ_SHARES;
+            const int Lm = hd.pmax + hd.pq, nqmax = (Lm + 1) * (Lm + 1);
+            const size_t zby = 16 * ((long)GC_GROUP * nqmax + 16L * nqmax + GC_GROUP * (Lm + 3)) +
+                               8L * GC_GROUP * (Lm + 4);
+            if (hd.ncq <= 16 && zby <= 96 * 1024) {
+                // the group's centers summed before the contraction
+                auto kz = stage == QBX_ADJ_L2QBXL ? kh_adj_e2qbxl_z<1> : kh_adj_e2qbxl_z<0>;
+                hipLaunchKernelGGL(kz, dim3(U.nunits, ny), dim3(th), zby, st, hd, td, U.nunits,
+                                   U.unit_box.p, U.unit_t0.p, R->target_boxes.p, R->l3_starts.p,
+                                   R->l3_lists.p, G->gc_by_tbox_starts.p, G->gc_by_tbox.p,
+                                   G->centers.p, nc, nqmax, in, out);
+                break;
+            }
+            auto kern = stage == QBX_ADJ_L2QBXL ? kh_adj_e2qbxl<1> : kh_adj_e2qbxl<0>;
             hipLaunchKernelGGL(kern, dim3(U.nunits, ny), dim3(th), 0, st, hd, td, U.nunits,
                                U.unit_box.p, U.unit_t0.p, R->target_boxes.p, R->l3_starts.p,
                                R->l3_lists.p, G->gc_by_tbox_starts.p, G->gc_by_tbox.p,
