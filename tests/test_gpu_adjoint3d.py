@@ -58,10 +58,11 @@ def _setup(nsub=2, fmm_order=10, mlw=64, qbx_order=3):
     return d, src, [("self", +1), (tgt, 0)]
 
 
-@pytest.mark.parametrize("qbx_order", [3, 6])
+@pytest.mark.parametrize("qbx_order", [2, 3, 6])
 def test_transposed_stages_dot_identity(qbx_order):
-    """qbx_order 3: the register forms of P2QBXL and its transpose; 6: the
-    general-order forms."""
+    """qbx_order 2 and 3: the register forms of P2QBXL and its transpose and
+    the matrix-core M2QBXL with its group-summed transpose (fewer than 16
+    QBX coefficients at 2); 6: the general-order forms."""
     from pytential_amd import fmm as F
     d, src, pairs = _setup(qbx_order=qbx_order)
     geo = src.qbx_fmm_geometry_data(tuple(pairs))

@@ -195,3 +195,35 @@ def test_green_identity_sphere():
           f"{fmm_vs_direct:.2e}", flush=True)
     assert err < 2e-3
     assert fmm_vs_direct < 1e-6
+
+
+@pytest.mark.parametrize("qbx_order", [1, 2])
+def test_fmmh3d_low_qbx_order_matches_oracle(qbx_order):
+    """The driver at qbx_order 1 and 2 (4 and 9 QBX coefficients: the
+    matrix-core M2QBXL with partly empty coefficient columns) against the
+    oracle's QBX-FMM on the same tree, 1e-12."""
+    import oracle
+
+    from pytential_amd import fmm as F
+    from pytential_amd.qbx import QBXLayerPotentialSource
+    from pytential_amd.surface import icosphere
+    discr = icosphere(2, order=4, fine_order=5)
+    rng = np.random.default_rng(1007)
+    v = rng.normal(size=(3, 200))
+    v /= np.linalg.norm(v, axis=0)
+    tgt = v * rng.uniform(1.15, 2.0, 200)
+    user_targets = np.concatenate([discr.nodes, tgt], axis=1)
+    sides = np.concatenate([np.ones(discr.ndofs, np.int32), np.zeros(200, np.int32)])
+    src = QBXLayerPotentialSource(discr, fine_order=discr.fine_order, qbx_order=qbx_order,
+                                  fmm_order=8, _max_leaf_refine_weight=64)
+    geo = src.qbx_fmm_geometry_data((("self", +1), (tgt, 0)))
+    ref = oracle.GeometryData(discr, user_targets, sides, max_leaf_refine_weight=64)
+    w = src.make_wrangler(geo, "helmholtz", K, "S")
+    sigma = rng.normal(size=discr.ndofs) + 1j * rng.normal(size=discr.ndofs)
+    pot_ref, _ = ref.fmm("helmholtz", K, "S", qbx_order, list(w._lo), discr.strengths(sigma))
+    strengths, nrm = w.reorder_sources(src.quad_density(torch.as_tensor(sigma)),
+                                       src.dev["quad_weights"], None)
+    e = _rel(_np(F.drive_fmm_fused(w, strengths, nrm)), pot_ref)
+    print(f"\n[h3] qbx_order {qbx_order}: fused driver vs oracle {e:.2e}", flush=True)
+    assert e < REL_TOL
+
