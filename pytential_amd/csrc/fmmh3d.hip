@@ -152,32 +152,31 @@ __device__ void sph_j_seq(int p, double x, double *j)
     for (int n = 0; n <= pp; ++n) j[n] *= sc;
 }
 
-// radial sequence into rad (complex, n = 0..p): j_n (sing 0) or
-// h_n = j_n + i y_n (sing 1, y_n by the forward recurrence); jtmp holds
-// p + 2 doubles
+// radial sequence into rad (complex, n = 0..p, p + 1 >= 2 slots): j_n
+// (sing 0, Miller's recurrence; jtmp holds p + 2 doubles) or h_n = j_n + i y_n
+// (sing 1, x > 0) by the forward recurrence h_{n+1} = (2n+1)/x h_n - h_{n-1}
+// from h_0 = (sin x - i cos x) / x and h_1: the recurrence is stable for the
+// dominant y_n, and the error it leaves in the small j_n of x < n stays at
+// the rounding of |h_n| (only h_n as a whole enters the singular expansions)
 __device__ void radial_seq(int sing, int p, double x, cplx *rad, double *jtmp)
 {
-    sph_j_seq(p, x, jtmp);
     if (!sing) {
+        sph_j_seq(p, x, jtmp);
         for (int n = 0; n <= p; ++n) rad[n] = {jtmp[n], 0.0};
         return;
     }
     double sn, cs;
     qbx_sincos(x, &sn, &cs);
     const double ix = qbx_rcp(x);
-    double ym2 = -cs * ix, ym1 = (-cs * ix - sn) * ix;
-    for (int n = 0; n <= p; ++n) {
-        double y;
-        if (n == 0) {
-            y = ym2;
-        } else if (n == 1) {
-            y = ym1;
-        } else {
-            y = (2.0 * n - 1.0) * ix * ym1 - ym2;
-            ym2 = ym1;
-            ym1 = y;
-        }
-        rad[n] = {jtmp[n], y};
+    cplx hm2 = {sn * ix, -cs * ix};
+    cplx hm1 = {(sn * ix - cs) * ix, (-cs * ix - sn) * ix};
+    rad[0] = hm2;
+    rad[1] = hm1;
+    for (int n = 2; n <= p; ++n) {
+        const cplx h = ((2.0 * n - 1.0) * ix) * hm1 - hm2;
+        rad[n] = h;
+        hm2 = hm1;
+        hm1 = h;
     }
 }
 
@@ -511,43 +510,30 @@ __global__ void __launch_bounds__(512) kh_p2e(HD hd, TD t, int nitems, const int
 }
 
 // S waves of one source about a center, in registers: F[IDX(n, m)] =
-// h_n(k r) Y_n^m(d) for n <= PF, d = s - c, r = |d| > 0.  y_n by the forward
-// recurrence, j_n by Miller's backward recurrence from N = max(PF, x) + 10 +
-// 4 sqrt(max(PF, x)) normalised by j_0 or j_1, Y by the column recurrences,
-// all unrolled.
+// h_n(k r) Y_n^m(d) for n <= PF, d = s - c, r = |d| > 0.  h_n by the forward
+// recurrence (as radial_seq: stable for the dominant y_n, the small j_n of
+// x < n accurate to the rounding of |h_n|), Y by the column recurrences, all
+// unrolled.
 template <int PF>
 __device__ __forceinline__ void s_waves_reg(double k, v3 d, cplx *F)
 {
-    constexpr int PJ = PF < 1 ? 1 : PF;
     const double r2 = d.x * d.x + d.y * d.y + d.z * d.z;
     const double ir = qbx_rsqrt(r2);
     const double r = r2 * ir;
     const double x = k * r, ix = qbx_rcp(x);
     double sn, cs;
     qbx_sincos(x, &sn, &cs);
-    double yv[PF + 1];
+    double yv[PF + 2], jv[PF + 2];
     yv[0] = -cs * ix;
-    if (PF >= 1) yv[PF >= 1 ? 1 : 0] = (-cs * ix - sn) * ix;
+    yv[1] = (-cs * ix - sn) * ix;
+    jv[0] = sn * ix;
+    jv[1] = (sn * ix - cs) * ix;
 #pragma unroll
-    for (int n = 2; n <= PF; ++n) yv[n] = (2.0 * n - 1.0) * ix * yv[n - 1] - yv[n - 2];
-    const double mx = fmax((double)PJ, x);
-    const int N = (int)mx + 10 + (int)(4.0 * sqrt(mx));
-    double fp1 = 0.0, f = 1e-30;
-    for (int n = N; n > PJ; --n) {
-        const double fm1 = (2.0 * n + 1.0) * ix * f - fp1;
-        fp1 = f;
-        f = fm1;
+    for (int n = 2; n <= PF; ++n) {
+        yv[n] = (2.0 * n - 1.0) * ix * yv[n - 1] - yv[n - 2];
+        jv[n] = (2.0 * n - 1.0) * ix * jv[n - 1] - jv[n - 2];
     }
-    double jv[PJ + 1];
-#pragma unroll
-    for (int n = PJ; n >= 0; --n) {
-        jv[n] = f;
-        const double fm1 = (2.0 * n + 1.0) * ix * f - fp1;
-        fp1 = f;
-        f = fm1;
-    }
-    const double j0 = sn * ix, j1 = (sn * ix - cs) * ix;
-    const double scl = fabs(j0) >= fabs(j1) ? j0 / jv[0] : j1 / jv[1];
+    const double scl = 1.0;
     const double z = d.z * ir;
     const cplx w = {d.x * ir, d.y * ir};
     cplx wm = {1.0, 0.0};
